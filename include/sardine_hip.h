@@ -1,0 +1,431 @@
+/*
+ * sardine_hip.h — C ABI of the MI355X (gfx950) calibrate-and-geocode path.
+ *
+ * Every entry point replaces one CPU stage of SARdine's Rust pipeline.  The
+ * comment above each declaration names the reference interface it replaces
+ * (paths relative to the SARdine repository root).
+ *
+ * Conventions
+ *  - Times are int64 microseconds on one caller-chosen epoch.  The reference
+ *    does all time arithmetic in whole microseconds (chrono `Duration`
+ *    quantisation, sardine/src/terrain_correction.rs:746-749,1067,1135 and
+ *    sardine/src/orbit.rs:520,546), so this representation is exact.
+ *  - Raster pointers marked [device] are HIP device pointers (inputs already
+ *    resident in HBM).  Small metadata (orbit, LUT vectors, grid points) is
+ *    passed as host memory and staged by the library.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *  - Every function returns a sardine_status; on failure a message is
+ *    available from sardine_last_error() (thread-local).
+ *  - There is no CPU fallback: without a usable gfx950 device every compute
+ *    entry point returns SARDINE_ERR_NO_DEVICE.
+ */
+#ifndef SARDINE_HIP_H
+#define SARDINE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SARDINE_ABI_VERSION 1
+
+/* ── Status codes ─────────────────────────────────────────────────────────
+ * One code per reference error variant on the path:
+ *   TerrainCorrectionError  sardine/src/terrain_correction.rs:45-67
+ *   OrbitError              sardine/src/orbit.rs:40-75
+ *   SpeckleError            sardine/src/speckle.rs:96-108
+ *   CalibrationError        sardine/src/apply_calibration.rs:64-84
+ *   DeburstError            sardine/src/deburst.rs:56-121
+ *   MergeError              sardine/src/merge_subswaths.rs:62-81          */
+typedef enum sardine_status {
+    SARDINE_OK = 0,
+    /* TerrainCorrectionError */
+    SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW = 10, /* Orbit(OutsideOrbitWindow): aborts the whole call */
+    SARDINE_ERR_TC_ORBIT_INSUFFICIENT = 11,   /* Orbit(InsufficientCoverage): < 8 state vectors */
+    SARDINE_ERR_TC_NO_GRID_POINTS = 12,
+    SARDINE_ERR_TC_NO_SUBSWATHS = 13,
+    SARDINE_ERR_TC_CONFIG = 14,
+    SARDINE_ERR_TC_PROJECTION = 15,
+    SARDINE_ERR_TC_DEM = 16,
+    /* SpeckleError */
+    SARDINE_ERR_SPECKLE_DIMENSION = 20,
+    SARDINE_ERR_SPECKLE_BAD_WINDOW = 21,
+    SARDINE_ERR_SPECKLE_BAD_ENL = 22,
+    SARDINE_ERR_SPECKLE_BAD_DAMPING = 23,
+    /* CalibrationError */
+    SARDINE_ERR_CAL_NO_CAL_VECTORS = 30,
+    SARDINE_ERR_CAL_NO_NOISE_VECTORS = 31,
+    SARDINE_ERR_CAL_LUT_RANGE = 32,
+    /* DeburstError */
+    SARDINE_ERR_DEBURST_NO_BURSTS = 40,
+    SARDINE_ERR_DEBURST_COUNT_MISMATCH = 41,
+    SARDINE_ERR_DEBURST_TIFF_TOO_SMALL = 42,
+    SARDINE_ERR_DEBURST_EMPTY_VALID_WINDOW = 43,
+    SARDINE_ERR_DEBURST_INVALID_OVERLAP = 44,
+    SARDINE_ERR_DEBURST_IMPLAUSIBLE_OVERLAP = 45,
+    /* MergeError */
+    SARDINE_ERR_MERGE_EMPTY_INPUTS = 50,
+    SARDINE_ERR_MERGE_PIXEL_SPACING = 51,
+    SARDINE_ERR_MERGE_EMPTY_SWATH = 52,
+    /* library / device */
+    SARDINE_ERR_INVALID_ARGUMENT = 90,
+    SARDINE_ERR_NO_DEVICE = 91,
+    SARDINE_ERR_HIP = 92,
+    SARDINE_ERR_OUT_OF_MEMORY = 93,
+    SARDINE_ERR_CAPACITY = 94
+} sardine_status;
+
+/* Thread-local message for the last failing call on this thread. */
+const char* sardine_last_error(void);
+int sardine_abi_version(void);
+/* Number of visible gfx9xx devices (0 when none); never initialises kernels. */
+int sardine_device_count(void);
+
+/* ── Geometry / orbit types ────────────────────────────────────────────── */
+
+/* StateVector, sardine/src/types.rs:142 */
+typedef struct sardine_state_vector {
+    int64_t time_us;        /* on the caller's epoch */
+    double position_m[3];   /* ECEF */
+    double velocity_m_s[3]; /* ECEF */
+} sardine_state_vector;
+
+/* OrbitData, sardine/src/types.rs:157 (state vectors sorted by time) */
+typedef struct sardine_orbit {
+    const sardine_state_vector* state_vectors; /* host */
+    int32_t count;
+} sardine_orbit;
+
+/* GeolocationGridPoint, sardine/src/types.rs:171 — the two fields the
+ * azimuth-time seed LUT reads (terrain_correction.rs:1080-1104). */
+typedef struct sardine_grid_point {
+    double latitude_deg;
+    int64_t azimuth_time_us;
+} sardine_grid_point;
+
+/* BoundingBox, sardine/src/types.rs:267 */
+typedef struct sardine_bbox {
+    double min_lat_deg, max_lat_deg, min_lon_deg, max_lon_deg;
+} sardine_bbox;
+
+/* The parts of SceneMetadata (types.rs:406-451) terrain_correction reads:
+ * the orbit, the bounding box, the first subswath's azimuth time interval
+ * (terrain_correction.rs:707-711) and the seed-LUT grid (the caller passes the
+ * IW2 grid when present, else the first one — the selection rule of
+ * terrain_correction.rs:1087-1091). */
+typedef struct sardine_scene {
+    sardine_orbit orbit;
+    sardine_bbox bounding_box;
+    double azimuth_time_interval_s;
+    const sardine_grid_point* grid_points; /* host */
+    int32_t n_grid_points;
+    int32_t n_subswaths; /* 0 → SARDINE_ERR_TC_NO_SUBSWATHS */
+} sardine_scene;
+
+/* A raster-backed RadarImage (pipeline_options.rs:100-135), i.e. the
+ * MergedSigma0 buffers (merge_subswaths.rs:106-162).  Opaque per-sample
+ * implementors have no GPU form and are rejected by construction. */
+typedef struct sardine_radar_image {
+    const float* data;  /* [device] lines*samples, row-major */
+    const float* nesz;  /* [device] same shape, or NULL (nesz_at → None) */
+    int64_t lines;
+    int64_t samples;
+    int64_t azimuth_start_time_us;
+    double near_slant_range_time_s;
+    double range_pixel_spacing_m;
+} sardine_radar_image;
+
+/* A geographic DEM raster with the GLO-30 tile sampling rule
+ * (dem.rs:351-372 + bilinear_on_grid dem.rs:385-409): origin is the centre of
+ * pixel (0,0), rows run south.  NaN cells are voids. */
+typedef struct sardine_dem_raster {
+    const float* data; /* [device] height*width */
+    int64_t width;
+    int64_t height;
+    double origin_lat_deg;
+    double origin_lon_deg;
+    double pixel_deg;
+} sardine_dem_raster;
+
+/* GeoidModel, geoid.rs:342-380 */
+typedef enum sardine_geoid_kind {
+    SARDINE_GEOID_ZERO = 0,
+    SARDINE_GEOID_GRID_2P5 = 1 /* Egm96 / Egm2008 73x144 2.5° grid, geoid.rs:220-251 */
+} sardine_geoid_kind;
+
+typedef struct sardine_geoid {
+    int32_t kind;
+    const float* grid_73x144; /* host, row 0 = 90°N, col 0 = 0°E; NULL for ZERO */
+} sardine_geoid;
+
+/* OutputCrs, output_crs.rs:70-96 (WGS84 and UTM are supported) */
+typedef enum sardine_crs_kind {
+    SARDINE_CRS_WGS84 = 0,
+    SARDINE_CRS_UTM_NORTH = 1,
+    SARDINE_CRS_UTM_SOUTH = 2
+} sardine_crs_kind;
+
+/* ResamplingKernel, pipeline_options.rs:74-90 */
+typedef enum sardine_resampling {
+    SARDINE_RESAMPLE_BILINEAR = 0,
+    SARDINE_RESAMPLE_BICUBIC = 1,
+    SARDINE_RESAMPLE_LANCZOS3 = 2
+} sardine_resampling;
+
+/* Arithmetic of the zero-Doppler / orbit / geometry part. */
+typedef enum sardine_precision {
+    SARDINE_PRECISION_FP64 = 0,       /* the reference's arithmetic */
+    SARDINE_PRECISION_FP32_CENTRED = 1 /* scene-centred FP32 variant */
+} sardine_precision;
+
+/* TerrainCorrectionConfig, terrain_correction.rs:100-235, plus the two env
+ * knobs of terrain_correction.rs:1328-1378 and an optional AOI window. */
+typedef struct sardine_tc_config {
+    double pixel_spacing;          /* degrees for WGS84, metres for UTM */
+    int32_t crs;                   /* sardine_crs_kind */
+    int32_t utm_zone;              /* 1..60 for UTM */
+    int32_t max_newton_iterations; /* reference default 10 */
+    double newton_tolerance_s;     /* reference default 1e-6 */
+    int32_t flatten;
+    int32_t compute_lia;
+    float noise_floor_margin_db;   /* -INFINITY disables NESZ masking */
+    int32_t resampling;            /* sardine_resampling */
+    int32_t projector_anchors;     /* SARDINE_TC_PROJECTOR_ANCHORS; 0 → 256 */
+    int32_t precision;             /* sardine_precision */
+    sardine_geoid geoid;
+    /* AOI window on the full output grid: rows [aoi_row0, aoi_row0+aoi_rows),
+     * cols [aoi_col0, aoi_col0+aoi_cols).  aoi_rows == 0 → the full grid.
+     * The per-row seed chain starts at aoi_col0 (the reference has no AOI;
+     * its full-grid result equals aoi = full grid). */
+    int64_t aoi_row0, aoi_col0, aoi_rows, aoi_cols;
+} sardine_tc_config;
+
+/* Fills cfg with TerrainCorrectionConfig::new(GeoidModel::Zero) defaults. */
+void sardine_tc_config_default(sardine_tc_config* cfg);
+
+#define SARDINE_NEWTON_HIST_SIZE 16 /* NEWTON_ITER_HIST_SIZE, terrain_correction.rs:95 */
+
+/* mask_code, terrain_correction.rs:346-376 */
+enum {
+    SARDINE_MASK_VALID = 0,
+    SARDINE_MASK_OUTSIDE_FOOTPRINT = 1,
+    SARDINE_MASK_DEM_MISSING = 2,
+    SARDINE_MASK_NON_CONVERGED = 3,
+    SARDINE_MASK_DEGENERATE_GEOMETRY = 4,
+    SARDINE_MASK_SHADOW = 5,
+    SARDINE_MASK_LAYOVER = 6,
+    SARDINE_MASK_FORESHORTENING = 7,
+    SARDINE_MASK_NOISE_MASKED = 8,
+    SARDINE_MASK_DEM_NEIGHBOUR_MISSING = 9
+};
+
+/* GeocodedImage, terrain_correction.rs:245-307.  Buffers are caller-owned
+ * device memory of rows*cols elements (AOI rows*cols when an AOI is set);
+ * cos_lia / mask are written only when compute_lia (may be NULL otherwise). */
+typedef struct sardine_geocoded_image {
+    float* data;     /* [device] */
+    float* cos_lia;  /* [device] or NULL */
+    uint8_t* mask;   /* [device] or NULL */
+    /* [device] or NULL: per pixel (line, sample) radar coordinates of every
+     * converged pixel, NaN elsewhere (a geolocation diagnostic with no
+     * reference counterpart; 2*rows*cols doubles). */
+    double* radar_coords;
+    int64_t rows, cols;                 /* filled by the call */
+    int64_t full_rows, full_cols;       /* full-grid shape */
+    double geotransform[6];             /* of the written window */
+    uint64_t total_pixel_count;
+    uint64_t valid_pixel_count;
+    uint64_t dem_missing_count;
+    uint64_t outside_footprint_count;
+    uint64_t non_converged_count;
+    uint64_t degenerate_geometry_count;
+    uint64_t flat_masked_count;
+    uint64_t noise_masked_count;
+    uint64_t newton_iter_histogram[SARDINE_NEWTON_HIST_SIZE];
+    /* diagnostics of the GPU seed-chain emulation (no reference counterpart) */
+    uint64_t chain_redo_rounds;
+} sardine_geocoded_image;
+
+/* Output grid of terrain_correction: compute_output_grid_extent
+ * (terrain_correction.rs:1174-1216) + n_rows/n_cols (:1310-1317) + the
+ * geotransform (:1805-1812).  Host-only, no device needed. */
+sardine_status sardine_tc_output_grid(const sardine_scene* scene, const sardine_tc_config* cfg,
+                                      int64_t* rows, int64_t* cols, double geotransform[6]);
+
+/* terrain_correction(merged, scene, dem, grids, config) -> GeocodedImage,
+ * terrain_correction.rs:1245-1836.  Synchronous: returns after the counters
+ * are on the host.  Output buffers must hold out->rows*out->cols elements of
+ * the (AOI) window; query the shape with sardine_tc_output_grid. */
+sardine_status sardine_terrain_correction(const sardine_radar_image* merged,
+                                          const sardine_scene* scene,
+                                          const sardine_dem_raster* dem,
+                                          const sardine_tc_config* cfg,
+                                          sardine_geocoded_image* out,
+                                          void* stream);
+
+/* ── Speckle ────────────────────────────────────────────────────────────── */
+
+/* SpeckleFilter, speckle.rs:61-91 */
+typedef enum sardine_speckle_kind {
+    SARDINE_SPECKLE_BOXCAR = 0,
+    SARDINE_SPECKLE_LEE = 1,
+    SARDINE_SPECKLE_FROST = 2,
+    SARDINE_SPECKLE_GAMMA_MAP = 3,
+    SARDINE_SPECKLE_REFINED_LEE = 4
+} sardine_speckle_kind;
+
+typedef struct sardine_speckle_filter {
+    int32_t kind;
+    int32_t window;  /* odd ≥ 3; ignored for REFINED_LEE (fixed 7) */
+    float enl;       /* LEE, GAMMA_MAP, REFINED_LEE */
+    float damping;   /* FROST */
+} sardine_speckle_filter;
+
+/* SpeckleKernel::apply(data, cols, rows) -> Vec<f32> (speckle.rs:586-603,
+ * apply_speckle_filter speckle.rs:131-180).  Out-of-place: `out` must not
+ * alias `data`.  `data_len` is the element count of `data` (the reference's
+ * DimensionMismatch check). */
+sardine_status sardine_speckle_apply(const float* data /*[device]*/, int64_t data_len,
+                                     int64_t cols, int64_t rows,
+                                     const sardine_speckle_filter* filter,
+                                     float* out /*[device]*/, void* stream);
+
+/* ── Deburst / calibration / merge ─────────────────────────────────────── */
+
+/* BurstEntry, types.rs:351-395 (fields the deburst reads) */
+typedef struct sardine_burst {
+    int64_t azimuth_time_us;
+    int64_t first_line;          /* TIFF row of the burst's line 0 */
+    int64_t first_valid_sample;
+    int64_t last_valid_sample;   /* exclusive */
+    int32_t burst_index;
+} sardine_burst;
+
+/* SubSwathMetadata, types.rs:281-337 (fields the path reads) */
+typedef struct sardine_subswath {
+    int32_t burst_count;
+    int64_t lines_per_burst;
+    double azimuth_time_interval_s;
+    double slant_range_time_s;
+    double range_pixel_spacing_m;
+} sardine_subswath;
+
+/* Result of deburst_subswath's integer layout (deburst.rs:230-344). */
+#define SARDINE_MAX_BURSTS 64
+typedef struct sardine_deburst_layout {
+    int64_t lines;                /* total debursted lines */
+    int64_t samples;              /* valid_cols */
+    int64_t valid_sample_offset;  /* first_valid */
+    int32_t n_bursts;
+    int64_t overlap[SARDINE_MAX_BURSTS];      /* overlap[k] between burst k and k+1 */
+    int64_t local_first[SARDINE_MAX_BURSTS];  /* ⌈ov/2⌉ rule */
+    int64_t local_end[SARDINE_MAX_BURSTS];    /* lps − ⌊ov/2⌋ rule */
+    int64_t out_line_start[SARDINE_MAX_BURSTS];
+} sardine_deburst_layout;
+
+/* compute_overlap_lines + the contribution ranges of deburst_subswath
+ * (deburst.rs:172-305).  Host-only; integer results are bit-exact. */
+sardine_status sardine_deburst_layout_compute(const sardine_subswath* sw,
+                                              const sardine_burst* bursts, int32_t n_bursts,
+                                              int64_t tiff_height,
+                                              sardine_deburst_layout* out);
+
+/* One LUT vector list: CalibrationVector.sigma_nought or
+ * NoiseRangeVector.noise_range_lut (calibration.rs:38-82), flattened. */
+typedef struct sardine_lut {
+    int32_t n_vectors;
+    const int32_t* lines;        /* host, n_vectors (sorted ascending) */
+    const int64_t* offsets;      /* host, n_vectors+1 into pixels/values */
+    const uint32_t* pixels;      /* host */
+    const float* values;         /* host */
+} sardine_lut;
+
+/* NoiseAzimuthVector (calibration.rs:86-96); n == 0 → factor 1.0 */
+typedef struct sardine_az_lut {
+    int32_t n;
+    const int32_t* lines;   /* host */
+    const float* values;    /* host */
+} sardine_az_lut;
+
+/* One subswath input to the fused deburst+calibrate+merge pass. */
+typedef struct sardine_swath_input {
+    const int16_t* raw;          /* [device] CInt16 TIFF rows: tiff_height x raw_cols x 2 */
+    int64_t raw_cols;
+    int64_t tiff_height;
+    sardine_subswath swath;
+    const sardine_burst* bursts; /* host, sorted by burst_index */
+    int32_t n_bursts;
+    sardine_lut cal;             /* sigma_nought LUT */
+    sardine_lut noise_range;
+    sardine_az_lut noise_azimuth;
+} sardine_swath_input;
+
+/* MergedSigma0 metadata (merge_subswaths.rs:106-162) + the merge layout. */
+#define SARDINE_MAX_SWATHS 8
+typedef struct sardine_merged_info {
+    int64_t lines, samples;
+    double near_slant_range_time_s;
+    double range_pixel_spacing_s;
+    double range_pixel_spacing_m;
+    int64_t azimuth_start_time_us;
+    uint32_t cal_lut_extrapolation_gap_px;
+    uint32_t noise_lut_extrapolation_gap_px;
+    int32_t n_swaths;
+    int64_t sw_out_offset[SARDINE_MAX_SWATHS];
+    int64_t seam[SARDINE_MAX_SWATHS];          /* seam[i] between swath i and i+1 */
+    int64_t swath_lines[SARDINE_MAX_SWATHS];   /* debursted lines per swath */
+    int64_t swath_samples[SARDINE_MAX_SWATHS];
+    int64_t swath_valid_offset[SARDINE_MAX_SWATHS];
+} sardine_merged_info;
+
+/* Shape + layout only (no device work): deburst_subswath layout per swath,
+ * apply_calibration's LUT coverage checks (apply_calibration.rs:421-462) and
+ * merge_subswaths' offsets/seams (merge_subswaths.rs:207-330). */
+sardine_status sardine_prepare_merged_layout(const sardine_swath_input* swaths, int32_t n_swaths,
+                                             sardine_merged_info* info);
+
+/* prepare_merged_scene's deburst → apply_calibration → merge chain
+ * (scene_prep.rs:598-699; deburst.rs:230, apply_calibration.rs:374,
+ * merge_subswaths.rs:207) fused into one HBM pass: the deburst row copy and
+ * the merge column copy become output addressing of the calibration kernel.
+ * sigma0/nesz [device] hold info.lines*info.samples floats. */
+sardine_status sardine_prepare_merged(const sardine_swath_input* swaths, int32_t n_swaths,
+                                      float* sigma0, float* nesz,
+                                      sardine_merged_info* info, void* stream);
+
+/* apply_calibration(deburst, cal, noise, tiff_line_origin) -> Sigma0Array
+ * (apply_calibration.rs:374-572) on an already debursted CInt16 array.
+ * out/nesz [device] hold lines*samples floats. */
+sardine_status sardine_apply_calibration(const int16_t* deburst /*[device] lines*samples*2*/,
+                                         int64_t lines, int64_t samples,
+                                         int64_t valid_sample_offset,
+                                         const sardine_lut* cal, const sardine_lut* noise_range,
+                                         const sardine_az_lut* noise_azimuth,
+                                         int64_t tiff_line_origin,
+                                         float* out, float* nesz,
+                                         uint32_t* cal_gap_px, uint32_t* noise_gap_px,
+                                         void* stream);
+
+/* ── dB epilogue ────────────────────────────────────────────────────────── */
+
+/* to_db_inplace(data, noise_floor_linear) (export.rs:59-94).  counts[0] =
+ * converted, counts[1] = noise-masked. */
+sardine_status sardine_to_db_inplace(float* data /*[device]*/, int64_t n, float noise_floor_linear,
+                                     uint64_t counts[2], void* stream);
+
+/* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */
+
+/* Fill a CInt16 raster with round(sigma * N(0,1)) real/imag parts from a
+ * counter-based generator keyed on (seed, element index); identical to the
+ * host generator sardine_amd.synthetic.complex_gaussian_dn. */
+sardine_status sardine_synth_cint16(int16_t* out /*[device]*/, int64_t n_complex,
+                                    uint64_t seed, uint64_t index_base, double sigma,
+                                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SARDINE_HIP_H */

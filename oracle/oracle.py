@@ -1,0 +1,329 @@
+"""ctypes front of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.  Every function
+runs the restatement of the reference algorithm on HOST arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+from sardine_amd import abi
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build(force: bool = False) -> pathlib.Path:
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        d, i64, i32, f = C.c_double, C.c_int64, C.c_int, C.c_float
+        P = C.c_void_p
+        L.oracle_geodetic_to_ecef.argtypes = [d, d, d, P]
+        L.oracle_ecef_to_geodetic.argtypes = [P, P]
+        L.oracle_interpolate_orbit_and_accel.argtypes = [C.POINTER(abi.Orbit), i64, P, P, P]
+        L.oracle_solve_zero_doppler.argtypes = [P, C.POINTER(abi.Orbit), i64, i32, d, C.POINTER(i64),
+                                                C.POINTER(i32), P]
+        L.oracle_sample.argtypes = [i32, P, i64, i64, d, d]
+        L.oracle_sample.restype = f
+        L.oracle_keys_weight.argtypes = [d]
+        L.oracle_keys_weight.restype = d
+        L.oracle_lanczos_weight.argtypes = [d]
+        L.oracle_lanczos_weight.restype = d
+        L.oracle_flat_cell_area.argtypes = [d, d, d]
+        L.oracle_flat_cell_area.restype = d
+        L.oracle_terrain_geometry.argtypes = [d, d, d, d, d, d, P, P, d, C.POINTER(d), C.POINTER(d)]
+        L.oracle_dem_elevation.argtypes = [P, i64, i64, d, d, d, d, d, C.POINTER(f)]
+        L.oracle_geoid_undulation.argtypes = [P, d, d]
+        L.oracle_geoid_undulation.restype = d
+        L.oracle_utm_forward.argtypes = [i32, i32, d, d, C.POINTER(d), C.POINTER(d)]
+        L.oracle_utm_inverse.argtypes = [i32, i32, d, d, C.POINTER(d), C.POINTER(d)]
+        L.oracle_project_row.argtypes = [i32, i32, d, d, d, i64, i64, P, P]
+        L.oracle_guess_azimuth_time.argtypes = [C.POINTER(abi.GridPoint), i32, d]
+        L.oracle_guess_azimuth_time.restype = i64
+        L.oracle_tc_output_grid.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.TcConfig), C.POINTER(i64),
+                                            C.POINTER(i64), P]
+        L.oracle_terrain_correction.argtypes = [C.POINTER(abi.RadarImage), C.POINTER(abi.Scene),
+                                                C.POINTER(abi.DemRaster), C.POINTER(abi.TcConfig),
+                                                C.POINTER(abi.GeocodedImage), i32, C.c_char_p]
+        L.oracle_deburst_layout.argtypes = [C.POINTER(abi.SubSwath), C.POINTER(abi.Burst), i32, i64,
+                                            C.POINTER(abi.DeburstLayout)]
+        L.oracle_deburst_copy.argtypes = [P, i64, C.POINTER(abi.Burst), C.POINTER(abi.DeburstLayout), P]
+        L.oracle_deburst_copy.restype = None
+        L.oracle_apply_calibration.argtypes = [P, i64, i64, i64, C.POINTER(abi.Lut), C.POINTER(abi.Lut),
+                                               C.POINTER(abi.AzLut), i64, P, P, C.POINTER(C.c_uint32),
+                                               C.POINTER(C.c_uint32), i32]
+        L.oracle_merge.argtypes = [i32, P, P, P, P, P, P, P, P, P, P, C.POINTER(abi.MergedInfo), i32]
+        L.oracle_speckle.argtypes = [P, i64, i64, i64, C.POINTER(abi.SpeckleFilterC), P, i32]
+        L.oracle_to_db_inplace.argtypes = [P, i64, f, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data
+
+
+def n_threads() -> int:
+    return int(os.environ.get("SARDINE_ORACLE_THREADS", "0"))
+
+
+# ── thin wrappers ───────────────────────────────────────────────────────
+def geodetic_to_ecef(lat, lon, h):
+    out = np.zeros(3)
+    lib().oracle_geodetic_to_ecef(lat, lon, h, _ptr(out))
+    return out
+
+
+def ecef_to_geodetic(e):
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    out = np.zeros(3)
+    lib().oracle_ecef_to_geodetic(_ptr(e), _ptr(out))
+    return out
+
+
+def orbit_c(t_us, pos, vel, keep):
+    n = len(t_us)
+    arr = (abi.StateVector * n)()
+    for i in range(n):
+        arr[i].time_us = int(t_us[i])
+        for c in range(3):
+            arr[i].position_m[c] = float(pos[i][c])
+            arr[i].velocity_m_s[c] = float(vel[i][c])
+    keep.append(arr)
+    return abi.Orbit(arr, n)
+
+
+def interpolate_orbit_and_accel(orbit: abi.Orbit, t_rel_us: int):
+    p, v, a = np.zeros(3), np.zeros(3), np.zeros(3)
+    st = lib().oracle_interpolate_orbit_and_accel(C.byref(orbit), int(t_rel_us), _ptr(p), _ptr(v), _ptr(a))
+    return st, p, v, a
+
+
+def solve_zero_doppler(target, orbit: abi.Orbit, seed_rel_us: int, max_iter=10, tol=1e-6):
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    out_t = C.c_int64()
+    it = C.c_int()
+    sat = np.zeros(3)
+    oc = lib().oracle_solve_zero_doppler(_ptr(t), C.byref(orbit), int(seed_rel_us), max_iter, tol,
+                                         C.byref(out_t), C.byref(it), _ptr(sat))
+    return oc, out_t.value, it.value, sat
+
+
+def sample(kind, data: np.ndarray, line, smp):
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    return lib().oracle_sample(kind, _ptr(d), d.shape[0], d.shape[1], float(line), float(smp))
+
+
+def terrain_geometry(lat, lon, he, hw, hn, hs, sat, target, spacing):
+    s = np.ascontiguousarray(sat, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    w, cl = C.c_double(), C.c_double()
+    oc = lib().oracle_terrain_geometry(lat, lon, he, hw, hn, hs, _ptr(s), _ptr(t), spacing, C.byref(w),
+                                       C.byref(cl))
+    return oc, w.value, cl.value
+
+
+def dem_elevation(dem: np.ndarray, spec, lat, lon):
+    h = C.c_float()
+    ok = lib().oracle_dem_elevation(_ptr(dem), dem.shape[1], dem.shape[0], spec.origin_lat_deg,
+                                    spec.origin_lon_deg, spec.pixel_deg, lat, lon, C.byref(h))
+    return (h.value if ok else None)
+
+
+def geoid_undulation(grid, lat, lon):
+    return lib().oracle_geoid_undulation(None if grid is None else _ptr(grid), lat, lon)
+
+
+def utm_forward(zone, south, lon, lat):
+    x, y = C.c_double(), C.c_double()
+    lib().oracle_utm_forward(zone, int(south), lon, lat, C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def utm_inverse(zone, south, x, y):
+    lon, lat = C.c_double(), C.c_double()
+    lib().oracle_utm_inverse(zone, int(south), x, y, C.byref(lon), C.byref(lat))
+    return lon.value, lat.value
+
+
+def project_row(zone, south, y, x_start, x_step, n_cols, n_anchors):
+    lon = np.zeros(n_cols)
+    lat = np.zeros(n_cols)
+    lib().oracle_project_row(zone, int(south), y, x_start, x_step, n_cols, n_anchors, _ptr(lon), _ptr(lat))
+    return lon, lat
+
+
+def tc_output_grid(scene: abi.Scene, cfg: abi.TcConfig):
+    r, c = C.c_int64(), C.c_int64()
+    gt = np.zeros(6)
+    st = lib().oracle_tc_output_grid(C.byref(scene), C.byref(cfg), C.byref(r), C.byref(c), _ptr(gt))
+    return st, r.value, c.value, gt
+
+
+class TcResult:
+    def __init__(self, status, img: abi.GeocodedImage, data, cos_lia, mask, err):
+        self.status = status
+        self.img = img
+        self.data = data
+        self.cos_lia = cos_lia
+        self.mask = mask
+        self.error = err
+
+    def counters(self):
+        i = self.img
+        return dict(valid=i.valid_pixel_count, dem_missing=i.dem_missing_count,
+                    outside_footprint=i.outside_footprint_count, non_converged=i.non_converged_count,
+                    degenerate=i.degenerate_geometry_count, flat_masked=i.flat_masked_count,
+                    noise_masked=i.noise_masked_count, hist=list(i.newton_iter_histogram))
+
+
+def terrain_correction(sigma0: np.ndarray, nesz, merged_meta: dict, scene: abi.Scene, dem: np.ndarray, dem_spec,
+                       cfg: abi.TcConfig, threads: int = None, coords: bool = False) -> TcResult:
+    """merged_meta: lines, samples, azimuth_start_time_us, near_slant_range_time_s, range_pixel_spacing_m."""
+    sigma0 = np.ascontiguousarray(sigma0, dtype=np.float32)
+    img = abi.RadarImage(_ptr(sigma0), None if nesz is None else _ptr(np.ascontiguousarray(nesz, np.float32)),
+                         merged_meta["lines"], merged_meta["samples"], merged_meta["azimuth_start_time_us"],
+                         merged_meta["near_slant_range_time_s"], merged_meta["range_pixel_spacing_m"])
+    nesz_keep = nesz
+    dem = np.ascontiguousarray(dem, dtype=np.float32)
+    dr = abi.DemRaster(_ptr(dem), dem.shape[1], dem.shape[0], dem_spec.origin_lat_deg, dem_spec.origin_lon_deg,
+                       dem_spec.pixel_deg)
+    st, rows, cols, gt = tc_output_grid(scene, cfg)
+    if st:
+        return TcResult(st, abi.GeocodedImage(), None, None, None, "grid")
+    if cfg.aoi_rows:
+        rows, cols = cfg.aoi_rows, cfg.aoi_cols
+    data = np.empty((rows, cols), np.float32)
+    cl = np.empty((rows, cols), np.float32) if cfg.compute_lia else None
+    mk = np.empty((rows, cols), np.uint8) if cfg.compute_lia else None
+    rc = np.empty((rows, cols, 2), np.float64) if coords else None
+    out = abi.GeocodedImage()
+    out.radar_coords = None if rc is None else _ptr(rc)
+    out.data = _ptr(data)
+    out.cos_lia = None if cl is None else _ptr(cl)
+    out.mask = None if mk is None else _ptr(mk)
+    err = C.create_string_buffer(256)
+    st = lib().oracle_terrain_correction(C.byref(img), C.byref(scene), C.byref(dr), C.byref(cfg), C.byref(out),
+                                         n_threads() if threads is None else threads, err)
+    del nesz_keep
+    r = TcResult(st, out, data, cl, mk, err.value.decode())
+    r.coords = rc
+    return r
+
+
+def deburst_layout(sw: abi.SubSwath, bursts, n, tiff_height):
+    L = abi.DeburstLayout()
+    st = lib().oracle_deburst_layout(C.byref(sw), bursts, n, tiff_height, C.byref(L))
+    return st, L
+
+
+def deburst_copy(raw: np.ndarray, bursts, L: abi.DeburstLayout):
+    out = np.empty((L.lines, L.samples, 2), np.int16)
+    raw = np.ascontiguousarray(raw)
+    lib().oracle_deburst_copy(_ptr(raw), raw.shape[1], bursts, C.byref(L), _ptr(out))
+    return out
+
+
+def apply_calibration(deb: np.ndarray, col_off: int, cal: abi.Lut, nr: abi.Lut, az: abi.AzLut, origin: int,
+                      threads: int = None):
+    deb = np.ascontiguousarray(deb, dtype=np.int16)
+    lines, samples = deb.shape[0], deb.shape[1]
+    out = np.empty((lines, samples), np.float32)
+    nesz = np.empty((lines, samples), np.float32)
+    g1, g2 = C.c_uint32(), C.c_uint32()
+    st = lib().oracle_apply_calibration(_ptr(deb), lines, samples, col_off, C.byref(cal), C.byref(nr), C.byref(az),
+                                        origin, _ptr(out), _ptr(nesz), C.byref(g1), C.byref(g2),
+                                        n_threads() if threads is None else threads)
+    return st, out, nesz, g1.value, g2.value
+
+
+def merge(sigmas, neszs, valid_offs, slant_t0s, rpss, az_starts, query_only=False):
+    n = len(sigmas)
+    sp = (C.c_void_p * n)(*[_ptr(s) for s in sigmas])
+    npp = (C.c_void_p * n)(*[_ptr(s) for s in neszs])
+    lines = np.array([s.shape[0] for s in sigmas], np.int64)
+    samples = np.array([s.shape[1] for s in sigmas], np.int64)
+    vo = np.array(valid_offs, np.int64)
+    st0 = np.array(slant_t0s, np.float64)
+    rp = np.array(rpss, np.float64)
+    az = np.array(az_starts, np.int64)
+    info = abi.MergedInfo()
+    st = lib().oracle_merge(n, sp, npp, _ptr(lines), _ptr(samples), _ptr(vo), _ptr(st0), _ptr(rp), _ptr(az),
+                            None, None, C.byref(info), 1)
+    if st or query_only:
+        return st, info, None, None
+    o = np.empty((info.lines, info.samples), np.float32)
+    on = np.empty((info.lines, info.samples), np.float32)
+    st = lib().oracle_merge(n, sp, npp, _ptr(lines), _ptr(samples), _ptr(vo), _ptr(st0), _ptr(rp), _ptr(az),
+                            _ptr(o), _ptr(on), C.byref(info), 0)
+    return st, info, o, on
+
+
+def speckle(data: np.ndarray, kind, window=7, enl=1.0, damping=1.0, cols=None, rows=None, threads: int = None):
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    if rows is None:
+        rows, cols = d.shape
+    out = np.empty(rows * cols if rows * cols > 0 else 1, np.float32)
+    f = abi.SpeckleFilterC(kind, window, enl, damping)
+    st = lib().oracle_speckle(_ptr(d), d.size, cols, rows, C.byref(f), _ptr(out),
+                              n_threads() if threads is None else threads)
+    return st, out[: rows * cols].reshape(rows, cols) if st == 0 else None
+
+
+def to_db_inplace(data: np.ndarray, floor_lin: float):
+    counts = np.zeros(2, np.uint64)
+    st = lib().oracle_to_db_inplace(_ptr(data), data.size, floor_lin, _ptr(counts))
+    return st, int(counts[0]), int(counts[1])
+
+
+# ── the full chain on a synthetic scene (host) ──────────────────────────
+def prepare_merged(scene_obj, raws=None, threads: int = None):
+    """deburst → apply_calibration → merge for every swath of a
+    sardine_amd.synthetic.SyntheticScene, as the reference's
+    prepare_merged_scene sequences them (scene_prep.rs:598-699)."""
+    sigmas, neszs, voffs, st0s, rpss, azs = [], [], [], [], [], []
+    gaps = []
+    layouts = []
+    for i, sw in enumerate(scene_obj.swaths):
+        raw = sw.raw_host() if raws is None else raws[i]
+        b = sw.bursts_c()
+        swc = abi.SubSwath(sw.n_bursts, sw.lines_per_burst, scene_obj.ati, sw.slant_range_time_s, scene_obj.rps_m)
+        st, L = deburst_layout(swc, b, sw.n_bursts, sw.tiff_height)
+        if st:
+            raise RuntimeError(f"deburst layout status {st}")
+        layouts.append(L)
+        deb = deburst_copy(raw, b, L)
+        st, s0, nz, g1, g2 = apply_calibration(deb, L.valid_sample_offset, sw.cal.as_c(), sw.noise.as_c(),
+                                               sw.noise_az.as_c(), 0, threads)
+        if st:
+            raise RuntimeError(f"calibration status {st}")
+        gaps.append((g1, g2))
+        sigmas.append(s0)
+        neszs.append(nz)
+        voffs.append(L.valid_sample_offset)
+        st0s.append(sw.slant_range_time_s)
+        rpss.append(scene_obj.rps_m)
+        azs.append(int(sw.burst_times_us[0]))
+    st, info, o, on = merge(sigmas, neszs, voffs, st0s, rpss, azs)
+    if st:
+        raise RuntimeError(f"merge status {st}")
+    info.cal_lut_extrapolation_gap_px = max(g[0] for g in gaps)
+    info.noise_lut_extrapolation_gap_px = max(g[1] for g in gaps)
+    return info, o, on, layouts

@@ -1,0 +1,68 @@
+"""Loader of the native library sardine_amd/libsardine_hip.so.
+
+There is no fallback: if the library is missing or cannot be loaded, every
+call raises NativeLibraryError.  Build it with `python -m sardine_amd.build`
+(or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+from . import abi
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libsardine_hip.so"
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise NativeLibraryError(f"{LIB_PATH} is not built; run `python -m sardine_amd.build`")
+    try:
+        L = C.CDLL(str(LIB_PATH))
+    except OSError as e:
+        raise NativeLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+    P, i32, i64, d, f, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_float, C.c_uint64
+    L.sardine_last_error.restype = C.c_char_p
+    L.sardine_last_error.argtypes = []
+    L.sardine_abi_version.restype = C.c_int
+    L.sardine_device_count.restype = C.c_int
+    L.sardine_tc_config_default.argtypes = [C.POINTER(abi.TcConfig)]
+    L.sardine_tc_config_default.restype = None
+    L.sardine_tc_output_grid.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.TcConfig), C.POINTER(i64),
+                                         C.POINTER(i64), P]
+    L.sardine_terrain_correction.argtypes = [C.POINTER(abi.RadarImage), C.POINTER(abi.Scene),
+                                             C.POINTER(abi.DemRaster), C.POINTER(abi.TcConfig),
+                                             C.POINTER(abi.GeocodedImage), P]
+    L.sardine_speckle_apply.argtypes = [P, i64, i64, i64, C.POINTER(abi.SpeckleFilterC), P, P]
+    L.sardine_deburst_layout_compute.argtypes = [C.POINTER(abi.SubSwath), C.POINTER(abi.Burst), i32, i64,
+                                                 C.POINTER(abi.DeburstLayout)]
+    L.sardine_prepare_merged_layout.argtypes = [C.POINTER(abi.SwathInput), i32, C.POINTER(abi.MergedInfo)]
+    L.sardine_prepare_merged.argtypes = [C.POINTER(abi.SwathInput), i32, P, P, C.POINTER(abi.MergedInfo), P]
+    L.sardine_apply_calibration.argtypes = [P, i64, i64, i64, C.POINTER(abi.Lut), C.POINTER(abi.Lut),
+                                            C.POINTER(abi.AzLut), i64, P, P, C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32), P]
+    L.sardine_to_db_inplace.argtypes = [P, i64, f, P, P]
+    L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]
+    for name in abi.EXPORTED_SYMBOLS:
+        if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",
+                        "sardine_tc_config_default"):
+            getattr(L, name).restype = C.c_int
+    if L.sardine_abi_version() != abi.ABI_VERSION:
+        raise NativeLibraryError(f"ABI version mismatch: library {L.sardine_abi_version()}, "
+                                 f"python {abi.ABI_VERSION}")
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().sardine_last_error().decode(errors="replace")

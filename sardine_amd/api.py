@@ -1,0 +1,376 @@
+"""Python mirror of the reference interface of the calibrate-and-geocode path.
+
+Names and argument meaning follow the Rust crate:
+  terrain_correction(merged, scene, dem, config) -> GeocodedImage
+        sardine/src/terrain_correction.rs:1245-1251 (the `grids` argument is
+        carried by SceneInputs.grid_points: the IW2 grid or the first one)
+  TerrainCorrectionConfig.new(geoid)         terrain_correction.rs:223-235
+  SpeckleFilter.{Boxcar,Lee,Frost,GammaMap,RefinedLee}(...).apply(data, cols, rows)
+        speckle.rs:61-91, 586-603;  apply_speckle_filter speckle.rs:131
+  prepare_merged_scene(swaths) -> MergedSigma0
+        deburst → apply_calibration → merge_subswaths, scene_prep.rs:598-699
+  apply_calibration(...) -> Sigma0Array      apply_calibration.rs:374
+  deburst_layout(...)                        deburst.rs:172-305
+  to_db_inplace(data, noise_floor_linear)    export.rs:59-94
+Rasters are torch tensors on the GPU (torch is used only to own device
+memory and streams); every computation runs in libsardine_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from ._native import last_error, lib
+from .errors import SpeckleError, raise_for
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _check(status: int):
+    if status != abi.OK:
+        raise_for(status, last_error())
+
+
+# ─── inputs ───────────────────────────────────────────────────────────────
+@dataclass
+class SceneInputs:
+    """The parts of SceneMetadata + geolocation grids terrain_correction reads."""
+    sv_time_us: np.ndarray      # (n,) int64
+    sv_pos: np.ndarray          # (n, 3)
+    sv_vel: np.ndarray          # (n, 3)
+    bbox: tuple                 # (min_lat, max_lat, min_lon, max_lon)
+    azimuth_time_interval_s: float
+    grid_lat: np.ndarray        # seed-LUT grid points
+    grid_time_us: np.ndarray
+    n_subswaths: int = 1
+    _keep: list = field(default_factory=list, repr=False)
+
+    def as_c(self) -> abi.Scene:
+        n = len(self.sv_time_us)
+        svs = (abi.StateVector * max(n, 1))()
+        for i in range(n):
+            svs[i].time_us = int(self.sv_time_us[i])
+            for c in range(3):
+                svs[i].position_m[c] = float(self.sv_pos[i][c])
+                svs[i].velocity_m_s[c] = float(self.sv_vel[i][c])
+        m = len(self.grid_time_us)
+        gp = (abi.GridPoint * max(m, 1))()
+        for i in range(m):
+            gp[i] = abi.GridPoint(float(self.grid_lat[i]), int(self.grid_time_us[i]))
+        self._keep = [svs, gp]
+        return abi.Scene(abi.Orbit(svs, n), abi.BBox(*self.bbox), float(self.azimuth_time_interval_s), gp, m,
+                         int(self.n_subswaths))
+
+    @classmethod
+    def from_synthetic(cls, sc) -> "SceneInputs":
+        return cls(sc.sv_t_us, sc.sv_pos, sc.sv_vel, sc.bbox, sc.ati, sc.grid_points, sc.grid_t_us, len(sc.swaths))
+
+
+@dataclass
+class MergedSigma0:
+    """Raster-backed RadarImage (merge_subswaths.rs:106-162) in HBM."""
+    data: object                # torch.float32 (lines, samples) on cuda
+    nesz: object                # same shape, or None
+    near_slant_range_time_s: float
+    range_pixel_spacing_m: float
+    azimuth_start_time_us: int
+    range_pixel_spacing_s: float = 0.0
+    cal_lut_extrapolation_gap_px: int = 0
+    noise_lut_extrapolation_gap_px: int = 0
+    info: Optional[abi.MergedInfo] = None
+
+    @property
+    def lines(self):
+        return int(self.data.shape[0])
+
+    @property
+    def samples(self):
+        return int(self.data.shape[1])
+
+    def as_c(self) -> abi.RadarImage:
+        return abi.RadarImage(self.data.data_ptr(), None if self.nesz is None else self.nesz.data_ptr(),
+                              self.lines, self.samples, int(self.azimuth_start_time_us),
+                              float(self.near_slant_range_time_s), float(self.range_pixel_spacing_m))
+
+
+@dataclass
+class DemRaster:
+    """Geographic DEM raster with the GLO-30 sampling rule (dem.rs:351-409)."""
+    data: object                # torch.float32 (height, width) on cuda
+    origin_lat_deg: float       # centre of pixel (0, 0)
+    origin_lon_deg: float
+    pixel_deg: float
+
+    def as_c(self) -> abi.DemRaster:
+        return abi.DemRaster(self.data.data_ptr(), int(self.data.shape[1]), int(self.data.shape[0]),
+                             float(self.origin_lat_deg), float(self.origin_lon_deg), float(self.pixel_deg))
+
+
+# ─── terrain correction ───────────────────────────────────────────────────
+class ResamplingKernel:
+    Bilinear = abi.RESAMPLE_BILINEAR
+    Bicubic = abi.RESAMPLE_BICUBIC
+    Lanczos3 = abi.RESAMPLE_LANCZOS3
+
+
+@dataclass
+class TerrainCorrectionConfig:
+    """TerrainCorrectionConfig (terrain_correction.rs:100-235)."""
+    geoid_grid: Optional[np.ndarray] = None       # None → GeoidModel::Zero; else 73x144 f32 grid
+    pixel_spacing_deg: float = 0.0001
+    crs: int = abi.CRS_WGS84
+    utm_zone: int = 0
+    max_newton_iterations: int = 10
+    newton_tolerance_s: float = 1e-6
+    flatten: bool = False
+    compute_lia: bool = False
+    noise_floor_margin_db: float = float("-inf")
+    resampling: int = abi.RESAMPLE_BILINEAR
+    projector_anchors: int = 256
+    precision: int = abi.PRECISION_FP64
+    aoi: Optional[tuple] = None                    # (row0, col0, rows, cols)
+
+    @classmethod
+    def new(cls, geoid_grid=None) -> "TerrainCorrectionConfig":
+        return cls(geoid_grid=geoid_grid)
+
+    def as_c(self) -> abi.TcConfig:
+        c = abi.TcConfig()
+        lib().sardine_tc_config_default(C.byref(c))
+        c.pixel_spacing = self.pixel_spacing_deg
+        c.crs = self.crs
+        c.utm_zone = self.utm_zone
+        c.max_newton_iterations = self.max_newton_iterations
+        c.newton_tolerance_s = self.newton_tolerance_s
+        c.flatten = int(self.flatten)
+        c.compute_lia = int(self.compute_lia)
+        c.noise_floor_margin_db = self.noise_floor_margin_db
+        c.resampling = self.resampling
+        c.projector_anchors = self.projector_anchors
+        c.precision = self.precision
+        if self.geoid_grid is not None:
+            g = np.ascontiguousarray(self.geoid_grid, dtype=np.float32).reshape(73, 144)
+            self._geoid_keep = g
+            c.geoid = abi.Geoid(abi.GEOID_GRID_2P5, g.ctypes.data)
+        if self.aoi is not None:
+            c.aoi_row0, c.aoi_col0, c.aoi_rows, c.aoi_cols = (int(v) for v in self.aoi)
+        return c
+
+
+@dataclass
+class GeocodedImage:
+    """GeocodedImage (terrain_correction.rs:245-307) with device rasters."""
+    data: object
+    rows: int
+    cols: int
+    geotransform: tuple
+    total_pixel_count: int
+    valid_pixel_count: int
+    dem_missing_count: int
+    outside_footprint_count: int
+    non_converged_count: int
+    degenerate_geometry_count: int
+    flat_masked_count: int
+    noise_masked_count: int
+    newton_iter_histogram: list
+    cos_lia: object = None
+    mask: object = None
+    full_rows: int = 0
+    full_cols: int = 0
+    chain_redo_rounds: int = 0
+    radar_coords: object = None
+
+    @property
+    def origin_lat_deg(self):
+        return self.geotransform[3]
+
+    @property
+    def origin_lon_deg(self):
+        return self.geotransform[0]
+
+    def valid_fraction(self) -> float:
+        return self.valid_pixel_count / self.total_pixel_count
+
+
+def output_grid(scene: SceneInputs, config: TerrainCorrectionConfig):
+    sc = scene.as_c()
+    cfg = config.as_c()
+    r, c = C.c_int64(), C.c_int64()
+    gt = (C.c_double * 6)()
+    _check(lib().sardine_tc_output_grid(C.byref(sc), C.byref(cfg), C.byref(r), C.byref(c), gt))
+    return r.value, c.value, tuple(gt)
+
+
+def terrain_correction(merged: MergedSigma0, scene: SceneInputs, dem: DemRaster,
+                       config: TerrainCorrectionConfig, stream=None, out=None,
+                       radar_coords: bool = False) -> GeocodedImage:
+    torch = _torch()
+    sc = scene.as_c()
+    cfg = config.as_c()
+    r, c, _ = output_grid(scene, config)
+    if config.aoi is not None:
+        r, c = int(config.aoi[2]), int(config.aoi[3])
+    dev = merged.data.device
+    if out is None:
+        data = torch.empty((r, c), dtype=torch.float32, device=dev)
+        cl = torch.empty((r, c), dtype=torch.float32, device=dev) if config.compute_lia else None
+        mk = torch.empty((r, c), dtype=torch.uint8, device=dev) if config.compute_lia else None
+        rc = torch.empty((r, c, 2), dtype=torch.float64, device=dev) if radar_coords else None
+    else:
+        data, cl, mk = out[:3]
+        rc = out[3] if len(out) > 3 else None
+    img = abi.GeocodedImage()
+    img.radar_coords = None if rc is None else rc.data_ptr()
+    img.data = data.data_ptr()
+    img.cos_lia = None if cl is None else cl.data_ptr()
+    img.mask = None if mk is None else mk.data_ptr()
+    rim = merged.as_c()
+    dr = dem.as_c()
+    _check(lib().sardine_terrain_correction(C.byref(rim), C.byref(sc), C.byref(dr), C.byref(cfg), C.byref(img),
+                                            _stream_ptr(stream)))
+    return GeocodedImage(data, img.rows, img.cols, tuple(img.geotransform), img.total_pixel_count,
+                         img.valid_pixel_count, img.dem_missing_count, img.outside_footprint_count,
+                         img.non_converged_count, img.degenerate_geometry_count, img.flat_masked_count,
+                         img.noise_masked_count, list(img.newton_iter_histogram), cl, mk, img.full_rows,
+                         img.full_cols, img.chain_redo_rounds, rc)
+
+
+# ─── speckle ──────────────────────────────────────────────────────────────
+@dataclass(frozen=True)
+class SpeckleFilter:
+    """SpeckleFilter (speckle.rs:61-91); `.apply` is SpeckleKernel::apply."""
+    kind: int
+    window: int = 7
+    enl: float = 1.0
+    damping: float = 1.0
+
+    @staticmethod
+    def Boxcar(window: int) -> "SpeckleFilter":
+        return SpeckleFilter(abi.SPECKLE_BOXCAR, window)
+
+    @staticmethod
+    def Lee(window: int, enl: float) -> "SpeckleFilter":
+        return SpeckleFilter(abi.SPECKLE_LEE, window, enl=enl)
+
+    @staticmethod
+    def Frost(window: int, damping: float) -> "SpeckleFilter":
+        return SpeckleFilter(abi.SPECKLE_FROST, window, damping=damping)
+
+    @staticmethod
+    def GammaMap(window: int, enl: float) -> "SpeckleFilter":
+        return SpeckleFilter(abi.SPECKLE_GAMMA_MAP, window, enl=enl)
+
+    @staticmethod
+    def RefinedLee(enl: float) -> "SpeckleFilter":
+        return SpeckleFilter(abi.SPECKLE_REFINED_LEE, 7, enl=enl)
+
+    def as_c(self) -> abi.SpeckleFilterC:
+        return abi.SpeckleFilterC(self.kind, self.window, self.enl, self.damping)
+
+    def apply(self, data, cols: int, rows: int, stream=None, out=None):
+        return apply_speckle_filter(data, cols, rows, self, stream=stream, out=out)
+
+
+def apply_speckle_filter(data, cols: int, rows: int, filt: SpeckleFilter, stream=None, out=None):
+    """Returns a fresh device tensor (speckle.rs:131-180 is out-of-place)."""
+    torch = _torch()
+    flat = data.reshape(-1)
+    if out is None:
+        out = torch.empty(max(rows * cols, 0), dtype=torch.float32, device=data.device)
+    f = filt.as_c()
+    _check(lib().sardine_speckle_apply(flat.data_ptr() if flat.numel() else None, flat.numel(), cols, rows,
+                                       C.byref(f), out.data_ptr() if out.numel() else None, _stream_ptr(stream)))
+    return out.reshape(rows, cols) if rows * cols > 0 else out
+
+
+def apply_speckle_step(data, cols: int, rows: int, choice: Optional[SpeckleFilter], stream=None):
+    """run.rs:108-123: replaces `data` with the filtered raster (or leaves it)."""
+    if choice is None:
+        return data
+    return choice.apply(data, cols, rows, stream=stream)
+
+
+# ─── deburst / calibration / merge ────────────────────────────────────────
+def deburst_layout(swath: abi.SubSwath, bursts, n_bursts: int, tiff_height: int) -> abi.DeburstLayout:
+    L = abi.DeburstLayout()
+    _check(lib().sardine_deburst_layout_compute(C.byref(swath), bursts, n_bursts, tiff_height, C.byref(L)))
+    return L
+
+
+def merged_layout(swath_inputs, n: int) -> abi.MergedInfo:
+    info = abi.MergedInfo()
+    _check(lib().sardine_prepare_merged_layout(swath_inputs, n, C.byref(info)))
+    return info
+
+
+def prepare_merged_scene(swath_inputs, n: int, device="cuda", stream=None, out=None) -> MergedSigma0:
+    """deburst → apply_calibration → merge (scene_prep.rs:598-699) in one pass."""
+    torch = _torch()
+    info = merged_layout(swath_inputs, n)
+    if out is None:
+        s0 = torch.empty((info.lines, info.samples), dtype=torch.float32, device=device)
+        nz = torch.empty((info.lines, info.samples), dtype=torch.float32, device=device)
+    else:
+        s0, nz = out
+    _check(lib().sardine_prepare_merged(swath_inputs, n, s0.data_ptr(), nz.data_ptr(), C.byref(info),
+                                        _stream_ptr(stream)))
+    return MergedSigma0(s0, nz, info.near_slant_range_time_s, info.range_pixel_spacing_m,
+                        info.azimuth_start_time_us, info.range_pixel_spacing_s, info.cal_lut_extrapolation_gap_px,
+                        info.noise_lut_extrapolation_gap_px, info)
+
+
+@dataclass
+class Sigma0Array:
+    data: object
+    nesz: object
+    lines: int
+    samples: int
+    valid_sample_offset: int
+    cal_lut_extrapolation_gap_px: int
+    noise_lut_extrapolation_gap_px: int
+
+
+def apply_calibration(deburst, valid_sample_offset: int, cal: abi.Lut, noise_range: abi.Lut,
+                      noise_azimuth: abi.AzLut, tiff_line_origin: int, stream=None) -> Sigma0Array:
+    """apply_calibration (apply_calibration.rs:374-572); `deburst` is an int16
+    device tensor of shape (lines, samples, 2)."""
+    torch = _torch()
+    lines, samples = int(deburst.shape[0]), int(deburst.shape[1])
+    out = torch.empty((lines, samples), dtype=torch.float32, device=deburst.device)
+    nz = torch.empty((lines, samples), dtype=torch.float32, device=deburst.device)
+    g1, g2 = C.c_uint32(), C.c_uint32()
+    _check(lib().sardine_apply_calibration(deburst.data_ptr(), lines, samples, valid_sample_offset, C.byref(cal),
+                                           C.byref(noise_range), C.byref(noise_azimuth), tiff_line_origin,
+                                           out.data_ptr(), nz.data_ptr(), C.byref(g1), C.byref(g2),
+                                           _stream_ptr(stream)))
+    return Sigma0Array(out, nz, lines, samples, valid_sample_offset, g1.value, g2.value)
+
+
+def to_db_inplace(data, noise_floor_linear: float, stream=None):
+    """export.rs:59-94; returns (converted, noise_masked)."""
+    counts = (C.c_uint64 * 2)()
+    _check(lib().sardine_to_db_inplace(data.data_ptr(), data.numel(), noise_floor_linear, counts,
+                                       _stream_ptr(stream)))
+    return int(counts[0]), int(counts[1])
+
+
+def synth_cint16(out, seed: int, index_base: int = 0, sigma: float = 100.0, stream=None):
+    """Fill an int16 device tensor (..., 2) with the synthetic DN generator."""
+    _check(lib().sardine_synth_cint16(out.data_ptr(), out.numel() // 2, seed, index_base, sigma,
+                                      _stream_ptr(stream)))
+    return out

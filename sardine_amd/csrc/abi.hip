@@ -1,0 +1,806 @@
+// abi.hip — host side of the C ABI declared in include/sardine_hip.h.
+//
+// This is the MI355X replacement of the Rust host code of the path: it
+// performs the reference's host-level set-up exactly (grid extent, seed LUT,
+// deburst/merge integer layout, LUT coverage checks, per-row constants),
+// stages the small metadata, launches the kernels and maps device-side
+// outcomes back onto the reference's error taxonomy.  There is no CPU
+// compute path: without a device every compute entry point fails with
+// SARDINE_ERR_NO_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sardine_hip.h"
+#include "cal_params.hpp"
+#include "geom.hpp"
+#include "tc_params.hpp"
+
+extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream);
+extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* values, const int64_t* offsets,
+                                          int n_vec, int64_t dense_cols, float* dense, hipStream_t stream);
+extern "C" hipError_t sd_launch_cal_merge(const sd::CalParams* p, hipStream_t stream);
+extern "C" hipError_t sd_launch_speckle(int kind, const float* in, float* out, int64_t cols, int64_t rows, int half,
+                                        float enl, float damping, hipStream_t stream);
+extern "C" hipError_t sd_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
+extern "C" hipError_t sd_launch_to_db(float* d, int64_t n, float floor_lin, unsigned long long* counts,
+                                      hipStream_t s);
+extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
+                                             hipStream_t s);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+sardine_status fail(sardine_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define SD_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (call);                                                                       \
+        if (_e != hipSuccess)                                                                         \
+            return fail(_e == hipErrorOutOfMemory ? SARDINE_ERR_OUT_OF_MEMORY : SARDINE_ERR_HIP,      \
+                        "%s failed: %s", #call, hipGetErrorString(_e));                               \
+    } while (0)
+
+// Device presence + gfx950 check, once per process.
+sardine_status require_device() {
+    static int state = -1;  // -1 unknown, 0 ok, 1 none
+    static std::string why;
+    if (state < 0) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            state = 1;
+            why = "no HIP device visible";
+        } else {
+            int dev = 0;
+            hipDeviceProp_t prop;
+            if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+                state = 1;
+                why = "cannot query the HIP device";
+            } else if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+                state = 1;
+                why = std::string("device is ") + prop.gcnArchName + ", the library is built for gfx950";
+            } else {
+                state = 0;
+            }
+        }
+    }
+    if (state) return fail(SARDINE_ERR_NO_DEVICE, "%s (no CPU fallback exists)", why.c_str());
+    return SARDINE_OK;
+}
+
+// Stream-ordered device scratch.
+struct DevBuf {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    hipError_t alloc(size_t bytes, hipStream_t stream) {
+        s = stream;
+        return hipMallocAsync(&p, bytes ? bytes : 1, stream);
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <typename T>
+hipError_t upload(DevBuf& b, const T* host, size_t n, hipStream_t s) {
+    hipError_t e = b.alloc(n * sizeof(T), s);
+    if (e != hipSuccess) return e;
+    if (n) return hipMemcpyAsync(b.p, host, n * sizeof(T), hipMemcpyHostToDevice, s);
+    return hipSuccess;
+}
+
+// ── seed LUT (terrain_correction.rs:1080-1137) ──────────────────────────
+struct SeedLut {
+    std::vector<double> lat;
+    std::vector<int64_t> t;
+};
+
+SeedLut build_seed_lut(const sardine_grid_point* pts, int n) {
+    std::vector<std::pair<double, int64_t>> v;
+    v.reserve(n);
+    for (int i = 0; i < n; ++i) v.emplace_back(pts[i].latitude_deg, pts[i].azimuth_time_us);
+    std::stable_sort(v.begin(), v.end(), [](auto& a, auto& b) { return a.first < b.first; });
+    SeedLut l;
+    for (auto& e : v) {
+        if (!l.lat.empty() && std::fabs(e.first - l.lat.back()) < 1e-6) continue;  // dedup_by keeps the first
+        l.lat.push_back(e.first);
+        l.t.push_back(e.second);
+    }
+    return l;
+}
+
+int64_t guess_azimuth_time(const SeedLut& l, double lat) {
+    const size_t n = l.lat.size();
+    if (n == 1) return l.t[0];
+    size_t lo = 0, hi = n;
+    while (lo < hi) {  // binary_search_by: an exact hit returns that entry
+        size_t mid = lo + (hi - lo) / 2;
+        if (l.lat[mid] < lat) lo = mid + 1;
+        else if (l.lat[mid] > lat) hi = mid;
+        else return l.t[mid];
+    }
+    if (lo == 0) return l.t[0];
+    if (lo >= n) return l.t[n - 1];
+    double frac = (lat - l.lat[lo - 1]) / (l.lat[lo] - l.lat[lo - 1]);
+    double dt_us = (double)(l.t[lo] - l.t[lo - 1]);
+    return l.t[lo - 1] + (int64_t)(frac * dt_us);
+}
+
+struct Grid {
+    int64_t rows = 0, cols = 0;
+    double y_top = 0, y_bot = 0, x_left = 0, x_right = 0;
+};
+
+// compute_output_grid_extent + n_rows/n_cols (terrain_correction.rs:1174-1216, 1304-1317)
+sardine_status output_grid(const sardine_scene* sc, const sardine_tc_config* cfg, Grid* g) {
+    const double sp = cfg->pixel_spacing;
+    if (!(sp > 0.0)) return fail(SARDINE_ERR_TC_CONFIG, "pixel spacing must be positive");
+    const sardine_bbox& bb = sc->bounding_box;
+    if (cfg->crs == SARDINE_CRS_WGS84) {
+        g->y_top = std::ceil(bb.max_lat_deg / sp) * sp;
+        g->y_bot = std::floor(bb.min_lat_deg / sp) * sp;
+        g->x_left = std::floor(bb.min_lon_deg / sp) * sp;
+        g->x_right = std::ceil(bb.max_lon_deg / sp) * sp;
+    } else if (cfg->crs == SARDINE_CRS_UTM_NORTH || cfg->crs == SARDINE_CRS_UTM_SOUTH) {
+        if (cfg->utm_zone < 1 || cfg->utm_zone > 60)
+            return fail(SARDINE_ERR_TC_PROJECTION, "UTM zone %d outside 1..60", cfg->utm_zone);
+        const sd::TmSeries ts = sd::tm_series();
+        const bool south = cfg->crs == SARDINE_CRS_UTM_SOUTH;
+        const double c[4][2] = {{bb.min_lon_deg, bb.min_lat_deg}, {bb.min_lon_deg, bb.max_lat_deg},
+                                {bb.max_lon_deg, bb.min_lat_deg}, {bb.max_lon_deg, bb.max_lat_deg}};
+        double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+        for (auto& q : c) {
+            double x, y;
+            sd::utm_forward(ts, cfg->utm_zone, south, q[0], q[1], &x, &y);
+            if (!std::isfinite(x) || !std::isfinite(y))
+                return fail(SARDINE_ERR_TC_PROJECTION, "scene corner (%.4f, %.4f) does not project", q[0], q[1]);
+            x0 = std::min(x0, x); x1 = std::max(x1, x);
+            y0 = std::min(y0, y); y1 = std::max(y1, y);
+        }
+        g->y_top = std::ceil(y1 / sp) * sp;
+        g->y_bot = std::floor(y0 / sp) * sp;
+        g->x_left = std::floor(x0 / sp) * sp;
+        g->x_right = std::ceil(x1 / sp) * sp;
+    } else {
+        return fail(SARDINE_ERR_TC_PROJECTION, "unsupported output CRS kind %d", cfg->crs);
+    }
+    const double nr = std::round((g->y_top - g->y_bot) / sp), nc = std::round((g->x_right - g->x_left) / sp);
+    g->rows = nr > 0 ? (int64_t)nr : 0;
+    g->cols = nc > 0 ? (int64_t)nc : 0;
+    if (g->rows == 0 || g->cols == 0)
+        return fail(SARDINE_ERR_TC_CONFIG, "output grid is empty: %lld rows x %lld cols", (long long)g->rows,
+                    (long long)g->cols);
+    return SARDINE_OK;
+}
+
+}  // namespace
+
+// ═════════════════════════════════════════════════════════════════════════
+extern "C" {
+
+const char* sardine_last_error(void) { return g_last_error.c_str(); }
+int sardine_abi_version(void) { return SARDINE_ABI_VERSION; }
+
+int sardine_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void sardine_tc_config_default(sardine_tc_config* c) {
+    // TerrainCorrectionConfig::new (terrain_correction.rs:223-235), GeoidModel::Zero
+    std::memset(c, 0, sizeof *c);
+    c->pixel_spacing = 0.0001;
+    c->crs = SARDINE_CRS_WGS84;
+    c->max_newton_iterations = 10;
+    c->newton_tolerance_s = 1e-6;
+    c->noise_floor_margin_db = -INFINITY;
+    c->resampling = SARDINE_RESAMPLE_BILINEAR;
+    c->projector_anchors = 256;
+    c->precision = SARDINE_PRECISION_FP64;
+    c->geoid.kind = SARDINE_GEOID_ZERO;
+}
+
+sardine_status sardine_tc_output_grid(const sardine_scene* scene, const sardine_tc_config* cfg, int64_t* rows,
+                                      int64_t* cols, double gt[6]) {
+    if (!scene || !cfg || !rows || !cols) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    Grid g;
+    sardine_status st = output_grid(scene, cfg, &g);
+    if (st) return st;
+    *rows = g.rows;
+    *cols = g.cols;
+    if (gt) {  // terrain_correction.rs:1805-1812
+        gt[0] = g.x_left; gt[1] = cfg->pixel_spacing; gt[2] = 0.0;
+        gt[3] = g.y_top; gt[4] = 0.0; gt[5] = -cfg->pixel_spacing;
+    }
+    return SARDINE_OK;
+}
+
+sardine_status sardine_terrain_correction(const sardine_radar_image* merged, const sardine_scene* scene,
+                                          const sardine_dem_raster* dem, const sardine_tc_config* cfg,
+                                          sardine_geocoded_image* out, void* stream_v) {
+    if (!merged || !scene || !dem || !cfg || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t stream = (hipStream_t)stream_v;
+    // ── validation in the reference's order (terrain_correction.rs:1262-1317) ──
+    if (scene->n_grid_points <= 0 || !scene->grid_points)
+        return fail(SARDINE_ERR_TC_NO_GRID_POINTS, "no geolocation grid points provided");
+    if (!(cfg->pixel_spacing > 0.0)) return fail(SARDINE_ERR_TC_CONFIG, "pixel spacing must be positive");
+    if (scene->n_subswaths <= 0) return fail(SARDINE_ERR_TC_NO_SUBSWATHS, "scene has no subswaths");
+    const double ati = scene->azimuth_time_interval_s;
+    if (!std::isfinite(ati) || ati <= 0.0)
+        return fail(SARDINE_ERR_TC_CONFIG, "invalid azimuth_time_interval_s in subswath metadata: %g", ati);
+    SeedLut lut = build_seed_lut(scene->grid_points, scene->n_grid_points);
+    Grid g;
+    sardine_status st = output_grid(scene, cfg, &g);
+    if (st) return st;
+    int anchors = cfg->projector_anchors == 0 ? 256 : cfg->projector_anchors;
+    if (anchors < 1) return fail(SARDINE_ERR_TC_CONFIG, "SARDINE_TC_PROJECTOR_ANCHORS must be >= 1 (default 256)");
+    if (anchors > sd::kMaxAnchorPts - 1)
+        return fail(SARDINE_ERR_TC_CONFIG, "projector anchors %d above the supported %d", anchors,
+                    sd::kMaxAnchorPts - 1);
+    if (cfg->max_newton_iterations < 0 || cfg->max_newton_iterations > 255)
+        return fail(SARDINE_ERR_TC_CONFIG, "max_newton_iterations %d outside 0..255", cfg->max_newton_iterations);
+    if (cfg->resampling < 0 || cfg->resampling > 2) return fail(SARDINE_ERR_TC_CONFIG, "bad resampling kernel");
+    if (cfg->precision != SARDINE_PRECISION_FP64 && cfg->precision != SARDINE_PRECISION_FP32_CENTRED)
+        return fail(SARDINE_ERR_TC_CONFIG, "bad precision selector");
+    if (scene->orbit.count > sd::kMaxSv || scene->orbit.count < 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", scene->orbit.count, sd::kMaxSv);
+    if (!merged->data || merged->lines <= 0 || merged->samples <= 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "radar image has no raster");
+    if (!dem->data || dem->width < 2 || dem->height < 2 || !(dem->pixel_deg > 0.0))
+        return fail(SARDINE_ERR_TC_DEM, "DEM raster must be at least 2x2 with positive spacing");
+    if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 && !cfg->geoid.grid_73x144)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "geoid grid pointer is null");
+
+    int64_t r0 = cfg->aoi_row0, c0 = cfg->aoi_col0, nr = cfg->aoi_rows, nc = cfg->aoi_cols;
+    if (nr == 0) { r0 = 0; c0 = 0; nr = g.rows; nc = g.cols; }
+    if (r0 < 0 || c0 < 0 || nr <= 0 || nc <= 0 || r0 + nr > g.rows || c0 + nc > g.cols)
+        return fail(SARDINE_ERR_TC_CONFIG, "AOI window outside the %lld x %lld output grid", (long long)g.rows,
+                    (long long)g.cols);
+    if (nr > 0x7fffffffLL) return fail(SARDINE_ERR_CAPACITY, "AOI has too many rows");
+    const bool compute_lia = cfg->compute_lia != 0;
+    if (!out->data || (compute_lia && (!out->cos_lia || !out->mask)))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "output buffers missing");
+    if ((st = require_device())) return st;
+
+    // ── orbit in device form, times relative to the first state vector ──
+    sd::OrbitDev orb;
+    std::memset(&orb, 0, sizeof orb);
+    orb.n = scene->orbit.count;
+    const int64_t epoch = orb.n > 0 ? scene->orbit.state_vectors[0].time_us : 0;
+    for (int i = 0; i < orb.n; ++i) {
+        const sardine_state_vector& s = scene->orbit.state_vectors[i];
+        orb.t_us[i] = s.time_us - epoch;
+        orb.tsec[i] = (double)(s.time_us - epoch) / 1e6;
+        for (int c = 0; c < 3; ++c) {
+            orb.pos[i][c] = s.position_m[c];
+            orb.vel[i][c] = s.velocity_m_s[c];
+        }
+    }
+
+    // ── per-row constants (terrain_correction.rs:1424-1511) ──
+    const double sp = cfg->pixel_spacing;
+    const bool utm = cfg->crs != SARDINE_CRS_WGS84;
+    const bool south = cfg->crs == SARDINE_CRS_UTM_SOUTH;
+    const bool need_geom = cfg->flatten || cfg->compute_lia;
+    const double step_deg = utm ? sp / 111320.0 : sp;
+    const int n_anch = (int)std::min<int64_t>(anchors, g.cols);
+    const double x_start = g.x_left + 0.5 * sp;
+    const double x_end = x_start + ((double)g.cols - 1.0) * sp;
+    const double anchor_dx = (x_end - x_start) / (double)n_anch;
+    const sd::TmSeries ts = sd::tm_series();
+    std::vector<int64_t> row_seed((size_t)nr);
+    std::vector<double> row_flat(need_geom ? (size_t)nr : 1, 0.0);
+    for (int64_t rr = 0; rr < nr; ++rr) {
+        const int64_t row = r0 + rr;
+        const double row_y = g.y_top - ((double)row + 0.5) * sp;
+        double lat_seed = row_y;
+        double ref_lon = g.x_left + ((double)g.cols / 2.0) * sp;
+        if (utm) {
+            double lon_mid;
+            sd::utm_inverse(ts, cfg->utm_zone, south, g.x_left + ((double)g.cols / 2.0) * sp, row_y, &lon_mid, &lat_seed);
+            if (need_geom) {  // row_lonlat[n_cols/2].lon from the anchor blend (output_crs.rs:395-414)
+                const int64_t col = g.cols / 2;
+                const double x = x_start + (double)col * sp;
+                int seg = 0;
+                while (seg + 1 < n_anch && x > x_start + (double)(seg + 1) * anchor_dx) ++seg;
+                const double xa = x_start + (double)seg * anchor_dx, xb = x_start + (double)(seg + 1) * anchor_dx;
+                double la0, lo0, la1, lo1;
+                sd::utm_inverse(ts, cfg->utm_zone, south, xa, row_y, &lo0, &la0);
+                sd::utm_inverse(ts, cfg->utm_zone, south, xb, row_y, &lo1, &la1);
+                const double t = xb == xa ? 0.0 : (x - xa) / (xb - xa);
+                ref_lon = lo0 + t * (lo1 - lo0);
+            }
+        }
+        row_seed[(size_t)rr] = guess_azimuth_time(lut, lat_seed) - epoch;
+        if (need_geom) row_flat[(size_t)rr] = sd::flat_cell_area(lat_seed, ref_lon, step_deg);
+    }
+
+    DevBuf d_orb, d_seed, d_flat, d_cnt, d_geoid;
+    SD_HIP(upload(d_orb, &orb, 1, stream));
+    SD_HIP(upload(d_seed, row_seed.data(), row_seed.size(), stream));
+    SD_HIP(upload(d_flat, row_flat.data(), row_flat.size(), stream));
+    if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5) SD_HIP(upload(d_geoid, cfg->geoid.grid_73x144, 73 * 144, stream));
+    SD_HIP(d_cnt.alloc((sd::kNumCounters + 1) * sizeof(unsigned long long), stream));
+    SD_HIP(sd_launch_zero_u64(d_cnt.as<unsigned long long>(), sd::kNumCounters + 1, stream));
+
+    sd::TcParams p;
+    std::memset(&p, 0, sizeof p);
+    p.orbit = d_orb.as<sd::OrbitDev>();
+    p.img = merged->data;
+    p.nesz = (std::isfinite(cfg->noise_floor_margin_db) && merged->nesz) ? merged->nesz : nullptr;
+    p.lines = merged->lines;
+    p.samples = merged->samples;
+    p.az_start_us = merged->azimuth_start_time_us - epoch;
+    p.near_range_m = merged->near_slant_range_time_s * sd::kC / 2.0;
+    p.rps = merged->range_pixel_spacing_m;
+    p.ati = ati;
+    p.dem = dem->data;
+    p.dem_w = dem->width;
+    p.dem_h = dem->height;
+    p.dem_olat = dem->origin_lat_deg;
+    p.dem_olon = dem->origin_lon_deg;
+    p.dem_pix = dem->pixel_deg;
+    p.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? d_geoid.as<float>() : nullptr;
+    p.crs = utm ? 1 : 0;
+    p.utm_zone = cfg->utm_zone;
+    p.utm_south = south ? 1 : 0;
+    p.n_anchor_pts = n_anch + 1;
+    p.spacing = sp;
+    p.lat_north = g.y_top;
+    p.lon_west = g.x_left;
+    p.x_start = x_start;
+    p.anchor_dx = anchor_dx;
+    p.full_cols = g.cols;
+    p.row0 = r0;
+    p.col0 = c0;
+    p.rows = nr;
+    p.cols = nc;
+    p.row_seed = d_seed.as<int64_t>();
+    p.row_flat_area = d_flat.as<double>();
+    p.max_iter = cfg->max_newton_iterations;
+    p.tol = cfg->newton_tolerance_s;
+    p.flatten = cfg->flatten ? 1 : 0;
+    p.compute_lia = compute_lia ? 1 : 0;
+    p.need_geom = need_geom ? 1 : 0;
+    p.resampling = cfg->resampling;
+    p.step_deg = step_deg;
+    // 10f32.powf(margin / 10.0) (terrain_correction.rs:1727), evaluated once
+    p.nesz_factor = std::pow(10.0f, cfg->noise_floor_margin_db / 10.0f);
+    p.out = out->data;
+    p.cos_lia = compute_lia ? out->cos_lia : nullptr;
+    p.mask = compute_lia ? out->mask : nullptr;
+    p.radar_coords = out->radar_coords;
+    p.counters = d_cnt.as<unsigned long long>();
+    p.error_flag = reinterpret_cast<int*>(d_cnt.as<unsigned long long>() + sd::kNumCounters);
+
+    if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED)
+        return fail(SARDINE_ERR_TC_CONFIG, "the FP32 scene-centred variant is not part of this build");
+    SD_HIP(sd_launch_tc_fp64(&p, stream));
+
+    unsigned long long cnt[sd::kNumCounters + 1];
+    SD_HIP(hipMemcpyAsync(cnt, d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost, stream));
+    SD_HIP(hipStreamSynchronize(stream));
+
+    const int err = (int)(cnt[sd::kNumCounters] & 0xffffffffull);
+    if (err & 1)
+        return fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW,
+                    "orbit interpolation failed: interpolation time outside orbit window; refusing to extrapolate");
+    if (err & 2)
+        return fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT,
+                    "orbit interpolation failed: need at least 8 vectors for interpolation, have %d", orb.n);
+
+    out->rows = nr;
+    out->cols = nc;
+    out->full_rows = g.rows;
+    out->full_cols = g.cols;
+    out->geotransform[0] = g.x_left + (double)c0 * sp;
+    out->geotransform[1] = sp;
+    out->geotransform[2] = 0.0;
+    out->geotransform[3] = g.y_top - (double)r0 * sp;
+    out->geotransform[4] = 0.0;
+    out->geotransform[5] = -sp;
+    out->total_pixel_count = (uint64_t)(nr * nc);
+    out->valid_pixel_count = cnt[0];
+    out->dem_missing_count = cnt[1];
+    out->outside_footprint_count = cnt[2];
+    out->non_converged_count = cnt[3];
+    out->degenerate_geometry_count = cnt[4];
+    out->flat_masked_count = cnt[5];
+    out->noise_masked_count = cnt[6];
+    for (int i = 0; i < SARDINE_NEWTON_HIST_SIZE; ++i) out->newton_iter_histogram[i] = cnt[7 + i];
+    out->chain_redo_rounds = cnt[sd::kCntRedo];
+    return SARDINE_OK;
+}
+
+// ── speckle (speckle.rs:131-180) ─────────────────────────────────────────
+sardine_status sardine_speckle_apply(const float* data, int64_t data_len, int64_t cols, int64_t rows,
+                                     const sardine_speckle_filter* f, float* out, void* stream) {
+    if (!f) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null filter");
+    if (cols < 0 || rows < 0 || (rows > 0 && cols > INT64_MAX / rows))
+        return fail(SARDINE_ERR_SPECKLE_DIMENSION, "data length %lld does not match cols x rows", (long long)data_len);
+    const int64_t expected = cols * rows;
+    if (data_len != expected)
+        return fail(SARDINE_ERR_SPECKLE_DIMENSION, "data length %lld does not match cols x rows = %lld",
+                    (long long)data_len, (long long)expected);
+    if (f->kind < SARDINE_SPECKLE_BOXCAR || f->kind > SARDINE_SPECKLE_REFINED_LEE)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown speckle filter kind %d", f->kind);
+    const int win = f->kind == SARDINE_SPECKLE_REFINED_LEE ? 7 : f->window;
+    if (win < 3 || win % 2 == 0) return fail(SARDINE_ERR_SPECKLE_BAD_WINDOW, "window size %d must be an odd integer >= 3", win);
+    if (f->kind == SARDINE_SPECKLE_LEE || f->kind == SARDINE_SPECKLE_GAMMA_MAP || f->kind == SARDINE_SPECKLE_REFINED_LEE)
+        if (!(f->enl > 0.0f) || !std::isfinite(f->enl))
+            return fail(SARDINE_ERR_SPECKLE_BAD_ENL, "ENL must be > 0; got %g", (double)f->enl);
+    if (f->kind == SARDINE_SPECKLE_FROST && (!std::isfinite(f->damping) || f->damping < 0.0f))
+        return fail(SARDINE_ERR_SPECKLE_BAD_DAMPING, "Frost damping factor must be >= 0 and finite; got %g",
+                    (double)f->damping);
+    if (win > 63) return fail(SARDINE_ERR_CAPACITY, "window %d above the supported 63", win);
+    if (expected == 0) return SARDINE_OK;
+    if (!data || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null raster");
+    if (data == out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "speckle filtering is out-of-place: out aliases data");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    SD_HIP(sd_launch_speckle(f->kind, data, out, cols, rows, win / 2, f->enl, f->damping, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+// ── deburst layout (deburst.rs:172-305) ──────────────────────────────────
+sardine_status sardine_deburst_layout_compute(const sardine_subswath* sw, const sardine_burst* b, int32_t n,
+                                              int64_t tiff_height, sardine_deburst_layout* out) {
+    if (!sw || !out || (n > 0 && !b)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(out, 0, sizeof *out);
+    if (n <= 0) return fail(SARDINE_ERR_DEBURST_NO_BURSTS, "burst list is empty");
+    if (n != sw->burst_count)
+        return fail(SARDINE_ERR_DEBURST_COUNT_MISMATCH, "subswath declares %d bursts but %d were given",
+                    sw->burst_count, n);
+    if (n > SARDINE_MAX_BURSTS) return fail(SARDINE_ERR_CAPACITY, "%d bursts above the supported %d", n, SARDINE_MAX_BURSTS);
+    const int64_t lps = sw->lines_per_burst;
+    if (tiff_height < (int64_t)n * lps)
+        return fail(SARDINE_ERR_DEBURST_TIFF_TOO_SMALL, "TIFF height %lld is too small for %d bursts x %lld lines",
+                    (long long)tiff_height, n, (long long)lps);
+    int64_t fv = b[0].first_valid_sample, lv = b[0].last_valid_sample;
+    for (int k = 1; k < n; ++k) {
+        fv = std::max(fv, b[k].first_valid_sample);
+        lv = std::min(lv, b[k].last_valid_sample);
+    }
+    if (fv >= lv)
+        return fail(SARDINE_ERR_DEBURST_EMPTY_VALID_WINDOW, "no valid range window: first_valid=%lld, last_valid=%lld",
+                    (long long)fv, (long long)lv);
+    for (int k = 0; k + 1 < n; ++k) {
+        const int64_t dt_us = b[k + 1].azimuth_time_us - b[k].azimuth_time_us;
+        const double dt_s = (double)dt_us * 1e-6;
+        const int64_t step = (int64_t)std::round(dt_s / sw->azimuth_time_interval_s);
+        const int64_t ov = lps - step;
+        if (ov <= 0)
+            return fail(SARDINE_ERR_DEBURST_INVALID_OVERLAP, "non-positive overlap between burst %d and burst %d: %lld",
+                        b[k].burst_index, b[k + 1].burst_index, (long long)ov);
+        if (ov < 50 || ov > 400)
+            return fail(SARDINE_ERR_DEBURST_IMPLAUSIBLE_OVERLAP,
+                        "implausible overlap between burst %d and burst %d: %lld lines is outside [50, 400]",
+                        b[k].burst_index, b[k + 1].burst_index, (long long)ov);
+        out->overlap[k] = ov;
+    }
+    int64_t total = 0;
+    for (int k = 0; k < n; ++k) {
+        const int64_t lf = k == 0 ? 0 : (out->overlap[k - 1] + 1) / 2;  // ⌈ov/2⌉
+        const int64_t le = k == n - 1 ? lps : lps - out->overlap[k] / 2;  // lps − ⌊ov/2⌋
+        out->local_first[k] = lf;
+        out->local_end[k] = le;
+        out->out_line_start[k] = total;
+        total += le - lf;
+    }
+    out->n_bursts = n;
+    out->lines = total;
+    out->samples = lv - fv;
+    out->valid_sample_offset = fv;
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── calibration + merge host set-up ───────────────────────────────────────
+namespace {
+
+void bracket_azimuth(const int32_t* lines, int n, int64_t tl, int* lo, int* hi) {  // apply_calibration.rs:190-202
+    const int32_t* it = std::lower_bound(lines, lines + n, tl, [](int32_t a, int64_t b) { return (int64_t)a < b; });
+    int i = (int)(it - lines);
+    if (it != lines + n && (int64_t)*it == tl) { *lo = *hi = i; return; }
+    if (i == 0) { *lo = *hi = 0; return; }
+    if (i >= n) { *lo = *hi = n - 1; return; }
+    *lo = i - 1;
+    *hi = i;
+}
+
+float lerp_f(float a, float b, float t) {
+    t = std::min(std::max(t, 0.0f), 1.0f);
+    return a + (b - a) * t;
+}
+
+// LUT coverage check (apply_calibration.rs:421-462); returns the max gap.
+sardine_status lut_gap(const sardine_lut* L, int64_t col_off, int64_t samples, bool is_cal, uint32_t* gap) {
+    if (!L || L->n_vectors <= 0)
+        return is_cal ? fail(SARDINE_ERR_CAL_NO_CAL_VECTORS, "calibration LUT has no vectors")
+                      : fail(SARDINE_ERR_CAL_NO_NOISE_VECTORS, "noise range LUT has no vectors");
+    const uint32_t req = (uint32_t)std::max<int64_t>(col_off + samples - 1, 0);
+    uint32_t mg = 0;
+    for (int i = 0; i < L->n_vectors; ++i) {
+        const int64_t a = L->offsets[i], e = L->offsets[i + 1];
+        if (e <= a)
+            return is_cal ? fail(SARDINE_ERR_CAL_NO_CAL_VECTORS, "calibration LUT has no vectors")
+                          : fail(SARDINE_ERR_CAL_NO_NOISE_VECTORS, "noise range LUT has no vectors");
+        const uint32_t mx = L->pixels[e - 1];
+        if (mx < req) {
+            const uint32_t g = req - mx;
+            if (g > 600)
+                return fail(SARDINE_ERR_CAL_LUT_RANGE,
+                            "calibration vector at line %d covers pixels up to %u, but output requires up to pixel %u",
+                            L->lines[i], mx, req);
+            mg = std::max(mg, g);
+        }
+    }
+    *gap = mg;
+    return SARDINE_OK;
+}
+
+// Per-line constants of one swath for `n_lines` merged lines.
+std::vector<sd::CalLine> line_table(const sardine_lut* cal, const sardine_lut* nr, const sardine_az_lut* az,
+                                    int64_t origin, int64_t n_lines, const sardine_deburst_layout* L,
+                                    const sardine_burst* bursts) {
+    std::vector<sd::CalLine> t((size_t)n_lines);
+    int k = 0;
+    for (int64_t l = 0; l < n_lines; ++l) {
+        sd::CalLine& c = t[(size_t)l];
+        std::memset(&c, 0, sizeof c);
+        if (L) {
+            while (k + 1 < L->n_bursts && l >= L->out_line_start[k + 1]) ++k;
+            c.tiff_row = bursts[k].first_line + L->local_first[k] + (l - L->out_line_start[k]);
+        } else {
+            c.tiff_row = l;
+        }
+        const int64_t tl = origin + l;
+        int lo, hi;
+        bracket_azimuth(cal->lines, cal->n_vectors, tl, &lo, &hi);
+        c.cal_lo = lo;
+        c.cal_hi = hi;
+        c.cal_t = lo == hi ? 0.0f
+                           : (float)(tl - (int64_t)cal->lines[lo]) / (float)((int64_t)cal->lines[hi] - (int64_t)cal->lines[lo]);
+        bracket_azimuth(nr->lines, nr->n_vectors, tl, &lo, &hi);
+        c.nr_lo = lo;
+        c.nr_hi = hi;
+        c.nr_t = lo == hi ? 0.0f
+                          : (float)(tl - (int64_t)nr->lines[lo]) / (float)((int64_t)nr->lines[hi] - (int64_t)nr->lines[lo]);
+        c.az = 1.0f;
+        if (az && az->n > 0) {  // interp_azimuth (apply_calibration.rs:168-181)
+            bracket_azimuth(az->lines, az->n, tl, &lo, &hi);
+            if (az->lines[hi] == az->lines[lo]) c.az = az->values[lo];
+            else {
+                float tt = (float)(tl - (int64_t)az->lines[lo]) / (float)((int64_t)az->lines[hi] - (int64_t)az->lines[lo]);
+                c.az = lerp_f(az->values[lo], az->values[hi], tt);
+            }
+        }
+    }
+    return t;
+}
+
+struct SwathDev {
+    DevBuf cal_px, cal_val, cal_off, nr_px, nr_val, nr_off, cal_dense, nr_dense, lines;
+};
+
+sardine_status stage_swath(const sardine_lut* cal, const sardine_lut* nr, int64_t dense_cols,
+                           const std::vector<sd::CalLine>& lt, SwathDev& d, hipStream_t s) {
+    const int64_t nc = cal->offsets[cal->n_vectors], nn = nr->offsets[nr->n_vectors];
+    SD_HIP(upload(d.cal_px, cal->pixels, (size_t)nc, s));
+    SD_HIP(upload(d.cal_val, cal->values, (size_t)nc, s));
+    SD_HIP(upload(d.cal_off, cal->offsets, (size_t)cal->n_vectors + 1, s));
+    SD_HIP(upload(d.nr_px, nr->pixels, (size_t)nn, s));
+    SD_HIP(upload(d.nr_val, nr->values, (size_t)nn, s));
+    SD_HIP(upload(d.nr_off, nr->offsets, (size_t)nr->n_vectors + 1, s));
+    SD_HIP(d.cal_dense.alloc((size_t)cal->n_vectors * dense_cols * sizeof(float), s));
+    SD_HIP(d.nr_dense.alloc((size_t)nr->n_vectors * dense_cols * sizeof(float), s));
+    SD_HIP(sd_launch_lut_dense(d.cal_px.as<uint32_t>(), d.cal_val.as<float>(), d.cal_off.as<int64_t>(), cal->n_vectors,
+                               dense_cols, d.cal_dense.as<float>(), s));
+    SD_HIP(sd_launch_lut_dense(d.nr_px.as<uint32_t>(), d.nr_val.as<float>(), d.nr_off.as<int64_t>(), nr->n_vectors,
+                               dense_cols, d.nr_dense.as<float>(), s));
+    SD_HIP(upload(d.lines, lt.data(), lt.size(), s));
+    return SARDINE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_prepare_merged_layout(const sardine_swath_input* sw, int32_t n, sardine_merged_info* info) {
+    if (!info || (n > 0 && !sw)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(info, 0, sizeof *info);
+    if (n <= 0) return fail(SARDINE_ERR_MERGE_EMPTY_INPUTS, "no subswaths to merge");
+    if (n > SARDINE_MAX_SWATHS) return fail(SARDINE_ERR_CAPACITY, "%d subswaths above the supported %d", n, SARDINE_MAX_SWATHS);
+    std::vector<sardine_deburst_layout> L((size_t)n);
+    uint32_t gc = 0, gn = 0;
+    for (int i = 0; i < n; ++i) {
+        sardine_status st = sardine_deburst_layout_compute(&sw[i].swath, sw[i].bursts, sw[i].n_bursts, sw[i].tiff_height,
+                                                           &L[(size_t)i]);
+        if (st) return st;
+        uint32_t a, b;
+        if ((st = lut_gap(&sw[i].cal, L[(size_t)i].valid_sample_offset, L[(size_t)i].samples, true, &a))) return st;
+        if ((st = lut_gap(&sw[i].noise_range, L[(size_t)i].valid_sample_offset, L[(size_t)i].samples, false, &b))) return st;
+        gc = std::max(gc, a);
+        gn = std::max(gn, b);
+    }
+    // merge_subswaths (merge_subswaths.rs:207-330)
+    const double rps = sw[0].swath.range_pixel_spacing_m;
+    for (int i = 1; i < n; ++i)
+        if (std::fabs(sw[i].swath.range_pixel_spacing_m - rps) > 1e-3)
+            return fail(SARDINE_ERR_MERGE_PIXEL_SPACING, "inconsistent range pixel spacing: first %g, swath %d has %g", rps, i,
+                        sw[i].swath.range_pixel_spacing_m);
+    for (int i = 0; i < n; ++i)
+        if (L[(size_t)i].samples == 0) return fail(SARDINE_ERR_MERGE_EMPTY_SWATH, "subswath %d is empty", i);
+    const double dt = rps * 2.0 / sd::kC;
+    std::vector<double> t0((size_t)n);
+    for (int i = 0; i < n; ++i) t0[(size_t)i] = sw[i].swath.slant_range_time_s + (double)L[(size_t)i].valid_sample_offset * dt;
+    int64_t out_lines = L[0].lines;
+    int64_t az0 = sw[0].bursts[0].azimuth_time_us;
+    for (int i = 0; i < n; ++i) {
+        const int64_t off = n == 1 ? 0 : (int64_t)std::round((t0[(size_t)i] - t0[0]) / dt);
+        info->sw_out_offset[i] = off;
+        info->swath_lines[i] = L[(size_t)i].lines;
+        info->swath_samples[i] = L[(size_t)i].samples;
+        info->swath_valid_offset[i] = L[(size_t)i].valid_sample_offset;
+        out_lines = std::min(out_lines, L[(size_t)i].lines);
+        az0 = std::min(az0, sw[i].bursts[0].azimuth_time_us);
+    }
+    const int64_t out_cols = info->sw_out_offset[n - 1] + L[(size_t)n - 1].samples;
+    for (int i = 0; i + 1 < n; ++i)
+        info->seam[i] = (info->sw_out_offset[i] + L[(size_t)i].samples + info->sw_out_offset[i + 1]) / 2;
+    info->seam[n - 1] = out_cols;
+    info->lines = out_lines;
+    info->samples = out_cols;
+    info->near_slant_range_time_s = t0[0];
+    info->range_pixel_spacing_s = dt;
+    info->range_pixel_spacing_m = rps;
+    info->azimuth_start_time_us = az0;
+    info->cal_lut_extrapolation_gap_px = gc;
+    info->noise_lut_extrapolation_gap_px = gn;
+    info->n_swaths = n;
+    return SARDINE_OK;
+}
+
+sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, float* sigma0, float* nesz,
+                                      sardine_merged_info* info, void* stream_v) {
+    sardine_status st = sardine_prepare_merged_layout(sw, n, info);
+    if (st) return st;
+    if (!sigma0 || !nesz) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null output raster");
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream_v;
+    std::vector<SwathDev> dev((size_t)n);
+    std::vector<std::vector<sd::CalLine>> tables((size_t)n);
+    sd::CalParams P;
+    std::memset(&P, 0, sizeof P);
+    P.n_sw = n;
+    P.out_lines = info->lines;
+    P.out_cols = info->samples;
+    P.sigma0 = sigma0;
+    P.nesz = nesz;
+    for (int i = 0; i < n; ++i) {
+        sardine_deburst_layout L;
+        if ((st = sardine_deburst_layout_compute(&sw[i].swath, sw[i].bursts, sw[i].n_bursts, sw[i].tiff_height, &L)))
+            return st;
+        if (!sw[i].raw) return fail(SARDINE_ERR_INVALID_ARGUMENT, "swath %d raster is null", i);
+        if (L.valid_sample_offset + L.samples > sw[i].raw_cols)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "swath %d valid window exceeds the raster width", i);
+        const int64_t dense_cols = L.valid_sample_offset + L.samples;
+        tables[(size_t)i] = line_table(&sw[i].cal, &sw[i].noise_range, &sw[i].noise_azimuth, sw[i].bursts[0].first_line,
+                                       info->lines, &L, sw[i].bursts);
+        if ((st = stage_swath(&sw[i].cal, &sw[i].noise_range, dense_cols, tables[(size_t)i], dev[(size_t)i], s))) return st;
+        sd::CalSwathDev& D = P.sw[i];
+        D.raw = sw[i].raw;
+        D.raw_cols = sw[i].raw_cols;
+        D.first_valid = L.valid_sample_offset;
+        D.cal_dense = dev[(size_t)i].cal_dense.as<float>();
+        D.nr_dense = dev[(size_t)i].nr_dense.as<float>();
+        D.dense_cols = dense_cols;
+        D.lines = dev[(size_t)i].lines.as<sd::CalLine>();
+        D.out_off = info->sw_out_offset[i];
+        const int64_t col_start = i == 0 ? 0 : info->seam[i - 1];
+        const int64_t col_end = i == n - 1 ? info->samples : info->seam[i];
+        D.own_end = col_end;
+        D.valid_start = std::max(col_start, D.out_off);
+        D.valid_end = std::min(col_end, D.out_off + L.samples);
+    }
+    SD_HIP(sd_launch_cal_merge(&P, s));
+    SD_HIP(hipStreamSynchronize(s));
+    return SARDINE_OK;
+}
+
+sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, int64_t samples, int64_t col_off,
+                                         const sardine_lut* cal, const sardine_lut* nr, const sardine_az_lut* az,
+                                         int64_t origin, float* out, float* nesz, uint32_t* cal_gap, uint32_t* nr_gap,
+                                         void* stream_v) {
+    if (!cal || !nr) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null LUT");
+    uint32_t a = 0, b = 0;
+    sardine_status st;
+    if ((st = lut_gap(cal, col_off, samples, true, &a))) return st;
+    if ((st = lut_gap(nr, col_off, samples, false, &b))) return st;
+    if (cal_gap) *cal_gap = a;
+    if (nr_gap) *nr_gap = b;
+    if (lines <= 0 || samples <= 0) return SARDINE_OK;
+    if (!deburst || !out || !nesz) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null raster");
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream_v;
+    auto lt = line_table(cal, nr, az, origin, lines, nullptr, nullptr);
+    SwathDev d;
+    if ((st = stage_swath(cal, nr, col_off + samples, lt, d, s))) return st;
+    sd::CalParams P;
+    std::memset(&P, 0, sizeof P);
+    P.n_sw = 1;
+    P.out_lines = lines;
+    P.out_cols = samples;
+    P.sigma0 = out;
+    P.nesz = nesz;
+    sd::CalSwathDev& D = P.sw[0];
+    // the debursted array is its own "TIFF": row = line, column 0 = TIFF col_off
+    D.raw = deburst - 2 * col_off;  // address arithmetic only; indices stay in [col_off, col_off+samples)
+    D.raw_cols = samples;
+    D.first_valid = col_off;
+    D.cal_dense = d.cal_dense.as<float>();
+    D.nr_dense = d.nr_dense.as<float>();
+    D.dense_cols = col_off + samples;
+    D.lines = d.lines.as<sd::CalLine>();
+    D.out_off = 0;
+    D.own_end = samples;
+    D.valid_start = 0;
+    D.valid_end = samples;
+    SD_HIP(sd_launch_cal_merge(&P, s));
+    SD_HIP(hipStreamSynchronize(s));
+    return SARDINE_OK;
+}
+
+sardine_status sardine_to_db_inplace(float* data, int64_t n, float floor_lin, uint64_t counts[2], void* stream_v) {
+    if (floor_lin < 0.0f) return fail(SARDINE_ERR_INVALID_ARGUMENT, "invalid noise floor %g", (double)floor_lin);
+    if (n < 0 || (n > 0 && !data)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad raster");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream_v;
+    DevBuf c;
+    SD_HIP(c.alloc(2 * sizeof(unsigned long long), s));
+    SD_HIP(sd_launch_zero_u64(c.as<unsigned long long>(), 2, s));
+    SD_HIP(sd_launch_to_db(data, n, floor_lin, c.as<unsigned long long>(), s));
+    unsigned long long h[2];
+    SD_HIP(hipMemcpyAsync(h, c.p, sizeof h, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    if (counts) {
+        counts[0] = h[0];
+        counts[1] = h[1];
+    }
+    return SARDINE_OK;
+}
+
+sardine_status sardine_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma, void* stream) {
+    if (n < 0 || (n > 0 && !out)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad raster");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    SD_HIP(sd_launch_synth_cint16(out, n, seed, base, sigma, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+// geom.hpp — FP64 geodesy, orbit interpolation and transverse Mercator used
+// by the terrain-correction kernels (device) and its per-row set-up (host).
+//
+// Reference semantics:
+//   geodetic_to_ecef          sardine/src/geodesy.rs:38-55
+//   Lagrange-8 pos/vel/accel  sardine/src/orbit.rs:498-596
+//   zero-Doppler Newton       sardine/src/terrain_correction.rs:1010-1073
+//   flat_cell_area / geometry sardine/src/terrain_correction.rs:576-651
+//   inverse UTM               proj4rs 0.1.10 behind output_crs.rs:313-340
+//                             (here: Krüger series to n^6, Karney 2011)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SD_HD __host__ __device__ __forceinline__
+
+namespace sd {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kDeg2Rad = kPi / 180.0;
+constexpr double kA = 6378137.0;
+constexpr double kF = 1.0 / 298.257223563;
+constexpr double kE2 = 2.0 * kF - kF * kF;
+constexpr double kC = 299792458.0;
+constexpr int kLagrange = 8;
+constexpr int kMaxSv = 64;
+
+struct d3 {
+    double x, y, z;
+};
+SD_HD d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+SD_HD double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+SD_HD d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// geodesy.rs:38-55 (same operation order: lat_deg*PI/180, n = a/sqrt(1-e2 s^2))
+SD_HD d3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
+    double lat = lat_deg * kPi / 180.0;
+    double lon = lon_deg * kPi / 180.0;
+    double sl, cl, so, co;
+    sl = sin(lat);
+    cl = cos(lat);
+    so = sin(lon);
+    co = cos(lon);
+    double n = kA / sqrt(1.0 - kE2 * sl * sl);
+    return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - kE2) + h) * sl};
+}
+
+// Orbit in device-friendly form.  Times are µs relative to the first state
+// vector; tsec is the same in seconds ((t_us) / 1e6, orbit.rs:545-547).
+struct OrbitDev {
+    int32_t n;
+    int32_t pad;
+    int64_t t_us[kMaxSv];
+    double tsec[kMaxSv];
+    double pos[kMaxSv][3];
+    double vel[kMaxSv][3];
+};
+
+enum : int { ORB_OK = 0, ORB_OUTSIDE = 1, ORB_INSUFFICIENT = 2 };
+
+// interpolate_orbit_and_accel (orbit.rs:498-596): same window rule, same
+// 1 ns reciprocal guard, same L'_k = L_k (Σ_j 1/(t-t_j) - 1/(t-t_k)).  The
+// basis L_k is formed as Π_{j≠k}(t-t_j)/(t_k-t_j) in the reference's order.
+SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
+    const int n = o.n;
+    if (n < kLagrange) return ORB_INSUFFICIENT;
+    if (t < o.t_us[0] || t > o.t_us[n - 1]) return ORB_OUTSIDE;
+    const double ts = (double)(t - o.t_us[0]) / 1e6;
+    // centre = largest i with t_i <= t: sorted nodes, short linear walk from
+    // an estimate (uniform spacing makes the estimate exact almost always).
+    int c = (int)((double)(n - 1) * (double)(t - o.t_us[0]) / (double)(o.t_us[n - 1] - o.t_us[0]));
+    c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+    while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
+    while (c > 0 && o.t_us[c] > t) --c;
+    int start;
+    if (c < 4) start = 0;
+    else if (c + 4 >= n) start = n - kLagrange;
+    else start = c - 3;
+    double tj[kLagrange], rd[kLagrange];
+    double tot = 0.0;
+#pragma unroll
+    for (int j = 0; j < kLagrange; ++j) {
+        tj[j] = o.tsec[start + j];
+        double diff = ts - tj[j];
+        rd[j] = 0.0;
+        if (fabs(diff) > 1e-9) {
+            rd[j] = 1.0 / diff;
+            tot += rd[j];
+        }
+    }
+    d3 p = {0, 0, 0}, v = {0, 0, 0}, a = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kLagrange; ++k) {
+        double basis = 1.0;
+#pragma unroll
+        for (int j = 0; j < kLagrange; ++j)
+            if (j != k) basis *= (ts - tj[j]) / (tj[k] - tj[j]);
+        double db = basis * (tot - rd[k]);
+        const double* pp = o.pos[start + k];
+        const double* vv = o.vel[start + k];
+        p.x += basis * pp[0]; p.y += basis * pp[1]; p.z += basis * pp[2];
+        v.x += basis * vv[0]; v.y += basis * vv[1]; v.z += basis * vv[2];
+        a.x += db * vv[0]; a.y += db * vv[1]; a.z += db * vv[2];
+    }
+    *P = p; *V = v; *A = a;
+    return ORB_OK;
+}
+
+enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT_OUTSIDE = 3, ZD_ORBIT_INSUFFICIENT = 4 };
+
+// solve_zero_doppler (terrain_correction.rs:1010-1073): convergence is tested
+// before the step, steps are truncated to whole µs.
+SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter, double tol, int64_t* t_out,
+                       int* iters, d3* sat) {
+    int64_t t = seed;
+    for (int i = 0; i < max_iter; ++i) {
+        d3 P, V, A;
+        int e = orbit_interp(o, t, &P, &V, &A);
+        if (e == ORB_OUTSIDE) return ZD_ORBIT_OUTSIDE;
+        if (e == ORB_INSUFFICIENT) return ZD_ORBIT_INSUFFICIENT;
+        d3 r = target - P;
+        double f = dot(r, V);
+        double vsq = dot(V, V);
+        double fp = -vsq + dot(r, A);
+        if (fabs(fp) < 1.0) return ZD_DEGENERATE;
+        double delta = -f / fp;
+        if (fabs(delta) < tol) {
+            *t_out = t;
+            *iters = i + 1 < 255 ? i + 1 : 255;
+            *sat = P;
+            return ZD_CONVERGED;
+        }
+        t = t + (int64_t)(delta * 1e6);
+    }
+    return ZD_NOT_CONVERGED;
+}
+
+// flat_cell_area (terrain_correction.rs:576-583)
+SD_HD double flat_cell_area(double lat, double lon, double s) {
+    d3 fe = geodetic_to_ecef(lat, lon + s, 0.0), fw = geodetic_to_ecef(lat, lon - s, 0.0);
+    d3 fn = geodetic_to_ecef(lat + s, lon, 0.0), fs = geodetic_to_ecef(lat - s, lon, 0.0);
+    d3 c = cross(fe - fw, fn - fs);
+    return sqrt(dot(c, c));
+}
+
+enum : int { TG_VALID = 0, TG_FORESHORTENING = 1, TG_LAYOVER = 2, TG_SHADOW = 3 };
+
+// compute_terrain_geometry_with_flat_area (terrain_correction.rs:593-651)
+SD_HD int terrain_geometry(double lat, double lon, double he, double hw, double hn, double hs, d3 sat, d3 target,
+                           double s, double flat_area, double* w_out, double* cos_lia_out) {
+    d3 pe = geodetic_to_ecef(lat, lon + s, he), pw = geodetic_to_ecef(lat, lon - s, hw);
+    d3 pn = geodetic_to_ecef(lat + s, lon, hn), ps = geodetic_to_ecef(lat - s, lon, hs);
+    d3 tn = cross(pe - pw, pn - ps);
+    d3 lk = sat - target;
+    double ln = sqrt(dot(lk, lk));
+    d3 look = ln == 0.0 ? d3{0, 0, 0} : d3{lk.x / ln, lk.y / ln, lk.z / ln};
+    double lp = dot(tn, look);
+    if (lp <= 0.0) return TG_SHADOW;
+    double ta = sqrt(dot(tn, tn));
+    if (ta == 0.0) return TG_SHADOW;
+    double cl = lp / ta;
+    cl = cl < 0.0 ? 0.0 : (cl > 1.0 ? 1.0 : cl);
+    double w = lp / flat_area;
+    *w_out = w;
+    *cos_lia_out = cl;
+    if (w < 0.05) return TG_FORESHORTENING;
+    if (w > 2.0) return TG_LAYOVER;
+    return TG_VALID;
+}
+
+// ── inverse transverse Mercator (UTM) ────────────────────────────────────
+// Krüger series of order n^6 in the third flattening n (Karney 2011,
+// "Transverse Mercator with an accuracy of a few nanometers", eqs. 7, 11, 35,
+// 36), evaluated with Clenshaw summation; τ from τ' by Newton (eqs. 19-21).
+struct TmSeries {
+    double A;       // rectifying radius
+    double beta[6]; // inverse series
+    double alpha[6];
+    double e;
+};
+
+SD_HD TmSeries tm_series() {
+    const double n = kF / (2.0 - kF);
+    const double n2 = n * n, n3 = n2 * n, n4 = n3 * n, n5 = n4 * n, n6 = n5 * n;
+    TmSeries s;
+    s.e = sqrt(kE2);
+    s.A = kA / (1.0 + n) * (1.0 + n2 / 4.0 + n4 / 64.0 + n6 / 256.0);
+    s.alpha[0] = n / 2 - 2 * n2 / 3 + 5 * n3 / 16 + 41 * n4 / 180 - 127 * n5 / 288 + 7891 * n6 / 37800;
+    s.alpha[1] = 13 * n2 / 48 - 3 * n3 / 5 + 557 * n4 / 1440 + 281 * n5 / 630 - 1983433 * n6 / 1935360;
+    s.alpha[2] = 61 * n3 / 240 - 103 * n4 / 140 + 15061 * n5 / 26880 + 167603 * n6 / 181440;
+    s.alpha[3] = 49561 * n4 / 161280 - 179 * n5 / 168 + 6601661 * n6 / 7257600;
+    s.alpha[4] = 34729 * n5 / 80640 - 3418889 * n6 / 1995840;
+    s.alpha[5] = 212378941 * n6 / 319334400;
+    s.beta[0] = n / 2 - 2 * n2 / 3 + 37 * n3 / 96 - n4 / 360 - 81 * n5 / 512 + 96199 * n6 / 604800;
+    s.beta[1] = n2 / 48 + n3 / 15 - 437 * n4 / 1440 + 46 * n5 / 105 - 1118711 * n6 / 3870720;
+    s.beta[2] = 17 * n3 / 480 - 37 * n4 / 840 - 209 * n5 / 4480 + 5569 * n6 / 90720;
+    s.beta[3] = 4397 * n4 / 161280 - 11 * n5 / 504 - 830251 * n6 / 7257600;
+    s.beta[4] = 4583 * n5 / 161280 - 108847 * n6 / 3991680;
+    s.beta[5] = 20648693 * n6 / 638668800;
+    return s;
+}
+
+constexpr double kUtmK0 = 0.9996;
+constexpr double kUtmE0 = 500000.0;
+constexpr double kUtmN0South = 10000000.0;
+
+// Σ_j c_j sin(2j ξ) cosh(2j η) and Σ_j c_j cos(2j ξ) sinh(2j η) by the complex
+// Clenshaw recurrence on z = ξ + iη.
+SD_HD void clenshaw_series(const double* c, double xi, double eta, double* sx, double* sy) {
+    double s2 = sin(2 * xi), c2 = cos(2 * xi), sh2 = sinh(2 * eta), ch2 = cosh(2 * eta);
+    // a = 2 cos(2z) = 2(c2 ch2 - i s2 sh2)
+    double ar = 2 * c2 * ch2, ai = -2 * s2 * sh2;
+    double y0r = 0, y0i = 0, y1r = 0, y1i = 0;
+    for (int k = 5; k >= 0; --k) {
+        double tr = ar * y0r - ai * y0i - y1r + c[k];
+        double ti = ar * y0i + ai * y0r - y1i;
+        y1r = y0r; y1i = y0i;
+        y0r = tr; y0i = ti;
+    }
+    // sum = sin(2z) * y0 ; sin(2z) = s2 ch2 + i c2 sh2
+    double zr = s2 * ch2, zi = c2 * sh2;
+    *sx = zr * y0r - zi * y0i;
+    *sy = zr * y0i + zi * y0r;
+}
+
+SD_HD void utm_inverse(const TmSeries& s, int zone, bool south, double x, double y, double* lon_deg,
+                       double* lat_deg) {
+    double xi = (y - (south ? kUtmN0South : 0.0)) / (kUtmK0 * s.A);
+    double eta = (x - kUtmE0) / (kUtmK0 * s.A);
+    double sx, sy;
+    clenshaw_series(s.beta, xi, eta, &sx, &sy);
+    double xip = xi - sx, etap = eta - sy;
+    double sh = sinh(etap), cx = cos(xip);
+    double taup = sin(xip) / sqrt(sh * sh + cx * cx);
+    double lam = atan2(sh, cx);
+    const double e = s.e, e2 = kE2;
+    double tau = taup;
+    for (int i = 0; i < 8; ++i) {
+        double s1 = sqrt(1.0 + tau * tau);
+        double sig = sinh(e * atanh(e * tau / s1));
+        double taui = tau * sqrt(1.0 + sig * sig) - sig * s1;
+        double dt = (taup - taui) / sqrt(1.0 + taui * taui) * (1.0 + (1.0 - e2) * tau * tau) / ((1.0 - e2) * s1);
+        tau += dt;
+        if (fabs(dt) < 1e-14) break;
+    }
+    *lat_deg = atan(tau) * 180.0 / kPi;
+    *lon_deg = (double)(zone * 6 - 183) + lam * 180.0 / kPi;
+}
+
+SD_HD void utm_forward(const TmSeries& s, int zone, bool south, double lon_deg, double lat_deg, double* x,
+                       double* y) {
+    double phi = lat_deg * kPi / 180.0, lam = (lon_deg - (double)(zone * 6 - 183)) * kPi / 180.0;
+    double sp = sin(phi);
+    double t = sinh(atanh(sp) - s.e * atanh(s.e * sp));
+    double xip = atan2(t, cos(lam));
+    double etap = atanh(sin(lam) / sqrt(1.0 + t * t));
+    double sx, sy;
+    clenshaw_series(s.alpha, xip, etap, &sx, &sy);
+    *x = kUtmE0 + kUtmK0 * s.A * (etap + sy);
+    *y = (south ? kUtmN0South : 0.0) + kUtmK0 * s.A * (xip + sx);
+}
+
+}  // namespace sd

@@ -1,0 +1,90 @@
+// misc.hip — small streaming kernels: the dB epilogue (export.rs:59-94),
+// the synthetic CInt16 generator used for benchmark inputs, and a zero fill
+// (used instead of hipMemsetAsync so the work can be captured in graphs).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sd {
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// to_db_inplace: NaN stays NaN; v <= floor → NaN (masked); else 10*log10(v).
+__global__ __launch_bounds__(256) void to_db_kernel(float* d, int64_t n, float floor_lin,
+                                                    unsigned long long* counts) {
+    unsigned conv = 0, masked = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = d[i];
+        if (isnan(v)) continue;
+        if (v <= floor_lin) {
+            d[i] = __builtin_nanf("");
+            ++masked;
+        } else {
+            d[i] = 10.0f * log10f(v);
+            ++conv;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        conv += __shfl_xor(conv, o);
+        masked += __shfl_xor(masked, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (conv) atomicAdd(&counts[0], (unsigned long long)conv);
+        if (masked) atomicAdd(&counts[1], (unsigned long long)masked);
+    }
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Same function as sardine_amd.synthetic.complex_gaussian_dn.
+__global__ __launch_bounds__(256) void synth_cint16_kernel(int16_t* out, int64_t n, uint64_t seed, uint64_t base,
+                                                           double sigma) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = seed * 0x9E3779B97F4A7C15ull + base + (uint64_t)i;
+        uint64_t h1 = splitmix64(k * 2ull), h2 = splitmix64(k * 2ull + 1ull);
+        double u1 = (double)(h1 >> 11) * (1.0 / 9007199254740992.0);
+        double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+        double r = sqrt(-2.0 * log1p(-u1));
+        double ang = 2.0 * 3.14159265358979323846 * u2;
+        double re = floor(sigma * r * cos(ang) + 0.5), im = floor(sigma * r * sin(ang) + 0.5);
+        re = fmin(fmax(re, -32767.0), 32767.0);
+        im = fmin(fmax(im, -32767.0), 32767.0);
+        reinterpret_cast<short2*>(out)[i] = make_short2((short)re, (short)im);
+    }
+}
+
+}  // namespace sd
+
+extern "C" hipError_t sd_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sd::zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sd_launch_to_db(float* d, int64_t n, float floor_lin, unsigned long long* counts,
+                                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(sd::to_db_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n, floor_lin, counts);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
+                                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 262144) blocks = 262144;
+    hipLaunchKernelGGL(sd::synth_cint16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, seed, base, sigma);
+    return hipGetLastError();
+}

@@ -1,0 +1,248 @@
+// speckle.hip — the five SpeckleKernel filters on gfx950.
+//
+// Reference: sardine/src/speckle.rs:131-603 (apply_speckle_filter,
+// window_mean_var, boxcar, lee, frost, gamma_map, refined_lee).
+//
+// Each 256-thread workgroup filters a 64 x 16 output tile.  The input tile
+// plus a `half`-pixel halo is staged once in LDS; samples outside the raster
+// are stored as NaN, which the reference's "finite samples only" rule skips
+// exactly like out-of-bounds samples.  Window sums are accumulated in f64 in
+// the reference's raster order (dy outer, dx inner), so mean/variance are
+// bit-identical to the CPU path; the filter formulas run in f32 as there.
+// Refined Lee accumulates all eight directional sub-windows in one pass over
+// the 7 x 7 neighbourhood with compile-time membership masks; each
+// direction still sees its samples in raster order.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sd {
+
+constexpr int kTileW = 64;
+constexpr int kTileH = 16;
+constexpr int kRowsPerThread = kTileH / 4;
+
+enum { SK_BOXCAR = 0, SK_LEE = 1, SK_FROST = 2, SK_GAMMA = 3, SK_RLEE = 4 };
+
+struct SpeckleParams {
+    const float* in;
+    float* out;
+    int64_t cols, rows;
+    int half;
+    float enl;
+    float damping;
+};
+
+// Stage the (kTileH + 2h) x (kTileW + 2h) input tile; pitch = kTileW + 2h.
+__device__ __forceinline__ void stage_tile(const SpeckleParams& p, float* tile, int half, int64_t r0, int64_t c0) {
+    const int tw = kTileW + 2 * half, th = kTileH + 2 * half;
+    const float qnan = __builtin_nanf("");
+    for (int i = threadIdx.x; i < tw * th; i += 256) {
+        int ty = i / tw, tx = i - ty * tw;
+        int64_t r = r0 - half + ty, c = c0 - half + tx;
+        float v = qnan;
+        if (r >= 0 && r < p.rows && c >= 0 && c < p.cols) v = p.in[r * p.cols + c];
+        tile[i] = v;
+    }
+}
+
+// Mean and population variance of the finite window samples (speckle.rs:218-241).
+template <int HALF>
+__device__ __forceinline__ bool win_stats(const float* t, int pitch, int half_rt, float* mean, float* var) {
+    const int half = HALF > 0 ? HALF : half_rt;
+    double sum = 0.0, sq = 0.0;
+    int n = 0;
+#pragma unroll
+    for (int dy = -half; dy <= half; ++dy)
+#pragma unroll
+        for (int dx = -half; dx <= half; ++dx) {
+            float v = t[dy * pitch + dx];
+            if (isfinite(v)) {
+                double x = (double)v;
+                sum += x;
+                sq += x * x;
+                ++n;
+            }
+        }
+    if (n == 0) return false;
+    double m = sum / (double)n;
+    double va = sq / (double)n - m * m;
+    va = va > 0.0 ? va : 0.0;
+    *mean = (float)m;
+    *var = (float)va;
+    return true;
+}
+
+template <int KIND, int HALF>
+__device__ __forceinline__ float filter_pixel(const SpeckleParams& p, const float* t, int pitch, float centre) {
+    const int half = HALF > 0 ? HALF : p.half;
+    const float qnan = __builtin_nanf("");
+    float mean, var;
+    if (!win_stats<HALF>(t, pitch, half, &mean, &var)) return qnan;
+    if (KIND == SK_BOXCAR) return mean;  // speckle.rs:247-263
+    if (KIND == SK_LEE) {                // speckle.rs:269-298
+        float cv_n_sq = 1.0f / p.enl;
+        float noise_var = mean * mean * cv_n_sq;
+        float signal_var = fmaxf(var - noise_var, 0.0f);
+        float denom = signal_var + noise_var;
+        float k = denom > 0.0f ? signal_var / denom : 0.0f;
+        return mean + k * (centre - mean);
+    }
+    if (KIND == SK_FROST) {  // speckle.rs:304-342
+        float cv_sq = mean > 0.0f ? var / (mean * mean) : 0.0f;
+        float alpha = p.damping * cv_sq;
+        double wd[2 * 31 + 1];
+        for (int d = 0; d <= 2 * half; ++d) wd[d] = (double)expf(-alpha * (float)d);
+        double wsum = 0.0, vw = 0.0;
+#pragma unroll
+        for (int dy = -half; dy <= half; ++dy)
+#pragma unroll
+            for (int dx = -half; dx <= half; ++dx) {
+                float v = t[dy * pitch + dx];
+                if (isfinite(v)) {
+                    double w = wd[abs(dy) + abs(dx)];
+                    wsum += w;
+                    vw += w * (double)v;
+                }
+            }
+        return wsum > 0.0 ? (float)(vw / wsum) : qnan;
+    }
+    // Gamma-MAP, speckle.rs:348-411
+    float enl = p.enl;
+    float cv_n_sq = 1.0f / enl;
+    float cv_max_sq = cv_n_sq < 1.0f ? 2.0f * cv_n_sq / (1.0f - cv_n_sq) : __builtin_inff();
+    float cv_sq = mean > 0.0f ? var / (mean * mean) : 0.0f;
+    if (cv_sq <= cv_n_sq) return mean;
+    if (cv_sq >= cv_max_sq) return centre;
+    float alpha = (1.0f + cv_n_sq) / (cv_sq - cv_n_sq);
+    float b = (alpha - enl - 1.0f) * mean;
+    float disc = b * b + 4.0f * alpha * enl * mean * centre;
+    return disc >= 0.0f ? ((enl - alpha + 1.0f) * mean + sqrtf(disc)) / (2.0f * enl) : mean;
+}
+
+// refined_lee_in_mask (speckle.rs:432-457) as an 8-bit direction set per offset.
+__host__ __device__ constexpr unsigned rl_dirs(int dy, int dx) {
+    if (dy == 0 && dx == 0) return 0u;
+    unsigned m = 0;
+    if (dy >= 1 && dy <= 3) m |= 1u << 0;                                           // S
+    if (dy >= -3 && dy <= -1) m |= 1u << 1;                                         // N
+    if (dx >= 1 && dx <= 3) m |= 1u << 2;                                           // E
+    if (dx >= -3 && dx <= -1) m |= 1u << 3;                                         // W
+    if ((dy - dx <= 1 && dx - dy <= 1) && dy + dx >= 1) m |= 1u << 4;               // SE
+    if ((dy - dx <= 1 && dx - dy <= 1) && dy + dx <= -1) m |= 1u << 5;              // NW
+    if ((dy + dx <= 1 && -(dy + dx) <= 1) && dx - dy >= 1) m |= 1u << 6;            // NE
+    if ((dy + dx <= 1 && -(dy + dx) <= 1) && dx - dy <= -1) m |= 1u << 7;           // SW
+    return m;
+}
+
+// Refined Lee, speckle.rs:522-569
+__device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, float centre, float cv_n_sq) {
+    double sum[8], sq[8];
+    int n[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) { sum[d] = 0.0; sq[d] = 0.0; n[d] = 0; }
+#pragma unroll
+    for (int dy = -3; dy <= 3; ++dy)
+#pragma unroll
+        for (int dx = -3; dx <= 3; ++dx) {
+            constexpr int dummy = 0;
+            (void)dummy;
+            const unsigned m = rl_dirs(dy, dx);
+            if (m == 0u) continue;
+            float v = t[dy * pitch + dx];
+            if (!isfinite(v)) continue;
+            double x = (double)v, xx = x * x;
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+                if (m & (1u << d)) {
+                    sum[d] += x;
+                    sq[d] += xx;
+                    ++n[d];
+                }
+        }
+    float best_cv = __builtin_inff(), best_mean = centre, best_var = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        if (n[d] == 0) continue;
+        double m = sum[d] / (double)n[d];
+        double va = sq[d] / (double)n[d] - m * m;
+        va = va > 0.0 ? va : 0.0;
+        float mf = (float)m, vf = (float)va;
+        if (!(mf > 0.0f)) continue;
+        float cv = vf / (mf * mf);
+        if (cv < best_cv) {
+            best_cv = cv;
+            best_mean = mf;
+            best_var = vf;
+        }
+    }
+    if (!isfinite(best_cv)) return centre;
+    float noise_var = best_mean * best_mean * cv_n_sq;
+    float signal_var = fmaxf(best_var - noise_var, 0.0f);
+    float denom = signal_var + noise_var;
+    float k = denom > 0.0f ? signal_var / denom : 0.0f;
+    return best_mean + k * (centre - best_mean);
+}
+
+template <int KIND, int HALF>
+__global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
+    extern __shared__ float tile[];
+    const int half = KIND == SK_RLEE ? 3 : (HALF > 0 ? HALF : p.half);
+    const int pitch = kTileW + 2 * half;
+    const int64_t c0 = (int64_t)blockIdx.x * kTileW;
+    const int64_t r0 = (int64_t)blockIdx.y * kTileH;
+    stage_tile(p, tile, half, r0, c0);
+    __syncthreads();
+    const int tx = threadIdx.x & (kTileW - 1);
+    const int ty0 = threadIdx.x >> 6;  // 0..3
+    const float cv_n_sq = 1.0f / p.enl;
+#pragma unroll 1
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const int ty = ty0 + 4 * k;
+        const int64_t r = r0 + ty, c = c0 + tx;
+        if (r >= p.rows || c >= p.cols) continue;
+        const float* t = tile + (ty + half) * pitch + (tx + half);
+        const float centre = t[0];
+        float o = __builtin_nanf("");
+        if (isfinite(centre)) {
+            if (KIND == SK_RLEE) o = refined_lee_pixel(t, pitch, centre, cv_n_sq);
+            else o = filter_pixel<KIND, HALF>(p, t, pitch, centre);
+        }
+        p.out[r * p.cols + c] = o;
+    }
+}
+
+template <int KIND, int HALF>
+static hipError_t launch_one(const SpeckleParams& p, hipStream_t s) {
+    const int half = KIND == SK_RLEE ? 3 : p.half;
+    size_t lds = (size_t)(kTileW + 2 * half) * (kTileH + 2 * half) * sizeof(float);
+    dim3 grid((unsigned)((p.cols + kTileW - 1) / kTileW), (unsigned)((p.rows + kTileH - 1) / kTileH));
+    hipLaunchKernelGGL((speckle_kernel<KIND, HALF>), grid, dim3(256), lds, s, p);
+    return hipGetLastError();
+}
+
+template <int KIND>
+static hipError_t launch_kind(const SpeckleParams& p, hipStream_t s) {
+    switch (p.half) {
+        case 1: return launch_one<KIND, 1>(p, s);
+        case 2: return launch_one<KIND, 2>(p, s);
+        case 3: return launch_one<KIND, 3>(p, s);
+        default: return launch_one<KIND, 0>(p, s);
+    }
+}
+
+}  // namespace sd
+
+// kind: sardine_speckle_kind; half = window/2 (≤ 31, checked by the caller).
+extern "C" hipError_t sd_launch_speckle(int kind, const float* in, float* out, int64_t cols, int64_t rows, int half,
+                                        float enl, float damping, hipStream_t stream) {
+    if (cols <= 0 || rows <= 0) return hipSuccess;
+    sd::SpeckleParams p{in, out, cols, rows, half, enl, damping};
+    switch (kind) {
+        case sd::SK_BOXCAR: return sd::launch_kind<sd::SK_BOXCAR>(p, stream);
+        case sd::SK_LEE: return sd::launch_kind<sd::SK_LEE>(p, stream);
+        case sd::SK_FROST: return sd::launch_kind<sd::SK_FROST>(p, stream);
+        case sd::SK_GAMMA: return sd::launch_kind<sd::SK_GAMMA>(p, stream);
+        default: p.half = 3; return sd::launch_one<sd::SK_RLEE, 3>(p, stream);
+    }
+}

@@ -1,0 +1,51 @@
+// tc_params.hpp — kernel-argument block of the terrain-correction kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "geom.hpp"
+
+namespace sd {
+
+constexpr int kHistSize = 16;                // NEWTON_ITER_HIST_SIZE (terrain_correction.rs:95)
+constexpr int kCntRedo = 7 + kHistSize;      // counters[kCntRedo] = chain redo rounds
+constexpr int kNumCounters = kCntRedo + 1;
+constexpr int kMaxAnchorPts = 1025;          // SARDINE_TC_PROJECTOR_ANCHORS ≤ 1024
+
+struct TcParams {
+    const OrbitDev* orbit;
+    // radar image (MergedSigma0 buffers)
+    const float* img;
+    const float* nesz;      // NULL → no NESZ masking
+    int64_t lines, samples;
+    int64_t az_start_us;    // relative to the first state vector
+    double near_range_m, rps, ati;
+    // DEM raster
+    const float* dem;
+    int64_t dem_w, dem_h;
+    double dem_olat, dem_olon, dem_pix;
+    const float* geoid;     // device 73x144 or NULL
+    // output grid
+    int crs, utm_zone, utm_south, n_anchor_pts;
+    double spacing, lat_north, lon_west;
+    double x_start, anchor_dx;
+    int64_t full_cols;
+    int64_t row0, col0, rows, cols;
+    const int64_t* row_seed;      // per AOI row, µs relative to the first state vector
+    const double* row_flat_area;  // per AOI row (need_geom)
+    // config
+    int max_iter;
+    double tol;
+    int flatten, compute_lia, need_geom, resampling;
+    double step_deg;
+    float nesz_factor;
+    // outputs
+    float* out;
+    float* cos_lia;
+    uint8_t* mask;
+    double* radar_coords;
+    unsigned long long* counters;  // kNumCounters
+    int* error_flag;
+};
+
+}  // namespace sd

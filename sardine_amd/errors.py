@@ -1,0 +1,59 @@
+"""Exception types mirroring the reference's error enums on the path.
+
+  TerrainCorrectionError  sardine/src/terrain_correction.rs:45-67
+  SpeckleError            sardine/src/speckle.rs:96-108
+  CalibrationError        sardine/src/apply_calibration.rs:64-84
+  DeburstError            sardine/src/deburst.rs:56-121
+  MergeError              sardine/src/merge_subswaths.rs:62-81
+Each carries the C-ABI status code (`.status`) and its name (`.variant`).
+"""
+from __future__ import annotations
+
+from . import abi
+
+
+class SardineError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.variant = abi.STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{self.variant}: {message}")
+
+
+class TerrainCorrectionError(SardineError):
+    pass
+
+
+class SpeckleError(SardineError):
+    pass
+
+
+class CalibrationError(SardineError):
+    pass
+
+
+class DeburstError(SardineError):
+    pass
+
+
+class MergeError(SardineError):
+    pass
+
+
+class DeviceError(SardineError):
+    pass
+
+
+def raise_for(status: int, message: str):
+    if status == abi.OK:
+        return
+    if 10 <= status < 20:
+        raise TerrainCorrectionError(status, message)
+    if 20 <= status < 30:
+        raise SpeckleError(status, message)
+    if 30 <= status < 40:
+        raise CalibrationError(status, message)
+    if 40 <= status < 50:
+        raise DeburstError(status, message)
+    if 50 <= status < 60:
+        raise MergeError(status, message)
+    raise DeviceError(status, message)
