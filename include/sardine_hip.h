@@ -415,6 +415,20 @@ sardine_status sardine_apply_calibration(const int16_t* deburst /*[device] lines
 sardine_status sardine_to_db_inplace(float* data /*[device]*/, int64_t n, float noise_floor_linear,
                                      uint64_t counts[2], void* stream);
 
+/* ── Kernel timing (diagnostic, no reference counterpart) ─────────────────
+ * With profiling on, every compute entry point brackets its main kernel with
+ * HIP events on the stream it launches on and waits for it; the duration of
+ * the last such kernel on this thread is then available by kernel family. */
+typedef enum sardine_kernel_family {
+    SARDINE_KERNEL_TC = 0,        /* terrain-correction kernel */
+    SARDINE_KERNEL_CALIBRATE = 1, /* fused deburst + calibration + merge kernel */
+    SARDINE_KERNEL_SPECKLE = 2,
+    SARDINE_KERNEL_TO_DB = 3
+} sardine_kernel_family;
+void sardine_set_profiling(int on);
+/* Milliseconds of the last profiled kernel of `family` on this thread, -1 if none. */
+double sardine_last_kernel_ms(int family);
+
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */
 
 /* Fill a CInt16 raster with round(sigma * N(0,1)) real/imag parts from a

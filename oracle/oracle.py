@@ -40,6 +40,7 @@ def lib():
         L.oracle_interpolate_orbit_and_accel.argtypes = [C.POINTER(abi.Orbit), i64, P, P, P]
         L.oracle_solve_zero_doppler.argtypes = [P, C.POINTER(abi.Orbit), i64, i32, d, C.POINTER(i64),
                                                 C.POINTER(i32), P]
+        L.oracle_to_image_coords.argtypes = [i64, i64, d, d, d, d, i64, i64, C.POINTER(d), C.POINTER(d)]
         L.oracle_sample.argtypes = [i32, P, i64, i64, d, d]
         L.oracle_sample.restype = f
         L.oracle_keys_weight.argtypes = [d]
@@ -124,6 +125,13 @@ def solve_zero_doppler(target, orbit: abi.Orbit, seed_rel_us: int, max_iter=10, 
     oc = lib().oracle_solve_zero_doppler(_ptr(t), C.byref(orbit), int(seed_rel_us), max_iter, tol,
                                          C.byref(out_t), C.byref(it), _ptr(sat))
     return oc, out_t.value, it.value, sat
+
+
+def to_image_coords(az_us, az_start_us, ati, slant, near_range, rps, n_lines, n_samples):
+    l, s = C.c_double(), C.c_double()
+    ok = lib().oracle_to_image_coords(int(az_us), int(az_start_us), ati, slant, near_range, rps, n_lines, n_samples,
+                                      C.byref(l), C.byref(s))
+    return (l.value, s.value) if ok else None
 
 
 def sample(kind, data: np.ndarray, line, smp):

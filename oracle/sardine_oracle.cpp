@@ -488,6 +488,17 @@ int64_t guess_azimuth_time(const SeedLut& l, double lat) {
     return l.t[idx - 1] + (int64_t)(frac * dt_us);
 }
 
+// RadarGeometry::to_image_coords (terrain_correction.rs:745-762); times in
+// µs on any common epoch.  Returns false outside [-0.5, n-0.5].
+bool to_image_coords(int64_t az, int64_t az_start, double ati, double slant, double near_range, double rps,
+                     int64_t n_lines, int64_t n_samples, double* line_o, double* sample_o) {
+    double line = (double)(az - az_start) / 1e6 / ati;
+    double sample = (slant - near_range) / rps;
+    *line_o = line;
+    *sample_o = sample;
+    return !(line < -0.5 || line > (double)n_lines - 0.5 || sample < -0.5 || sample > (double)n_samples - 0.5);
+}
+
 void set_err(char* buf, const char* msg) {
     if (buf) std::snprintf(buf, 256, "%s", msg);
 }
@@ -533,6 +544,11 @@ int oracle_solve_zero_doppler(const double target[3], const sardine_orbit* orbit
     *iterations = it;
     sat_pos[0] = p.x; sat_pos[1] = p.y; sat_pos[2] = p.z;
     return r;
+}
+
+int oracle_to_image_coords(int64_t az, int64_t az_start, double ati, double slant, double near_range, double rps,
+                           int64_t n_lines, int64_t n_samples, double* line, double* sample) {
+    return to_image_coords(az, az_start, ati, slant, near_range, rps, n_lines, n_samples, line, sample) ? 1 : 0;
 }
 
 float oracle_sample(int kind, const float* data, int64_t lines, int64_t samples, double line, double sample) {
@@ -733,12 +749,11 @@ int oracle_terrain_correction(const sardine_radar_image* merged, const sardine_s
                     A.hist[std::min(iters, SARDINE_NEWTON_HIST_SIZE - 1)]++;
                     prev = az;
                     double slant = norm3(sub3(target, sat));
-                    // to_image_coords (:745-762)
-                    double line = (double)(az - az_start) / 1e6 / ati;
-                    double sample = (slant - near_range_m) / rps;
+                    double line, sample;
+                    bool inside = to_image_coords(az, az_start, ati, slant, near_range_m, rps, n_lines, n_samples,
+                                                  &line, &sample);
                     if (orc) { orc[2 * cc] = line; orc[2 * cc + 1] = sample; }
-                    if (line < -0.5 || line > (double)n_lines - 0.5 || sample < -0.5 ||
-                        sample > (double)n_samples - 0.5) {
+                    if (!inside) {
                         A.outside++;
                         if (om) om[cc] = SARDINE_MASK_OUTSIDE_FOOTPRINT;
                         continue;

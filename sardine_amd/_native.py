@@ -53,9 +53,13 @@ def lib():
                                             C.POINTER(C.c_uint32), P]
     L.sardine_to_db_inplace.argtypes = [P, i64, f, P, P]
     L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]
+    L.sardine_set_profiling.argtypes = [C.c_int]
+    L.sardine_set_profiling.restype = None
+    L.sardine_last_kernel_ms.argtypes = [C.c_int]
+    L.sardine_last_kernel_ms.restype = C.c_double
     for name in abi.EXPORTED_SYMBOLS:
         if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",
-                        "sardine_tc_config_default"):
+                        "sardine_tc_config_default", "sardine_set_profiling", "sardine_last_kernel_ms"):
             getattr(L, name).restype = C.c_int
     if L.sardine_abi_version() != abi.ABI_VERSION:
         raise NativeLibraryError(f"ABI version mismatch: library {L.sardine_abi_version()}, "

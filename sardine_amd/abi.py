@@ -191,4 +191,7 @@ EXPORTED_SYMBOLS = (
     "sardine_apply_calibration",
     "sardine_to_db_inplace",
     "sardine_synth_cint16",
+    "sardine_set_profiling",
+    "sardine_last_kernel_ms",
 )
+KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB = 0, 1, 2, 3

@@ -37,6 +37,30 @@ extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t s
 namespace {
 
 thread_local std::string g_last_error;
+thread_local bool g_profile = false;
+thread_local double g_kernel_ms[4] = {-1.0, -1.0, -1.0, -1.0};
+
+// Brackets one kernel launch with events when profiling is on.
+struct KernelTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    int fam;
+    KernelTimer(int family, hipStream_t stream) : s(stream), fam(family) {
+        if (g_profile && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, s);
+    }
+    void stop() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, s);
+        if (hipEventSynchronize(b) == hipSuccess) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, a, b) == hipSuccess) g_kernel_ms[fam] = ms;
+        }
+    }
+    ~KernelTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
 
 sardine_status fail(sardine_status st, const char* fmt, ...) {
     char buf[512];
@@ -201,6 +225,8 @@ sardine_status output_grid(const sardine_scene* sc, const sardine_tc_config* cfg
 extern "C" {
 
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
+void sardine_set_profiling(int on) { g_profile = on != 0; }
+double sardine_last_kernel_ms(int family) { return family >= 0 && family < 4 ? g_kernel_ms[family] : -1.0; }
 int sardine_abi_version(void) { return SARDINE_ABI_VERSION; }
 
 int sardine_device_count(void) {
@@ -398,7 +424,11 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
 
     if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED)
         return fail(SARDINE_ERR_TC_CONFIG, "the FP32 scene-centred variant is not part of this build");
-    SD_HIP(sd_launch_tc_fp64(&p, stream));
+    {
+        KernelTimer kt(SARDINE_KERNEL_TC, stream);
+        SD_HIP(sd_launch_tc_fp64(&p, stream));
+        kt.stop();
+    }
 
     unsigned long long cnt[sd::kNumCounters + 1];
     SD_HIP(hipMemcpyAsync(cnt, d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost, stream));
@@ -461,7 +491,9 @@ sardine_status sardine_speckle_apply(const float* data, int64_t data_len, int64_
     if (data == out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "speckle filtering is out-of-place: out aliases data");
     sardine_status st;
     if ((st = require_device())) return st;
+    KernelTimer kt(SARDINE_KERNEL_SPECKLE, (hipStream_t)stream);
     SD_HIP(sd_launch_speckle(f->kind, data, out, cols, rows, win / 2, f->enl, f->damping, (hipStream_t)stream));
+    kt.stop();
     return SARDINE_OK;
 }
 
@@ -727,7 +759,11 @@ sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, 
         D.valid_start = std::max(col_start, D.out_off);
         D.valid_end = std::min(col_end, D.out_off + L.samples);
     }
-    SD_HIP(sd_launch_cal_merge(&P, s));
+    {
+        KernelTimer kt(SARDINE_KERNEL_CALIBRATE, s);
+        SD_HIP(sd_launch_cal_merge(&P, s));
+        kt.stop();
+    }
     SD_HIP(hipStreamSynchronize(s));
     return SARDINE_OK;
 }
@@ -784,7 +820,11 @@ sardine_status sardine_to_db_inplace(float* data, int64_t n, float floor_lin, ui
     DevBuf c;
     SD_HIP(c.alloc(2 * sizeof(unsigned long long), s));
     SD_HIP(sd_launch_zero_u64(c.as<unsigned long long>(), 2, s));
-    SD_HIP(sd_launch_to_db(data, n, floor_lin, c.as<unsigned long long>(), s));
+    {
+        KernelTimer kt(SARDINE_KERNEL_TO_DB, s);
+        SD_HIP(sd_launch_to_db(data, n, floor_lin, c.as<unsigned long long>(), s));
+        kt.stop();
+    }
     unsigned long long h[2];
     SD_HIP(hipMemcpyAsync(h, c.p, sizeof h, hipMemcpyDeviceToHost, s));
     SD_HIP(hipStreamSynchronize(s));

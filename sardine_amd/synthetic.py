@@ -501,7 +501,7 @@ def scene_iw_full(seed: int = 1, n_swaths: int = 3, n_bursts: int = 9, lines_per
                   dem_max_side: int = None) -> SyntheticScene:
     """Full single-IW-scene: 3 subswaths x 9 bursts of 1500 x 21000 CInt16,
     integer midpoint deburst (burst overlap `overlap_lines`) and IW merge,
-    fBm DEM at 1 arcsec, 10 m UTM output.  Smaller arguments give scaled
+    fBm DEM at 1 arcsec, 10 m UTM output over the footprint's bounding box.  Smaller arguments give scaled
     copies with the same structure."""
     rng = np.random.default_rng(seed)
     ati = 2.055556e-3
@@ -545,10 +545,8 @@ def scene_iw_full(seed: int = 1, n_swaths: int = 3, n_bursts: int = 9, lines_per
     corners_r = np.array([near_range, far_range, near_range, far_range])
     flat, flon = forward_geocode(orbit, corners_t, corners_r, 300.0)
     if out_rows is None:
-        # inner box of the parallelogram footprint
-        lats = np.sort(flat)
-        lons = np.sort(flon)
-        bbox = (float(lats[1]), float(lats[2]), float(lons[1]), float(lons[2]))
+        # the footprint's bounding box, as an annotation's geolocation grid gives it
+        bbox = (float(flat.min()), float(flat.max()), float(flon.min()), float(flon.max()))
     else:
         sp_deg = 10.0 / METRES_PER_DEG
         bbox = _aligned_bbox(clat, clon, out_rows, out_cols, sp_deg)

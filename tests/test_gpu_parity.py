@@ -1,0 +1,217 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars (from the path's contract): masks, counters and integer layouts
+bit-exact; radar geolocation within 1e-3 line/sample; sigma0 within 1e-3 dB
+(relative 2.3e-4); calibration and speckle window statistics bit-exact where
+the arithmetic is the same IEEE sequence.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from sardine_amd import TerrainCorrectionError, abi, api, synthetic
+from sardine_amd.workload import DeviceScene
+
+pytestmark = pytest.mark.gpu
+
+DB_REL = 10 ** (1e-3 / 10) - 1  # 1e-3 dB as a relative bound (≈2.3e-4)
+
+
+def _meta(info):
+    return dict(lines=info.lines, samples=info.samples, azimuth_start_time_us=info.azimuth_start_time_us,
+                near_slant_range_time_s=info.near_slant_range_time_s,
+                range_pixel_spacing_m=info.range_pixel_spacing_m)
+
+
+@pytest.fixture(scope="module")
+def small(cuda):
+    sc = synthetic.scene_small_plain(5, lines=512, samples=1024, out_rows=400, out_cols=400, dem_side=512)
+    ds = DeviceScene.build(sc, cuda, host_raw=True)
+    info, s0, nz, _ = O.prepare_merged(sc)
+    return sc, ds, info, s0, nz
+
+
+@pytest.fixture(scope="module")
+def iw(cuda):
+    sc = synthetic.scene_iw_full(2, n_swaths=3, n_bursts=3, lines_per_burst=400, raw_cols=1600,
+                                 overlap_lines=60, swath_overlap=120, utm=True, dem_max_side=None)
+    ds = DeviceScene.build(sc, cuda, host_raw=True)
+    info, s0, nz, layouts = O.prepare_merged(sc)
+    return sc, ds, info, s0, nz, layouts
+
+
+def test_calibration_small_bit_exact(small):
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    assert (m.lines, m.samples) == (info.lines, info.samples)
+    assert np.array_equal(m.data.cpu().numpy(), s0)
+    assert np.array_equal(m.nesz.cpu().numpy(), nz)
+    assert m.near_slant_range_time_s == info.near_slant_range_time_s
+    assert m.azimuth_start_time_us == info.azimuth_start_time_us
+
+
+def test_deburst_merge_calibration_iw_bit_exact(iw):
+    sc, ds, info, s0, nz, layouts = iw
+    m = ds.prepare_merged()
+    gi = m.info
+    assert (gi.lines, gi.samples) == (info.lines, info.samples)
+    assert list(gi.sw_out_offset)[:3] == list(info.sw_out_offset)[:3]
+    assert list(gi.seam)[:2] == list(info.seam)[:2]
+    assert gi.cal_lut_extrapolation_gap_px == info.cal_lut_extrapolation_gap_px
+    assert gi.noise_lut_extrapolation_gap_px == info.noise_lut_extrapolation_gap_px
+    a, b = m.data.cpu().numpy(), m.nesz.cpu().numpy()
+    assert np.array_equal(np.isnan(a), np.isnan(s0))
+    assert np.array_equal(a, s0, equal_nan=True)
+    assert np.array_equal(b, nz, equal_nan=True)
+
+
+def _tc_compare(sc, ds, info, s0, nz, m, **kw):
+    cfg = ds.tc_config(compute_lia=True, **kw)
+    img = ds.terrain_correction(m, cfg, radar_coords=True)
+    keep = []
+    ccfg = sc.tc_config(compute_lia=1, **{k: (int(v) if isinstance(v, bool) else v) for k, v in kw.items()})
+    ref = O.terrain_correction(s0, nz, _meta(info), sc.scene_c(keep), sc.dem(), sc.dem_spec, ccfg, coords=True)
+    assert ref.status == 0, ref.error
+    mk = img.mask.cpu().numpy()
+    assert np.array_equal(mk, ref.mask), np.bincount((mk != ref.mask).ravel())
+    rc = img.radar_coords.cpu().numpy()
+    both = np.isfinite(ref.coords[..., 0])
+    assert np.array_equal(np.isfinite(rc[..., 0]), both)
+    dl = np.abs(rc[..., 0][both] - ref.coords[..., 0][both])
+    ds_ = np.abs(rc[..., 1][both] - ref.coords[..., 1][both])
+    assert dl.max() <= 1e-3 and ds_.max() <= 1e-3, (dl.max(), ds_.max())
+    d = img.data.cpu().numpy()
+    assert np.array_equal(np.isnan(d), np.isnan(ref.data))
+    v = np.isfinite(ref.data)
+    rel = np.abs(d[v] - ref.data[v]) / np.maximum(np.abs(ref.data[v]), 1e-30)
+    assert rel.max() <= DB_REL, rel.max()
+    c = ref.counters()
+    assert img.valid_pixel_count == c["valid"]
+    assert img.dem_missing_count == c["dem_missing"]
+    assert img.outside_footprint_count == c["outside_footprint"]
+    assert img.non_converged_count == c["non_converged"]
+    assert img.degenerate_geometry_count == c["degenerate"]
+    assert img.flat_masked_count == c["flat_masked"]
+    assert img.noise_masked_count == c["noise_masked"]
+    assert img.newton_iter_histogram == c["hist"]
+    cl = img.cos_lia.cpu().numpy()
+    assert np.array_equal(np.isnan(cl), np.isnan(ref.cos_lia))
+    f = np.isfinite(ref.cos_lia)
+    if f.any():
+        assert np.abs(cl[f] - ref.cos_lia[f]).max() <= 1e-5
+    return img, ref
+
+
+def test_tc_wgs84_bilinear(small):
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    img, ref = _tc_compare(sc, ds, info, s0, nz, m)
+    assert img.valid_pixel_count > 0.3 * img.total_pixel_count
+    assert tuple(img.geotransform) == tuple(ref.img.geotransform)
+
+
+@pytest.mark.parametrize("kernel", [abi.RESAMPLE_BICUBIC, abi.RESAMPLE_LANCZOS3])
+def test_tc_wgs84_resamplers(small, kernel):
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    _tc_compare(sc, ds, info, s0, nz, m, resampling=kernel)
+
+
+def test_tc_flatten_and_nesz(small):
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    img, ref = _tc_compare(sc, ds, info, s0, nz, m, flatten=True, noise_floor_margin_db=3.0)
+    assert img.flat_masked_count + img.noise_masked_count > 0
+
+
+def test_tc_utm_iw(iw):
+    sc, ds, info, s0, nz, _ = iw
+    m = ds.prepare_merged()
+    img, ref = _tc_compare(sc, ds, info, s0, nz, m)
+    assert img.valid_pixel_count > 0
+
+
+def test_tc_aoi_window(small):
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    cfg = ds.tc_config(compute_lia=True, aoi=(37, 51, 130, 177))
+    img = ds.terrain_correction(m, cfg)
+    keep = []
+    ccfg = sc.tc_config(compute_lia=1, aoi_row0=37, aoi_col0=51, aoi_rows=130, aoi_cols=177)
+    ref = O.terrain_correction(s0, nz, _meta(info), sc.scene_c(keep), sc.dem(), sc.dem_spec, ccfg)
+    assert np.array_equal(img.mask.cpu().numpy(), ref.mask)
+    d = img.data.cpu().numpy()
+    v = np.isfinite(ref.data)
+    assert np.array_equal(np.isfinite(d), v)
+    assert (np.abs(d[v] - ref.data[v]) / ref.data[v]).max() <= DB_REL
+
+
+SPECKLES = [
+    (abi.SPECKLE_BOXCAR, 3, 1.0, 1.0), (abi.SPECKLE_BOXCAR, 7, 1.0, 1.0),
+    (abi.SPECKLE_LEE, 5, 1.0, 1.0), (abi.SPECKLE_LEE, 5, 4.0, 1.0),
+    (abi.SPECKLE_FROST, 5, 1.0, 1.0), (abi.SPECKLE_FROST, 7, 1.0, 2.0),
+    (abi.SPECKLE_GAMMA_MAP, 5, 1.0, 1.0), (abi.SPECKLE_GAMMA_MAP, 7, 4.0, 1.0),
+    (abi.SPECKLE_REFINED_LEE, 7, 1.0, 1.0), (abi.SPECKLE_REFINED_LEE, 7, 4.0, 1.0),
+    (abi.SPECKLE_BOXCAR, 9, 1.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("kind,window,enl,damping", SPECKLES)
+def test_speckle_filters(cuda, kind, window, enl, damping):
+    import torch
+
+    rng = np.random.default_rng(kind * 100 + window)
+    rows, cols = 137, 211
+    x = rng.exponential(0.1, (rows, cols)).astype(np.float32)
+    x[rng.random((rows, cols)) < 0.03] = np.nan  # nodata, as the TC output has
+    x[:5, :9] = np.nan
+    st, ref = O.speckle(x, kind, window, enl, damping)
+    assert st == 0
+    f = api.SpeckleFilter(kind, window, enl, damping)
+    out = f.apply(torch.from_numpy(x).to(cuda), cols, rows).cpu().numpy()
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    v = np.isfinite(ref)
+    rel = np.abs(out[v] - ref[v]) / np.maximum(np.abs(ref[v]), 1e-30)
+    assert rel.max() <= DB_REL, rel.max()
+    if kind in (abi.SPECKLE_BOXCAR, abi.SPECKLE_LEE, abi.SPECKLE_REFINED_LEE):
+        assert np.array_equal(out[v], ref[v])  # same IEEE sequence end to end
+
+
+def test_to_db(cuda):
+    import torch
+
+    rng = np.random.default_rng(3)
+    x = rng.exponential(0.1, 100_000).astype(np.float32)
+    x[::17] = np.nan
+    x[::23] = 1e-6
+    ref = x.copy()
+    st, conv, masked = O.to_db_inplace(ref, 1e-5)
+    t = torch.from_numpy(x).to(cuda)
+    c2, m2 = api.to_db_inplace(t, 1e-5)
+    assert (c2, m2) == (conv, masked)
+    out = t.cpu().numpy()
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    v = np.isfinite(ref)
+    assert np.abs(out[v] - ref[v]).max() <= 1e-4
+
+
+def test_synth_generator_matches_host(cuda):
+    import torch
+
+    t = torch.empty((3000, 2), dtype=torch.int16, device=cuda)
+    api.synth_cint16(t, 77, 1234, 100.0)
+    host = synthetic.complex_gaussian_dn(77, 1234, 3000)
+    assert np.abs(t.cpu().numpy().astype(np.int32) - host.astype(np.int32)).max() <= 1
+    assert (t.cpu().numpy() == host).mean() > 0.999
+
+
+def test_orbit_window_error_aborts(small):
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    scene = api.SceneInputs.from_synthetic(sc)
+    scene.sv_time_us = scene.sv_time_us[:9]  # orbit ends long before the scene → Newton leaves the window
+    scene.sv_pos = scene.sv_pos[:9]
+    scene.sv_vel = scene.sv_vel[:9]
+    with pytest.raises(TerrainCorrectionError) as e:
+        api.terrain_correction(m, scene, ds.dem, ds.tc_config())
+    assert getattr(e.value, "status", None) == abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
