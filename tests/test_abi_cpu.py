@@ -1,0 +1,148 @@
+"""CPU-side checks of the C ABI (no GPU needed).
+
+The library must load, export every symbol include/sardine_hip.h declares,
+compute the host-only layouts exactly as the oracle does, reject invalid
+arguments with the reference's error variants, and refuse compute work
+without a gfx950 device (there is no CPU fallback).
+"""
+import ctypes as C
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from sardine_amd import DeburstError, _native, abi, api, synthetic
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def test_header_symbols_are_exported():
+    hdr = (ROOT / "include" / "sardine_hip.h").read_text()
+    declared = set(re.findall(r"\b(sardine_[a-z0-9_]+)\s*\(", hdr))
+    assert declared == set(abi.EXPORTED_SYMBOLS), declared ^ set(abi.EXPORTED_SYMBOLS)
+    lib = C.CDLL(str(_native.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert _native.lib().sardine_abi_version() == abi.ABI_VERSION
+
+
+def test_struct_sizes_match_header_layout():
+    # field order/size mismatches would show up as a size mismatch here
+    assert C.sizeof(abi.StateVector) == 56
+    assert C.sizeof(abi.GridPoint) == 16
+    assert C.sizeof(abi.SpeckleFilterC) == 16
+    assert C.sizeof(abi.DeburstLayout) == 8 * 3 + 8 + 4 * 64 * 8
+
+
+def _scene(sc, keep):
+    return sc.scene_c(keep)
+
+
+@pytest.mark.parametrize("utm", [False, True])
+def test_output_grid_matches_oracle(utm):
+    sc = synthetic.scene_iw_full(3, n_swaths=2, n_bursts=2, lines_per_burst=300, raw_cols=1000, overlap_lines=60,
+                                 swath_overlap=80, utm=utm)
+    keep = []
+    cfg = sc.tc_config()
+    st, r, c, gt = O.tc_output_grid(_scene(sc, keep), cfg)
+    r2, c2, gt2 = api.output_grid(api.SceneInputs.from_synthetic(sc), api.TerrainCorrectionConfig(
+        pixel_spacing_deg=sc.pixel_spacing, crs=sc.crs, utm_zone=sc.utm_zone))
+    assert st == 0 and (r, c) == (r2, c2)
+    assert np.allclose(gt, gt2, rtol=0, atol=1e-6 if utm else 1e-12)
+
+
+def test_deburst_and_merge_layout_match_oracle():
+    sc = synthetic.scene_iw_full(4, n_swaths=3, n_bursts=4, lines_per_burst=500, raw_cols=1500, overlap_lines=77,
+                                 swath_overlap=100)
+    keep = []
+    swc = sc.swath_inputs_c([0, 0, 0], keep)
+    info = api.merged_layout(swc, 3)
+    for i, sw in enumerate(sc.swaths):
+        b = sw.bursts_c()
+        s = abi.SubSwath(sw.n_bursts, sw.lines_per_burst, sc.ati, sw.slant_range_time_s, sc.rps_m)
+        st, Lo = O.deburst_layout(s, b, sw.n_bursts, sw.tiff_height)
+        Ln = api.deburst_layout(s, b, sw.n_bursts, sw.tiff_height)
+        assert st == 0
+        for f in ("lines", "samples", "valid_sample_offset", "n_bursts"):
+            assert getattr(Lo, f) == getattr(Ln, f)
+        assert list(Lo.overlap)[:3] == list(Ln.overlap)[:3]
+        assert list(Lo.local_first)[:4] == list(Ln.local_first)[:4]
+        assert list(Lo.local_end)[:4] == list(Ln.local_end)[:4]
+        assert info.swath_lines[i] == Lo.lines
+    # merge offsets/seams from the oracle's merge on dummy arrays of the same shapes
+    sig = [np.zeros((int(info.swath_lines[i]), int(info.swath_samples[i])), np.float32) for i in range(3)]
+    st, mi, _, _ = O.merge(sig, sig, [int(info.swath_valid_offset[i]) for i in range(3)],
+                           [sw.slant_range_time_s for sw in sc.swaths], [sc.rps_m] * 3,
+                           [int(sw.burst_times_us[0]) for sw in sc.swaths], query_only=True)
+    assert st == 0
+    assert (mi.lines, mi.samples) == (info.lines, info.samples)
+    assert list(mi.sw_out_offset)[:3] == list(info.sw_out_offset)[:3]
+    assert list(mi.seam)[:2] == list(info.seam)[:2]
+    assert mi.near_slant_range_time_s == info.near_slant_range_time_s
+    assert mi.azimuth_start_time_us == info.azimuth_start_time_us
+
+
+def test_deburst_errors_through_abi():
+    sw = abi.SubSwath(2, 1000, 2e-3, 0.005, 2.33)
+    b = (abi.Burst * 2)(abi.Burst(0, 0, 0, 100, 0), abi.Burst(100_000, 1000, 0, 100, 1))
+    with pytest.raises(DeburstError) as e:
+        api.deburst_layout(sw, b, 2, 2000)
+    assert e.value.status == abi.ERR_DEBURST_IMPLAUSIBLE_OVERLAP
+    with pytest.raises(Exception) as e:
+        api.deburst_layout(sw, b, 0, 2000)
+    assert e.value.status == abi.ERR_DEBURST_NO_BURSTS
+
+
+def test_speckle_argument_errors_precede_device_check():
+    L = _native.lib()
+    d = np.zeros(25, np.float32)
+    cases = [((abi.SPECKLE_BOXCAR, 4, 1.0, 1.0), 25, abi.ERR_SPECKLE_BAD_WINDOW),
+             ((abi.SPECKLE_LEE, 3, 0.0, 1.0), 25, abi.ERR_SPECKLE_BAD_ENL),
+             ((abi.SPECKLE_REFINED_LEE, 0, float("nan"), 1.0), 25, abi.ERR_SPECKLE_BAD_ENL),
+             ((abi.SPECKLE_FROST, 3, 1.0, -1.0), 25, abi.ERR_SPECKLE_BAD_DAMPING),
+             ((abi.SPECKLE_BOXCAR, 3, 1.0, 1.0), 24, abi.ERR_SPECKLE_DIMENSION)]
+    for (k, w, e, dmp), n, exp in cases:
+        f = abi.SpeckleFilterC(k, w, e, dmp)
+        st = L.sardine_speckle_apply(d.ctypes.data, n, 5, 5, C.byref(f), d.ctypes.data, None)
+        assert st == exp, (k, w, st)
+
+
+def test_compute_without_device_fails_loudly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    L = _native.lib()
+    f = abi.SpeckleFilterC(abi.SPECKLE_BOXCAR, 3, 1.0, 1.0)
+    a = np.zeros(25, np.float32)
+    b = np.zeros(25, np.float32)
+    st = L.sardine_speckle_apply(a.ctypes.data, 25, 5, 5, C.byref(f), b.ctypes.data, None)
+    assert st == abi.ERR_NO_DEVICE
+    assert "no CPU fallback" in _native.last_error()
+    counts = (C.c_uint64 * 2)()
+    assert L.sardine_to_db_inplace(a.ctypes.data, 25, 0.0, counts, None) == abi.ERR_NO_DEVICE
+
+
+def test_tc_config_validation_errors():
+    sc = synthetic.scene_small_plain(0, lines=64, samples=64, out_rows=8, out_cols=8, dem_side=8)
+    scene = api.SceneInputs.from_synthetic(sc)
+    cfg = api.TerrainCorrectionConfig(pixel_spacing_deg=-1.0)
+    with pytest.raises(Exception) as e:
+        api.output_grid(scene, cfg)
+    assert e.value.status == abi.ERR_TC_CONFIG
+    L = _native.lib()
+    keep = []
+    s = sc.scene_c(keep)
+    s.n_grid_points = 0
+    c = sc.tc_config()
+    img = abi.GeocodedImage()
+    st = L.sardine_terrain_correction(C.byref(abi.RadarImage()), C.byref(s), C.byref(abi.DemRaster()), C.byref(c),
+                                      C.byref(img), None)
+    assert st == abi.ERR_TC_NO_GRID_POINTS
+    s = sc.scene_c(keep)
+    s.n_subswaths = 0
+    st = L.sardine_terrain_correction(C.byref(abi.RadarImage()), C.byref(s), C.byref(abi.DemRaster()), C.byref(c),
+                                      C.byref(img), None)
+    assert st == abi.ERR_TC_NO_SUBSWATHS
