@@ -193,5 +193,6 @@ EXPORTED_SYMBOLS = (
     "sardine_synth_cint16",
     "sardine_set_profiling",
     "sardine_last_kernel_ms",
+    "sardine_set_tc_segment",
 )
 KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB = 0, 1, 2, 3

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -38,6 +39,7 @@ namespace {
 
 thread_local std::string g_last_error;
 thread_local bool g_profile = false;
+std::atomic<int64_t> g_tc_min_seg{32};  // minimum columns per lane segment (tunable, see sardine_set_tc_segment)
 thread_local double g_kernel_ms[4] = {-1.0, -1.0, -1.0, -1.0};
 
 // Brackets one kernel launch with events when profiling is on.
@@ -226,6 +228,7 @@ extern "C" {
 
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
 void sardine_set_profiling(int on) { g_profile = on != 0; }
+void sardine_set_tc_segment(int64_t min_columns) { g_tc_min_seg = min_columns < 1 ? 1 : min_columns; }
 double sardine_last_kernel_ms(int family) { return family >= 0 && family < 4 ? g_kernel_ms[family] : -1.0; }
 int sardine_abi_version(void) { return SARDINE_ABI_VERSION; }
 
@@ -404,6 +407,23 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.col0 = c0;
     p.rows = nr;
     p.cols = nc;
+    // work decomposition: one lane per segment of seg_len columns, whole rows per workgroup
+    {
+        int64_t L = std::max<int64_t>(g_tc_min_seg.load(), (nc + 255) / 256);
+        int64_t nseg = (nc + L - 1) / L;
+        const int64_t lds_row = utm ? (int64_t)2 * (n_anch + 1) * (int64_t)sizeof(double) : 0;
+        const int64_t cap_rows = lds_row ? std::max<int64_t>(1, (int64_t)sd::kMaxTcLds / lds_row) : 256;
+        int64_t rpw = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(256 / nseg, cap_rows), nr));
+        // keep at least half the lanes busy on short rows by shortening segments
+        while (L / 2 >= std::max<int64_t>(g_tc_min_seg.load(), 8) && rpw * nseg < 128 && rpw < nr && rpw < cap_rows) {
+            L = (L + 1) / 2;
+            nseg = (nc + L - 1) / L;
+            rpw = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(256 / nseg, cap_rows), nr));
+        }
+        p.seg_len = (int)L;
+        p.nseg = (int)nseg;
+        p.rows_per_wg = (int)rpw;
+    }
     p.row_seed = d_seed.as<int64_t>();
     p.row_flat_area = d_flat.as<double>();
     p.max_iter = cfg->max_newton_iterations;

@@ -11,6 +11,7 @@ constexpr int kHistSize = 16;                // NEWTON_ITER_HIST_SIZE (terrain_c
 constexpr int kCntRedo = 7 + kHistSize;      // counters[kCntRedo] = chain redo rounds
 constexpr int kNumCounters = kCntRedo + 1;
 constexpr int kMaxAnchorPts = 1025;          // SARDINE_TC_PROJECTOR_ANCHORS ≤ 1024
+constexpr size_t kMaxTcLds = 64 * 1024;      // dynamic LDS budget of the UTM anchors
 
 struct TcParams {
     const OrbitDev* orbit;
@@ -31,6 +32,7 @@ struct TcParams {
     double x_start, anchor_dx;
     int64_t full_cols;
     int64_t row0, col0, rows, cols;
+    int seg_len, nseg, rows_per_wg;  // work decomposition (see tc.hip)
     const int64_t* row_seed;      // per AOI row, µs relative to the first state vector
     const double* row_flat_area;  // per AOI row (need_geom)
     // config

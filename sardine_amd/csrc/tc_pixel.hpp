@@ -1,0 +1,135 @@
+// tc_pixel.hpp — per-pixel pieces of terrain correction shared by the
+// terrain-correction kernels: DEM and geoid sampling, the three resamplers.
+// Reference: sardine/src/dem.rs:351-409, 694-723; geoid.rs:220-251;
+// terrain_correction.rs:770-983.
+#pragma once
+
+#include "geom.hpp"
+#include "tc_params.hpp"
+
+namespace sd {
+
+__device__ __forceinline__ bool dem_sample(const TcParams& p, double lat, double lon, float* h) {
+    // dem.rs:694-723 lon wrap, GLO-30 tile rule dem.rs:351-372, bilinear_on_grid dem.rs:385-409
+    double lon_n = lon >= 180.0 ? lon - 360.0 : lon;
+    double row_f = (p.dem_olat - lat) / p.dem_pix;
+    double col_f = (lon_n - p.dem_olon) / p.dem_pix;
+    double max_row = (double)(p.dem_h - 1), max_col = (double)(p.dem_w - 1);
+    if (row_f < -0.5 || row_f > max_row + 0.5 || col_f < -0.5 || col_f > max_col + 0.5) return false;
+    row_f = fmin(fmax(row_f, 0.0), max_row);
+    col_f = fmin(fmax(col_f, 0.0), max_col);
+    int64_t max_rc = p.dem_h - 1;
+    int64_t r0 = (int64_t)row_f;
+    int64_t lim_r = max_rc > 0 ? max_rc - 1 : 0;
+    r0 = r0 < lim_r ? r0 : lim_r;
+    int64_t c0 = (int64_t)col_f;
+    int64_t lim_c = p.dem_w >= 2 ? p.dem_w - 2 : 0;
+    c0 = c0 < lim_c ? c0 : lim_c;
+    double dr = row_f - (double)r0, dc = col_f - (double)c0;
+    const float* d = p.dem + r0 * p.dem_w + c0;
+    float v00 = d[0], v01 = d[1], v10 = d[p.dem_w], v11 = d[p.dem_w + 1];
+    if (isnan(v00) || isnan(v01) || isnan(v10) || isnan(v11)) return false;
+    double v = (double)v00 * (1.0 - dr) * (1.0 - dc) + (double)v01 * (1.0 - dr) * dc +
+               (double)v10 * dr * (1.0 - dc) + (double)v11 * dr * dc;
+    *h = (float)v;
+    return true;
+}
+
+__device__ __forceinline__ double geoid_n(const TcParams& p, double lat, double lon) {
+    // geoid.rs:220-251 (2.5° grid, lon wrap, lat clamp)
+    if (!p.geoid) return 0.0;
+    double l = fmod(lon, 360.0);
+    if (l < 0.0) l += 360.0;
+    double la = fmin(fmax(lat, -90.0), 90.0);
+    double row_f = (90.0 - la) / 2.5, col_f = l / 2.5;
+    int r0 = (int)row_f;
+    r0 = r0 < 71 ? r0 : 71;
+    int c0 = ((int)col_f) % 144, c1 = (c0 + 1) % 144;
+    double dr = row_f - r0;
+    double dc = fmin(fmax(col_f - c0, 0.0), 1.0);
+    const float* g = p.geoid;
+    double v00 = g[r0 * 144 + c0], v01 = g[r0 * 144 + c1], v10 = g[(r0 + 1) * 144 + c0], v11 = g[(r0 + 1) * 144 + c1];
+    return v00 * (1.0 - dr) * (1.0 - dc) + v01 * (1.0 - dr) * dc + v10 * dr * (1.0 - dc) + v11 * dr * dc;
+}
+
+__device__ __forceinline__ double keys_w(double t) {  // terrain_correction.rs:820-829
+    t = fabs(t);
+    if (t < 1.0) return 1.5 * t * t * t - 2.5 * t * t + 1.0;
+    if (t < 2.0) return -0.5 * t * t * t + 2.5 * t * t - 4.0 * t + 2.0;
+    return 0.0;
+}
+
+__device__ __forceinline__ double lanczos_w(double x) {  // terrain_correction.rs:938-950
+    double ax = fabs(x);
+    if (ax < 1e-10) return 1.0;
+    if (ax < 3.0) {
+        double px = kPi * x;
+        double pxa = kPi * x / 3.0;
+        return sin(px) * sin(pxa) / (px * pxa);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// terrain_correction.rs:770-797
+__device__ __forceinline__ float bilinear(const float* d, int64_t nl, int64_t ns, double line, double sample) {
+    int64_t ml = nl > 0 ? nl - 1 : 0, ms = ns > 0 ? ns - 1 : 0;
+    int64_t l0 = clampi((int64_t)floor(line), 0, ml), l1 = l0 + 1 < ml ? l0 + 1 : ml;
+    int64_t s0 = clampi((int64_t)floor(sample), 0, ms), s1 = s0 + 1 < ms ? s0 + 1 : ms;
+    double dl = line - (double)l0, ds = sample - (double)s0;
+    float v00 = d[l0 * ns + s0], v01 = d[l0 * ns + s1], v10 = d[l1 * ns + s0], v11 = d[l1 * ns + s1];
+    if (isnan(v00) || isnan(v01) || isnan(v10) || isnan(v11)) return __builtin_nanf("");
+    double w00 = (1.0 - dl) * (1.0 - ds), w01 = (1.0 - dl) * ds, w10 = dl * (1.0 - ds), w11 = dl * ds;
+    return (float)((double)v00 * w00 + (double)v01 * w01 + (double)v10 * w10 + (double)v11 * w11);
+}
+
+// terrain_correction.rs:842-872
+__device__ __forceinline__ float bicubic(const float* d, int64_t nl, int64_t ns, double line, double sample) {
+    int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
+    double dl = line - (double)lf, ds = sample - (double)sf;
+    double ws[4];
+#pragma unroll
+    for (int si = 0; si < 4; ++si) ws[si] = keys_w((double)(si - 1) - ds);
+    double acc = 0.0;
+#pragma unroll
+    for (int li = -1; li <= 2; ++li) {
+        const float* row = d + clampi(lf + li, 0, nl - 1) * ns;
+        double wl = keys_w((double)li - dl);
+#pragma unroll
+        for (int si = -1; si <= 2; ++si) {
+            float v = row[clampi(sf + si, 0, ns - 1)];
+            if (isnan(v)) return __builtin_nanf("");
+            acc += (double)v * wl * ws[si + 1];
+        }
+    }
+    return (float)acc;
+}
+
+// terrain_correction.rs:958-983
+__device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns, double line, double sample) {
+    int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
+    double ws[6];
+#pragma unroll
+    for (int si = 0; si < 6; ++si) ws[si] = lanczos_w(sample - (double)(sf + si - 2));
+    double acc = 0.0;
+    for (int li = -2; li <= 3; ++li) {
+        const float* row = d + clampi(lf + li, 0, nl - 1) * ns;
+        double wl = lanczos_w(line - (double)(lf + li));
+#pragma unroll
+        for (int si = -2; si <= 3; ++si) {
+            float v = row[clampi(sf + si, 0, ns - 1)];
+            if (isnan(v)) return __builtin_nanf("");
+            acc += (double)v * wl * ws[si + 2];
+        }
+    }
+    return (float)acc;
+}
+
+__device__ __forceinline__ float resample(const TcParams& p, double line, double sample) {
+    if (p.resampling == 1) return bicubic(p.img, p.lines, p.samples, line, sample);
+    if (p.resampling == 2) return lanczos3(p.img, p.lines, p.samples, line, sample);
+    return bilinear(p.img, p.lines, p.samples, line, sample);
+}
+
+}  // namespace sd

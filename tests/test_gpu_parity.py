@@ -215,3 +215,21 @@ def test_orbit_window_error_aborts(small):
     with pytest.raises(TerrainCorrectionError) as e:
         api.terrain_correction(m, scene, ds.dem, ds.tc_config())
     assert getattr(e.value, "status", None) == abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
+
+
+@pytest.mark.parametrize("min_seg", [1, 5, 64, 100000])
+def test_tc_chain_segmentation_invariant(small, min_seg):
+    """Segment length (lanes chaining 1..all columns of a row) must not
+    change a single output bit: the seed-chain verification makes every
+    decomposition equal the reference's sequential chain."""
+    from sardine_amd import _native
+
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    L = _native.lib()
+    try:
+        L.sardine_set_tc_segment(min_seg)
+        img, ref = _tc_compare(sc, ds, info, s0, nz, m)
+    finally:
+        L.sardine_set_tc_segment(32)
+    assert img.valid_pixel_count == ref.counters()["valid"]
