@@ -426,10 +426,10 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_TO_DB = 3
 } sardine_kernel_family;
 void sardine_set_profiling(int on);
-/* Minimum number of output columns one lane chains sequentially in the
- * terrain-correction kernel (default 32; a tuning knob with no effect on
- * results — the seed-chain verification makes every value exact). */
-void sardine_set_tc_segment(int64_t min_columns);
+/* Number of output columns one lane chains sequentially in the
+ * terrain-correction kernel (0 = automatic, the default).  A tuning knob
+ * with no effect on results: the seed-chain fix-up makes every value exact. */
+void sardine_set_tc_segment(int64_t columns);
 /* Milliseconds of the last profiled kernel of `family` on this thread, -1 if none. */
 double sardine_last_kernel_ms(int family);
 

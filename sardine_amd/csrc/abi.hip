@@ -39,7 +39,7 @@ namespace {
 
 thread_local std::string g_last_error;
 thread_local bool g_profile = false;
-std::atomic<int64_t> g_tc_min_seg{32};  // minimum columns per lane segment (tunable, see sardine_set_tc_segment)
+std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 thread_local double g_kernel_ms[4] = {-1.0, -1.0, -1.0, -1.0};
 
 // Brackets one kernel launch with events when profiling is on.
@@ -228,7 +228,7 @@ extern "C" {
 
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
 void sardine_set_profiling(int on) { g_profile = on != 0; }
-void sardine_set_tc_segment(int64_t min_columns) { g_tc_min_seg = min_columns < 1 ? 1 : min_columns; }
+void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : columns; }
 double sardine_last_kernel_ms(int family) { return family >= 0 && family < 4 ? g_kernel_ms[family] : -1.0; }
 int sardine_abi_version(void) { return SARDINE_ABI_VERSION; }
 
@@ -328,6 +328,7 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
             orb.vel[i][c] = s.velocity_m_s[c];
         }
     }
+    sd::orbit_fill_bary(orb);
 
     // ── per-row constants (terrain_correction.rs:1424-1511) ──
     const double sp = cfg->pixel_spacing;
@@ -407,22 +408,24 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.col0 = c0;
     p.rows = nr;
     p.cols = nc;
-    // work decomposition: one lane per segment of seg_len columns, whole rows per workgroup
+    // work decomposition: one lane per (row, segment of seg_len columns)
+    DevBuf d_anch, d_rec;
     {
-        int64_t L = std::max<int64_t>(g_tc_min_seg.load(), (nc + 255) / 256);
-        int64_t nseg = (nc + L - 1) / L;
-        const int64_t lds_row = utm ? (int64_t)2 * (n_anch + 1) * (int64_t)sizeof(double) : 0;
-        const int64_t cap_rows = lds_row ? std::max<int64_t>(1, (int64_t)sd::kMaxTcLds / lds_row) : 256;
-        int64_t rpw = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(256 / nseg, cap_rows), nr));
-        // keep at least half the lanes busy on short rows by shortening segments
-        while (L / 2 >= std::max<int64_t>(g_tc_min_seg.load(), 8) && rpw * nseg < 128 && rpw < nr && rpw < cap_rows) {
-            L = (L + 1) / 2;
-            nseg = (nc + L - 1) / L;
-            rpw = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(256 / nseg, cap_rows), nr));
+        int64_t L = g_tc_seg.load();
+        if (L <= 0) {  // aim for ~2 waves of 64 lanes per SIMD over the whole grid
+            const int64_t target_lanes = (int64_t)256 * 4 * 2 * 64;
+            L = std::min<int64_t>(128, std::max<int64_t>(16, (nr * nc + target_lanes - 1) / target_lanes));
         }
+        L = std::min<int64_t>(L, nc);
         p.seg_len = (int)L;
-        p.nseg = (int)nseg;
-        p.rows_per_wg = (int)rpw;
+        p.nseg = (int)((nc + L - 1) / L);
+        p.rows_per_wg = 64;
+        SD_HIP(d_rec.alloc((size_t)nr * p.nseg * sizeof(sd::SegRecord), stream));
+        p.records = d_rec.as<sd::SegRecord>();
+        if (utm) {
+            SD_HIP(d_anch.alloc((size_t)nr * 2 * (n_anch + 1) * sizeof(double), stream));
+            p.anchors = d_anch.as<double>();
+        }
     }
     p.row_seed = d_seed.as<int64_t>();
     p.row_flat_area = d_flat.as<double>();

@@ -38,37 +38,83 @@ SD_HD d3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
     double lat = lat_deg * kPi / 180.0;
     double lon = lon_deg * kPi / 180.0;
     double sl, cl, so, co;
+#ifdef __HIP_DEVICE_COMPILE__
+    sincos(lat, &sl, &cl);
+    sincos(lon, &so, &co);
+#else
     sl = sin(lat);
     cl = cos(lat);
     so = sin(lon);
     co = cos(lon);
+#endif
+    double n = kA / sqrt(1.0 - kE2 * sl * sl);
+    return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - kE2) + h) * sl};
+}
+
+SD_HD void sincos_deg(double deg, double* s, double* c) {
+    const double r = deg * kPi / 180.0;
+#ifdef __HIP_DEVICE_COMPILE__
+    sincos(r, s, c);
+#else
+    *s = sin(r);
+    *c = cos(r);
+#endif
+}
+
+// geodetic_to_ecef from the sines/cosines of its (degree) arguments: the
+// same operations after the trigonometry, so sharing one angle's sin/cos
+// between several calls changes no bit.
+SD_HD d3 ecef_from_sc(double sl, double cl, double so, double co, double h) {
     double n = kA / sqrt(1.0 - kE2 * sl * sl);
     return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - kE2) + h) * sl};
 }
 
 // Orbit in device-friendly form.  Times are µs relative to the first state
 // vector; tsec is the same in seconds ((t_us) / 1e6, orbit.rs:545-547).
-struct OrbitDev {
+struct alignas(16) OrbitDev {
     int32_t n;
     int32_t pad;
     int64_t t_us[kMaxSv];
     double tsec[kMaxSv];
     double pos[kMaxSv][3];
     double vel[kMaxSv][3];
+    double bary[kMaxSv][kLagrange];  // per window start
 };
 
 enum : int { ORB_OK = 0, ORB_OUTSIDE = 1, ORB_INSUFFICIENT = 2 };
 
-// interpolate_orbit_and_accel (orbit.rs:498-596): same window rule, same
-// 1 ns reciprocal guard, same L'_k = L_k (Σ_j 1/(t-t_j) - 1/(t-t_k)).  The
-// basis L_k is formed as Π_{j≠k}(t-t_j)/(t_k-t_j) in the reference's order.
+// Σ_k of the Lagrange basis (and its derivative) times the window's nodes.
+SD_HD void orbit_accumulate(const OrbitDev& o, int s0, const double* pre, const double* suf, const double* rd,
+                            double tot, d3* P, d3* V, d3* A) {
+#pragma clang fp contract(fast)
+    const double* w = o.bary[s0];
+    d3 p = {0, 0, 0}, v = {0, 0, 0}, a = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kLagrange; ++k) {
+        const double basis = w[k] * (pre[k] * suf[k + 1]);
+        const double db = basis * (tot - rd[k]);
+        const double* pp = o.pos[s0 + k];
+        const double* vv = o.vel[s0 + k];
+        p.x += basis * pp[0]; p.y += basis * pp[1]; p.z += basis * pp[2];
+        v.x += basis * vv[0]; v.y += basis * vv[1]; v.z += basis * vv[2];
+        a.x += db * vv[0]; a.y += db * vv[1]; a.z += db * vv[2];
+    }
+    *P = p; *V = v; *A = a;
+}
+
+// interpolate_orbit_and_accel (orbit.rs:498-596): the reference's window
+// rule, its refusal to extrapolate, its 1 ns reciprocal guard and its
+// derivative L'_k = L_k (Σ_j 1/(t-t_j) - 1/(t-t_k)).  The basis is formed in
+// barycentric form L_k = w_k Π_{j≠k}(t-t_j) with w_k = 1/Π_{j≠k}(t_k-t_j)
+// precomputed per window (OrbitDev::bary), and the reciprocals 1/(t-t_j)
+// from one division of the full product; this agrees with the reference's
+// 56-division product to a few ulps and costs one division instead of 64.
 SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
+#pragma clang fp contract(fast)
     const int n = o.n;
     if (n < kLagrange) return ORB_INSUFFICIENT;
     if (t < o.t_us[0] || t > o.t_us[n - 1]) return ORB_OUTSIDE;
     const double ts = (double)(t - o.t_us[0]) / 1e6;
-    // centre = largest i with t_i <= t: sorted nodes, short linear walk from
-    // an estimate (uniform spacing makes the estimate exact almost always).
     int c = (int)((double)(n - 1) * (double)(t - o.t_us[0]) / (double)(o.t_us[n - 1] - o.t_us[0]));
     c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
     while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
@@ -77,34 +123,51 @@ SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
     if (c < 4) start = 0;
     else if (c + 4 >= n) start = n - kLagrange;
     else start = c - 3;
-    double tj[kLagrange], rd[kLagrange];
-    double tot = 0.0;
+    double d[kLagrange], pre[kLagrange + 1], suf[kLagrange + 1];
+    bool guard = false;
 #pragma unroll
     for (int j = 0; j < kLagrange; ++j) {
-        tj[j] = o.tsec[start + j];
-        double diff = ts - tj[j];
-        rd[j] = 0.0;
-        if (fabs(diff) > 1e-9) {
-            rd[j] = 1.0 / diff;
+        d[j] = ts - o.tsec[start + j];
+        guard |= !(fabs(d[j]) > 1e-9);
+    }
+    pre[0] = 1.0;
+#pragma unroll
+    for (int j = 0; j < kLagrange; ++j) pre[j + 1] = pre[j] * d[j];
+    suf[kLagrange] = 1.0;
+#pragma unroll
+    for (int j = kLagrange - 1; j >= 0; --j) suf[j] = suf[j + 1] * d[j];
+    double rd[kLagrange];
+    double tot = 0.0;
+    if (!guard) {
+        const double inv_all = 1.0 / pre[kLagrange];
+#pragma unroll
+        for (int j = 0; j < kLagrange; ++j) {
+            rd[j] = inv_all * (pre[j] * suf[j + 1]);
             tot += rd[j];
         }
-    }
-    d3 p = {0, 0, 0}, v = {0, 0, 0}, a = {0, 0, 0};
+    } else {
 #pragma unroll
-    for (int k = 0; k < kLagrange; ++k) {
-        double basis = 1.0;
-#pragma unroll
-        for (int j = 0; j < kLagrange; ++j)
-            if (j != k) basis *= (ts - tj[j]) / (tj[k] - tj[j]);
-        double db = basis * (tot - rd[k]);
-        const double* pp = o.pos[start + k];
-        const double* vv = o.vel[start + k];
-        p.x += basis * pp[0]; p.y += basis * pp[1]; p.z += basis * pp[2];
-        v.x += basis * vv[0]; v.y += basis * vv[1]; v.z += basis * vv[2];
-        a.x += db * vv[0]; a.y += db * vv[1]; a.z += db * vv[2];
+        for (int j = 0; j < kLagrange; ++j) {
+            rd[j] = 0.0;
+            if (fabs(d[j]) > 1e-9) {
+                rd[j] = 1.0 / d[j];
+                tot += rd[j];
+            }
+        }
     }
-    *P = p; *V = v; *A = a;
+    orbit_accumulate(o, start, pre, suf, rd, tot, P, V, A);
     return ORB_OK;
+}
+
+// Barycentric weights w_k = 1/Π_{j≠k}(t_k-t_j) of every 8-point window.
+inline void orbit_fill_bary(OrbitDev& o) {
+    for (int s = 0; s + kLagrange <= o.n; ++s)
+        for (int k = 0; k < kLagrange; ++k) {
+            double prod = 1.0;
+            for (int j = 0; j < kLagrange; ++j)
+                if (j != k) prod *= o.tsec[s + k] - o.tsec[s + j];
+            o.bary[s][k] = 1.0 / prod;
+        }
 }
 
 enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT_OUTSIDE = 3, ZD_ORBIT_INSUFFICIENT = 4 };
@@ -113,6 +176,7 @@ enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT
 // before the step, steps are truncated to whole µs.
 SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter, double tol, int64_t* t_out,
                        int* iters, d3* sat) {
+#pragma clang fp contract(fast)
     int64_t t = seed;
     for (int i = 0; i < max_iter; ++i) {
         d3 P, V, A;
@@ -146,11 +210,20 @@ SD_HD double flat_cell_area(double lat, double lon, double s) {
 
 enum : int { TG_VALID = 0, TG_FORESHORTENING = 1, TG_LAYOVER = 2, TG_SHADOW = 3 };
 
+
+
 // compute_terrain_geometry_with_flat_area (terrain_correction.rs:593-651)
 SD_HD int terrain_geometry(double lat, double lon, double he, double hw, double hn, double hs, d3 sat, d3 target,
                            double s, double flat_area, double* w_out, double* cos_lia_out) {
-    d3 pe = geodetic_to_ecef(lat, lon + s, he), pw = geodetic_to_ecef(lat, lon - s, hw);
-    d3 pn = geodetic_to_ecef(lat + s, lon, hn), ps = geodetic_to_ecef(lat - s, lon, hs);
+    double sl, clat, so, co, soe, coe, sow, cow, sln, cln, sls, cls;
+    sincos_deg(lat, &sl, &clat);
+    sincos_deg(lon, &so, &co);
+    sincos_deg(lon + s, &soe, &coe);
+    sincos_deg(lon - s, &sow, &cow);
+    sincos_deg(lat + s, &sln, &cln);
+    sincos_deg(lat - s, &sls, &cls);
+    d3 pe = ecef_from_sc(sl, clat, soe, coe, he), pw = ecef_from_sc(sl, clat, sow, cow, hw);
+    d3 pn = ecef_from_sc(sln, cln, so, co, hn), ps = ecef_from_sc(sls, cls, so, co, hs);
     d3 tn = cross(pe - pw, pn - ps);
     d3 lk = sat - target;
     double ln = sqrt(dot(lk, lk));

@@ -12,17 +12,18 @@
 // row (terrain_correction.rs:1443,1580) and truncates every step to whole µs,
 // so a converged time is floor or ceil of the root depending on the side the
 // seed comes from.  Here every lane runs that chain sequentially over one
-// segment of L consecutive columns of a row (a workgroup holds one or more
-// whole rows), which costs the reference's ~2 orbit evaluations per pixel.
+// segment of L consecutive columns of a row, which costs the reference's ~2
+// orbit evaluations per pixel.  A wave holds 64 adjacent rows of the same
+// segment, so its DEM and radar gathers fall on neighbouring cache lines.
 // Only a segment's starting seed is unknown in parallel; it is taken from a
 // speculative solve of the previous segment's last pixel (phase A), which is
 // within 1 µs of that pixel's chained value and so approaches the next pixel
-// from the same side.  A verification loop (phase C) then compares each
-// segment's seed with the true chained value of the segments to its left;
-// where they differ it walks the old and the corrected chains in lockstep,
-// rewriting pixels until both converge to the same µs (after which the rest
-// of the segment is unchanged), and repeats until no seed differs.  The
-// output is therefore exactly the sequential chain of the reference.
+// from the same side.  Each segment records its seed and its last converged
+// time; tc_fixup_kernel then walks every row's segments in order, and where
+// a seed differs from the chained value to its left it replays the old and
+// the corrected chains in lockstep, rewriting pixels until both converge to
+// the same µs (after which the rest of the segment is unchanged).  The output
+// is therefore exactly the sequential chain of the reference.
 #include "geom.hpp"
 #include "tc_params.hpp"
 #include "tc_pixel.hpp"
@@ -224,192 +225,210 @@ __device__ __forceinline__ Solve solve(const TcParams& p, const OrbitDev& orb, c
     return s;
 }
 
-__global__ __launch_bounds__(256) void tc_chain_kernel(TcParams p) {
-    extern __shared__ double anchors[];  // per row: lon[npts] then lat[npts]
-    __shared__ long long sh_t[256];
-    __shared__ int sh_has[256];
-
-    const int tid = threadIdx.x;
-    const int lr = tid / p.nseg, seg = tid - lr * p.nseg;
-    const int64_t rr = (int64_t)blockIdx.x * p.rows_per_wg + lr;
-    const bool lane_ok = lr < p.rows_per_wg && rr < p.rows;
-    const int64_t row = p.row0 + rr;
-    const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
-    const OrbitDev& orb = *p.orbit;
-    unsigned long long* G = p.counters;
-    const int npts = p.n_anchor_pts;
-
-    // UTM: each row's n_anchors+1 exact inverse projections (output_crs.rs:364-393)
-    if (p.crs != 0) {
-        const TmSeries ts = tm_series();
-        for (int i = tid; i < p.rows_per_wg * npts; i += 256) {
-            const int r_ = i / npts, k = i - r_ * npts;
-            const int64_t rr2 = (int64_t)blockIdx.x * p.rows_per_wg + r_;
-            if (rr2 >= p.rows) continue;
-            const double y2 = p.lat_north - ((double)(p.row0 + rr2) + 0.5) * p.spacing;
-            double lo, la;
-            utm_inverse(ts, p.utm_zone, p.utm_south != 0, p.x_start + (double)k * p.anchor_dx, y2, &lo, &la);
-            anchors[r_ * 2 * npts + k] = lo;
-            anchors[r_ * 2 * npts + npts + k] = la;
-        }
-        __syncthreads();
-    }
-    const double* alon = anchors + lr * 2 * npts;
-    const double* alat = alon + npts;
-
-    const int64_t c_begin = lane_ok ? (int64_t)seg * p.seg_len : 0;
-    const int64_t c_end = lane_ok ? min((int64_t)(seg + 1) * p.seg_len, p.cols) : 0;
-    const long long row_seed = lane_ok ? p.row_seed[rr] : 0;
-    const double flat_area = (lane_ok && p.need_geom) ? p.row_flat_area[rr] : 0.0;
-    const int64_t obase = rr * p.cols;
-
-    // ── phase A: speculative time of this segment's last DEM-valid pixel ──
-    long long a_t = 0;
-    bool a_has = false;
-    if (lane_ok && seg + 1 < p.nseg) {
-        for (int64_t cc = c_end - 1; cc >= c_begin; --cc) {
-            PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
-            if (!q.dem_ok) continue;
-            Solve s = solve(p, orb, q, row_seed, 40);
-            if (s.oc == ZD_CONVERGED) {
-                a_t = s.t;
-                a_has = true;
-            }
-            break;
-        }
-    }
-    sh_t[tid] = a_t;
-    sh_has[tid] = a_has ? 1 : 0;
-    __syncthreads();
-    long long S = row_seed;
-    for (int q = seg - 1; q >= 0; --q)
-        if (sh_has[lr * p.nseg + q]) {
-            S = sh_t[lr * p.nseg + q];
-            break;
-        }
-    __syncthreads();
-
-    // ── phase B: the reference's sequential chain over the segment ──
-    LaneCounts C;
+__device__ __forceinline__ void zero_counts(LaneCounts& C) {
 #pragma unroll
     for (int i = 0; i < 7; ++i) C.c[i] = 0;
     C.h1 = C.h2 = C.h3 = 0;
     C.err_outside = C.err_short = 0;
-    long long last_t = 0;
+}
+
+// Wave-reduce each lane's counts and add them to the global counters.
+__device__ __forceinline__ void flush_counts(const TcParams& p, const LaneCounts& C, bool set_errors) {
+    unsigned long long* G = p.counters;
+    const int lane = threadIdx.x & 63;
+    int v[10] = {C.c[0], C.c[1], C.c[2], C.c[3], C.c[4], C.c[5], C.c[6], C.h1, C.h2, C.h3};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        long long x = v[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if (lane == 0 && x) atomicAdd(&G[i < 7 ? i : 7 + (i - 6)], (unsigned long long)x);
+    }
+    if (set_errors && C.err_outside > 0) atomicOr(p.error_flag, 1);
+    if (set_errors && C.err_short > 0) atomicOr(p.error_flag, 2);
+}
+
+__device__ __forceinline__ void load_orbit(const TcParams& p, OrbitDev& orb) {
+    const uint4* src = reinterpret_cast<const uint4*>(p.orbit);
+    uint4* dst = reinterpret_cast<uint4*>(&orb);
+    for (int i = threadIdx.x; i < (int)(sizeof(OrbitDev) / sizeof(uint4)); i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+// Project_row anchors of every AOI row (output_crs.rs:364-393):
+// anchors[row][0..npts) = lon, anchors[row][npts..2 npts) = lat.
+__global__ __launch_bounds__(256) void utm_anchor_kernel(TcParams p) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int npts = p.n_anchor_pts;
+    if (i >= p.rows * npts) return;
+    const int64_t rr = i / npts;
+    const int k = (int)(i - rr * npts);
+    const double y = p.lat_north - ((double)(p.row0 + rr) + 0.5) * p.spacing;
+    const TmSeries ts = tm_series();
+    double lo, la;
+    utm_inverse(ts, p.utm_zone, p.utm_south != 0, p.x_start + (double)k * p.anchor_dx, y, &lo, &la);
+    p.anchors[rr * 2 * npts + k] = lo;
+    p.anchors[rr * 2 * npts + npts + k] = la;
+}
+
+// Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
+// a workgroup take 4 consecutive segments.
+__global__ __launch_bounds__(256) void tc_segment_kernel(TcParams p) {
+    __shared__ OrbitDev orb;
+    load_orbit(p, orb);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t rr = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t seg = (int64_t)blockIdx.y * 4 + w;
+    const bool lane_ok = rr < p.rows && seg < p.nseg;
+    if (!lane_ok) return;  // no barriers below
+    const int64_t row = p.row0 + rr;
+    const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
+    const int npts = p.n_anchor_pts;
+    const double* alon = p.crs != 0 ? p.anchors + rr * 2 * npts : nullptr;
+    const double* alat = p.crs != 0 ? alon + npts : nullptr;
+    const int64_t c_begin = seg * p.seg_len;
+    const int64_t c_end = min(c_begin + p.seg_len, p.cols);
+    const long long row_seed = p.row_seed[rr];
+    const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
+    const int64_t obase = rr * p.cols;
+
+    // phase A: speculative time of the previous segment's last DEM-valid pixel
+    long long S = row_seed;
+    if (seg > 0) {
+        for (int64_t cc = c_begin - 1; cc >= c_begin - p.seg_len; --cc) {
+            PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+            if (!q.dem_ok) continue;
+            Solve s = solve(p, orb, q, row_seed, 40);
+            if (s.oc == ZD_CONVERGED) S = s.t;
+            break;
+        }
+    }
+
+    // phase B: the reference's sequential chain over the segment
+    LaneCounts C;
+    zero_counts(C);
+    long long seed = S, last_t = 0;
     bool last_has = false, any_newton = false;
-    {
-        long long seed = S;
-        for (int64_t cc = c_begin; cc < c_end; ++cc) {
-            PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
-            if (!q.dem_ok) {  // DEM_MISSING (terrain_correction.rs:1529-1536)
-                write_dem_missing(p, obase + cc);
-                C.c[1] += 1;
-                continue;
-            }
-            any_newton = true;
-            Solve s = solve(p, orb, q, seed, p.max_iter);
-            if (s.oc == ZD_CONVERGED) {
-                seed = s.t;
-                last_t = s.t;
-                last_has = true;
-            }
-            PixelOut v = finish_pixel(p, q, s.oc, s.t, s.sat, flat_area);
-            if (v.cat == CAT_ORBIT) {
-                if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
-                else C.err_short += 1;
-            } else {
-                count(C, v.cat, s.it, 1, G);
-            }
-            write_pixel(p, obase + cc, v);
+    for (int64_t cc = c_begin; cc < c_end; ++cc) {
+        PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+        if (!q.dem_ok) {  // DEM_MISSING (terrain_correction.rs:1529-1536)
+            write_dem_missing(p, obase + cc);
+            C.c[1] += 1;
+            continue;
         }
+        any_newton = true;
+        Solve s = solve(p, orb, q, seed, p.max_iter);
+        if (s.oc == ZD_CONVERGED) {
+            seed = s.t;
+            last_t = s.t;
+            last_has = true;
+        }
+        PixelOut v = finish_pixel(p, q, s.oc, s.t, s.sat, flat_area);
+        if (v.cat == CAT_ORBIT) {
+            if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
+            else C.err_short += 1;
+        } else {
+            count(C, v.cat, s.it, 1, p.counters);
+        }
+        write_pixel(p, obase + cc, v);
     }
+    SegRecord rec;
+    rec.seed = S;
+    rec.last = last_t;
+    rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0);
+    rec.err_outside = C.err_outside;
+    rec.err_short = C.err_short;
+    rec.pad = 0;
+    p.records[rr * p.nseg + seg] = rec;
+    flush_counts(p, C, false);  // orbit errors are decided after the fix-up
+}
 
-    // ── phase C: verify each segment's seed against the chain to its left ──
-    unsigned rounds = 0;
-    for (int round = 0; round < 512; ++round) {
-        sh_t[tid] = last_t;
-        sh_has[tid] = (lane_ok && last_has) ? 1 : 0;
-        __syncthreads();
-        long long E = row_seed;
-        for (int q = seg - 1; q >= 0; --q)
-            if (sh_has[lr * p.nseg + q]) {
-                E = sh_t[lr * p.nseg + q];
-                break;
-            }
-        __syncthreads();
-        const bool mism = lane_ok && any_newton && E != S;
-        if (!__syncthreads_or(mism)) break;
-        ++rounds;
-        if (!mism) continue;
-        // Walk the old (seed S) and corrected (seed E) chains in lockstep.
-        long long so = S, sn = E, new_last = 0;
-        bool new_has = false, synced = false;
-        for (int64_t cc = c_begin; cc < c_end; ++cc) {
-            PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
-            if (!q.dem_ok) continue;  // outcome independent of the seed
-            Solve o = solve(p, orb, q, so, p.max_iter);
-            Solve n = solve(p, orb, q, sn, p.max_iter);
-            if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
-                if (o.it != n.it) {  // same pixel result, different iteration count
-                    PixelOut v = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
-                    count(C, v.cat, o.it, -1, G);
-                    count(C, v.cat, n.it, 1, G);
+// One lane per row: walk the row's segments in order and correct every
+// segment whose starting seed differs from the chained value to its left.
+__global__ __launch_bounds__(256) void tc_fixup_kernel(TcParams p) {
+    __shared__ OrbitDev orb;
+    load_orbit(p, orb);
+    const int64_t rr = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (rr >= p.rows) return;
+    const int64_t row = p.row0 + rr;
+    const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
+    const int npts = p.n_anchor_pts;
+    const double* alon = p.crs != 0 ? p.anchors + rr * 2 * npts : nullptr;
+    const double* alat = p.crs != 0 ? alon + npts : nullptr;
+    const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
+    const int64_t obase = rr * p.cols;
+    SegRecord* recs = p.records + rr * p.nseg;
+    LaneCounts C;
+    zero_counts(C);
+    unsigned redone = 0;
+    long long E = p.row_seed[rr];
+    for (int64_t sg = 0; sg < p.nseg; ++sg) {
+        SegRecord rec = recs[sg];
+        C.err_outside += rec.err_outside;
+        C.err_short += rec.err_short;
+        if ((rec.flags & 2) && rec.seed != E) {
+            ++redone;
+            // replay the old (rec.seed) and corrected (E) chains in lockstep
+            const int64_t c_begin = sg * p.seg_len, c_end = min(c_begin + p.seg_len, p.cols);
+            long long so = rec.seed, sn = E, new_last = 0;
+            bool new_has = false, synced = false;
+            for (int64_t cc = c_begin; cc < c_end; ++cc) {
+                PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+                if (!q.dem_ok) continue;  // outcome independent of the seed
+                Solve o = solve(p, orb, q, so, p.max_iter);
+                Solve n = solve(p, orb, q, sn, p.max_iter);
+                if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
+                    if (o.it != n.it) {  // same result, different iteration count
+                        PixelOut v = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
+                        count(C, v.cat, o.it, -1, p.counters);
+                        count(C, v.cat, n.it, 1, p.counters);
+                    }
+                    synced = true;
+                    break;
                 }
-                synced = true;
-                break;
+                PixelOut vo = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
+                PixelOut vn = finish_pixel(p, q, n.oc, n.t, n.sat, flat_area);
+                if (vo.cat == CAT_ORBIT) {
+                    if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
+                    else C.err_short -= 1;
+                } else {
+                    count(C, vo.cat, o.it, -1, p.counters);
+                }
+                if (vn.cat == CAT_ORBIT) {
+                    if (n.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
+                    else C.err_short += 1;
+                } else {
+                    count(C, vn.cat, n.it, 1, p.counters);
+                }
+                write_pixel(p, obase + cc, vn);
+                if (o.oc == ZD_CONVERGED) so = o.t;
+                if (n.oc == ZD_CONVERGED) {
+                    sn = n.t;
+                    new_last = n.t;
+                    new_has = true;
+                }
             }
-            PixelOut vo = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
-            PixelOut vn = finish_pixel(p, q, n.oc, n.t, n.sat, flat_area);
-            if (vo.cat == CAT_ORBIT) {
-                if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
-                else C.err_short -= 1;
-            } else {
-                count(C, vo.cat, o.it, -1, G);
-            }
-            if (vn.cat == CAT_ORBIT) {
-                if (n.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
-                else C.err_short += 1;
-            } else {
-                count(C, vn.cat, n.it, 1, G);
-            }
-            write_pixel(p, obase + cc, vn);
-            if (o.oc == ZD_CONVERGED) so = o.t;
-            if (n.oc == ZD_CONVERGED) {
-                sn = n.t;
-                new_last = n.t;
-                new_has = true;
+            if (!synced) {
+                rec.last = new_last;
+                rec.flags = (rec.flags & ~1) | (new_has ? 1 : 0);
             }
         }
-        if (!synced) {
-            last_t = new_last;
-            last_has = new_has;
-        }
-        S = E;
+        if (rec.flags & 1) E = rec.last;
     }
-
-    // ── counters: wave-reduce each lane's counts, one atomic per wave ──
-    const int lane = tid & 63;
-    auto flush = [&](int idx, int x) {
-        long long v = x;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0 && v) atomicAdd(&G[idx], (unsigned long long)v);
-    };
-#pragma unroll
-    for (int i = 0; i < 7; ++i) flush(i, C.c[i]);
-    flush(7 + 1, C.h1);
-    flush(7 + 2, C.h2);
-    flush(7 + 3, C.h3);
-    if (tid == 0 && rounds) atomicAdd(&G[kCntRedo], (unsigned long long)rounds);
-    if (C.err_outside > 0) atomicOr(p.error_flag, 1);
-    if (C.err_short > 0) atomicOr(p.error_flag, 2);
+    // C holds the segment pass's orbit errors plus the replay's corrections
+    flush_counts(p, C, true);
+    if (redone) atomicAdd(&p.counters[kCntRedo], (unsigned long long)redone);
 }
 
 }  // namespace sd
 
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream) {
-    const unsigned blocks = (unsigned)((p->rows + p->rows_per_wg - 1) / p->rows_per_wg);
-    const size_t lds = p->crs != 0 ? (size_t)p->rows_per_wg * 2 * p->n_anchor_pts * sizeof(double) : 0;
-    hipLaunchKernelGGL(sd::tc_chain_kernel, dim3(blocks), dim3(256), lds, stream, *p);
+    if (p->crs != 0) {
+        const int64_t n = p->rows * p->n_anchor_pts;
+        hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
+    }
+    dim3 grid((unsigned)((p->rows + 63) / 64), (unsigned)((p->nseg + 3) / 4));
+    hipLaunchKernelGGL(sd::tc_segment_kernel, grid, dim3(256), 0, stream, *p);
+    hipLaunchKernelGGL(sd::tc_fixup_kernel, dim3((unsigned)((p->rows + 255) / 256)), dim3(256), 0, stream, *p);
     return hipGetLastError();
 }

@@ -13,6 +13,17 @@ constexpr int kNumCounters = kCntRedo + 1;
 constexpr int kMaxAnchorPts = 1025;          // SARDINE_TC_PROJECTOR_ANCHORS ≤ 1024
 constexpr size_t kMaxTcLds = 64 * 1024;      // dynamic LDS budget of the UTM anchors
 
+// Chain bookkeeping of one (row, segment): the seed the segment started
+// from, its last converged time, and its orbit-error counts.
+struct SegRecord {
+    int64_t seed;
+    int64_t last;
+    int32_t flags;      // bit 0: last is valid; bit 1: the segment ran a Newton solve
+    int32_t err_outside;
+    int32_t err_short;
+    int32_t pad;
+};
+
 struct TcParams {
     const OrbitDev* orbit;
     // radar image (MergedSigma0 buffers)
@@ -33,6 +44,8 @@ struct TcParams {
     int64_t full_cols;
     int64_t row0, col0, rows, cols;
     int seg_len, nseg, rows_per_wg;  // work decomposition (see tc.hip)
+    double* anchors;                  // UTM: per AOI row lon[npts], lat[npts]
+    SegRecord* records;               // per (AOI row, segment)
     const int64_t* row_seed;      // per AOI row, µs relative to the first state vector
     const double* row_flat_area;  // per AOI row (need_geom)
     // config
