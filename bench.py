@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--tc-segment", type=int, default=0, help="columns per lane segment (0 = automatic)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (reference kernel for HBM byte counters)")
     return ap.parse_args()
@@ -86,6 +87,7 @@ def main():
 
     dev = torch.device("cuda:0")
     L = _native.lib()
+    L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
         a.fill_(1.0)

@@ -393,6 +393,7 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.dem_olat = dem->origin_lat_deg;
     p.dem_olon = dem->origin_lon_deg;
     p.dem_pix = dem->pixel_deg;
+    p.dem_inv_pix = 1.0 / dem->pixel_deg;
     p.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? d_geoid.as<float>() : nullptr;
     p.crs = utm ? 1 : 0;
     p.utm_zone = cfg->utm_zone;
@@ -414,7 +415,7 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
         int64_t L = g_tc_seg.load();
         if (L <= 0) {  // aim for ~2 waves of 64 lanes per SIMD over the whole grid
             const int64_t target_lanes = (int64_t)256 * 4 * 2 * 64;
-            L = std::min<int64_t>(128, std::max<int64_t>(16, (nr * nc + target_lanes - 1) / target_lanes));
+            L = std::min<int64_t>(32, std::max<int64_t>(16, (nr * nc + target_lanes - 1) / target_lanes));
         }
         L = std::min<int64_t>(L, nc);
         p.seg_len = (int)L;
@@ -436,6 +437,8 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.need_geom = need_geom ? 1 : 0;
     p.resampling = cfg->resampling;
     p.step_deg = step_deg;
+    p.step_sin = std::sin(step_deg * sd::kPi / 180.0);
+    p.step_cos = std::cos(step_deg * sd::kPi / 180.0);
     // 10f32.powf(margin / 10.0) (terrain_correction.rs:1727), evaluated once
     p.nesz_factor = std::pow(10.0f, cfg->noise_floor_margin_db / 10.0f);
     p.out = out->data;

@@ -212,17 +212,22 @@ enum : int { TG_VALID = 0, TG_FORESHORTENING = 1, TG_LAYOVER = 2, TG_SHADOW = 3 
 
 
 
-// compute_terrain_geometry_with_flat_area (terrain_correction.rs:593-651)
-SD_HD int terrain_geometry(double lat, double lon, double he, double hw, double hn, double hs, d3 sat, d3 target,
-                           double s, double flat_area, double* w_out, double* cos_lia_out) {
-    double sl, clat, so, co, soe, coe, sow, cow, sln, cln, sls, cls;
-    sincos_deg(lat, &sl, &clat);
-    sincos_deg(lon, &so, &co);
-    sincos_deg(lon + s, &soe, &coe);
-    sincos_deg(lon - s, &sow, &cow);
-    sincos_deg(lat + s, &sln, &cln);
-    sincos_deg(lat - s, &sls, &cls);
-    d3 pe = ecef_from_sc(sl, clat, soe, coe, he), pw = ecef_from_sc(sl, clat, sow, cow, hw);
+// compute_terrain_geometry_with_flat_area (terrain_correction.rs:593-651).
+// The five points share trigonometry: (sl, cl, so, co) are the sines/cosines
+// of the centre latitude/longitude and (ss, cs) those of the cardinal step
+// s (all in radians of the degree arguments); the neighbours' sines/cosines
+// follow by angle addition, sin(a±b) = sin a cos b ± cos a sin b.  That
+// moves each neighbour by ~1e-10 m against evaluating sin((lon+s)π/180)
+// directly, far below anything the facet normal or the 0.05 / 2.0 mask
+// thresholds resolve.
+SD_HD int terrain_geometry_sc(double sl, double cl, double so, double co, double ss, double cs, double he,
+                              double hw, double hn, double hs, d3 sat, d3 target, double flat_area, double* w_out,
+                              double* cos_lia_out) {
+    const double soe = so * cs + co * ss, coe = co * cs - so * ss;  // lon + s
+    const double sow = so * cs - co * ss, cow = co * cs + so * ss;  // lon - s
+    const double sln = sl * cs + cl * ss, cln = cl * cs - sl * ss;  // lat + s
+    const double sls = sl * cs - cl * ss, cls = cl * cs + sl * ss;  // lat - s
+    d3 pe = ecef_from_sc(sl, cl, soe, coe, he), pw = ecef_from_sc(sl, cl, sow, cow, hw);
     d3 pn = ecef_from_sc(sln, cln, so, co, hn), ps = ecef_from_sc(sls, cls, so, co, hs);
     d3 tn = cross(pe - pw, pn - ps);
     d3 lk = sat - target;
@@ -232,11 +237,11 @@ SD_HD int terrain_geometry(double lat, double lon, double he, double hw, double 
     if (lp <= 0.0) return TG_SHADOW;
     double ta = sqrt(dot(tn, tn));
     if (ta == 0.0) return TG_SHADOW;
-    double cl = lp / ta;
-    cl = cl < 0.0 ? 0.0 : (cl > 1.0 ? 1.0 : cl);
+    double cosl = lp / ta;
+    cosl = cosl < 0.0 ? 0.0 : (cosl > 1.0 ? 1.0 : cosl);
     double w = lp / flat_area;
     *w_out = w;
-    *cos_lia_out = cl;
+    *cos_lia_out = cosl;
     if (w < 0.05) return TG_FORESHORTENING;
     if (w > 2.0) return TG_LAYOVER;
     return TG_VALID;

@@ -61,6 +61,7 @@ __device__ __forceinline__ void count(LaneCounts& L, int cat, int it, int sign, 
 
 struct PixelPos {
     double lat, lon;
+    double sl, cl, so, co;  // sines/cosines of lat/lon (radians of the degrees)
     d3 target;
     bool dem_ok;
 };
@@ -87,7 +88,12 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
     float hf = 0.0f;
     q.dem_ok = dem_sample(p, q.lat, q.lon, &hf) && isfinite(hf);
     q.target = {0.0, 0.0, 0.0};
-    if (q.dem_ok) q.target = geodetic_to_ecef(q.lat, q.lon, (double)hf + geoid_n(p, q.lat, q.lon));
+    q.sl = q.cl = q.so = q.co = 0.0;
+    if (q.dem_ok) {
+        sincos_deg(q.lat, &q.sl, &q.cl);
+        sincos_deg(q.lon, &q.so, &q.co);
+        q.target = ecef_from_sc(q.sl, q.cl, q.so, q.co, (double)hf + geoid_n(p, q.lat, q.lon));
+    }
     return q;
 }
 
@@ -150,7 +156,8 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
             }
         } else {
             double w = 0.0, cl = 0.0;
-            int g = terrain_geometry(q.lat, q.lon, he, hw, hn, hs, sat, q.target, st, flat_area, &w, &cl);
+            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, p.step_sin, p.step_cos, he, hw, hn, hs, sat, q.target,
+                                        flat_area, &w, &cl);
             if (g == TG_VALID) {
                 o.cos_lia = (float)cl;
                 if (p.flatten) sigma0 /= (float)w;
@@ -271,16 +278,79 @@ __global__ __launch_bounds__(256) void utm_anchor_kernel(TcParams p) {
     p.anchors[rr * 2 * npts + npts + k] = la;
 }
 
+// Output staging.  Lane = row means a wave's per-pixel stores land on 64
+// different rows; stored one pixel at a time they leave 64 partly written
+// cache lines per array in L2 and each is written back many times.  The
+// segment kernel instead stages each wave's pixels in LDS and writes them out
+// every kStageCols columns (f32 arrays) or kMaskCols columns (the byte mask)
+// as contiguous row pieces.
+constexpr int kStageCols = 8;
+constexpr int kMaskCols = 32;
+struct WaveStage {
+    float d[64][kStageCols + 1];  // +1 word: row-to-row bank skew
+    float c[64][kStageCols + 1];
+    unsigned char m[64][kMaskCols + 4];
+};
+
+// Orders the LDS accesses of one wave's lanes (lanes of a wave execute in
+// lockstep; this keeps the compiler from moving accesses across it).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes the n staged f32 columns starting at output column cs of rows
+// rr0..rr0+63 (rows past p.rows are skipped).
+__device__ __forceinline__ void flush_f32(const TcParams& p, const WaveStage& st, int64_t rr0, int64_t cs, int n) {
+    const int lane = threadIdx.x & 63;
+    wave_sync();
+#pragma unroll 1
+    for (int i = 0; i < kStageCols; ++i) {
+        const int e = lane + 64 * i, r = e / kStageCols, c = e % kStageCols;
+        if (c < n && rr0 + r < p.rows) {
+            const int64_t o = (rr0 + r) * p.cols + cs + c;
+            p.out[o] = st.d[r][c];
+            if (p.compute_lia) p.cos_lia[o] = st.c[r][c];
+        }
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& st, int64_t rr0, int64_t cs, int n) {
+    const int lane = threadIdx.x & 63;
+    wave_sync();
+#pragma unroll 1
+    for (int i = 0; i < kMaskCols; ++i) {
+        const int e = lane + 64 * i, r = e / kMaskCols, c = e % kMaskCols;
+        if (c < n && rr0 + r < p.rows) p.mask[(rr0 + r) * p.cols + cs + c] = st.m[r][c];
+    }
+    wave_sync();
+}
+
+// Occupancy hint for the segment kernel (waves per SIMD the register
+// allocator aims for); a build option so that it can be tuned per compiler.
+#ifdef SD_TC_SEGMENT_WAVES_PER_EU
+#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
+#else
+#define SD_SEGMENT_OCCUPANCY
+#endif
+
 // Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
-// a workgroup take 4 consecutive segments.
-__global__ __launch_bounds__(256) void tc_segment_kernel(TcParams p) {
+// a workgroup take 4 consecutive segments.  Lanes past the last row compute
+// the last row again and discard it, so every lane reaches the wave-wide
+// flushes.
+__global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(TcParams p) {
     __shared__ OrbitDev orb;
+    __shared__ WaveStage stage[4];
     load_orbit(p, orb);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t rr = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t rr0 = (int64_t)blockIdx.x * 64;
     const int64_t seg = (int64_t)blockIdx.y * 4 + w;
-    const bool lane_ok = rr < p.rows && seg < p.nseg;
-    if (!lane_ok) return;  // no barriers below
+    if (seg >= p.nseg) return;  // whole wave; no block barriers below
+    WaveStage& st = stage[w];
+    const bool row_ok = rr0 + lane < p.rows;
+    const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;
     const int64_t row = p.row0 + rr;
     const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
     const int npts = p.n_anchor_pts;
@@ -310,36 +380,68 @@ __global__ __launch_bounds__(256) void tc_segment_kernel(TcParams p) {
     long long seed = S, last_t = 0;
     bool last_has = false, any_newton = false;
     for (int64_t cc = c_begin; cc < c_end; ++cc) {
+        const int j = (int)(cc - c_begin);
         PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+        float v_s0, v_cos;
+        unsigned char v_mask;
         if (!q.dem_ok) {  // DEM_MISSING (terrain_correction.rs:1529-1536)
-            write_dem_missing(p, obase + cc);
-            C.c[1] += 1;
-            continue;
-        }
-        any_newton = true;
-        Solve s = solve(p, orb, q, seed, p.max_iter);
-        if (s.oc == ZD_CONVERGED) {
-            seed = s.t;
-            last_t = s.t;
-            last_has = true;
-        }
-        PixelOut v = finish_pixel(p, q, s.oc, s.t, s.sat, flat_area);
-        if (v.cat == CAT_ORBIT) {
-            if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
-            else C.err_short += 1;
+            v_s0 = v_cos = __builtin_nanf("");
+            v_mask = 2;
+            if (row_ok) {
+                C.c[1] += 1;
+                if (p.radar_coords) {
+                    p.radar_coords[2 * (obase + cc)] = __builtin_nan("");
+                    p.radar_coords[2 * (obase + cc) + 1] = __builtin_nan("");
+                }
+            }
         } else {
-            count(C, v.cat, s.it, 1, p.counters);
+            any_newton = true;
+            Solve s = solve(p, orb, q, seed, p.max_iter);
+            if (s.oc == ZD_CONVERGED) {
+                seed = s.t;
+                last_t = s.t;
+                last_has = true;
+            }
+            PixelOut v = finish_pixel(p, q, s.oc, s.t, s.sat, flat_area);
+            if (row_ok) {
+                if (v.cat == CAT_ORBIT) {
+                    if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
+                    else C.err_short += 1;
+                } else {
+                    count(C, v.cat, s.it, 1, p.counters);
+                }
+                if (p.radar_coords) {
+                    p.radar_coords[2 * (obase + cc)] = v.line;
+                    p.radar_coords[2 * (obase + cc) + 1] = v.sample;
+                }
+            }
+            v_s0 = v.sigma0;
+            v_cos = v.cos_lia;
+            v_mask = v.mask;
         }
-        write_pixel(p, obase + cc, v);
+        st.d[lane][j % kStageCols] = v_s0;
+        st.c[lane][j % kStageCols] = v_cos;
+        st.m[lane][j % kMaskCols] = v_mask;
+        const bool last = cc + 1 == c_end;
+        if ((j + 1) % kStageCols == 0 || last) {
+            const int n = j % kStageCols + 1;
+            flush_f32(p, st, rr0, cc + 1 - n, n);
+        }
+        if (p.compute_lia && ((j + 1) % kMaskCols == 0 || last)) {
+            const int n = j % kMaskCols + 1;
+            flush_mask(p, st, rr0, cc + 1 - n, n);
+        }
     }
-    SegRecord rec;
-    rec.seed = S;
-    rec.last = last_t;
-    rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0);
-    rec.err_outside = C.err_outside;
-    rec.err_short = C.err_short;
-    rec.pad = 0;
-    p.records[rr * p.nseg + seg] = rec;
+    if (row_ok) {
+        SegRecord rec;
+        rec.seed = S;
+        rec.last = last_t;
+        rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0);
+        rec.err_outside = C.err_outside;
+        rec.err_short = C.err_short;
+        rec.pad = 0;
+        p.records[rr * p.nseg + seg] = rec;
+    }
     flush_counts(p, C, false);  // orbit errors are decided after the fix-up
 }
 

@@ -35,7 +35,7 @@ struct TcParams {
     // DEM raster
     const float* dem;
     int64_t dem_w, dem_h;
-    double dem_olat, dem_olon, dem_pix;
+    double dem_olat, dem_olon, dem_pix, dem_inv_pix;
     const float* geoid;     // device 73x144 or NULL
     // output grid
     int crs, utm_zone, utm_south, n_anchor_pts;
@@ -53,6 +53,7 @@ struct TcParams {
     double tol;
     int flatten, compute_lia, need_geom, resampling;
     double step_deg;
+    double step_sin, step_cos;  // sin/cos of step_deg in radians
     float nesz_factor;
     // outputs
     float* out;

@@ -12,8 +12,8 @@ namespace sd {
 __device__ __forceinline__ bool dem_sample(const TcParams& p, double lat, double lon, float* h) {
     // dem.rs:694-723 lon wrap, GLO-30 tile rule dem.rs:351-372, bilinear_on_grid dem.rs:385-409
     double lon_n = lon >= 180.0 ? lon - 360.0 : lon;
-    double row_f = (p.dem_olat - lat) / p.dem_pix;
-    double col_f = (lon_n - p.dem_olon) / p.dem_pix;
+    double row_f = (p.dem_olat - lat) * p.dem_inv_pix;
+    double col_f = (lon_n - p.dem_olon) * p.dem_inv_pix;
     double max_row = (double)(p.dem_h - 1), max_col = (double)(p.dem_w - 1);
     if (row_f < -0.5 || row_f > max_row + 0.5 || col_f < -0.5 || col_f > max_col + 0.5) return false;
     row_f = fmin(fmax(row_f, 0.0), max_row);
