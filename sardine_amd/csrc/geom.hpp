@@ -172,17 +172,47 @@ inline void orbit_fill_bary(OrbitDev& o) {
 
 enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT_OUTSIDE = 3, ZD_ORBIT_INSUFFICIENT = 4 };
 
+// The last orbit evaluation of a chain.  A pixel's Newton solve starts at
+// the previous pixel's converged time, which is exactly the time of that
+// pixel's last evaluation, so the first iteration of most pixels finds its
+// (P, V, A) here.  orbit_interp is a pure function of t: a hit returns the
+// same bits as a fresh evaluation.
+struct OrbitCache {
+    int64_t t;
+    d3 P, V, A;
+    bool valid;
+};
+
+SD_HD OrbitCache orbit_cache_empty() {
+    OrbitCache c;
+    c.t = 0;
+    c.P = c.V = c.A = d3{0.0, 0.0, 0.0};
+    c.valid = false;
+    return c;
+}
+
 // solve_zero_doppler (terrain_correction.rs:1010-1073): convergence is tested
 // before the step, steps are truncated to whole µs.
 SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter, double tol, int64_t* t_out,
-                       int* iters, d3* sat) {
+                       int* iters, d3* sat, OrbitCache* cache) {
 #pragma clang fp contract(fast)
     int64_t t = seed;
     for (int i = 0; i < max_iter; ++i) {
         d3 P, V, A;
-        int e = orbit_interp(o, t, &P, &V, &A);
-        if (e == ORB_OUTSIDE) return ZD_ORBIT_OUTSIDE;
-        if (e == ORB_INSUFFICIENT) return ZD_ORBIT_INSUFFICIENT;
+        if (cache->valid && cache->t == t) {
+            P = cache->P;
+            V = cache->V;
+            A = cache->A;
+        } else {
+            int e = orbit_interp(o, t, &P, &V, &A);
+            if (e == ORB_OUTSIDE) return ZD_ORBIT_OUTSIDE;
+            if (e == ORB_INSUFFICIENT) return ZD_ORBIT_INSUFFICIENT;
+            cache->t = t;
+            cache->P = P;
+            cache->V = V;
+            cache->A = A;
+            cache->valid = true;
+        }
         d3 r = target - P;
         double f = dot(r, V);
         double vsq = dot(V, V);

@@ -223,12 +223,12 @@ struct Solve {
 };
 
 __device__ __forceinline__ Solve solve(const TcParams& p, const OrbitDev& orb, const PixelPos& q, int64_t seed,
-                                       int max_iter) {
+                                       int max_iter, OrbitCache& cache) {
     Solve s;
     s.t = 0;
     s.it = 0;
     s.sat = {0.0, 0.0, 0.0};
-    s.oc = zero_doppler(orb, q.target, seed, max_iter, p.tol, &s.t, &s.it, &s.sat);
+    s.oc = zero_doppler(orb, q.target, seed, max_iter, p.tol, &s.t, &s.it, &s.sat, &cache);
     return s;
 }
 
@@ -364,11 +364,12 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
 
     // phase A: speculative time of the previous segment's last DEM-valid pixel
     long long S = row_seed;
+    OrbitCache cache = orbit_cache_empty();
     if (seg > 0) {
         for (int64_t cc = c_begin - 1; cc >= c_begin - p.seg_len; --cc) {
             PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
             if (!q.dem_ok) continue;
-            Solve s = solve(p, orb, q, row_seed, 40);
+            Solve s = solve(p, orb, q, row_seed, 40, cache);
             if (s.oc == ZD_CONVERGED) S = s.t;
             break;
         }
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
             }
         } else {
             any_newton = true;
-            Solve s = solve(p, orb, q, seed, p.max_iter);
+            Solve s = solve(p, orb, q, seed, p.max_iter, cache);
             if (s.oc == ZD_CONVERGED) {
                 seed = s.t;
                 last_t = s.t;
@@ -473,12 +474,13 @@ __global__ __launch_bounds__(256) void tc_fixup_kernel(TcParams p) {
             // replay the old (rec.seed) and corrected (E) chains in lockstep
             const int64_t c_begin = sg * p.seg_len, c_end = min(c_begin + p.seg_len, p.cols);
             long long so = rec.seed, sn = E, new_last = 0;
+            OrbitCache co = orbit_cache_empty(), cn = orbit_cache_empty();
             bool new_has = false, synced = false;
             for (int64_t cc = c_begin; cc < c_end; ++cc) {
                 PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
                 if (!q.dem_ok) continue;  // outcome independent of the seed
-                Solve o = solve(p, orb, q, so, p.max_iter);
-                Solve n = solve(p, orb, q, sn, p.max_iter);
+                Solve o = solve(p, orb, q, so, p.max_iter, co);
+                Solve n = solve(p, orb, q, sn, p.max_iter, cn);
                 if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
                     if (o.it != n.it) {  // same result, different iteration count
                         PixelOut v = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
