@@ -387,6 +387,8 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.near_range_m = merged->near_slant_range_time_s * sd::kC / 2.0;
     p.rps = merged->range_pixel_spacing_m;
     p.ati = ati;
+    p.line_per_us = 1.0 / (1e6 * ati);
+    p.inv_rps = 1.0 / p.rps;
     p.dem = dem->data;
     p.dem_w = dem->width;
     p.dem_h = dem->height;
@@ -394,6 +396,10 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.dem_olon = dem->origin_lon_deg;
     p.dem_pix = dem->pixel_deg;
     p.dem_inv_pix = 1.0 / dem->pixel_deg;
+    p.dem_max_row = (double)(dem->height - 1);
+    p.dem_max_col = (double)(dem->width - 1);
+    p.dem_lim_row = dem->height - 1 > 0 ? dem->height - 2 : 0;
+    p.dem_lim_col = dem->width >= 2 ? dem->width - 2 : 0;
     p.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? d_geoid.as<float>() : nullptr;
     p.crs = utm ? 1 : 0;
     p.utm_zone = cfg->utm_zone;
@@ -404,6 +410,7 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     p.lon_west = g.x_left;
     p.x_start = x_start;
     p.anchor_dx = anchor_dx;
+    p.inv_anchor_dx = anchor_dx > 0.0 ? 1.0 / anchor_dx : 0.0;
     p.full_cols = g.cols;
     p.row0 = r0;
     p.col0 = c0;

@@ -64,9 +64,14 @@ SD_HD void sincos_deg(double deg, double* s, double* c) {
 // geodetic_to_ecef from the sines/cosines of its (degree) arguments: the
 // same operations after the trigonometry, so sharing one angle's sin/cos
 // between several calls changes no bit.
-SD_HD d3 ecef_from_sc(double sl, double cl, double so, double co, double h) {
-    double n = kA / sqrt(1.0 - kE2 * sl * sl);
+SD_HD double prime_vertical(double sl) { return kA / sqrt(1.0 - kE2 * sl * sl); }
+
+SD_HD d3 ecef_from_n(double n, double sl, double cl, double so, double co, double h) {
     return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - kE2) + h) * sl};
+}
+
+SD_HD d3 ecef_from_sc(double sl, double cl, double so, double co, double h) {
+    return ecef_from_n(prime_vertical(sl), sl, cl, so, co, h);
 }
 
 // Orbit in device-friendly form.  Times are µs relative to the first state
@@ -249,20 +254,23 @@ enum : int { TG_VALID = 0, TG_FORESHORTENING = 1, TG_LAYOVER = 2, TG_SHADOW = 3 
 // follow by angle addition, sin(a±b) = sin a cos b ± cos a sin b.  That
 // moves each neighbour by ~1e-10 m against evaluating sin((lon+s)π/180)
 // directly, far below anything the facet normal or the 0.05 / 2.0 mask
-// thresholds resolve.
-SD_HD int terrain_geometry_sc(double sl, double cl, double so, double co, double ss, double cs, double he,
-                              double hw, double hn, double hs, d3 sat, d3 target, double flat_area, double* w_out,
-                              double* cos_lia_out) {
+// thresholds resolve.  The east/west points have the centre's latitude and
+// so its prime-vertical radius n_c.  The look vector is normalised with one
+// reciprocal (a few ulp from three divisions).
+SD_HD int terrain_geometry_sc(double sl, double cl, double so, double co, double n_c, double ss, double cs,
+                              double he, double hw, double hn, double hs, d3 sat, d3 target, double flat_area,
+                              double* w_out, double* cos_lia_out) {
     const double soe = so * cs + co * ss, coe = co * cs - so * ss;  // lon + s
     const double sow = so * cs - co * ss, cow = co * cs + so * ss;  // lon - s
     const double sln = sl * cs + cl * ss, cln = cl * cs - sl * ss;  // lat + s
     const double sls = sl * cs - cl * ss, cls = cl * cs + sl * ss;  // lat - s
-    d3 pe = ecef_from_sc(sl, cl, soe, coe, he), pw = ecef_from_sc(sl, cl, sow, cow, hw);
+    d3 pe = ecef_from_n(n_c, sl, cl, soe, coe, he), pw = ecef_from_n(n_c, sl, cl, sow, cow, hw);
     d3 pn = ecef_from_sc(sln, cln, so, co, hn), ps = ecef_from_sc(sls, cls, so, co, hs);
     d3 tn = cross(pe - pw, pn - ps);
     d3 lk = sat - target;
     double ln = sqrt(dot(lk, lk));
-    d3 look = ln == 0.0 ? d3{0, 0, 0} : d3{lk.x / ln, lk.y / ln, lk.z / ln};
+    const double il = 1.0 / ln;
+    d3 look = ln == 0.0 ? d3{0, 0, 0} : d3{lk.x * il, lk.y * il, lk.z * il};
     double lp = dot(tn, look);
     if (lp <= 0.0) return TG_SHADOW;
     double ta = sqrt(dot(tn, tn));

@@ -62,21 +62,23 @@ __device__ __forceinline__ void count(LaneCounts& L, int cat, int it, int sign, 
 struct PixelPos {
     double lat, lon;
     double sl, cl, so, co;  // sines/cosines of lat/lon (radians of the degrees)
+    double n;               // prime-vertical radius at lat
     d3 target;
     bool dem_ok;
 };
 
 // Output-grid position of a column and its DEM/ECEF target.
+template <bool UTM>
 __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, double row_y, const double* alon,
                                                const double* alat) {
     PixelPos q;
-    if (p.crs == 0) {
+    if constexpr (!UTM) {
         q.lon = p.lon_west + ((double)col + 0.5) * p.spacing;
         q.lat = row_y;
     } else {  // Projector::project_row blend (output_crs.rs:395-414)
         const double x = p.x_start + (double)col * p.spacing;
         const int na = p.n_anchor_pts - 1;
-        int seg = (int)floor((x - p.x_start) / p.anchor_dx);
+        int seg = (int)floor((x - p.x_start) * p.inv_anchor_dx);  // a guess: the walk below decides
         seg = seg < 0 ? 0 : (seg > na - 1 ? na - 1 : seg);
         while (seg > 0 && !(x > p.x_start + (double)seg * p.anchor_dx)) --seg;
         while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
@@ -88,11 +90,12 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
     float hf = 0.0f;
     q.dem_ok = dem_sample(p, q.lat, q.lon, &hf) && isfinite(hf);
     q.target = {0.0, 0.0, 0.0};
-    q.sl = q.cl = q.so = q.co = 0.0;
+    q.sl = q.cl = q.so = q.co = q.n = 0.0;
     if (q.dem_ok) {
         sincos_deg(q.lat, &q.sl, &q.cl);
         sincos_deg(q.lon, &q.so, &q.co);
-        q.target = ecef_from_sc(q.sl, q.cl, q.so, q.co, (double)hf + geoid_n(p, q.lat, q.lon));
+        q.n = prime_vertical(q.sl);
+        q.target = ecef_from_n(q.n, q.sl, q.cl, q.so, q.co, (double)hf + geoid_n(p, q.lat, q.lon));
     }
     return q;
 }
@@ -105,6 +108,7 @@ struct PixelOut {
 };
 
 // Everything after the Newton solve (terrain_correction.rs:1558-1739).
+template <int RS>
 __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelPos& q, int oc, int64_t t, d3 sat,
                                                  double flat_area) {
     PixelOut o;
@@ -129,24 +133,27 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
     }
     d3 rv = q.target - sat;
     const double slant = sqrt(dot(rv, rv));
-    const double line = (double)(t - p.az_start_us) / 1e6 / p.ati;
-    const double sample = (slant - p.near_range_m) / p.rps;
+    const double line = (double)(t - p.az_start_us) * p.line_per_us;
+    const double sample = (slant - p.near_range_m) * p.inv_rps;
     o.line = line;
     o.sample = sample;
     if (line < -0.5 || line > (double)p.lines - 0.5 || sample < -0.5 || sample > (double)p.samples - 0.5) {
         o.cat = CAT_OUT;
         return o;
     }
-    float sigma0 = resample(p, line, sample);
+    float sigma0 = resample<RS>(p, line, sample);
     if (isnan(sigma0)) {
         o.cat = CAT_OUT;
         return o;
     }
     if (p.need_geom) {
         const double st = p.step_deg;
-        float he, hw, hn, hs;
-        bool ok = dem_sample(p, q.lat, q.lon + st, &he) && dem_sample(p, q.lat, q.lon - st, &hw) &&
-                  dem_sample(p, q.lat + st, q.lon, &hn) && dem_sample(p, q.lat - st, q.lon, &hs);
+        float he = 0.0f, hw = 0.0f, hn = 0.0f, hs = 0.0f;
+        const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
+        bool ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon + st), &he);
+        ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon - st), &hw) && ok;
+        ok = dem_bilinear(p, dem_row_axis(p, q.lat + st), cc, &hn) && ok;
+        ok = dem_bilinear(p, dem_row_axis(p, q.lat - st), cc, &hs) && ok;
         if (ok) ok = isfinite(he) && isfinite(hw) && isfinite(hn) && isfinite(hs);
         if (!ok) {
             o.mask = 9;
@@ -156,7 +163,7 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
             }
         } else {
             double w = 0.0, cl = 0.0;
-            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, p.step_sin, p.step_cos, he, hw, hn, hs, sat, q.target,
+            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, he, hw, hn, hs, sat, q.target,
                                         flat_area, &w, &cl);
             if (g == TG_VALID) {
                 o.cos_lia = (float)cl;
@@ -328,18 +335,18 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
     wave_sync();
 }
 
-// Occupancy hint for the segment kernel (waves per SIMD the register
-// allocator aims for); a build option so that it can be tuned per compiler.
-#ifdef SD_TC_SEGMENT_WAVES_PER_EU
-#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
-#else
-#define SD_SEGMENT_OCCUPANCY
+// Occupancy floor of the chain kernels (waves per SIMD the register
+// allocator must allow); a build option so that it can be tuned per compiler.
+#ifndef SD_TC_SEGMENT_WAVES_PER_EU
+#define SD_TC_SEGMENT_WAVES_PER_EU 2
 #endif
+#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
 
 // Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
 // a workgroup take 4 consecutive segments.  Lanes past the last row compute
 // the last row again and discard it, so every lane reaches the wave-wide
 // flushes.
+template <int RS, bool UTM>
 __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(TcParams p) {
     __shared__ OrbitDev orb;
     __shared__ WaveStage stage[4];
@@ -354,8 +361,8 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
     const int64_t row = p.row0 + rr;
     const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
     const int npts = p.n_anchor_pts;
-    const double* alon = p.crs != 0 ? p.anchors + rr * 2 * npts : nullptr;
-    const double* alat = p.crs != 0 ? alon + npts : nullptr;
+    const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
+    const double* alat = UTM ? alon + npts : nullptr;
     const int64_t c_begin = seg * p.seg_len;
     const int64_t c_end = min(c_begin + p.seg_len, p.cols);
     const long long row_seed = p.row_seed[rr];
@@ -367,7 +374,7 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
     OrbitCache cache = orbit_cache_empty();
     if (seg > 0) {
         for (int64_t cc = c_begin - 1; cc >= c_begin - p.seg_len; --cc) {
-            PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+            PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
             if (!q.dem_ok) continue;
             Solve s = solve(p, orb, q, row_seed, 40, cache);
             if (s.oc == ZD_CONVERGED) S = s.t;
@@ -382,7 +389,7 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
     bool last_has = false, any_newton = false;
     for (int64_t cc = c_begin; cc < c_end; ++cc) {
         const int j = (int)(cc - c_begin);
-        PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+        PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
         float v_s0, v_cos;
         unsigned char v_mask;
         if (!q.dem_ok) {  // DEM_MISSING (terrain_correction.rs:1529-1536)
@@ -403,7 +410,7 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
                 last_t = s.t;
                 last_has = true;
             }
-            PixelOut v = finish_pixel(p, q, s.oc, s.t, s.sat, flat_area);
+            PixelOut v = finish_pixel<RS>(p, q, s.oc, s.t, s.sat, flat_area);
             if (row_ok) {
                 if (v.cat == CAT_ORBIT) {
                     if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
@@ -448,7 +455,8 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
 
 // One lane per row: walk the row's segments in order and correct every
 // segment whose starting seed differs from the chained value to its left.
-__global__ __launch_bounds__(256) void tc_fixup_kernel(TcParams p) {
+template <int RS, bool UTM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tc_fixup_kernel(TcParams p) {
     __shared__ OrbitDev orb;
     load_orbit(p, orb);
     const int64_t rr = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -456,8 +464,8 @@ __global__ __launch_bounds__(256) void tc_fixup_kernel(TcParams p) {
     const int64_t row = p.row0 + rr;
     const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
     const int npts = p.n_anchor_pts;
-    const double* alon = p.crs != 0 ? p.anchors + rr * 2 * npts : nullptr;
-    const double* alat = p.crs != 0 ? alon + npts : nullptr;
+    const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
+    const double* alat = UTM ? alon + npts : nullptr;
     const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
     const int64_t obase = rr * p.cols;
     SegRecord* recs = p.records + rr * p.nseg;
@@ -477,21 +485,21 @@ __global__ __launch_bounds__(256) void tc_fixup_kernel(TcParams p) {
             OrbitCache co = orbit_cache_empty(), cn = orbit_cache_empty();
             bool new_has = false, synced = false;
             for (int64_t cc = c_begin; cc < c_end; ++cc) {
-                PixelPos q = prep_pixel(p, p.col0 + cc, row_y, alon, alat);
+                PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
                 if (!q.dem_ok) continue;  // outcome independent of the seed
                 Solve o = solve(p, orb, q, so, p.max_iter, co);
                 Solve n = solve(p, orb, q, sn, p.max_iter, cn);
                 if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
                     if (o.it != n.it) {  // same result, different iteration count
-                        PixelOut v = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
+                        PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, flat_area);
                         count(C, v.cat, o.it, -1, p.counters);
                         count(C, v.cat, n.it, 1, p.counters);
                     }
                     synced = true;
                     break;
                 }
-                PixelOut vo = finish_pixel(p, q, o.oc, o.t, o.sat, flat_area);
-                PixelOut vn = finish_pixel(p, q, n.oc, n.t, n.sat, flat_area);
+                PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, flat_area);
+                PixelOut vn = finish_pixel<RS>(p, q, n.oc, n.t, n.sat, flat_area);
                 if (vo.cat == CAT_ORBIT) {
                     if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
                     else C.err_short -= 1;
@@ -526,13 +534,29 @@ __global__ __launch_bounds__(256) void tc_fixup_kernel(TcParams p) {
 
 }  // namespace sd
 
+namespace {
+template <int RS, bool UTM>
+void launch_chain(const sd::TcParams* p, hipStream_t stream) {
+    dim3 grid((unsigned)((p->rows + 63) / 64), (unsigned)((p->nseg + 3) / 4));
+    hipLaunchKernelGGL((sd::tc_segment_kernel<RS, UTM>), grid, dim3(256), 0, stream, *p);
+    hipLaunchKernelGGL((sd::tc_fixup_kernel<RS, UTM>), dim3((unsigned)((p->rows + 255) / 256)), dim3(256), 0, stream,
+                       *p);
+}
+}  // namespace
+
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream) {
-    if (p->crs != 0) {
+    const bool utm = p->crs != 0;
+    if (utm) {
         const int64_t n = p->rows * p->n_anchor_pts;
         hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
     }
-    dim3 grid((unsigned)((p->rows + 63) / 64), (unsigned)((p->nseg + 3) / 4));
-    hipLaunchKernelGGL(sd::tc_segment_kernel, grid, dim3(256), 0, stream, *p);
-    hipLaunchKernelGGL(sd::tc_fixup_kernel, dim3((unsigned)((p->rows + 255) / 256)), dim3(256), 0, stream, *p);
+    switch (p->resampling * 2 + (utm ? 1 : 0)) {
+        case 0: launch_chain<0, false>(p, stream); break;
+        case 1: launch_chain<0, true>(p, stream); break;
+        case 2: launch_chain<1, false>(p, stream); break;
+        case 3: launch_chain<1, true>(p, stream); break;
+        case 4: launch_chain<2, false>(p, stream); break;
+        default: launch_chain<2, true>(p, stream); break;
+    }
     return hipGetLastError();
 }

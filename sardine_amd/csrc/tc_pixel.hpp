@@ -9,36 +9,58 @@
 
 namespace sd {
 
-__device__ __forceinline__ bool dem_sample(const TcParams& p, double lat, double lon, float* h) {
-    // dem.rs:694-723 lon wrap, GLO-30 tile rule dem.rs:351-372, bilinear_on_grid dem.rs:385-409
-    double lon_n = lon >= 180.0 ? lon - 360.0 : lon;
-    double row_f = (p.dem_olat - lat) * p.dem_inv_pix;
-    double col_f = (lon_n - p.dem_olon) * p.dem_inv_pix;
-    double max_row = (double)(p.dem_h - 1), max_col = (double)(p.dem_w - 1);
-    if (row_f < -0.5 || row_f > max_row + 0.5 || col_f < -0.5 || col_f > max_col + 0.5) return false;
-    row_f = fmin(fmax(row_f, 0.0), max_row);
-    col_f = fmin(fmax(col_f, 0.0), max_col);
-    int64_t max_rc = p.dem_h - 1;
-    int64_t r0 = (int64_t)row_f;
-    int64_t lim_r = max_rc > 0 ? max_rc - 1 : 0;
-    r0 = r0 < lim_r ? r0 : lim_r;
-    int64_t c0 = (int64_t)col_f;
-    int64_t lim_c = p.dem_w >= 2 ? p.dem_w - 2 : 0;
-    c0 = c0 < lim_c ? c0 : lim_c;
-    double dr = row_f - (double)r0, dc = col_f - (double)c0;
-    const float* d = p.dem + r0 * p.dem_w + c0;
+// One axis of a DEM bilinear lookup: the lower grid index, the fraction and
+// whether the coordinate is inside the raster (half a pixel of margin).  The
+// row axis depends on latitude only and the column axis on longitude only,
+// so the terrain geometry's east/west samples share the centre's row axis and
+// its north/south samples share the centre's column axis.
+struct DemAxis {
+    int64_t i0;
+    double f;
+    bool ok;
+};
+
+// dem.rs:351-372 / 385-409 (GLO-30 grid rule, bilinear_on_grid), per axis.
+__device__ __forceinline__ DemAxis dem_axis(double x, double max_x, int64_t lim) {
+    DemAxis a;
+    a.ok = !(x < -0.5 || x > max_x + 0.5);
+    x = fmin(fmax(x, 0.0), max_x);
+    int64_t i0 = (int64_t)x;
+    a.i0 = i0 < lim ? i0 : lim;
+    a.f = x - (double)a.i0;
+    return a;
+}
+
+__device__ __forceinline__ DemAxis dem_row_axis(const TcParams& p, double lat) {
+    return dem_axis((p.dem_olat - lat) * p.dem_inv_pix, p.dem_max_row, p.dem_lim_row);
+}
+
+__device__ __forceinline__ DemAxis dem_col_axis(const TcParams& p, double lon) {
+    // dem.rs:694-723 longitude wrap
+    const double lon_n = lon >= 180.0 ? lon - 360.0 : lon;
+    return dem_axis((lon_n - p.dem_olon) * p.dem_inv_pix, p.dem_max_col, p.dem_lim_col);
+}
+
+__device__ __forceinline__ bool dem_bilinear(const TcParams& p, DemAxis r, DemAxis c, float* h) {
+    if (!(r.ok && c.ok)) return false;
+    const float* d = p.dem + r.i0 * p.dem_w + c.i0;
     float v00 = d[0], v01 = d[1], v10 = d[p.dem_w], v11 = d[p.dem_w + 1];
     if (isnan(v00) || isnan(v01) || isnan(v10) || isnan(v11)) return false;
+    const double dr = r.f, dc = c.f;
     double v = (double)v00 * (1.0 - dr) * (1.0 - dc) + (double)v01 * (1.0 - dr) * dc +
                (double)v10 * dr * (1.0 - dc) + (double)v11 * dr * dc;
     *h = (float)v;
     return true;
 }
 
+__device__ __forceinline__ bool dem_sample(const TcParams& p, double lat, double lon, float* h) {
+    return dem_bilinear(p, dem_row_axis(p, lat), dem_col_axis(p, lon), h);
+}
+
 __device__ __forceinline__ double geoid_n(const TcParams& p, double lat, double lon) {
     // geoid.rs:220-251 (2.5° grid, lon wrap, lat clamp)
     if (!p.geoid) return 0.0;
-    double l = fmod(lon, 360.0);
+    double l = (lon < 360.0 && lon > -360.0) ? lon : fmod(lon, 360.0);  // fmod is exact: identity below 360
     if (l < 0.0) l += 360.0;
     double la = fmin(fmax(lat, -90.0), 90.0);
     double row_f = (90.0 - la) / 2.5, col_f = l / 2.5;
@@ -126,10 +148,11 @@ __device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns
     return (float)acc;
 }
 
+template <int RS>
 __device__ __forceinline__ float resample(const TcParams& p, double line, double sample) {
-    if (p.resampling == 1) return bicubic(p.img, p.lines, p.samples, line, sample);
-    if (p.resampling == 2) return lanczos3(p.img, p.lines, p.samples, line, sample);
-    return bilinear(p.img, p.lines, p.samples, line, sample);
+    if constexpr (RS == 1) return bicubic(p.img, p.lines, p.samples, line, sample);
+    else if constexpr (RS == 2) return lanczos3(p.img, p.lines, p.samples, line, sample);
+    else return bilinear(p.img, p.lines, p.samples, line, sample);
 }
 
 }  // namespace sd
