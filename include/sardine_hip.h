@@ -14,6 +14,12 @@
  *    resident in HBM).  Small metadata (orbit, LUT vectors, grid points) is
  *    passed as host memory and staged by the library.
  *  - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *    Calls are stream-ordered.  Entry points whose results include host
+ *    values computed on the device (sardine_terrain_correction's counters,
+ *    sardine_to_db_inplace's counts) return after their work has finished;
+ *    the others (calibration, speckle, synthetic fill) return once the work
+ *    is enqueued, and their outputs are valid for later work on `stream`.
+ *    Host metadata is copied before the call returns.
  *  - Every function returns a sardine_status; on failure a message is
  *    available from sardine_last_error() (thread-local).
  *  - There is no CPU fallback: without a usable gfx950 device every compute
@@ -417,7 +423,7 @@ sardine_status sardine_to_db_inplace(float* data /*[device]*/, int64_t n, float 
 
 /* ── Kernel timing (diagnostic, no reference counterpart) ─────────────────
  * With profiling on, every compute entry point brackets its main kernel with
- * HIP events on the stream it launches on and waits for it; the duration of
+ * HIP events on the stream it launches on (without waiting); the duration of
  * the last such kernel on this thread is then available by kernel family. */
 typedef enum sardine_kernel_family {
     SARDINE_KERNEL_TC = 0,        /* terrain-correction kernel */
@@ -430,7 +436,8 @@ void sardine_set_profiling(int on);
  * terrain-correction kernel (0 = automatic, the default).  A tuning knob
  * with no effect on results: the seed-chain fix-up makes every value exact. */
 void sardine_set_tc_segment(int64_t columns);
-/* Milliseconds of the last profiled kernel of `family` on this thread, -1 if none. */
+/* Milliseconds of the last profiled kernel of `family` on this thread, -1 if
+ * none; waits for that kernel to finish. */
 double sardine_last_kernel_ms(int family);
 
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */

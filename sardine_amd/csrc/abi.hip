@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <deque>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,6 +26,7 @@
 #include "tc_params.hpp"
 
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream);
+extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t stream);
 extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* values, const int64_t* offsets,
                                           int n_vec, int64_t dense_cols, float* dense, hipStream_t stream);
 extern "C" hipError_t sd_launch_cal_merge(const sd::CalParams* p, hipStream_t stream);
@@ -42,25 +45,27 @@ thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 thread_local double g_kernel_ms[4] = {-1.0, -1.0, -1.0, -1.0};
 
-// Brackets one kernel launch with events when profiling is on.
-struct KernelTimer {
+// Brackets one kernel launch with events when profiling is on.  Nothing
+// waits here: sardine_last_kernel_ms waits for the end event when read.
+struct TimerSlot {
     hipEvent_t a = nullptr, b = nullptr;
+    bool pending = false;
+};
+thread_local TimerSlot g_slots[4];
+
+struct KernelTimer {
+    TimerSlot* slot = nullptr;
     hipStream_t s;
-    int fam;
-    KernelTimer(int family, hipStream_t stream) : s(stream), fam(family) {
-        if (g_profile && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, s);
+    KernelTimer(int family, hipStream_t stream) : s(stream) {
+        if (!g_profile) return;
+        TimerSlot& t = g_slots[family];
+        if (!t.a && hipEventCreate(&t.a) != hipSuccess) return;
+        if (!t.b && hipEventCreate(&t.b) != hipSuccess) return;
+        if (hipEventRecord(t.a, s) == hipSuccess) slot = &t;
     }
     void stop() {
-        if (!a || !b) return;
-        (void)hipEventRecord(b, s);
-        if (hipEventSynchronize(b) == hipSuccess) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, a, b) == hipSuccess) g_kernel_ms[fam] = ms;
-        }
-    }
-    ~KernelTimer() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
+        if (slot && hipEventRecord(slot->b, s) == hipSuccess) slot->pending = true;
+        slot = nullptr;
     }
 };
 
@@ -102,6 +107,13 @@ sardine_status require_device() {
                 why = std::string("device is ") + prop.gcnArchName + ", the library is built for gfx950";
             } else {
                 state = 0;
+                // Keep memory freed by hipFreeAsync in the device's default pool
+                // (the per-call work buffers are reallocated on every call).
+                hipMemPool_t pool;
+                if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                    uint64_t keep = UINT64_MAX;
+                    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+                }
             }
         }
     }
@@ -129,11 +141,76 @@ struct DevBuf {
     }
 };
 
+// Host→device copies go through pinned staging blocks: from pageable memory
+// hipMemcpyAsync copies synchronously, which would make every call wait for
+// all earlier work on its stream.  A block is reused once the event recorded
+// after its copy has completed.
+class StagingPool {
+   public:
+    hipError_t copy(void* dst, const void* src, size_t n, hipStream_t s) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        std::lock_guard<std::mutex> g(mu_);
+        Block* b = nullptr;
+        size_t busy = 0;
+        for (Block& x : blocks_) {
+            if (x.busy && hipEventQuery(x.ev) == hipSuccess) x.busy = false;
+            busy += x.busy ? 1 : 0;
+            if (!x.busy && x.dev == dev && x.cap >= n && (!b || x.cap < b->cap)) b = &x;
+        }
+        if (!b && busy >= kMaxBusy) {  // bound the pinned footprint: wait for the oldest copy
+            for (Block& x : blocks_)
+                if (x.busy && x.dev == dev && x.cap >= n) {
+                    if ((e = hipEventSynchronize(x.ev)) != hipSuccess) return e;
+                    x.busy = false;
+                    b = &x;
+                    break;
+                }
+        }
+        if (!b) {
+            Block nb;
+            nb.dev = dev;
+            nb.cap = 64 * 1024;
+            while (nb.cap < n) nb.cap *= 2;
+            if ((e = hipHostMalloc(&nb.p, nb.cap, hipHostMallocDefault)) != hipSuccess) return e;
+            if ((e = hipEventCreateWithFlags(&nb.ev, hipEventDisableTiming)) != hipSuccess) {
+                (void)hipHostFree(nb.p);
+                return e;
+            }
+            blocks_.push_back(nb);
+            b = &blocks_.back();
+        }
+        std::memcpy(b->p, src, n);
+        if ((e = hipMemcpyAsync(dst, b->p, n, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(b->ev, s)) != hipSuccess) return e;
+        b->busy = true;
+        return hipSuccess;
+    }
+
+   private:
+    struct Block {
+        void* p = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        int dev = 0;
+        bool busy = false;
+    };
+    static constexpr size_t kMaxBusy = 64;
+    std::mutex mu_;
+    std::deque<Block> blocks_;  // stable addresses; blocks live until the process ends
+};
+
+StagingPool& staging() {
+    static StagingPool* pool = new StagingPool();  // never destroyed: pinned memory outlives static teardown
+    return *pool;
+}
+
 template <typename T>
 hipError_t upload(DevBuf& b, const T* host, size_t n, hipStream_t s) {
     hipError_t e = b.alloc(n * sizeof(T), s);
     if (e != hipSuccess) return e;
-    if (n) return hipMemcpyAsync(b.p, host, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (n) return staging().copy(b.p, host, n * sizeof(T), s);
     return hipSuccess;
 }
 
@@ -229,7 +306,17 @@ extern "C" {
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
 void sardine_set_profiling(int on) { g_profile = on != 0; }
 void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : columns; }
-double sardine_last_kernel_ms(int family) { return family >= 0 && family < 4 ? g_kernel_ms[family] : -1.0; }
+double sardine_last_kernel_ms(int family) {
+    if (family < 0 || family >= 4) return -1.0;
+    TimerSlot& t = g_slots[family];
+    if (t.pending) {
+        t.pending = false;
+        float ms = 0.0f;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
+            g_kernel_ms[family] = ms;
+    }
+    return g_kernel_ms[family];
+}
 int sardine_abi_version(void) { return SARDINE_ABI_VERSION; }
 
 int sardine_device_count(void) {
@@ -340,38 +427,38 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     const double x_start = g.x_left + 0.5 * sp;
     const double x_end = x_start + ((double)g.cols - 1.0) * sp;
     const double anchor_dx = (x_end - x_start) / (double)n_anch;
-    const sd::TmSeries ts = sd::tm_series();
-    std::vector<int64_t> row_seed((size_t)nr);
-    std::vector<double> row_flat(need_geom ? (size_t)nr : 1, 0.0);
-    for (int64_t rr = 0; rr < nr; ++rr) {
-        const int64_t row = r0 + rr;
-        const double row_y = g.y_top - ((double)row + 0.5) * sp;
-        double lat_seed = row_y;
-        double ref_lon = g.x_left + ((double)g.cols / 2.0) * sp;
-        if (utm) {
-            double lon_mid;
-            sd::utm_inverse(ts, cfg->utm_zone, south, g.x_left + ((double)g.cols / 2.0) * sp, row_y, &lon_mid, &lat_seed);
-            if (need_geom) {  // row_lonlat[n_cols/2].lon from the anchor blend (output_crs.rs:395-414)
-                const int64_t col = g.cols / 2;
-                const double x = x_start + (double)col * sp;
-                int seg = 0;
-                while (seg + 1 < n_anch && x > x_start + (double)(seg + 1) * anchor_dx) ++seg;
-                const double xa = x_start + (double)seg * anchor_dx, xb = x_start + (double)(seg + 1) * anchor_dx;
-                double la0, lo0, la1, lo1;
-                sd::utm_inverse(ts, cfg->utm_zone, south, xa, row_y, &lo0, &la0);
-                sd::utm_inverse(ts, cfg->utm_zone, south, xb, row_y, &lo1, &la1);
-                const double t = xb == xa ? 0.0 : (x - xa) / (xb - xa);
-                ref_lon = lo0 + t * (lo1 - lo0);
-            }
-        }
-        row_seed[(size_t)rr] = guess_azimuth_time(lut, lat_seed) - epoch;
-        if (need_geom) row_flat[(size_t)rr] = sd::flat_cell_area(lat_seed, ref_lon, step_deg);
-    }
-
-    DevBuf d_orb, d_seed, d_flat, d_cnt, d_geoid;
+    DevBuf d_orb, d_seed, d_flat, d_cnt, d_geoid, d_lut_lat, d_lut_t;
     SD_HIP(upload(d_orb, &orb, 1, stream));
-    SD_HIP(upload(d_seed, row_seed.data(), row_seed.size(), stream));
-    SD_HIP(upload(d_flat, row_flat.data(), row_flat.size(), stream));
+    // per-row seeds and flat-cell areas on the device (tc_rows_kernel)
+    SD_HIP(upload(d_lut_lat, lut.lat.data(), lut.lat.size(), stream));
+    SD_HIP(upload(d_lut_t, lut.t.data(), lut.t.size(), stream));
+    SD_HIP(d_seed.alloc((size_t)nr * sizeof(int64_t), stream));
+    SD_HIP(d_flat.alloc((need_geom ? (size_t)nr : 1) * sizeof(double), stream));
+    {
+        sd::RowPrepParams q;
+        std::memset(&q, 0, sizeof q);
+        q.lut_lat = d_lut_lat.as<double>();
+        q.lut_t = d_lut_t.as<int64_t>();
+        q.n_lut = (int)lut.lat.size();
+        q.epoch = epoch;
+        q.y_top = g.y_top;
+        q.x_left = g.x_left;
+        q.spacing = sp;
+        q.x_start = x_start;
+        q.anchor_dx = anchor_dx;
+        q.step_deg = step_deg;
+        q.full_cols = g.cols;
+        q.row0 = r0;
+        q.rows = nr;
+        q.n_anch = n_anch;
+        q.utm = utm ? 1 : 0;
+        q.south = south ? 1 : 0;
+        q.zone = cfg->utm_zone;
+        q.need_geom = need_geom ? 1 : 0;
+        q.row_seed = d_seed.as<int64_t>();
+        q.row_flat_area = d_flat.as<double>();
+        SD_HIP(sd_launch_tc_rows(&q, stream));
+    }
     if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5) SD_HIP(upload(d_geoid, cfg->geoid.grid_73x144, 73 * 144, stream));
     SD_HIP(d_cnt.alloc((sd::kNumCounters + 1) * sizeof(unsigned long long), stream));
     SD_HIP(sd_launch_zero_u64(d_cnt.as<unsigned long long>(), sd::kNumCounters + 1, stream));
@@ -797,8 +884,7 @@ sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, 
         SD_HIP(sd_launch_cal_merge(&P, s));
         kt.stop();
     }
-    SD_HIP(hipStreamSynchronize(s));
-    return SARDINE_OK;
+    return SARDINE_OK;  // stream-ordered: the work buffers are released after the kernel
 }
 
 sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, int64_t samples, int64_t col_off,
@@ -840,7 +926,6 @@ sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, 
     D.valid_start = 0;
     D.valid_end = samples;
     SD_HIP(sd_launch_cal_merge(&P, s));
-    SD_HIP(hipStreamSynchronize(s));
     return SARDINE_OK;
 }
 

@@ -269,6 +269,59 @@ __device__ __forceinline__ void load_orbit(const TcParams& p, OrbitDev& orb) {
     __syncthreads();
 }
 
+// guess_azimuth_time (terrain_correction.rs:1107-1136): linear
+// interpolation in the latitude-sorted grid-point table, clamped at both
+// ends; an exact latitude hit returns that entry.
+__device__ __forceinline__ int64_t guess_azimuth_time(const RowPrepParams& q, double lat) {
+    const int n = q.n_lut;
+    if (n == 1) return q.lut_t[0];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        const double v = q.lut_lat[mid];
+        if (v < lat) lo = mid + 1;
+        else if (v > lat) hi = mid;
+        else return q.lut_t[mid];
+    }
+    if (lo == 0) return q.lut_t[0];
+    if (lo >= n) return q.lut_t[n - 1];
+    const double frac = (lat - q.lut_lat[lo - 1]) / (q.lut_lat[lo] - q.lut_lat[lo - 1]);
+    const double dt_us = (double)(q.lut_t[lo] - q.lut_t[lo - 1]);
+    return q.lut_t[lo - 1] + (int64_t)(frac * dt_us);
+}
+
+// One lane per AOI row: the row's Newton seed (the grid-point guess at the
+// row's seed latitude) and, for the terrain geometry, the flat-cell area at
+// (seed latitude, centre longitude) (terrain_correction.rs:1424-1511).
+__global__ __launch_bounds__(256) void tc_rows_kernel(RowPrepParams q) {
+    const int64_t rr = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (rr >= q.rows) return;
+    const int64_t row = q.row0 + rr;
+    const double row_y = q.y_top - ((double)row + 0.5) * q.spacing;
+    double lat_seed = row_y;
+    double ref_lon = q.x_left + ((double)q.full_cols / 2.0) * q.spacing;
+    if (q.utm) {
+        const TmSeries ts = tm_series();
+        double lon_mid;
+        utm_inverse(ts, q.zone, q.south != 0, q.x_left + ((double)q.full_cols / 2.0) * q.spacing, row_y, &lon_mid,
+                    &lat_seed);
+        if (q.need_geom) {  // row_lonlat[n_cols/2].lon from the anchor blend (output_crs.rs:395-414)
+            const int64_t col = q.full_cols / 2;
+            const double x = q.x_start + (double)col * q.spacing;
+            int seg = 0;
+            while (seg + 1 < q.n_anch && x > q.x_start + (double)(seg + 1) * q.anchor_dx) ++seg;
+            const double xa = q.x_start + (double)seg * q.anchor_dx, xb = q.x_start + (double)(seg + 1) * q.anchor_dx;
+            double la0, lo0, la1, lo1;
+            utm_inverse(ts, q.zone, q.south != 0, xa, row_y, &lo0, &la0);
+            utm_inverse(ts, q.zone, q.south != 0, xb, row_y, &lo1, &la1);
+            const double t = xb == xa ? 0.0 : (x - xa) / (xb - xa);
+            ref_lon = lo0 + t * (lo1 - lo0);
+        }
+    }
+    q.row_seed[rr] = guess_azimuth_time(q, lat_seed) - q.epoch;
+    if (q.need_geom) q.row_flat_area[rr] = flat_cell_area(lat_seed, ref_lon, q.step_deg);
+}
+
 // Project_row anchors of every AOI row (output_crs.rs:364-393):
 // anchors[row][0..npts) = lon, anchors[row][npts..2 npts) = lat.
 __global__ __launch_bounds__(256) void utm_anchor_kernel(TcParams p) {
@@ -543,6 +596,12 @@ void launch_chain(const sd::TcParams* p, hipStream_t stream) {
                        *p);
 }
 }  // namespace
+
+extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t stream) {
+    if (q->rows > 0)
+        hipLaunchKernelGGL(sd::tc_rows_kernel, dim3((unsigned)((q->rows + 255) / 256)), dim3(256), 0, stream, *q);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream) {
     const bool utm = p->crs != 0;

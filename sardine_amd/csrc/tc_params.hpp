@@ -68,4 +68,18 @@ struct TcParams {
     int* error_flag;
 };
 
+// Per-row seeds and flat-cell areas (terrain_correction.rs:1424-1511),
+// computed on the device by tc_rows_kernel.
+struct RowPrepParams {
+    const double* lut_lat;  // geolocation-grid seed table, sorted by latitude
+    const int64_t* lut_t;   // azimuth time (absolute µs) of each entry
+    int n_lut;
+    int64_t epoch;          // time of the first state vector (µs)
+    double y_top, x_left, spacing, x_start, anchor_dx, step_deg;
+    int64_t full_cols, row0, rows;
+    int n_anch, utm, south, zone, need_geom;
+    int64_t* row_seed;      // out: µs relative to epoch
+    double* row_flat_area;  // out (need_geom)
+};
+
 }  // namespace sd
