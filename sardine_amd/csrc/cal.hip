@@ -76,37 +76,44 @@ __device__ __forceinline__ void cal_pixel(const CalSwathDev& s, const CalLine& L
     *o_n = nv * inv * valid;
 }
 
-// One thread per merged pixel, 4 consecutive columns per thread.
+// One workgroup per merged line; consecutive lanes take consecutive merged
+// columns, so every raw, LUT and output access of a wave is one contiguous
+// run.  The line's per-swath constants are loaded once into LDS.
+constexpr int kCalUnroll = 4;
+
 __global__ __launch_bounds__(256) void cal_merge_kernel(CalParams P) {
-    const int64_t line = blockIdx.y;
-    const int64_t base = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
-    float os[4], on[4];
-    const float qnan = __builtin_nanf("");
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int64_t oc = base + j;
-        os[j] = qnan;
-        on[j] = qnan;
-        if (oc >= P.out_cols) continue;
-        int s = 0;
-        while (s + 1 < P.n_sw && oc >= P.sw[s].own_end) ++s;
-        const CalSwathDev& S = P.sw[s];
-        if (oc >= S.valid_start && oc < S.valid_end) {
-            const CalLine& L = S.lines[line];
-            cal_pixel(S, L, oc - S.out_off, &os[j], &on[j]);
-        }
+    __shared__ CalSwathDev sw[kMaxSwaths];
+    __shared__ CalLine ln[kMaxSwaths];
+    const int64_t line = blockIdx.x;
+    if (threadIdx.x < (unsigned)P.n_sw) {
+        sw[threadIdx.x] = P.sw[threadIdx.x];
+        ln[threadIdx.x] = P.sw[threadIdx.x].lines[line];
     }
-    const int64_t o = line * P.out_cols + base;
-    if (base + 3 < P.out_cols && (o & 3) == 0) {
-        *reinterpret_cast<float4*>(P.sigma0 + o) = make_float4(os[0], os[1], os[2], os[3]);
-        *reinterpret_cast<float4*>(P.nesz + o) = make_float4(on[0], on[1], on[2], on[3]);
-    } else {
+    __syncthreads();
+    const float qnan = __builtin_nanf("");
+    const int n_sw = P.n_sw;
+    float* const out_s = P.sigma0 + line * P.out_cols;
+    float* const out_n = P.nesz + line * P.out_cols;
+    for (int64_t c0 = threadIdx.x; c0 < P.out_cols; c0 += 256 * kCalUnroll) {
+        float vs[kCalUnroll], vn[kCalUnroll];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (base + j < P.out_cols) {
-                P.sigma0[o + j] = os[j];
-                P.nesz[o + j] = on[j];
+        for (int j = 0; j < kCalUnroll; ++j) {
+            const int64_t oc = c0 + 256 * j;
+            vs[j] = qnan;
+            vn[j] = qnan;
+            if (oc >= P.out_cols) continue;
+            int s = 0;
+            while (s + 1 < n_sw && oc >= sw[s].own_end) ++s;
+            if (oc >= sw[s].valid_start && oc < sw[s].valid_end) cal_pixel(sw[s], ln[s], oc - sw[s].out_off, &vs[j], &vn[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kCalUnroll; ++j) {
+            const int64_t oc = c0 + 256 * j;
+            if (oc < P.out_cols) {
+                out_s[oc] = vs[j];
+                out_n[oc] = vn[j];
             }
+        }
     }
 }
 
@@ -123,7 +130,6 @@ extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* v
 
 extern "C" hipError_t sd_launch_cal_merge(const sd::CalParams* p, hipStream_t stream) {
     if (p->out_lines <= 0 || p->out_cols <= 0) return hipSuccess;
-    dim3 grid((unsigned)((p->out_cols + 1023) / 1024), (unsigned)p->out_lines);
-    hipLaunchKernelGGL(sd::cal_merge_kernel, grid, dim3(256), 0, stream, *p);
+    hipLaunchKernelGGL(sd::cal_merge_kernel, dim3((unsigned)p->out_lines), dim3(256), 0, stream, *p);
     return hipGetLastError();
 }
