@@ -7,8 +7,9 @@ are already resident in HBM:
   correction (FP64 Newton, bilinear, 10 m UTM grid, shadow/layover mask +
   cos_lia sidecars)  →  Refined Lee 7x7.
 Workload: 3 subswaths x 9 bursts x 1500 lines x 21000 samples CInt16 (burst
-overlap 50 lines), fBm DEM (Hurst 0.8, 0-3000 m) at 1 arcsec, output the
-scene's inner 10 m UTM grid.  value = output grid pixels per second (Mpx/s).
+overlap 50 lines), fBm DEM (Hurst 0.8, 0-3000 m) at 1 arcsec, output a 10 m
+UTM grid of about 25000 (east) x 18000 (north) cells around the scene centre.
+value = output grid pixels per second (Mpx/s).
 
 Printed JSON line (rank 0): metric/value/unit, timing, the dominant kernel's
 roofline (its duration measured with HIP events on its launch stream), and a
@@ -73,6 +74,18 @@ def tc_flops(img, need_geom: bool) -> float:
     return float(f)
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE passes of this bench command, each
+    scaled by what the counter reads for a copy kernel of known size), or
+    None when no summary for this workload exists."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}_traffic.json"
+    if not f.exists():
+        return None, None
+    d = json.loads(f.read_text())
+    return int(d["traffic_bytes_per_launch"]), str(f.relative_to(ROOT))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -101,8 +114,8 @@ def main():
 
     t0 = time.time()
     if args.workload == "iw_full":
-        sc = synthetic.scene_iw_full(1)
-        workload = "iw_full_3x9x1500x21000_utm10m_bilinear_refinedlee7"
+        sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+        workload = "iw_full_3x9x1500x21000_utm10m_25000x18000_bilinear_refinedlee7"
     else:
         sc = synthetic.scene_small_plain(0)
         workload = "small_plain_2048x4096_wgs84_2048x2048_bilinear"
@@ -133,7 +146,7 @@ def main():
     torch.cuda.synchronize()
 
     L.sardine_set_profiling(1)
-    k_tc, k_cal, k_sp, lat = [], [], [], []
+    k_tc, k_seg, k_cal, k_sp, lat = [], [], [], [], []
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     ev[0].record(stream)
@@ -142,6 +155,7 @@ def main():
         img = step()
         ev[i + 1].record(stream)
         k_tc.append(L.sardine_last_kernel_ms(abi.KERNEL_TC))
+        k_seg.append(L.sardine_last_kernel_ms(abi.KERNEL_TC_SEGMENT))
         k_cal.append(L.sardine_last_kernel_ms(abi.KERNEL_CALIBRATE))
         k_sp.append(L.sardine_last_kernel_ms(abi.KERNEL_SPECKLE))
     torch.cuda.synchronize()
@@ -153,35 +167,43 @@ def main():
     px = rows * cols
     value = px / (ms_per_step / 1e3) / 1e6
     log(f"steps: {['%.1f' % x for x in lat]} ms (wall {wall:.2f}s); kernels tc {np.mean(k_tc):.2f} "
+        f"(segment {np.mean(k_seg):.2f}) "
         f"cal {np.mean(k_cal):.2f} speckle {np.mean(k_sp) if filt else 0:.2f} ms")
 
-    # dominant kernel roofline
+    # dominant kernel roofline: the per-pixel chain kernel (every output
+    # pixel's full computation runs in it once; the fix-up kernel only
+    # replays pixels whose chain seed changed)
     tc_ms = float(np.mean(k_tc))
+    seg_ms = float(np.mean(k_seg))
     flops = tc_flops(img, cfg.compute_lia)
-    tflops = flops / (tc_ms / 1e3) / 1e12
+    tflops = flops / (seg_ms / 1e3) / 1e12
     merged_bytes = info.lines * info.samples * 4
     dem_bytes = sc.dem_spec.height * sc.dem_spec.width * 4
     tc_bytes = px * (4 + (5 if cfg.compute_lia else 0)) + merged_bytes + dem_bytes
     cal_ms = float(np.mean(k_cal))
     cal_bytes = info.lines * info.samples * 12
     sp_ms = float(np.mean(k_sp)) if filt else None
+    traffic, traffic_src = measured_traffic("tc_segment_kernel", workload)
     roofline = {
-        "kernel": "tc_fp64_kernel",
+        "kernel": "tc_segment_kernel",
         "bound": "mfma",
         "achieved": round(tflops, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(tflops / FP64_PEAK_TFLOPS, 5),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "flops_per_launch": flops,
-        "launch_ms": round(tc_ms, 4),
+        "launch_ms": round(seg_ms, 4),
+        "tc_total_ms": round(tc_ms, 4),
         "note": "FP64 VALU-bound (no dense contraction on the path); peak is MI355X FP64 78.6 TFLOP/s",
     }
     other = {
         "calibrate_merge": {"ms": round(cal_ms, 4), "algorithmic_bytes": cal_bytes,
                             "GBps": round(cal_bytes / (cal_ms / 1e3) / 1e9, 1),
                             "hbm_frac": round(cal_bytes / (cal_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "tc_hbm": {"algorithmic_bytes": tc_bytes, "GBps": round(tc_bytes / (tc_ms / 1e3) / 1e9, 1)},
+        "tc_hbm": {"ms": round(tc_ms, 4), "algorithmic_bytes": tc_bytes,
+                   "GBps": round(tc_bytes / (tc_ms / 1e3) / 1e9, 1)},
     }
     if filt:
         sb = px * 8
@@ -248,15 +270,15 @@ def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
         return time.perf_counter() - t1, res
 
     r_mid = rows // 2
-    probe_rows = 32
-    dt, _ = tc_sample(r_mid, probe_rows)
+    probe_rows = 64 * cores  # one 64-row chunk per thread, as the full run uses them
+    dt, _ = tc_sample(r_mid - probe_rows // 2, probe_rows)
     per_row = dt / probe_rows
-    n_tc = int(max(64, min(rows // 2, 0.6 * target_s / max(per_row, 1e-6))))
+    n_tc = int(max(64 * cores, min(rows // 2, 0.6 * target_s / max(per_row, 1e-6))))
     dt_tc, res = tc_sample(max(0, r_mid - n_tc // 2), n_tc)
     tc_per_px = dt_tc / (n_tc * cols)
     # calibration on the first merged lines of swath 0 (raw DN copied back)
     sw = sc.swaths[0]
-    n_cal = 512
+    n_cal = 4096
     raw = ds.raws[0][:n_cal].cpu().numpy()
     fv = int(max(sw.first_valid))
     lv = int(min(sw.last_valid))

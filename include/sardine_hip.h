@@ -429,13 +429,19 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_TC = 0,        /* terrain-correction kernel */
     SARDINE_KERNEL_CALIBRATE = 1, /* fused deburst + calibration + merge kernel */
     SARDINE_KERNEL_SPECKLE = 2,
-    SARDINE_KERNEL_TO_DB = 3
+    SARDINE_KERNEL_TO_DB = 3,
+    SARDINE_KERNEL_TC_SEGMENT = 4 /* the per-pixel chain kernel of terrain correction alone */
 } sardine_kernel_family;
 void sardine_set_profiling(int on);
 /* Number of output columns one lane chains sequentially in the
  * terrain-correction kernel (0 = automatic, the default).  A tuning knob
  * with no effect on results: the seed-chain fix-up makes every value exact. */
 void sardine_set_tc_segment(int64_t columns);
+/* Columns before each segment over which the terrain-correction kernel runs
+ * a short chain to predict the segment's starting seed (default 2; 0 starts
+ * every segment at the row seed).  A tuning knob with no effect on results:
+ * a wrong prediction only costs the fix-up pass a replay. */
+void sardine_set_tc_lead(int32_t columns);
 /* Milliseconds of the last profiled kernel of `family` on this thread, -1 if
  * none; waits for that kernel to finish. */
 double sardine_last_kernel_ms(int family);

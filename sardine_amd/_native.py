@@ -59,10 +59,12 @@ def lib():
     L.sardine_last_kernel_ms.restype = C.c_double
     L.sardine_set_tc_segment.argtypes = [C.c_int64]
     L.sardine_set_tc_segment.restype = None
+    L.sardine_set_tc_lead.argtypes = [C.c_int32]
+    L.sardine_set_tc_lead.restype = None
     for name in abi.EXPORTED_SYMBOLS:
         if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",
                         "sardine_tc_config_default", "sardine_set_profiling", "sardine_last_kernel_ms",
-                        "sardine_set_tc_segment"):
+                        "sardine_set_tc_segment", "sardine_set_tc_lead"):
             getattr(L, name).restype = C.c_int
     if L.sardine_abi_version() != abi.ABI_VERSION:
         raise NativeLibraryError(f"ABI version mismatch: library {L.sardine_abi_version()}, "

@@ -194,5 +194,6 @@ EXPORTED_SYMBOLS = (
     "sardine_set_profiling",
     "sardine_last_kernel_ms",
     "sardine_set_tc_segment",
+    "sardine_set_tc_lead",
 )
-KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB = 0, 1, 2, 3
+KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT = 0, 1, 2, 3, 4

@@ -25,7 +25,8 @@
 #include "geom.hpp"
 #include "tc_params.hpp"
 
-extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream);
+extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start,
+                                        hipEvent_t seg_end);
 extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t stream);
 extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* values, const int64_t* offsets,
                                           int n_vec, int64_t dense_cols, float* dense, hipStream_t stream);
@@ -43,7 +44,9 @@ namespace {
 thread_local std::string g_last_error;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
-thread_local double g_kernel_ms[4] = {-1.0, -1.0, -1.0, -1.0};
+std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
+constexpr int kFamilies = 5;
+thread_local double g_kernel_ms[kFamilies] = {-1.0, -1.0, -1.0, -1.0, -1.0};
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
 // waits here: sardine_last_kernel_ms waits for the end event when read.
@@ -51,7 +54,7 @@ struct TimerSlot {
     hipEvent_t a = nullptr, b = nullptr;
     bool pending = false;
 };
-thread_local TimerSlot g_slots[4];
+thread_local TimerSlot g_slots[kFamilies];
 
 struct KernelTimer {
     TimerSlot* slot = nullptr;
@@ -306,8 +309,9 @@ extern "C" {
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
 void sardine_set_profiling(int on) { g_profile = on != 0; }
 void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : columns; }
+void sardine_set_tc_lead(int32_t columns) { g_tc_lead = columns < 0 ? 0 : (columns > 64 ? 64 : columns); }
 double sardine_last_kernel_ms(int family) {
-    if (family < 0 || family >= 4) return -1.0;
+    if (family < 0 || family >= kFamilies) return -1.0;
     TimerSlot& t = g_slots[family];
     if (t.pending) {
         t.pending = false;
@@ -513,6 +517,7 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
         }
         L = std::min<int64_t>(L, nc);
         p.seg_len = (int)L;
+        p.lead = g_tc_lead.load();
         p.nseg = (int)((nc + L - 1) / L);
         p.rows_per_wg = 64;
         SD_HIP(d_rec.alloc((size_t)nr * p.nseg * sizeof(sd::SegRecord), stream));
@@ -546,7 +551,14 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
         return fail(SARDINE_ERR_TC_CONFIG, "the FP32 scene-centred variant is not part of this build");
     {
         KernelTimer kt(SARDINE_KERNEL_TC, stream);
-        SD_HIP(sd_launch_tc_fp64(&p, stream));
+        // the per-pixel chain kernel alone is timed as its own family
+        TimerSlot* seg = nullptr;
+        if (g_profile) {
+            TimerSlot& t = g_slots[SARDINE_KERNEL_TC_SEGMENT];
+            if ((t.a || hipEventCreate(&t.a) == hipSuccess) && (t.b || hipEventCreate(&t.b) == hipSuccess)) seg = &t;
+        }
+        SD_HIP(sd_launch_tc_fp64(&p, stream, seg ? seg->a : nullptr, seg ? seg->b : nullptr));
+        if (seg) seg->pending = true;
         kt.stop();
     }
 

@@ -15,10 +15,12 @@
 // segment of L consecutive columns of a row, which costs the reference's ~2
 // orbit evaluations per pixel.  A wave holds 64 adjacent rows of the same
 // segment, so its DEM and radar gathers fall on neighbouring cache lines.
-// Only a segment's starting seed is unknown in parallel; it is taken from a
-// speculative solve of the previous segment's last pixel (phase A), which is
-// within 1 µs of that pixel's chained value and so approaches the next pixel
-// from the same side.  Each segment records its seed and its last converged
+// Only a segment's starting seed is unknown in parallel.  A converged time
+// is floor or ceil of the root by the side the seed approaches from, and the
+// chain approaches each root from the previous pixel's; phase A therefore
+// runs a short chain over the last few columns of the previous segment,
+// started at the row seed, and takes its end as the speculative seed.  Each
+// segment records its seed and its last converged
 // time; tc_fixup_kernel then walks every row's segments in order, and where
 // a seed differs from the chained value to its left it replays the old and
 // the corrected chains in lockstep, rewriting pixels until both converge to
@@ -422,16 +424,24 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
     const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
     const int64_t obase = rr * p.cols;
 
-    // phase A: speculative time of the previous segment's last DEM-valid pixel
+    // phase A: the chain's value at the end of the previous segment, from a
+    // short chain over the p.lead columns before this segment started at the
+    // row seed.  Its first solve (from the far row seed) may settle on either
+    // side of its root; the following ones approach their roots from the
+    // previous pixel's side, as the reference's chain does, and so usually
+    // reproduce its value exactly.
     long long S = row_seed;
     OrbitCache cache = orbit_cache_empty();
     if (seg > 0) {
-        for (int64_t cc = c_begin - 1; cc >= c_begin - p.seg_len; --cc) {
+        bool have = false;
+        for (int64_t cc = c_begin > p.lead ? c_begin - p.lead : 0; cc < c_begin; ++cc) {
             PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
             if (!q.dem_ok) continue;
-            Solve s = solve(p, orb, q, row_seed, 40, cache);
-            if (s.oc == ZD_CONVERGED) S = s.t;
-            break;
+            Solve s = solve(p, orb, q, S, have ? p.max_iter : 40, cache);
+            if (s.oc == ZD_CONVERGED) {
+                S = s.t;
+                have = true;
+            }
         }
     }
 
@@ -526,8 +536,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     zero_counts(C);
     unsigned redone = 0;
     long long E = p.row_seed[rr];
+    SegRecord r1 = recs[0], r2 = p.nseg > 1 ? recs[1] : r1;  // two records in flight ahead
     for (int64_t sg = 0; sg < p.nseg; ++sg) {
-        SegRecord rec = recs[sg];
+        SegRecord rec = r1;
+        r1 = r2;
+        if (sg + 2 < p.nseg) r2 = recs[sg + 2];
         C.err_outside += rec.err_outside;
         C.err_short += rec.err_short;
         if ((rec.flags & 2) && rec.seed != E) {
@@ -589,9 +602,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 
 namespace {
 template <int RS, bool UTM>
-void launch_chain(const sd::TcParams* p, hipStream_t stream) {
+void launch_chain(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start, hipEvent_t seg_end) {
     dim3 grid((unsigned)((p->rows + 63) / 64), (unsigned)((p->nseg + 3) / 4));
+    if (seg_start) (void)hipEventRecord(seg_start, stream);
     hipLaunchKernelGGL((sd::tc_segment_kernel<RS, UTM>), grid, dim3(256), 0, stream, *p);
+    if (seg_end) (void)hipEventRecord(seg_end, stream);
     hipLaunchKernelGGL((sd::tc_fixup_kernel<RS, UTM>), dim3((unsigned)((p->rows + 255) / 256)), dim3(256), 0, stream,
                        *p);
 }
@@ -603,19 +618,21 @@ extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t 
     return hipGetLastError();
 }
 
-extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream) {
+// seg_start / seg_end: optional events recorded around the segment kernel.
+extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start,
+                                        hipEvent_t seg_end) {
     const bool utm = p->crs != 0;
     if (utm) {
         const int64_t n = p->rows * p->n_anchor_pts;
         hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
     }
     switch (p->resampling * 2 + (utm ? 1 : 0)) {
-        case 0: launch_chain<0, false>(p, stream); break;
-        case 1: launch_chain<0, true>(p, stream); break;
-        case 2: launch_chain<1, false>(p, stream); break;
-        case 3: launch_chain<1, true>(p, stream); break;
-        case 4: launch_chain<2, false>(p, stream); break;
-        default: launch_chain<2, true>(p, stream); break;
+        case 0: launch_chain<0, false>(p, stream, seg_start, seg_end); break;
+        case 1: launch_chain<0, true>(p, stream, seg_start, seg_end); break;
+        case 2: launch_chain<1, false>(p, stream, seg_start, seg_end); break;
+        case 3: launch_chain<1, true>(p, stream, seg_start, seg_end); break;
+        case 4: launch_chain<2, false>(p, stream, seg_start, seg_end); break;
+        default: launch_chain<2, true>(p, stream, seg_start, seg_end); break;
     }
     return hipGetLastError();
 }

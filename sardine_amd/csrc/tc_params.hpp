@@ -48,6 +48,7 @@ struct TcParams {
     int64_t full_cols;
     int64_t row0, col0, rows, cols;
     int seg_len, nseg, rows_per_wg;  // work decomposition (see tc.hip)
+    int lead;                        // phase-A lead columns
     double* anchors;                  // UTM: per AOI row lon[npts], lat[npts]
     SegRecord* records;               // per (AOI row, segment)
     const int64_t* row_seed;      // per AOI row, µs relative to the first state vector

@@ -547,6 +547,12 @@ def scene_iw_full(seed: int = 1, n_swaths: int = 3, n_bursts: int = 9, lines_per
     if out_rows is None:
         # the footprint's bounding box, as an annotation's geolocation grid gives it
         bbox = (float(flat.min()), float(flat.max()), float(flon.min()), float(flon.max()))
+    elif utm:
+        # a lat/lon box spanning about out_rows x out_cols 10 m cells around
+        # the scene centre (the grid is the UTM extent of its corners)
+        half_lat = 0.5 * out_rows * 10.0 / METRES_PER_DEG
+        half_lon = 0.5 * out_cols * 10.0 / (METRES_PER_DEG * math.cos(math.radians(clat)))
+        bbox = (clat - half_lat, clat + half_lat, clon - half_lon, clon + half_lon)
     else:
         sp_deg = 10.0 / METRES_PER_DEG
         bbox = _aligned_bbox(clat, clon, out_rows, out_cols, sp_deg)

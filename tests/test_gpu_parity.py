@@ -234,11 +234,12 @@ def test_orbit_window_error_aborts(small):
     assert getattr(e.value, "status", None) == abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
 
 
-@pytest.mark.parametrize("min_seg", [1, 5, 64, 100000])
-def test_tc_chain_segmentation_invariant(small, min_seg):
-    """Segment length (lanes chaining 1 column .. whole rows) must not change
-    a single output bit: the seed-chain fix-up makes every decomposition
-    equal the reference's sequential chain."""
+@pytest.mark.parametrize("min_seg,lead", [(1, 2), (5, 2), (64, 2), (100000, 2), (5, 0), (16, 0), (32, 1)])
+def test_tc_chain_segmentation_invariant(small, min_seg, lead):
+    """Segment length (lanes chaining 1 column .. whole rows) and the phase-A
+    lead must not change a single output bit: the seed-chain fix-up makes
+    every decomposition equal the reference's sequential chain.  lead 0
+    starts every segment at the row seed, so the fix-up replays most of them."""
     from sardine_amd import _native
 
     sc, ds, info, s0, nz = small
@@ -246,7 +247,11 @@ def test_tc_chain_segmentation_invariant(small, min_seg):
     L = _native.lib()
     try:
         L.sardine_set_tc_segment(min_seg)
+        L.sardine_set_tc_lead(lead)
         img, ref = _tc_compare(sc, ds, info, s0, nz, m)
     finally:
         L.sardine_set_tc_segment(0)
+        L.sardine_set_tc_lead(2)
     assert img.valid_pixel_count == ref.counters()["valid"]
+    if lead == 0 and min_seg < 64:
+        assert img.chain_redo_rounds > 0  # the replay path ran
