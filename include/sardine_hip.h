@@ -446,6 +446,62 @@ void sardine_set_tc_lead(int32_t columns);
  * none; waits for that kernel to finish. */
 double sardine_last_kernel_ms(int family);
 
+/* ── Resident scenes and batched geocoding (no reference counterpart: the
+ * reference geocodes one scene per call; these entry points run many
+ * terrain_correction + SpeckleKernel::apply jobs — AOI tiles of uneven size,
+ * mixed resamplers, CRSs and filters, on one or more device-resident scenes
+ * — in one set of persistent launches with the results each job would get
+ * from sardine_terrain_correction and sardine_speckle_apply alone). ──── */
+
+typedef struct sardine_resident_scene_s* sardine_resident_scene;
+
+/* Keeps a scene's orbit and azimuth-time seed table on the device.  The
+ * merged and DEM rasters stay caller-owned device memory and must outlive
+ * the handle; the host metadata is copied. */
+sardine_status sardine_scene_create(const sardine_radar_image* merged, const sardine_scene* scene,
+                                    const sardine_dem_raster* dem, sardine_resident_scene* out);
+void sardine_scene_destroy(sardine_resident_scene scene);
+
+typedef struct sardine_batch_job {
+    int32_t scene;                  /* index into the scenes passed with the batch */
+    int32_t status;                 /* out: this job's sardine_status */
+    sardine_tc_config config;       /* output grid, AOI window, resampler, ... */
+    sardine_geocoded_image* out;    /* device buffers in; shape and counters out */
+    sardine_speckle_filter speckle; /* kind < 0: no speckle step */
+    float* speckle_out;             /* [device] rows*cols floats when speckle.kind >= 0 */
+} sardine_batch_job;
+
+/* Launch shape of a batch: chain segment length (0 = automatic), persistent
+ * workgroups per CU of the chain pass and of the speckle pass. */
+typedef struct sardine_batch_tuning {
+    int32_t segment_columns;
+    int32_t tc_groups_per_cu;
+    int32_t speckle_groups_per_cu;
+} sardine_batch_tuning;
+
+typedef struct sardine_batch_s* sardine_batch;
+
+/* An executor for up to max_jobs jobs per run.  use_graph != 0 captures the
+ * launch sequence once as a HIP graph (recaptured when the tuning or the
+ * largest speckle window changes) and replays it every run. */
+sardine_status sardine_batch_create(int32_t max_jobs, int32_t use_graph, sardine_batch* out);
+void sardine_batch_destroy(sardine_batch batch);
+sardine_status sardine_batch_set_tuning(sardine_batch batch, const sardine_batch_tuning* tuning);
+sardine_status sardine_batch_get_tuning(sardine_batch batch, sardine_batch_tuning* tuning);
+
+/* Runs every job and returns when all have finished.  A job that fails
+ * validation is skipped with its status set; the others still run.  Returns
+ * SARDINE_OK when every job succeeded, else the first failing job's status.
+ * device_ms (optional) receives the device time of the run's launches. */
+sardine_status sardine_batch_run(sardine_batch batch, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                 sardine_batch_job* jobs, int32_t n_jobs, void* stream, double* device_ms);
+
+/* Project autotuner: times candidate tunings on this batch (two runs each,
+ * the faster counts), keeps the fastest in the batch and returns it. */
+sardine_status sardine_batch_autotune(sardine_batch batch, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                      sardine_batch_job* jobs, int32_t n_jobs, void* stream,
+                                      sardine_batch_tuning* best, double* best_ms);
+
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */
 
 /* Fill a CInt16 raster with round(sigma * N(0,1)) real/imag parts from a

@@ -61,10 +61,23 @@ def lib():
     L.sardine_set_tc_segment.restype = None
     L.sardine_set_tc_lead.argtypes = [C.c_int32]
     L.sardine_set_tc_lead.restype = None
+    PP = C.POINTER(C.c_void_p)
+    L.sardine_scene_create.argtypes = [C.POINTER(abi.RadarImage), C.POINTER(abi.Scene), C.POINTER(abi.DemRaster), PP]
+    L.sardine_scene_destroy.argtypes = [P]
+    L.sardine_scene_destroy.restype = None
+    L.sardine_batch_create.argtypes = [i32, i32, PP]
+    L.sardine_batch_destroy.argtypes = [P]
+    L.sardine_batch_destroy.restype = None
+    L.sardine_batch_set_tuning.argtypes = [P, C.POINTER(abi.BatchTuning)]
+    L.sardine_batch_get_tuning.argtypes = [P, C.POINTER(abi.BatchTuning)]
+    L.sardine_batch_run.argtypes = [P, PP, i32, C.POINTER(abi.BatchJob), i32, P, C.POINTER(d)]
+    L.sardine_batch_autotune.argtypes = [P, PP, i32, C.POINTER(abi.BatchJob), i32, P, C.POINTER(abi.BatchTuning),
+                                         C.POINTER(d)]
     for name in abi.EXPORTED_SYMBOLS:
         if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",
                         "sardine_tc_config_default", "sardine_set_profiling", "sardine_last_kernel_ms",
-                        "sardine_set_tc_segment", "sardine_set_tc_lead"):
+                        "sardine_set_tc_segment", "sardine_set_tc_lead", "sardine_scene_destroy",
+                        "sardine_batch_destroy"):
             getattr(L, name).restype = C.c_int
     if L.sardine_abi_version() != abi.ABI_VERSION:
         raise NativeLibraryError(f"ABI version mismatch: library {L.sardine_abi_version()}, "

@@ -131,6 +131,16 @@ class SpeckleFilterC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("window", C.c_int32), ("enl", C.c_float), ("damping", C.c_float)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("status", C.c_int32), ("config", TcConfig),
+                ("out", C.POINTER(GeocodedImage)), ("speckle", SpeckleFilterC), ("speckle_out", C.c_void_p)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("segment_columns", C.c_int32), ("tc_groups_per_cu", C.c_int32),
+                ("speckle_groups_per_cu", C.c_int32)]
+
+
 class Burst(C.Structure):
     _fields_ = [("azimuth_time_us", C.c_int64), ("first_line", C.c_int64),
                 ("first_valid_sample", C.c_int64), ("last_valid_sample", C.c_int64),
@@ -195,5 +205,13 @@ EXPORTED_SYMBOLS = (
     "sardine_last_kernel_ms",
     "sardine_set_tc_segment",
     "sardine_set_tc_lead",
+    "sardine_scene_create",
+    "sardine_scene_destroy",
+    "sardine_batch_create",
+    "sardine_batch_destroy",
+    "sardine_batch_set_tuning",
+    "sardine_batch_get_tuning",
+    "sardine_batch_run",
+    "sardine_batch_autotune",
 )
 KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT = 0, 1, 2, 3, 4

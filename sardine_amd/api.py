@@ -374,3 +374,139 @@ def synth_cint16(out, seed: int, index_base: int = 0, sigma: float = 100.0, stre
     _check(lib().sardine_synth_cint16(out.data_ptr(), out.numel() // 2, seed, index_base, sigma,
                                       _stream_ptr(stream)))
     return out
+
+
+# ─── resident scenes and batched geocoding (no reference counterpart) ─────
+class ResidentScene:
+    """A scene kept on the device for batched geocoding: orbit and seed table
+    uploaded once; the merged and DEM rasters stay the caller's (keep
+    references to them for the scene's lifetime)."""
+
+    def __init__(self, merged: MergedSigma0, scene: SceneInputs, dem: DemRaster):
+        self.merged, self.scene, self.dem = merged, scene, dem
+        self._c = (merged.as_c(), scene.as_c(), dem.as_c())
+        h = C.c_void_p()
+        _check(lib().sardine_scene_create(C.byref(self._c[0]), C.byref(self._c[1]), C.byref(self._c[2]),
+                                          C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().sardine_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class BatchJob:
+    """One AOI geocode request: the scene index, its TerrainCorrectionConfig
+    (AOI window, CRS, resampler, ...) and an optional speckle filter."""
+    scene: int
+    config: TerrainCorrectionConfig
+    speckle: Optional[SpeckleFilter] = None
+    out: Optional[tuple] = None       # (data, cos_lia, mask) device tensors; allocated when None
+    speckle_out: object = None
+
+
+@dataclass
+class BatchResult:
+    image: GeocodedImage
+    speckled: object  # the speckle output tensor, or None
+    status: int
+
+
+class Batch:
+    """Batched executor (sardine_batch_*): every job of a run shares the
+    persistent chain and speckle launches (replayed from one HIP graph when
+    use_graph)."""
+
+    def __init__(self, max_jobs: int = 256, use_graph: bool = True):
+        h = C.c_void_p()
+        _check(lib().sardine_batch_create(int(max_jobs), 1 if use_graph else 0, C.byref(h)))
+        self.handle = h
+        self.max_jobs = max_jobs
+        self.last_device_ms = None
+
+    def close(self):
+        if self.handle:
+            lib().sardine_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tuning(self) -> abi.BatchTuning:
+        t = abi.BatchTuning()
+        _check(lib().sardine_batch_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    @tuning.setter
+    def tuning(self, t: abi.BatchTuning):
+        _check(lib().sardine_batch_set_tuning(self.handle, C.byref(t)))
+
+    def _prepare(self, scenes, jobs):
+        torch = _torch()
+        arr = (C.c_void_p * len(scenes))(*[s.handle for s in scenes])
+        cj = (abi.BatchJob * len(jobs))()
+        keep = []
+        for i, j in enumerate(jobs):
+            sc = scenes[j.scene]
+            r, c, _ = output_grid(sc.scene, j.config)
+            if j.config.aoi is not None:
+                r, c = int(j.config.aoi[2]), int(j.config.aoi[3])
+            dev = sc.merged.data.device
+            if j.out is None:
+                data = torch.empty((r, c), dtype=torch.float32, device=dev)
+                cl = torch.empty((r, c), dtype=torch.float32, device=dev) if j.config.compute_lia else None
+                mk = torch.empty((r, c), dtype=torch.uint8, device=dev) if j.config.compute_lia else None
+                j.out = (data, cl, mk)
+            if j.speckle is not None and j.speckle_out is None:
+                j.speckle_out = torch.empty((r, c), dtype=torch.float32, device=dev)
+            data, cl, mk = j.out
+            img = abi.GeocodedImage()
+            img.data = data.data_ptr()
+            img.cos_lia = None if cl is None else cl.data_ptr()
+            img.mask = None if mk is None else mk.data_ptr()
+            keep.append(img)
+            cj[i].scene = j.scene
+            cj[i].config = j.config.as_c()
+            cj[i].out = C.pointer(img)
+            cj[i].speckle = j.speckle.as_c() if j.speckle is not None else abi.SpeckleFilterC(-1, 0, 0.0, 0.0)
+            cj[i].speckle_out = None if j.speckle is None else j.speckle_out.data_ptr()
+        return arr, cj, keep
+
+    def run(self, scenes, jobs, stream=None, check: bool = True):
+        arr, cj, keep = self._prepare(scenes, jobs)
+        ms = C.c_double(0.0)
+        st = lib().sardine_batch_run(self.handle, arr, len(scenes), cj, len(jobs), _stream_ptr(stream), C.byref(ms))
+        if check:
+            _check(st)
+        self.last_device_ms = ms.value
+        out = []
+        for i, j in enumerate(jobs):
+            img = keep[i]
+            data, cl, mk = j.out
+            gi = GeocodedImage(data, img.rows, img.cols, tuple(img.geotransform), img.total_pixel_count,
+                               img.valid_pixel_count, img.dem_missing_count, img.outside_footprint_count,
+                               img.non_converged_count, img.degenerate_geometry_count, img.flat_masked_count,
+                               img.noise_masked_count, list(img.newton_iter_histogram), cl, mk, img.full_rows,
+                               img.full_cols, img.chain_redo_rounds, None)
+            out.append(BatchResult(gi, j.speckle_out if j.speckle is not None else None, cj[i].status))
+        return out
+
+    def autotune(self, scenes, jobs, stream=None):
+        arr, cj, keep = self._prepare(scenes, jobs)
+        best = abi.BatchTuning()
+        ms = C.c_double(0.0)
+        _check(lib().sardine_batch_autotune(self.handle, arr, len(scenes), cj, len(jobs), _stream_ptr(stream),
+                                            C.byref(best), C.byref(ms)))
+        return best, ms.value

@@ -1,7 +1,8 @@
 """Build recipe of the native library sardine_amd/libsardine_hip.so.
 
-One hipcc invocation compiles every HIP source for gfx950 (the kernels and
-the host side of the C ABI) into a shared library next to this file.
+hipcc compiles every HIP source for gfx950 (the kernels and the host side of
+the C ABI), one object per source in parallel, and links them into a shared
+library next to this file.
 Flags: -ffp-contract=off keeps the reference's separately rounded
 multiplies and adds; packed-FP32 instructions are disabled.
 """
@@ -12,11 +13,12 @@ import pathlib
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libsardine_hip.so"
-SOURCES = ["abi.hip", "tc.hip", "cal.hip", "speckle.hip", "misc.hip"]
+SOURCES = ["abi.hip", "tc.hip", "tc_batch.hip", "cal.hip", "speckle.hip", "misc.hip"]
 
 
 def hipcc() -> str:
@@ -32,9 +34,7 @@ def flags() -> list:
             "-Wno-pass-failed", "-Wno-unused-command-line-argument"]
 
 
-def build(verbose: bool = False) -> pathlib.Path:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *flags(), "-shared", "-o", str(tmp), *[str(CSRC / s) for s in SOURCES]]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -44,7 +44,21 @@ def build(verbose: bool = False) -> pathlib.Path:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{err}")
     if verbose and err.strip():
         print(err, file=sys.stderr)
-    os.replace(tmp, LIB)
+
+
+def build(verbose: bool = False) -> pathlib.Path:
+    """Compile each source to an object in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with tempfile.TemporaryDirectory(prefix="sardine_build_") as td:
+        objs = [pathlib.Path(td) / (pathlib.Path(s).stem + ".o") for s in SOURCES]
+        cmds = [[hipcc(), *flags(), "-c", str(CSRC / s), "-o", str(o)] for s, o in zip(SOURCES, objs)]
+        workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(lambda c: _run(c, verbose), cmds))
+        tmp = LIB.with_suffix(".so.tmp")
+        _run([hipcc(), "--offload-arch=gfx950", "-shared", "-o", str(tmp), *map(str, objs)], verbose)
+        os.replace(tmp, LIB)
     return LIB
 
 

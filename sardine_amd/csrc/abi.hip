@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <cmath>
 #include <cstdarg>
@@ -301,6 +302,288 @@ sardine_status output_grid(const sardine_scene* sc, const sardine_tc_config* cfg
     return SARDINE_OK;
 }
 
+// ── terrain correction: planning shared by the per-call and batched paths ──
+
+// apply_speckle_filter's parameter checks (speckle.rs:131-180), in order.
+sardine_status validate_filter(const sardine_speckle_filter* f, int* win_out) {
+    if (f->kind < SARDINE_SPECKLE_BOXCAR || f->kind > SARDINE_SPECKLE_REFINED_LEE)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown speckle filter kind %d", f->kind);
+    const int win = f->kind == SARDINE_SPECKLE_REFINED_LEE ? 7 : f->window;
+    if (win < 3 || win % 2 == 0) return fail(SARDINE_ERR_SPECKLE_BAD_WINDOW, "window size %d must be an odd integer >= 3", win);
+    if (f->kind == SARDINE_SPECKLE_LEE || f->kind == SARDINE_SPECKLE_GAMMA_MAP || f->kind == SARDINE_SPECKLE_REFINED_LEE)
+        if (!(f->enl > 0.0f) || !std::isfinite(f->enl))
+            return fail(SARDINE_ERR_SPECKLE_BAD_ENL, "ENL must be > 0; got %g", (double)f->enl);
+    if (f->kind == SARDINE_SPECKLE_FROST && (!std::isfinite(f->damping) || f->damping < 0.0f))
+        return fail(SARDINE_ERR_SPECKLE_BAD_DAMPING, "Frost damping factor must be >= 0 and finite; got %g",
+                    (double)f->damping);
+    if (win > 63) return fail(SARDINE_ERR_CAPACITY, "window %d above the supported 63", win);
+    *win_out = win;
+    return SARDINE_OK;
+}
+
+// Orbit in device form, times relative to the first state vector.
+sd::OrbitDev make_orbit_dev(const sardine_orbit& o) {
+    sd::OrbitDev orb;
+    std::memset(&orb, 0, sizeof orb);
+    orb.n = o.count;
+    const int64_t epoch = orb.n > 0 ? o.state_vectors[0].time_us : 0;
+    for (int i = 0; i < orb.n; ++i) {
+        const sardine_state_vector& s = o.state_vectors[i];
+        orb.t_us[i] = s.time_us - epoch;
+        orb.tsec[i] = (double)(s.time_us - epoch) / 1e6;
+        for (int c = 0; c < 3; ++c) {
+            orb.pos[i][c] = s.position_m[c];
+            orb.vel[i][c] = s.velocity_m_s[c];
+        }
+    }
+    sd::orbit_fill_bary(orb);
+    return orb;
+}
+
+// Device copies of a scene's orbit and seed table.
+struct SceneDev {
+    const sd::OrbitDev* orb = nullptr;
+    const double* lut_lat = nullptr;
+    const int64_t* lut_t = nullptr;
+    int n_lut = 0;
+    int64_t epoch = 0;
+};
+
+// One terrain-correction job: validated, with its grid and kernel arguments
+// filled except for the per-call work buffers.
+struct TcPlan {
+    Grid g;
+    int64_t r0 = 0, c0 = 0, nr = 0, nc = 0;
+    bool utm = false, need_geom = false, compute_lia = false;
+    int n_anch = 0;
+    sd::TcParams p;
+    sd::RowPrepParams q;
+};
+
+// Validation in the reference's order (terrain_correction.rs:1262-1317).
+sardine_status validate_tc(const sardine_radar_image* merged, const sardine_scene* scene,
+                           const sardine_dem_raster* dem, const sardine_tc_config* cfg, Grid* g) {
+    if (scene->n_grid_points <= 0 || !scene->grid_points)
+        return fail(SARDINE_ERR_TC_NO_GRID_POINTS, "no geolocation grid points provided");
+    if (!(cfg->pixel_spacing > 0.0)) return fail(SARDINE_ERR_TC_CONFIG, "pixel spacing must be positive");
+    if (scene->n_subswaths <= 0) return fail(SARDINE_ERR_TC_NO_SUBSWATHS, "scene has no subswaths");
+    const double ati = scene->azimuth_time_interval_s;
+    if (!std::isfinite(ati) || ati <= 0.0)
+        return fail(SARDINE_ERR_TC_CONFIG, "invalid azimuth_time_interval_s in subswath metadata: %g", ati);
+    sardine_status st = output_grid(scene, cfg, g);
+    if (st) return st;
+    const int anchors = cfg->projector_anchors == 0 ? 256 : cfg->projector_anchors;
+    if (anchors < 1) return fail(SARDINE_ERR_TC_CONFIG, "SARDINE_TC_PROJECTOR_ANCHORS must be >= 1 (default 256)");
+    if (anchors > sd::kMaxAnchorPts - 1)
+        return fail(SARDINE_ERR_TC_CONFIG, "projector anchors %d above the supported %d", anchors,
+                    sd::kMaxAnchorPts - 1);
+    if (cfg->max_newton_iterations < 0 || cfg->max_newton_iterations > 255)
+        return fail(SARDINE_ERR_TC_CONFIG, "max_newton_iterations %d outside 0..255", cfg->max_newton_iterations);
+    if (cfg->resampling < 0 || cfg->resampling > 2) return fail(SARDINE_ERR_TC_CONFIG, "bad resampling kernel");
+    if (cfg->precision != SARDINE_PRECISION_FP64 && cfg->precision != SARDINE_PRECISION_FP32_CENTRED)
+        return fail(SARDINE_ERR_TC_CONFIG, "bad precision selector");
+    if (scene->orbit.count > sd::kMaxSv || scene->orbit.count < 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", scene->orbit.count, sd::kMaxSv);
+    if (!merged->data || merged->lines <= 0 || merged->samples <= 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "radar image has no raster");
+    if (!dem->data || dem->width < 2 || dem->height < 2 || !(dem->pixel_deg > 0.0))
+        return fail(SARDINE_ERR_TC_DEM, "DEM raster must be at least 2x2 with positive spacing");
+    if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 && !cfg->geoid.grid_73x144)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "geoid grid pointer is null");
+    return SARDINE_OK;
+}
+
+// Everything of one job that needs no device work: grid, AOI, kernel
+// arguments (terrain_correction.rs:1304-1511).  Work buffers (records,
+// anchors, row seeds, flat areas, counters, geoid) are set by the caller.
+sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* scene, const sardine_dem_raster* dem,
+                       const sardine_tc_config* cfg, const sardine_geocoded_image* out, const SceneDev& sdv,
+                       TcPlan* pl) {
+    sardine_status st = validate_tc(merged, scene, dem, cfg, &pl->g);
+    if (st) return st;
+    const Grid& g = pl->g;
+    int64_t r0 = cfg->aoi_row0, c0 = cfg->aoi_col0, nr = cfg->aoi_rows, nc = cfg->aoi_cols;
+    if (nr == 0) { r0 = 0; c0 = 0; nr = g.rows; nc = g.cols; }
+    if (r0 < 0 || c0 < 0 || nr <= 0 || nc <= 0 || r0 + nr > g.rows || c0 + nc > g.cols)
+        return fail(SARDINE_ERR_TC_CONFIG, "AOI window outside the %lld x %lld output grid", (long long)g.rows,
+                    (long long)g.cols);
+    if (nr > 0x7fffffffLL) return fail(SARDINE_ERR_CAPACITY, "AOI has too many rows");
+    const bool compute_lia = cfg->compute_lia != 0;
+    if (!out->data || (compute_lia && (!out->cos_lia || !out->mask)))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "output buffers missing");
+    if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED)
+        return fail(SARDINE_ERR_TC_CONFIG, "the FP32 scene-centred variant is not part of this build");
+    pl->r0 = r0; pl->c0 = c0; pl->nr = nr; pl->nc = nc;
+    const int anchors = cfg->projector_anchors == 0 ? 256 : cfg->projector_anchors;
+    const double sp = cfg->pixel_spacing;
+    const bool utm = cfg->crs != SARDINE_CRS_WGS84;
+    const bool south = cfg->crs == SARDINE_CRS_UTM_SOUTH;
+    const bool need_geom = cfg->flatten || cfg->compute_lia;
+    const double step_deg = utm ? sp / 111320.0 : sp;
+    const int n_anch = (int)std::min<int64_t>(anchors, g.cols);
+    const double x_start = g.x_left + 0.5 * sp;
+    const double x_end = x_start + ((double)g.cols - 1.0) * sp;
+    const double anchor_dx = (x_end - x_start) / (double)n_anch;
+    pl->utm = utm; pl->need_geom = need_geom; pl->compute_lia = compute_lia; pl->n_anch = n_anch;
+
+    sd::RowPrepParams& q = pl->q;
+    std::memset(&q, 0, sizeof q);
+    q.lut_lat = sdv.lut_lat;
+    q.lut_t = sdv.lut_t;
+    q.n_lut = sdv.n_lut;
+    q.epoch = sdv.epoch;
+    q.y_top = g.y_top;
+    q.x_left = g.x_left;
+    q.spacing = sp;
+    q.x_start = x_start;
+    q.anchor_dx = anchor_dx;
+    q.step_deg = step_deg;
+    q.full_cols = g.cols;
+    q.row0 = r0;
+    q.rows = nr;
+    q.n_anch = n_anch;
+    q.utm = utm ? 1 : 0;
+    q.south = south ? 1 : 0;
+    q.zone = cfg->utm_zone;
+    q.need_geom = need_geom ? 1 : 0;
+
+    const double ati = scene->azimuth_time_interval_s;
+    sd::TcParams& p = pl->p;
+    std::memset(&p, 0, sizeof p);
+    p.orbit = sdv.orb;
+    p.img = merged->data;
+    p.nesz = (std::isfinite(cfg->noise_floor_margin_db) && merged->nesz) ? merged->nesz : nullptr;
+    p.lines = merged->lines;
+    p.samples = merged->samples;
+    p.az_start_us = merged->azimuth_start_time_us - sdv.epoch;
+    p.near_range_m = merged->near_slant_range_time_s * sd::kC / 2.0;
+    p.rps = merged->range_pixel_spacing_m;
+    p.ati = ati;
+    p.line_per_us = 1.0 / (1e6 * ati);
+    p.inv_rps = 1.0 / p.rps;
+    p.dem = dem->data;
+    p.dem_w = dem->width;
+    p.dem_h = dem->height;
+    p.dem_olat = dem->origin_lat_deg;
+    p.dem_olon = dem->origin_lon_deg;
+    p.dem_pix = dem->pixel_deg;
+    p.dem_inv_pix = 1.0 / dem->pixel_deg;
+    p.dem_max_row = (double)(dem->height - 1);
+    p.dem_max_col = (double)(dem->width - 1);
+    p.dem_lim_row = dem->height - 1 > 0 ? dem->height - 2 : 0;
+    p.dem_lim_col = dem->width >= 2 ? dem->width - 2 : 0;
+    p.crs = utm ? 1 : 0;
+    p.utm_zone = cfg->utm_zone;
+    p.utm_south = south ? 1 : 0;
+    p.n_anchor_pts = n_anch + 1;
+    p.spacing = sp;
+    p.lat_north = g.y_top;
+    p.lon_west = g.x_left;
+    p.x_start = x_start;
+    p.anchor_dx = anchor_dx;
+    p.inv_anchor_dx = anchor_dx > 0.0 ? 1.0 / anchor_dx : 0.0;
+    p.full_cols = g.cols;
+    p.row0 = r0;
+    p.col0 = c0;
+    p.rows = nr;
+    p.cols = nc;
+    p.max_iter = cfg->max_newton_iterations;
+    p.tol = cfg->newton_tolerance_s;
+    p.flatten = cfg->flatten ? 1 : 0;
+    p.compute_lia = compute_lia ? 1 : 0;
+    p.need_geom = need_geom ? 1 : 0;
+    p.resampling = cfg->resampling;
+    p.step_deg = step_deg;
+    p.step_sin = std::sin(step_deg * sd::kPi / 180.0);
+    p.step_cos = std::cos(step_deg * sd::kPi / 180.0);
+    // 10f32.powf(margin / 10.0) (terrain_correction.rs:1727), evaluated once
+    p.nesz_factor = std::pow(10.0f, cfg->noise_floor_margin_db / 10.0f);
+    p.out = out->data;
+    p.cos_lia = compute_lia ? out->cos_lia : nullptr;
+    p.mask = compute_lia ? out->mask : nullptr;
+    p.radar_coords = out->radar_coords;
+    p.lead = g_tc_lead.load();
+    return SARDINE_OK;
+}
+
+// Segment length of the chain decomposition: `requested` > 0 forces it,
+// otherwise aim for ~2 waves of 64 lanes per SIMD over the whole grid.
+void set_decomposition(sd::TcParams& p, int64_t requested) {
+    int64_t L = requested;
+    if (L <= 0) {
+        const int64_t target_lanes = (int64_t)256 * 4 * 2 * 64;
+        L = std::min<int64_t>(32, std::max<int64_t>(16, (p.rows * p.cols + target_lanes - 1) / target_lanes));
+    }
+    L = std::min<int64_t>(L, p.cols);
+    p.seg_len = (int)L;
+    p.nseg = (int)((p.cols + L - 1) / L);
+    p.rows_per_wg = 64;
+}
+
+// Bytes of the per-job work buffers, each rounded to 256.
+struct TcWork {
+    size_t records = 0, anchors = 0, seed = 0, flat = 0, geoid = 0;
+    size_t total() const { return records + anchors + seed + flat + geoid; }
+};
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
+    TcWork w;
+    w.records = round256((size_t)pl.nr * pl.p.nseg * sizeof(sd::SegRecord));
+    w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * (pl.n_anch + 1) * sizeof(double)) : 0;
+    w.seed = round256((size_t)pl.nr * sizeof(int64_t));
+    w.flat = pl.need_geom ? round256((size_t)pl.nr * sizeof(double)) : 0;
+    w.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? round256(73 * 144 * sizeof(float)) : 0;
+    return w;
+}
+// Points the job's kernel arguments at its slice of a work arena.
+void assign_work(TcPlan& pl, const TcWork& w, char* base) {
+    pl.p.records = reinterpret_cast<sd::SegRecord*>(base);
+    base += w.records;
+    pl.p.anchors = w.anchors ? reinterpret_cast<double*>(base) : nullptr;
+    base += w.anchors;
+    pl.p.row_seed = reinterpret_cast<const int64_t*>(base);
+    pl.q.row_seed = reinterpret_cast<int64_t*>(base);
+    base += w.seed;
+    pl.p.row_flat_area = w.flat ? reinterpret_cast<const double*>(base) : nullptr;
+    pl.q.row_flat_area = w.flat ? reinterpret_cast<double*>(base) : nullptr;
+    base += w.flat;
+    pl.p.geoid = w.geoid ? reinterpret_cast<const float*>(base) : nullptr;
+}
+
+// Host result of one job from its counters (GeocodedImage fields).
+sardine_status finish_tc(const TcPlan& pl, const unsigned long long* cnt, int n_sv, sardine_geocoded_image* out) {
+    const int err = (int)(cnt[sd::kNumCounters] & 0xffffffffull);
+    if (err & 1)
+        return fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW,
+                    "orbit interpolation failed: interpolation time outside orbit window; refusing to extrapolate");
+    if (err & 2)
+        return fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT,
+                    "orbit interpolation failed: need at least 8 vectors for interpolation, have %d", n_sv);
+    const Grid& g = pl.g;
+    const double sp = pl.p.spacing;
+    out->rows = pl.nr;
+    out->cols = pl.nc;
+    out->full_rows = g.rows;
+    out->full_cols = g.cols;
+    out->geotransform[0] = g.x_left + (double)pl.c0 * sp;
+    out->geotransform[1] = sp;
+    out->geotransform[2] = 0.0;
+    out->geotransform[3] = g.y_top - (double)pl.r0 * sp;
+    out->geotransform[4] = 0.0;
+    out->geotransform[5] = -sp;
+    out->total_pixel_count = (uint64_t)(pl.nr * pl.nc);
+    out->valid_pixel_count = cnt[0];
+    out->dem_missing_count = cnt[1];
+    out->outside_footprint_count = cnt[2];
+    out->non_converged_count = cnt[3];
+    out->degenerate_geometry_count = cnt[4];
+    out->flat_masked_count = cnt[5];
+    out->noise_masked_count = cnt[6];
+    for (int i = 0; i < SARDINE_NEWTON_HIST_SIZE; ++i) out->newton_iter_histogram[i] = cnt[7 + i];
+    out->chain_redo_rounds = cnt[sd::kCntRedo];
+    return SARDINE_OK;
+}
+
 }  // namespace
 
 // ═════════════════════════════════════════════════════════════════════════
@@ -363,192 +646,36 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
                                           sardine_geocoded_image* out, void* stream_v) {
     if (!merged || !scene || !dem || !cfg || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     hipStream_t stream = (hipStream_t)stream_v;
-    // ── validation in the reference's order (terrain_correction.rs:1262-1317) ──
-    if (scene->n_grid_points <= 0 || !scene->grid_points)
-        return fail(SARDINE_ERR_TC_NO_GRID_POINTS, "no geolocation grid points provided");
-    if (!(cfg->pixel_spacing > 0.0)) return fail(SARDINE_ERR_TC_CONFIG, "pixel spacing must be positive");
-    if (scene->n_subswaths <= 0) return fail(SARDINE_ERR_TC_NO_SUBSWATHS, "scene has no subswaths");
-    const double ati = scene->azimuth_time_interval_s;
-    if (!std::isfinite(ati) || ati <= 0.0)
-        return fail(SARDINE_ERR_TC_CONFIG, "invalid azimuth_time_interval_s in subswath metadata: %g", ati);
-    SeedLut lut = build_seed_lut(scene->grid_points, scene->n_grid_points);
-    Grid g;
-    sardine_status st = output_grid(scene, cfg, &g);
+    const SeedLut lut = scene->grid_points && scene->n_grid_points > 0
+                            ? build_seed_lut(scene->grid_points, scene->n_grid_points)
+                            : SeedLut();
+    SceneDev sdv;  // device pointers are set once the scene is uploaded
+    sdv.n_lut = (int)lut.lat.size();
+    sdv.epoch = scene->orbit.count > 0 ? scene->orbit.state_vectors[0].time_us : 0;
+    TcPlan pl;
+    sardine_status st = plan_tc(merged, scene, dem, cfg, out, sdv, &pl);
     if (st) return st;
-    int anchors = cfg->projector_anchors == 0 ? 256 : cfg->projector_anchors;
-    if (anchors < 1) return fail(SARDINE_ERR_TC_CONFIG, "SARDINE_TC_PROJECTOR_ANCHORS must be >= 1 (default 256)");
-    if (anchors > sd::kMaxAnchorPts - 1)
-        return fail(SARDINE_ERR_TC_CONFIG, "projector anchors %d above the supported %d", anchors,
-                    sd::kMaxAnchorPts - 1);
-    if (cfg->max_newton_iterations < 0 || cfg->max_newton_iterations > 255)
-        return fail(SARDINE_ERR_TC_CONFIG, "max_newton_iterations %d outside 0..255", cfg->max_newton_iterations);
-    if (cfg->resampling < 0 || cfg->resampling > 2) return fail(SARDINE_ERR_TC_CONFIG, "bad resampling kernel");
-    if (cfg->precision != SARDINE_PRECISION_FP64 && cfg->precision != SARDINE_PRECISION_FP32_CENTRED)
-        return fail(SARDINE_ERR_TC_CONFIG, "bad precision selector");
-    if (scene->orbit.count > sd::kMaxSv || scene->orbit.count < 0)
-        return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", scene->orbit.count, sd::kMaxSv);
-    if (!merged->data || merged->lines <= 0 || merged->samples <= 0)
-        return fail(SARDINE_ERR_INVALID_ARGUMENT, "radar image has no raster");
-    if (!dem->data || dem->width < 2 || dem->height < 2 || !(dem->pixel_deg > 0.0))
-        return fail(SARDINE_ERR_TC_DEM, "DEM raster must be at least 2x2 with positive spacing");
-    if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 && !cfg->geoid.grid_73x144)
-        return fail(SARDINE_ERR_INVALID_ARGUMENT, "geoid grid pointer is null");
-
-    int64_t r0 = cfg->aoi_row0, c0 = cfg->aoi_col0, nr = cfg->aoi_rows, nc = cfg->aoi_cols;
-    if (nr == 0) { r0 = 0; c0 = 0; nr = g.rows; nc = g.cols; }
-    if (r0 < 0 || c0 < 0 || nr <= 0 || nc <= 0 || r0 + nr > g.rows || c0 + nc > g.cols)
-        return fail(SARDINE_ERR_TC_CONFIG, "AOI window outside the %lld x %lld output grid", (long long)g.rows,
-                    (long long)g.cols);
-    if (nr > 0x7fffffffLL) return fail(SARDINE_ERR_CAPACITY, "AOI has too many rows");
-    const bool compute_lia = cfg->compute_lia != 0;
-    if (!out->data || (compute_lia && (!out->cos_lia || !out->mask)))
-        return fail(SARDINE_ERR_INVALID_ARGUMENT, "output buffers missing");
     if ((st = require_device())) return st;
 
-    // ── orbit in device form, times relative to the first state vector ──
-    sd::OrbitDev orb;
-    std::memset(&orb, 0, sizeof orb);
-    orb.n = scene->orbit.count;
-    const int64_t epoch = orb.n > 0 ? scene->orbit.state_vectors[0].time_us : 0;
-    for (int i = 0; i < orb.n; ++i) {
-        const sardine_state_vector& s = scene->orbit.state_vectors[i];
-        orb.t_us[i] = s.time_us - epoch;
-        orb.tsec[i] = (double)(s.time_us - epoch) / 1e6;
-        for (int c = 0; c < 3; ++c) {
-            orb.pos[i][c] = s.position_m[c];
-            orb.vel[i][c] = s.velocity_m_s[c];
-        }
-    }
-    sd::orbit_fill_bary(orb);
-
-    // ── per-row constants (terrain_correction.rs:1424-1511) ──
-    const double sp = cfg->pixel_spacing;
-    const bool utm = cfg->crs != SARDINE_CRS_WGS84;
-    const bool south = cfg->crs == SARDINE_CRS_UTM_SOUTH;
-    const bool need_geom = cfg->flatten || cfg->compute_lia;
-    const double step_deg = utm ? sp / 111320.0 : sp;
-    const int n_anch = (int)std::min<int64_t>(anchors, g.cols);
-    const double x_start = g.x_left + 0.5 * sp;
-    const double x_end = x_start + ((double)g.cols - 1.0) * sp;
-    const double anchor_dx = (x_end - x_start) / (double)n_anch;
-    DevBuf d_orb, d_seed, d_flat, d_cnt, d_geoid, d_lut_lat, d_lut_t;
+    // scene context of this call: orbit and seed table on the device
+    const sd::OrbitDev orb = make_orbit_dev(scene->orbit);
+    DevBuf d_orb, d_lut_lat, d_lut_t, d_work, d_cnt;
     SD_HIP(upload(d_orb, &orb, 1, stream));
-    // per-row seeds and flat-cell areas on the device (tc_rows_kernel)
     SD_HIP(upload(d_lut_lat, lut.lat.data(), lut.lat.size(), stream));
     SD_HIP(upload(d_lut_t, lut.t.data(), lut.t.size(), stream));
-    SD_HIP(d_seed.alloc((size_t)nr * sizeof(int64_t), stream));
-    SD_HIP(d_flat.alloc((need_geom ? (size_t)nr : 1) * sizeof(double), stream));
-    {
-        sd::RowPrepParams q;
-        std::memset(&q, 0, sizeof q);
-        q.lut_lat = d_lut_lat.as<double>();
-        q.lut_t = d_lut_t.as<int64_t>();
-        q.n_lut = (int)lut.lat.size();
-        q.epoch = epoch;
-        q.y_top = g.y_top;
-        q.x_left = g.x_left;
-        q.spacing = sp;
-        q.x_start = x_start;
-        q.anchor_dx = anchor_dx;
-        q.step_deg = step_deg;
-        q.full_cols = g.cols;
-        q.row0 = r0;
-        q.rows = nr;
-        q.n_anch = n_anch;
-        q.utm = utm ? 1 : 0;
-        q.south = south ? 1 : 0;
-        q.zone = cfg->utm_zone;
-        q.need_geom = need_geom ? 1 : 0;
-        q.row_seed = d_seed.as<int64_t>();
-        q.row_flat_area = d_flat.as<double>();
-        SD_HIP(sd_launch_tc_rows(&q, stream));
-    }
-    if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5) SD_HIP(upload(d_geoid, cfg->geoid.grid_73x144, 73 * 144, stream));
+    pl.p.orbit = d_orb.as<sd::OrbitDev>();
+    pl.q.lut_lat = d_lut_lat.as<double>();
+    pl.q.lut_t = d_lut_t.as<int64_t>();
+    set_decomposition(pl.p, g_tc_seg.load());
+    const TcWork w = tc_work_bytes(pl, cfg);
+    SD_HIP(d_work.alloc(w.total(), stream));
+    assign_work(pl, w, d_work.as<char>());
+    if (w.geoid) SD_HIP(staging().copy(const_cast<float*>(pl.p.geoid), cfg->geoid.grid_73x144, 73 * 144 * sizeof(float), stream));
     SD_HIP(d_cnt.alloc((sd::kNumCounters + 1) * sizeof(unsigned long long), stream));
     SD_HIP(sd_launch_zero_u64(d_cnt.as<unsigned long long>(), sd::kNumCounters + 1, stream));
-
-    sd::TcParams p;
-    std::memset(&p, 0, sizeof p);
-    p.orbit = d_orb.as<sd::OrbitDev>();
-    p.img = merged->data;
-    p.nesz = (std::isfinite(cfg->noise_floor_margin_db) && merged->nesz) ? merged->nesz : nullptr;
-    p.lines = merged->lines;
-    p.samples = merged->samples;
-    p.az_start_us = merged->azimuth_start_time_us - epoch;
-    p.near_range_m = merged->near_slant_range_time_s * sd::kC / 2.0;
-    p.rps = merged->range_pixel_spacing_m;
-    p.ati = ati;
-    p.line_per_us = 1.0 / (1e6 * ati);
-    p.inv_rps = 1.0 / p.rps;
-    p.dem = dem->data;
-    p.dem_w = dem->width;
-    p.dem_h = dem->height;
-    p.dem_olat = dem->origin_lat_deg;
-    p.dem_olon = dem->origin_lon_deg;
-    p.dem_pix = dem->pixel_deg;
-    p.dem_inv_pix = 1.0 / dem->pixel_deg;
-    p.dem_max_row = (double)(dem->height - 1);
-    p.dem_max_col = (double)(dem->width - 1);
-    p.dem_lim_row = dem->height - 1 > 0 ? dem->height - 2 : 0;
-    p.dem_lim_col = dem->width >= 2 ? dem->width - 2 : 0;
-    p.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? d_geoid.as<float>() : nullptr;
-    p.crs = utm ? 1 : 0;
-    p.utm_zone = cfg->utm_zone;
-    p.utm_south = south ? 1 : 0;
-    p.n_anchor_pts = n_anch + 1;
-    p.spacing = sp;
-    p.lat_north = g.y_top;
-    p.lon_west = g.x_left;
-    p.x_start = x_start;
-    p.anchor_dx = anchor_dx;
-    p.inv_anchor_dx = anchor_dx > 0.0 ? 1.0 / anchor_dx : 0.0;
-    p.full_cols = g.cols;
-    p.row0 = r0;
-    p.col0 = c0;
-    p.rows = nr;
-    p.cols = nc;
-    // work decomposition: one lane per (row, segment of seg_len columns)
-    DevBuf d_anch, d_rec;
-    {
-        int64_t L = g_tc_seg.load();
-        if (L <= 0) {  // aim for ~2 waves of 64 lanes per SIMD over the whole grid
-            const int64_t target_lanes = (int64_t)256 * 4 * 2 * 64;
-            L = std::min<int64_t>(32, std::max<int64_t>(16, (nr * nc + target_lanes - 1) / target_lanes));
-        }
-        L = std::min<int64_t>(L, nc);
-        p.seg_len = (int)L;
-        p.lead = g_tc_lead.load();
-        p.nseg = (int)((nc + L - 1) / L);
-        p.rows_per_wg = 64;
-        SD_HIP(d_rec.alloc((size_t)nr * p.nseg * sizeof(sd::SegRecord), stream));
-        p.records = d_rec.as<sd::SegRecord>();
-        if (utm) {
-            SD_HIP(d_anch.alloc((size_t)nr * 2 * (n_anch + 1) * sizeof(double), stream));
-            p.anchors = d_anch.as<double>();
-        }
-    }
-    p.row_seed = d_seed.as<int64_t>();
-    p.row_flat_area = d_flat.as<double>();
-    p.max_iter = cfg->max_newton_iterations;
-    p.tol = cfg->newton_tolerance_s;
-    p.flatten = cfg->flatten ? 1 : 0;
-    p.compute_lia = compute_lia ? 1 : 0;
-    p.need_geom = need_geom ? 1 : 0;
-    p.resampling = cfg->resampling;
-    p.step_deg = step_deg;
-    p.step_sin = std::sin(step_deg * sd::kPi / 180.0);
-    p.step_cos = std::cos(step_deg * sd::kPi / 180.0);
-    // 10f32.powf(margin / 10.0) (terrain_correction.rs:1727), evaluated once
-    p.nesz_factor = std::pow(10.0f, cfg->noise_floor_margin_db / 10.0f);
-    p.out = out->data;
-    p.cos_lia = compute_lia ? out->cos_lia : nullptr;
-    p.mask = compute_lia ? out->mask : nullptr;
-    p.radar_coords = out->radar_coords;
-    p.counters = d_cnt.as<unsigned long long>();
-    p.error_flag = reinterpret_cast<int*>(d_cnt.as<unsigned long long>() + sd::kNumCounters);
-
-    if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED)
-        return fail(SARDINE_ERR_TC_CONFIG, "the FP32 scene-centred variant is not part of this build");
+    pl.p.counters = d_cnt.as<unsigned long long>();
+    pl.p.error_flag = reinterpret_cast<int*>(d_cnt.as<unsigned long long>() + sd::kNumCounters);
+    SD_HIP(sd_launch_tc_rows(&pl.q, stream));
     {
         KernelTimer kt(SARDINE_KERNEL_TC, stream);
         // the per-pixel chain kernel alone is timed as its own family
@@ -557,44 +684,14 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
             TimerSlot& t = g_slots[SARDINE_KERNEL_TC_SEGMENT];
             if ((t.a || hipEventCreate(&t.a) == hipSuccess) && (t.b || hipEventCreate(&t.b) == hipSuccess)) seg = &t;
         }
-        SD_HIP(sd_launch_tc_fp64(&p, stream, seg ? seg->a : nullptr, seg ? seg->b : nullptr));
+        SD_HIP(sd_launch_tc_fp64(&pl.p, stream, seg ? seg->a : nullptr, seg ? seg->b : nullptr));
         if (seg) seg->pending = true;
         kt.stop();
     }
-
     unsigned long long cnt[sd::kNumCounters + 1];
     SD_HIP(hipMemcpyAsync(cnt, d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost, stream));
     SD_HIP(hipStreamSynchronize(stream));
-
-    const int err = (int)(cnt[sd::kNumCounters] & 0xffffffffull);
-    if (err & 1)
-        return fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW,
-                    "orbit interpolation failed: interpolation time outside orbit window; refusing to extrapolate");
-    if (err & 2)
-        return fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT,
-                    "orbit interpolation failed: need at least 8 vectors for interpolation, have %d", orb.n);
-
-    out->rows = nr;
-    out->cols = nc;
-    out->full_rows = g.rows;
-    out->full_cols = g.cols;
-    out->geotransform[0] = g.x_left + (double)c0 * sp;
-    out->geotransform[1] = sp;
-    out->geotransform[2] = 0.0;
-    out->geotransform[3] = g.y_top - (double)r0 * sp;
-    out->geotransform[4] = 0.0;
-    out->geotransform[5] = -sp;
-    out->total_pixel_count = (uint64_t)(nr * nc);
-    out->valid_pixel_count = cnt[0];
-    out->dem_missing_count = cnt[1];
-    out->outside_footprint_count = cnt[2];
-    out->non_converged_count = cnt[3];
-    out->degenerate_geometry_count = cnt[4];
-    out->flat_masked_count = cnt[5];
-    out->noise_masked_count = cnt[6];
-    for (int i = 0; i < SARDINE_NEWTON_HIST_SIZE; ++i) out->newton_iter_histogram[i] = cnt[7 + i];
-    out->chain_redo_rounds = cnt[sd::kCntRedo];
-    return SARDINE_OK;
+    return finish_tc(pl, cnt, orb.n, out);
 }
 
 // ── speckle (speckle.rs:131-180) ─────────────────────────────────────────
@@ -607,21 +704,12 @@ sardine_status sardine_speckle_apply(const float* data, int64_t data_len, int64_
     if (data_len != expected)
         return fail(SARDINE_ERR_SPECKLE_DIMENSION, "data length %lld does not match cols x rows = %lld",
                     (long long)data_len, (long long)expected);
-    if (f->kind < SARDINE_SPECKLE_BOXCAR || f->kind > SARDINE_SPECKLE_REFINED_LEE)
-        return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown speckle filter kind %d", f->kind);
-    const int win = f->kind == SARDINE_SPECKLE_REFINED_LEE ? 7 : f->window;
-    if (win < 3 || win % 2 == 0) return fail(SARDINE_ERR_SPECKLE_BAD_WINDOW, "window size %d must be an odd integer >= 3", win);
-    if (f->kind == SARDINE_SPECKLE_LEE || f->kind == SARDINE_SPECKLE_GAMMA_MAP || f->kind == SARDINE_SPECKLE_REFINED_LEE)
-        if (!(f->enl > 0.0f) || !std::isfinite(f->enl))
-            return fail(SARDINE_ERR_SPECKLE_BAD_ENL, "ENL must be > 0; got %g", (double)f->enl);
-    if (f->kind == SARDINE_SPECKLE_FROST && (!std::isfinite(f->damping) || f->damping < 0.0f))
-        return fail(SARDINE_ERR_SPECKLE_BAD_DAMPING, "Frost damping factor must be >= 0 and finite; got %g",
-                    (double)f->damping);
-    if (win > 63) return fail(SARDINE_ERR_CAPACITY, "window %d above the supported 63", win);
+    int win = 0;
+    sardine_status st = validate_filter(f, &win);
+    if (st) return st;
     if (expected == 0) return SARDINE_OK;
     if (!data || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null raster");
     if (data == out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "speckle filtering is out-of-place: out aliases data");
-    sardine_status st;
     if ((st = require_device())) return st;
     KernelTimer kt(SARDINE_KERNEL_SPECKLE, (hipStream_t)stream);
     SD_HIP(sd_launch_speckle(f->kind, data, out, cols, rows, win / 2, f->enl, f->damping, (hipStream_t)stream));
@@ -970,6 +1058,417 @@ sardine_status sardine_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint
     sardine_status st;
     if ((st = require_device())) return st;
     SD_HIP(sd_launch_synth_cint16(out, n, seed, base, sigma, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── resident scenes and batched geocoding ────────────────────────────────
+namespace {
+
+extern "C" hipError_t sd_launch_batch_tc(const sd::BatchHeader* H, const sd::BatchJob* J, unsigned int* queue,
+                                         int n_cu, int tc_groups_per_cu, hipStream_t stream);
+extern "C" int64_t sd_speckle_tiles(int64_t cols, int64_t rows, int64_t* tiles_x);
+extern "C" size_t sd_speckle_job_size(void);
+extern "C" void sd_speckle_job_fill(void* entry, int kind, const float* in, float* out, int64_t cols, int64_t rows,
+                                    int half, float enl, float damping, int64_t tile_begin, int64_t tiles_x);
+extern "C" hipError_t sd_launch_batch_speckle(const void* H, const void* J, unsigned int* queue, int n_groups,
+                                              int max_half, hipStream_t stream);
+
+struct SpeckleBatchHeaderHost {  // layout of sd::SpeckleBatchHeader (speckle.hip)
+    int32_t n_jobs;
+    int32_t pad;
+    int64_t n_tiles;
+};
+
+}  // namespace
+
+struct sardine_resident_scene_s {
+    sardine_radar_image merged;
+    sardine_dem_raster dem;
+    sardine_scene scene;  // points into the vectors below
+    std::vector<sardine_state_vector> svs;
+    std::vector<sardine_grid_point> gps;
+    int device = 0;
+    void* d_mem = nullptr;  // orbit + seed table
+    SceneDev dev;
+};
+
+struct sardine_batch_s {
+    int max_jobs = 0;
+    bool use_graph = false;
+    int n_cu = 0;
+    int device = 0;
+    sardine_batch_tuning tuning{0, 2, 4};
+    size_t sp_job_size = 0;
+    // device tables (one allocation) and their pinned host images
+    void* d_mem = nullptr;
+    sd::BatchHeader* d_H = nullptr;
+    sd::BatchJob* d_J = nullptr;
+    void* d_SH = nullptr;
+    void* d_SJ = nullptr;
+    unsigned long long* d_cnt = nullptr;  // max_jobs x (kNumCounters + 1)
+    unsigned long long* d_queue = nullptr;  // [0] chain items, [1] speckle tiles
+    void* h_mem = nullptr;
+    sd::BatchHeader* h_H = nullptr;
+    sd::BatchJob* h_J = nullptr;
+    SpeckleBatchHeaderHost* h_SH = nullptr;
+    char* h_SJ = nullptr;
+    unsigned long long* h_cnt = nullptr;
+    hipStream_t work = nullptr;  // graph replays run here, ordered against the caller's stream by events
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraphExec_t exec = nullptr;
+    sardine_batch_tuning graph_tuning{-1, -1, -1};
+    int graph_half = -1;
+};
+
+namespace {
+
+size_t cnt_stride() { return sd::kNumCounters + 1; }
+
+// The launch sequence of one run (eager or under capture).
+hipError_t batch_launches(sardine_batch_s* b, int max_half, hipStream_t s) {
+    hipError_t e;
+    if ((e = sd_launch_zero_u64(b->d_cnt, (int64_t)b->max_jobs * cnt_stride(), s)) != hipSuccess) return e;
+    if ((e = sd_launch_zero_u64(b->d_queue, 2, s)) != hipSuccess) return e;
+    unsigned int* q = reinterpret_cast<unsigned int*>(b->d_queue);
+    if ((e = sd_launch_batch_tc(b->d_H, b->d_J, q, b->n_cu, b->tuning.tc_groups_per_cu, s)) != hipSuccess) return e;
+    return sd_launch_batch_speckle(b->d_SH, b->d_SJ, q + 2, b->n_cu * b->tuning.speckle_groups_per_cu, max_half, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_scene_create(const sardine_radar_image* merged, const sardine_scene* scene,
+                                    const sardine_dem_raster* dem, sardine_resident_scene* out) {
+    if (!merged || !scene || !dem || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (scene->n_grid_points <= 0 || !scene->grid_points)
+        return fail(SARDINE_ERR_TC_NO_GRID_POINTS, "no geolocation grid points provided");
+    if (scene->orbit.count > sd::kMaxSv || scene->orbit.count < 0 || (scene->orbit.count > 0 && !scene->orbit.state_vectors))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", scene->orbit.count, sd::kMaxSv);
+    sardine_status st;
+    if ((st = require_device())) return st;
+    std::unique_ptr<sardine_resident_scene_s> r(new sardine_resident_scene_s());
+    r->merged = *merged;
+    r->dem = *dem;
+    r->scene = *scene;
+    r->svs.assign(scene->orbit.state_vectors, scene->orbit.state_vectors + scene->orbit.count);
+    r->gps.assign(scene->grid_points, scene->grid_points + scene->n_grid_points);
+    r->scene.orbit.state_vectors = r->svs.data();
+    r->scene.grid_points = r->gps.data();
+    SD_HIP(hipGetDevice(&r->device));
+    const sd::OrbitDev orb = make_orbit_dev(r->scene.orbit);
+    const SeedLut lut = build_seed_lut(r->scene.grid_points, r->scene.n_grid_points);
+    const size_t n = lut.lat.size();
+    const size_t bytes = round256(sizeof orb) + round256(n * sizeof(double)) + round256(n * sizeof(int64_t));
+    SD_HIP(hipMalloc(&r->d_mem, bytes));
+    char* base = static_cast<char*>(r->d_mem);
+    SD_HIP(hipMemcpy(base, &orb, sizeof orb, hipMemcpyHostToDevice));
+    SD_HIP(hipMemcpy(base + round256(sizeof orb), lut.lat.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    SD_HIP(hipMemcpy(base + round256(sizeof orb) + round256(n * sizeof(double)), lut.t.data(), n * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+    r->dev.orb = reinterpret_cast<const sd::OrbitDev*>(base);
+    r->dev.lut_lat = reinterpret_cast<const double*>(base + round256(sizeof orb));
+    r->dev.lut_t = reinterpret_cast<const int64_t*>(base + round256(sizeof orb) + round256(n * sizeof(double)));
+    r->dev.n_lut = (int)n;
+    r->dev.epoch = orb.n > 0 ? r->svs[0].time_us : 0;
+    *out = r.release();
+    return SARDINE_OK;
+}
+
+void sardine_scene_destroy(sardine_resident_scene r) {
+    if (!r) return;
+    if (r->d_mem) (void)hipFree(r->d_mem);
+    delete r;
+}
+
+sardine_status sardine_batch_create(int32_t max_jobs, int32_t use_graph, sardine_batch* out) {
+    if (!out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (max_jobs < 1 || max_jobs > 65536) return fail(SARDINE_ERR_CAPACITY, "max_jobs %d outside 1..65536", max_jobs);
+    sardine_status st;
+    if ((st = require_device())) return st;
+    std::unique_ptr<sardine_batch_s> b(new sardine_batch_s());
+    b->max_jobs = max_jobs;
+    b->use_graph = use_graph != 0;
+    SD_HIP(hipGetDevice(&b->device));
+    hipDeviceProp_t prop;
+    SD_HIP(hipGetDeviceProperties(&prop, b->device));
+    b->n_cu = prop.multiProcessorCount;
+    b->sp_job_size = sd_speckle_job_size();
+    const size_t szH = round256(sizeof(sd::BatchHeader)), szJ = round256((size_t)max_jobs * sizeof(sd::BatchJob));
+    const size_t szSH = round256(sizeof(SpeckleBatchHeaderHost)), szSJ = round256((size_t)max_jobs * b->sp_job_size);
+    const size_t szC = round256((size_t)max_jobs * cnt_stride() * sizeof(unsigned long long));
+    const size_t szQ = 256;
+    SD_HIP(hipMalloc(&b->d_mem, szH + szJ + szSH + szSJ + szC + szQ));
+    char* d = static_cast<char*>(b->d_mem);
+    b->d_H = reinterpret_cast<sd::BatchHeader*>(d);
+    b->d_J = reinterpret_cast<sd::BatchJob*>(d + szH);
+    b->d_SH = d + szH + szJ;
+    b->d_SJ = d + szH + szJ + szSH;
+    b->d_cnt = reinterpret_cast<unsigned long long*>(d + szH + szJ + szSH + szSJ);
+    b->d_queue = reinterpret_cast<unsigned long long*>(d + szH + szJ + szSH + szSJ + szC);
+    SD_HIP(hipHostMalloc(&b->h_mem, szH + szJ + szSH + szSJ + szC, hipHostMallocDefault));
+    char* h = static_cast<char*>(b->h_mem);
+    b->h_H = reinterpret_cast<sd::BatchHeader*>(h);
+    b->h_J = reinterpret_cast<sd::BatchJob*>(h + szH);
+    b->h_SH = reinterpret_cast<SpeckleBatchHeaderHost*>(h + szH + szJ);
+    b->h_SJ = h + szH + szJ + szSH;
+    b->h_cnt = reinterpret_cast<unsigned long long*>(h + szH + szJ + szSH + szSJ);
+    SD_HIP(hipStreamCreateWithFlags(&b->work, hipStreamNonBlocking));
+    SD_HIP(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+    SD_HIP(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
+    SD_HIP(hipEventCreate(&b->t0));
+    SD_HIP(hipEventCreate(&b->t1));
+    *out = b.release();
+    return SARDINE_OK;
+}
+
+void sardine_batch_destroy(sardine_batch b) {
+    if (!b) return;
+    if (b->work) (void)hipStreamSynchronize(b->work);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (hipEvent_t e : {b->ev_in, b->ev_out, b->t0, b->t1})
+        if (e) (void)hipEventDestroy(e);
+    if (b->work) (void)hipStreamDestroy(b->work);
+    if (b->d_mem) (void)hipFree(b->d_mem);
+    if (b->h_mem) (void)hipHostFree(b->h_mem);
+    delete b;
+}
+
+sardine_status sardine_batch_set_tuning(sardine_batch b, const sardine_batch_tuning* t) {
+    if (!b || !t) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    if (t->segment_columns < 0 || t->segment_columns > 4096 || t->tc_groups_per_cu < 1 || t->tc_groups_per_cu > 8 ||
+        t->speckle_groups_per_cu < 1 || t->speckle_groups_per_cu > 16)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "tuning out of range");
+    b->tuning = *t;
+    return SARDINE_OK;
+}
+
+sardine_status sardine_batch_get_tuning(sardine_batch b, sardine_batch_tuning* t) {
+    if (!b || !t) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    *t = b->tuning;
+    return SARDINE_OK;
+}
+
+sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                 sardine_batch_job* jobs, int32_t n_jobs, void* stream_v, double* device_ms) {
+    if (!b || (n_jobs > 0 && (!jobs || !scenes))) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_jobs < 0 || n_jobs > b->max_jobs)
+        return fail(SARDINE_ERR_CAPACITY, "%d jobs for a batch of at most %d", n_jobs, b->max_jobs);
+    hipStream_t stream = (hipStream_t)stream_v;
+    sardine_status st, first = SARDINE_OK;
+    std::string first_msg;
+    // plan every job; failures are recorded per job and the job is left out
+    std::vector<TcPlan> plans((size_t)n_jobs);
+    std::vector<TcWork> works((size_t)n_jobs);
+    std::vector<int> run_idx;
+    std::vector<int> halves((size_t)n_jobs, 0);
+    size_t arena = 0;
+    for (int i = 0; i < n_jobs; ++i) {
+        sardine_batch_job& jb = jobs[i];
+        st = SARDINE_OK;
+        if (jb.scene < 0 || jb.scene >= n_scenes || !scenes[jb.scene] || !jb.out)
+            st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: bad scene index or output", i);
+        const sardine_resident_scene_s* r = st ? nullptr : scenes[jb.scene];
+        if (!st && r->device != b->device) st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: scene on another device", i);
+        if (!st) st = plan_tc(&r->merged, &r->scene, &r->dem, &jb.config, jb.out, r->dev, &plans[(size_t)i]);
+        if (!st && jb.speckle.kind >= 0) {
+            int win = 0;
+            st = validate_filter(&jb.speckle, &win);
+            if (!st && (!jb.speckle_out || jb.speckle_out == jb.out->data))
+                st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: speckle output missing or aliasing the data", i);
+            halves[(size_t)i] = win / 2;
+        }
+        jb.status = st;
+        if (st) {
+            if (!first) {
+                first = st;
+                first_msg = g_last_error;
+            }
+            continue;
+        }
+        set_decomposition(plans[(size_t)i].p, b->tuning.segment_columns);
+        works[(size_t)i] = tc_work_bytes(plans[(size_t)i], &jb.config);
+        arena += works[(size_t)i].total();
+        run_idx.push_back(i);
+    }
+    if (run_idx.empty()) {
+        if (first) g_last_error = first_msg;
+        return first;
+    }
+    if ((st = require_device())) return st;
+    DevBuf d_work;
+    SD_HIP(d_work.alloc(arena, stream));
+    char* base = d_work.as<char>();
+    // job tables
+    sd::BatchHeader& H = *b->h_H;
+    std::memset(&H, 0, sizeof H);
+    SpeckleBatchHeaderHost& SH = *b->h_SH;
+    std::memset(&SH, 0, sizeof SH);
+    int max_half = 1;
+    int k = 0;
+    for (int i : run_idx) {
+        TcPlan& pl = plans[(size_t)i];
+        assign_work(pl, works[(size_t)i], base);
+        if (works[(size_t)i].geoid)
+            SD_HIP(staging().copy(const_cast<float*>(pl.p.geoid), jobs[i].config.geoid.grid_73x144,
+                                  73 * 144 * sizeof(float), stream));
+        base += works[(size_t)i].total();
+        pl.p.counters = b->d_cnt + (size_t)k * cnt_stride();
+        pl.p.error_flag = reinterpret_cast<int*>(pl.p.counters + sd::kNumCounters);
+        sd::BatchJob& J = b->h_J[k];
+        std::memset(&J, 0, sizeof J);
+        J.p = pl.p;
+        J.q = pl.q;
+        J.cls = pl.p.resampling * 2 + (pl.utm ? 1 : 0);
+        J.tc_row_blocks = (pl.nr + 63) / 64;
+        J.tc_begin = H.n_tc;
+        H.n_tc += J.tc_row_blocks * ((pl.p.nseg + 3) / 4);
+        J.fix_begin = H.n_fix;
+        H.n_fix += (pl.nr + 255) / 256;
+        J.row_begin = H.n_rows;
+        H.n_rows += pl.nr;
+        J.anchor_begin = H.n_anchor;
+        if (pl.utm) H.n_anchor += pl.nr * pl.p.n_anchor_pts;
+        // speckle entry (a job without a filter contributes no tiles)
+        const sardine_batch_job& jb = jobs[i];
+        int64_t tiles_x = 1, tiles = 0;
+        if (jb.speckle.kind >= 0) {
+            tiles = sd_speckle_tiles(pl.nc, pl.nr, &tiles_x);
+            max_half = std::max(max_half, halves[(size_t)i]);
+        }
+        sd_speckle_job_fill(b->h_SJ + (size_t)k * b->sp_job_size, jb.speckle.kind >= 0 ? jb.speckle.kind : 0,
+                            jb.out->data, jb.speckle_out, pl.nc, pl.nr, halves[(size_t)i], jb.speckle.enl,
+                            jb.speckle.damping, SH.n_tiles, tiles_x);
+        SH.n_tiles += tiles;
+        ++k;
+    }
+    H.n_jobs = k;
+    SH.n_jobs = k;
+    if (H.n_tc >= 0xffffffffLL || SH.n_tiles >= 0xffffffffLL)
+        return fail(SARDINE_ERR_CAPACITY, "batch work above 2^32 items");
+    // run: tables in, launches, counters out.  Graph replays go on the
+    // batch's own stream, ordered after the caller's work by events.
+    hipStream_t s = b->use_graph ? b->work : stream;
+    if (b->use_graph) {
+        SD_HIP(hipEventRecord(b->ev_in, stream));
+        SD_HIP(hipStreamWaitEvent(s, b->ev_in, 0));
+    }
+    SD_HIP(hipMemcpyAsync(b->d_H, b->h_H, sizeof(sd::BatchHeader), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(b->d_J, b->h_J, (size_t)k * sizeof(sd::BatchJob), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(b->d_SH, b->h_SH, sizeof(SpeckleBatchHeaderHost), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(b->d_SJ, b->h_SJ, (size_t)k * b->sp_job_size, hipMemcpyHostToDevice, s));
+    SD_HIP(hipEventRecord(b->t0, s));
+    if (b->use_graph) {
+        const sardine_batch_tuning& t = b->tuning;
+        const sardine_batch_tuning& g = b->graph_tuning;
+        if (!b->exec || g.tc_groups_per_cu != t.tc_groups_per_cu || g.speckle_groups_per_cu != t.speckle_groups_per_cu ||
+            b->graph_half != max_half) {
+            if (b->exec) {
+                SD_HIP(hipGraphExecDestroy(b->exec));
+                b->exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            SD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = batch_launches(b, max_half, s);
+            const hipError_t ce = hipStreamEndCapture(s, &graph);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(SARDINE_ERR_HIP, "graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(SARDINE_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+            b->graph_tuning = t;
+            b->graph_half = max_half;
+        }
+        SD_HIP(hipGraphLaunch(b->exec, s));
+    } else {
+        SD_HIP(batch_launches(b, max_half, s));
+    }
+    SD_HIP(hipEventRecord(b->t1, s));
+    SD_HIP(hipMemcpyAsync(b->h_cnt, b->d_cnt, (size_t)k * cnt_stride() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    if (b->use_graph) {
+        SD_HIP(hipEventRecord(b->ev_out, s));
+        SD_HIP(hipStreamWaitEvent(stream, b->ev_out, 0));  // later work on the caller's stream sees the outputs
+    }
+    SD_HIP(hipStreamSynchronize(s));
+    if (device_ms) {
+        float ms = 0.0f;
+        SD_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+        *device_ms = ms;
+    }
+    k = 0;
+    for (int i : run_idx) {
+        st = finish_tc(plans[(size_t)i], b->h_cnt + (size_t)k * cnt_stride(), scenes[jobs[i].scene]->scene.orbit.count,
+                       jobs[i].out);
+        jobs[i].status = st;
+        if (st && !first) {
+            first = st;
+            first_msg = g_last_error;
+        }
+        ++k;
+    }
+    if (first) g_last_error = first_msg;
+    return first;
+}
+
+sardine_status sardine_batch_autotune(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                      sardine_batch_job* jobs, int32_t n_jobs, void* stream,
+                                      sardine_batch_tuning* best, double* best_ms) {
+    if (!b) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    static const int segs[] = {16, 24, 32, 48};
+    static const int tcg[] = {1, 2};
+    static const int spg[] = {2, 4, 8};
+    const sardine_batch_tuning keep = b->tuning;
+    sardine_batch_tuning chosen = keep;
+    double chosen_ms = 1e30;
+    sardine_status st;
+    auto measure = [&](const sardine_batch_tuning& t, double* ms) -> sardine_status {
+        b->tuning = t;
+        double a = 0.0, c = 0.0;
+        sardine_status s1 = sardine_batch_run(b, scenes, n_scenes, jobs, n_jobs, stream, &a);
+        if (s1) return s1;
+        s1 = sardine_batch_run(b, scenes, n_scenes, jobs, n_jobs, stream, &c);
+        if (s1) return s1;
+        *ms = std::min(a, c);
+        return SARDINE_OK;
+    };
+    // chain pass shape first (segment length x workgroups per CU), then the
+    // speckle pass's workgroups per CU with the chosen chain shape
+    for (int sg : segs)
+        for (int g : tcg) {
+            sardine_batch_tuning t{sg, g, keep.speckle_groups_per_cu};
+            double ms = 0.0;
+            if ((st = measure(t, &ms))) {
+                b->tuning = keep;
+                return st;
+            }
+            if (ms < chosen_ms) {
+                chosen_ms = ms;
+                chosen = t;
+            }
+        }
+    for (int g : spg) {
+        sardine_batch_tuning t{chosen.segment_columns, chosen.tc_groups_per_cu, g};
+        if (g == chosen.speckle_groups_per_cu) continue;
+        double ms = 0.0;
+        if ((st = measure(t, &ms))) {
+            b->tuning = keep;
+            return st;
+        }
+        if (ms < chosen_ms) {
+            chosen_ms = ms;
+            chosen = t;
+        }
+    }
+    b->tuning = chosen;
+    if (best) *best = chosen;
+    if (best_ms) *best_ms = chosen_ms;
     return SARDINE_OK;
 }
 

@@ -274,12 +274,9 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
 }
 
 template <int KIND, int HALF>
-__device__ __forceinline__ void speckle_tile(const SpeckleParams& p) {
-    extern __shared__ float tile[];
+__device__ __forceinline__ void speckle_tile(const SpeckleParams& p, float* tile, int64_t r0, int64_t c0) {
     const int half = KIND == SK_RLEE ? 3 : (HALF > 0 ? HALF : p.half);
     const int pitch = kTileW + 2 * half;
-    const int64_t c0 = (int64_t)blockIdx.x * kTileW;
-    const int64_t r0 = (int64_t)blockIdx.y * kTileH;
     stage_tile(p, tile, half, r0, c0);
     __syncthreads();
     const int tx = threadIdx.x & (kTileW - 1);
@@ -303,13 +300,76 @@ __device__ __forceinline__ void speckle_tile(const SpeckleParams& p) {
 
 template <int KIND, int HALF>
 __global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
-    speckle_tile<KIND, HALF>(p);
+    extern __shared__ float tile[];
+    speckle_tile<KIND, HALF>(p, tile, (int64_t)blockIdx.y * kTileH, (int64_t)blockIdx.x * kTileW);
 }
 
 // Refined Lee keeps 16 f64 accumulators per pixel; 4 waves per SIMD fit
 // without spilling.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void refined_lee_kernel(SpeckleParams p) {
-    speckle_tile<SK_RLEE, 3>(p);
+    extern __shared__ float tile[];
+    speckle_tile<SK_RLEE, 3>(p, tile, (int64_t)blockIdx.y * kTileH, (int64_t)blockIdx.x * kTileW);
+}
+
+// ── batched filtering (speckle steps of a batch of geocoded tiles) ──────
+
+// One job: a raster and its filter.  Tiles of all jobs form one index
+// space; tile_begin is the job's first tile, tiles_x its tiles per row.
+struct SpeckleJob {
+    SpeckleParams p;
+    int32_t kind;
+    int32_t pad;
+    int64_t tile_begin, tiles_x;
+};
+
+struct SpeckleBatchHeader {
+    int32_t n_jobs;
+    int32_t pad;
+    int64_t n_tiles;
+};
+
+template <int KIND>
+__device__ __forceinline__ void batch_tile_kind(const SpeckleParams& p, float* tile, int64_t r0, int64_t c0) {
+    switch (p.half) {
+        case 1: speckle_tile<KIND, 1>(p, tile, r0, c0); break;
+        case 2: speckle_tile<KIND, 2>(p, tile, r0, c0); break;
+        case 3: speckle_tile<KIND, 3>(p, tile, r0, c0); break;
+        default: speckle_tile<KIND, 0>(p, tile, r0, c0); break;
+    }
+}
+
+// Persistent: workgroups pull tiles of any job from `queue` (zero at start);
+// the LDS tile is sized for the largest window of the batch.
+__global__ __launch_bounds__(256) void batch_speckle_kernel(const SpeckleBatchHeader* __restrict__ H,
+                                                            const SpeckleJob* __restrict__ J, unsigned int* queue) {
+    extern __shared__ float tile[];
+    __shared__ unsigned int s_item;
+    const int n_jobs = H->n_jobs;
+    const int64_t n = H->n_tiles;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const int64_t item = (int64_t)s_item;
+        if (item >= n) break;
+        int lo = 0, hi = n_jobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (J[mid].tile_begin <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const int j = __builtin_amdgcn_readfirstlane(lo);
+        const SpeckleJob& job = J[j];
+        const int64_t t = item - job.tile_begin;
+        const int64_t r0 = (t / job.tiles_x) * kTileH, c0 = (t % job.tiles_x) * kTileW;
+        switch (job.kind) {
+            case SK_BOXCAR: batch_tile_kind<SK_BOXCAR>(job.p, tile, r0, c0); break;
+            case SK_LEE: batch_tile_kind<SK_LEE>(job.p, tile, r0, c0); break;
+            case SK_FROST: batch_tile_kind<SK_FROST>(job.p, tile, r0, c0); break;
+            case SK_GAMMA: batch_tile_kind<SK_GAMMA>(job.p, tile, r0, c0); break;
+            default: speckle_tile<SK_RLEE, 3>(job.p, tile, r0, c0); break;
+        }
+        __syncthreads();  // the tile and the item slot are rewritten next
+    }
 }
 
 template <int KIND, int HALF>
@@ -346,4 +406,31 @@ extern "C" hipError_t sd_launch_speckle(int kind, const float* in, float* out, i
         case sd::SK_GAMMA: return sd::launch_kind<sd::SK_GAMMA>(p, stream);
         default: p.half = 3; return sd::launch_one<sd::SK_RLEE, 3>(p, stream);
     }
+}
+
+// Tiles per job for the batched pass (host helper for the job table).
+extern "C" int64_t sd_speckle_tiles(int64_t cols, int64_t rows, int64_t* tiles_x) {
+    *tiles_x = (cols + sd::kTileW - 1) / sd::kTileW;
+    return *tiles_x * ((rows + sd::kTileH - 1) / sd::kTileH);
+}
+
+extern "C" size_t sd_speckle_job_size(void) { return sizeof(sd::SpeckleJob); }
+
+// Fills one job-table entry (kept here so the host needs no kernel types).
+extern "C" void sd_speckle_job_fill(void* entry, int kind, const float* in, float* out, int64_t cols, int64_t rows,
+                                    int half, float enl, float damping, int64_t tile_begin, int64_t tiles_x) {
+    sd::SpeckleJob& j = *static_cast<sd::SpeckleJob*>(entry);
+    j.p = sd::SpeckleParams{in, out, cols, rows, kind == sd::SK_RLEE ? 3 : half, enl, damping};
+    j.kind = kind;
+    j.pad = 0;
+    j.tile_begin = tile_begin;
+    j.tiles_x = tiles_x;
+}
+
+extern "C" hipError_t sd_launch_batch_speckle(const void* H, const void* J, unsigned int* queue, int n_groups,
+                                              int max_half, hipStream_t stream) {
+    const size_t lds = (size_t)(sd::kTileW + 2 * max_half) * (sd::kTileH + 2 * max_half) * sizeof(float);
+    hipLaunchKernelGGL(sd::batch_speckle_kernel, dim3((unsigned)n_groups), dim3(256), lds, stream,
+                       static_cast<const sd::SpeckleBatchHeader*>(H), static_cast<const sd::SpeckleJob*>(J), queue);
+    return hipGetLastError();
 }

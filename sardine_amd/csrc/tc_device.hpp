@@ -1,0 +1,571 @@
+// tc_device.hpp — device pieces of terrain correction shared by the
+// per-call kernels (tc.hip) and the batched kernels (tc_batch.hip): per-pixel
+// preparation, Newton solve, resampling and geometry, output staging, the
+// per-row seed and anchor computations, and the chain bodies of the segment
+// and fix-up passes.  Reference: sardine/src/terrain_correction.rs:1245-1836.
+#pragma once
+
+#include "geom.hpp"
+#include "tc_params.hpp"
+#include "tc_pixel.hpp"
+
+namespace sd {
+
+// Outcome categories of one pixel (the counters of GeocodedImage).
+enum : int { CAT_VALID = 0, CAT_DEM = 1, CAT_OUT = 2, CAT_NC = 3, CAT_DEG = 4, CAT_FLAT = 5, CAT_NOISE = 6,
+             CAT_ORBIT = 7 };
+
+struct LaneCounts {
+    int c[7];
+    int h1, h2, h3;  // histogram bins 1..3 (the rest go straight to global counters)
+    int err_outside, err_short;
+};
+
+__device__ __forceinline__ void count(LaneCounts& L, int cat, int it, int sign, unsigned long long* g) {
+    switch (cat) {
+        case CAT_VALID: L.c[0] += sign; break;
+        case CAT_DEM: L.c[1] += sign; break;
+        case CAT_OUT: L.c[2] += sign; break;
+        case CAT_NC: L.c[3] += sign; break;
+        case CAT_DEG: L.c[4] += sign; break;
+        case CAT_FLAT: L.c[5] += sign; break;
+        case CAT_NOISE: L.c[6] += sign; break;
+        default: break;
+    }
+    if (cat == CAT_VALID || cat == CAT_OUT || cat == CAT_FLAT || cat == CAT_NOISE) {
+        if (it == 1) L.h1 += sign;
+        else if (it == 2) L.h2 += sign;
+        else if (it == 3) L.h3 += sign;
+        else atomicAdd(&g[7 + (it < kHistSize - 1 ? it : kHistSize - 1)], (unsigned long long)(long long)sign);
+    }
+}
+
+struct PixelPos {
+    double lat, lon;
+    double sl, cl, so, co;  // sines/cosines of lat/lon (radians of the degrees)
+    double n;               // prime-vertical radius at lat
+    d3 target;
+    bool dem_ok;
+};
+
+// Output-grid position of a column and its DEM/ECEF target.
+template <bool UTM>
+__device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, double row_y, const double* alon,
+                                               const double* alat) {
+    PixelPos q;
+    if constexpr (!UTM) {
+        q.lon = p.lon_west + ((double)col + 0.5) * p.spacing;
+        q.lat = row_y;
+    } else {  // Projector::project_row blend (output_crs.rs:395-414)
+        const double x = p.x_start + (double)col * p.spacing;
+        const int na = p.n_anchor_pts - 1;
+        int seg = (int)floor((x - p.x_start) * p.inv_anchor_dx);  // a guess: the walk below decides
+        seg = seg < 0 ? 0 : (seg > na - 1 ? na - 1 : seg);
+        while (seg > 0 && !(x > p.x_start + (double)seg * p.anchor_dx)) --seg;
+        while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
+        const double x0 = p.x_start + (double)seg * p.anchor_dx, x1 = p.x_start + (double)(seg + 1) * p.anchor_dx;
+        const double t = (x1 == x0) ? 0.0 : (x - x0) / (x1 - x0);
+        q.lon = alon[seg] + t * (alon[seg + 1] - alon[seg]);
+        q.lat = alat[seg] + t * (alat[seg + 1] - alat[seg]);
+    }
+    float hf = 0.0f;
+    q.dem_ok = dem_sample(p, q.lat, q.lon, &hf) && isfinite(hf);
+    q.target = {0.0, 0.0, 0.0};
+    q.sl = q.cl = q.so = q.co = q.n = 0.0;
+    if (q.dem_ok) {
+        sincos_deg(q.lat, &q.sl, &q.cl);
+        sincos_deg(q.lon, &q.so, &q.co);
+        q.n = prime_vertical(q.sl);
+        q.target = ecef_from_n(q.n, q.sl, q.cl, q.so, q.co, (double)hf + geoid_n(p, q.lat, q.lon));
+    }
+    return q;
+}
+
+struct PixelOut {
+    float sigma0, cos_lia;
+    double line, sample;
+    int cat;
+    unsigned char mask;
+};
+
+// Everything after the Newton solve (terrain_correction.rs:1558-1739).
+template <int RS>
+__device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelPos& q, int oc, int64_t t, d3 sat,
+                                                 double flat_area) {
+    PixelOut o;
+    o.sigma0 = __builtin_nanf("");
+    o.cos_lia = __builtin_nanf("");
+    o.line = __builtin_nan("");
+    o.sample = __builtin_nan("");
+    o.mask = 1;  // OUTSIDE_FOOTPRINT until decided
+    if (oc == ZD_ORBIT_OUTSIDE || oc == ZD_ORBIT_INSUFFICIENT) {
+        o.cat = CAT_ORBIT;
+        return o;
+    }
+    if (oc == ZD_NOT_CONVERGED) {
+        o.cat = CAT_NC;
+        o.mask = 3;
+        return o;
+    }
+    if (oc == ZD_DEGENERATE) {
+        o.cat = CAT_DEG;
+        o.mask = 4;
+        return o;
+    }
+    d3 rv = q.target - sat;
+    const double slant = sqrt(dot(rv, rv));
+    const double line = (double)(t - p.az_start_us) * p.line_per_us;
+    const double sample = (slant - p.near_range_m) * p.inv_rps;
+    o.line = line;
+    o.sample = sample;
+    if (line < -0.5 || line > (double)p.lines - 0.5 || sample < -0.5 || sample > (double)p.samples - 0.5) {
+        o.cat = CAT_OUT;
+        return o;
+    }
+    float sigma0 = resample<RS>(p, line, sample);
+    if (isnan(sigma0)) {
+        o.cat = CAT_OUT;
+        return o;
+    }
+    if (p.need_geom) {
+        const double st = p.step_deg;
+        float he = 0.0f, hw = 0.0f, hn = 0.0f, hs = 0.0f;
+        const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
+        bool ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon + st), &he);
+        ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon - st), &hw) && ok;
+        ok = dem_bilinear(p, dem_row_axis(p, q.lat + st), cc, &hn) && ok;
+        ok = dem_bilinear(p, dem_row_axis(p, q.lat - st), cc, &hs) && ok;
+        if (ok) ok = isfinite(he) && isfinite(hw) && isfinite(hn) && isfinite(hs);
+        if (!ok) {
+            o.mask = 9;
+            if (p.flatten) {
+                o.cat = CAT_FLAT;
+                return o;
+            }
+        } else {
+            double w = 0.0, cl = 0.0;
+            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, he, hw, hn, hs, sat, q.target,
+                                        flat_area, &w, &cl);
+            if (g == TG_VALID) {
+                o.cos_lia = (float)cl;
+                if (p.flatten) sigma0 /= (float)w;
+            } else if (g == TG_FORESHORTENING || g == TG_LAYOVER) {
+                o.cos_lia = (float)cl;
+                o.mask = g == TG_LAYOVER ? 6 : 7;
+                if (p.flatten) {
+                    o.cat = CAT_FLAT;
+                    return o;
+                }
+            } else {
+                o.mask = 5;
+                if (p.flatten) {
+                    o.cat = CAT_FLAT;
+                    return o;
+                }
+            }
+        }
+    }
+    if (p.nesz) {
+        float nz = bilinear(p.nesz, p.lines, p.samples, line, sample);
+        if (!isnan(nz) && sigma0 <= nz * p.nesz_factor) {
+            o.cat = CAT_NOISE;
+            o.mask = 8;
+            return o;
+        }
+    }
+    o.sigma0 = sigma0;
+    o.mask = 0;
+    o.cat = CAT_VALID;
+    return o;
+}
+
+__device__ __forceinline__ void write_pixel(const TcParams& p, int64_t o, const PixelOut& v) {
+    p.out[o] = v.sigma0;
+    if (p.compute_lia) {
+        p.cos_lia[o] = v.cos_lia;
+        p.mask[o] = v.mask;
+    }
+    if (p.radar_coords) {
+        p.radar_coords[2 * o] = v.line;
+        p.radar_coords[2 * o + 1] = v.sample;
+    }
+}
+
+__device__ __forceinline__ void write_dem_missing(const TcParams& p, int64_t o) {
+    p.out[o] = __builtin_nanf("");
+    if (p.compute_lia) {
+        p.cos_lia[o] = __builtin_nanf("");
+        p.mask[o] = 2;
+    }
+    if (p.radar_coords) {
+        p.radar_coords[2 * o] = __builtin_nan("");
+        p.radar_coords[2 * o + 1] = __builtin_nan("");
+    }
+}
+
+// Newton from `seed` for one pixel (dem_ok pixels only).
+struct Solve {
+    int oc, it;
+    int64_t t;
+    d3 sat;
+};
+
+__device__ __forceinline__ Solve solve(const TcParams& p, const OrbitDev& orb, const PixelPos& q, int64_t seed,
+                                       int max_iter, OrbitCache& cache) {
+    Solve s;
+    s.t = 0;
+    s.it = 0;
+    s.sat = {0.0, 0.0, 0.0};
+    s.oc = zero_doppler(orb, q.target, seed, max_iter, p.tol, &s.t, &s.it, &s.sat, &cache);
+    return s;
+}
+
+__device__ __forceinline__ void zero_counts(LaneCounts& C) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) C.c[i] = 0;
+    C.h1 = C.h2 = C.h3 = 0;
+    C.err_outside = C.err_short = 0;
+}
+
+// Wave-reduce each lane's counts and add them to the global counters.
+__device__ __forceinline__ void flush_counts(const TcParams& p, const LaneCounts& C, bool set_errors) {
+    unsigned long long* G = p.counters;
+    const int lane = threadIdx.x & 63;
+    int v[10] = {C.c[0], C.c[1], C.c[2], C.c[3], C.c[4], C.c[5], C.c[6], C.h1, C.h2, C.h3};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        long long x = v[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if (lane == 0 && x) atomicAdd(&G[i < 7 ? i : 7 + (i - 6)], (unsigned long long)x);
+    }
+    if (set_errors && C.err_outside > 0) atomicOr(p.error_flag, 1);
+    if (set_errors && C.err_short > 0) atomicOr(p.error_flag, 2);
+}
+
+__device__ __forceinline__ void load_orbit(const TcParams& p, OrbitDev& orb) {
+    const uint4* src = reinterpret_cast<const uint4*>(p.orbit);
+    uint4* dst = reinterpret_cast<uint4*>(&orb);
+    for (int i = threadIdx.x; i < (int)(sizeof(OrbitDev) / sizeof(uint4)); i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+// guess_azimuth_time (terrain_correction.rs:1107-1136): linear
+// interpolation in the latitude-sorted grid-point table, clamped at both
+// ends; an exact latitude hit returns that entry.
+__device__ __forceinline__ int64_t guess_azimuth_time(const RowPrepParams& q, double lat) {
+    const int n = q.n_lut;
+    if (n == 1) return q.lut_t[0];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        const double v = q.lut_lat[mid];
+        if (v < lat) lo = mid + 1;
+        else if (v > lat) hi = mid;
+        else return q.lut_t[mid];
+    }
+    if (lo == 0) return q.lut_t[0];
+    if (lo >= n) return q.lut_t[n - 1];
+    const double frac = (lat - q.lut_lat[lo - 1]) / (q.lut_lat[lo] - q.lut_lat[lo - 1]);
+    const double dt_us = (double)(q.lut_t[lo] - q.lut_t[lo - 1]);
+    return q.lut_t[lo - 1] + (int64_t)(frac * dt_us);
+}
+
+// One lane per AOI row: the row's Newton seed (the grid-point guess at the
+// row's seed latitude) and, for the terrain geometry, the flat-cell area at
+// (seed latitude, centre longitude) (terrain_correction.rs:1424-1511).
+__device__ __forceinline__ void row_prep(const RowPrepParams& q, int64_t rr) {
+    const int64_t row = q.row0 + rr;
+    const double row_y = q.y_top - ((double)row + 0.5) * q.spacing;
+    double lat_seed = row_y;
+    double ref_lon = q.x_left + ((double)q.full_cols / 2.0) * q.spacing;
+    if (q.utm) {
+        const TmSeries ts = tm_series();
+        double lon_mid;
+        utm_inverse(ts, q.zone, q.south != 0, q.x_left + ((double)q.full_cols / 2.0) * q.spacing, row_y, &lon_mid,
+                    &lat_seed);
+        if (q.need_geom) {  // row_lonlat[n_cols/2].lon from the anchor blend (output_crs.rs:395-414)
+            const int64_t col = q.full_cols / 2;
+            const double x = q.x_start + (double)col * q.spacing;
+            int seg = 0;
+            while (seg + 1 < q.n_anch && x > q.x_start + (double)(seg + 1) * q.anchor_dx) ++seg;
+            const double xa = q.x_start + (double)seg * q.anchor_dx, xb = q.x_start + (double)(seg + 1) * q.anchor_dx;
+            double la0, lo0, la1, lo1;
+            utm_inverse(ts, q.zone, q.south != 0, xa, row_y, &lo0, &la0);
+            utm_inverse(ts, q.zone, q.south != 0, xb, row_y, &lo1, &la1);
+            const double t = xb == xa ? 0.0 : (x - xa) / (xb - xa);
+            ref_lon = lo0 + t * (lo1 - lo0);
+        }
+    }
+    q.row_seed[rr] = guess_azimuth_time(q, lat_seed) - q.epoch;
+    if (q.need_geom) q.row_flat_area[rr] = flat_cell_area(lat_seed, ref_lon, q.step_deg);
+}
+
+// Project_row anchors of every AOI row (output_crs.rs:364-393):
+// anchors[row][0..npts) = lon, anchors[row][npts..2 npts) = lat; i indexes
+// (row, anchor point).
+__device__ __forceinline__ void anchor_point(const TcParams& p, int64_t i) {
+    const int npts = p.n_anchor_pts;
+    const int64_t rr = i / npts;
+    const int k = (int)(i - rr * npts);
+    const double y = p.lat_north - ((double)(p.row0 + rr) + 0.5) * p.spacing;
+    const TmSeries ts = tm_series();
+    double lo, la;
+    utm_inverse(ts, p.utm_zone, p.utm_south != 0, p.x_start + (double)k * p.anchor_dx, y, &lo, &la);
+    p.anchors[rr * 2 * npts + k] = lo;
+    p.anchors[rr * 2 * npts + npts + k] = la;
+}
+
+// Output staging.  Lane = row means a wave's per-pixel stores land on 64
+// different rows; stored one pixel at a time they leave 64 partly written
+// cache lines per array in L2 and each is written back many times.  The
+// segment kernel instead stages each wave's pixels in LDS and writes them out
+// every kStageCols columns (f32 arrays) or kMaskCols columns (the byte mask)
+// as contiguous row pieces.
+constexpr int kStageCols = 8;
+constexpr int kMaskCols = 32;
+struct WaveStage {
+    float d[64][kStageCols + 1];  // +1 word: row-to-row bank skew
+    float c[64][kStageCols + 1];
+    unsigned char m[64][kMaskCols + 4];
+};
+
+// Orders the LDS accesses of one wave's lanes (lanes of a wave execute in
+// lockstep; this keeps the compiler from moving accesses across it).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes the n staged f32 columns starting at output column cs of rows
+// rr0..rr0+63 (rows past p.rows are skipped).
+__device__ __forceinline__ void flush_f32(const TcParams& p, const WaveStage& st, int64_t rr0, int64_t cs, int n) {
+    const int lane = threadIdx.x & 63;
+    wave_sync();
+#pragma unroll 1
+    for (int i = 0; i < kStageCols; ++i) {
+        const int e = lane + 64 * i, r = e / kStageCols, c = e % kStageCols;
+        if (c < n && rr0 + r < p.rows) {
+            const int64_t o = (rr0 + r) * p.cols + cs + c;
+            p.out[o] = st.d[r][c];
+            if (p.compute_lia) p.cos_lia[o] = st.c[r][c];
+        }
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& st, int64_t rr0, int64_t cs, int n) {
+    const int lane = threadIdx.x & 63;
+    wave_sync();
+#pragma unroll 1
+    for (int i = 0; i < kMaskCols; ++i) {
+        const int e = lane + 64 * i, r = e / kMaskCols, c = e % kMaskCols;
+        if (c < n && rr0 + r < p.rows) p.mask[(rr0 + r) * p.cols + cs + c] = st.m[r][c];
+    }
+    wave_sync();
+}
+
+// Occupancy floor of the chain kernels (waves per SIMD the register
+// allocator must allow); a build option so that it can be tuned per compiler.
+#ifndef SD_TC_SEGMENT_WAVES_PER_EU
+#define SD_TC_SEGMENT_WAVES_PER_EU 2
+#endif
+#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
+
+// Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
+// a workgroup take 4 consecutive segments.  Lanes past the last row compute
+// the last row again and discard it, so every lane reaches the wave-wide
+// flushes.
+template <int RS, bool UTM>
+__device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
+                                             int64_t seg) {
+    const int lane = threadIdx.x & 63;
+    const bool row_ok = rr0 + lane < p.rows;
+    const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;
+    const int64_t row = p.row0 + rr;
+    const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
+    const int npts = p.n_anchor_pts;
+    const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
+    const double* alat = UTM ? alon + npts : nullptr;
+    const int64_t c_begin = seg * p.seg_len;
+    const int64_t c_end = min(c_begin + p.seg_len, p.cols);
+    const long long row_seed = p.row_seed[rr];
+    const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
+    const int64_t obase = rr * p.cols;
+
+    // phase A: the chain's value at the end of the previous segment, from a
+    // short chain over the p.lead columns before this segment started at the
+    // row seed.  Its first solve (from the far row seed) may settle on either
+    // side of its root; the following ones approach their roots from the
+    // previous pixel's side, as the reference's chain does, and so usually
+    // reproduce its value exactly.
+    long long S = row_seed;
+    OrbitCache cache = orbit_cache_empty();
+    if (seg > 0) {
+        bool have = false;
+        for (int64_t cc = c_begin > p.lead ? c_begin - p.lead : 0; cc < c_begin; ++cc) {
+            PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
+            if (!q.dem_ok) continue;
+            Solve s = solve(p, orb, q, S, have ? p.max_iter : 40, cache);
+            if (s.oc == ZD_CONVERGED) {
+                S = s.t;
+                have = true;
+            }
+        }
+    }
+
+    // phase B: the reference's sequential chain over the segment
+    LaneCounts C;
+    zero_counts(C);
+    long long seed = S, last_t = 0;
+    bool last_has = false, any_newton = false;
+    for (int64_t cc = c_begin; cc < c_end; ++cc) {
+        const int j = (int)(cc - c_begin);
+        PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
+        float v_s0, v_cos;
+        unsigned char v_mask;
+        if (!q.dem_ok) {  // DEM_MISSING (terrain_correction.rs:1529-1536)
+            v_s0 = v_cos = __builtin_nanf("");
+            v_mask = 2;
+            if (row_ok) {
+                C.c[1] += 1;
+                if (p.radar_coords) {
+                    p.radar_coords[2 * (obase + cc)] = __builtin_nan("");
+                    p.radar_coords[2 * (obase + cc) + 1] = __builtin_nan("");
+                }
+            }
+        } else {
+            any_newton = true;
+            Solve s = solve(p, orb, q, seed, p.max_iter, cache);
+            if (s.oc == ZD_CONVERGED) {
+                seed = s.t;
+                last_t = s.t;
+                last_has = true;
+            }
+            PixelOut v = finish_pixel<RS>(p, q, s.oc, s.t, s.sat, flat_area);
+            if (row_ok) {
+                if (v.cat == CAT_ORBIT) {
+                    if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
+                    else C.err_short += 1;
+                } else {
+                    count(C, v.cat, s.it, 1, p.counters);
+                }
+                if (p.radar_coords) {
+                    p.radar_coords[2 * (obase + cc)] = v.line;
+                    p.radar_coords[2 * (obase + cc) + 1] = v.sample;
+                }
+            }
+            v_s0 = v.sigma0;
+            v_cos = v.cos_lia;
+            v_mask = v.mask;
+        }
+        st.d[lane][j % kStageCols] = v_s0;
+        st.c[lane][j % kStageCols] = v_cos;
+        st.m[lane][j % kMaskCols] = v_mask;
+        const bool last = cc + 1 == c_end;
+        if ((j + 1) % kStageCols == 0 || last) {
+            const int n = j % kStageCols + 1;
+            flush_f32(p, st, rr0, cc + 1 - n, n);
+        }
+        if (p.compute_lia && ((j + 1) % kMaskCols == 0 || last)) {
+            const int n = j % kMaskCols + 1;
+            flush_mask(p, st, rr0, cc + 1 - n, n);
+        }
+    }
+    if (row_ok) {
+        SegRecord rec;
+        rec.seed = S;
+        rec.last = last_t;
+        rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0);
+        rec.err_outside = C.err_outside;
+        rec.err_short = C.err_short;
+        rec.pad = 0;
+        p.records[rr * p.nseg + seg] = rec;
+    }
+    flush_counts(p, C, false);  // orbit errors are decided after the fix-up
+}
+
+// One lane per row: walk the row's segments in order and correct every
+// segment whose starting seed differs from the chained value to its left.
+// Lanes whose row is past the end (valid false) only join the wave-wide
+// counter reduction.
+template <int RS, bool UTM>
+__device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb, int64_t rr, bool valid) {
+    LaneCounts C;
+    zero_counts(C);
+    unsigned redone = 0;
+    if (valid) {
+        const int64_t row = p.row0 + rr;
+        const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
+        const int npts = p.n_anchor_pts;
+        const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
+        const double* alat = UTM ? alon + npts : nullptr;
+        const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
+        const int64_t obase = rr * p.cols;
+        SegRecord* recs = p.records + rr * p.nseg;
+        long long E = p.row_seed[rr];
+        SegRecord r1 = recs[0], r2 = p.nseg > 1 ? recs[1] : r1;  // two records in flight ahead
+        for (int64_t sg = 0; sg < p.nseg; ++sg) {
+            SegRecord rec = r1;
+            r1 = r2;
+            if (sg + 2 < p.nseg) r2 = recs[sg + 2];
+            C.err_outside += rec.err_outside;
+            C.err_short += rec.err_short;
+            if ((rec.flags & 2) && rec.seed != E) {
+                ++redone;
+                // replay the old (rec.seed) and corrected (E) chains in lockstep
+                const int64_t c_begin = sg * p.seg_len, c_end = min(c_begin + p.seg_len, p.cols);
+                long long so = rec.seed, sn = E, new_last = 0;
+                OrbitCache co = orbit_cache_empty(), cn = orbit_cache_empty();
+                bool new_has = false, synced = false;
+                for (int64_t cc = c_begin; cc < c_end; ++cc) {
+                    PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
+                    if (!q.dem_ok) continue;  // outcome independent of the seed
+                    Solve o = solve(p, orb, q, so, p.max_iter, co);
+                    Solve n = solve(p, orb, q, sn, p.max_iter, cn);
+                    if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
+                        if (o.it != n.it) {  // same result, different iteration count
+                            PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, flat_area);
+                            count(C, v.cat, o.it, -1, p.counters);
+                            count(C, v.cat, n.it, 1, p.counters);
+                        }
+                        synced = true;
+                        break;
+                    }
+                    PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, flat_area);
+                    PixelOut vn = finish_pixel<RS>(p, q, n.oc, n.t, n.sat, flat_area);
+                    if (vo.cat == CAT_ORBIT) {
+                        if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
+                        else C.err_short -= 1;
+                    } else {
+                        count(C, vo.cat, o.it, -1, p.counters);
+                    }
+                    if (vn.cat == CAT_ORBIT) {
+                        if (n.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
+                        else C.err_short += 1;
+                    } else {
+                        count(C, vn.cat, n.it, 1, p.counters);
+                    }
+                    write_pixel(p, obase + cc, vn);
+                    if (o.oc == ZD_CONVERGED) so = o.t;
+                    if (n.oc == ZD_CONVERGED) {
+                        sn = n.t;
+                        new_last = n.t;
+                        new_has = true;
+                    }
+                }
+                if (!synced) {
+                    rec.last = new_last;
+                    rec.flags = (rec.flags & ~1) | (new_has ? 1 : 0);
+                }
+            }
+            if (rec.flags & 1) E = rec.last;
+        }
+    }
+    // C holds the segment pass's orbit errors plus the replay's corrections
+    flush_counts(p, C, true);
+    if (redone) atomicAdd(&p.counters[kCntRedo], (unsigned long long)redone);
+}
+
+}  // namespace sd

@@ -83,4 +83,24 @@ struct RowPrepParams {
     double* row_flat_area;  // out (need_geom)
 };
 
+// One terrain-correction job of a batch (tc_batch.hip).  The *_begin
+// fields are the job's first index in each pass's combined index space.
+struct BatchJob {
+    TcParams p;
+    RowPrepParams q;
+    int32_t cls;            // resampler * 2 + (UTM ? 1 : 0)
+    int32_t pad;
+    int64_t tc_begin;       // chain items: 64 rows x 4 segments each
+    int64_t tc_row_blocks;  // ceil(rows / 64)
+    int64_t fix_begin;      // fix-up blocks of 256 rows
+    int64_t row_begin;      // rows pass (one lane per AOI row)
+    int64_t anchor_begin;   // anchor pass (one lane per row x anchor point; UTM jobs)
+};
+
+struct BatchHeader {
+    int32_t n_jobs;
+    int32_t pad;
+    int64_t n_tc, n_fix, n_rows, n_anchor;  // totals of the passes
+};
+
 }  // namespace sd
