@@ -48,7 +48,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["iw_full", "small_plain"], default="iw_full")
+    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64"], default="iw_full",
+                    help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--quiet", action="store_true")
@@ -100,6 +101,8 @@ def main():
 
     dev = torch.device("cuda:0")
     L = _native.lib()
+    if args.workload == "batch64":
+        return bench_batch64(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -302,6 +305,109 @@ def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
     log("cpu baseline:", sample)
     return {"value": round(value, 4), "unit": "Mpx/s", "cores": cores, "kind": "port", "sample": sample,
             "seconds_measured": round(dt + dt_tc + dt_cal + (dt_sp if filt is not None else 0.0), 2)}
+
+
+def bench_batch64(args, dev):
+    """64 square AOI tiles of the full IW scene's UTM grid (log-uniform sides
+    256..4096, random placement, random resampler and speckle filter): one
+    batched run (persistent launches replayed from a HIP graph, launch shape
+    from the autotuner) against the same 64 tiles as one plain
+    terrain_correction + speckle call each."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from sardine_amd import api, synthetic
+    from sardine_amd.workload import DeviceScene
+
+    t0 = time.time()
+    sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+    ds = DeviceScene.build(sc, dev)
+    m = ds.prepare_merged()
+    torch.cuda.synchronize()
+    rows, cols, _ = api.output_grid(ds.scene, ds.tc_config())
+    rs = api.ResidentScene(m, ds.scene, ds.dem)
+    rng = np.random.default_rng(64)
+    F = api.SpeckleFilter
+    filters = [None, F.Lee(5, 4.0), F.Frost(5, 1.0), F.GammaMap(7, 4.0), F.RefinedLee(4.0)]
+    jobs = []
+    for _ in range(64):
+        side = int(round(math.exp(rng.uniform(math.log(256), math.log(4096)))))
+        r0, c0 = int(rng.integers(0, rows - side + 1)), int(rng.integers(0, cols - side + 1))
+        cfg = ds.tc_config(aoi=(r0, c0, side, side), resampling=int(rng.integers(0, 3)))
+        jobs.append(api.BatchJob(0, cfg, filters[int(rng.integers(0, len(filters)))]))
+    px = sum(j.config.aoi[2] * j.config.aoi[3] for j in jobs)
+    log(f"scene built in {time.time() - t0:.1f}s; 64 tiles, {px / 1e6:.1f} Mpx")
+    stream = torch.cuda.current_stream(dev)
+
+    # plain: one terrain_correction (+ speckle) call per tile, outputs preallocated
+    outs = [(torch.empty(j.config.aoi[2:], dtype=torch.float32, device=dev),
+             torch.empty(j.config.aoi[2] * j.config.aoi[3], dtype=torch.float32, device=dev)) for j in jobs]
+
+    def plain():
+        for j, (o, so) in zip(jobs, outs):
+            img = api.terrain_correction(m, ds.scene, ds.dem, j.config, stream=stream, out=(o, None, None))
+            if j.speckle is not None:
+                j.speckle.apply(img.data, img.cols, img.rows, stream=stream, out=so)
+
+    plain()
+    torch.cuda.synchronize()
+    reps = max(1, args.steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for i in range(reps):
+        plain()
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    plain_lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]
+    plain_ms = float(np.median(plain_lat))
+    log(f"plain: {['%.1f' % x for x in plain_lat]} ms")
+
+    b = api.Batch(max_jobs=64, use_graph=True)
+    best, tune_ms = b.autotune([rs], jobs, stream=stream)
+    log(f"autotuner: segment {best.segment_columns}, tc groups/CU {best.tc_groups_per_cu}, speckle groups/CU "
+        f"{best.speckle_groups_per_cu} -> {tune_ms:.2f} ms device")
+    prep = b.prepare([rs], jobs)
+    b.execute(prep, stream=stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    dev_ms = []
+    for i in range(reps):
+        b.execute(prep, stream=stream)
+        ev[i + 1].record(stream)
+        dev_ms.append(b.last_device_ms)
+    torch.cuda.synchronize()
+    batch_lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]
+    batch_ms = float(np.median(batch_lat))
+    log(f"batched: {['%.1f' % x for x in batch_lat]} ms (device {['%.1f' % x for x in dev_ms]})")
+    srt = sorted(batch_lat)
+    line = {
+        "metric": "batched AOI geocode Mpx/s (64 tiles, one persistent graph launch) vs one call per tile",
+        "value": round(px / (batch_ms / 1e3) / 1e6, 3),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": 1,
+        "ms_per_step": round(batch_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batch64_iw_full_utm10m_tiles256-4096_mixed_resample_speckle", "tiles": 64,
+                   "tile_px": px, "grid": [rows, cols]},
+        "latency_ms": {"p50": round(srt[len(srt) // 2], 3), "p99": round(srt[-1], 3)},
+        "plain_one_call_per_tile": {"ms_per_step": round(plain_ms, 3),
+                                    "value": round(px / (plain_ms / 1e3) / 1e6, 3)},
+        "speedup_vs_plain": round(plain_ms / batch_ms, 3),
+        "tuning": {"segment_columns": best.segment_columns, "tc_groups_per_cu": best.tc_groups_per_cu,
+                   "speckle_groups_per_cu": best.speckle_groups_per_cu},
+    }
+    print(json.dumps(line), flush=True)
+    b.close()
+    rs.close()
 
 
 if __name__ == "__main__":

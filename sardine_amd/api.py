@@ -484,8 +484,13 @@ class Batch:
             cj[i].speckle_out = None if j.speckle is None else j.speckle_out.data_ptr()
         return arr, cj, keep
 
-    def run(self, scenes, jobs, stream=None, check: bool = True):
-        arr, cj, keep = self._prepare(scenes, jobs)
+    def prepare(self, scenes, jobs):
+        """Builds the C job table once (outputs are allocated on first use);
+        `execute` can then run it any number of times."""
+        return (list(scenes), list(jobs)) + self._prepare(scenes, jobs)
+
+    def execute(self, prepared, stream=None, check: bool = True):
+        scenes, jobs, arr, cj, keep = prepared
         ms = C.c_double(0.0)
         st = lib().sardine_batch_run(self.handle, arr, len(scenes), cj, len(jobs), _stream_ptr(stream), C.byref(ms))
         if check:
@@ -502,6 +507,9 @@ class Batch:
                                img.full_cols, img.chain_redo_rounds, None)
             out.append(BatchResult(gi, j.speckle_out if j.speckle is not None else None, cj[i].status))
         return out
+
+    def run(self, scenes, jobs, stream=None, check: bool = True):
+        return self.execute(self.prepare(scenes, jobs), stream=stream, check=check)
 
     def autotune(self, scenes, jobs, stream=None):
         arr, cj, keep = self._prepare(scenes, jobs)

@@ -393,6 +393,18 @@ sardine_status validate_tc(const sardine_radar_image* merged, const sardine_scen
     return SARDINE_OK;
 }
 
+// The project_row anchor segment a grid column blends in: the same
+// arithmetic as the kernels' walk (output_crs.rs:395-414).
+int anchor_segment(const sd::TcParams& p, int64_t col) {
+    const double x = p.x_start + (double)col * p.spacing;
+    const int na = p.n_anchor_pts - 1;
+    int seg = (int)std::floor((x - p.x_start) * p.inv_anchor_dx);
+    seg = seg < 0 ? 0 : (seg > na - 1 ? na - 1 : seg);
+    while (seg > 0 && !(x > p.x_start + (double)seg * p.anchor_dx)) --seg;
+    while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
+    return seg;
+}
+
 // Everything of one job that needs no device work: grid, AOI, kernel
 // arguments (terrain_correction.rs:1304-1511).  Work buffers (records,
 // anchors, row seeds, flat areas, counters, geoid) are set by the caller.
@@ -503,6 +515,10 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     p.mask = compute_lia ? out->mask : nullptr;
     p.radar_coords = out->radar_coords;
     p.lead = g_tc_lead.load();
+    if (utm) {  // the anchors the AOI's columns blend between
+        p.anchor0 = anchor_segment(p, c0);
+        p.anchor_stride = anchor_segment(p, c0 + nc - 1) + 2 - p.anchor0;
+    }
     return SARDINE_OK;
 }
 
@@ -529,7 +545,7 @@ size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
     TcWork w;
     w.records = round256((size_t)pl.nr * pl.p.nseg * sizeof(sd::SegRecord));
-    w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * (pl.n_anch + 1) * sizeof(double)) : 0;
+    w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * pl.p.anchor_stride * sizeof(double)) : 0;
     w.seed = round256((size_t)pl.nr * sizeof(int64_t));
     w.flat = pl.need_geom ? round256((size_t)pl.nr * sizeof(double)) : 0;
     w.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? round256(73 * 144 * sizeof(float)) : 0;
@@ -1332,7 +1348,7 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
         J.row_begin = H.n_rows;
         H.n_rows += pl.nr;
         J.anchor_begin = H.n_anchor;
-        if (pl.utm) H.n_anchor += pl.nr * pl.p.n_anchor_pts;
+        if (pl.utm) H.n_anchor += pl.nr * pl.p.anchor_stride;
         // speckle entry (a job without a filter contributes no tiles)
         const sardine_batch_job& jb = jobs[i];
         int64_t tiles_x = 1, tiles = 0;

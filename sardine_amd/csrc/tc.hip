@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void tc_rows_kernel(RowPrepParams q) {
 // One lane per (AOI row, anchor point).
 __global__ __launch_bounds__(256) void utm_anchor_kernel(TcParams p) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < p.rows * p.n_anchor_pts) anchor_point(p, i);
+    if (i < p.rows * p.anchor_stride) anchor_point(p, i);
 }
 
 template <int RS, bool UTM>
@@ -86,7 +86,7 @@ extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t strea
                                         hipEvent_t seg_end) {
     const bool utm = p->crs != 0;
     if (utm) {
-        const int64_t n = p->rows * p->n_anchor_pts;
+        const int64_t n = p->rows * p->anchor_stride;
         hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
     }
     switch (p->resampling * 2 + (utm ? 1 : 0)) {

@@ -65,8 +65,9 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
         while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
         const double x0 = p.x_start + (double)seg * p.anchor_dx, x1 = p.x_start + (double)(seg + 1) * p.anchor_dx;
         const double t = (x1 == x0) ? 0.0 : (x - x0) / (x1 - x0);
-        q.lon = alon[seg] + t * (alon[seg + 1] - alon[seg]);
-        q.lat = alat[seg] + t * (alat[seg + 1] - alat[seg]);
+        const int a = seg - p.anchor0;  // the row stores anchors anchor0.. only
+        q.lon = alon[a] + t * (alon[a + 1] - alon[a]);
+        q.lat = alat[a] + t * (alat[a + 1] - alat[a]);
     }
     float hf = 0.0f;
     q.dem_ok = dem_sample(p, q.lat, q.lon, &hf) && isfinite(hf);
@@ -302,16 +303,17 @@ __device__ __forceinline__ void row_prep(const RowPrepParams& q, int64_t rr) {
 }
 
 // Project_row anchors of every AOI row (output_crs.rs:364-393):
-// anchors[row][0..npts) = lon, anchors[row][npts..2 npts) = lat; i indexes
-// (row, anchor point).
+// anchors[row][0..stride) = lon, anchors[row][stride..2 stride) = lat of
+// anchor points anchor0 .. anchor0 + stride - 1 (the ones the AOI's columns
+// blend between); i indexes (row, stored point).
 __device__ __forceinline__ void anchor_point(const TcParams& p, int64_t i) {
-    const int npts = p.n_anchor_pts;
+    const int npts = p.anchor_stride;
     const int64_t rr = i / npts;
     const int k = (int)(i - rr * npts);
     const double y = p.lat_north - ((double)(p.row0 + rr) + 0.5) * p.spacing;
     const TmSeries ts = tm_series();
     double lo, la;
-    utm_inverse(ts, p.utm_zone, p.utm_south != 0, p.x_start + (double)k * p.anchor_dx, y, &lo, &la);
+    utm_inverse(ts, p.utm_zone, p.utm_south != 0, p.x_start + (double)(p.anchor0 + k) * p.anchor_dx, y, &lo, &la);
     p.anchors[rr * 2 * npts + k] = lo;
     p.anchors[rr * 2 * npts + npts + k] = la;
 }
@@ -385,7 +387,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;
     const int64_t row = p.row0 + rr;
     const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
-    const int npts = p.n_anchor_pts;
+    const int npts = p.anchor_stride;
     const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
     const double* alat = UTM ? alon + npts : nullptr;
     const int64_t c_begin = seg * p.seg_len;
@@ -498,7 +500,7 @@ __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb
     if (valid) {
         const int64_t row = p.row0 + rr;
         const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
-        const int npts = p.n_anchor_pts;
+        const int npts = p.anchor_stride;
         const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
         const double* alat = UTM ? alon + npts : nullptr;
         const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;

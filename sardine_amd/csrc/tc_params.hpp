@@ -49,7 +49,8 @@ struct TcParams {
     int64_t row0, col0, rows, cols;
     int seg_len, nseg, rows_per_wg;  // work decomposition (see tc.hip)
     int lead;                        // phase-A lead columns
-    double* anchors;                  // UTM: per AOI row lon[npts], lat[npts]
+    double* anchors;                  // UTM: per AOI row lon[stride], lat[stride] of anchors anchor0..
+    int anchor0, anchor_stride;       // first stored anchor index, stored anchors per row
     SegRecord* records;               // per (AOI row, segment)
     const int64_t* row_seed;      // per AOI row, µs relative to the first state vector
     const double* row_flat_area;  // per AOI row (need_geom)

@@ -81,16 +81,6 @@ __device__ __forceinline__ double keys_w(double t) {  // terrain_correction.rs:8
     return 0.0;
 }
 
-__device__ __forceinline__ double lanczos_w(double x) {  // terrain_correction.rs:938-950
-    double ax = fabs(x);
-    if (ax < 1e-10) return 1.0;
-    if (ax < 3.0) {
-        double px = kPi * x;
-        double pxa = kPi * x / 3.0;
-        return sin(px) * sin(pxa) / (px * pxa);
-    }
-    return 0.0;
-}
 
 __device__ __forceinline__ int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -107,43 +97,77 @@ __device__ __forceinline__ float bilinear(const float* d, int64_t nl, int64_t ns
 }
 
 // terrain_correction.rs:842-872
+// A NaN tap makes the reference return NaN at once; here every tap is
+// gathered first and a NaN tap turns the f64 sum into NaN instead, which
+// yields the same NaN result and the same bits otherwise (the sum runs in
+// the reference's order).
 __device__ __forceinline__ float bicubic(const float* d, int64_t nl, int64_t ns, double line, double sample) {
     int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
     double dl = line - (double)lf, ds = sample - (double)sf;
     double ws[4];
+    int64_t cs[4];
 #pragma unroll
-    for (int si = 0; si < 4; ++si) ws[si] = keys_w((double)(si - 1) - ds);
+    for (int si = 0; si < 4; ++si) {
+        ws[si] = keys_w((double)(si - 1) - ds);
+        cs[si] = clampi(sf + si - 1, 0, ns - 1);
+    }
     double acc = 0.0;
 #pragma unroll
     for (int li = -1; li <= 2; ++li) {
         const float* row = d + clampi(lf + li, 0, nl - 1) * ns;
         double wl = keys_w((double)li - dl);
 #pragma unroll
-        for (int si = -1; si <= 2; ++si) {
-            float v = row[clampi(sf + si, 0, ns - 1)];
-            if (isnan(v)) return __builtin_nanf("");
-            acc += (double)v * wl * ws[si + 1];
-        }
+        for (int si = 0; si < 4; ++si) acc += (double)row[cs[si]] * wl * ws[si];
     }
     return (float)acc;
 }
 
 // terrain_correction.rs:958-983
+// The six Lanczos-3 weights L(x) = sin(πx)·sin(πx/3) / (πx·πx/3) (1 for
+// |x| < 1e-10, 0 for |x| >= 3; terrain_correction.rs:938-950) at
+// x = u + 2 - k, k = 0..5, of one axis (u = the fractional coordinate in
+// [0, 1)).  sin(π(u+m)) = (-1)^m sin(πu)
+// and sin(π(u+m)/3) follows from sin/cos(πu/3) by angle addition, so one
+// sin and one sincos replace the reference's twelve sin calls; the weights
+// agree with the direct form to ~1e-15.
+__device__ __forceinline__ void lanczos_axis(double u, double w[6]) {
+    const double s1 = sin(kPi * u);
+    double s3, c3;
+    sincos(kPi * u / 3.0, &s3, &c3);
+    constexpr double h = 0.86602540378443864676;  // sin(π/3)
+    // cos and sin of π(2 - k)/3 for k = 0..5
+    const double C[6] = {-0.5, 0.5, 1.0, 0.5, -0.5, -1.0};
+    const double S[6] = {h, h, 0.0, -h, -h, 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double x = u + (double)(2 - k);
+        const double ax = fabs(x);
+        double v = 0.0;
+        if (ax < 1e-10) {
+            v = 1.0;
+        } else if (ax < 3.0) {
+            const double px = kPi * x, pxa = kPi * x / 3.0;
+            const double sp = (k & 1) ? -s1 : s1;
+            v = sp * (s3 * C[k] + c3 * S[k]) / (px * pxa);
+        }
+        w[k] = v;
+    }
+}
+
 __device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns, double line, double sample) {
     int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
-    double ws[6];
+    double ws[6], wl[6];
+    lanczos_axis(sample - (double)sf, ws);
+    lanczos_axis(line - (double)lf, wl);
+    int64_t cs[6];
 #pragma unroll
-    for (int si = 0; si < 6; ++si) ws[si] = lanczos_w(sample - (double)(sf + si - 2));
-    double acc = 0.0;
-    for (int li = -2; li <= 3; ++li) {
-        const float* row = d + clampi(lf + li, 0, nl - 1) * ns;
-        double wl = lanczos_w(line - (double)(lf + li));
+    for (int si = 0; si < 6; ++si) cs[si] = clampi(sf + si - 2, 0, ns - 1);
+    double acc = 0.0;  // NaN taps: see bicubic
 #pragma unroll
-        for (int si = -2; si <= 3; ++si) {
-            float v = row[clampi(sf + si, 0, ns - 1)];
-            if (isnan(v)) return __builtin_nanf("");
-            acc += (double)v * wl * ws[si + 2];
-        }
+    for (int li = 0; li < 6; ++li) {
+        const float* row = d + clampi(lf + li - 2, 0, nl - 1) * ns;
+#pragma unroll
+        for (int si = 0; si < 6; ++si) acc += (double)row[cs[si]] * wl[li] * ws[si];
     }
     return (float)acc;
 }

@@ -146,6 +146,28 @@ def test_tc_aoi_window(small):
     assert (np.abs(d[v] - ref.data[v]) / ref.data[v]).max() <= DB_REL
 
 
+@pytest.mark.parametrize("aoi,resampling", [((5, 17, 97, 301), 2), ((100, 2216, 50, 1), 0), ((0, 900, 1, 700), 1),
+                                            ((1800, 0, 52, 2217), 0)])
+def test_tc_utm_aoi_windows(iw, aoi, resampling):
+    """UTM AOI windows (the kernels store only the project_row anchors the
+    window's columns blend between) against the oracle on the same window."""
+    sc, ds, info, s0, nz, _ = iw
+    m = ds.prepare_merged()
+    img = ds.terrain_correction(m, ds.tc_config(compute_lia=True, aoi=aoi, resampling=resampling))
+    keep = []
+    ccfg = sc.tc_config(compute_lia=1, aoi_row0=aoi[0], aoi_col0=aoi[1], aoi_rows=aoi[2], aoi_cols=aoi[3],
+                        resampling=resampling)
+    ref = O.terrain_correction(s0, nz, _meta(info), sc.scene_c(keep), sc.dem(), sc.dem_spec, ccfg)
+    assert ref.status == 0, ref.error
+    assert np.array_equal(img.mask.cpu().numpy(), ref.mask)
+    d = img.data.cpu().numpy()
+    v = np.isfinite(ref.data)
+    assert np.array_equal(np.isfinite(d), v)
+    if v.any():
+        assert (np.abs(d[v] - ref.data[v]) / ref.data[v]).max() <= DB_REL
+    assert img.valid_pixel_count == ref.counters()["valid"]
+
+
 SPECKLES = [
     (abi.SPECKLE_BOXCAR, 3, 1.0, 1.0), (abi.SPECKLE_BOXCAR, 7, 1.0, 1.0),
     (abi.SPECKLE_LEE, 5, 1.0, 1.0), (abi.SPECKLE_LEE, 5, 4.0, 1.0),
