@@ -502,6 +502,52 @@ sardine_status sardine_batch_autotune(sardine_batch batch, const sardine_residen
                                       sardine_batch_job* jobs, int32_t n_jobs, void* stream,
                                       sardine_batch_tuning* best, double* best_ms);
 
+/* ── Serving (no reference counterpart) ────────────────────────────────────
+ * A request loop over resident scenes.  Requests are admitted in arrival
+ * order while their output and work memory fits the HBM budget, grouped
+ * into a batch as they arrive (up to max_batch), and run with up to `slots`
+ * batches in flight at once on separate streams, so work admitted while
+ * another batch is on the GPU overlaps it.  Each result equals what the
+ * per-call path gives for that request alone. */
+
+typedef struct sardine_server_s* sardine_server;
+
+typedef struct sardine_server_options {
+    uint64_t hbm_budget_bytes; /* outputs held until release + work of batches in flight */
+    int32_t max_batch;         /* requests per batch */
+    int32_t slots;             /* batches in flight */
+    int32_t use_graph;         /* replay each slot's launches from a HIP graph */
+    int32_t pad;
+} sardine_server_options;
+
+typedef struct sardine_request {
+    int32_t scene;                  /* index into the server's scenes */
+    int32_t pad;
+    sardine_tc_config config;       /* output grid, AOI window, resampler, ... */
+    sardine_speckle_filter speckle; /* kind < 0: none */
+} sardine_request;
+
+typedef struct sardine_response {
+    int32_t status;                 /* the request's sardine_status */
+    int32_t done;                   /* 0 while queued or running */
+    sardine_geocoded_image image;   /* data: [device] server-owned rows*cols floats until release */
+    float* speckled;                /* [device] server-owned filtered data, or NULL */
+    double t_submit_us, t_admit_us, t_launch_us, t_done_us; /* host steady clock */
+    int32_t batch_size;             /* requests in the batch it ran in */
+    int32_t pad;
+} sardine_response;
+
+sardine_status sardine_server_create(const sardine_resident_scene* scenes, int32_t n_scenes,
+                                     const sardine_server_options* options, sardine_server* out);
+/* Queues a request (thread-safe, never waits for the GPU). */
+sardine_status sardine_server_submit(sardine_server server, const sardine_request* request, uint64_t* ticket);
+/* Copies the request's response; with block != 0 waits until it is done. */
+sardine_status sardine_server_wait(sardine_server server, uint64_t ticket, int32_t block, sardine_response* out);
+/* Frees a finished request's outputs and forgets the ticket. */
+sardine_status sardine_server_release(sardine_server server, uint64_t ticket);
+/* Stops the loop after the requests already admitted have finished. */
+void sardine_server_destroy(sardine_server server);
+
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */
 
 /* Fill a CInt16 raster with round(sigma * N(0,1)) real/imag parts from a

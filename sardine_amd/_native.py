@@ -73,11 +73,17 @@ def lib():
     L.sardine_batch_run.argtypes = [P, PP, i32, C.POINTER(abi.BatchJob), i32, P, C.POINTER(d)]
     L.sardine_batch_autotune.argtypes = [P, PP, i32, C.POINTER(abi.BatchJob), i32, P, C.POINTER(abi.BatchTuning),
                                          C.POINTER(d)]
+    L.sardine_server_create.argtypes = [PP, i32, C.POINTER(abi.ServerOptions), PP]
+    L.sardine_server_submit.argtypes = [P, C.POINTER(abi.Request), C.POINTER(u64)]
+    L.sardine_server_wait.argtypes = [P, u64, i32, C.POINTER(abi.Response)]
+    L.sardine_server_release.argtypes = [P, u64]
+    L.sardine_server_destroy.argtypes = [P]
+    L.sardine_server_destroy.restype = None
     for name in abi.EXPORTED_SYMBOLS:
         if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",
                         "sardine_tc_config_default", "sardine_set_profiling", "sardine_last_kernel_ms",
                         "sardine_set_tc_segment", "sardine_set_tc_lead", "sardine_scene_destroy",
-                        "sardine_batch_destroy"):
+                        "sardine_batch_destroy", "sardine_server_destroy"):
             getattr(L, name).restype = C.c_int
     if L.sardine_abi_version() != abi.ABI_VERSION:
         raise NativeLibraryError(f"ABI version mismatch: library {L.sardine_abi_version()}, "

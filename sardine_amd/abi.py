@@ -141,6 +141,21 @@ class BatchTuning(C.Structure):
                 ("speckle_groups_per_cu", C.c_int32)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_int32), ("slots", C.c_int32),
+                ("use_graph", C.c_int32), ("pad", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("pad", C.c_int32), ("config", TcConfig), ("speckle", SpeckleFilterC)]
+
+
+class Response(C.Structure):
+    _fields_ = [("status", C.c_int32), ("done", C.c_int32), ("image", GeocodedImage), ("speckled", C.c_void_p),
+                ("t_submit_us", C.c_double), ("t_admit_us", C.c_double), ("t_launch_us", C.c_double),
+                ("t_done_us", C.c_double), ("batch_size", C.c_int32), ("pad", C.c_int32)]
+
+
 class Burst(C.Structure):
     _fields_ = [("azimuth_time_us", C.c_int64), ("first_line", C.c_int64),
                 ("first_valid_sample", C.c_int64), ("last_valid_sample", C.c_int64),
@@ -213,5 +228,10 @@ EXPORTED_SYMBOLS = (
     "sardine_batch_get_tuning",
     "sardine_batch_run",
     "sardine_batch_autotune",
+    "sardine_server_create",
+    "sardine_server_submit",
+    "sardine_server_wait",
+    "sardine_server_release",
+    "sardine_server_destroy",
 )
 KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT = 0, 1, 2, 3, 4

@@ -518,3 +518,85 @@ class Batch:
         _check(lib().sardine_batch_autotune(self.handle, arr, len(scenes), cj, len(jobs), _stream_ptr(stream),
                                             C.byref(best), C.byref(ms)))
         return best, ms.value
+
+
+# ─── serving (no reference counterpart) ───────────────────────────────────
+class _DeviceView:
+    """Zero-copy view of server-owned device memory for torch.as_tensor."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (int(ptr), False),
+                                         "version": 3, "strides": None}
+
+
+@dataclass
+class ServedResult:
+    status: int
+    image: Optional[GeocodedImage]
+    speckled: object
+    t_submit_us: float
+    t_admit_us: float
+    t_launch_us: float
+    t_done_us: float
+    batch_size: int
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.t_done_us - self.t_submit_us) / 1e3
+
+
+class Server:
+    """The native request loop (sardine_server_*): requests on resident
+    scenes are admitted by HBM budget, batched as they arrive and run with
+    several batches in flight.  Result tensors view server-owned memory
+    until `release`."""
+
+    def __init__(self, scenes, hbm_budget_bytes: int, max_batch: int = 64, slots: int = 2, use_graph: bool = True):
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.handle for s in self.scenes])
+        opt = abi.ServerOptions(int(hbm_budget_bytes), int(max_batch), int(slots), 1 if use_graph else 0, 0)
+        h = C.c_void_p()
+        _check(lib().sardine_server_create(arr, len(self.scenes), C.byref(opt), C.byref(h)))
+        self.handle = h
+
+    def submit(self, scene: int, config: TerrainCorrectionConfig, speckle: Optional[SpeckleFilter] = None) -> int:
+        r = abi.Request()
+        r.scene = scene
+        r.config = config.as_c()
+        r.speckle = speckle.as_c() if speckle is not None else abi.SpeckleFilterC(-1, 0, 0.0, 0.0)
+        t = C.c_uint64()
+        _check(lib().sardine_server_submit(self.handle, C.byref(r), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, block: bool = True) -> ServedResult:
+        torch = _torch()
+        r = abi.Response()
+        _check(lib().sardine_server_wait(self.handle, ticket, 1 if block else 0, C.byref(r)))
+        img = None
+        sp = None
+        if r.done and r.status == 0:
+            g = r.image
+            data = torch.as_tensor(_DeviceView(g.data, (g.rows, g.cols), "<f4"), device="cuda")
+            img = GeocodedImage(data, g.rows, g.cols, tuple(g.geotransform), g.total_pixel_count,
+                                g.valid_pixel_count, g.dem_missing_count, g.outside_footprint_count,
+                                g.non_converged_count, g.degenerate_geometry_count, g.flat_masked_count,
+                                g.noise_masked_count, list(g.newton_iter_histogram), None, None, g.full_rows,
+                                g.full_cols, g.chain_redo_rounds, None)
+            if r.speckled:
+                sp = torch.as_tensor(_DeviceView(r.speckled, (g.rows, g.cols), "<f4"), device="cuda")
+        return ServedResult(r.status, img, sp, r.t_submit_us, r.t_admit_us, r.t_launch_us, r.t_done_us,
+                            r.batch_size)
+
+    def release(self, ticket: int):
+        _check(lib().sardine_server_release(self.handle, ticket))
+
+    def close(self):
+        if self.handle:
+            lib().sardine_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,10 +11,14 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
 #include <deque>
 #include <memory>
 #include <mutex>
-#include <cmath>
+#include <thread>
+#include <unordered_map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1269,18 +1273,40 @@ sardine_status sardine_batch_get_tuning(sardine_batch b, sardine_batch_tuning* t
     return SARDINE_OK;
 }
 
-sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
-                                 sardine_batch_job* jobs, int32_t n_jobs, void* stream_v, double* device_ms) {
+}  // extern "C"
+
+namespace {
+
+// One batch run in flight: the plans of its jobs and its work arena.
+struct BatchRun {
+    std::vector<TcPlan> plans;
+    std::vector<int> run_idx;
+    DevBuf work;
+    hipStream_t s = nullptr;  // where the launches went
+    sardine_status first = SARDINE_OK;
+    std::string first_msg;
+    bool launched = false;
+};
+
+void note_failure(BatchRun& run, sardine_status st) {
+    if (st && !run.first) {
+        run.first = st;
+        run.first_msg = g_last_error;
+    }
+}
+
+// Plans every job, fills and uploads the job tables and enqueues the
+// launches (graph replay or eager) and the counter read-back on the
+// batch's stream; b->ev_out marks the end.  Per-job validation failures go
+// to jobs[i].status.
+sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* scenes, int32_t n_scenes,
+                           sardine_batch_job* jobs, int32_t n_jobs, hipStream_t stream, BatchRun& run) {
     if (!b || (n_jobs > 0 && (!jobs || !scenes))) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     if (n_jobs < 0 || n_jobs > b->max_jobs)
         return fail(SARDINE_ERR_CAPACITY, "%d jobs for a batch of at most %d", n_jobs, b->max_jobs);
-    hipStream_t stream = (hipStream_t)stream_v;
-    sardine_status st, first = SARDINE_OK;
-    std::string first_msg;
-    // plan every job; failures are recorded per job and the job is left out
-    std::vector<TcPlan> plans((size_t)n_jobs);
+    sardine_status st;
+    run.plans.assign((size_t)n_jobs, TcPlan());
     std::vector<TcWork> works((size_t)n_jobs);
-    std::vector<int> run_idx;
     std::vector<int> halves((size_t)n_jobs, 0);
     size_t arena = 0;
     for (int i = 0; i < n_jobs; ++i) {
@@ -1290,7 +1316,7 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
             st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: bad scene index or output", i);
         const sardine_resident_scene_s* r = st ? nullptr : scenes[jb.scene];
         if (!st && r->device != b->device) st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: scene on another device", i);
-        if (!st) st = plan_tc(&r->merged, &r->scene, &r->dem, &jb.config, jb.out, r->dev, &plans[(size_t)i]);
+        if (!st) st = plan_tc(&r->merged, &r->scene, &r->dem, &jb.config, jb.out, r->dev, &run.plans[(size_t)i]);
         if (!st && jb.speckle.kind >= 0) {
             int win = 0;
             st = validate_filter(&jb.speckle, &win);
@@ -1300,38 +1326,36 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
         }
         jb.status = st;
         if (st) {
-            if (!first) {
-                first = st;
-                first_msg = g_last_error;
-            }
+            note_failure(run, st);
             continue;
         }
-        set_decomposition(plans[(size_t)i].p, b->tuning.segment_columns);
-        works[(size_t)i] = tc_work_bytes(plans[(size_t)i], &jb.config);
+        set_decomposition(run.plans[(size_t)i].p, b->tuning.segment_columns);
+        works[(size_t)i] = tc_work_bytes(run.plans[(size_t)i], &jb.config);
         arena += works[(size_t)i].total();
-        run_idx.push_back(i);
+        run.run_idx.push_back(i);
     }
-    if (run_idx.empty()) {
-        if (first) g_last_error = first_msg;
-        return first;
-    }
+    if (run.run_idx.empty()) return SARDINE_OK;
     if ((st = require_device())) return st;
-    DevBuf d_work;
-    SD_HIP(d_work.alloc(arena, stream));
-    char* base = d_work.as<char>();
-    // job tables
+    hipStream_t s = b->use_graph ? b->work : stream;
+    run.s = s;
+    if (b->use_graph) {  // the batch's stream starts after the caller's earlier work
+        SD_HIP(hipEventRecord(b->ev_in, stream));
+        SD_HIP(hipStreamWaitEvent(s, b->ev_in, 0));
+    }
+    SD_HIP(run.work.alloc(arena, s));
+    char* base = run.work.as<char>();
     sd::BatchHeader& H = *b->h_H;
     std::memset(&H, 0, sizeof H);
     SpeckleBatchHeaderHost& SH = *b->h_SH;
     std::memset(&SH, 0, sizeof SH);
     int max_half = 1;
     int k = 0;
-    for (int i : run_idx) {
-        TcPlan& pl = plans[(size_t)i];
+    for (int i : run.run_idx) {
+        TcPlan& pl = run.plans[(size_t)i];
         assign_work(pl, works[(size_t)i], base);
         if (works[(size_t)i].geoid)
             SD_HIP(staging().copy(const_cast<float*>(pl.p.geoid), jobs[i].config.geoid.grid_73x144,
-                                  73 * 144 * sizeof(float), stream));
+                                  73 * 144 * sizeof(float), s));
         base += works[(size_t)i].total();
         pl.p.counters = b->d_cnt + (size_t)k * cnt_stride();
         pl.p.error_flag = reinterpret_cast<int*>(pl.p.counters + sd::kNumCounters);
@@ -1349,7 +1373,6 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
         H.n_rows += pl.nr;
         J.anchor_begin = H.n_anchor;
         if (pl.utm) H.n_anchor += pl.nr * pl.p.anchor_stride;
-        // speckle entry (a job without a filter contributes no tiles)
         const sardine_batch_job& jb = jobs[i];
         int64_t tiles_x = 1, tiles = 0;
         if (jb.speckle.kind >= 0) {
@@ -1366,13 +1389,6 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
     SH.n_jobs = k;
     if (H.n_tc >= 0xffffffffLL || SH.n_tiles >= 0xffffffffLL)
         return fail(SARDINE_ERR_CAPACITY, "batch work above 2^32 items");
-    // run: tables in, launches, counters out.  Graph replays go on the
-    // batch's own stream, ordered after the caller's work by events.
-    hipStream_t s = b->use_graph ? b->work : stream;
-    if (b->use_graph) {
-        SD_HIP(hipEventRecord(b->ev_in, stream));
-        SD_HIP(hipStreamWaitEvent(s, b->ev_in, 0));
-    }
     SD_HIP(hipMemcpyAsync(b->d_H, b->h_H, sizeof(sd::BatchHeader), hipMemcpyHostToDevice, s));
     SD_HIP(hipMemcpyAsync(b->d_J, b->h_J, (size_t)k * sizeof(sd::BatchJob), hipMemcpyHostToDevice, s));
     SD_HIP(hipMemcpyAsync(b->d_SH, b->h_SH, sizeof(SpeckleBatchHeaderHost), hipMemcpyHostToDevice, s));
@@ -1408,29 +1424,49 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
     SD_HIP(hipEventRecord(b->t1, s));
     SD_HIP(hipMemcpyAsync(b->h_cnt, b->d_cnt, (size_t)k * cnt_stride() * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
-    if (b->use_graph) {
-        SD_HIP(hipEventRecord(b->ev_out, s));
-        SD_HIP(hipStreamWaitEvent(stream, b->ev_out, 0));  // later work on the caller's stream sees the outputs
-    }
-    SD_HIP(hipStreamSynchronize(s));
-    if (device_ms) {
-        float ms = 0.0f;
-        SD_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
-        *device_ms = ms;
-    }
-    k = 0;
-    for (int i : run_idx) {
-        st = finish_tc(plans[(size_t)i], b->h_cnt + (size_t)k * cnt_stride(), scenes[jobs[i].scene]->scene.orbit.count,
-                       jobs[i].out);
-        jobs[i].status = st;
-        if (st && !first) {
-            first = st;
-            first_msg = g_last_error;
+    SD_HIP(hipEventRecord(b->ev_out, s));
+    if (b->use_graph) SD_HIP(hipStreamWaitEvent(stream, b->ev_out, 0));  // later caller work sees the outputs
+    run.launched = true;
+    return SARDINE_OK;
+}
+
+// Waits for a launched run and fills every job's result.
+sardine_status batch_end(sardine_batch_s* b, const sardine_resident_scene* scenes, sardine_batch_job* jobs,
+                         BatchRun& run, double* device_ms) {
+    if (run.launched) {
+        SD_HIP(hipEventSynchronize(b->ev_out));
+        if (device_ms) {
+            float ms = 0.0f;
+            SD_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+            *device_ms = ms;
         }
-        ++k;
+        int k = 0;
+        for (int i : run.run_idx) {
+            const sardine_status st = finish_tc(run.plans[(size_t)i], b->h_cnt + (size_t)k * cnt_stride(),
+                                                scenes[jobs[i].scene]->scene.orbit.count, jobs[i].out);
+            jobs[i].status = st;
+            note_failure(run, st);
+            ++k;
+        }
+        run.launched = false;
     }
-    if (first) g_last_error = first_msg;
-    return first;
+    if (run.first) g_last_error = run.first_msg;
+    return run.first;
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                 sardine_batch_job* jobs, int32_t n_jobs, void* stream_v, double* device_ms) {
+    BatchRun run;
+    sardine_status st = batch_begin(b, scenes, n_scenes, jobs, n_jobs, (hipStream_t)stream_v, run);
+    if (st) {
+        if (run.launched) (void)hipEventSynchronize(b->ev_out);  // never leave work reading freed tables
+        return st;
+    }
+    return batch_end(b, scenes, jobs, run, device_ms);
 }
 
 sardine_status sardine_batch_autotune(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
@@ -1486,6 +1522,281 @@ sardine_status sardine_batch_autotune(sardine_batch b, const sardine_resident_sc
     if (best) *best = chosen;
     if (best_ms) *best_ms = chosen_ms;
     return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── serving ──────────────────────────────────────────────────────────────
+namespace {
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct ServerReq {
+    uint64_t ticket = 0;
+    sardine_request req{};
+    int64_t rows = 0, cols = 0;
+    size_t out_bytes = 0, work_bytes = 0;
+    float* d_out = nullptr;
+    float* d_speckled = nullptr;
+    sardine_geocoded_image img{};
+    sardine_response resp{};
+};
+
+struct ServerSlot {
+    sardine_batch_s* b = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<ServerReq*> reqs;
+    std::vector<sardine_batch_job> jobs;
+    std::unique_ptr<BatchRun> run{new BatchRun()};
+    size_t work_bytes = 0;
+    bool busy = false;
+    ServerSlot() = default;
+    ServerSlot(const ServerSlot&) = delete;
+    ServerSlot& operator=(const ServerSlot&) = delete;
+    ~ServerSlot() {
+        run.reset();
+        if (b) sardine_batch_destroy(b);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace
+
+struct sardine_server_s {
+    std::vector<sardine_resident_scene> scenes;
+    sardine_server_options opt{};
+    int device = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServerReq*> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<ServerReq>> reqs;
+    std::vector<std::unique_ptr<ServerSlot>> slots;
+    hipStream_t release_stream = nullptr;
+    uint64_t next_ticket = 1;
+    uint64_t used = 0;  // bytes held: admitted outputs + work of batches in flight
+    bool stop = false;
+    std::thread loop;
+    std::string loop_error;
+};
+
+namespace {
+
+// Output grid shape and memory estimate of a request (host only).
+sardine_status request_shape(const sardine_server_s* sv, const sardine_request* r, ServerReq* q) {
+    if (r->scene < 0 || r->scene >= (int)sv->scenes.size())
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "scene index %d outside 0..%d", r->scene, (int)sv->scenes.size() - 1);
+    Grid g;
+    sardine_status st = output_grid(&sv->scenes[(size_t)r->scene]->scene, &r->config, &g);
+    if (st) return st;
+    int64_t nr = r->config.aoi_rows, nc = r->config.aoi_cols;
+    if (nr == 0) {
+        nr = g.rows;
+        nc = g.cols;
+    }
+    if (nr <= 0 || nc <= 0) return fail(SARDINE_ERR_TC_CONFIG, "empty AOI window");
+    if (r->speckle.kind >= 0) {
+        int win = 0;
+        if ((st = validate_filter(&r->speckle, &win))) return st;
+    }
+    q->rows = nr;
+    q->cols = nc;
+    const size_t px = (size_t)nr * (size_t)nc;
+    q->out_bytes = round256(px * sizeof(float)) + (r->speckle.kind >= 0 ? round256(px * sizeof(float)) : 0);
+    // chain records (segments >= 16 columns), row seeds, flat areas, anchors
+    q->work_bytes = px * 2 + (size_t)nr * (16 + 2 * 8 * 260) + 4096;
+    return SARDINE_OK;
+}
+
+// Finishes a completed slot: results, timestamps, memory accounting.
+void complete_slot(sardine_server_s* sv, ServerSlot& sl) {
+    (void)batch_end(sl.b, sv->scenes.data(), sl.jobs.data(), *sl.run, nullptr);
+    const double t = now_us();
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        ServerReq* q = sl.reqs[i];
+        q->resp.status = sl.jobs[i].status;
+        q->resp.image = q->img;
+        q->resp.speckled = q->d_speckled;
+        q->resp.t_done_us = t;
+        q->resp.done = 1;
+    }
+    sl.run.reset(new BatchRun());  // releases the work arena (stream-ordered)
+    sv->used -= sl.work_bytes;
+    sl.work_bytes = 0;
+    sl.reqs.clear();
+    sl.jobs.clear();
+    sl.busy = false;
+}
+
+// Admits pending requests in order into a free slot while memory allows,
+// then launches the batch.  Called with the lock held.
+bool admit_and_launch(sardine_server_s* sv, ServerSlot& sl) {
+    const double t = now_us();
+    while (!sv->pending.empty() && (int)sl.reqs.size() < sv->opt.max_batch) {
+        ServerReq* q = sv->pending.front();
+        const size_t need = q->out_bytes + q->work_bytes;
+        if (sv->used + need > sv->opt.hbm_budget_bytes) break;  // in order: wait for memory
+        const size_t px = (size_t)q->rows * (size_t)q->cols;
+        void* a = nullptr;
+        if (hipMallocAsync(&a, q->out_bytes, sl.stream) != hipSuccess) break;
+        q->d_out = static_cast<float*>(a);
+        q->d_speckled = q->req.speckle.kind >= 0 ? q->d_out + round256(px * sizeof(float)) / sizeof(float) : nullptr;
+        sv->used += need;  // outputs until release, work until the batch completes
+        sl.work_bytes += q->work_bytes;
+        q->resp.t_admit_us = t;
+        std::memset(&q->img, 0, sizeof q->img);
+        q->img.data = q->d_out;
+        sardine_batch_job j;
+        std::memset(&j, 0, sizeof j);
+        j.scene = q->req.scene;
+        j.config = q->req.config;
+        j.out = &q->img;
+        j.speckle = q->req.speckle;
+        j.speckle_out = q->d_speckled;
+        sl.reqs.push_back(q);
+        sl.jobs.push_back(j);
+        sv->pending.pop_front();
+    }
+    if (sl.reqs.empty()) return false;
+    const double tl = now_us();
+    for (ServerReq* q : sl.reqs) {
+        q->resp.t_launch_us = tl;
+        q->resp.batch_size = (int)sl.reqs.size();
+    }
+    sl.busy = true;
+    const sardine_status st = batch_begin(sl.b, sv->scenes.data(), (int)sv->scenes.size(), sl.jobs.data(),
+                                          (int)sl.jobs.size(), sl.stream, *sl.run);
+    if (st) {  // the batch could not be launched: every request fails with this status
+        if (sl.run->launched) (void)hipEventSynchronize(sl.b->ev_out);
+        for (size_t i = 0; i < sl.jobs.size(); ++i)
+            if (!sl.jobs[i].status) sl.jobs[i].status = st;
+        sl.run->launched = false;
+        complete_slot(sv, sl);
+    }
+    return true;
+}
+
+void server_loop(sardine_server_s* sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+        bool progressed = false;
+        for (auto& sp : sv->slots) {
+            ServerSlot& sl = *sp;
+            if (sl.busy && (!sl.run->launched || hipEventQuery(sl.b->ev_out) == hipSuccess)) {
+                complete_slot(sv, sl);
+                progressed = true;
+            }
+        }
+        if (progressed) sv->cv_done.notify_all();
+        for (auto& sp : sv->slots)
+            if (!sp->busy && !sv->pending.empty() && admit_and_launch(sv, *sp)) progressed = true;
+        bool busy = false;
+        for (auto& sp : sv->slots) busy |= sp->busy;
+        if (sv->stop && !busy && (sv->pending.empty() || !progressed)) break;
+        if (!progressed) sv->cv_work.wait_for(lk, std::chrono::microseconds(busy ? 50 : 2000));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_server_create(const sardine_resident_scene* scenes, int32_t n_scenes,
+                                     const sardine_server_options* opt, sardine_server* out) {
+    if (!scenes || n_scenes <= 0 || !opt || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (opt->max_batch < 1 || opt->max_batch > 4096 || opt->slots < 1 || opt->slots > 8 || opt->hbm_budget_bytes == 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "server options out of range");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    std::unique_ptr<sardine_server_s> sv(new sardine_server_s());
+    sv->scenes.assign(scenes, scenes + n_scenes);
+    for (auto s : sv->scenes)
+        if (!s) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null scene");
+    sv->opt = *opt;
+    SD_HIP(hipGetDevice(&sv->device));
+    SD_HIP(hipStreamCreateWithFlags(&sv->release_stream, hipStreamNonBlocking));
+    for (int i = 0; i < opt->slots; ++i) {
+        sv->slots.emplace_back(new ServerSlot());
+        ServerSlot& sl = *sv->slots.back();
+        if ((st = sardine_batch_create(opt->max_batch, opt->use_graph, &sl.b))) return st;
+        SD_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+    }
+    sardine_server_s* raw = sv.get();
+    sv->loop = std::thread(server_loop, raw);
+    *out = sv.release();
+    return SARDINE_OK;
+}
+
+sardine_status sardine_server_submit(sardine_server sv, const sardine_request* r, uint64_t* ticket) {
+    if (!sv || !r || !ticket) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    std::unique_ptr<ServerReq> q(new ServerReq());
+    q->req = *r;
+    sardine_status st = request_shape(sv, r, q.get());
+    if (st) return st;
+    if (q->out_bytes + q->work_bytes > sv->opt.hbm_budget_bytes)
+        return fail(SARDINE_ERR_CAPACITY, "request needs %zu bytes, above the server budget", q->out_bytes + q->work_bytes);
+    q->resp.t_submit_us = now_us();
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (sv->stop) return fail(SARDINE_ERR_INVALID_ARGUMENT, "server is stopping");
+    q->ticket = sv->next_ticket++;
+    *ticket = q->ticket;
+    sv->pending.push_back(q.get());
+    sv->reqs.emplace(q->ticket, std::move(q));
+    sv->cv_work.notify_one();
+    return SARDINE_OK;
+}
+
+sardine_status sardine_server_wait(sardine_server sv, uint64_t ticket, int32_t block, sardine_response* out) {
+    if (!sv || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+    ServerReq* q = it->second.get();
+    if (block) sv->cv_done.wait(lk, [&] { return q->resp.done != 0; });
+    *out = q->resp;
+    return SARDINE_OK;
+}
+
+sardine_status sardine_server_release(sardine_server sv, uint64_t ticket) {
+    if (!sv) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+    ServerReq* q = it->second.get();
+    if (!q->resp.done) return fail(SARDINE_ERR_INVALID_ARGUMENT, "request %llu has not finished", (unsigned long long)ticket);
+    if (q->d_out) {
+        (void)hipFreeAsync(q->d_out, sv->release_stream);
+        sv->used -= q->out_bytes;
+    }
+    sv->reqs.erase(it);
+    sv->cv_work.notify_one();
+    return SARDINE_OK;
+}
+
+void sardine_server_destroy(sardine_server sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+        // requests not yet admitted are dropped
+        for (ServerReq* q : sv->pending) {
+            q->resp.status = SARDINE_ERR_INVALID_ARGUMENT;
+            q->resp.done = 1;
+        }
+        sv->pending.clear();
+        sv->cv_work.notify_one();
+        sv->cv_done.notify_all();
+    }
+    if (sv->loop.joinable()) sv->loop.join();
+    for (auto& kv : sv->reqs)
+        if (kv.second->d_out) (void)hipFreeAsync(kv.second->d_out, sv->release_stream);
+    (void)hipStreamSynchronize(sv->release_stream);
+    sv->slots.clear();
+    (void)hipStreamDestroy(sv->release_stream);
+    delete sv;
 }
 
 }  // extern "C"
