@@ -48,8 +48,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64"], default="iw_full",
-                    help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile")
+    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve"], default="iw_full",
+                    help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
+                         "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request")
+    ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--quiet", action="store_true")
@@ -103,6 +105,8 @@ def main():
     L = _native.lib()
     if args.workload == "batch64":
         return bench_batch64(args, dev)
+    if args.workload == "serve":
+        return bench_serve(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -408,6 +412,160 @@ def bench_batch64(args, dev):
     print(json.dumps(line), flush=True)
     b.close()
     rs.close()
+
+
+def bench_serve(args, dev):
+    """Four full IW scenes resident in HBM (distinct seeds and orbits); AOI
+    requests with log-uniform square sides 256..8192, random scene, CRS
+    (UTM or EPSG:4326), resampler and speckle filter.  The native server
+    (HBM-budget admission, batches formed on arrival, two batches in
+    flight) against one plain terrain_correction (+ speckle) call per
+    request, FIFO: saturated throughput of both, then latency at the same
+    Poisson arrival rate (80 % of the plain loop's capacity)."""
+    import math
+    import threading
+
+    import numpy as np
+    import torch
+
+    from sardine_amd import abi, api, synthetic
+    from sardine_amd.workload import DeviceScene
+
+    t0 = time.time()
+    scenes = []
+    for seed in (1, 2, 3, 4):
+        sc = synthetic.scene_iw_full(seed, out_rows=18000, out_cols=25000)
+        ds = DeviceScene.build(sc, dev)
+        m = ds.prepare_merged()
+        torch.cuda.synchronize()
+        ds.raws, ds._swc = [], None  # the CInt16 rasters are not needed once calibrated
+        grids = {}
+        for crs in ("utm", "wgs84"):
+            cfg = ds.tc_config() if crs == "utm" else ds.tc_config(crs=abi.CRS_WGS84, utm_zone=0,
+                                                                     pixel_spacing_deg=10.0 / 111320.0)
+            grids[crs] = (cfg, api.output_grid(ds.scene, cfg)[:2])
+        scenes.append((ds, m, api.ResidentScene(m, ds.scene, ds.dem), grids))
+    torch.cuda.empty_cache()
+    log(f"4 scenes resident in {time.time() - t0:.1f}s")
+
+    rng = np.random.default_rng(2024)
+    F = api.SpeckleFilter
+    filters = [None, F.Lee(5, 4.0), F.Frost(5, 1.0), F.GammaMap(7, 4.0), F.RefinedLee(4.0)]
+
+    def make_requests(n):
+        out = []
+        for _ in range(n):
+            si = int(rng.integers(0, 4))
+            ds, _, _, grids = scenes[si]
+            base, (rows, cols) = grids["utm" if rng.random() < 0.5 else "wgs84"]
+            side = int(round(math.exp(rng.uniform(math.log(256), math.log(8192)))))
+            h, w = min(side, rows), min(side, cols)
+            r0, c0 = int(rng.integers(0, rows - h + 1)), int(rng.integers(0, cols - w + 1))
+            cfg = api.TerrainCorrectionConfig(pixel_spacing_deg=base.pixel_spacing_deg, crs=base.crs,
+                                              utm_zone=base.utm_zone, aoi=(r0, c0, h, w),
+                                              resampling=int(rng.integers(0, 3)))
+            out.append((si, cfg, filters[int(rng.integers(0, len(filters)))], h * w))
+        return out
+
+    n = max(8, args.requests)
+    reqs = make_requests(n)
+    px = sum(r[3] for r in reqs)
+    log(f"{n} requests, {px / 1e9:.2f} Gpx")
+    stream = torch.cuda.current_stream(dev)
+
+    def plain_one(r):
+        si, cfg, filt, _ = r
+        ds, m, _, _ = scenes[si]
+        img = api.terrain_correction(m, ds.scene, ds.dem, cfg, stream=stream)
+        if filt is not None:
+            filt.apply(img.data, img.cols, img.rows, stream=stream)
+        torch.cuda.synchronize()
+
+    def plain_run(arrivals):
+        start = time.perf_counter()
+        lat = []
+        for r, a in zip(reqs, arrivals):
+            now = time.perf_counter() - start
+            if now < a:
+                time.sleep(a - now)
+            plain_one(r)
+            lat.append((time.perf_counter() - start - a) * 1e3)
+        return time.perf_counter() - start, lat
+
+    srv = api.Server([s[2] for s in scenes], hbm_budget_bytes=64 << 30, max_batch=64, slots=2)
+
+    def served_run(arrivals):
+        tickets = [None] * n
+        start = time.perf_counter()
+
+        def submitter():
+            for i, (r, a) in enumerate(zip(reqs, arrivals)):
+                now = time.perf_counter() - start
+                if now < a:
+                    time.sleep(a - now)
+                tickets[i] = srv.submit(r[0], r[1], r[2])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, sizes = [], []
+        for i in range(n):
+            while tickets[i] is None:
+                time.sleep(20e-6)
+            res = srv.wait(tickets[i])
+            assert res.status == 0, res.status
+            lat.append(res.latency_ms)
+            sizes.append(res.batch_size)
+            srv.release(tickets[i])
+        th.join()
+        return time.perf_counter() - start, lat, sizes
+
+    for r in reqs[:8]:  # warm-up of both paths
+        plain_one(r)
+    zero = [0.0] * n
+    t_plain_sat, _ = plain_run(zero)
+    t_srv_sat, _, sizes_sat = served_run(zero)
+    cap_plain = n / t_plain_sat
+    rate = 0.8 * cap_plain
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n)).tolist()
+    t_plain, lat_plain = plain_run(arrivals)
+    t_srv, lat_srv, sizes = served_run(arrivals)
+    srv.close()
+
+    def pct(v, q):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+    log(f"saturated: plain {px / t_plain_sat / 1e6:.0f} Mpx/s, served {px / t_srv_sat / 1e6:.0f} Mpx/s "
+        f"(mean batch {np.mean(sizes_sat):.1f})")
+    log(f"Poisson {rate:.0f} req/s: plain p50 {pct(lat_plain, .5):.1f} p99 {pct(lat_plain, .99):.1f} ms; served "
+        f"p50 {pct(lat_srv, .5):.1f} p99 {pct(lat_srv, .99):.1f} ms (mean batch {np.mean(sizes):.1f})")
+    line = {
+        "metric": "served AOI geocode Mpx/s (4 resident IW scenes, Poisson requests) & p50/p99 latency",
+        "value": round(px / t_srv_sat / 1e6, 3),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 8,
+        "ms_per_step": round(t_srv_sat * 1e3 / n, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "serve_4x_iw_full_aoi256-8192_utm_or_wgs84_mixed", "requests": n,
+                   "request_px": px, "hbm_budget_gb": 64, "max_batch": 64, "slots": 2},
+        "saturated": {"served_mpx_s": round(px / t_srv_sat / 1e6, 3),
+                      "plain_mpx_s": round(px / t_plain_sat / 1e6, 3),
+                      "speedup": round(t_plain_sat / t_srv_sat, 3), "served_mean_batch": float(np.mean(sizes_sat))},
+        "poisson": {"rate_req_s": round(rate, 2),
+                    "served": {"mpx_s": round(px / t_srv / 1e6, 3), "p50_ms": round(pct(lat_srv, .5), 3),
+                               "p99_ms": round(pct(lat_srv, .99), 3), "mean_batch": float(np.mean(sizes))},
+                    "plain": {"mpx_s": round(px / t_plain / 1e6, 3), "p50_ms": round(pct(lat_plain, .5), 3),
+                              "p99_ms": round(pct(lat_plain, .99), 3)}},
+    }
+    print(json.dumps(line), flush=True)
+    for s in scenes:
+        s[2].close()
 
 
 if __name__ == "__main__":
