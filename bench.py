@@ -48,9 +48,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve"], default="iw_full",
+    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32"], default="iw_full",
                     help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
-                         "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request")
+                         "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request; "
+                         "fp32: the FP32 scene-centred variant's errors and speed against FP64")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
@@ -107,6 +108,8 @@ def main():
         return bench_batch64(args, dev)
     if args.workload == "serve":
         return bench_serve(args, dev)
+    if args.workload == "fp32":
+        return bench_fp32(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -566,6 +569,79 @@ def bench_serve(args, dev):
     print(json.dumps(line), flush=True)
     for s in scenes:
         s[2].close()
+
+
+def bench_fp32(args, dev):
+    """The full IW scene geocoded by the FP32 scene-centred variant and by the
+    FP64 path (which the parity tests hold to the oracle), per resampler and
+    with flattening: max/RMS azimuth-time error (s), range-pixel error and
+    σ0 dB error of the variant over the pixels valid in both, mask
+    agreement, and the TC kernel time of each."""
+    import numpy as np
+    import torch
+
+    from sardine_amd import _native, abi, api, synthetic
+    from sardine_amd.workload import DeviceScene
+
+    L = _native.lib()
+    sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+    ds = DeviceScene.build(sc, dev)
+    m = ds.prepare_merged()
+    torch.cuda.synchronize()
+    ds.raws, ds._swc = [], None
+    rows, cols, _ = api.output_grid(ds.scene, ds.tc_config())
+    ati = sc.ati
+    variants = [("bilinear", dict(resampling=abi.RESAMPLE_BILINEAR, compute_lia=True)),
+                ("bicubic", dict(resampling=abi.RESAMPLE_BICUBIC, compute_lia=True)),
+                ("lanczos3", dict(resampling=abi.RESAMPLE_LANCZOS3, compute_lia=True)),
+                ("bilinear_flatten", dict(resampling=abi.RESAMPLE_BILINEAR, compute_lia=True, flatten=True))]
+    out = {}
+
+    def run(prec, kw, coords):
+        cfg = ds.tc_config(precision=prec, **kw)
+        L.sardine_set_profiling(1)
+        img = api.terrain_correction(m, ds.scene, ds.dem, cfg, radar_coords=coords)
+        torch.cuda.synchronize()
+        ms = L.sardine_last_kernel_ms(abi.KERNEL_TC_SEGMENT)
+        L.sardine_set_profiling(0)
+        return img, ms
+
+    for name, kw in variants:
+        ref, _ = run(abi.PRECISION_FP64, kw, True)
+        v32, _ = run(abi.PRECISION_FP32_CENTRED, kw, True)
+        _, ms64 = run(abi.PRECISION_FP64, kw, False)
+        _, ms32 = run(abi.PRECISION_FP32_CENTRED, kw, False)
+        a, b = ref.radar_coords, v32.radar_coords
+        both = torch.isfinite(a[..., 0]) & torch.isfinite(b[..., 0])
+        dl = (b[..., 0] - a[..., 0])[both].double()
+        ds_ = (b[..., 1] - a[..., 1])[both].double()
+        s64, s32 = ref.data, v32.data
+        ok = torch.isfinite(s64) & torch.isfinite(s32) & (s64 > 0) & (s32 > 0)
+        ddb = (10.0 * torch.log10(s32[ok].double() / s64[ok].double())).abs()
+        mask_eq = float((ref.mask == v32.mask).double().mean())
+        out[name] = {
+            "azimuth_time_err_s": {"max": float(dl.abs().max() * ati), "rms": float(dl.pow(2).mean().sqrt() * ati)},
+            "range_px_err": {"max": float(ds_.abs().max()), "rms": float(ds_.pow(2).mean().sqrt())},
+            "sigma0_db_err": {"max": float(ddb.max()), "rms": float(ddb.pow(2).mean().sqrt()),
+                              "p99": float(torch.quantile(ddb[:: max(1, ddb.numel() // 2000000)].float(), 0.99))},
+            "mask_agreement": mask_eq,
+            "valid_fp64": ref.valid_pixel_count, "valid_fp32": v32.valid_pixel_count,
+            "tc_kernel_ms": {"fp64": round(ms64, 3), "fp32": round(ms32, 3)},
+            "mpx_s": {"fp64": round(rows * cols / ms64 / 1e3, 1), "fp32": round(rows * cols / ms32 / 1e3, 1)},
+        }
+        log(f"{name}: {json.dumps(out[name])}")
+        del ref, v32
+        torch.cuda.empty_cache()
+    line = {
+        "metric": "FP32 scene-centred variant vs FP64: errors and TC Mpx/s",
+        "value": out["bilinear"]["mpx_s"]["fp32"],
+        "unit": "Mpx/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": out["bilinear"]["tc_kernel_ms"]["fp32"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "fp32_iw_full_utm10m_25000x18000", "grid": [rows, cols]},
+        "variants": out,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -33,6 +33,8 @@
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start,
                                         hipEvent_t seg_end);
 extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t stream);
+extern "C" hipError_t sd_launch_tc_anchors(const sd::TcParams* p, hipStream_t stream);
+extern "C" hipError_t sd_launch_tc_fp32(const sd::TcParams* p, float* ends, hipStream_t stream);
 extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* values, const int64_t* offsets,
                                           int n_vec, int64_t dense_cols, float* dense, hipStream_t stream);
 extern "C" hipError_t sd_launch_cal_merge(const sd::CalParams* p, hipStream_t stream);
@@ -427,8 +429,6 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     const bool compute_lia = cfg->compute_lia != 0;
     if (!out->data || (compute_lia && (!out->cos_lia || !out->mask)))
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "output buffers missing");
-    if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED)
-        return fail(SARDINE_ERR_TC_CONFIG, "the FP32 scene-centred variant is not part of this build");
     pl->r0 = r0; pl->c0 = c0; pl->nr = nr; pl->nc = nc;
     const int anchors = cfg->projector_anchors == 0 ? 256 : cfg->projector_anchors;
     const double sp = cfg->pixel_spacing;
@@ -526,6 +526,29 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     return SARDINE_OK;
 }
 
+// The FP32 variant's orbit: node times in seconds from t_ref, positions in
+// metres from `centre`, barycentric weights of the rounded node times.
+sd::OrbitF32 make_orbit_f32(const sd::OrbitDev& o, const double centre[3], int64_t t_ref_us) {
+    sd::OrbitF32 f;
+    std::memset(&f, 0, sizeof f);
+    f.n = o.n;
+    for (int i = 0; i < o.n; ++i) {
+        f.t[i] = (float)((double)(o.t_us[i] - t_ref_us) * 1e-6);
+        for (int c = 0; c < 3; ++c) {
+            f.pos[i][c] = (float)(o.pos[i][c] - centre[c]);
+            f.vel[i][c] = (float)o.vel[i][c];
+        }
+    }
+    for (int s = 0; s + sd::kLagrange <= o.n; ++s)
+        for (int k = 0; k < sd::kLagrange; ++k) {
+            double prod = 1.0;
+            for (int j = 0; j < sd::kLagrange; ++j)
+                if (j != k) prod *= (double)f.t[s + k] - (double)f.t[s + j];
+            f.bary[s][k] = (float)(1.0 / prod);
+        }
+    return f;
+}
+
 // Segment length of the chain decomposition: `requested` > 0 forces it,
 // otherwise aim for ~2 waves of 64 lanes per SIMD over the whole grid.
 void set_decomposition(sd::TcParams& p, int64_t requested) {
@@ -548,7 +571,10 @@ struct TcWork {
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
     TcWork w;
-    w.records = round256((size_t)pl.nr * pl.p.nseg * sizeof(sd::SegRecord));
+    // the FP32 variant solves every pixel on its own: no chain records
+    w.records = cfg->precision == SARDINE_PRECISION_FP32_CENTRED
+                    ? 0
+                    : round256((size_t)pl.nr * pl.p.nseg * sizeof(sd::SegRecord));
     w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * pl.p.anchor_stride * sizeof(double)) : 0;
     w.seed = round256((size_t)pl.nr * sizeof(int64_t));
     w.flat = pl.need_geom ? round256((size_t)pl.nr * sizeof(double)) : 0;
@@ -696,7 +722,30 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     pl.p.counters = d_cnt.as<unsigned long long>();
     pl.p.error_flag = reinterpret_cast<int*>(d_cnt.as<unsigned long long>() + sd::kNumCounters);
     SD_HIP(sd_launch_tc_rows(&pl.q, stream));
-    {
+    DevBuf d_orb32;
+    if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED) {
+        // scene centre: the bounding box centre on the ellipsoid; reference
+        // time: the middle of the orbit
+        const sardine_bbox& bb = scene->bounding_box;
+        const sd::d3 c = sd::geodetic_to_ecef(0.5 * (bb.min_lat_deg + bb.max_lat_deg),
+                                              0.5 * (bb.min_lon_deg + bb.max_lon_deg), 0.0);
+        pl.p.centre[0] = c.x;
+        pl.p.centre[1] = c.y;
+        pl.p.centre[2] = c.z;
+        pl.p.t_ref_us = orb.n > 0 ? (orb.t_us[0] + orb.t_us[orb.n - 1]) / 2 : 0;
+        pl.p.tol32 = 3e-5f;  // above the ~1e-5 s rounding noise of the FP32 Doppler residual
+        const sd::OrbitF32 o32 = make_orbit_f32(orb, pl.p.centre, pl.p.t_ref_us);
+        SD_HIP(upload(d_orb32, &o32, 1, stream));
+        pl.p.orbit32 = d_orb32.as<sd::OrbitF32>();
+        SD_HIP(sd_launch_tc_anchors(&pl.p, stream));
+        KernelTimer kt(SARDINE_KERNEL_TC, stream);
+        KernelTimer kt2(SARDINE_KERNEL_TC_SEGMENT, stream);
+        DevBuf d_ends;
+        SD_HIP(d_ends.alloc((size_t)pl.nr * 2 * sizeof(float), stream));
+        SD_HIP(sd_launch_tc_fp32(&pl.p, d_ends.as<float>(), stream));
+        kt2.stop();
+        kt.stop();
+    } else {
         KernelTimer kt(SARDINE_KERNEL_TC, stream);
         // the per-pixel chain kernel alone is timed as its own family
         TimerSlot* seg = nullptr;
@@ -1317,6 +1366,8 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
         const sardine_resident_scene_s* r = st ? nullptr : scenes[jb.scene];
         if (!st && r->device != b->device) st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: scene on another device", i);
         if (!st) st = plan_tc(&r->merged, &r->scene, &r->dem, &jb.config, jb.out, r->dev, &run.plans[(size_t)i]);
+        if (!st && jb.config.precision == SARDINE_PRECISION_FP32_CENTRED)
+            st = fail(SARDINE_ERR_TC_CONFIG, "job %d: the FP32 variant runs through sardine_terrain_correction", i);
         if (!st && jb.speckle.kind >= 0) {
             int win = 0;
             st = validate_filter(&jb.speckle, &win);

@@ -82,13 +82,19 @@ extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t 
 }
 
 // seg_start / seg_end: optional events recorded around the segment kernel.
+// project_row anchors of the AOI rows (UTM grids only).
+extern "C" hipError_t sd_launch_tc_anchors(const sd::TcParams* p, hipStream_t stream) {
+    if (p->crs == 0) return hipSuccess;
+    const int64_t n = p->rows * p->anchor_stride;
+    hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start,
                                         hipEvent_t seg_end) {
     const bool utm = p->crs != 0;
-    if (utm) {
-        const int64_t n = p->rows * p->anchor_stride;
-        hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
-    }
+    hipError_t e = sd_launch_tc_anchors(p, stream);
+    if (e != hipSuccess) return e;
     switch (p->resampling * 2 + (utm ? 1 : 0)) {
         case 0: launch_chain<0, false>(p, stream, seg_start, seg_end); break;
         case 1: launch_chain<0, true>(p, stream, seg_start, seg_end); break;

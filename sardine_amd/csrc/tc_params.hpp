@@ -24,8 +24,24 @@ struct SegRecord {
     int32_t pad;
 };
 
+// Single-precision orbit of the FP32 scene-centred variant: times in
+// seconds from a reference time, positions in metres from a scene centre.
+struct alignas(16) OrbitF32 {
+    int32_t n;
+    int32_t pad[3];
+    float t[kMaxSv];
+    float pos[kMaxSv][3];
+    float vel[kMaxSv][3];
+    float bary[kMaxSv][kLagrange];
+};
+
 struct TcParams {
     const OrbitDev* orbit;
+    // FP32 scene-centred variant (precision SARDINE_PRECISION_FP32_CENTRED)
+    const OrbitF32* orbit32;
+    double centre[3];       // scene centre, ECEF metres
+    int64_t t_ref_us;       // reference time, µs relative to the first state vector
+    float tol32;            // Newton tolerance of the variant, seconds
     // radar image (MergedSigma0 buffers)
     const float* img;
     const float* nesz;      // NULL → no NESZ masking
