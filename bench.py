@@ -152,7 +152,8 @@ def main():
 
     for i in range(args.warmup):
         img = step()
-        log(f"warmup {i}: valid {img.valid_pixel_count}/{img.total_pixel_count}, redo {img.chain_redo_rounds}")
+        log(f"warmup {i}: valid {img.valid_pixel_count}/{img.total_pixel_count}, redo {img.chain_redo_rounds}, "
+            f"newton iterations {img.newton_iter_histogram[:8]}")
     torch.cuda.synchronize()
 
     L.sardine_set_profiling(1)

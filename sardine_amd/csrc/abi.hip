@@ -343,6 +343,7 @@ sd::OrbitDev make_orbit_dev(const sardine_orbit& o) {
         }
     }
     sd::orbit_fill_bary(orb);
+    if (orb.n >= 2 && orb.t_us[orb.n - 1] > 0) orb.guess_scale = (double)(orb.n - 1) / (double)orb.t_us[orb.n - 1];
     return orb;
 }
 
@@ -390,10 +391,17 @@ sardine_status validate_tc(const sardine_radar_image* merged, const sardine_scen
         return fail(SARDINE_ERR_TC_CONFIG, "bad precision selector");
     if (scene->orbit.count > sd::kMaxSv || scene->orbit.count < 0)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", scene->orbit.count, sd::kMaxSv);
+    for (int i = 1; i < scene->orbit.count; ++i)
+        if (!(scene->orbit.state_vectors[i].time_us > scene->orbit.state_vectors[i - 1].time_us))
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "state vector times must be strictly increasing (vector %d)", i);
     if (!merged->data || merged->lines <= 0 || merged->samples <= 0)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "radar image has no raster");
+    if (merged->lines >= INT32_MAX || merged->samples >= INT32_MAX)
+        return fail(SARDINE_ERR_CAPACITY, "radar image dimensions must be below 2^31");
     if (!dem->data || dem->width < 2 || dem->height < 2 || !(dem->pixel_deg > 0.0))
         return fail(SARDINE_ERR_TC_DEM, "DEM raster must be at least 2x2 with positive spacing");
+    if (dem->width >= INT32_MAX || dem->height >= INT32_MAX)
+        return fail(SARDINE_ERR_CAPACITY, "DEM dimensions must be below 2^31");
     if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 && !cfg->geoid.grid_73x144)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "geoid grid pointer is null");
     return SARDINE_OK;
@@ -476,6 +484,7 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     p.rps = merged->range_pixel_spacing_m;
     p.ati = ati;
     p.line_per_us = 1.0 / (1e6 * ati);
+    p.inv_ati = 1.0 / ati;
     p.inv_rps = 1.0 / p.rps;
     p.dem = dem->data;
     p.dem_w = dem->width;
@@ -486,8 +495,10 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     p.dem_inv_pix = 1.0 / dem->pixel_deg;
     p.dem_max_row = (double)(dem->height - 1);
     p.dem_max_col = (double)(dem->width - 1);
-    p.dem_lim_row = dem->height - 1 > 0 ? dem->height - 2 : 0;
-    p.dem_lim_col = dem->width >= 2 ? dem->width - 2 : 0;
+    p.dem_hi_row = p.dem_max_row + 0.5;
+    p.dem_hi_col = p.dem_max_col + 0.5;
+    p.dem_lim_row = (int)(dem->height - 2);  // validated: 2 <= width, height < 2^31
+    p.dem_lim_col = (int)(dem->width - 2);
     p.crs = utm ? 1 : 0;
     p.utm_zone = cfg->utm_zone;
     p.utm_south = south ? 1 : 0;

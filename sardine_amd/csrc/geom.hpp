@@ -33,6 +33,19 @@ SD_HD d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 SD_HD double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 SD_HD d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 
+// x / d correctly rounded, for a divisor d known with its reciprocal
+// y = RN(1/d): q0 = x·y is within an ulp of the quotient, r = x - q0·d is
+// exact with an FMA, and q0 + r·y rounds to the correctly rounded quotient
+// (Markstein, IBM J. Res. Dev. 34(1), 1990).  Three FMA-pipe operations
+// instead of the ~10 of a general division; the reference's divisions by
+// constants (1e6, the DEM pixel size, the azimuth interval, the range pixel
+// spacing) are reproduced bit for bit.
+SD_HD double div_rn(double x, double d, double y) {
+    const double q0 = x * y;
+    const double r = fma(-q0, d, x);
+    return fma(r, y, q0);
+}
+
 // geodesy.rs:38-55 (same operation order: lat_deg*PI/180, n = a/sqrt(1-e2 s^2))
 SD_HD d3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
     double lat = lat_deg * kPi / 180.0;
@@ -79,6 +92,7 @@ SD_HD d3 ecef_from_sc(double sl, double cl, double so, double co, double h) {
 struct alignas(16) OrbitDev {
     int32_t n;
     int32_t pad;
+    double guess_scale;  // (n - 1) / (t_us[n-1] - t_us[0]): first guess of the bracketing node
     int64_t t_us[kMaxSv];
     double tsec[kMaxSv];
     double pos[kMaxSv][3];
@@ -119,8 +133,9 @@ SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
     const int n = o.n;
     if (n < kLagrange) return ORB_INSUFFICIENT;
     if (t < o.t_us[0] || t > o.t_us[n - 1]) return ORB_OUTSIDE;
-    const double ts = (double)(t - o.t_us[0]) / 1e6;
-    int c = (int)((double)(n - 1) * (double)(t - o.t_us[0]) / (double)(o.t_us[n - 1] - o.t_us[0]));
+    const double tr = (double)(t - o.t_us[0]);
+    const double ts = div_rn(tr, 1e6, 1e-6);
+    int c = (int)(tr * o.guess_scale);  // a guess: the walks below decide
     c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
     while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
     while (c > 0 && o.t_us[c] > t) --c;
@@ -128,13 +143,12 @@ SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
     if (c < 4) start = 0;
     else if (c + 4 >= n) start = n - kLagrange;
     else start = c - 3;
+    // The 1 ns guard: times are whole µs and t_us[c] <= t < t_us[c+1], so
+    // |t - t_j| <= 1e-9 s holds for node c alone, when t is its time.
+    const bool guard = t == o.t_us[c];
     double d[kLagrange], pre[kLagrange + 1], suf[kLagrange + 1];
-    bool guard = false;
 #pragma unroll
-    for (int j = 0; j < kLagrange; ++j) {
-        d[j] = ts - o.tsec[start + j];
-        guard |= !(fabs(d[j]) > 1e-9);
-    }
+    for (int j = 0; j < kLagrange; ++j) d[j] = ts - o.tsec[start + j];
     pre[0] = 1.0;
 #pragma unroll
     for (int j = 0; j < kLagrange; ++j) pre[j + 1] = pre[j] * d[j];
@@ -255,29 +269,50 @@ enum : int { TG_VALID = 0, TG_FORESHORTENING = 1, TG_LAYOVER = 2, TG_SHADOW = 3 
 // moves each neighbour by ~1e-10 m against evaluating sin((lon+s)π/180)
 // directly, far below anything the facet normal or the 0.05 / 2.0 mask
 // thresholds resolve.  The east/west points have the centre's latitude and
-// so its prime-vertical radius n_c.  The look vector is normalised with one
-// reciprocal (a few ulp from three divisions).
+// so its prime-vertical radius n_c; the north/south radii follow from n_c by
+// the binomial series of n = a (x_c (1 + ε))^-1/2 in ε = (x - x_c) / x_c,
+// x = 1 - e² sin²φ (|ε| < 1e-5 for steps below 1°; the series to ε³ is exact
+// to ~1e-20).  The look vector is not normalised: lp = n·(S - T) / |S - T|
+// with one reciprocal square root, cos_lia = lp / |n| likewise, and
+// w = lp · (1 / flat_area).  Each quantity is within a few ulp of the
+// reference's three divisions and two square roots, which moves w and
+// cos_lia by ~1e-15 relative: the masks (thresholds 0, 0.05, 2) do not
+// resolve it.
+SD_HD double prime_vertical_near(double n_c, double sl, double sl2) {
+    const double inv_xc = (n_c * (1.0 / kA)) * (n_c * (1.0 / kA));
+    const double eps = kE2 * ((sl - sl2) * (sl + sl2)) * inv_xc;
+    return n_c * (1.0 + eps * (-0.5 + eps * (0.375 - 0.3125 * eps)));
+}
+
+SD_HD double rsqrt_d(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return rsqrt(x);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 SD_HD int terrain_geometry_sc(double sl, double cl, double so, double co, double n_c, double ss, double cs,
-                              double he, double hw, double hn, double hs, d3 sat, d3 target, double flat_area,
+                              double he, double hw, double hn, double hs, d3 sat, d3 target, double inv_flat_area,
                               double* w_out, double* cos_lia_out) {
     const double soe = so * cs + co * ss, coe = co * cs - so * ss;  // lon + s
     const double sow = so * cs - co * ss, cow = co * cs + so * ss;  // lon - s
     const double sln = sl * cs + cl * ss, cln = cl * cs - sl * ss;  // lat + s
     const double sls = sl * cs - cl * ss, cls = cl * cs + sl * ss;  // lat - s
     d3 pe = ecef_from_n(n_c, sl, cl, soe, coe, he), pw = ecef_from_n(n_c, sl, cl, sow, cow, hw);
-    d3 pn = ecef_from_sc(sln, cln, so, co, hn), ps = ecef_from_sc(sls, cls, so, co, hs);
+    d3 pn = ecef_from_n(prime_vertical_near(n_c, sl, sln), sln, cln, so, co, hn);
+    d3 ps = ecef_from_n(prime_vertical_near(n_c, sl, sls), sls, cls, so, co, hs);
     d3 tn = cross(pe - pw, pn - ps);
     d3 lk = sat - target;
-    double ln = sqrt(dot(lk, lk));
-    const double il = 1.0 / ln;
-    d3 look = ln == 0.0 ? d3{0, 0, 0} : d3{lk.x * il, lk.y * il, lk.z * il};
-    double lp = dot(tn, look);
-    if (lp <= 0.0) return TG_SHADOW;
-    double ta = sqrt(dot(tn, tn));
-    if (ta == 0.0) return TG_SHADOW;
-    double cosl = lp / ta;
+    const double l2 = dot(lk, lk);
+    const double lr = dot(tn, lk);
+    if (!(l2 > 0.0) || lr <= 0.0) return TG_SHADOW;  // |S - T| = 0: zero look vector, lp = 0
+    const double lp = lr * rsqrt_d(l2);
+    const double t2 = dot(tn, tn);
+    if (t2 == 0.0) return TG_SHADOW;
+    double cosl = lp * rsqrt_d(t2);
     cosl = cosl < 0.0 ? 0.0 : (cosl > 1.0 ? 1.0 : cosl);
-    double w = lp / flat_area;
+    double w = lp * inv_flat_area;
     *w_out = w;
     *cos_lia_out = cosl;
     if (w < 0.05) return TG_FORESHORTENING;

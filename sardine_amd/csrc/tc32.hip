@@ -224,14 +224,8 @@ __device__ __forceinline__ PixelOut finish32(const TcParams& p, const PixelPos& 
         return o;
     }
     if (p.need_geom) {
-        const double st = p.step_deg;
         float he = 0.0f, hw = 0.0f, hn = 0.0f, hs = 0.0f;
-        const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
-        bool ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon + st), &he);
-        ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon - st), &hw) && ok;
-        ok = dem_bilinear(p, dem_row_axis(p, q.lat + st), cc, &hn) && ok;
-        ok = dem_bilinear(p, dem_row_axis(p, q.lat - st), cc, &hs) && ok;
-        if (!ok) {
+        if (!nb_heights(p, q.lat, q.lon, q.nb, &he, &hw, &hn, &hs)) {
             o.mask = 9;
             if (p.flatten) {
                 o.cat = CAT_FLAT;
@@ -312,7 +306,7 @@ __global__ __launch_bounds__(256) void tc32_seed_kernel(TcParams p, float* ends)
     const double t_row = (double)(p.row_seed[rr] - p.t_ref_us) * 1e-6;
     double e[2] = {t_row, t_row};
     for (int k = 0; k < 2; ++k) {
-        const PixelPos q = prep_pixel<UTM>(p, p.col0 + (k ? p.cols - 1 : 0), row_y, alon, alat);
+        const PixelPos q = prep_pixel<UTM, false>(p, p.col0 + (k ? p.cols - 1 : 0), row_y, alon, alat);
         if (!q.dem_ok) continue;
         OrbitCache cache = orbit_cache_empty();
         int64_t t = 0;

@@ -46,10 +46,13 @@ struct PixelPos {
     double n;               // prime-vertical radius at lat
     d3 target;
     bool dem_ok;
+    NbTaps nb;  // the terrain geometry's neighbour taps (when p.need_geom)
 };
 
-// Output-grid position of a column and its DEM/ECEF target.
-template <bool UTM>
+// Output-grid position of a column and its DEM/ECEF target.  The centre's
+// and (for the terrain geometry) the neighbours' DEM taps are issued
+// together: one memory round trip instead of two.
+template <bool UTM, bool NB = true>
 __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, double row_y, const double* alon,
                                                const double* alat) {
     PixelPos q;
@@ -69,8 +72,11 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
         q.lon = alon[a] + t * (alon[a + 1] - alon[a]);
         q.lat = alat[a] + t * (alat[a + 1] - alat[a]);
     }
-    float hf = 0.0f;
-    q.dem_ok = dem_sample(p, q.lat, q.lon, &hf) && isfinite(hf);
+    const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
+    const DemTaps ct = dem_load(p, rc, cc);
+    if (NB && p.need_geom) q.nb = nb_load(p, q.lat, q.lon, rc, cc);
+    const float hf = dem_interp(rc, cc, ct);
+    q.dem_ok = isfinite(hf);
     q.target = {0.0, 0.0, 0.0};
     q.sl = q.cl = q.so = q.co = q.n = 0.0;
     if (q.dem_ok) {
@@ -92,7 +98,7 @@ struct PixelOut {
 // Everything after the Newton solve (terrain_correction.rs:1558-1739).
 template <int RS>
 __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelPos& q, int oc, int64_t t, d3 sat,
-                                                 double flat_area) {
+                                                 double inv_flat_area) {
     PixelOut o;
     o.sigma0 = __builtin_nanf("");
     o.cos_lia = __builtin_nanf("");
@@ -115,29 +121,35 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
     }
     d3 rv = q.target - sat;
     const double slant = sqrt(dot(rv, rv));
-    const double line = (double)(t - p.az_start_us) * p.line_per_us;
-    const double sample = (slant - p.near_range_m) * p.inv_rps;
+    // to_image_coords (terrain_correction.rs:742-744): µs / 1e6 / ati and
+    // (R - R0) / rps, each division correctly rounded as in the reference
+    const double line = div_rn(div_rn((double)(t - p.az_start_us), 1e6, 1e-6), p.ati, p.inv_ati);
+    const double sample = div_rn(slant - p.near_range_m, p.rps, p.inv_rps);
     o.line = line;
     o.sample = sample;
     if (line < -0.5 || line > (double)p.lines - 0.5 || sample < -0.5 || sample > (double)p.samples - 0.5) {
         o.cat = CAT_OUT;
         return o;
     }
-    float sigma0 = resample<RS>(p, line, sample);
+    // the NESZ lookup's taps (same position) are loaded with the image's
+    const Bilin nb_idx = bilin_index(p.lines, p.samples, line, sample);
+    float sigma0;
+    DemTaps nzt = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (RS == 0) {
+        const DemTaps it = bilin_load(p.img, p.samples, nb_idx);
+        if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
+        sigma0 = bilin_interp(nb_idx, it);
+    } else {
+        if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
+        sigma0 = resample<RS>(p, line, sample);
+    }
     if (isnan(sigma0)) {
         o.cat = CAT_OUT;
         return o;
     }
     if (p.need_geom) {
-        const double st = p.step_deg;
         float he = 0.0f, hw = 0.0f, hn = 0.0f, hs = 0.0f;
-        const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
-        bool ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon + st), &he);
-        ok = dem_bilinear(p, rc, dem_col_axis(p, q.lon - st), &hw) && ok;
-        ok = dem_bilinear(p, dem_row_axis(p, q.lat + st), cc, &hn) && ok;
-        ok = dem_bilinear(p, dem_row_axis(p, q.lat - st), cc, &hs) && ok;
-        if (ok) ok = isfinite(he) && isfinite(hw) && isfinite(hn) && isfinite(hs);
-        if (!ok) {
+        if (!nb_heights(p, q.lat, q.lon, q.nb, &he, &hw, &hn, &hs)) {
             o.mask = 9;
             if (p.flatten) {
                 o.cat = CAT_FLAT;
@@ -146,7 +158,7 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
         } else {
             double w = 0.0, cl = 0.0;
             int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, he, hw, hn, hs, sat, q.target,
-                                        flat_area, &w, &cl);
+                                        inv_flat_area, &w, &cl);
             if (g == TG_VALID) {
                 o.cos_lia = (float)cl;
                 if (p.flatten) sigma0 /= (float)w;
@@ -167,7 +179,7 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
         }
     }
     if (p.nesz) {
-        float nz = bilinear(p.nesz, p.lines, p.samples, line, sample);
+        float nz = bilin_interp(nb_idx, nzt);
         if (!isnan(nz) && sigma0 <= nz * p.nesz_factor) {
             o.cat = CAT_NOISE;
             o.mask = 8;
@@ -393,7 +405,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     const int64_t c_begin = seg * p.seg_len;
     const int64_t c_end = min(c_begin + p.seg_len, p.cols);
     const long long row_seed = p.row_seed[rr];
-    const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
+    const double inv_flat_area = p.need_geom ? 1.0 / p.row_flat_area[rr] : 0.0;
     const int64_t obase = rr * p.cols;
 
     // phase A: the chain's value at the end of the previous segment, from a
@@ -407,7 +419,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     if (seg > 0) {
         bool have = false;
         for (int64_t cc = c_begin > p.lead ? c_begin - p.lead : 0; cc < c_begin; ++cc) {
-            PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
+            PixelPos q = prep_pixel<UTM, false>(p, p.col0 + cc, row_y, alon, alat);
             if (!q.dem_ok) continue;
             Solve s = solve(p, orb, q, S, have ? p.max_iter : 40, cache);
             if (s.oc == ZD_CONVERGED) {
@@ -445,7 +457,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
                 last_t = s.t;
                 last_has = true;
             }
-            PixelOut v = finish_pixel<RS>(p, q, s.oc, s.t, s.sat, flat_area);
+            PixelOut v = finish_pixel<RS>(p, q, s.oc, s.t, s.sat, inv_flat_area);
             if (row_ok) {
                 if (v.cat == CAT_ORBIT) {
                     if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
@@ -503,7 +515,7 @@ __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb
         const int npts = p.anchor_stride;
         const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
         const double* alat = UTM ? alon + npts : nullptr;
-        const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
+        const double inv_flat_area = p.need_geom ? 1.0 / p.row_flat_area[rr] : 0.0;
         const int64_t obase = rr * p.cols;
         SegRecord* recs = p.records + rr * p.nseg;
         long long E = p.row_seed[rr];
@@ -528,15 +540,15 @@ __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb
                     Solve n = solve(p, orb, q, sn, p.max_iter, cn);
                     if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
                         if (o.it != n.it) {  // same result, different iteration count
-                            PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, flat_area);
+                            PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
                             count(C, v.cat, o.it, -1, p.counters);
                             count(C, v.cat, n.it, 1, p.counters);
                         }
                         synced = true;
                         break;
                     }
-                    PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, flat_area);
-                    PixelOut vn = finish_pixel<RS>(p, q, n.oc, n.t, n.sat, flat_area);
+                    PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
+                    PixelOut vn = finish_pixel<RS>(p, q, n.oc, n.t, n.sat, inv_flat_area);
                     if (vo.cat == CAT_ORBIT) {
                         if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
                         else C.err_short -= 1;

@@ -48,14 +48,16 @@ struct TcParams {
     int64_t lines, samples;
     int64_t az_start_us;    // relative to the first state vector
     double near_range_m, rps, ati;
-    double line_per_us;     // 1 / (1e6 · ati): azimuth µs → line
-    double inv_rps;         // 1 / range pixel spacing
+    double line_per_us;     // 1 / (1e6 · ati): azimuth µs → line (FP32 variant)
+    double inv_ati;         // RN(1 / ati), for the exact division by ati
+    double inv_rps;         // RN(1 / range pixel spacing)
     // DEM raster
     const float* dem;
     int64_t dem_w, dem_h;
     double dem_olat, dem_olon, dem_pix, dem_inv_pix;
     double dem_max_row, dem_max_col;   // height-1, width-1
-    int64_t dem_lim_row, dem_lim_col;  // largest lower bilinear index
+    double dem_hi_row, dem_hi_col;     // height-0.5, width-0.5 (the outer margin)
+    int dem_lim_row, dem_lim_col;      // largest lower bilinear index
     const float* geoid;     // device 73x144 or NULL
     // output grid
     int crs, utm_zone, utm_south, n_anchor_pts;

@@ -9,52 +9,100 @@
 
 namespace sd {
 
-// One axis of a DEM bilinear lookup: the lower grid index, the fraction and
-// whether the coordinate is inside the raster (half a pixel of margin).  The
-// row axis depends on latitude only and the column axis on longitude only,
-// so the terrain geometry's east/west samples share the centre's row axis and
-// its north/south samples share the centre's column axis.
+// One axis of a DEM bilinear lookup: the lower grid index and the fraction.
+// A coordinate outside the raster (beyond its half-pixel margin) or NaN
+// gets a NaN fraction, which makes the interpolated height NaN: the
+// reference's None.  The row axis depends on latitude only and the column
+// axis on longitude only, so the terrain geometry's east/west samples share
+// the centre's row axis and its north/south samples share the centre's
+// column axis.  Indices are 32-bit (the host rejects larger rasters).
 struct DemAxis {
-    int64_t i0;
+    int i0;
     double f;
-    bool ok;
 };
 
-// dem.rs:351-372 / 385-409 (GLO-30 grid rule, bilinear_on_grid), per axis.
-__device__ __forceinline__ DemAxis dem_axis(double x, double max_x, int64_t lim) {
+// dem.rs:351-372 / 385-409 (GLO-30 grid rule, bilinear_on_grid), per axis:
+// x is the fractional pixel coordinate, max_x = n - 1, hi_x = n - 0.5 and
+// lim = n - 2, the largest lower index.
+__device__ __forceinline__ DemAxis dem_axis(double x, double max_x, double hi_x, int lim) {
     DemAxis a;
-    a.ok = !(x < -0.5 || x > max_x + 0.5);
+    const bool ok = x >= -0.5 && x <= hi_x;
     x = fmin(fmax(x, 0.0), max_x);
-    int64_t i0 = (int64_t)x;
+    const int i0 = (int)x;
     a.i0 = i0 < lim ? i0 : lim;
-    a.f = x - (double)a.i0;
+    a.f = ok ? x - (double)a.i0 : __builtin_nan("");
     return a;
 }
 
+// (origin - lat) / pixel and (lon - origin) / pixel, divided exactly as the
+// reference does (div_rn).
 __device__ __forceinline__ DemAxis dem_row_axis(const TcParams& p, double lat) {
-    return dem_axis((p.dem_olat - lat) * p.dem_inv_pix, p.dem_max_row, p.dem_lim_row);
+    return dem_axis(div_rn(p.dem_olat - lat, p.dem_pix, p.dem_inv_pix), p.dem_max_row, p.dem_hi_row, p.dem_lim_row);
 }
 
 __device__ __forceinline__ DemAxis dem_col_axis(const TcParams& p, double lon) {
     // dem.rs:694-723 longitude wrap
     const double lon_n = lon >= 180.0 ? lon - 360.0 : lon;
-    return dem_axis((lon_n - p.dem_olon) * p.dem_inv_pix, p.dem_max_col, p.dem_lim_col);
+    return dem_axis(div_rn(lon_n - p.dem_olon, p.dem_pix, p.dem_inv_pix), p.dem_max_col, p.dem_hi_col,
+                    p.dem_lim_col);
 }
 
-__device__ __forceinline__ bool dem_bilinear(const TcParams& p, DemAxis r, DemAxis c, float* h) {
-    if (!(r.ok && c.ok)) return false;
-    const float* d = p.dem + r.i0 * p.dem_w + c.i0;
-    float v00 = d[0], v01 = d[1], v10 = d[p.dem_w], v11 = d[p.dem_w + 1];
-    if (isnan(v00) || isnan(v01) || isnan(v10) || isnan(v11)) return false;
+// The four taps of one bilinear lookup.  dem_axis clamps the indices into
+// the raster whatever the coordinate (width and height are at least 2), so
+// the loads are always in bounds and need no branch.
+struct DemTaps {
+    float v00, v01, v10, v11;
+};
+
+__device__ __forceinline__ DemTaps dem_load(const TcParams& p, DemAxis r, DemAxis c) {
+    const float* d = p.dem + (size_t)(unsigned)r.i0 * (size_t)p.dem_w + (unsigned)c.i0;
+    return {d[0], d[1], d[p.dem_w], d[p.dem_w + 1]};
+}
+
+// The reference's sum in its order; NaN when a tap is NaN (the reference
+// returns None for any NaN tap) or the point is outside (NaN fraction).
+// Callers test the result with isfinite.
+__device__ __forceinline__ float dem_interp(DemAxis r, DemAxis c, const DemTaps& t) {
     const double dr = r.f, dc = c.f;
-    double v = (double)v00 * (1.0 - dr) * (1.0 - dc) + (double)v01 * (1.0 - dr) * dc +
-               (double)v10 * dr * (1.0 - dc) + (double)v11 * dr * dc;
-    *h = (float)v;
-    return true;
+    const double v = (double)t.v00 * (1.0 - dr) * (1.0 - dc) + (double)t.v01 * (1.0 - dr) * dc +
+                     (double)t.v10 * dr * (1.0 - dc) + (double)t.v11 * dr * dc;
+    return (float)v;
 }
 
-__device__ __forceinline__ bool dem_sample(const TcParams& p, double lat, double lon, float* h) {
-    return dem_bilinear(p, dem_row_axis(p, lat), dem_col_axis(p, lon), h);
+__device__ __forceinline__ float dem_sample(const TcParams& p, double lat, double lon) {
+    const DemAxis r = dem_row_axis(p, lat), c = dem_col_axis(p, lon);
+    return dem_interp(r, c, dem_load(p, r, c));
+}
+
+// DEM taps of the terrain geometry's four cardinal neighbours (east, west,
+// north, south at ±step_deg; terrain_correction.rs:599-606).  They depend on
+// the pixel's position only, so the kernels load them together with the
+// centre's taps, before the Newton solve, and their latency overlaps it.
+struct NbTaps {
+    DemTaps e, w, n, s;
+};
+
+__device__ __forceinline__ NbTaps nb_load(const TcParams& p, double lat, double lon, DemAxis rc, DemAxis cc) {
+    const double st = p.step_deg;
+    NbTaps t;
+    t.e = dem_load(p, rc, dem_col_axis(p, lon + st));
+    t.w = dem_load(p, rc, dem_col_axis(p, lon - st));
+    t.n = dem_load(p, dem_row_axis(p, lat + st), cc);
+    t.s = dem_load(p, dem_row_axis(p, lat - st), cc);
+    return t;
+}
+
+// Heights of the four neighbours from their taps; false when any is outside
+// the DEM or touches a void (DEM_NEIGHBOUR_MISSING).
+__device__ __forceinline__ bool nb_heights(const TcParams& p, double lat, double lon, const NbTaps& t, float* he,
+                                           float* hw, float* hn, float* hs) {
+    const double st = p.step_deg;
+    const DemAxis rc = dem_row_axis(p, lat), cc = dem_col_axis(p, lon);
+    *he = dem_interp(rc, dem_col_axis(p, lon + st), t.e);
+    *hw = dem_interp(rc, dem_col_axis(p, lon - st), t.w);
+    *hn = dem_interp(dem_row_axis(p, lat + st), cc, t.n);
+    *hs = dem_interp(dem_row_axis(p, lat - st), cc, t.s);
+    return isfinite(*he) && isfinite(*hw) && isfinite(*hn) && isfinite(*hs);
 }
 
 __device__ __forceinline__ double geoid_n(const TcParams& p, double lat, double lon) {
@@ -82,18 +130,50 @@ __device__ __forceinline__ double keys_w(double t) {  // terrain_correction.rs:8
 }
 
 
-__device__ __forceinline__ int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// terrain_correction.rs:770-797
-__device__ __forceinline__ float bilinear(const float* d, int64_t nl, int64_t ns, double line, double sample) {
-    int64_t ml = nl > 0 ? nl - 1 : 0, ms = ns > 0 ? ns - 1 : 0;
-    int64_t l0 = clampi((int64_t)floor(line), 0, ml), l1 = l0 + 1 < ml ? l0 + 1 : ml;
-    int64_t s0 = clampi((int64_t)floor(sample), 0, ms), s1 = s0 + 1 < ms ? s0 + 1 : ms;
-    double dl = line - (double)l0, ds = sample - (double)s0;
-    float v00 = d[l0 * ns + s0], v01 = d[l0 * ns + s1], v10 = d[l1 * ns + s0], v11 = d[l1 * ns + s1];
-    if (isnan(v00) || isnan(v01) || isnan(v10) || isnan(v11)) return __builtin_nanf("");
+// terrain_correction.rs:770-797, split into the tap indices, the loads and
+// the interpolation so that the NESZ lookup at the same (line, sample) can
+// issue its loads with the image's.  The indices are clamped into the image
+// (32-bit: the host rejects larger images); (line, sample) are inside
+// [-0.5, n - 0.5] here.
+struct Bilin {
+    int l0, l1, s0, s1;
+    double dl, ds;
+};
+
+__device__ __forceinline__ Bilin bilin_index(int64_t nl, int64_t ns, double line, double sample) {
+    const int ml = nl > 0 ? (int)nl - 1 : 0, ms = ns > 0 ? (int)ns - 1 : 0;
+    Bilin b;
+    b.l0 = clampi((int)floor(line), 0, ml);
+    b.l1 = b.l0 + 1 < ml ? b.l0 + 1 : ml;
+    b.s0 = clampi((int)floor(sample), 0, ms);
+    b.s1 = b.s0 + 1 < ms ? b.s0 + 1 : ms;
+    b.dl = line - (double)b.l0;
+    b.ds = sample - (double)b.s0;
+    return b;
+}
+
+__device__ __forceinline__ const float* image_row(const float* d, int64_t ns, int l) {
+    return d + (size_t)(unsigned)l * (size_t)ns;
+}
+
+__device__ __forceinline__ DemTaps bilin_load(const float* d, int64_t ns, const Bilin& b) {
+    const float* r0 = image_row(d, ns, b.l0);
+    const float* r1 = image_row(d, ns, b.l1);
+    return {r0[b.s0], r0[b.s1], r1[b.s0], r1[b.s1]};
+}
+
+// NaN when any tap is NaN, as the reference's early return.
+__device__ __forceinline__ float bilin_interp(const Bilin& b, const DemTaps& t) {
+    const double dl = b.dl, ds = b.ds;
     double w00 = (1.0 - dl) * (1.0 - ds), w01 = (1.0 - dl) * ds, w10 = dl * (1.0 - ds), w11 = dl * ds;
-    return (float)((double)v00 * w00 + (double)v01 * w01 + (double)v10 * w10 + (double)v11 * w11);
+    return (float)((double)t.v00 * w00 + (double)t.v01 * w01 + (double)t.v10 * w10 + (double)t.v11 * w11);
+}
+
+__device__ __forceinline__ float bilinear(const float* d, int64_t nl, int64_t ns, double line, double sample) {
+    const Bilin b = bilin_index(nl, ns, line, sample);
+    return bilin_interp(b, bilin_load(d, ns, b));
 }
 
 // terrain_correction.rs:842-872
@@ -102,19 +182,19 @@ __device__ __forceinline__ float bilinear(const float* d, int64_t nl, int64_t ns
 // yields the same NaN result and the same bits otherwise (the sum runs in
 // the reference's order).
 __device__ __forceinline__ float bicubic(const float* d, int64_t nl, int64_t ns, double line, double sample) {
-    int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
+    const int lf = (int)floor(line), sf = (int)floor(sample);
     double dl = line - (double)lf, ds = sample - (double)sf;
     double ws[4];
-    int64_t cs[4];
+    int cs[4];
 #pragma unroll
     for (int si = 0; si < 4; ++si) {
         ws[si] = keys_w((double)(si - 1) - ds);
-        cs[si] = clampi(sf + si - 1, 0, ns - 1);
+        cs[si] = clampi(sf + si - 1, 0, (int)ns - 1);
     }
     double acc = 0.0;
 #pragma unroll
     for (int li = -1; li <= 2; ++li) {
-        const float* row = d + clampi(lf + li, 0, nl - 1) * ns;
+        const float* row = image_row(d, ns, clampi(lf + li, 0, (int)nl - 1));
         double wl = keys_w((double)li - dl);
 #pragma unroll
         for (int si = 0; si < 4; ++si) acc += (double)row[cs[si]] * wl * ws[si];
@@ -155,17 +235,17 @@ __device__ __forceinline__ void lanczos_axis(double u, double w[6]) {
 }
 
 __device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns, double line, double sample) {
-    int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
+    const int lf = (int)floor(line), sf = (int)floor(sample);
     double ws[6], wl[6];
     lanczos_axis(sample - (double)sf, ws);
     lanczos_axis(line - (double)lf, wl);
-    int64_t cs[6];
+    int cs[6];
 #pragma unroll
-    for (int si = 0; si < 6; ++si) cs[si] = clampi(sf + si - 2, 0, ns - 1);
+    for (int si = 0; si < 6; ++si) cs[si] = clampi(sf + si - 2, 0, (int)ns - 1);
     double acc = 0.0;  // NaN taps: see bicubic
 #pragma unroll
     for (int li = 0; li < 6; ++li) {
-        const float* row = d + clampi(lf + li - 2, 0, nl - 1) * ns;
+        const float* row = image_row(d, ns, clampi(lf + li - 2, 0, (int)nl - 1));
 #pragma unroll
         for (int si = 0; si < 6; ++si) acc += (double)row[cs[si]] * wl[li] * ws[si];
     }
