@@ -224,8 +224,8 @@ __device__ __forceinline__ PixelOut finish32(const TcParams& p, const PixelPos& 
         return o;
     }
     if (p.need_geom) {
-        float he = 0.0f, hw = 0.0f, hn = 0.0f, hs = 0.0f;
-        if (!nb_heights(p, q.lat, q.lon, q.nb, &he, &hw, &hn, &hs)) {
+        const float he = q.nb.e, hw = q.nb.w, hn = q.nb.n, hs = q.nb.s;
+        if (!q.nb.ok) {
             o.mask = 9;
             if (p.flatten) {
                 o.cat = CAT_FLAT;

@@ -46,12 +46,13 @@ struct PixelPos {
     double n;               // prime-vertical radius at lat
     d3 target;
     bool dem_ok;
-    NbTaps nb;  // the terrain geometry's neighbour taps (when p.need_geom)
+    NbHeights nb;  // the terrain geometry's neighbour heights (when p.need_geom)
 };
 
 // Output-grid position of a column and its DEM/ECEF target.  The centre's
 // and (for the terrain geometry) the neighbours' DEM taps are issued
-// together: one memory round trip instead of two.
+// together: one memory round trip instead of two; only the four neighbour
+// heights stay live across the Newton solve.
 template <bool UTM, bool NB = true>
 __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, double row_y, const double* alon,
                                                const double* alat) {
@@ -74,7 +75,7 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
     }
     const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
     const DemTaps ct = dem_load(p, rc, cc);
-    if (NB && p.need_geom) q.nb = nb_load(p, q.lat, q.lon, rc, cc);
+    if (NB && p.need_geom) q.nb = nb_heights(p, q.lat, q.lon, rc, cc);
     const float hf = dem_interp(rc, cc, ct);
     q.dem_ok = isfinite(hf);
     q.target = {0.0, 0.0, 0.0};
@@ -148,8 +149,7 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
         return o;
     }
     if (p.need_geom) {
-        float he = 0.0f, hw = 0.0f, hn = 0.0f, hs = 0.0f;
-        if (!nb_heights(p, q.lat, q.lon, q.nb, &he, &hw, &hn, &hs)) {
+        if (!q.nb.ok) {
             o.mask = 9;
             if (p.flatten) {
                 o.cat = CAT_FLAT;
@@ -157,8 +157,8 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
             }
         } else {
             double w = 0.0, cl = 0.0;
-            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, he, hw, hn, hs, sat, q.target,
-                                        inv_flat_area, &w, &cl);
+            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w, q.nb.n,
+                                        q.nb.s, sat, q.target, inv_flat_area, &w, &cl);
             if (g == TG_VALID) {
                 o.cos_lia = (float)cl;
                 if (p.flatten) sigma0 /= (float)w;

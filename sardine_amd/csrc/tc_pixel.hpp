@@ -74,35 +74,29 @@ __device__ __forceinline__ float dem_sample(const TcParams& p, double lat, doubl
     return dem_interp(r, c, dem_load(p, r, c));
 }
 
-// DEM taps of the terrain geometry's four cardinal neighbours (east, west,
+// Heights of the terrain geometry's four cardinal neighbours (east, west,
 // north, south at ±step_deg; terrain_correction.rs:599-606).  They depend on
-// the pixel's position only, so the kernels load them together with the
-// centre's taps, before the Newton solve, and their latency overlaps it.
-struct NbTaps {
-    DemTaps e, w, n, s;
+// the pixel's position only, so the kernels load their taps together with
+// the centre's, before the Newton solve; ok is false when any is outside
+// the DEM or touches a void (DEM_NEIGHBOUR_MISSING).
+struct NbHeights {
+    float e, w, n, s;
+    bool ok;
 };
 
-__device__ __forceinline__ NbTaps nb_load(const TcParams& p, double lat, double lon, DemAxis rc, DemAxis cc) {
+__device__ __forceinline__ NbHeights nb_heights(const TcParams& p, double lat, double lon, DemAxis rc, DemAxis cc) {
     const double st = p.step_deg;
-    NbTaps t;
-    t.e = dem_load(p, rc, dem_col_axis(p, lon + st));
-    t.w = dem_load(p, rc, dem_col_axis(p, lon - st));
-    t.n = dem_load(p, dem_row_axis(p, lat + st), cc);
-    t.s = dem_load(p, dem_row_axis(p, lat - st), cc);
-    return t;
-}
-
-// Heights of the four neighbours from their taps; false when any is outside
-// the DEM or touches a void (DEM_NEIGHBOUR_MISSING).
-__device__ __forceinline__ bool nb_heights(const TcParams& p, double lat, double lon, const NbTaps& t, float* he,
-                                           float* hw, float* hn, float* hs) {
-    const double st = p.step_deg;
-    const DemAxis rc = dem_row_axis(p, lat), cc = dem_col_axis(p, lon);
-    *he = dem_interp(rc, dem_col_axis(p, lon + st), t.e);
-    *hw = dem_interp(rc, dem_col_axis(p, lon - st), t.w);
-    *hn = dem_interp(dem_row_axis(p, lat + st), cc, t.n);
-    *hs = dem_interp(dem_row_axis(p, lat - st), cc, t.s);
-    return isfinite(*he) && isfinite(*hw) && isfinite(*hn) && isfinite(*hs);
+    const DemAxis ce = dem_col_axis(p, lon + st), cw = dem_col_axis(p, lon - st);
+    const DemAxis rn = dem_row_axis(p, lat + st), rs = dem_row_axis(p, lat - st);
+    const DemTaps te = dem_load(p, rc, ce), tw = dem_load(p, rc, cw);
+    const DemTaps tn = dem_load(p, rn, cc), ts = dem_load(p, rs, cc);
+    NbHeights h;
+    h.e = dem_interp(rc, ce, te);
+    h.w = dem_interp(rc, cw, tw);
+    h.n = dem_interp(rn, cc, tn);
+    h.s = dem_interp(rs, cc, ts);
+    h.ok = isfinite(h.e) && isfinite(h.w) && isfinite(h.n) && isfinite(h.s);
+    return h;
 }
 
 __device__ __forceinline__ double geoid_n(const TcParams& p, double lat, double lon) {
