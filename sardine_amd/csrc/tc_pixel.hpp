@@ -172,9 +172,8 @@ __device__ __forceinline__ float bilinear(const float* d, int64_t nl, int64_t ns
 
 // terrain_correction.rs:842-872
 // A NaN tap makes the reference return NaN at once; here every tap is
-// gathered first and a NaN tap turns the f64 sum into NaN instead, which
-// yields the same NaN result and the same bits otherwise (the sum runs in
-// the reference's order).
+// gathered first and a NaN tap turns the f64 sum into NaN instead.  The
+// taps are summed row by row (see lanczos3).
 __device__ __forceinline__ float bicubic(const float* d, int64_t nl, int64_t ns, double line, double sample) {
     const int lf = (int)floor(line), sf = (int)floor(sample);
     double dl = line - (double)lf, ds = sample - (double)sf;
@@ -185,49 +184,103 @@ __device__ __forceinline__ float bicubic(const float* d, int64_t nl, int64_t ns,
         ws[si] = keys_w((double)(si - 1) - ds);
         cs[si] = clampi(sf + si - 1, 0, (int)ns - 1);
     }
-    double acc = 0.0;
+    double acc = 0.0;  // row by row, as lanczos3
 #pragma unroll
     for (int li = -1; li <= 2; ++li) {
         const float* row = image_row(d, ns, clampi(lf + li, 0, (int)nl - 1));
-        double wl = keys_w((double)li - dl);
+        const double wl = keys_w((double)li - dl);
+        double rs = 0.0;
 #pragma unroll
-        for (int si = 0; si < 4; ++si) acc += (double)row[cs[si]] * wl * ws[si];
+        for (int si = 0; si < 4; ++si) rs += (double)row[cs[si]] * ws[si];
+        acc += rs * wl;
     }
     return (float)acc;
 }
 
-// terrain_correction.rs:958-983
-// The six Lanczos-3 weights L(x) = sin(πx)·sin(πx/3) / (πx·πx/3) (1 for
-// |x| < 1e-10, 0 for |x| >= 3; terrain_correction.rs:938-950) at
-// x = u + 2 - k, k = 0..5, of one axis (u = the fractional coordinate in
-// [0, 1)).  sin(π(u+m)) = (-1)^m sin(πu)
-// and sin(π(u+m)/3) follows from sin/cos(πu/3) by angle addition, so one
-// sin and one sincos replace the reference's twelve sin calls; the weights
-// agree with the direct form to ~1e-15.
-__device__ __forceinline__ void lanczos_axis(double u, double w[6]) {
-    const double s1 = sin(kPi * u);
-    double s3, c3;
-    sincos(kPi * u / 3.0, &s3, &c3);
-    constexpr double h = 0.86602540378443864676;  // sin(π/3)
-    // cos and sin of π(2 - k)/3 for k = 0..5
-    const double C[6] = {-0.5, 0.5, 1.0, 0.5, -0.5, -1.0};
-    const double S[6] = {h, h, 0.0, -h, -h, 0.0};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const double x = u + (double)(2 - k);
-        const double ax = fabs(x);
-        double v = 0.0;
-        if (ax < 1e-10) {
-            v = 1.0;
-        } else if (ax < 3.0) {
-            const double px = kPi * x, pxa = kPi * x / 3.0;
-            const double sp = (k & 1) ? -s1 : s1;
-            v = sp * (s3 * C[k] + c3 * S[k]) / (px * pxa);
-        }
-        w[k] = v;
-    }
+// sin and cos of |φ| <= π/6 by their Taylor series in Horner form on φ²
+// (first omitted terms below 2e-19 there); no range reduction is needed.
+__device__ __forceinline__ void sincos_small(double f, double* s, double* c) {
+    const double z = f * f;
+    double ps = -7.6471637318198164759e-13;  // -1/15!
+    ps = fma(ps, z, 1.6059043836821614599e-10);
+    ps = fma(ps, z, -2.5052108385441718775e-08);
+    ps = fma(ps, z, 2.7557319223985890653e-06);
+    ps = fma(ps, z, -1.9841269841269841270e-04);
+    ps = fma(ps, z, 8.3333333333333333333e-03);
+    ps = fma(ps, z, -1.6666666666666666667e-01);
+    double pc = 4.7794773323873852974e-14;  // 1/16!
+    pc = fma(pc, z, -1.1470745597729724714e-11);
+    pc = fma(pc, z, 2.0876756987868098979e-09);
+    pc = fma(pc, z, -2.7557319223985890653e-07);
+    pc = fma(pc, z, 2.4801587301587301587e-05);
+    pc = fma(pc, z, -1.3888888888888888889e-03);
+    pc = fma(pc, z, 4.1666666666666666667e-02);
+    pc = fma(pc, z, -0.5);
+    *s = fma(f * z, ps, f);
+    *c = fma(z, pc, 1.0);
 }
 
+// terrain_correction.rs:938-950: the six Lanczos-3 weights
+// L(x) = sin(πx)·sin(πx/3) / (πx·πx/3) (1 for |x| < 1e-10, 0 for |x| >= 3)
+// at the tap offsets x = u + 2 - k, k = 0..5, of one axis (u = the
+// fractional coordinate in [0, 1)).  Every offset is r + m for the offset r
+// nearest zero (r = u or u - 1, |r| <= 1/2, exact) and an integer m, so
+// sin(πx) = (-1)^m sin(πr), sin(πx/3) = sin(πr/3) cos(mπ/3) + cos(πr/3)
+// sin(mπ/3), and sin(πr) = 3s - 4s³ with s = sin(πr/3) (no cancellation for
+// |s| <= 1/2).  Expanding about r keeps every factor relatively accurate
+// where its tap's weight is near 1 (x → 0); the six 1/x² come from one
+// division (prefix and suffix products).  The weights agree with the
+// reference's twelve sin calls to ~1e-15 absolute.
+__device__ __forceinline__ void lanczos_axis(double u, double w[6]) {
+    const bool up = u >= 0.5;
+    const double r = up ? u - 1.0 : u;
+    double s3, c3;
+    sincos_small(kPi * r * (1.0 / 3.0), &s3, &c3);
+    const double s1 = s3 * (3.0 - 4.0 * (s3 * s3));
+    constexpr double h = 0.86602540378443864676;  // sin(π/3)
+    constexpr double K = 3.0 / (kPi * kPi);
+    // cos and sin of mπ/3 for m = -3..3
+    const double CM[7] = {-1.0, -0.5, 0.5, 1.0, 0.5, -0.5, -1.0};
+    const double SM[7] = {0.0, -h, -h, 0.0, h, h, 0.0};
+    double a[7], d[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const int m = i - 3;
+        const double x = r + (double)m;
+        const double ax = fabs(x);
+        const double sp = (m & 1) ? -s1 : s1;
+        if (ax < 1e-10) {
+            a[i] = 1.0;
+            d[i] = 1.0;
+        } else if (ax < 3.0) {
+            a[i] = sp * (s3 * CM[i] + c3 * SM[i]) * K;
+            d[i] = x * x;
+        } else {
+            a[i] = 0.0;
+            d[i] = 1.0;
+        }
+    }
+    double pre[8], suf[8];
+    pre[0] = 1.0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) pre[i + 1] = pre[i] * d[i];
+    suf[7] = 1.0;
+#pragma unroll
+    for (int i = 6; i >= 0; --i) suf[i] = suf[i + 1] * d[i];
+    const double inv = 1.0 / pre[7];
+    double W[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) W[i] = a[i] * (inv * (pre[i] * suf[i + 1]));
+    // tap k has m = 2 - k (r = u) or m = 3 - k (r = u - 1)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = up ? W[6 - k] : W[5 - k];
+}
+
+// terrain_correction.rs:958-983.  The 36 taps are summed row by row (the
+// row's taps times their column weights, then times the row weight); the
+// reference adds v·wl·ws tap by tap, so the f64 sums differ in their last
+// bits only (~1e-15 relative, below the f32 result's resolution in all but
+// ~1e-8 of the samples).  A NaN tap makes the sum NaN: the reference's NaN.
 __device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns, double line, double sample) {
     const int lf = (int)floor(line), sf = (int)floor(sample);
     double ws[6], wl[6];
@@ -236,12 +289,14 @@ __device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns
     int cs[6];
 #pragma unroll
     for (int si = 0; si < 6; ++si) cs[si] = clampi(sf + si - 2, 0, (int)ns - 1);
-    double acc = 0.0;  // NaN taps: see bicubic
+    double acc = 0.0;
 #pragma unroll
     for (int li = 0; li < 6; ++li) {
         const float* row = image_row(d, ns, clampi(lf + li - 2, 0, (int)nl - 1));
+        double rs = 0.0;
 #pragma unroll
-        for (int si = 0; si < 6; ++si) acc += (double)row[cs[si]] * wl[li] * ws[si];
+        for (int si = 0; si < 6; ++si) rs += (double)row[cs[si]] * ws[si];
+        acc += rs * wl[li];
     }
     return (float)acc;
 }
