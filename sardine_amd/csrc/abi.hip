@@ -1028,6 +1028,7 @@ sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, 
     std::vector<std::vector<sd::CalLine>> tables((size_t)n);
     sd::CalParams P;
     std::memset(&P, 0, sizeof P);
+    if (info->samples >= INT32_MAX) return fail(SARDINE_ERR_CAPACITY, "merged raster wider than 2^31");
     P.n_sw = n;
     P.out_lines = info->lines;
     P.out_cols = info->samples;
@@ -1038,6 +1039,9 @@ sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, 
         if ((st = sardine_deburst_layout_compute(&sw[i].swath, sw[i].bursts, sw[i].n_bursts, sw[i].tiff_height, &L)))
             return st;
         if (!sw[i].raw) return fail(SARDINE_ERR_INVALID_ARGUMENT, "swath %d raster is null", i);
+        if (reinterpret_cast<uintptr_t>(sw[i].raw) % 4 != 0)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "swath %d CInt16 raster must be 4-byte aligned", i);
+        if (sw[i].raw_cols >= INT32_MAX) return fail(SARDINE_ERR_CAPACITY, "swath %d raster wider than 2^31", i);
         if (L.valid_sample_offset + L.samples > sw[i].raw_cols)
             return fail(SARDINE_ERR_INVALID_ARGUMENT, "swath %d valid window exceeds the raster width", i);
         const int64_t dense_cols = L.valid_sample_offset + L.samples;
@@ -1080,6 +1084,9 @@ sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, 
     if (nr_gap) *nr_gap = b;
     if (lines <= 0 || samples <= 0) return SARDINE_OK;
     if (!deburst || !out || !nesz) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null raster");
+    if (reinterpret_cast<uintptr_t>(deburst) % 4 != 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "CInt16 raster must be 4-byte aligned");
+    if (col_off + samples >= INT32_MAX) return fail(SARDINE_ERR_CAPACITY, "raster wider than 2^31");
     if ((st = require_device())) return st;
     hipStream_t s = (hipStream_t)stream_v;
     auto lt = line_table(cal, nr, az, origin, lines, nullptr, nullptr);

@@ -56,62 +56,112 @@ __global__ void lut_dense_kernel(const uint32_t* __restrict__ pixels, const floa
     dense[(int64_t)v * dense_cols + c] = out;
 }
 
-__device__ __forceinline__ void cal_pixel(const CalSwathDev& s, const CalLine& L, int64_t in_col, float* o_s,
-                                          float* o_n) {
-    const int64_t tc = in_col + s.first_valid;  // TIFF column
-    const int16_t* px = s.raw + (L.tiff_row * s.raw_cols + tc) * 2;
-    const int32_t i = px[0], q = px[1];
-    const float p = (float)(i * i + q * q);
-    const float* cd = s.cal_dense;
-    const float* nd = s.nr_dense;
-    float k = L.cal_lo == L.cal_hi ? cd[L.cal_lo * s.dense_cols + tc]
-                                   : lerp_clamped(cd[L.cal_lo * s.dense_cols + tc], cd[L.cal_hi * s.dense_cols + tc], L.cal_t);
-    float nr = L.nr_lo == L.nr_hi ? nd[L.nr_lo * s.dense_cols + tc]
-                                  : lerp_clamped(nd[L.nr_lo * s.dense_cols + tc], nd[L.nr_hi * s.dense_cols + tc], L.nr_t);
-    float k2 = k * k;
-    float valid = k2 > 0.0f ? 1.0f : 0.0f;
-    float inv = k2 > 0.0f ? 1.0f / k2 : 0.0f;
-    float nv = nr * L.az;
+// One merged pixel (apply_calibration.rs:520-572 per-pixel arithmetic in the
+// reference's f32 order) from its CInt16 word w (I in the low half) and the
+// range-LUT taps of its column in the line's two bracketing vectors.
+__device__ __forceinline__ void cal_pixel(uint32_t w, float klo, float khi, float nlo, float nhi, const CalLine& L,
+                                          float* o_s, float* o_n) {
+    const int32_t i = (int16_t)(w & 0xffffu), q = (int16_t)(w >> 16);
+    // i32 power; 2 * 32768^2 wraps as the reference's release build does
+    const float p = (float)(int32_t)((uint32_t)(i * i) + (uint32_t)(q * q));
+    const float k = L.cal_lo == L.cal_hi ? klo : lerp_clamped(klo, khi, L.cal_t);
+    const float nr = L.nr_lo == L.nr_hi ? nlo : lerp_clamped(nlo, nhi, L.nr_t);
+    const float k2 = k * k;
+    const float valid = k2 > 0.0f ? 1.0f : 0.0f;
+    const float inv = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+    const float nv = nr * L.az;
     *o_s = fmaxf(p - nv, 0.0f) * inv * valid;
     *o_n = nv * inv * valid;
 }
 
-// One workgroup per merged line; consecutive lanes take consecutive merged
-// columns, so every raw, LUT and output access of a wave is one contiguous
-// run.  The line's per-swath constants are loaded once into LDS.
-constexpr int kCalUnroll = 4;
+// A workgroup takes kCalLines merged lines x kCalCols merged columns (4
+// columns per thread, 256 apart, so every access of a wave is one contiguous
+// run).  The sub-swaths own consecutive column ranges (swath s up to its
+// seam, the last one to the end of the line), so the block is walked swath
+// by swath and every swath constant is uniform.  Lines of one burst share
+// their LUT brackets: the range-LUT taps of a column are loaded once and
+// reloaded only when a line's bracket differs, and the raw words of all the
+// block's lines are loaded before any is used.  Column arithmetic is 32-bit
+// (the host rejects wider rasters).
+constexpr int kCalLines = 8;
+constexpr int kCalPer = 4;
+constexpr int kCalCols = 256 * kCalPer;
 
 __global__ __launch_bounds__(256) void cal_merge_kernel(CalParams P) {
-    __shared__ CalSwathDev sw[kMaxSwaths];
-    __shared__ CalLine ln[kMaxSwaths];
-    const int64_t line = blockIdx.x;
-    if (threadIdx.x < (unsigned)P.n_sw) {
-        sw[threadIdx.x] = P.sw[threadIdx.x];
-        ln[threadIdx.x] = P.sw[threadIdx.x].lines[line];
+    __shared__ CalLine ln[kMaxSwaths][kCalLines];
+    const int c_begin = (int)blockIdx.x * kCalCols;
+    const int c_end = min(c_begin + kCalCols, (int)P.out_cols);
+    const int64_t l_begin = (int64_t)blockIdx.y * kCalLines;
+    const int nl = (int)min((int64_t)kCalLines, P.out_lines - l_begin);
+    for (int i = threadIdx.x; i < P.n_sw * kCalLines; i += 256) {
+        const int s = i / kCalLines, l = i % kCalLines;
+        if (l < nl) ln[s][l] = P.sw[s].lines[l_begin + l];
     }
     __syncthreads();
     const float qnan = __builtin_nanf("");
-    const int n_sw = P.n_sw;
-    float* const out_s = P.sigma0 + line * P.out_cols;
-    float* const out_n = P.nesz + line * P.out_cols;
-    for (int64_t c0 = threadIdx.x; c0 < P.out_cols; c0 += 256 * kCalUnroll) {
-        float vs[kCalUnroll], vn[kCalUnroll];
+    int own_start = 0;
+    for (int s = 0; s < P.n_sw; ++s) {
+        const CalSwathDev& S = P.sw[s];
+        const int own_end = s + 1 < P.n_sw ? max((int)S.own_end, own_start) : (int)P.out_cols;
+        const int lo = max(own_start, c_begin), hi = min(own_end, c_end);
+        own_start = own_end;
+        if (lo >= hi) continue;
+        const int vs = (int)S.valid_start, ve = (int)S.valid_end;
+        const int shift = (int)(S.first_valid - S.out_off);  // TIFF column = merged column + shift
+        const uint32_t* raw = reinterpret_cast<const uint32_t*>(S.raw);
+        int oc[kCalPer];
+        bool own[kCalPer], val[kCalPer];
 #pragma unroll
-        for (int j = 0; j < kCalUnroll; ++j) {
-            const int64_t oc = c0 + 256 * j;
-            vs[j] = qnan;
-            vn[j] = qnan;
-            if (oc >= P.out_cols) continue;
-            int s = 0;
-            while (s + 1 < n_sw && oc >= sw[s].own_end) ++s;
-            if (oc >= sw[s].valid_start && oc < sw[s].valid_end) cal_pixel(sw[s], ln[s], oc - sw[s].out_off, &vs[j], &vn[j]);
+        for (int j = 0; j < kCalPer; ++j) {
+            oc[j] = lo + (int)threadIdx.x + 256 * j;
+            own[j] = oc[j] < hi;
+            val[j] = own[j] && oc[j] >= vs && oc[j] < ve;
         }
+        uint32_t w[kCalLines][kCalPer];
 #pragma unroll
-        for (int j = 0; j < kCalUnroll; ++j) {
-            const int64_t oc = c0 + 256 * j;
-            if (oc < P.out_cols) {
-                out_s[oc] = vs[j];
-                out_n[oc] = vn[j];
+        for (int l = 0; l < kCalLines; ++l) {
+            const uint32_t* row = raw + (l < nl ? ln[s][l].tiff_row : 0) * S.raw_cols + shift;
+#pragma unroll
+            for (int j = 0; j < kCalPer; ++j) w[l][j] = (l < nl && val[j]) ? row[oc[j]] : 0u;
+        }
+        float klo[kCalPer], khi[kCalPer], nlo[kCalPer], nhi[kCalPer];
+        int bk_lo = -1, bk_hi = -1, bn_lo = -1, bn_hi = -1;  // brackets the taps hold
+#pragma unroll
+        for (int l = 0; l < kCalLines; ++l) {
+            if (l >= nl) break;
+            const CalLine L = ln[s][l];
+            if (L.cal_lo != bk_lo || L.cal_hi != bk_hi) {
+                const float* a = S.cal_dense + (int64_t)L.cal_lo * S.dense_cols + shift;
+                const float* b = S.cal_dense + (int64_t)L.cal_hi * S.dense_cols + shift;
+#pragma unroll
+                for (int j = 0; j < kCalPer; ++j) {
+                    klo[j] = val[j] ? a[oc[j]] : 0.0f;
+                    khi[j] = val[j] ? b[oc[j]] : 0.0f;
+                }
+                bk_lo = L.cal_lo;
+                bk_hi = L.cal_hi;
+            }
+            if (L.nr_lo != bn_lo || L.nr_hi != bn_hi) {
+                const float* a = S.nr_dense + (int64_t)L.nr_lo * S.dense_cols + shift;
+                const float* b = S.nr_dense + (int64_t)L.nr_hi * S.dense_cols + shift;
+#pragma unroll
+                for (int j = 0; j < kCalPer; ++j) {
+                    nlo[j] = val[j] ? a[oc[j]] : 0.0f;
+                    nhi[j] = val[j] ? b[oc[j]] : 0.0f;
+                }
+                bn_lo = L.nr_lo;
+                bn_hi = L.nr_hi;
+            }
+            float* const out_s = P.sigma0 + (l_begin + l) * P.out_cols;
+            float* const out_n = P.nesz + (l_begin + l) * P.out_cols;
+#pragma unroll
+            for (int j = 0; j < kCalPer; ++j) {
+                float vs_ = qnan, vn_ = qnan;
+                if (val[j]) cal_pixel(w[l][j], klo[j], khi[j], nlo[j], nhi[j], L, &vs_, &vn_);
+                if (own[j]) {
+                    out_s[oc[j]] = vs_;
+                    out_n[oc[j]] = vn_;
+                }
             }
         }
     }
@@ -130,6 +180,8 @@ extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* v
 
 extern "C" hipError_t sd_launch_cal_merge(const sd::CalParams* p, hipStream_t stream) {
     if (p->out_lines <= 0 || p->out_cols <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sd::cal_merge_kernel, dim3((unsigned)p->out_lines), dim3(256), 0, stream, *p);
+    dim3 grid((unsigned)((p->out_cols + sd::kCalCols - 1) / sd::kCalCols),
+              (unsigned)((p->out_lines + sd::kCalLines - 1) / sd::kCalLines));
+    hipLaunchKernelGGL(sd::cal_merge_kernel, grid, dim3(256), 0, stream, *p);
     return hipGetLastError();
 }
