@@ -327,6 +327,50 @@ sardine_status validate_filter(const sardine_speckle_filter* f, int* win_out) {
     return SARDINE_OK;
 }
 
+// Coefficients of each central interval's Lagrange-8 interpolants as
+// polynomials in τ = ts - mid (sd::OrbitPoly), in binary128: each basis
+// polynomial Π_{j≠k}(τ + mid - x_j) / Π_{j≠k}(x_k - x_j) is expanded in
+// powers of τ and weighted by the node's position / velocity; the
+// acceleration polynomial is the derivative of the velocity one.  The node
+// times x_j are the double seconds the reference interpolates with.
+void orbit_fill_poly(sd::OrbitDev& o) {
+    using q128 = __float128;
+    for (int c = 0; c + 1 < o.n; ++c) {
+        sd::OrbitPoly& P = o.poly[c];
+        const int s = sd::orbit_window_start(c, o.n);
+        if (s < 0 || s + sd::kLagrange > o.n) continue;  // fewer than 8 vectors: never evaluated
+        P.mid = (double)(((q128)o.tsec[c] + (q128)o.tsec[c + 1]) / 2);
+        q128 cp[3][8] = {}, cv[3][8] = {};
+        for (int k = 0; k < sd::kLagrange; ++k) {
+            q128 poly[8] = {1};
+            int deg = 0;
+            q128 den = 1;
+            for (int j = 0; j < sd::kLagrange; ++j) {
+                if (j == k) continue;
+                const q128 a = (q128)P.mid - (q128)o.tsec[s + j];
+                for (int i = deg + 1; i >= 1; --i) poly[i] = poly[i] * a + poly[i - 1];
+                poly[0] = poly[0] * a;
+                ++deg;
+                den *= (q128)o.tsec[s + k] - (q128)o.tsec[s + j];
+            }
+            for (int d = 0; d < 3; ++d) {
+                const q128 yp = (q128)o.pos[s + k][d] / den, yv = (q128)o.vel[s + k][d] / den;
+                for (int i = 0; i < 8; ++i) {
+                    cp[d][i] += poly[i] * yp;
+                    cv[d][i] += poly[i] * yv;
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            for (int i = 0; i < 8; ++i) {
+                P.cp[d][i] = (double)cp[d][i];
+                P.cv[d][i] = (double)cv[d][i];
+            }
+            for (int i = 1; i < 8; ++i) P.ca[d][i - 1] = (double)(cv[d][i] * i);
+        }
+    }
+}
+
 // Orbit in device form, times relative to the first state vector.
 sd::OrbitDev make_orbit_dev(const sardine_orbit& o) {
     sd::OrbitDev orb;
@@ -342,7 +386,7 @@ sd::OrbitDev make_orbit_dev(const sardine_orbit& o) {
             orb.vel[i][c] = s.velocity_m_s[c];
         }
     }
-    sd::orbit_fill_bary(orb);
+    orbit_fill_poly(orb);
     if (orb.n >= 2 && orb.t_us[orb.n - 1] > 0) orb.guess_scale = (double)(orb.n - 1) / (double)orb.t_us[orb.n - 1];
     return orb;
 }

@@ -83,8 +83,14 @@ __device__ __forceinline__ void cal_pixel(uint32_t w, float klo, float khi, floa
 // reloaded only when a line's bracket differs, and the raw words of all the
 // block's lines are loaded before any is used.  Column arithmetic is 32-bit
 // (the host rejects wider rasters).
-constexpr int kCalLines = 8;
-constexpr int kCalPer = 4;
+#ifndef SD_CAL_LINES
+#define SD_CAL_LINES 8
+#endif
+#ifndef SD_CAL_PER
+#define SD_CAL_PER 4
+#endif
+constexpr int kCalLines = SD_CAL_LINES;  // block shape: build options so that it can be tuned per device
+constexpr int kCalPer = SD_CAL_PER;
 constexpr int kCalCols = 256 * kCalPer;
 
 __global__ __launch_bounds__(256) void cal_merge_kernel(CalParams P) {

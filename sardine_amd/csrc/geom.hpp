@@ -11,6 +11,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #define SD_HD __host__ __device__ __forceinline__
@@ -87,8 +88,29 @@ SD_HD d3 ecef_from_sc(double sl, double cl, double so, double co, double h) {
     return ecef_from_n(prime_vertical(sl), sl, cl, so, co, h);
 }
 
+// The Lagrange-8 interpolants of one central interval [t_us[c], t_us[c+1])
+// (the interval whose times share one window under the reference's window
+// rule) as polynomials in τ = ts - mid, mid = the interval's midpoint in
+// seconds: position and velocity of degree 7, acceleration (the derivative
+// of the velocity interpolant, which is the reference's Σ v_k L'_k) of
+// degree 6, ascending powers.  A degree-7 interpolant is exactly a degree-7
+// polynomial, so this is the same function as the reference's product form;
+// the host computes the coefficients in binary128 from the double node times
+// and state vectors, and |τ| <= half a node spacing keeps Horner's rounding
+// at ~1 ulp of the result (~1e-9 m, below the reference's own product-form
+// rounding of ~3e-9 m against the exact interpolant).
+struct OrbitPoly {
+    double mid;
+    double pad0;
+    double cp[3][8];
+    double cv[3][8];
+    double ca[3][7];
+    double pad1;
+};
+
 // Orbit in device-friendly form.  Times are µs relative to the first state
-// vector; tsec is the same in seconds ((t_us) / 1e6, orbit.rs:545-547).
+// vector; tsec is the same in seconds ((t_us) / 1e6, orbit.rs:545-547).  The
+// kernels copy the part up to poly[n-2] into LDS (orbit_lds_bytes).
 struct alignas(16) OrbitDev {
     int32_t n;
     int32_t pad;
@@ -97,39 +119,31 @@ struct alignas(16) OrbitDev {
     double tsec[kMaxSv];
     double pos[kMaxSv][3];
     double vel[kMaxSv][3];
-    double bary[kMaxSv][kLagrange];  // per window start
+    OrbitPoly poly[kMaxSv - 1];
 };
+
+SD_HD size_t orbit_lds_bytes(int n) {
+    return offsetof(OrbitDev, poly) + (size_t)(n > 1 ? n - 1 : 0) * sizeof(OrbitPoly);
+}
 
 enum : int { ORB_OK = 0, ORB_OUTSIDE = 1, ORB_INSUFFICIENT = 2 };
 
-// Σ_k of the Lagrange basis (and its derivative) times the window's nodes.
-SD_HD void orbit_accumulate(const OrbitDev& o, int s0, const double* pre, const double* suf, const double* rd,
-                            double tot, d3* P, d3* V, d3* A) {
-#pragma clang fp contract(fast)
-    const double* w = o.bary[s0];
-    d3 p = {0, 0, 0}, v = {0, 0, 0}, a = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < kLagrange; ++k) {
-        const double basis = w[k] * (pre[k] * suf[k + 1]);
-        const double db = basis * (tot - rd[k]);
-        const double* pp = o.pos[s0 + k];
-        const double* vv = o.vel[s0 + k];
-        p.x += basis * pp[0]; p.y += basis * pp[1]; p.z += basis * pp[2];
-        v.x += basis * vv[0]; v.y += basis * vv[1]; v.z += basis * vv[2];
-        a.x += db * vv[0]; a.y += db * vv[1]; a.z += db * vv[2];
-    }
-    *P = p; *V = v; *A = a;
+// Start of the 8-point window of the node c bracketing t (orbit.rs:530-533).
+SD_HD int orbit_window_start(int c, int n) {
+    if (c < 4) return 0;
+    if (c + 4 >= n) return n - kLagrange;
+    return c - 3;
 }
 
 // interpolate_orbit_and_accel (orbit.rs:498-596): the reference's window
-// rule, its refusal to extrapolate, its 1 ns reciprocal guard and its
-// derivative L'_k = L_k (Σ_j 1/(t-t_j) - 1/(t-t_k)).  The basis is formed in
-// barycentric form L_k = w_k Π_{j≠k}(t-t_j) with w_k = 1/Π_{j≠k}(t_k-t_j)
-// precomputed per window (OrbitDev::bary), and the reciprocals 1/(t-t_j)
-// from one division of the full product; this agrees with the reference's
-// 56-division product to a few ulps and costs one division instead of 64.
+// rule and its refusal to extrapolate.  Between nodes it evaluates the
+// interval's polynomials (OrbitPoly); at a node time (t = t_us[c] exactly;
+// times are whole µs, so the reference's 1 ns reciprocal guard fires there
+// and only there) it takes the reference's own values: every basis but the
+// node's is exactly 0 and the node's is exactly 1, so P and V are the node's
+// state vector and A = v_c · Σ_{j≠c} 1/(t - t_j) over the window, summed in
+// window order.
 SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
-#pragma clang fp contract(fast)
     const int n = o.n;
     if (n < kLagrange) return ORB_INSUFFICIENT;
     if (t < o.t_us[0] || t > o.t_us[n - 1]) return ORB_OUTSIDE;
@@ -139,54 +153,37 @@ SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
     c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
     while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
     while (c > 0 && o.t_us[c] > t) --c;
-    int start;
-    if (c < 4) start = 0;
-    else if (c + 4 >= n) start = n - kLagrange;
-    else start = c - 3;
-    // The 1 ns guard: times are whole µs and t_us[c] <= t < t_us[c+1], so
-    // |t - t_j| <= 1e-9 s holds for node c alone, when t is its time.
-    const bool guard = t == o.t_us[c];
-    double d[kLagrange], pre[kLagrange + 1], suf[kLagrange + 1];
-#pragma unroll
-    for (int j = 0; j < kLagrange; ++j) d[j] = ts - o.tsec[start + j];
-    pre[0] = 1.0;
-#pragma unroll
-    for (int j = 0; j < kLagrange; ++j) pre[j + 1] = pre[j] * d[j];
-    suf[kLagrange] = 1.0;
-#pragma unroll
-    for (int j = kLagrange - 1; j >= 0; --j) suf[j] = suf[j + 1] * d[j];
-    double rd[kLagrange];
-    double tot = 0.0;
-    if (!guard) {
-        const double inv_all = 1.0 / pre[kLagrange];
-#pragma unroll
-        for (int j = 0; j < kLagrange; ++j) {
-            rd[j] = inv_all * (pre[j] * suf[j + 1]);
-            tot += rd[j];
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < kLagrange; ++j) {
-            rd[j] = 0.0;
-            if (fabs(d[j]) > 1e-9) {
-                rd[j] = 1.0 / d[j];
-                tot += rd[j];
-            }
-        }
+    if (t == o.t_us[c]) {
+        const int start = orbit_window_start(c, n);
+        double tot = 0.0;
+        for (int j = 0; j < kLagrange; ++j)
+            if (start + j != c) tot += 1.0 / (ts - o.tsec[start + j]);
+        *P = {o.pos[c][0], o.pos[c][1], o.pos[c][2]};
+        *V = {o.vel[c][0], o.vel[c][1], o.vel[c][2]};
+        *A = {tot * o.vel[c][0], tot * o.vel[c][1], tot * o.vel[c][2]};
+        return ORB_OK;
     }
-    orbit_accumulate(o, start, pre, suf, rd, tot, P, V, A);
-    return ORB_OK;
-}
-
-// Barycentric weights w_k = 1/Π_{j≠k}(t_k-t_j) of every 8-point window.
-inline void orbit_fill_bary(OrbitDev& o) {
-    for (int s = 0; s + kLagrange <= o.n; ++s)
-        for (int k = 0; k < kLagrange; ++k) {
-            double prod = 1.0;
-            for (int j = 0; j < kLagrange; ++j)
-                if (j != k) prod *= o.tsec[s + k] - o.tsec[s + j];
-            o.bary[s][k] = 1.0 / prod;
+    const OrbitPoly& q = o.poly[c];
+    const double tau = ts - q.mid;
+    double r[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double pp = q.cp[d][7], vv = q.cv[d][7], aa = q.ca[d][6];
+#pragma unroll
+        for (int i = 6; i >= 0; --i) {
+            pp = fma(pp, tau, q.cp[d][i]);
+            vv = fma(vv, tau, q.cv[d][i]);
         }
+#pragma unroll
+        for (int i = 5; i >= 0; --i) aa = fma(aa, tau, q.ca[d][i]);
+        r[d] = pp;
+        r[3 + d] = vv;
+        r[6 + d] = aa;
+    }
+    *P = {r[0], r[1], r[2]};
+    *V = {r[3], r[4], r[5]};
+    *A = {r[6], r[7], r[8]};
+    return ORB_OK;
 }
 
 enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT_OUTSIDE = 3, ZD_ORBIT_INSUFFICIENT = 4 };

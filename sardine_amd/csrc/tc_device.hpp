@@ -256,10 +256,12 @@ __device__ __forceinline__ void flush_counts(const TcParams& p, const LaneCounts
     if (set_errors && C.err_short > 0) atomicOr(p.error_flag, 2);
 }
 
+// Copies the orbit into LDS up to its last interval polynomial.
 __device__ __forceinline__ void load_orbit(const TcParams& p, OrbitDev& orb) {
     const uint4* src = reinterpret_cast<const uint4*>(p.orbit);
     uint4* dst = reinterpret_cast<uint4*>(&orb);
-    for (int i = threadIdx.x; i < (int)(sizeof(OrbitDev) / sizeof(uint4)); i += blockDim.x) dst[i] = src[i];
+    const int n16 = (int)(orbit_lds_bytes(p.orbit->n) / sizeof(uint4));
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
     __syncthreads();
 }
 
