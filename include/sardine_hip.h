@@ -260,6 +260,16 @@ typedef struct sardine_geocoded_image {
 sardine_status sardine_tc_output_grid(const sardine_scene* scene, const sardine_tc_config* cfg,
                                       int64_t* rows, int64_t* cols, double geotransform[6]);
 
+/* interpolate_orbit_and_accel (orbit.rs:498-596): position, velocity and
+ * acceleration at time_us (absolute µs, the state vectors' clock), evaluated
+ * on the host by the same code the terrain-correction kernels run (the
+ * per-interval polynomial form, the reference's values at node times).
+ * Errors as the reference: SARDINE_ERR_TC_ORBIT_INSUFFICIENT below 8
+ * vectors, SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW outside the orbit.
+ * Host-only, no device needed. */
+sardine_status sardine_orbit_interpolate(const sardine_orbit* orbit, int64_t time_us, double pos[3],
+                                         double vel[3], double acc[3]);
+
 /* terrain_correction(merged, scene, dem, grids, config) -> GeocodedImage,
  * terrain_correction.rs:1245-1836.  Synchronous: returns after the counters
  * are on the host.  Output buffers must hold out->rows*out->cols elements of

@@ -40,6 +40,7 @@ def lib():
     L.sardine_tc_config_default.restype = None
     L.sardine_tc_output_grid.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.TcConfig), C.POINTER(i64),
                                          C.POINTER(i64), P]
+    L.sardine_orbit_interpolate.argtypes = [C.POINTER(abi.Orbit), i64, P, P, P]
     L.sardine_terrain_correction.argtypes = [C.POINTER(abi.RadarImage), C.POINTER(abi.Scene),
                                              C.POINTER(abi.DemRaster), C.POINTER(abi.TcConfig),
                                              C.POINTER(abi.GeocodedImage), P]

@@ -208,6 +208,7 @@ EXPORTED_SYMBOLS = (
     "sardine_device_count",
     "sardine_tc_config_default",
     "sardine_tc_output_grid",
+    "sardine_orbit_interpolate",
     "sardine_terrain_correction",
     "sardine_speckle_apply",
     "sardine_deburst_layout_compute",

@@ -742,6 +742,28 @@ sardine_status sardine_tc_output_grid(const sardine_scene* scene, const sardine_
     return SARDINE_OK;
 }
 
+sardine_status sardine_orbit_interpolate(const sardine_orbit* orbit, int64_t time_us, double pos[3], double vel[3],
+                                         double acc[3]) {
+    if (!orbit || !pos || !vel || !acc || (orbit->count > 0 && !orbit->state_vectors))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    if (orbit->count > sd::kMaxSv || orbit->count < 0)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", orbit->count, sd::kMaxSv);
+    for (int i = 1; i < orbit->count; ++i)
+        if (!(orbit->state_vectors[i].time_us > orbit->state_vectors[i - 1].time_us))
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "state vector times must be strictly increasing (vector %d)", i);
+    const std::unique_ptr<sd::OrbitDev> o(new sd::OrbitDev(make_orbit_dev(*orbit)));
+    const int64_t epoch = orbit->count > 0 ? orbit->state_vectors[0].time_us : 0;
+    sd::d3 P, V, A;
+    const int e = sd::orbit_interp(*o, time_us - epoch, &P, &V, &A);
+    if (e == sd::ORB_INSUFFICIENT)
+        return fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT, "need at least 8 state vectors, have %d", orbit->count);
+    if (e == sd::ORB_OUTSIDE) return fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW, "time outside the orbit window");
+    pos[0] = P.x; pos[1] = P.y; pos[2] = P.z;
+    vel[0] = V.x; vel[1] = V.y; vel[2] = V.z;
+    acc[0] = A.x; acc[1] = A.y; acc[2] = A.z;
+    return SARDINE_OK;
+}
+
 sardine_status sardine_terrain_correction(const sardine_radar_image* merged, const sardine_scene* scene,
                                           const sardine_dem_raster* dem, const sardine_tc_config* cfg,
                                           sardine_geocoded_image* out, void* stream_v) {

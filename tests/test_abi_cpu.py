@@ -53,6 +53,89 @@ def test_output_grid_matches_oracle(utm):
     assert np.allclose(gt, gt2, rtol=0, atol=1e-6 if utm else 1e-12)
 
 
+def test_orbit_model_matches_oracle():
+    """The kernels' orbit model (per-interval Lagrange-8 polynomials, the
+    reference's own values at node times), evaluated on the host through the
+    C ABI, against the oracle's product-form interpolate_orbit_and_accel
+    (orbit.rs:498-596) and against the exact interpolant (mpmath, 50 digits).
+
+    Exact equality at every state-vector time (where the reference's 1 ns
+    guard changes its acceleration formula); elsewhere within the product
+    form's own rounding.  The reference forms L'_k as L_k (Σ_j 1/(t-t_j) -
+    1/(t-t_k)), which cancels near a node: the sum is ~1/|t - t_node| and
+    its rounding, ~2.2e-16/|t - t_node|, is multiplied by velocities of
+    ~7.5e3 m/s over 8 terms (1 µs away: 1.4e-7 m/s² observed), so the
+    acceleration tolerance against the oracle widens as 2e-11 / |t - t_node|
+    there; against the exact interpolant the model stays within 1e-12 m/s²
+    everywhere."""
+    mp = pytest.importorskip("mpmath")
+    sc = synthetic.scene_iw_full(1)
+    keep = []
+    orb = O.orbit_c(sc.sv_t_us, sc.sv_pos, sc.sv_vel, keep)
+    L = _native.lib()
+    T = [int(t) for t in sc.sv_t_us]
+    t0, t1 = T[0], T[-1]
+    rng = np.random.default_rng(7)
+    times = T + [int(t) for t in rng.integers(t0, t1 + 1, 4000)] + [t0 + 1, t1 - 1, T[5] + 1, T[5] - 1, T[9] + 7]
+    p, v, a = np.zeros(3), np.zeros(3), np.zeros(3)
+
+    def model(t):
+        assert L.sardine_orbit_interpolate(C.byref(orb), t, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == abi.OK
+        return p.copy(), v.copy(), a.copy()
+
+    for t in times:
+        mp_, mv, ma = model(t)
+        st, rp, rv, ra = O.interpolate_orbit_and_accel(orb, t - t0)
+        assert st == 0
+        if t in T:
+            assert np.array_equal(mp_, rp) and np.array_equal(mv, rv) and np.array_equal(ma, ra), t
+            continue
+        gap_s = min(abs(t - tn) for tn in T) * 1e-6
+        assert np.abs(mp_ - rp).max() <= 1e-8, t
+        assert np.abs(mv - rv).max() <= 1e-10, t
+        assert np.abs(ma - ra).max() <= 1e-10 + 2e-11 / gap_s, t
+
+    # exact interpolant (same window rule) at a few times, near nodes included
+    mp.mp.dps = 50
+    n = len(T)
+    xs_all = [mp.mpf(tn - t0) / 10**6 for tn in T]
+    for t in [t0 + 1, T[5] + 1, T[9] + 7, t1 - 1] + [int(x) for x in rng.integers(t0, t1, 6)]:
+        c = max(i for i in range(n) if T[i] <= t)
+        s = 0 if c < 4 else (n - 8 if c + 4 >= n else c - 3)
+        ts = mp.mpf(t - t0) / 10**6
+        xs = xs_all[s:s + 8]
+        exact_p, exact_a = [mp.mpf(0)] * 3, [mp.mpf(0)] * 3
+        for k in range(8):
+            lk = mp.mpf(1)
+            dk = mp.mpf(0)
+            for j in range(8):
+                if j != k:
+                    lk *= (ts - xs[j]) / (xs[k] - xs[j])
+            for m in range(8):
+                if m == k:
+                    continue
+                term = 1 / (xs[k] - xs[m])
+                for j in range(8):
+                    if j not in (k, m):
+                        term *= (ts - xs[j]) / (xs[k] - xs[j])
+                dk += term
+            for d in range(3):
+                exact_p[d] += lk * mp.mpf(float(sc.sv_pos[s + k][d]))
+                exact_a[d] += dk * mp.mpf(float(sc.sv_vel[s + k][d]))
+        mp_, _, ma = model(t)
+        assert max(abs(float(mp_[d] - exact_p[d])) for d in range(3)) <= 2e-9, t
+        assert max(abs(float(ma[d] - exact_a[d])) for d in range(3)) <= 1e-12, t
+    assert L.sardine_orbit_interpolate(C.byref(orb), t1 + 1, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == \
+        abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
+    short = O.orbit_c(sc.sv_t_us[:7], sc.sv_pos[:7], sc.sv_vel[:7], keep)
+    assert L.sardine_orbit_interpolate(C.byref(short), t0 + 5, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == \
+        abi.ERR_TC_ORBIT_INSUFFICIENT
+
+
+def _np_ptr(x):
+    return x.ctypes.data_as(C.c_void_p)
+
+
 def test_deburst_and_merge_layout_match_oracle():
     sc = synthetic.scene_iw_full(4, n_swaths=3, n_bursts=4, lines_per_burst=500, raw_cols=1500, overlap_lines=77,
                                  swath_overlap=100)
