@@ -19,6 +19,7 @@ bounded sample of the same workload on the host cores.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import pathlib
@@ -64,6 +65,13 @@ def parse():
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def recent_kernel_ms(L, family: int, n: int) -> list:
+    """Durations of the last n launches of a kernel family (HIP events)."""
+    buf = (C.c_double * max(1, min(n, 64)))()
+    k = L.sardine_recent_kernel_ms(family, C.cast(buf, C.c_void_p), len(buf))
+    return list(buf[:k]) or [float("nan")]
 
 
 def tc_flops(img, need_geom: bool) -> float:
@@ -156,8 +164,10 @@ def main():
             f"newton iterations {img.newton_iter_histogram[:8]}")
     torch.cuda.synchronize()
 
+    # timed steps: the library's kernel timers record HIP events on each
+    # kernel's launch stream without waiting; they are read after the
+    # timed region, so nothing waits on the host between steps
     L.sardine_set_profiling(1)
-    k_tc, k_seg, k_cal, k_sp, lat = [], [], [], [], []
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     ev[0].record(stream)
@@ -165,14 +175,12 @@ def main():
     for i in range(args.steps):
         img = step()
         ev[i + 1].record(stream)
-        k_tc.append(L.sardine_last_kernel_ms(abi.KERNEL_TC))
-        k_seg.append(L.sardine_last_kernel_ms(abi.KERNEL_TC_SEGMENT))
-        k_cal.append(L.sardine_last_kernel_ms(abi.KERNEL_CALIBRATE))
-        k_sp.append(L.sardine_last_kernel_ms(abi.KERNEL_SPECKLE))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
-    L.sardine_set_profiling(0)
     lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    k_tc, k_seg, k_cal, k_sp = (recent_kernel_ms(L, f, args.steps) for f in (
+        abi.KERNEL_TC, abi.KERNEL_TC_SEGMENT, abi.KERNEL_CALIBRATE, abi.KERNEL_SPECKLE))
+    L.sardine_set_profiling(0)
     total_ms = ev[0].elapsed_time(ev[-1])
     ms_per_step = total_ms / args.steps
     px = rows * cols

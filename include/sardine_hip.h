@@ -442,6 +442,8 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_TO_DB = 3,
     SARDINE_KERNEL_TC_SEGMENT = 4 /* the per-pixel chain kernel of terrain correction alone */
 } sardine_kernel_family;
+/* Switches the kernel timers of the calling thread on (creating their
+ * events) or off. */
 void sardine_set_profiling(int on);
 /* Number of output columns one lane chains sequentially in the
  * terrain-correction kernel (0 = automatic, the default).  A tuning knob
@@ -455,6 +457,12 @@ void sardine_set_tc_lead(int32_t columns);
 /* Milliseconds of the last profiled kernel of `family` on this thread, -1 if
  * none; waits for that kernel to finish. */
 double sardine_last_kernel_ms(int family);
+/* The durations (ms) of the family's most recent launches, oldest first,
+ * at most min(max, 64); returns how many were written.  The timers record
+ * events without waiting, so a caller can time many calls back to back
+ * and read them afterwards; reading waits for the events it reads.
+ * No reference counterpart (diagnostics). */
+int32_t sardine_recent_kernel_ms(int family, double* ms, int32_t max);
 
 /* ── Resident scenes and batched geocoding (no reference counterpart: the
  * reference geocodes one scene per call; these entry points run many

@@ -58,6 +58,7 @@ def lib():
     L.sardine_set_profiling.restype = None
     L.sardine_last_kernel_ms.argtypes = [C.c_int]
     L.sardine_last_kernel_ms.restype = C.c_double
+    L.sardine_recent_kernel_ms.argtypes = [C.c_int, P, C.c_int32]
     L.sardine_set_tc_segment.argtypes = [C.c_int64]
     L.sardine_set_tc_segment.restype = None
     L.sardine_set_tc_lead.argtypes = [C.c_int32]

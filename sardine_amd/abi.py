@@ -219,6 +219,7 @@ EXPORTED_SYMBOLS = (
     "sardine_synth_cint16",
     "sardine_set_profiling",
     "sardine_last_kernel_ms",
+    "sardine_recent_kernel_ms",
     "sardine_set_tc_segment",
     "sardine_set_tc_lead",
     "sardine_scene_create",

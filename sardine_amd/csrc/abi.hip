@@ -53,29 +53,58 @@ thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
 constexpr int kFamilies = 5;
-thread_local double g_kernel_ms[kFamilies] = {-1.0, -1.0, -1.0, -1.0, -1.0};
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
 // waits here: sardine_last_kernel_ms waits for the end event when read.
-struct TimerSlot {
+// Kernel timers: per family, a ring of event pairs recorded around the
+// family's main kernel on its launch stream.  Recording never blocks; a
+// reader synchronises on the events it reads, so a caller can time a run
+// without waiting between its steps and read the durations afterwards.
+constexpr int kTimerRing = 64;
+struct TimerEntry {
     hipEvent_t a = nullptr, b = nullptr;
     bool pending = false;
+    double ms = -1.0;
+};
+struct TimerSlot {
+    TimerEntry e[kTimerRing];
+    int64_t next = 0;  // entries next-kTimerRing .. next-1 are the most recent
 };
 thread_local TimerSlot g_slots[kFamilies];
 
+// The next ring entry of a family, with its events created; null when
+// profiling is off or event creation fails.
+TimerEntry* timer_next(int family) {
+    if (!g_profile) return nullptr;
+    TimerSlot& t = g_slots[family];
+    TimerEntry& e = t.e[t.next % kTimerRing];
+    if (!e.a && hipEventCreate(&e.a) != hipSuccess) return nullptr;
+    if (!e.b && hipEventCreate(&e.b) != hipSuccess) return nullptr;
+    e.pending = false;
+    e.ms = -1.0;
+    ++t.next;
+    return &e;
+}
+
+double timer_read(TimerEntry& e) {
+    if (e.pending) {
+        e.pending = false;
+        float ms = 0.0f;
+        if (hipEventSynchronize(e.b) == hipSuccess && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) e.ms = ms;
+    }
+    return e.ms;
+}
+
 struct KernelTimer {
-    TimerSlot* slot = nullptr;
+    TimerEntry* entry = nullptr;
     hipStream_t s;
     KernelTimer(int family, hipStream_t stream) : s(stream) {
-        if (!g_profile) return;
-        TimerSlot& t = g_slots[family];
-        if (!t.a && hipEventCreate(&t.a) != hipSuccess) return;
-        if (!t.b && hipEventCreate(&t.b) != hipSuccess) return;
-        if (hipEventRecord(t.a, s) == hipSuccess) slot = &t;
+        TimerEntry* e = timer_next(family);
+        if (e && hipEventRecord(e->a, s) == hipSuccess) entry = e;
     }
     void stop() {
-        if (slot && hipEventRecord(slot->b, s) == hipSuccess) slot->pending = true;
-        slot = nullptr;
+        if (entry && hipEventRecord(entry->b, s) == hipSuccess) entry->pending = true;
+        entry = nullptr;
     }
 };
 
@@ -691,19 +720,30 @@ sardine_status finish_tc(const TcPlan& pl, const unsigned long long* cnt, int n_
 extern "C" {
 
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
-void sardine_set_profiling(int on) { g_profile = on != 0; }
+void sardine_set_profiling(int on) {
+    g_profile = on != 0;
+    if (!g_profile) return;
+    // create this thread's timer events now, outside whatever is timed next
+    for (TimerSlot& t : g_slots)
+        for (TimerEntry& e : t.e) {
+            if (!e.a) (void)hipEventCreate(&e.a);
+            if (!e.b) (void)hipEventCreate(&e.b);
+        }
+}
 void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : columns; }
 void sardine_set_tc_lead(int32_t columns) { g_tc_lead = columns < 0 ? 0 : (columns > 64 ? 64 : columns); }
 double sardine_last_kernel_ms(int family) {
     if (family < 0 || family >= kFamilies) return -1.0;
     TimerSlot& t = g_slots[family];
-    if (t.pending) {
-        t.pending = false;
-        float ms = 0.0f;
-        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
-            g_kernel_ms[family] = ms;
-    }
-    return g_kernel_ms[family];
+    if (t.next == 0) return -1.0;
+    return timer_read(t.e[(t.next - 1) % kTimerRing]);
+}
+int32_t sardine_recent_kernel_ms(int family, double* ms, int32_t max) {
+    if (family < 0 || family >= kFamilies || !ms || max <= 0) return 0;
+    TimerSlot& t = g_slots[family];
+    const int64_t n = std::min<int64_t>({(int64_t)max, t.next, (int64_t)kTimerRing});
+    for (int64_t i = 0; i < n; ++i) ms[i] = timer_read(t.e[(t.next - n + i) % kTimerRing]);
+    return (int32_t)n;
 }
 int sardine_abi_version(void) { return SARDINE_ABI_VERSION; }
 
@@ -825,11 +865,7 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     } else {
         KernelTimer kt(SARDINE_KERNEL_TC, stream);
         // the per-pixel chain kernel alone is timed as its own family
-        TimerSlot* seg = nullptr;
-        if (g_profile) {
-            TimerSlot& t = g_slots[SARDINE_KERNEL_TC_SEGMENT];
-            if ((t.a || hipEventCreate(&t.a) == hipSuccess) && (t.b || hipEventCreate(&t.b) == hipSuccess)) seg = &t;
-        }
+        TimerEntry* seg = timer_next(SARDINE_KERNEL_TC_SEGMENT);
         SD_HIP(sd_launch_tc_fp64(&pl.p, stream, seg ? seg->a : nullptr, seg ? seg->b : nullptr));
         if (seg) seg->pending = true;
         kt.stop();
