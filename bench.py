@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import gc
 import json
 import os
 import pathlib
@@ -47,8 +48,8 @@ FP64_PEAK_TFLOPS = 78.6
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32"], default="iw_full",
                     help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
                          "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request; "
@@ -65,6 +66,15 @@ def parse():
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def settle_gc() -> None:
+    """Collect once and move every object created so far out of the cyclic
+    garbage collector's view (gc.freeze): a full pass over the set-up's
+    objects takes tens of ms and would otherwise land inside a timed step
+    (timeit turns the collector off for the same reason)."""
+    gc.collect()
+    gc.freeze()
 
 
 def recent_kernel_ms(L, family: int, n: int) -> list:
@@ -158,17 +168,18 @@ def main():
             filt.apply(img.data, img.cols, img.rows, stream=stream, out=sp)
         return img
 
+    # the kernel timers (created here, before the warm-up, so that the GPU is
+    # busy right up to the timed region) record HIP events on each kernel's
+    # launch stream without waiting; they are read after the timed region
+    L.sardine_set_profiling(1)
     for i in range(args.warmup):
         img = step()
         log(f"warmup {i}: valid {img.valid_pixel_count}/{img.total_pixel_count}, redo {img.chain_redo_rounds}, "
             f"newton iterations {img.newton_iter_histogram[:8]}")
-    torch.cuda.synchronize()
 
-    # timed steps: the library's kernel timers record HIP events on each
-    # kernel's launch stream without waiting; they are read after the
-    # timed region, so nothing waits on the host between steps
-    L.sardine_set_profiling(1)
+    # timed steps: nothing waits on the host between steps
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    settle_gc()
     torch.cuda.synchronize()
     ev[0].record(stream)
     t_wall = time.perf_counter()
@@ -369,6 +380,7 @@ def bench_batch64(args, dev):
 
     plain()
     torch.cuda.synchronize()
+    settle_gc()
     reps = max(1, args.steps)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
     ev[0].record(stream)
@@ -387,6 +399,7 @@ def bench_batch64(args, dev):
     prep = b.prepare([rs], jobs)
     b.execute(prep, stream=stream)
     torch.cuda.synchronize()
+    settle_gc()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
     ev[0].record(stream)
     dev_ms = []
@@ -505,6 +518,7 @@ def bench_serve(args, dev):
         return time.perf_counter() - start, lat
 
     srv = api.Server([s[2] for s in scenes], hbm_budget_bytes=64 << 30, max_batch=64, slots=2)
+    settle_gc()
 
     def served_run(arrivals):
         tickets = [None] * n

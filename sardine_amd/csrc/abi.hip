@@ -649,8 +649,8 @@ void set_decomposition(sd::TcParams& p, int64_t requested) {
 
 // Bytes of the per-job work buffers, each rounded to 256.
 struct TcWork {
-    size_t records = 0, anchors = 0, seed = 0, flat = 0, geoid = 0;
-    size_t total() const { return records + anchors + seed + flat + geoid; }
+    size_t records = 0, flags = 0, anchors = 0, seed = 0, flat = 0, geoid = 0;
+    size_t total() const { return records + flags + anchors + seed + flat + geoid; }
 };
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
@@ -659,6 +659,7 @@ TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
     w.records = cfg->precision == SARDINE_PRECISION_FP32_CENTRED
                     ? 0
                     : round256((size_t)pl.nr * pl.p.nseg * sizeof(sd::SegRecord));
+    w.flags = w.records ? round256((size_t)pl.nr * sizeof(int32_t)) : 0;
     w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * pl.p.anchor_stride * sizeof(double)) : 0;
     w.seed = round256((size_t)pl.nr * sizeof(int64_t));
     w.flat = pl.need_geom ? round256((size_t)pl.nr * sizeof(double)) : 0;
@@ -669,6 +670,8 @@ TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
 void assign_work(TcPlan& pl, const TcWork& w, char* base) {
     pl.p.records = reinterpret_cast<sd::SegRecord*>(base);
     base += w.records;
+    pl.p.row_flags = w.flags ? reinterpret_cast<int32_t*>(base) : nullptr;
+    base += w.flags;
     pl.p.anchors = w.anchors ? reinterpret_cast<double*>(base) : nullptr;
     base += w.anchors;
     pl.p.row_seed = reinterpret_cast<const int64_t*>(base);

@@ -53,6 +53,12 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(Tc
     segment_wave<RS, UTM>(p, orb, stage[w], (int64_t)blockIdx.x * 64, seg);
 }
 
+// One wave per AOI row (check_row).
+__global__ __launch_bounds__(256) void tc_check_kernel(TcParams p) {
+    const int64_t rr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (rr < p.rows) check_row(p, rr);
+}
+
 template <int RS, bool UTM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tc_fixup_kernel(TcParams p) {
     __shared__ OrbitDev orb;
@@ -70,6 +76,7 @@ void launch_chain(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_star
     if (seg_start) (void)hipEventRecord(seg_start, stream);
     hipLaunchKernelGGL((sd::tc_segment_kernel<RS, UTM>), grid, dim3(256), 0, stream, *p);
     if (seg_end) (void)hipEventRecord(seg_end, stream);
+    hipLaunchKernelGGL(sd::tc_check_kernel, dim3((unsigned)((p->rows + 3) / 4)), dim3(256), 0, stream, *p);
     hipLaunchKernelGGL((sd::tc_fixup_kernel<RS, UTM>), dim3((unsigned)((p->rows + 255) / 256)), dim3(256), 0, stream,
                        *p);
 }

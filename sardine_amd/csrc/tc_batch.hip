@@ -89,6 +89,18 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void batch_tc_kernel(cons
     }
 }
 
+// Row check before the fix-up pass: one wave per row of every job.
+__global__ __launch_bounds__(256) void batch_check_kernel(const BatchHeader* __restrict__ H,
+                                                          const BatchJob* __restrict__ J) {
+    const int n_jobs = H->n_jobs;
+    const int64_t n = H->n_rows;
+    const int64_t wave0 = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6), waves = (int64_t)gridDim.x * 4;
+    for (int64_t i = wave0; i < n; i += waves) {
+        const int j = __builtin_amdgcn_readfirstlane(find_job(n_jobs, i, [&](int k) { return J[k].row_begin; }));
+        check_row(J[j].p, i - J[j].row_begin);
+    }
+}
+
 // Fix-up pass: blocks of 256 rows of one job each.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_fixup_kernel(
     const BatchHeader* __restrict__ H, const BatchJob* __restrict__ J) {
@@ -127,6 +139,7 @@ extern "C" hipError_t sd_launch_batch_tc(const sd::BatchHeader* H, const sd::Bat
     hipLaunchKernelGGL(sd::batch_anchor_kernel, dim3(1024), dim3(256), 0, stream, H, J);
     hipLaunchKernelGGL(sd::batch_tc_kernel, dim3((unsigned)(n_cu * tc_groups_per_cu)), dim3(256), 0, stream, H, J,
                        queue);
+    hipLaunchKernelGGL(sd::batch_check_kernel, dim3((unsigned)(n_cu * 8)), dim3(256), 0, stream, H, J);
     hipLaunchKernelGGL(sd::batch_fixup_kernel, dim3((unsigned)(n_cu * 2)), dim3(256), 0, stream, H, J);
     return hipGetLastError();
 }

@@ -502,16 +502,59 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     flush_counts(p, C, false);  // orbit errors are decided after the fix-up
 }
 
+// One wave per AOI row, before the fix-up pass: a segment that ran a Newton
+// solve must have started from the chained value to its left (the last
+// converged time of the nearest segment to its left that has one, or the
+// row seed).  The row's records are read 64 at a time (coalesced); the
+// chained value entering each lane is the `last` of the nearest lane below
+// it whose segment has one (ballot + shuffle), else the value carried in
+// from the previous 64.  A row with any mismatch is flagged for the fix-up
+// walk, which also adds its orbit-error counts; the others add theirs here.
+__device__ __forceinline__ void check_row(const TcParams& p, int64_t rr) {
+    const int lane = threadIdx.x & 63;
+    const SegRecord* recs = p.records + rr * p.nseg;
+    long long carry = p.row_seed[rr];
+    bool bad = false;
+    int eo = 0, es = 0;
+    for (int64_t k0 = 0; k0 < p.nseg; k0 += 64) {
+        const int64_t k = k0 + lane;
+        SegRecord r;
+        r.seed = r.last = 0;
+        r.flags = r.err_outside = r.err_short = r.pad = 0;
+        if (k < p.nseg) r = recs[k];
+        const unsigned long long has = __ballot((r.flags & 1) != 0);
+        const unsigned long long below = has & ((1ull << lane) - 1ull);
+        const int src = below ? 63 - __clzll((long long)below) : lane;
+        const long long left = __shfl(r.last, src);
+        const long long E = below ? left : carry;
+        if ((r.flags & 2) && r.seed != E) bad = true;
+        eo += r.err_outside;
+        es += r.err_short;
+        if (has) carry = __shfl(r.last, 63 - __clzll((long long)has));
+    }
+    const bool row_bad = __any(bad);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        eo += __shfl_xor(eo, o);
+        es += __shfl_xor(es, o);
+    }
+    if (lane == 0) {
+        p.row_flags[rr] = row_bad ? 1 : 0;
+        if (!row_bad && eo > 0) atomicOr(p.error_flag, 1);
+        if (!row_bad && es > 0) atomicOr(p.error_flag, 2);
+    }
+}
+
 // One lane per row: walk the row's segments in order and correct every
 // segment whose starting seed differs from the chained value to its left.
-// Lanes whose row is past the end (valid false) only join the wave-wide
-// counter reduction.
+// Only rows check_row flagged are walked; the other lanes (and lanes whose
+// row is past the end) only join the wave-wide counter reduction.
 template <int RS, bool UTM>
 __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb, int64_t rr, bool valid) {
     LaneCounts C;
     zero_counts(C);
     unsigned redone = 0;
-    if (valid) {
+    if (valid && p.row_flags[rr]) {
         const int64_t row = p.row0 + rr;
         const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
         const int npts = p.anchor_stride;

@@ -70,6 +70,7 @@ struct TcParams {
     double* anchors;                  // UTM: per AOI row lon[stride], lat[stride] of anchors anchor0..
     int anchor0, anchor_stride;       // first stored anchor index, stored anchors per row
     SegRecord* records;               // per (AOI row, segment)
+    int32_t* row_flags;               // per AOI row: 1 = the fix-up pass walks the row
     const int64_t* row_seed;      // per AOI row, µs relative to the first state vector
     const double* row_flat_area;  // per AOI row (need_geom)
     // config
