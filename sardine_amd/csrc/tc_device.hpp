@@ -389,13 +389,50 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
 #endif
 #define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
 
+// The FP64 constants the per-pixel code reads (grid, DEM, radar geometry,
+// Newton tolerance) as a copy of the kernel arguments whose doubles sit in
+// vector registers: kept in scalar registers across the pixel loop they
+// exceed the 102 SGPRs of a wave and are spilled and reloaded one
+// v_readlane per dword per use; the kernel has spare VGPRs at its
+// occupancy.  The empty asm only pins each value to a VGPR.
+__device__ __forceinline__ void pin_vgpr(double& x) { asm volatile("" : "+v"(x)); }
+
+__device__ __forceinline__ TcParams vgpr_params(const TcParams& p) {
+    TcParams q = p;
+    pin_vgpr(q.spacing);
+    pin_vgpr(q.lat_north);
+    pin_vgpr(q.lon_west);
+    pin_vgpr(q.x_start);
+    pin_vgpr(q.anchor_dx);
+    pin_vgpr(q.inv_anchor_dx);
+    pin_vgpr(q.dem_olat);
+    pin_vgpr(q.dem_olon);
+    pin_vgpr(q.dem_pix);
+    pin_vgpr(q.dem_inv_pix);
+    pin_vgpr(q.dem_max_row);
+    pin_vgpr(q.dem_max_col);
+    pin_vgpr(q.dem_hi_row);
+    pin_vgpr(q.dem_hi_col);
+    pin_vgpr(q.step_deg);
+    pin_vgpr(q.step_sin);
+    pin_vgpr(q.step_cos);
+    pin_vgpr(q.near_range_m);
+    pin_vgpr(q.rps);
+    pin_vgpr(q.inv_rps);
+    pin_vgpr(q.ati);
+    pin_vgpr(q.inv_ati);
+    pin_vgpr(q.tol);
+    return q;
+}
+
 // Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
 // a workgroup take 4 consecutive segments.  Lanes past the last row compute
 // the last row again and discard it, so every lane reaches the wave-wide
 // flushes.
 template <int RS, bool UTM>
-__device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
+__device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                              int64_t seg) {
+    const TcParams p = vgpr_params(p_arg);
     const int lane = threadIdx.x & 63;
     const bool row_ok = rr0 + lane < p.rows;
     const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;
