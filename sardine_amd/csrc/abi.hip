@@ -420,6 +420,60 @@ sd::OrbitDev make_orbit_dev(const sardine_orbit& o) {
     return orb;
 }
 
+// The binary128 interval polynomials cost ~0.2 ms of host time per orbit;
+// calls on the same state vectors (compared bit for bit) reuse the last few
+// results.
+class OrbitDevCache {
+   public:
+    std::shared_ptr<const sd::OrbitDev> get(const sardine_orbit& o) {
+        const size_t bytes = (size_t)std::max(o.count, 0) * sizeof(sardine_state_vector);
+        std::lock_guard<std::mutex> g(mu_);
+        ++tick_;
+        for (Entry& e : entries_)
+            if (e.key.size() == bytes && (bytes == 0 || std::memcmp(e.key.data(), o.state_vectors, bytes) == 0)) {
+                e.used = tick_;
+                return e.orb;
+            }
+        auto orb = std::make_shared<const sd::OrbitDev>(make_orbit_dev(o));
+        Entry ne;
+        ne.key.assign(reinterpret_cast<const char*>(o.state_vectors), reinterpret_cast<const char*>(o.state_vectors) + bytes);
+        ne.orb = orb;
+        ne.used = tick_;
+        if (entries_.size() < kEntries) {
+            entries_.push_back(std::move(ne));
+        } else {
+            auto lru = std::min_element(entries_.begin(), entries_.end(),
+                                        [](const Entry& a, const Entry& b) { return a.used < b.used; });
+            *lru = std::move(ne);
+        }
+        return orb;
+    }
+
+   private:
+    struct Entry {
+        std::vector<char> key;
+        std::shared_ptr<const sd::OrbitDev> orb;
+        uint64_t used = 0;
+    };
+    static constexpr size_t kEntries = 8;
+    std::mutex mu_;
+    std::vector<Entry> entries_;
+    uint64_t tick_ = 0;
+};
+
+OrbitDevCache& orbit_cache() {
+    static OrbitDevCache* c = new OrbitDevCache();
+    return *c;
+}
+
+// Uploads an orbit: the whole struct is allocated, the part the kernels
+// read (up to its last interval polynomial) is copied.
+hipError_t upload_orbit(DevBuf& b, const sd::OrbitDev& o, hipStream_t s) {
+    hipError_t e = b.alloc(sizeof(sd::OrbitDev), s);
+    if (e != hipSuccess) return e;
+    return staging().copy(b.p, &o, sd::orbit_lds_bytes(o.n), s);
+}
+
 // Device copies of a scene's orbit and seed table.
 struct SceneDev {
     const sd::OrbitDev* orb = nullptr;
@@ -794,7 +848,7 @@ sardine_status sardine_orbit_interpolate(const sardine_orbit* orbit, int64_t tim
     for (int i = 1; i < orbit->count; ++i)
         if (!(orbit->state_vectors[i].time_us > orbit->state_vectors[i - 1].time_us))
             return fail(SARDINE_ERR_INVALID_ARGUMENT, "state vector times must be strictly increasing (vector %d)", i);
-    const std::unique_ptr<sd::OrbitDev> o(new sd::OrbitDev(make_orbit_dev(*orbit)));
+    const std::shared_ptr<const sd::OrbitDev> o = orbit_cache().get(*orbit);
     const int64_t epoch = orbit->count > 0 ? orbit->state_vectors[0].time_us : 0;
     sd::d3 P, V, A;
     const int e = sd::orbit_interp(*o, time_us - epoch, &P, &V, &A);
@@ -824,9 +878,10 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     if ((st = require_device())) return st;
 
     // scene context of this call: orbit and seed table on the device
-    const sd::OrbitDev orb = make_orbit_dev(scene->orbit);
+    const std::shared_ptr<const sd::OrbitDev> orb_p = orbit_cache().get(scene->orbit);
+    const sd::OrbitDev& orb = *orb_p;
     DevBuf d_orb, d_lut_lat, d_lut_t, d_work, d_cnt;
-    SD_HIP(upload(d_orb, &orb, 1, stream));
+    SD_HIP(upload_orbit(d_orb, orb, stream));
     SD_HIP(upload(d_lut_lat, lut.lat.data(), lut.lat.size(), stream));
     SD_HIP(upload(d_lut_t, lut.t.data(), lut.t.size(), stream));
     pl.p.orbit = d_orb.as<sd::OrbitDev>();

@@ -390,12 +390,15 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
 #define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
 
 // The FP64 constants the per-pixel code reads (grid, DEM, radar geometry,
-// Newton tolerance) as a copy of the kernel arguments whose doubles sit in
-// vector registers: kept in scalar registers across the pixel loop they
+// Newton tolerance) as a copy of the kernel arguments held in vector
+// registers: kept in scalar registers across the pixel loop they
 // exceed the 102 SGPRs of a wave and are spilled and reloaded one
 // v_readlane per dword per use; the kernel has spare VGPRs at its
 // occupancy.  The empty asm only pins each value to a VGPR.
-__device__ __forceinline__ void pin_vgpr(double& x) { asm volatile("" : "+v"(x)); }
+template <typename T>
+__device__ __forceinline__ void pin_vgpr(T& x) {
+    asm volatile("" : "+v"(x));
+}
 
 __device__ __forceinline__ TcParams vgpr_params(const TcParams& p) {
     TcParams q = p;
@@ -429,10 +432,13 @@ __device__ __forceinline__ TcParams vgpr_params(const TcParams& p) {
 // a workgroup take 4 consecutive segments.  Lanes past the last row compute
 // the last row again and discard it, so every lane reaches the wave-wide
 // flushes.
-template <int RS, bool UTM>
+// PIN: use vgpr_params (pays where the kernel has spare VGPRs: the UTM
+// bilinear and bicubic chains; the Lanczos, EPSG:4326 and batched kernels
+// are at the register limit already and would spill to scratch instead).
+template <int RS, bool UTM, bool PIN = (UTM && RS <= 1)>
 __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                              int64_t seg) {
-    const TcParams p = vgpr_params(p_arg);
+    const TcParams p = PIN ? vgpr_params(p_arg) : p_arg;
     const int lane = threadIdx.x & 63;
     const bool row_ok = rr0 + lane < p.rows;
     const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;
