@@ -693,7 +693,10 @@ void set_decomposition(sd::TcParams& p, int64_t requested) {
     int64_t L = requested;
     if (L <= 0) {
         const int64_t target_lanes = (int64_t)256 * 4 * 2 * 64;
-        L = std::min<int64_t>(32, std::max<int64_t>(16, (p.rows * p.cols + target_lanes - 1) / target_lanes));
+        // the 2-column lead of every segment is overhead: as long a segment
+        // as still fills the GPU (measured on the 473.7 Mpx UTM grid:
+        // L = 32 25.5 ms, 64 25.1 ms, 128 25.1 ms)
+        L = std::min<int64_t>(64, std::max<int64_t>(16, (p.rows * p.cols + target_lanes - 1) / target_lanes));
     }
     L = std::min<int64_t>(L, p.cols);
     p.seg_len = (int)L;
