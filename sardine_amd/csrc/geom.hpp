@@ -65,8 +65,10 @@ SD_HD d3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
     return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - kE2) + h) * sl};
 }
 
+// sin and cos of deg·π/180, the radians formed as the reference does
+// ((deg·π) / 180, the division exact by div_rn).
 SD_HD void sincos_deg(double deg, double* s, double* c) {
-    const double r = deg * kPi / 180.0;
+    const double r = div_rn(deg * kPi, 180.0, 1.0 / 180.0);
 #ifdef __HIP_DEVICE_COMPILE__
     sincos(r, s, c);
 #else

@@ -118,64 +118,100 @@ __device__ __forceinline__ float keys_w32(float t) {
     return 0.0f;
 }
 
+// Lanczos-3 weights of one axis in FP32, by the FP64 path's expansion about
+// the tap offset r nearest zero (lanczos_axis): one polynomial sincos of
+// πr/3 (|πr/3| <= π/6; Taylor to φ⁹ / φ⁸), sin(πr) = 3s - 4s³, angle
+// addition for the other taps and one division for the six 1/x².
 __device__ __forceinline__ void lanczos_axis32(float u, float w[6]) {
-    const float pi = 3.14159265358979f;
-    const float s1 = sinf(pi * u);
-    float s3, c3;
-    sincosf(pi * u / 3.0f, &s3, &c3);
+    const bool up = u >= 0.5f;
+    const float r = up ? u - 1.0f : u;
+    const float f = 1.04719755f * r;  // π/3 · r
+    const float z = f * f;
+    const float s3 = f + f * z * (-1.66666667e-1f + z * (8.33333333e-3f + z * (-1.98412698e-4f + z * 2.75573192e-6f)));
+    const float c3 = 1.0f + z * (-0.5f + z * (4.16666667e-2f + z * (-1.38888889e-3f + z * 2.48015873e-5f)));
+    const float s1 = s3 * (3.0f - 4.0f * (s3 * s3));
     const float h = 0.866025404f;
-    const float C[6] = {-0.5f, 0.5f, 1.0f, 0.5f, -0.5f, -1.0f};
-    const float S[6] = {h, h, 0.0f, -h, -h, 0.0f};
+    const float K = 0.303963551f;  // 3 / π²
+    const float CM[7] = {-1.0f, -0.5f, 0.5f, 1.0f, 0.5f, -0.5f, -1.0f};
+    const float SM[7] = {0.0f, -h, -h, 0.0f, h, h, 0.0f};
+    float a[7], d[7];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const float x = u + (float)(2 - k), ax = fabsf(x);
-        float v = 0.0f;
-        if (ax < 1e-6f) v = 1.0f;
-        else if (ax < 3.0f) v = ((k & 1) ? -s1 : s1) * (s3 * C[k] + c3 * S[k]) / (pi * x * (pi * x / 3.0f));
-        w[k] = v;
+    for (int i = 0; i < 7; ++i) {
+        const int m = i - 3;
+        const float x = r + (float)m, ax = fabsf(x);
+        const float sp = (m & 1) ? -s1 : s1;
+        if (ax < 1e-6f) {
+            a[i] = 1.0f;
+            d[i] = 1.0f;
+        } else if (ax < 3.0f) {
+            a[i] = sp * (s3 * CM[i] + c3 * SM[i]) * K;
+            d[i] = x * x;
+        } else {
+            a[i] = 0.0f;
+            d[i] = 1.0f;
+        }
     }
+    float pre[8], suf[8];
+    pre[0] = 1.0f;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) pre[i + 1] = pre[i] * d[i];
+    suf[7] = 1.0f;
+#pragma unroll
+    for (int i = 6; i >= 0; --i) suf[i] = suf[i + 1] * d[i];
+    const float inv = 1.0f / pre[7];
+    float W[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) W[i] = a[i] * (inv * (pre[i] * suf[i + 1]));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = up ? W[6 - k] : W[5 - k];
 }
 
 // The three resamplers with FP32 weights and sums (NaN taps propagate).
 template <int RS>
 __device__ __forceinline__ float resample32(const float* d, int64_t nl, int64_t ns, double line, double sample) {
-    const int64_t lf = (int64_t)floor(line), sf = (int64_t)floor(sample);
+    const int lf = (int)floor(line), sf = (int)floor(sample);
     const float dl = (float)(line - (double)lf), ds = (float)(sample - (double)sf);
     if constexpr (RS == 0) {
-        const int64_t ml = nl > 0 ? nl - 1 : 0, ms = ns > 0 ? ns - 1 : 0;
-        const int64_t l0 = clampi(lf, 0, ml), l1 = l0 + 1 < ml ? l0 + 1 : ml;
-        const int64_t s0 = clampi(sf, 0, ms), s1 = s0 + 1 < ms ? s0 + 1 : ms;
+        const int ml = nl > 0 ? (int)nl - 1 : 0, ms = ns > 0 ? (int)ns - 1 : 0;
+        const int l0 = clampi(lf, 0, ml), l1 = l0 + 1 < ml ? l0 + 1 : ml;
+        const int s0 = clampi(sf, 0, ms), s1 = s0 + 1 < ms ? s0 + 1 : ms;
         const float el = (float)(line - (double)l0), es = (float)(sample - (double)s0);
-        const float v00 = d[l0 * ns + s0], v01 = d[l0 * ns + s1], v10 = d[l1 * ns + s0], v11 = d[l1 * ns + s1];
+        const float* r0 = image_row(d, ns, l0);
+        const float* r1 = image_row(d, ns, l1);
+        const float v00 = r0[s0], v01 = r0[s1], v10 = r1[s0], v11 = r1[s1];
         return v00 * (1.0f - el) * (1.0f - es) + v01 * (1.0f - el) * es + v10 * el * (1.0f - es) + v11 * el * es;
     } else if constexpr (RS == 1) {
         float ws[4], acc = 0.0f;
-        int64_t cs[4];
+        int cs[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             ws[k] = keys_w32((float)(k - 1) - ds);
-            cs[k] = clampi(sf + k - 1, 0, ns - 1);
+            cs[k] = clampi(sf + k - 1, 0, (int)ns - 1);
         }
 #pragma unroll
         for (int li = -1; li <= 2; ++li) {
-            const float* row = d + clampi(lf + li, 0, nl - 1) * ns;
+            const float* row = image_row(d, ns, clampi(lf + li, 0, (int)nl - 1));
             const float wl = keys_w32((float)li - dl);
+            float rs = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc += row[cs[k]] * wl * ws[k];
+            for (int k = 0; k < 4; ++k) rs += row[cs[k]] * ws[k];
+            acc += rs * wl;
         }
         return acc;
     } else {
         float ws[6], wl[6], acc = 0.0f;
         lanczos_axis32(ds, ws);
         lanczos_axis32(dl, wl);
-        int64_t cs[6];
+        int cs[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) cs[k] = clampi(sf + k - 2, 0, ns - 1);
+        for (int k = 0; k < 6; ++k) cs[k] = clampi(sf + k - 2, 0, (int)ns - 1);
 #pragma unroll
         for (int li = 0; li < 6; ++li) {
-            const float* row = d + clampi(lf + li - 2, 0, nl - 1) * ns;
+            const float* row = image_row(d, ns, clampi(lf + li - 2, 0, (int)nl - 1));
+            float rs = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc += row[cs[k]] * wl[li] * ws[k];
+            for (int k = 0; k < 6; ++k) rs += row[cs[k]] * ws[k];
+            acc += rs * wl[li];
         }
         return acc;
     }
