@@ -164,6 +164,26 @@ __host__ __device__ constexpr unsigned rl_dirs(int dy, int dx) {
     return m;
 }
 
+// The directions whose first member in raster order (dy outer, dx inner)
+// is (dy, dx): there a direction's sums start, so its accumulators take the
+// member's values instead of adding them to 0.0.  (0.0 + x differs from x
+// only for x = -0.0, which changes nothing observable: a direction whose
+// sum is a zero has mean 0 and is never picked, speckle.rs:546.)
+struct RlFirstTable {
+    unsigned m[7][7];
+};
+constexpr RlFirstTable rl_first_table() {
+    RlFirstTable t{};
+    unsigned seen = 0;
+    for (int y = -3; y <= 3; ++y)
+        for (int x = -3; x <= 3; ++x) {
+            t.m[y + 3][x + 3] = rl_dirs(y, x) & ~seen;
+            seen |= rl_dirs(y, x);
+        }
+    return t;
+}
+constexpr RlFirstTable kRlFirst = rl_first_table();
+
 // x / D for a constant window count D, correctly rounded: q0 = x·RN(1/D)
 // is within an ulp of x/D and one FMA residual step (Markstein 1990, the
 // closing step of the IEEE division sequence) rounds it correctly.  Checked
@@ -215,12 +235,18 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
         for (int dx = -3; dx <= 3; ++dx) {
             const unsigned m = rl_dirs(dy, dx);
             if (m == 0u) continue;
+            const unsigned first = kRlFirst.m[dy + 3][dx + 3];
             const double x = (double)t[dy * pitch + dx], xx = x * x;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
                 if (m & (1u << d)) {
-                    sum[d] += x;
-                    sq[d] += xx;
+                    if (first & (1u << d)) {
+                        sum[d] = x;
+                        sq[d] = xx;
+                    } else {
+                        sum[d] += x;
+                        sq[d] += xx;
+                    }
                 }
             // one window row at a time: keeps the scheduler from hoisting
             // all 48 LDS reads (and their registers) to the top
@@ -231,15 +257,54 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
     for (int d = 0; d < 8; ++d) all += sum[d];
     float best_cv = __builtin_inff(), best_mean = centre, best_var = 0.0f;
     if (isfinite(all)) {
+        // Every direction's mean and variance, and its CV² = var / mean² from
+        // a reciprocal (within 2.4e-7 relative of the correctly rounded
+        // quotient).  These decide the pick unless another candidate is
+        // within 2^-19 relative of the smallest (more than twice the error
+        // of both), or a mean² is zero, subnormal or infinite; then every
+        // CV² is recomputed by exact division (rl_pick), which resolves
+        // ties to the first direction as the reference does.
+        float mfv[8], vfv[8], cva[8];
+        bool okv[8], special = false;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {  // S, N, E, W strips: 21 members
-            const double m = div_const<21>(sum[d]);
-            rl_pick(m, div_const<21>(sq[d]) - m * m, &best_cv, &best_mean, &best_var);
+        for (int d = 0; d < 8; ++d) {  // S, N, E, W strips: 21 members; diagonal bands: 9
+            const double m = d < 4 ? div_const<21>(sum[d]) : div_const<9>(sum[d]);
+            double va = (d < 4 ? div_const<21>(sq[d]) : div_const<9>(sq[d])) - m * m;
+            va = va > 0.0 ? va : 0.0;
+            const float mf = (float)m, vf = (float)va, m2 = mf * mf;
+            mfv[d] = mf;
+            vfv[d] = vf;
+            okv[d] = mf > 0.0f;
+            cva[d] = vf * __builtin_amdgcn_rcpf(m2);
+            special |= okv[d] && !(m2 >= 1.17549435e-38f && cva[d] <= 3.40282347e38f);
         }
+        int bi = -1;
 #pragma unroll
-        for (int d = 4; d < 8; ++d) {  // diagonal bands: 9 members
-            const double m = div_const<9>(sum[d]);
-            rl_pick(m, div_const<9>(sq[d]) - m * m, &best_cv, &best_mean, &best_var);
+        for (int d = 0; d < 8; ++d)
+            if (okv[d] && cva[d] < best_cv) {
+                best_cv = cva[d];
+                best_mean = mfv[d];
+                best_var = vfv[d];
+                bi = d;
+            }
+        const float lim = best_cv * (1.0f + 0x1p-19f);
+        bool close = special;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) close |= okv[d] && d != bi && !(cva[d] > lim);
+        if (close) {
+            best_cv = __builtin_inff();
+            best_mean = centre;
+            best_var = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+                if (okv[d]) {
+                    const float cv = vfv[d] / (mfv[d] * mfv[d]);
+                    if (cv < best_cv) {
+                        best_cv = cv;
+                        best_mean = mfv[d];
+                        best_var = vfv[d];
+                    }
+                }
         }
         return rl_update(best_cv, best_mean, best_var, centre, cv_n_sq);
     }
