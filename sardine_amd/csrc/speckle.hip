@@ -261,9 +261,9 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
         // a reciprocal (within 2.4e-7 relative of the correctly rounded
         // quotient).  These decide the pick unless another candidate is
         // within 2^-19 relative of the smallest (more than twice the error
-        // of both), or a mean² is zero, subnormal or infinite; then every
-        // CV² is recomputed by exact division (rl_pick), which resolves
-        // ties to the first direction as the reference does.
+        // of both), or a mean², its reciprocal or a CV² is not a normal
+        // number; then every CV² is recomputed by exact division, which
+        // resolves ties to the first direction as the reference does.
         float mfv[8], vfv[8], cva[8];
         bool okv[8], special = false;
 #pragma unroll
@@ -276,7 +276,9 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
             vfv[d] = vf;
             okv[d] = mf > 0.0f;
             cva[d] = vf * __builtin_amdgcn_rcpf(m2);
-            special |= okv[d] && !(m2 >= 1.17549435e-38f && cva[d] <= 3.40282347e38f);
+            // the reciprocal and the quotient must both be normal numbers
+            special |= okv[d] && !(m2 >= 0x1p-126f && m2 <= 0x1p126f &&
+                                   (cva[d] == 0.0f || (cva[d] >= 0x1p-126f && cva[d] <= 3.40282347e38f)));
         }
         int bi = -1;
 #pragma unroll

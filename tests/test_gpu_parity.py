@@ -216,6 +216,38 @@ def test_refined_lee_dense_wide_range(cuda, enl):
     assert np.array_equal(out, ref)
 
 
+def test_refined_lee_pick_edge_cases(cuda):
+    """The direction pick's fast path (reciprocal CV²) must hand every case it
+    cannot decide to exact division: constant patches (all CV² = 0, ties
+    resolved to the first direction), mirror-symmetric patches (equal
+    non-zero CV² in pairs of directions), means whose square underflows or
+    is subnormal, means whose square overflows or whose reciprocal would be
+    subnormal; bit-exact against the oracle."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    rows, cols = 160, 192
+    x = rng.exponential(1.0, (rows, cols)).astype(np.float32)
+    x[0:32, 0:32] = 2.5                                   # constant
+    sym = rng.exponential(1.0, (16, 32)).astype(np.float32)
+    x[32:64, 0:32] = np.concatenate([sym, sym[::-1]])     # mirrored rows: N/S ties
+    x[64:96, 0:32] = np.concatenate([sym.T[:, :16], sym.T[:, :16][:, ::-1]], axis=1)  # mirrored columns: E/W ties
+    x[96:128, 0:32] = 1e-22 * rng.exponential(1.0, (32, 32))   # mean² underflows
+    x[128:160, 0:32] = 1e-20 * rng.exponential(1.0, (32, 32))  # mean² subnormal
+    x[0:32, 32:64] = 1e19 * rng.exponential(1.0, (32, 32))     # mean² above 2^126
+    x[32:64, 32:64] = 3e19 * rng.exponential(1.0, (32, 32))    # mean² overflows
+    x[64:96, 32:64] = 0.0                                      # all-zero windows
+    x[96:128, 32:64] = np.where(rng.random((32, 32)) < 0.5, 0.0, 1.0)
+    x[128:160, 32:64] = 7.0
+    x[130:140, 40:50] = 7.0000005                             # near ties
+    for enl in (1.0, 4.0):
+        st, ref = O.speckle(x, abi.SPECKLE_REFINED_LEE, 7, enl, 1.0)
+        assert st == 0
+        out = api.SpeckleFilter(abi.SPECKLE_REFINED_LEE, 7, enl, 1.0).apply(
+            torch.from_numpy(x).to(cuda), cols, rows).cpu().numpy()
+        assert np.array_equal(out, ref, equal_nan=True), np.argwhere(~((out == ref) | (np.isnan(out) & np.isnan(ref))))[:5]
+
+
 def test_to_db(cuda):
     import torch
 
