@@ -261,9 +261,11 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
         // a reciprocal (within 2.4e-7 relative of the correctly rounded
         // quotient).  These decide the pick unless another candidate is
         // within 2^-19 relative of the smallest (more than twice the error
-        // of both), or a mean², its reciprocal or a CV² is not a normal
-        // number; then every CV² is recomputed by exact division, which
-        // resolves ties to the first direction as the reference does.
+        // of both), or a mean² or its reciprocal is not a normal number;
+        // then every CV² is recomputed by exact division, which resolves
+        // ties to the first direction as the reference does.  (A quotient
+        // below 2^-126 needs var < mean²·2^-126, and var is either 0 or at
+        // least the ~1e-16·mean² rounding residue of sq/n - mean².)
         float mfv[8], vfv[8], cva[8];
         bool okv[8], special = false;
 #pragma unroll
@@ -276,9 +278,10 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
             vfv[d] = vf;
             okv[d] = mf > 0.0f;
             cva[d] = vf * __builtin_amdgcn_rcpf(m2);
-            // the reciprocal and the quotient must both be normal numbers
-            special |= okv[d] && !(m2 >= 0x1p-126f && m2 <= 0x1p126f &&
-                                   (cva[d] == 0.0f || (cva[d] >= 0x1p-126f && cva[d] <= 3.40282347e38f)));
+            // mean² and its reciprocal normal: biased exponent 1..252 (an
+            // infinite or NaN CV² is never picked on either path, and a
+            // NaN one fails the near-tie test below)
+            special |= okv[d] && (__float_as_uint(m2) >> 23) - 1u >= 252u;
         }
         int bi = -1;
 #pragma unroll
