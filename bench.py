@@ -445,8 +445,9 @@ def bench_serve(args, dev):
     (UTM or EPSG:4326), resampler and speckle filter.  The native server
     (HBM-budget admission, batches formed on arrival, two batches in
     flight) against one plain terrain_correction (+ speckle) call per
-    request, FIFO: saturated throughput of both, then latency at the same
-    Poisson arrival rate (80 % of the plain loop's capacity)."""
+    request, FIFO: saturated throughput of both, latency at the same Poisson
+    arrival rate (80 % of the plain loop's capacity), and the highest rate
+    the server sustains within the plain loop's p99 there."""
     import math
     import threading
 
@@ -555,11 +556,32 @@ def bench_serve(args, dev):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n)).tolist()
     t_plain, lat_plain = plain_run(arrivals)
     t_srv, lat_srv, sizes = served_run(arrivals)
-    srv.close()
 
     def pct(v, q):
         v = sorted(v)
         return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+    # throughput at equal p99: the plain loop's p99 at 80 % of its capacity
+    # is the latency bound; the same request sequence is offered to the
+    # server at rising rates (arrival times compressed by f) up to its
+    # saturated capacity, and the highest rate whose p99 stays within the
+    # bound is the served throughput at equal p99
+    slo = pct(lat_plain, .99)
+    sweep, best_f = [], None
+    f = 1.0
+    while f * rate <= 1.05 * n / t_srv_sat:
+        t_f, lat_f, sz_f = served_run([a / f for a in arrivals])
+        p99_f = pct(lat_f, .99)
+        sweep.append({"rate_req_s": round(f * rate, 2), "mpx_s": round(px / t_f / 1e6, 3),
+                      "p50_ms": round(pct(lat_f, .5), 3), "p99_ms": round(p99_f, 3),
+                      "mean_batch": float(np.mean(sz_f))})
+        log(f"served at {f * rate:.0f} req/s ({f:.2f}x): p50 {pct(lat_f, .5):.1f} p99 {p99_f:.1f} ms, "
+            f"{px / t_f / 1e6:.0f} Mpx/s, mean batch {np.mean(sz_f):.1f}")
+        if p99_f > slo:
+            break
+        best_f = f
+        f += 0.25
+    srv.close()
 
     log(f"saturated: plain {px / t_plain_sat / 1e6:.0f} Mpx/s, served {px / t_srv_sat / 1e6:.0f} Mpx/s "
         f"(mean batch {np.mean(sizes_sat):.1f})")
@@ -588,6 +610,9 @@ def bench_serve(args, dev):
                                "p99_ms": round(pct(lat_srv, .99), 3), "mean_batch": float(np.mean(sizes))},
                     "plain": {"mpx_s": round(px / t_plain / 1e6, 3), "p50_ms": round(pct(lat_plain, .5), 3),
                               "p99_ms": round(pct(lat_plain, .99), 3)}},
+        "equal_p99": {"p99_bound_ms": round(slo, 3), "plain_rate_req_s": round(rate, 2),
+                      "served_max_rate_req_s": round(best_f * rate, 2) if best_f else None,
+                      "throughput_ratio": best_f, "served_sweep": sweep},
     }
     print(json.dumps(line), flush=True)
     for s in scenes:
