@@ -50,10 +50,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32"], default="iw_full",
+    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32", "cog"],
+                    default="iw_full",
                     help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
                          "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request; "
-                         "fp32: the FP32 scene-centred variant's errors and speed against FP64")
+                         "fp32: the FP32 scene-centred variant's errors and speed against FP64; "
+                         "cog: the COG export (pyramid + tile pack, and the whole file write) of the "
+                         "full-size dB product")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
@@ -128,6 +131,8 @@ def main():
         return bench_serve(args, dev)
     if args.workload == "fp32":
         return bench_fp32(args, dev)
+    if args.workload == "cog":
+        return bench_cog(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -688,6 +693,123 @@ def bench_fp32(args, dev):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "fp32_iw_full_utm10m_25000x18000", "grid": [rows, cols]},
         "variants": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
+
+def bench_cog(args, dev):
+    """COG export of the full-size geocoded dB product (tile 512, as the
+    reference pipeline writes it, run.rs:433-441): the device pyramid + pack
+    (value, HBM roofline), then the whole file write (device pack, PCIe,
+    file system) against the oracle's writer on the same raster, whose file
+    must be identical byte for byte."""
+    import hashlib
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from sardine_amd import _native, abi, api, synthetic
+    from sardine_amd.workload import DeviceScene
+
+    L = _native.lib()
+    stream = torch.cuda.current_stream(dev)
+    t0 = time.time()
+    sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+    ds = DeviceScene.build(sc, dev)
+    cfg = ds.tc_config()
+    img = ds.terrain_correction(ds.prepare_merged(stream=stream), cfg, stream=stream)
+    api.to_db_inplace(img.data, 0.0, stream=stream)
+    rows, cols, gt = img.rows, img.cols, list(img.geotransform)
+    data = img.data.reshape(rows, cols).contiguous()
+    del img
+    torch.cuda.synchronize()
+    log(f"dB product {rows}x{cols} ready in {time.time() - t0:.1f}s")
+    ts = 512
+    lay = api.cog_layout(cols, rows, ts, cfg.crs)
+    tiles = torch.empty(lay.tile_bytes // 4, dtype=torch.float32, device=dev)
+    # algorithmic bytes of one pack: the full raster read once, every tile
+    # written once, and every overview but the coarsest read once (by the
+    # launch that makes the next level)
+    lvl_bytes = [lay.level[k].tiles_x * lay.level[k].tiles_y * ts * ts * 4 for k in range(lay.n_levels)]
+    alg_bytes = rows * cols * 4 + lay.tile_bytes + sum(lvl_bytes[1:lay.n_levels - 1])
+    L.sardine_set_profiling(1)
+    for _ in range(max(1, args.warmup)):
+        api.cog_pack(data, lay, tiles=tiles, stream=stream)
+    steps = max(args.steps, 1)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    settle_gc()
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(steps):
+        api.cog_pack(data, lay, tiles=tiles, stream=stream)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    k_ms = recent_kernel_ms(L, abi.KERNEL_COG_PACK, steps)
+    L.sardine_set_profiling(0)
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    pack_ms = float(np.mean(k_ms))
+    gbs = alg_bytes / (pack_ms / 1e3) / 1e9
+    log(f"pack: {['%.3f' % x for x in k_ms]} ms per pack ({lay.n_levels} levels, {lay.tile_bytes / 1e9:.2f} GB "
+        f"of tiles), {gbs:.0f} GB/s algorithmic")
+    del tiles
+    traffic, traffic_src = measured_traffic("cog_level_kernel", "cog_iw_full_25000x18000_ts512")
+
+    # the whole file, device path vs the oracle writer (one thread)
+    tmp = pathlib.Path("/dev/shm") if pathlib.Path("/dev/shm").is_dir() else pathlib.Path("/tmp")
+    gpu_path, ref_path = tmp / f"sardine_bench_gpu_{os.getpid()}.tif", tmp / f"sardine_bench_ref_{os.getpid()}.tif"
+    try:
+        api.write_cog_with_crs(gpu_path, data, cols, rows, gt, crs=cfg.crs, utm_zone=cfg.utm_zone, tile_size=ts)
+        t = time.perf_counter()
+        api.write_cog_with_crs(gpu_path, data, cols, rows, gt, crs=cfg.crs, utm_zone=cfg.utm_zone, tile_size=ts)
+        gpu_write_s = time.perf_counter() - t
+        host = data.cpu().numpy()
+        t = time.perf_counter()
+        st = O.write_cog(ref_path, host, gt, crs=cfg.crs, utm_zone=cfg.utm_zone, tile_size=ts)
+        cpu_write_s = time.perf_counter() - t
+        assert st == 0, st
+
+        def digest(p):
+            h = hashlib.sha256()
+            with open(p, "rb") as f:
+                while True:
+                    b = f.read(64 << 20)
+                    if not b:
+                        break
+                    h.update(b)
+            return h.hexdigest()
+
+        same = gpu_path.stat().st_size == ref_path.stat().st_size and digest(gpu_path) == digest(ref_path)
+    finally:
+        for p in (gpu_path, ref_path):
+            if p.exists():
+                p.unlink()
+    fb = lay.file_bytes
+    log(f"file {fb / 1e9:.2f} GB: device path {gpu_write_s:.2f}s, oracle writer {cpu_write_s:.2f}s, "
+        f"identical={same}")
+    px = rows * cols
+    line = {
+        "metric": "COG export Mpx/s (overview pyramid + tile pack, inputs in HBM)",
+        "value": round(px / (pack_ms / 1e3) / 1e6, 1),
+        "unit": "Mpx/s",
+        "n_gpus": 1, "steps": steps, "warmup": max(1, args.warmup), "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cog_iw_full_25000x18000_ts512", "raster": [rows, cols], "tile_size": ts,
+                   "levels": lay.n_levels, "bigtiff": bool(lay.bigtiff), "file_bytes": fb},
+        "roofline": {"kernel": "cog_level_kernel (all launches of one pack)", "bound": "hbm",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": alg_bytes, "launch_ms": round(pack_ms, 4)},
+        "write_file": {"device_path_s": round(gpu_write_s, 3), "GBps": round(fb / gpu_write_s / 1e9, 2),
+                       "identical_to_oracle": bool(same), "fs": str(tmp)},
+        "cpu_baseline": {"value": round(px / cpu_write_s / 1e6, 1), "unit": "Mpx/s (whole file write)",
+                         "cores": 1, "kind": "port",
+                         "sample": "the oracle's COG writer (restatement of export.rs:329-760) on the whole "
+                                   "raster, file included; compare with write_file",
+                         "seconds": round(cpu_write_s, 3),
+                         "device_path_speedup": round(cpu_write_s / gpu_write_s, 2)},
     }
     print(json.dumps(line), flush=True)
 

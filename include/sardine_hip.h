@@ -75,6 +75,10 @@ typedef enum sardine_status {
     SARDINE_ERR_MERGE_EMPTY_INPUTS = 50,
     SARDINE_ERR_MERGE_PIXEL_SPACING = 51,
     SARDINE_ERR_MERGE_EMPTY_SWATH = 52,
+    /* ExportError (export.rs) */
+    SARDINE_ERR_EXPORT_INVALID_GEOTRANSFORM = 60,
+    SARDINE_ERR_EXPORT_INVALID_TILE_SIZE = 61,
+    SARDINE_ERR_EXPORT_IO = 62,
     /* library / device */
     SARDINE_ERR_INVALID_ARGUMENT = 90,
     SARDINE_ERR_NO_DEVICE = 91,
@@ -431,6 +435,69 @@ sardine_status sardine_apply_calibration(const int16_t* deburst /*[device] lines
 sardine_status sardine_to_db_inplace(float* data /*[device]*/, int64_t n, float noise_floor_linear,
                                      uint64_t counts[2], void* stream);
 
+/* ── Cloud-Optimised GeoTIFF export (export.rs:329-760) ───────────────────
+ * The file is the reference's byte for byte: a classic TIFF (BigTIFF when
+ * rows*cols*4 exceeds 2^32-1), one 17-entry IFD per level with the coarsest
+ * overview first and the full resolution last, then the pixel-scale,
+ * tiepoint and GeoKey blocks, then every level's tiles (row-major tile
+ * order, each tile tile_size^2 float32 values, padded with NaN).  Overviews
+ * are 2x2 means of the non-NaN pixels (downsample_2x, export.rs:362-392),
+ * halved until both sides are <= tile_size; there is always at least one.
+ * The pyramid and the tile packing run on the device; the host writes the
+ * header it computes and streams the packed tiles to the file. */
+#define SARDINE_COG_MAX_LEVELS 32
+
+typedef struct sardine_cog_level {
+    int64_t cols, rows;            /* raster size of the level */
+    int64_t tiles_x, tiles_y;      /* ceil(cols / tile_size), ceil(rows / tile_size) */
+    uint64_t ifd_offset;           /* file offset of the level's IFD */
+    uint64_t tile_offsets_offset;  /* file offset of its TileOffsets array */
+    uint64_t tile_counts_offset;   /* file offset of its TileByteCounts array */
+    uint64_t tile_data_offset;     /* file offset of its first tile */
+} sardine_cog_level;
+
+typedef struct sardine_cog_layout {
+    int32_t n_levels;              /* overviews + 1; level[n_levels-1] is the full raster */
+    int32_t bigtiff;               /* 1: BigTIFF, 0: classic TIFF */
+    uint32_t tile_size;
+    int32_t geo_key_shorts;        /* 16 (WGS84) or 20 (UTM) */
+    uint64_t header_bytes;         /* bytes before the first tile (= level[0].tile_data_offset) */
+    uint64_t tile_bytes;           /* bytes of all tiles of all levels */
+    uint64_t file_bytes;           /* header_bytes + tile_bytes */
+    uint64_t pixel_scale_offset, tiepoint_offset, geo_keys_offset;
+    sardine_cog_level level[SARDINE_COG_MAX_LEVELS];
+} sardine_cog_layout;
+
+/* The pyramid, the TIFF flavour and every offset of write_cog_raw_inner
+ * (export.rs:430-535) for a cols x rows raster.  Host only.  Errors:
+ * SARDINE_ERR_EXPORT_INVALID_TILE_SIZE (not a power of two in [64, 4096]),
+ * SARDINE_ERR_INVALID_ARGUMENT (negative size, unknown CRS). */
+sardine_status sardine_cog_layout_compute(int64_t cols, int64_t rows, uint32_t tile_size, int32_t crs,
+                                          sardine_cog_layout* layout);
+
+/* The header region of the file (header, IFDs, tile arrays, GeoTIFF blocks;
+ * export.rs:537-670): layout->header_bytes bytes into `out`.  Host only.
+ * crs is a sardine_crs_kind, utm_zone 1..60 for UTM (OutputCrs::epsg and
+ * geo_key_directory, output_crs.rs:167-175,265-287).  Errors:
+ * SARDINE_ERR_EXPORT_INVALID_GEOTRANSFORM (pixel width or -gt[5] not finite
+ * and > 0), SARDINE_ERR_INVALID_ARGUMENT (capacity, CRS, zone). */
+sardine_status sardine_cog_header(const sardine_cog_layout* layout, const double geotransform[6], int32_t crs,
+                                  int32_t utm_zone, uint8_t* out, uint64_t capacity);
+
+/* The tile region of the file (export.rs:672-712, with downsample_2x for
+ * every overview): layout->tile_bytes bytes at `tiles` [device, 16-byte
+ * aligned], from the full raster `data` [device, layout's cols x rows,
+ * row-major].  Enqueued on `stream`; returns without waiting. */
+sardine_status sardine_cog_pack(const float* data, const sardine_cog_layout* layout, float* tiles, void* stream);
+
+/* write_cog_with_crs (export.rs:329-354): validation in the reference's order
+ * (geotransform, then tile size), the layout, the device pack, and the file
+ * at `path`.  Returns after the file is closed.  I/O failures return
+ * SARDINE_ERR_EXPORT_IO. */
+sardine_status sardine_write_cog_with_crs(const char* path, const float* data /*[device]*/, int64_t cols,
+                                          int64_t rows, const double geotransform[6], int32_t crs,
+                                          int32_t utm_zone, uint32_t tile_size, void* stream);
+
 /* ── Kernel timing (diagnostic, no reference counterpart) ─────────────────
  * With profiling on, every compute entry point brackets its main kernel with
  * HIP events on the stream it launches on (without waiting); the duration of
@@ -440,7 +507,8 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_CALIBRATE = 1, /* fused deburst + calibration + merge kernel */
     SARDINE_KERNEL_SPECKLE = 2,
     SARDINE_KERNEL_TO_DB = 3,
-    SARDINE_KERNEL_TC_SEGMENT = 4 /* the per-pixel chain kernel of terrain correction alone */
+    SARDINE_KERNEL_TC_SEGMENT = 4, /* the per-pixel chain kernel of terrain correction alone */
+    SARDINE_KERNEL_COG_PACK = 5    /* the COG export's pyramid + tile pack (all its launches) */
 } sardine_kernel_family;
 /* Switches the kernel timers of the calling thread on (creating their
  * events) or off. */

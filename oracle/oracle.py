@@ -73,6 +73,9 @@ def lib():
         L.oracle_merge.argtypes = [i32, P, P, P, P, P, P, P, P, P, P, C.POINTER(abi.MergedInfo), i32]
         L.oracle_speckle.argtypes = [P, i64, i64, i64, C.POINTER(abi.SpeckleFilterC), P, i32]
         L.oracle_to_db_inplace.argtypes = [P, i64, f, P]
+        L.oracle_downsample_2x.argtypes = [P, i64, i64, P]
+        L.oracle_write_cog.argtypes = [C.c_char_p, P, i64, i64, P, i32, i32, i32]
+        L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -299,6 +302,38 @@ def to_db_inplace(data: np.ndarray, floor_lin: float):
     counts = np.zeros(2, np.uint64)
     st = lib().oracle_to_db_inplace(_ptr(data), data.size, floor_lin, _ptr(counts))
     return st, int(counts[0]), int(counts[1])
+
+
+def downsample_2x(src: np.ndarray) -> np.ndarray:
+    """export.rs:362-392 on a (rows, cols) float32 array."""
+    src = np.ascontiguousarray(src, np.float32)
+    rows, cols = src.shape
+    out = np.empty(max(((rows + 1) // 2) * ((cols + 1) // 2), 1), np.float32)
+    lib().oracle_downsample_2x(_ptr(src), cols, rows, _ptr(out))
+    return out[: ((rows + 1) // 2) * ((cols + 1) // 2)].reshape((rows + 1) // 2, (cols + 1) // 2)
+
+
+def write_cog(path, data: np.ndarray, geotransform, crs: int = 0, utm_zone: int = 0, tile_size: int = 512,
+              cols: int = None, rows: int = None) -> int:
+    """write_cog_with_crs (export.rs:329-760); returns the status."""
+    data = np.ascontiguousarray(data, np.float32)
+    if rows is None:
+        rows, cols = data.shape
+    gt = np.ascontiguousarray(geotransform, np.float64)
+    return lib().oracle_write_cog(str(path).encode(), _ptr(data) if data.size else None, cols, rows, _ptr(gt), crs,
+                                  utm_zone, tile_size)
+
+
+def cog_header(cols: int, rows: int, geotransform, crs: int = 0, utm_zone: int = 0, tile_size: int = 512):
+    """(status, bytes before the first tile) of write_cog_with_crs's file."""
+    gt = np.ascontiguousarray(geotransform, np.float64)
+    n = C.c_uint64()
+    st = lib().oracle_cog_header(cols, rows, _ptr(gt), crs, utm_zone, tile_size, None, 0, C.byref(n))
+    if st:
+        return st, None
+    out = np.empty(n.value, np.uint8)
+    st = lib().oracle_cog_header(cols, rows, _ptr(gt), crs, utm_zone, tile_size, _ptr(out), n.value, C.byref(n))
+    return st, out.tobytes()
 
 
 # ── the full chain on a synthetic scene (host) ──────────────────────────

@@ -24,6 +24,9 @@
 //   deburst / merge layout     deburst.rs:467-503, merge_subswaths.rs:500-725
 //   speckle                    speckle.rs:621-1000 known values
 //   to_db                      export.rs:1227-1275 known values
+//   downsample_2x / COG writer export.rs:1583-1733 (tile-size and geotransform
+//                              errors, header and 17-entry IFDs, tile data
+//                              round trip, 2x2 means and NaN rule)
 //   UTM                        published Krüger-series (Karney 2011) values,
 //                              forward/inverse round trip (proj4rs is not
 //                              vendored: parity unpinned below ±1 m)
@@ -1274,6 +1277,279 @@ int oracle_to_db_inplace(float* data, int64_t n, float floor_lin, uint64_t count
     counts[0] = conv;
     counts[1] = masked;
     return 0;
+}
+
+}  // extern "C"
+
+// ── COG export (export.rs:329-760) ─────────────────────────────────────────
+namespace {
+
+// downsample_2x (export.rs:362-392): 2x2 means of the non-NaN pixels in f64,
+// NaN where all four (inside the raster) are NaN.
+void downsample_2x(const float* src, int64_t sc, int64_t sr, std::vector<float>& dst, int64_t* dc_out,
+                   int64_t* dr_out) {
+    const int64_t dc = (sc + 1) / 2, dr = (sr + 1) / 2;
+    dst.assign((size_t)(dc * dr), std::numeric_limits<float>::quiet_NaN());
+    for (int64_t r = 0; r < dr; ++r)
+        for (int64_t c = 0; c < dc; ++c) {
+            double sum = 0.0;
+            uint32_t count = 0;
+            for (int64_t a = 0; a < 2; ++a)
+                for (int64_t b = 0; b < 2; ++b) {
+                    const int64_t y = 2 * r + a, x = 2 * c + b;
+                    if (y < sr && x < sc) {
+                        const float v = src[y * sc + x];
+                        if (!std::isnan(v)) {
+                            sum += (double)v;
+                            ++count;
+                        }
+                    }
+                }
+            if (count > 0) dst[(size_t)(r * dc + c)] = (float)(sum / (double)count);
+        }
+    *dc_out = dc;
+    *dr_out = dr;
+}
+
+// OutputCrs::geo_key_directory (output_crs.rs:265-287) for WGS84 / UTM.
+std::vector<uint16_t> geo_key_directory(int crs, int zone) {
+    if (crs == SARDINE_CRS_WGS84) return {1, 1, 0, 3, 1024, 0, 1, 2, 1025, 0, 1, 1, 2048, 0, 1, 4326};
+    const uint16_t epsg = (uint16_t)((crs == SARDINE_CRS_UTM_SOUTH ? 32700 : 32600) + zone);
+    return {1, 1, 0, 4, 1024, 0, 1, 1, 1025, 0, 1, 1, 3072, 0, 1, epsg, 3076, 0, 1, 9001};
+}
+
+template <typename T>
+void put(std::vector<uint8_t>& b, T v) {
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+}
+
+// write_cog_raw_inner's pyramid sizes, format choice, offsets and the bytes
+// before the first tile (export.rs:430-670).  levels: coarsest overview
+// first, full raster last.
+struct CogPlan {
+    std::vector<std::pair<int64_t, int64_t>> levels;
+    std::vector<uint64_t> tile_data;  // first tile's offset per level
+    std::vector<uint8_t> head;
+};
+
+int cog_plan(int64_t cols, int64_t rows, const double gt[6], int crs, int zone, int32_t tile_size, CogPlan* P) {
+    const double pw = gt[1], ph = -gt[5];
+    if (!std::isfinite(pw) || pw <= 0.0 || !std::isfinite(ph) || ph <= 0.0) return SARDINE_ERR_EXPORT_INVALID_GEOTRANSFORM;
+    if (tile_size < 64 || tile_size > 4096 || (tile_size & (tile_size - 1)) != 0) return SARDINE_ERR_EXPORT_INVALID_TILE_SIZE;
+    const std::vector<uint16_t> keys = geo_key_directory(crs, zone);
+    const int64_t ts = tile_size;
+    std::vector<std::pair<int64_t, int64_t>> ov;
+    int64_t c = (cols + 1) / 2, r = (rows + 1) / 2;
+    while (c > ts || r > ts) {
+        ov.push_back({c, r});
+        c = (c + 1) / 2;
+        r = (r + 1) / 2;
+    }
+    ov.push_back({c, r});
+    std::reverse(ov.begin(), ov.end());
+    P->levels = ov;
+    P->levels.push_back({cols, rows});
+    const bool big = (unsigned __int128)rows * (unsigned __int128)cols * 4 > 0xffffffffull;
+    const uint64_t n_entries = 17, geo_bytes = keys.size() * 2;
+    const uint64_t ifd_fixed = big ? 8 + n_entries * 20 + 8 : 2 + n_entries * 12 + 4;
+    const uint64_t arr_entry = big ? 8 : 4, header = big ? 16 : 8;
+    auto ntiles = [&](int64_t cc, int64_t rr) { return ((cc + ts - 1) / ts) * ((rr + ts - 1) / ts); };
+    uint64_t off = header;
+    std::vector<uint64_t> ifd_off, toff_arr, tbc_arr;
+    for (auto& l : P->levels) {
+        ifd_off.push_back(off);
+        off += ifd_fixed;
+        const uint64_t n = (uint64_t)ntiles(l.first, l.second);
+        toff_arr.push_back(off);
+        off += n * arr_entry;
+        tbc_arr.push_back(off);
+        off += n * arr_entry;
+    }
+    const uint64_t scale_off = off;
+    off += 24;
+    const uint64_t tie_off = off;
+    off += 48;
+    const uint64_t key_off = off;
+    off += geo_bytes;
+    const uint64_t tile_bytes = (uint64_t)ts * (uint64_t)ts * 4;
+    std::vector<std::vector<uint64_t>> tile_off;
+    for (auto& l : P->levels) {
+        std::vector<uint64_t> t;
+        P->tile_data.push_back(off);
+        for (int64_t i = 0; i < ntiles(l.first, l.second); ++i) {
+            t.push_back(off);
+            off += tile_bytes;
+        }
+        tile_off.push_back(t);
+    }
+    std::vector<uint8_t>& b = P->head;
+    b.clear();
+    b.push_back('I');
+    b.push_back('I');
+    if (big) {
+        put<uint16_t>(b, 43);
+        put<uint16_t>(b, 8);
+        put<uint16_t>(b, 0);
+        put<uint64_t>(b, ifd_off[0]);
+    } else {
+        put<uint16_t>(b, 42);
+        put<uint32_t>(b, (uint32_t)ifd_off[0]);
+    }
+    for (size_t k = 0; k < P->levels.size(); ++k) {
+        const int64_t lc = P->levels[k].first, lr = P->levels[k].second;
+        const uint64_t n = (uint64_t)ntiles(lc, lr);
+        const uint64_t next = k + 1 < P->levels.size() ? ifd_off[k + 1] : 0;
+        if (big) {
+            put<uint64_t>(b, n_entries);
+            auto e = [&](uint16_t tag, uint16_t typ, uint64_t cnt, uint64_t val) {
+                put<uint16_t>(b, tag);
+                put<uint16_t>(b, typ);
+                put<uint64_t>(b, cnt);
+                put<uint64_t>(b, val);
+            };
+            const uint64_t rat = (1ull << 32) | 72ull;
+            e(256, 4, 1, (uint64_t)lc);
+            e(257, 4, 1, (uint64_t)lr);
+            e(258, 3, 1, 32);
+            e(259, 3, 1, 1);
+            e(262, 3, 1, 1);
+            e(277, 3, 1, 1);
+            e(282, 5, 1, rat);
+            e(283, 5, 1, rat);
+            e(296, 3, 1, 1);
+            e(322, 4, 1, (uint64_t)ts);
+            e(323, 4, 1, (uint64_t)ts);
+            e(324, 16, n, toff_arr[k]);
+            e(325, 16, n, tbc_arr[k]);
+            e(339, 3, 1, 3);
+            e(33550, 12, 3, scale_off);
+            e(33922, 12, 6, tie_off);
+            e(34735, 3, keys.size(), key_off);
+            put<uint64_t>(b, next);
+        } else {
+            put<uint16_t>(b, (uint16_t)n_entries);
+            auto e = [&](uint16_t tag, uint16_t typ, uint32_t cnt, uint32_t val) {
+                put<uint16_t>(b, tag);
+                put<uint16_t>(b, typ);
+                put<uint32_t>(b, cnt);
+                put<uint32_t>(b, val);
+            };
+            e(256, 4, 1, (uint32_t)lc);
+            e(257, 4, 1, (uint32_t)lr);
+            e(258, 3, 1, 32);
+            e(259, 3, 1, 1);
+            e(262, 3, 1, 1);
+            e(277, 3, 1, 1);
+            e(282, 4, 1, 72);
+            e(283, 4, 1, 72);
+            e(296, 3, 1, 1);
+            e(322, 4, 1, (uint32_t)ts);
+            e(323, 4, 1, (uint32_t)ts);
+            e(324, 4, (uint32_t)n, (uint32_t)toff_arr[k]);
+            e(325, 4, (uint32_t)n, (uint32_t)tbc_arr[k]);
+            e(339, 3, 1, 3);
+            e(33550, 12, 3, (uint32_t)scale_off);
+            e(33922, 12, 6, (uint32_t)tie_off);
+            e(34735, 3, (uint32_t)keys.size(), (uint32_t)key_off);
+            put<uint32_t>(b, (uint32_t)next);
+        }
+        for (uint64_t o : tile_off[k]) {
+            if (big) put<uint64_t>(b, o);
+            else put<uint32_t>(b, (uint32_t)o);
+        }
+        for (uint64_t i = 0; i < n; ++i) {
+            if (big) put<uint64_t>(b, tile_bytes);
+            else put<uint32_t>(b, (uint32_t)tile_bytes);
+        }
+    }
+    put<double>(b, gt[1]);
+    put<double>(b, -gt[5]);
+    put<double>(b, 0.0);
+    put<double>(b, 0.0);
+    put<double>(b, 0.0);
+    put<double>(b, 0.0);
+    put<double>(b, gt[0]);
+    put<double>(b, gt[3]);
+    put<double>(b, 0.0);
+    for (uint16_t k : keys) put<uint16_t>(b, k);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_downsample_2x(const float* src, int64_t sc, int64_t sr, float* dst) {
+    std::vector<float> d;
+    int64_t dc, dr;
+    downsample_2x(src, sc, sr, d, &dc, &dr);
+    std::memcpy(dst, d.data(), d.size() * sizeof(float));
+    return 0;
+}
+
+// The bytes before the first tile (header, IFDs, tile arrays, GeoTIFF
+// blocks) of write_cog_with_crs's file; *size = their count (written to
+// `out` when it holds them).
+int oracle_cog_header(int64_t cols, int64_t rows, const double gt[6], int crs, int zone, int32_t tile_size,
+                      uint8_t* out, uint64_t cap, uint64_t* size) {
+    CogPlan P;
+    const int st = cog_plan(cols, rows, gt, crs, zone, tile_size, &P);
+    if (st) return st;
+    *size = P.head.size();
+    if (out && cap >= P.head.size()) std::memcpy(out, P.head.data(), P.head.size());
+    return 0;
+}
+
+// write_cog_with_crs + write_cog_raw_inner (export.rs:329-760): validation,
+// the overview pyramid (coarsest first), the header and every level's
+// NaN-padded tiles, written to `path`.
+int oracle_write_cog(const char* path, const float* data, int64_t cols, int64_t rows, const double gt[6], int crs,
+                     int zone, int32_t tile_size) {
+    CogPlan P;
+    const int st = cog_plan(cols, rows, gt, crs, zone, tile_size, &P);
+    if (st) return st;
+    // overview rasters, finest first as the reference builds them
+    std::vector<std::vector<float>> ov;
+    {
+        std::vector<float> d;
+        int64_t c, r;
+        downsample_2x(data, cols, rows, d, &c, &r);
+        for (size_t k = 0; k + 2 < P.levels.size(); ++k) {
+            std::vector<float> n;
+            int64_t nc, nr;
+            downsample_2x(d.data(), c, r, n, &nc, &nr);
+            ov.push_back(std::move(d));
+            d.swap(n);
+            c = nc;
+            r = nr;
+        }
+        ov.push_back(std::move(d));
+    }
+    std::reverse(ov.begin(), ov.end());
+    std::vector<const float*> srcs;
+    for (auto& v : ov) srcs.push_back(v.data());
+    srcs.push_back(data);
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return SARDINE_ERR_EXPORT_IO;
+    bool ok = std::fwrite(P.head.data(), 1, P.head.size(), f) == P.head.size();
+    const int64_t ts = tile_size;
+    std::vector<float> tile((size_t)(ts * ts));
+    for (size_t k = 0; k < P.levels.size() && ok; ++k) {
+        const int64_t c = P.levels[k].first, r = P.levels[k].second;
+        const float* src = srcs[k];
+        const int64_t nx = (c + ts - 1) / ts, ny = (r + ts - 1) / ts;
+        for (int64_t ty = 0; ty < ny && ok; ++ty)
+            for (int64_t tx = 0; tx < nx && ok; ++tx) {
+                std::fill(tile.begin(), tile.end(), std::numeric_limits<float>::quiet_NaN());
+                const int64_t r0 = ty * ts, c0 = tx * ts, r1 = std::min(r0 + ts, r), c1 = std::min(c0 + ts, c);
+                for (int64_t y = r0; y < r1; ++y)
+                    std::memcpy(&tile[(size_t)((y - r0) * ts)], src + y * c + c0, (size_t)(c1 - c0) * sizeof(float));
+                ok = std::fwrite(tile.data(), sizeof(float), tile.size(), f) == tile.size();
+            }
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : SARDINE_ERR_EXPORT_IO;
 }
 
 }  // extern "C"

@@ -7,15 +7,17 @@ package mirrors the reference's Rust interface for the path on top of it.
 from . import abi
 from .api import (DemRaster, GeocodedImage, MergedSigma0, ResamplingKernel, SceneInputs, Sigma0Array,
                   SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_speckle_filter,
-                  apply_speckle_step, deburst_layout, merged_layout, output_grid, prepare_merged_scene,
-                  synth_cint16, terrain_correction, to_db_inplace)
-from .errors import (CalibrationError, DeburstError, DeviceError, MergeError, SardineError, SpeckleError,
-                     TerrainCorrectionError)
+                  apply_speckle_step, cog_header, cog_layout, cog_pack, deburst_layout, merged_layout,
+                  output_grid, prepare_merged_scene, synth_cint16, terrain_correction, to_db_inplace,
+                  write_cog_with_crs)
+from .errors import (CalibrationError, DeburstError, DeviceError, ExportError, MergeError, SardineError,
+                     SpeckleError, TerrainCorrectionError)
 
 __all__ = [
     "abi", "DemRaster", "GeocodedImage", "MergedSigma0", "ResamplingKernel", "SceneInputs", "Sigma0Array",
     "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_speckle_filter", "apply_speckle_step",
-    "deburst_layout", "merged_layout", "output_grid", "prepare_merged_scene", "synth_cint16", "terrain_correction",
-    "to_db_inplace", "CalibrationError", "DeburstError", "DeviceError", "MergeError", "SardineError",
+    "cog_header", "cog_layout", "cog_pack", "deburst_layout", "merged_layout", "output_grid",
+    "prepare_merged_scene", "synth_cint16", "terrain_correction", "to_db_inplace", "write_cog_with_crs",
+    "CalibrationError", "DeburstError", "DeviceError", "ExportError", "MergeError", "SardineError",
     "SpeckleError", "TerrainCorrectionError",
 ]

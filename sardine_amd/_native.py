@@ -53,6 +53,10 @@ def lib():
                                             C.POINTER(abi.AzLut), i64, P, P, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), P]
     L.sardine_to_db_inplace.argtypes = [P, i64, f, P, P]
+    L.sardine_cog_layout_compute.argtypes = [i64, i64, C.c_uint32, i32, C.POINTER(abi.CogLayout)]
+    L.sardine_cog_header.argtypes = [C.POINTER(abi.CogLayout), P, i32, i32, P, u64]
+    L.sardine_cog_pack.argtypes = [P, C.POINTER(abi.CogLayout), P, P]
+    L.sardine_write_cog_with_crs.argtypes = [C.c_char_p, P, i64, i64, P, i32, i32, C.c_uint32, P]
     L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]
     L.sardine_set_profiling.argtypes = [C.c_int]
     L.sardine_set_profiling.restype = None

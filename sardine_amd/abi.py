@@ -12,6 +12,7 @@ ABI_VERSION = 1
 NEWTON_HIST_SIZE = 16
 MAX_BURSTS = 64
 MAX_SWATHS = 8
+COG_MAX_LEVELS = 32
 
 # sardine_status
 OK = 0
@@ -38,6 +39,9 @@ ERR_DEBURST_IMPLAUSIBLE_OVERLAP = 45
 ERR_MERGE_EMPTY_INPUTS = 50
 ERR_MERGE_PIXEL_SPACING = 51
 ERR_MERGE_EMPTY_SWATH = 52
+ERR_EXPORT_INVALID_GEOTRANSFORM = 60
+ERR_EXPORT_INVALID_TILE_SIZE = 61
+ERR_EXPORT_IO = 62
 ERR_INVALID_ARGUMENT = 90
 ERR_NO_DEVICE = 91
 ERR_HIP = 92
@@ -201,6 +205,19 @@ class MergedInfo(C.Structure):
                 ("swath_valid_offset", C.c_int64 * MAX_SWATHS)]
 
 
+class CogLevel(C.Structure):
+    _fields_ = [("cols", C.c_int64), ("rows", C.c_int64), ("tiles_x", C.c_int64), ("tiles_y", C.c_int64),
+                ("ifd_offset", C.c_uint64), ("tile_offsets_offset", C.c_uint64),
+                ("tile_counts_offset", C.c_uint64), ("tile_data_offset", C.c_uint64)]
+
+
+class CogLayout(C.Structure):
+    _fields_ = [("n_levels", C.c_int32), ("bigtiff", C.c_int32), ("tile_size", C.c_uint32),
+                ("geo_key_shorts", C.c_int32), ("header_bytes", C.c_uint64), ("tile_bytes", C.c_uint64),
+                ("file_bytes", C.c_uint64), ("pixel_scale_offset", C.c_uint64), ("tiepoint_offset", C.c_uint64),
+                ("geo_keys_offset", C.c_uint64), ("level", CogLevel * COG_MAX_LEVELS)]
+
+
 # Exported symbols of libsardine_hip.so (every function include/sardine_hip.h declares).
 EXPORTED_SYMBOLS = (
     "sardine_last_error",
@@ -216,6 +233,10 @@ EXPORTED_SYMBOLS = (
     "sardine_prepare_merged",
     "sardine_apply_calibration",
     "sardine_to_db_inplace",
+    "sardine_cog_layout_compute",
+    "sardine_cog_header",
+    "sardine_cog_pack",
+    "sardine_write_cog_with_crs",
     "sardine_synth_cint16",
     "sardine_set_profiling",
     "sardine_last_kernel_ms",
@@ -236,4 +257,4 @@ EXPORTED_SYMBOLS = (
     "sardine_server_release",
     "sardine_server_destroy",
 )
-KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT = 0, 1, 2, 3, 4
+KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT, KERNEL_COG_PACK = 0, 1, 2, 3, 4, 5

@@ -12,6 +12,8 @@ Names and argument meaning follow the Rust crate:
   apply_calibration(...) -> Sigma0Array      apply_calibration.rs:374
   deburst_layout(...)                        deburst.rs:172-305
   to_db_inplace(data, noise_floor_linear)    export.rs:59-94
+  write_cog_with_crs(path, data, cols, rows, geotransform, crs, tile_size)
+        export.rs:329-354 (cog_layout / cog_header / cog_pack expose its parts)
 Rasters are torch tensors on the GPU (torch is used only to own device
 memory and streams); every computation runs in libsardine_hip.so.
 """
@@ -367,6 +369,48 @@ def to_db_inplace(data, noise_floor_linear: float, stream=None):
     _check(lib().sardine_to_db_inplace(data.data_ptr(), data.numel(), noise_floor_linear, counts,
                                        _stream_ptr(stream)))
     return int(counts[0]), int(counts[1])
+
+
+def cog_layout(cols: int, rows: int, tile_size: int, crs: int = abi.CRS_WGS84) -> abi.CogLayout:
+    """The pyramid levels, TIFF flavour and file offsets of a COG of a
+    cols x rows raster (write_cog_raw_inner, export.rs:430-535)."""
+    L = abi.CogLayout()
+    _check(lib().sardine_cog_layout_compute(cols, rows, tile_size, crs, C.byref(L)))
+    return L
+
+
+def cog_header(layout: abi.CogLayout, geotransform, crs: int = abi.CRS_WGS84, utm_zone: int = 0) -> bytes:
+    """The bytes of the file before its first tile (export.rs:537-670)."""
+    gt = (C.c_double * 6)(*geotransform)
+    buf = (C.c_uint8 * layout.header_bytes)()
+    _check(lib().sardine_cog_header(C.byref(layout), gt, crs, utm_zone, buf, layout.header_bytes))
+    return bytes(buf)
+
+
+def cog_pack(data, layout: abi.CogLayout, tiles=None, stream=None):
+    """The file's tile region (every level, coarsest first, NaN-padded
+    tiles) packed on the device from a float32 device raster; returns the
+    float32 device tensor of layout.tile_bytes / 4 values."""
+    torch = _torch()
+    full = layout.level[layout.n_levels - 1]
+    if data.dtype != torch.float32 or data.numel() != full.cols * full.rows or not data.is_contiguous():
+        raise ValueError("data must be a contiguous float32 raster of the layout's size")
+    if tiles is None:
+        tiles = torch.empty(layout.tile_bytes // 4, dtype=torch.float32, device=data.device)
+    _check(lib().sardine_cog_pack(data.data_ptr(), C.byref(layout), tiles.data_ptr(), _stream_ptr(stream)))
+    return tiles
+
+
+def write_cog_with_crs(path, data, cols: int, rows: int, geotransform, crs: int = abi.CRS_WGS84,
+                       tile_size: int = 512, utm_zone: int = 0, stream=None):
+    """write_cog_with_crs (export.rs:329-354): a tiled float32 COG with its
+    overview pyramid; `data` is a float32 device raster of rows x cols."""
+    torch = _torch()
+    if data.dtype != torch.float32 or data.numel() != cols * rows or not data.is_contiguous():
+        raise ValueError("data must be a contiguous float32 raster of rows x cols")
+    gt = (C.c_double * 6)(*geotransform)
+    _check(lib().sardine_write_cog_with_crs(str(path).encode(), data.data_ptr(), cols, rows, gt, crs, utm_zone,
+                                            tile_size, _stream_ptr(stream)))
 
 
 def synth_cint16(out, seed: int, index_base: int = 0, sigma: float = 100.0, stream=None):

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cerrno>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -43,6 +44,17 @@ extern "C" hipError_t sd_launch_speckle(int kind, const float* in, float* out, i
 extern "C" hipError_t sd_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_to_db(float* d, int64_t n, float floor_lin, unsigned long long* counts,
                                       hipStream_t s);
+namespace sd {
+struct CogLevelDev {  // layout of the struct in export.hip
+    float* tiles;
+    int64_t cols, rows;
+    int64_t tiles_x;
+    int64_t pad_cols, pad_rows;
+};
+}  // namespace sd
+extern "C" hipError_t sd_launch_cog_level(const float* src, int src_tiled, const sd::CogLevelDev* S,
+                                          const sd::CogLevelDev* D, const sd::CogLevelDev* copy, int lg,
+                                          hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -52,7 +64,7 @@ thread_local std::string g_last_error;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
-constexpr int kFamilies = 5;
+constexpr int kFamilies = 6;
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
 // waits here: sardine_last_kernel_ms waits for the end event when read.
@@ -1308,6 +1320,345 @@ sardine_status sardine_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint
     sardine_status st;
     if ((st = require_device())) return st;
     SD_HIP(sd_launch_synth_cint16(out, n, seed, base, sigma, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── Cloud-Optimised GeoTIFF export (export.rs:329-760) ───────────────────
+namespace {
+
+bool cog_tile_size_ok(uint32_t ts) { return ts >= 64 && ts <= 4096 && (ts & (ts - 1)) == 0; }
+
+// export.rs:338-342 (InvalidGeotransform)
+sardine_status cog_check_geotransform(const double* gt) {
+    if (!gt) return fail(SARDINE_ERR_INVALID_ARGUMENT, "geotransform is null");
+    const double pw = gt[1], ph = -gt[5];
+    if (!std::isfinite(pw) || pw <= 0.0 || !std::isfinite(ph) || ph <= 0.0)
+        return fail(SARDINE_ERR_EXPORT_INVALID_GEOTRANSFORM,
+                    "pixel width/height must be finite and positive: geotransform [%g, %g, %g, %g, %g, %g]", gt[0],
+                    gt[1], gt[2], gt[3], gt[4], gt[5]);
+    return SARDINE_OK;
+}
+
+// OutputCrs::geo_key_directory (output_crs.rs:265-287) for the CRSs the path
+// produces; epsg() (output_crs.rs:167-175).
+sardine_status cog_geo_keys(int32_t crs, int32_t zone, std::vector<uint16_t>* keys) {
+    if (crs == SARDINE_CRS_WGS84) {
+        *keys = {1, 1, 0, 3, 1024, 0, 1, 2, 1025, 0, 1, 1, 2048, 0, 1, 4326};
+        return SARDINE_OK;
+    }
+    if (crs != SARDINE_CRS_UTM_NORTH && crs != SARDINE_CRS_UTM_SOUTH)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown CRS %d", crs);
+    if (zone < 1 || zone > 60) return fail(SARDINE_ERR_INVALID_ARGUMENT, "UTM zone %d outside 1..60", zone);
+    const uint16_t epsg = (uint16_t)((crs == SARDINE_CRS_UTM_NORTH ? 32600 : 32700) + zone);
+    *keys = {1, 1, 0, 4, 1024, 0, 1, 1, 1025, 0, 1, 1, 3072, 0, 1, epsg, 3076, 0, 1, 9001};
+    return SARDINE_OK;
+}
+
+// write_cog_raw_inner's pyramid, format choice and offset pass
+// (export.rs:430-535).
+sardine_status cog_layout(int64_t cols, int64_t rows, uint32_t ts, int32_t geo_shorts, sardine_cog_layout* L) {
+    if (!cog_tile_size_ok(ts))
+        return fail(SARDINE_ERR_EXPORT_INVALID_TILE_SIZE,
+                    "COG tile size must be a power of two >= 64 and <= 4096; got %u", ts);
+    if (cols < 0 || rows < 0) return fail(SARDINE_ERR_INVALID_ARGUMENT, "negative raster size");
+    std::memset(L, 0, sizeof *L);
+    // overview sizes, finest first, then reversed (coarsest first)
+    std::vector<std::pair<int64_t, int64_t>> lv;
+    int64_t c = (cols + 1) / 2, r = (rows + 1) / 2;
+    while (c > (int64_t)ts || r > (int64_t)ts) {
+        lv.push_back({c, r});
+        c = (c + 1) / 2;
+        r = (r + 1) / 2;
+    }
+    lv.push_back({c, r});
+    std::reverse(lv.begin(), lv.end());
+    lv.push_back({cols, rows});
+    if ((int)lv.size() > SARDINE_COG_MAX_LEVELS)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "%zu pyramid levels exceed %d", lv.size(), SARDINE_COG_MAX_LEVELS);
+    // rows*cols*4 > u32::MAX with the reference's saturating products
+    const uint64_t uc = (uint64_t)cols, ur = (uint64_t)rows;
+    const bool big = uc != 0 && ur != 0 && (ur > UINT64_MAX / uc || ur * uc > 0xffffffffull / 4);
+    const uint64_t n_entries = 17, geo_bytes = (uint64_t)geo_shorts * 2;
+    const uint64_t ifd_fixed = big ? 8 + n_entries * 20 + 8 : 2 + n_entries * 12 + 4;
+    const uint64_t entry = big ? 8 : 4;
+    L->n_levels = (int32_t)lv.size();
+    L->bigtiff = big ? 1 : 0;
+    L->tile_size = ts;
+    L->geo_key_shorts = geo_shorts;
+    uint64_t off = big ? 16 : 8;
+    for (size_t k = 0; k < lv.size(); ++k) {
+        sardine_cog_level& l = L->level[k];
+        l.cols = lv[k].first;
+        l.rows = lv[k].second;
+        l.tiles_x = (l.cols + ts - 1) / ts;
+        l.tiles_y = (l.rows + ts - 1) / ts;
+        const uint64_t n = (uint64_t)(l.tiles_x * l.tiles_y);
+        l.ifd_offset = off;
+        off += ifd_fixed;
+        l.tile_offsets_offset = off;
+        off += n * entry;
+        l.tile_counts_offset = off;
+        off += n * entry;
+    }
+    L->pixel_scale_offset = off;
+    off += 24;
+    L->tiepoint_offset = off;
+    off += 48;
+    L->geo_keys_offset = off;
+    off += geo_bytes;
+    L->header_bytes = off;
+    const uint64_t tile_bytes = (uint64_t)ts * ts * 4;
+    for (int k = 0; k < L->n_levels; ++k) {
+        sardine_cog_level& l = L->level[k];
+        l.tile_data_offset = off;
+        off += (uint64_t)(l.tiles_x * l.tiles_y) * tile_bytes;
+    }
+    L->file_bytes = off;
+    L->tile_bytes = off - L->header_bytes;
+    return SARDINE_OK;
+}
+
+template <typename T>
+void put_le(std::vector<uint8_t>& b, T v) {  // the host is little-endian (x86-64)
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+}
+
+// Header, IFDs, tile arrays and GeoTIFF blocks (export.rs:537-670).  Classic
+// TIFF offsets are the low 32 bits of the 64-bit ones, as the reference
+// writes them.
+void cog_header_bytes(const sardine_cog_layout& L, const double* gt, const std::vector<uint16_t>& keys,
+                      std::vector<uint8_t>& b) {
+    const bool big = L.bigtiff != 0;
+    const uint64_t ts = L.tile_size, tile_bytes = ts * ts * 4;
+    b.clear();
+    b.reserve(L.header_bytes);
+    b.push_back('I');
+    b.push_back('I');
+    if (big) {
+        put_le<uint16_t>(b, 43);
+        put_le<uint16_t>(b, 8);
+        put_le<uint16_t>(b, 0);
+        put_le<uint64_t>(b, L.level[0].ifd_offset);
+    } else {
+        put_le<uint16_t>(b, 42);
+        put_le<uint32_t>(b, (uint32_t)L.level[0].ifd_offset);
+    }
+    auto entry = [&](uint16_t tag, uint16_t type, uint64_t count, uint64_t value) {
+        put_le<uint16_t>(b, tag);
+        put_le<uint16_t>(b, type);
+        if (big) {
+            put_le<uint64_t>(b, count);
+            put_le<uint64_t>(b, value);
+        } else {
+            put_le<uint32_t>(b, (uint32_t)count);
+            put_le<uint32_t>(b, (uint32_t)value);
+        }
+    };
+    for (int k = 0; k < L.n_levels; ++k) {
+        const sardine_cog_level& l = L.level[k];
+        const uint64_t n = (uint64_t)(l.tiles_x * l.tiles_y);
+        if (big) put_le<uint64_t>(b, 17);
+        else put_le<uint16_t>(b, 17);
+        entry(256, 4, 1, (uint64_t)l.cols);  // ImageWidth
+        entry(257, 4, 1, (uint64_t)l.rows);  // ImageLength
+        entry(258, 3, 1, 32);                // BitsPerSample
+        entry(259, 3, 1, 1);                 // Compression: none
+        entry(262, 3, 1, 1);                 // PhotometricInterpretation
+        entry(277, 3, 1, 1);                 // SamplesPerPixel
+        // X/YResolution: an inline RATIONAL 72/1 in BigTIFF, a LONG 72 in
+        // classic TIFF (export.rs:638-646)
+        entry(282, big ? 5 : 4, 1, big ? ((1ull << 32) | 72ull) : 72ull);
+        entry(283, big ? 5 : 4, 1, big ? ((1ull << 32) | 72ull) : 72ull);
+        entry(296, 3, 1, 1);                       // ResolutionUnit
+        entry(322, 4, 1, ts);                      // TileWidth
+        entry(323, 4, 1, ts);                      // TileLength
+        entry(324, big ? 16 : 4, n, l.tile_offsets_offset);
+        entry(325, big ? 16 : 4, n, l.tile_counts_offset);
+        entry(339, 3, 1, 3);                       // SampleFormat: IEEE float
+        entry(33550, 12, 3, L.pixel_scale_offset);  // ModelPixelScale
+        entry(33922, 12, 6, L.tiepoint_offset);     // ModelTiepoint
+        entry(34735, 3, keys.size(), L.geo_keys_offset);
+        const uint64_t next = k + 1 < L.n_levels ? L.level[k + 1].ifd_offset : 0;
+        if (big) put_le<uint64_t>(b, next);
+        else put_le<uint32_t>(b, (uint32_t)next);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t o = l.tile_data_offset + i * tile_bytes;
+            if (big) put_le<uint64_t>(b, o);
+            else put_le<uint32_t>(b, (uint32_t)o);
+        }
+        for (uint64_t i = 0; i < n; ++i) {
+            if (big) put_le<uint64_t>(b, tile_bytes);
+            else put_le<uint32_t>(b, (uint32_t)tile_bytes);
+        }
+    }
+    const double blocks[9] = {gt[1], -gt[5], 0.0, 0.0, 0.0, 0.0, gt[0], gt[3], 0.0};
+    for (double v : blocks) put_le<double>(b, v);
+    for (uint16_t k : keys) put_le<uint16_t>(b, k);
+}
+
+// The layout a caller passes back, re-derived from its full size, tile size
+// and key count so that a stale or edited struct cannot steer the kernels.
+sardine_status cog_relayout(const sardine_cog_layout* in, sardine_cog_layout* L) {
+    if (!in || in->n_levels < 2 || in->n_levels > SARDINE_COG_MAX_LEVELS)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "invalid COG layout");
+    if (in->geo_key_shorts != 16 && in->geo_key_shorts != 20)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "invalid COG layout (geo key count %d)", in->geo_key_shorts);
+    const sardine_cog_level& f = in->level[in->n_levels - 1];
+    sardine_status st;
+    if ((st = cog_layout(f.cols, f.rows, in->tile_size, in->geo_key_shorts, L))) return st;
+    if (L->n_levels != in->n_levels || L->tile_bytes != in->tile_bytes || L->header_bytes != in->header_bytes)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "COG layout does not match its full raster size");
+    return SARDINE_OK;
+}
+
+// The device pyramid + pack of every level into `tiles` (export.rs:672-712).
+sardine_status cog_pack(const float* data, const sardine_cog_layout& L, float* tiles, hipStream_t s) {
+    if (L.tile_bytes == 0) return SARDINE_OK;
+    int lg = 0;
+    while ((1u << lg) < L.tile_size) ++lg;
+    auto dev_level = [&](int k) {
+        const sardine_cog_level& l = L.level[k];
+        sd::CogLevelDev d;
+        d.tiles = tiles + (l.tile_data_offset - L.header_bytes) / sizeof(float);
+        d.cols = l.cols;
+        d.rows = l.rows;
+        d.tiles_x = l.tiles_x;
+        d.pad_cols = l.tiles_x * (int64_t)L.tile_size;
+        d.pad_rows = l.tiles_y * (int64_t)L.tile_size;
+        return d;
+    };
+    const int n = L.n_levels;
+    KernelTimer kt(SARDINE_KERNEL_COG_PACK, s);
+    const sd::CogLevelDev full = dev_level(n - 1);
+    sd::CogLevelDev d = dev_level(n - 2);
+    SD_HIP(sd_launch_cog_level(data, 0, &full, &d, &full, lg, s));
+    for (int k = n - 3; k >= 0; --k) {
+        const sd::CogLevelDev src = d, none{};
+        d = dev_level(k);
+        SD_HIP(sd_launch_cog_level(src.tiles, 1, &src, &d, &none, lg, s));
+    }
+    kt.stop();
+    return SARDINE_OK;
+}
+
+struct PinnedPair {  // the two host buffers of the file writer's copy/write overlap
+    void* p[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~PinnedPair() {
+        for (int i = 0; i < 2; ++i) {
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+            if (p[i]) (void)hipHostFree(p[i]);
+        }
+    }
+};
+
+struct FileCloser {
+    std::FILE* f = nullptr;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_cog_layout_compute(int64_t cols, int64_t rows, uint32_t tile_size, int32_t crs,
+                                          sardine_cog_layout* layout) {
+    if (!layout) return fail(SARDINE_ERR_INVALID_ARGUMENT, "layout is null");
+    if (crs != SARDINE_CRS_WGS84 && crs != SARDINE_CRS_UTM_NORTH && crs != SARDINE_CRS_UTM_SOUTH)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown CRS %d", crs);
+    return cog_layout(cols, rows, tile_size, crs == SARDINE_CRS_WGS84 ? 16 : 20, layout);
+}
+
+sardine_status sardine_cog_header(const sardine_cog_layout* layout, const double geotransform[6], int32_t crs,
+                                  int32_t utm_zone, uint8_t* out, uint64_t capacity) {
+    sardine_status st;
+    if ((st = cog_check_geotransform(geotransform))) return st;
+    std::vector<uint16_t> keys;
+    if ((st = cog_geo_keys(crs, utm_zone, &keys))) return st;
+    sardine_cog_layout L;
+    if ((st = cog_relayout(layout, &L))) return st;
+    if ((int32_t)keys.size() != L.geo_key_shorts)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "layout was computed for another CRS");
+    if (!out || capacity < L.header_bytes)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "header needs %llu bytes", (unsigned long long)L.header_bytes);
+    std::vector<uint8_t> b;
+    cog_header_bytes(L, geotransform, keys, b);
+    std::memcpy(out, b.data(), b.size());
+    return SARDINE_OK;
+}
+
+sardine_status sardine_cog_pack(const float* data, const sardine_cog_layout* layout, float* tiles, void* stream) {
+    sardine_status st;
+    sardine_cog_layout L;
+    if ((st = cog_relayout(layout, &L))) return st;
+    const sardine_cog_level& f = L.level[L.n_levels - 1];
+    if (f.cols > 0 && f.rows > 0 && !data) return fail(SARDINE_ERR_INVALID_ARGUMENT, "data is null");
+    if (L.tile_bytes > 0 && (!tiles || reinterpret_cast<uintptr_t>(tiles) % 16 != 0))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "tiles must be a 16-byte aligned device buffer");
+    if ((st = require_device())) return st;
+    return cog_pack(data, L, tiles, (hipStream_t)stream);
+}
+
+sardine_status sardine_write_cog_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+                                          const double geotransform[6], int32_t crs, int32_t utm_zone,
+                                          uint32_t tile_size, void* stream_v) {
+    sardine_status st;
+    if ((st = cog_check_geotransform(geotransform))) return st;
+    if (!cog_tile_size_ok(tile_size))
+        return fail(SARDINE_ERR_EXPORT_INVALID_TILE_SIZE,
+                    "COG tile size must be a power of two >= 64 and <= 4096; got %u", tile_size);
+    std::vector<uint16_t> keys;
+    if ((st = cog_geo_keys(crs, utm_zone, &keys))) return st;
+    if (!path) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path is null");
+    if (cols < 0 || rows < 0 || (cols > 0 && rows > 0 && !data))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad raster");
+    sardine_cog_layout L;
+    if ((st = cog_layout(cols, rows, tile_size, (int32_t)keys.size(), &L))) return st;
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream_v;
+    DevBuf tiles;
+    SD_HIP(tiles.alloc(L.tile_bytes, s));
+    if ((st = cog_pack(data, L, tiles.as<float>(), s))) return st;
+    std::vector<uint8_t> head;
+    cog_header_bytes(L, geotransform, keys, head);
+    FileCloser fc;
+    fc.f = std::fopen(path, "wb");
+    if (!fc.f) return fail(SARDINE_ERR_EXPORT_IO, "cannot create %s: %s", path, std::strerror(errno));
+    if (std::fwrite(head.data(), 1, head.size(), fc.f) != head.size())
+        return fail(SARDINE_ERR_EXPORT_IO, "writing %s: %s", path, std::strerror(errno));
+    // Tiles: device → two pinned chunks in turn, each written while the
+    // next is copied.
+    const uint64_t chunk = std::min<uint64_t>(L.tile_bytes, 32ull << 20);
+    if (chunk > 0) {
+        PinnedPair pp;
+        for (int i = 0; i < 2; ++i) {
+            SD_HIP(hipHostMalloc(&pp.p[i], chunk, hipHostMallocDefault));
+            SD_HIP(hipEventCreateWithFlags(&pp.ev[i], hipEventDisableTiming));
+        }
+        const uint8_t* src = tiles.as<uint8_t>();
+        const uint64_t n_chunks = (L.tile_bytes + chunk - 1) / chunk;
+        auto issue = [&](uint64_t i) -> hipError_t {
+            const uint64_t off = i * chunk, n = std::min(chunk, L.tile_bytes - off);
+            hipError_t e = hipMemcpyAsync(pp.p[i % 2], src + off, n, hipMemcpyDeviceToHost, s);
+            return e != hipSuccess ? e : hipEventRecord(pp.ev[i % 2], s);
+        };
+        SD_HIP(issue(0));
+        for (uint64_t i = 0; i < n_chunks; ++i) {
+            SD_HIP(hipEventSynchronize(pp.ev[i % 2]));
+            if (i + 1 < n_chunks) SD_HIP(issue(i + 1));
+            const uint64_t n = std::min(chunk, L.tile_bytes - i * chunk);
+            if (std::fwrite(pp.p[i % 2], 1, n, fc.f) != n)
+                return fail(SARDINE_ERR_EXPORT_IO, "writing %s: %s", path, std::strerror(errno));
+        }
+    }
+    const int rc = std::fclose(fc.f);
+    fc.f = nullptr;
+    if (rc != 0) return fail(SARDINE_ERR_EXPORT_IO, "closing %s: %s", path, std::strerror(errno));
     return SARDINE_OK;
 }
 

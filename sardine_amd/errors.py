@@ -5,6 +5,7 @@
   CalibrationError        sardine/src/apply_calibration.rs:64-84
   DeburstError            sardine/src/deburst.rs:56-121
   MergeError              sardine/src/merge_subswaths.rs:62-81
+  ExportError             sardine/src/export.rs:26-39
 Each carries the C-ABI status code (`.status`) and its name (`.variant`).
 """
 from __future__ import annotations
@@ -39,6 +40,10 @@ class MergeError(SardineError):
     pass
 
 
+class ExportError(SardineError):
+    pass
+
+
 class DeviceError(SardineError):
     pass
 
@@ -56,4 +61,6 @@ def raise_for(status: int, message: str):
         raise DeburstError(status, message)
     if 50 <= status < 60:
         raise MergeError(status, message)
+    if 60 <= status < 70:
+        raise ExportError(status, message)
     raise DeviceError(status, message)

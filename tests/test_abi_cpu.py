@@ -229,3 +229,60 @@ def test_tc_config_validation_errors():
     st = L.sardine_terrain_correction(C.byref(abi.RadarImage()), C.byref(s), C.byref(abi.DemRaster()), C.byref(c),
                                       C.byref(img), None)
     assert st == abi.ERR_TC_NO_SUBSWATHS
+
+
+# ── COG export: the host half (layout, header bytes, validation order) ──
+COG_GT = [8.0, 0.0001, 0.0, 52.0, 0.0, -0.0001]
+
+
+@pytest.mark.parametrize("cols,rows,ts,crs,zone", [
+    (1024, 512, 512, abi.CRS_WGS84, 0), (300, 200, 64, abi.CRS_UTM_NORTH, 33), (129, 130, 64, abi.CRS_UTM_SOUTH, 1),
+    (1, 1, 64, abi.CRS_WGS84, 0), (0, 0, 64, abi.CRS_WGS84, 0), (0, 500, 128, abi.CRS_WGS84, 0),
+    (25000, 1, 4096, abi.CRS_UTM_NORTH, 60), (32768, 32767, 512, abi.CRS_WGS84, 0),
+    (32768, 32768, 512, abi.CRS_WGS84, 0), (100000, 70001, 256, abi.CRS_UTM_SOUTH, 58)])
+def test_cog_header_matches_oracle(cols, rows, ts, crs, zone):
+    """Layout and header bytes (both classic TIFF and BigTIFF) are host code:
+    they are compared with the oracle without a device."""
+    L = api.cog_layout(cols, rows, ts, crs)
+    gt = [500000.0, 10.0, 0.0, 5.2e6, 0.0, -10.0] if crs else COG_GT
+    head = api.cog_header(L, gt, crs, zone)
+    st, ref = O.cog_header(cols, rows, gt, crs, zone, ts)
+    assert st == 0 and head == ref
+    assert L.bigtiff == int(rows * cols * 4 > 0xFFFFFFFF)
+    assert L.level[L.n_levels - 1].cols == cols and L.level[L.n_levels - 1].rows == rows
+    n_tiles = sum(L.level[k].tiles_x * L.level[k].tiles_y for k in range(L.n_levels))
+    assert L.tile_bytes == n_tiles * ts * ts * 4 and L.file_bytes == L.header_bytes + L.tile_bytes
+    assert L.level[0].tile_data_offset == L.header_bytes == len(head)
+
+
+def test_cog_argument_errors_precede_device_check(tmp_path):
+    L = _native.lib()
+    d = np.ones(4, np.float32)
+    gt = (C.c_double * 6)(*COG_GT)
+    bad_gt = (C.c_double * 6)(8.0, 0.0, 0.0, 52.0, 0.0, -0.0001)
+    p = str(tmp_path / "x.tiff").encode()
+    for ts in (100, 32, 8192, 0):
+        assert L.sardine_write_cog_with_crs(p, d.ctypes.data, 2, 2, gt, 0, 0, ts, None) == \
+            abi.ERR_EXPORT_INVALID_TILE_SIZE
+    assert L.sardine_write_cog_with_crs(p, d.ctypes.data, 2, 2, bad_gt, 0, 0, 100, None) == \
+        abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+    assert L.sardine_write_cog_with_crs(p, d.ctypes.data, 2, 2, gt, abi.CRS_UTM_NORTH, 61, 512, None) == \
+        abi.ERR_INVALID_ARGUMENT
+    assert L.sardine_write_cog_with_crs(p, d.ctypes.data, 2, 2, gt, 7, 0, 512, None) == abi.ERR_INVALID_ARGUMENT
+    lay = abi.CogLayout()
+    assert L.sardine_cog_layout_compute(2, 2, 96, 0, C.byref(lay)) == abi.ERR_EXPORT_INVALID_TILE_SIZE
+    assert L.sardine_cog_layout_compute(-1, 2, 64, 0, C.byref(lay)) == abi.ERR_INVALID_ARGUMENT
+    # a layout edited after it was computed is refused
+    lay = api.cog_layout(300, 200, 64)
+    lay.tile_bytes += 4
+    buf = (C.c_uint8 * 4096)()
+    assert L.sardine_cog_header(C.byref(lay), gt, 0, 0, buf, 4096) == abi.ERR_INVALID_ARGUMENT
+    lay = api.cog_layout(300, 200, 64)
+    assert L.sardine_cog_header(C.byref(lay), gt, abi.CRS_UTM_NORTH, 33, buf, 4096) == abi.ERR_INVALID_ARGUMENT
+    assert L.sardine_cog_header(C.byref(lay), gt, 0, 0, buf, 10) == abi.ERR_INVALID_ARGUMENT
+    import torch
+
+    if not torch.cuda.is_available():
+        assert L.sardine_write_cog_with_crs(p, d.ctypes.data, 2, 2, gt, 0, 0, 64, None) == abi.ERR_NO_DEVICE
+        assert L.sardine_cog_pack(d.ctypes.data, C.byref(lay), d.ctypes.data, None) == abi.ERR_NO_DEVICE
+        assert not (tmp_path / "x.tiff").exists()

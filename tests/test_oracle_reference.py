@@ -614,6 +614,186 @@ def test_to_db_reference_cases():
     assert O.to_db_inplace(np.ones(1, np.float32), -0.001)[0] != 0
 
 
+# ── COG export (export.rs:1583-1740) ──────────────────────────────────────
+GT_COG = [8.0, 0.0001, 0.0, 52.0, 0.0, -0.0001]
+
+
+def _pyramid_nanmean(a):
+    """2x2 block means of the non-NaN values by numpy's nanmean over the
+    NaN-padded blocks (a different summation route from the oracle's)."""
+    r, c = a.shape
+    p = np.full((r + r % 2, c + c % 2), np.nan, np.float64)
+    p[:r, :c] = a
+    blocks = p.reshape(p.shape[0] // 2, 2, p.shape[1] // 2, 2)
+    with np.errstate(all="ignore"):
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)
+            return np.nanmean(blocks, axis=(1, 3)).astype(np.float32)
+
+
+def _ulp_diff(a, b):
+    ai, bi = a.view(np.int32).astype(np.int64), b.view(np.int32).astype(np.int64)
+    fin = np.isfinite(a) & np.isfinite(b)
+    return np.abs(ai - bi)[fin].max(initial=0)
+
+
+def test_downsample_2x_reference_cases():
+    d = O.downsample_2x(np.arange(16, dtype=np.float32).reshape(4, 4))  # export.rs:1697-1715
+    assert d.shape == (2, 2) and np.array_equal(d, [[2.5, 4.5], [10.5, 12.5]])
+    s = np.full((2, 2), np.nan, np.float32)  # export.rs:1717-1731
+    s[0, 0] = 1.0
+    assert O.downsample_2x(s)[0, 0] == 1.0
+    assert np.isnan(O.downsample_2x(np.full((2, 2), np.nan, np.float32))[0, 0])
+
+
+def test_downsample_2x_against_nanmean():
+    rng = np.random.default_rng(7)
+    for shape in [(1, 1), (1, 5), (2, 3), (7, 9), (33, 64), (65, 31)]:
+        a = (rng.normal(0, 3, shape) * 10.0 ** rng.integers(-3, 4, shape)).astype(np.float32)
+        a[rng.random(shape) < 0.3] = np.nan
+        if shape[0] >= 2 and shape[1] >= 2:
+            a[:2, :2] = np.nan  # one all-NaN block
+        d = O.downsample_2x(a)
+        e = _pyramid_nanmean(a)
+        assert d.shape == ((shape[0] + 1) // 2, (shape[1] + 1) // 2)
+        assert np.array_equal(np.isnan(d), np.isnan(e))
+        assert _ulp_diff(d, e) <= 1
+    c = np.full((5, 7), 0.3, np.float32)  # a constant raster stays constant
+    assert np.all(O.downsample_2x(c) == np.float32(0.3))
+
+
+def test_write_cog_rejects_invalid_tile_size(tmp_path):
+    d = np.ones((2, 2), np.float32)  # export.rs:1583-1595
+    for ts in (100, 32, 8192):
+        assert O.write_cog(tmp_path / "a.tiff", d, GT_COG, tile_size=ts) == abi.ERR_EXPORT_INVALID_TILE_SIZE
+    assert not (tmp_path / "a.tiff").exists()
+
+
+def test_write_cog_rejects_invalid_geotransform(tmp_path):
+    d = np.ones((2, 2), np.float32)  # export.rs:1597-1603
+    for gt in ([8.0, 0.0, 0.0, 52.0, 0.0, -0.0001], [8.0, np.nan, 0.0, 52.0, 0.0, -0.0001],
+               [8.0, 1e-4, 0.0, 52.0, 0.0, 1e-4], [8.0, np.inf, 0.0, 52.0, 0.0, -1e-4]):
+        assert O.write_cog(tmp_path / "b.tiff", d, gt, tile_size=512) == abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+    # the geotransform is checked before the tile size (export.rs:338-345)
+    assert O.write_cog(tmp_path / "b.tiff", d, [8.0, 0.0, 0, 52.0, 0, -1e-4], tile_size=100) == \
+        abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+
+
+def test_write_cog_header_reference_case(tmp_path):
+    rows, cols = 512, 1024  # export.rs:1605-1628
+    data = (np.arange(rows * cols, dtype=np.float32) * np.float32(0.001)).reshape(rows, cols)
+    path = tmp_path / "h.tiff"
+    assert O.write_cog(path, data, GT_COG, tile_size=512) == 0
+    buf = path.read_bytes()
+    assert buf[:4] == b"II*\x00"
+    ifd = int.from_bytes(buf[4:8], "little")
+    assert 0 < ifd < len(buf)
+    assert int.from_bytes(buf[ifd:ifd + 2], "little") == 17
+
+
+def test_write_cog_tile_round_trip_reference_case(tmp_path):
+    # export.rs:1630-1695, read back the way that test reads it
+    data = np.arange(64 * 64, dtype=np.float32).reshape(64, 64)
+    path = tmp_path / "ramp.tiff"
+    assert O.write_cog(path, data, [0.0, 0.001, 0.0, 1.0, 0.0, -0.001], tile_size=64) == 0
+    buf = path.read_bytes()
+    u32 = lambda o: int.from_bytes(buf[o:o + 4], "little")
+    ifd0 = u32(4)
+    n0 = int.from_bytes(buf[ifd0:ifd0 + 2], "little")
+    ifd1 = u32(ifd0 + 2 + n0 * 12)
+    assert 0 < ifd1 < len(buf)
+    n1 = int.from_bytes(buf[ifd1:ifd1 + 2], "little")
+    off = None
+    for i in range(n1):
+        e = ifd1 + 2 + i * 12
+        if int.from_bytes(buf[e:e + 2], "little") == 324:
+            off = u32(e + 8)
+    tile = np.frombuffer(buf, "<f4", 64 * 64, u32(off))
+    assert tile[0] == 0.0 and tile[64] == 64.0 and tile[64 * 64 - 1] == 4095.0
+
+
+def _expected_levels(cols, rows, ts):
+    """Level sizes by the rule of export.rs:441-456: halve until both sides
+    are <= ts (at least once), coarsest first, then the full raster."""
+    out = []
+    c, r = (cols + 1) // 2, (rows + 1) // 2
+    while c > ts or r > ts:
+        out.append((c, r))
+        c, r = (c + 1) // 2, (r + 1) // 2
+    out.append((c, r))
+    return out[::-1] + [(cols, rows)]
+
+
+@pytest.mark.parametrize("rows,cols,ts,crs,zone", [(512, 1024, 512, 0, 0), (200, 300, 64, 1, 33),
+                                                   (130, 129, 64, 2, 17), (64, 64, 64, 0, 0), (1, 1, 64, 0, 0),
+                                                   (700, 90, 128, 1, 60)])
+def test_write_cog_structure_parsed(tmp_path, rows, cols, ts, crs, zone):
+    import tiffread
+    rng = np.random.default_rng(rows * 7 + cols)
+    data = rng.normal(0, 1, (rows, cols)).astype(np.float32)
+    data[rng.random((rows, cols)) < 0.1] = np.nan
+    gt = [500000.0, 10.0, 0.0, 5.2e6, 0.0, -10.0] if crs else [8.0, 1e-4, 0.0, 52.0, 0.0, -1e-4]
+    path = tmp_path / "s.tiff"
+    assert O.write_cog(path, data, gt, crs=crs, utm_zone=zone, tile_size=ts) == 0
+    buf = path.read_bytes()
+    t = tiffread.parse(buf)
+    assert not t["bigtiff"]
+    levels = _expected_levels(cols, rows, ts)
+    assert [(f["tags"][256][1][0], f["tags"][257][1][0]) for f in t["ifds"]] == levels
+    keys = [1, 1, 0, 3, 1024, 0, 1, 2, 1025, 0, 1, 1, 2048, 0, 1, 4326] if crs == 0 else \
+        [1, 1, 0, 4, 1024, 0, 1, 1, 1025, 0, 1, 1, 3072, 0, 1, (32600 if crs == 1 else 32700) + zone, 3076, 0, 1, 9001]
+    for f in t["ifds"]:
+        g = f["tags"]
+        assert f["order"] == [256, 257, 258, 259, 262, 277, 282, 283, 296, 322, 323, 324, 325, 339, 33550, 33922,
+                              34735]
+        assert g[258][1] == [32] and g[259][1] == [1] and g[277][1] == [1] and g[339][1] == [3]
+        assert g[282] == (4, [72]) and g[322][1] == [ts] and g[323][1] == [ts]
+        assert g[33550][1] == [gt[1], -gt[5], 0.0]
+        assert g[33922][1] == [0.0, 0.0, 0.0, gt[0], gt[3], 0.0]
+        assert g[34735][1] == keys
+    # tiles: contiguous after the header, coarsest level first, NaN padding
+    first = t["ifds"][0]["tags"][324][1][0]
+    n_tiles = sum(len(f["tags"][324][1]) for f in t["ifds"])
+    assert len(buf) == first + n_tiles * ts * ts * 4
+    full = tiffread.level_raster(buf, t["ifds"][-1], padded=True)
+    assert np.array_equal(full[:rows, :cols].view(np.uint32), data.view(np.uint32))
+    assert np.isnan(full[rows:, :]).all() and np.isnan(full[:, cols:]).all()
+    lv = data
+    for f in reversed(t["ifds"][:-1]):
+        lv = _pyramid_nanmean(lv)
+        got = tiffread.level_raster(buf, f)
+        assert np.array_equal(np.isnan(got), np.isnan(lv)) and _ulp_diff(got, lv) <= 1
+        lv = got
+
+
+def test_write_cog_read_by_pillow(tmp_path):
+    from PIL import Image
+    rows, cols = 300, 200
+    data = np.linspace(-5, 5, rows * cols, dtype=np.float32).reshape(rows, cols)
+    path = tmp_path / "p.tiff"
+    assert O.write_cog(path, data, GT_COG, tile_size=64) == 0
+    with Image.open(path) as im:
+        assert im.n_frames == len(_expected_levels(cols, rows, 64))
+        im.seek(im.n_frames - 1)
+        assert im.size == (cols, rows) and im.mode == "F"
+        assert np.array_equal(np.asarray(im, np.float32), data)
+
+
+def test_cog_header_bigtiff_choice():
+    # rows*cols*4 > u32::MAX selects BigTIFF (export.rs:461-466)
+    st, h = O.cog_header(32768, 32767, GT_COG, tile_size=512)
+    assert st == 0 and h[:4] == b"II*\x00"
+    st, h = O.cog_header(32768, 32768, GT_COG, tile_size=512)
+    assert st == 0 and h[:8] == b"II+\x00\x08\x00\x00\x00"
+    import tiffread
+    t = tiffread.parse(h + b"\0" * 0)
+    assert t["bigtiff"] and len(t["ifds"]) == len(_expected_levels(32768, 32768, 512))
+    g = t["ifds"][-1]["tags"]
+    assert g[256][1] == [32768] and g[282] == (5, [(72, 1)]) and g[324][0] == 16
+    assert len(g[324][1]) == 64 * 64 and g[325][1] == [512 * 512 * 4] * 4096
+
+
 # ── full chain invariants on a small synthetic scene ──────────────────────
 def test_small_scene_chain_invariants():
     sc = synthetic.scene_small_plain(2, lines=256, samples=512, out_rows=160, out_cols=160, dem_side=256)
