@@ -18,9 +18,11 @@
 // ceil(c/ts)*ts), so every padding value of every tile is written.
 //
 // Bandwidth: HBM-bound streaming (the first launch reads 4 B and writes
-// 5 B per full-resolution pixel; the rest add 1/3 of that).
+// 5 B per full-resolution pixel; the rest add about 1/4 of that).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 namespace sd {
 
@@ -50,60 +52,105 @@ __device__ __forceinline__ float patch_mean(float a, float b, float c, float d) 
     return count > 0 ? (float)(sum / (double)count) : __builtin_nanf("");
 }
 
+// The 4-column x 2-row source patch of output pair (xo, yo) into v.
 // TILED: the source is a level in its tiles (else the row-major full
 // raster).  VEC: row-major source rows are 16-byte aligned at every
-// multiple of 4 columns.  COPY: also write the source into `copy`'s tiles.
+// multiple of 4 columns.
+template <bool TILED, bool VEC>
+__device__ __forceinline__ void load_patch(const float* __restrict__ src, const CogLevelDev& S, int64_t xs,
+                                           int64_t yo, int lg, float (&v)[2][4]) {
+    const float qnan = __builtin_nanf("");
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int64_t ys = 2 * yo + r;
+        if (TILED) {
+            // inside the source's padded area the tiles hold NaN beyond its
+            // raster, so only the padded bound is tested
+            if (xs < S.pad_cols && ys < S.pad_rows) {
+                const float4 q = *reinterpret_cast<const float4*>(src + tile_index(xs, ys, S.tiles_x, lg));
+                v[r][0] = q.x;
+                v[r][1] = q.y;
+                v[r][2] = q.z;
+                v[r][3] = q.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[r][j] = qnan;
+            }
+        } else {
+            if (ys < S.rows && VEC && xs + 3 < S.cols) {
+                const float4 q = *reinterpret_cast<const float4*>(src + ys * S.cols + xs);
+                v[r][0] = q.x;
+                v[r][1] = q.y;
+                v[r][2] = q.z;
+                v[r][3] = q.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    v[r][j] = (ys < S.rows && xs + j < S.cols) ? src[ys * S.cols + xs + j] : qnan;
+            }
+        }
+    }
+}
+
+// Rows of output pairs a thread handles per loop trip (their loads are all
+// issued before any store), the workgroups per launch the grid is capped at
+// (0: no cap) and whether tiles are written with non-temporal stores (they
+// are not read again on the device except as the next level's source).
+// Build options, so that they can be tuned per device; on MI355X at
+// 25479 x 18590 (tile 512) 2 rows, no cap and non-temporal stores measured
+// best (capping the grid at 1-8k workgroups cost 8-20%).
+#ifndef SD_COG_ROWS
+#define SD_COG_ROWS 2
+#endif
+#ifndef SD_COG_WGS
+#define SD_COG_WGS 0
+#endif
+#ifndef SD_COG_NT
+#define SD_COG_NT 1
+#endif
+constexpr int kCogRows = SD_COG_ROWS;
+constexpr int64_t kCogWgs = SD_COG_WGS;
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <typename V>
+__device__ __forceinline__ void put(V* p, V v) {
+    if (SD_COG_NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// COPY: also write the source into `copy`'s tiles (the full raster's).
 template <bool TILED, bool VEC, bool COPY>
 __global__ __launch_bounds__(256) void cog_level_kernel(const float* __restrict__ src, CogLevelDev S,
                                                         CogLevelDev D, CogLevelDev copy, int lg) {
-    const float qnan = __builtin_nanf("");
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // output column pair
     const int64_t xo = 2 * g, xs = 4 * g;
     if (xo >= D.pad_cols) return;
-    for (int64_t yo = blockIdx.y; yo < D.pad_rows; yo += gridDim.y) {
-        float v[2][4];
+    const int64_t step = gridDim.y;
+    for (int64_t y0 = blockIdx.y; y0 < D.pad_rows; y0 += kCogRows * step) {
+        float v[kCogRows][2][4];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int64_t ys = 2 * yo + r;
-            if (TILED) {
-                // inside the source's padded area the tiles hold NaN beyond
-                // its raster, so only the padded bound is tested
-                if (xs < S.pad_cols && ys < S.pad_rows) {
-                    const float4 q = *reinterpret_cast<const float4*>(src + tile_index(xs, ys, S.tiles_x, lg));
-                    v[r][0] = q.x;
-                    v[r][1] = q.y;
-                    v[r][2] = q.z;
-                    v[r][3] = q.w;
-                } else {
+        for (int u = 0; u < kCogRows; ++u) {
+            const int64_t yo = y0 + u * step;
+            if (u == 0 || yo < D.pad_rows) load_patch<TILED, VEC>(src, S, xs, yo, lg, v[u]);
+        }
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[r][j] = qnan;
-                }
-            } else {
-                const float* row = src + ys * S.cols + xs;
-                if (ys < S.rows && VEC && xs + 3 < S.cols) {
-                    const float4 q = *reinterpret_cast<const float4*>(row);
-                    v[r][0] = q.x;
-                    v[r][1] = q.y;
-                    v[r][2] = q.z;
-                    v[r][3] = q.w;
-                } else {
+        for (int u = 0; u < kCogRows; ++u) {
+            const int64_t yo = y0 + u * step;
+            if (u > 0 && yo >= D.pad_rows) break;
+            if (COPY) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[r][j] = (ys < S.rows && xs + j < S.cols) ? row[j] : qnan;
+                for (int r = 0; r < 2; ++r) {
+                    const int64_t ys = 2 * yo + r;
+                    if (xs < copy.pad_cols && ys < copy.pad_rows)
+                        put(reinterpret_cast<f4v*>(copy.tiles + tile_index(xs, ys, copy.tiles_x, lg)),
+                            f4v{v[u][r][0], v[u][r][1], v[u][r][2], v[u][r][3]});
                 }
             }
+            const float m0 = patch_mean(v[u][0][0], v[u][0][1], v[u][1][0], v[u][1][1]);
+            const float m1 = patch_mean(v[u][0][2], v[u][0][3], v[u][1][2], v[u][1][3]);
+            put(reinterpret_cast<f2v*>(D.tiles + tile_index(xo, yo, D.tiles_x, lg)), f2v{m0, m1});
         }
-        if (COPY) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int64_t ys = 2 * yo + r;
-                if (xs < copy.pad_cols && ys < copy.pad_rows)
-                    *reinterpret_cast<float4*>(copy.tiles + tile_index(xs, ys, copy.tiles_x, lg)) =
-                        make_float4(v[r][0], v[r][1], v[r][2], v[r][3]);
-            }
-        }
-        const float m0 = patch_mean(v[0][0], v[0][1], v[1][0], v[1][1]);
-        const float m1 = patch_mean(v[0][2], v[0][3], v[1][2], v[1][3]);
-        *reinterpret_cast<float2*>(D.tiles + tile_index(xo, yo, D.tiles_x, lg)) = make_float2(m0, m1);
     }
 }
 
@@ -111,9 +158,11 @@ template <bool TILED, bool VEC, bool COPY>
 hipError_t launch_level(const float* src, const CogLevelDev& S, const CogLevelDev& D, const CogLevelDev& copy,
                         int lg, hipStream_t s) {
     const int64_t pairs = D.pad_cols / 2;
-    const unsigned gx = (unsigned)((pairs + 255) / 256);
-    const unsigned gy = (unsigned)(D.pad_rows < 65535 ? D.pad_rows : 65535);
-    hipLaunchKernelGGL((cog_level_kernel<TILED, VEC, COPY>), dim3(gx, gy), dim3(256), 0, s, src, S, D, copy, lg);
+    const int64_t gx = (pairs + 255) / 256;
+    int64_t gy = D.pad_rows < 65535 ? D.pad_rows : 65535;
+    if (kCogWgs > 0) gy = std::min<int64_t>(gy, std::max<int64_t>(1, (kCogWgs + gx - 1) / gx));
+    hipLaunchKernelGGL((cog_level_kernel<TILED, VEC, COPY>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, src,
+                       S, D, copy, lg);
     return hipGetLastError();
 }
 
