@@ -769,6 +769,17 @@ def bench_cog(args, dev):
     # the whole file, device path vs the oracle writer (one thread)
     tmp = pathlib.Path("/dev/shm") if pathlib.Path("/dev/shm").is_dir() else pathlib.Path("/tmp")
     gpu_path, ref_path = tmp / f"sardine_bench_gpu_{os.getpid()}.tif", tmp / f"sardine_bench_ref_{os.getpid()}.tif"
+
+    def digest(p):
+        h = hashlib.sha256()
+        with open(p, "rb") as f:
+            while True:
+                b = f.read(64 << 20)
+                if not b:
+                    break
+                h.update(b)
+        return h.hexdigest()
+
     try:
         api.write_cog_with_crs(gpu_path, data, cols, rows, gt, crs=cfg.crs, utm_zone=cfg.utm_zone, tile_size=ts)
         t = time.perf_counter()
@@ -782,22 +793,26 @@ def bench_cog(args, dev):
             cpu_write_s = time.perf_counter() - t
             assert st == 0, st
 
-        def digest(p):
-            h = hashlib.sha256()
-            with open(p, "rb") as f:
-                while True:
-                    b = f.read(64 << 20)
-                    if not b:
-                        break
-                    h.update(b)
-            return h.hexdigest()
-
         if cpu_write_s is not None:
             same = gpu_path.stat().st_size == ref_path.stat().st_size and digest(gpu_path) == digest(ref_path)
+        # the striped GeoTIFF of the same raster (the pipeline's default output)
+        api.write_geotiff_with_crs(gpu_path, data, cols, rows, gt, crs=cfg.crs, utm_zone=cfg.utm_zone)
+        t = time.perf_counter()
+        api.write_geotiff_with_crs(gpu_path, data, cols, rows, gt, crs=cfg.crs, utm_zone=cfg.utm_zone)
+        tif_gpu_s = time.perf_counter() - t
+        tif_bytes = gpu_path.stat().st_size
+        tif_cpu_s, tif_same = None, None
+        if cpu_write_s is not None:
+            t = time.perf_counter()
+            assert O.write_geotiff(ref_path, host, gt, crs=cfg.crs, utm_zone=cfg.utm_zone) == 0
+            tif_cpu_s = time.perf_counter() - t
+            tif_same = tif_bytes == ref_path.stat().st_size and digest(gpu_path) == digest(ref_path)
     finally:
         for p in (gpu_path, ref_path):
             if p.exists():
                 p.unlink()
+    log(f"striped GeoTIFF {tif_bytes / 1e9:.2f} GB: device path {tif_gpu_s:.2f}s, oracle writer {tif_cpu_s}s, "
+        f"identical={tif_same}")
     fb = lay.file_bytes
     log(f"file {fb / 1e9:.2f} GB: device path {gpu_write_s:.2f}s, oracle writer {cpu_write_s}s, identical={same}")
     px = rows * cols
@@ -815,6 +830,10 @@ def bench_cog(args, dev):
                      "algorithmic_bytes": alg_bytes, "launch_ms": round(pack_ms, 4)},
         "write_file": {"device_path_s": round(gpu_write_s, 3), "GBps": round(fb / gpu_write_s / 1e9, 2),
                        "identical_to_oracle": same, "fs": str(tmp)},
+        "striped_geotiff": {"file_bytes": tif_bytes, "device_path_s": round(tif_gpu_s, 3),
+                            "GBps": round(tif_bytes / tif_gpu_s / 1e9, 2),
+                            "oracle_writer_s": None if tif_cpu_s is None else round(tif_cpu_s, 3),
+                            "identical_to_oracle": tif_same},
         "cpu_baseline": None if cpu_write_s is None else {
             "value": round(px / cpu_write_s / 1e6, 1), "unit": "Mpx/s (whole file write)", "cores": 1,
             "kind": "port",

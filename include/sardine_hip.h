@@ -498,6 +498,37 @@ sardine_status sardine_write_cog_with_crs(const char* path, const float* data /*
                                           int64_t rows, const double geotransform[6], int32_t crs,
                                           int32_t utm_zone, uint32_t tile_size, void* stream);
 
+/* ── Striped GeoTIFF / BigTIFF export (export.rs:139-284, 762-1219) ──────
+ * One IFD, 64 rows per strip, the raster bytes as they lie in memory after
+ * the header; BigTIFF when needs_bigtiff's estimate (export.rs:762-786)
+ * exceeds 2^32-1.  The device raster is streamed to the file through two
+ * pinned chunks (no kernel: the file holds the raster unchanged).  Files are
+ * the reference's byte for byte.  A raster needs at least one row (the
+ * reference's strip arithmetic underflows on zero rows). */
+
+/* The bytes before the raster (*size of them; copied to `out` when it is
+ * non-null and `capacity` holds them).  bytes_per_pixel 4 (float32) or 1
+ * (uint8); geotransform NULL = write_tiff_no_crs's plain TIFF.  Host only. */
+sardine_status sardine_geotiff_header(int64_t cols, int64_t rows, int32_t bytes_per_pixel,
+                                      const double geotransform[6], int32_t crs, int32_t utm_zone, uint8_t* out,
+                                      uint64_t capacity, uint64_t* size);
+
+/* write_geotiff_with_crs (export.rs:139-187): float32 raster [device]. */
+sardine_status sardine_write_geotiff_with_crs(const char* path, const float* data /*[device]*/, int64_t cols,
+                                              int64_t rows, const double geotransform[6], int32_t crs,
+                                              int32_t utm_zone, void* stream);
+
+/* write_geotiff_u8_with_crs (export.rs:204-245): uint8 raster [device], e.g.
+ * the terrain-correction quality mask. */
+sardine_status sardine_write_geotiff_u8_with_crs(const char* path, const uint8_t* data /*[device]*/, int64_t cols,
+                                                 int64_t rows, const double geotransform[6], int32_t crs,
+                                                 int32_t utm_zone, void* stream);
+
+/* write_tiff_no_crs (export.rs:252-284): float32 raster [device], no
+ * GeoTIFF tags (radar-geometry rasters). */
+sardine_status sardine_write_tiff_no_crs(const char* path, const float* data /*[device]*/, int64_t cols,
+                                         int64_t rows, void* stream);
+
 /* ── Kernel timing (diagnostic, no reference counterpart) ─────────────────
  * With profiling on, every compute entry point brackets its main kernel with
  * HIP events on the stream it launches on (without waiting); the duration of

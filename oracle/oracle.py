@@ -75,6 +75,9 @@ def lib():
         L.oracle_to_db_inplace.argtypes = [P, i64, f, P]
         L.oracle_downsample_2x.argtypes = [P, i64, i64, P]
         L.oracle_write_cog.argtypes = [C.c_char_p, P, i64, i64, P, i32, i32, i32]
+        L.oracle_needs_bigtiff.argtypes = [C.c_uint64] * 4
+        L.oracle_geotiff_header.argtypes = [i64, i64, i32, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.oracle_write_geotiff.argtypes = [C.c_char_p, P, i64, i64, i32, P, i32, i32, i32]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
@@ -334,6 +337,37 @@ def cog_header(cols: int, rows: int, geotransform, crs: int = 0, utm_zone: int =
     out = np.empty(n.value, np.uint8)
     st = lib().oracle_cog_header(cols, rows, _ptr(gt), crs, utm_zone, tile_size, _ptr(out), n.value, C.byref(n))
     return st, out.tobytes()
+
+
+def needs_bigtiff(rows: int, cols: int, bpp: int, geo_shorts: int) -> bool:
+    """export.rs:762-786."""
+    return bool(lib().oracle_needs_bigtiff(rows, cols, bpp, geo_shorts))
+
+
+def geotiff_header(cols: int, rows: int, bpp: int = 4, geotransform=None, crs: int = 0, utm_zone: int = 0,
+                   force: int = 0):
+    """(status, bytes before the raster) of the striped writers; geotransform
+    None = write_tiff_no_crs; force 1/2 = the classic / BigTIFF inner writer."""
+    gt = None if geotransform is None else np.ascontiguousarray(geotransform, np.float64)
+    n = C.c_uint64()
+    gp = None if gt is None else _ptr(gt)
+    st = lib().oracle_geotiff_header(cols, rows, bpp, gp, crs, utm_zone, force, None, 0, C.byref(n))
+    if st:
+        return st, None
+    out = np.empty(n.value, np.uint8)
+    st = lib().oracle_geotiff_header(cols, rows, bpp, gp, crs, utm_zone, force, _ptr(out), n.value, C.byref(n))
+    return st, out.tobytes()
+
+
+def write_geotiff(path, data: np.ndarray, geotransform=None, crs: int = 0, utm_zone: int = 0, force: int = 0) -> int:
+    """write_geotiff_with_crs (float32 data), write_geotiff_u8_with_crs (uint8
+    data) or, with geotransform None, write_tiff_no_crs (export.rs:139-284)."""
+    data = np.ascontiguousarray(data)
+    assert data.dtype in (np.float32, np.uint8)
+    rows, cols = data.shape
+    gt = None if geotransform is None else np.ascontiguousarray(geotransform, np.float64)
+    return lib().oracle_write_geotiff(str(path).encode(), _ptr(data) if data.size else None, cols, rows,
+                                      data.dtype.itemsize, None if gt is None else _ptr(gt), crs, utm_zone, force)
 
 
 # ── the full chain on a synthetic scene (host) ──────────────────────────

@@ -27,6 +27,9 @@
 //   downsample_2x / COG writer export.rs:1583-1733 (tile-size and geotransform
 //                              errors, header and 17-entry IFDs, tile data
 //                              round trip, 2x2 means and NaN rule)
+//   striped GeoTIFF writers    export.rs:1276-1580 (geotransform errors, key
+//                              offsets, BigTIFF threshold and layout, u8 tags,
+//                              the no-CRS IFD, UTM keys)
 //   UTM                        published Krüger-series (Karney 2011) values,
 //                              forward/inverse round trip (proj4rs is not
 //                              vendored: parity unpinned below ±1 m)
@@ -1548,6 +1551,191 @@ int oracle_write_cog(const char* path, const float* data, int64_t cols, int64_t 
                 ok = std::fwrite(tile.data(), sizeof(float), tile.size(), f) == tile.size();
             }
     }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : SARDINE_ERR_EXPORT_IO;
+}
+
+}  // extern "C"
+
+// ── striped GeoTIFF / BigTIFF (export.rs:139-284, 762-1219) ─────────────────
+namespace {
+
+// needs_bigtiff (export.rs:762-786), with the reference's saturating sums.
+bool needs_bigtiff(uint64_t rows, uint64_t cols, uint64_t bpp, uint64_t geo_shorts) {
+    const uint64_t prelude = geo_shorts > 0 ? 8 + 198 + 16 + 24 + 48 + geo_shorts * 2 : 8 + 162 + 16;
+    auto sat_mul = [](uint64_t a, uint64_t b) { return (b != 0 && a > UINT64_MAX / b) ? UINT64_MAX : a * b; };
+    auto sat_add = [](uint64_t a, uint64_t b) { return a > UINT64_MAX - b ? UINT64_MAX : a + b; };
+    const uint64_t image = sat_mul(sat_mul(rows, cols), bpp);
+    const uint64_t strips = (rows + 63) / 64;
+    return sat_add(sat_add(prelude, sat_mul(strips, 8)), image) > 0xffffffffull;
+}
+
+// The bytes before the raster of write_geotiff_raw_inner (export.rs:788-1010,
+// classic) or write_bigtiff_raw_inner (export.rs:1023-1219); keys empty = no
+// GeoTIFF tags.  rows >= 1.
+void striped_header(uint64_t cols, uint64_t rows, uint32_t bpp, const double* gt, const std::vector<uint16_t>& keys,
+                    bool big, std::vector<uint8_t>& b) {
+    const bool geo = !keys.empty();
+    const uint16_t bits = (uint16_t)(bpp * 8), sf = bpp == 1 ? 1 : 3;
+    b.clear();
+    if (!big) {
+        const uint16_t n_entries = geo ? 16 : 13;
+        const uint32_t geo_bytes = (uint32_t)keys.size() * 2;
+        const uint32_t n_strips = ((uint32_t)rows + 63u) / 64u;
+        const uint32_t row_bytes = (uint32_t)cols * bpp;
+        const uint32_t full = 64u * row_bytes;
+        const uint32_t last = ((uint32_t)rows - (n_strips - 1u) * 64u) * row_bytes;
+        const uint32_t ifd_size = 2u + n_entries * 12u + 4u;
+        const uint32_t so = 8u + ifd_size, sb = so + n_strips * 4u, xr = sb + n_strips * 4u, yr = xr + 8u;
+        const uint32_t ps = yr + 8u, tp = ps + 24u, gk = tp + 48u, im = geo ? gk + geo_bytes : yr + 8u;
+        b.push_back('I');
+        b.push_back('I');
+        put<uint16_t>(b, 42);
+        put<uint32_t>(b, 8);
+        put<uint16_t>(b, n_entries);
+        auto e = [&](uint16_t tag, uint16_t typ, uint32_t cnt, uint32_t val) {
+            put<uint16_t>(b, tag);
+            put<uint16_t>(b, typ);
+            put<uint32_t>(b, cnt);
+            put<uint32_t>(b, val);
+        };
+        e(256, 4, 1, (uint32_t)cols);
+        e(257, 4, 1, (uint32_t)rows);
+        e(258, 3, 1, bits);
+        e(259, 3, 1, 1);
+        e(262, 3, 1, 1);
+        e(273, 4, n_strips, so);
+        e(277, 3, 1, 1);
+        e(278, 4, 1, 64);
+        e(279, 4, n_strips, sb);
+        e(282, 5, 1, xr);
+        e(283, 5, 1, yr);
+        e(296, 3, 1, 1);
+        e(339, 3, 1, sf);
+        if (geo) {
+            e(33550, 12, 3, ps);
+            e(33922, 12, 6, tp);
+            e(34735, 3, (uint32_t)keys.size(), gk);
+        }
+        put<uint32_t>(b, 0);
+        for (uint32_t i = 0; i < n_strips; ++i) put<uint32_t>(b, (uint32_t)((uint64_t)im + (uint64_t)i * full));
+        for (uint32_t i = 0; i < n_strips; ++i) put<uint32_t>(b, i == n_strips - 1 ? last : full);
+        put<uint32_t>(b, 72);
+        put<uint32_t>(b, 1);
+        put<uint32_t>(b, 72);
+        put<uint32_t>(b, 1);
+    } else {
+        const uint64_t n_entries = geo ? 16 : 13;
+        const uint64_t geo_bytes = keys.size() * 2;
+        const uint64_t n_strips = (rows + 63) / 64;
+        const uint64_t row_bytes = cols * bpp, full = 64 * row_bytes;
+        const uint64_t last = (rows - (n_strips - 1) * 64) * row_bytes;
+        const uint64_t ifd_end = 16 + 8 + n_entries * 20 + 8;
+        const bool inl = n_strips == 1;
+        const uint64_t so = inl ? 0 : ifd_end, sb = inl ? 0 : ifd_end + n_strips * 8;
+        const uint64_t after = inl ? ifd_end : ifd_end + 2 * n_strips * 8;
+        const uint64_t ps = geo ? after : 0, tp = geo ? after + 24 : 0, gk = geo ? after + 72 : 0;
+        const uint64_t im = geo ? gk + geo_bytes : after;
+        b.push_back('I');
+        b.push_back('I');
+        put<uint16_t>(b, 43);
+        put<uint16_t>(b, 8);
+        put<uint16_t>(b, 0);
+        put<uint64_t>(b, 16);
+        put<uint64_t>(b, n_entries);
+        auto e = [&](uint16_t tag, uint16_t typ, uint64_t cnt, uint64_t val) {
+            put<uint16_t>(b, tag);
+            put<uint16_t>(b, typ);
+            put<uint64_t>(b, cnt);
+            put<uint64_t>(b, val);
+        };
+        const uint64_t rat = (1ull << 32) | 72ull;
+        e(256, 4, 1, cols);
+        e(257, 4, 1, rows);
+        e(258, 3, 1, bits);
+        e(259, 3, 1, 1);
+        e(262, 3, 1, 1);
+        e(273, 16, n_strips, inl ? im : so);
+        e(277, 3, 1, 1);
+        e(278, 4, 1, 64);
+        e(279, 16, n_strips, inl ? last : sb);
+        e(282, 5, 1, rat);
+        e(283, 5, 1, rat);
+        e(296, 3, 1, 1);
+        e(339, 3, 1, sf);
+        if (geo) {
+            e(33550, 12, 3, ps);
+            e(33922, 12, 6, tp);
+            e(34735, 3, keys.size(), gk);
+        }
+        put<uint64_t>(b, 0);
+        if (!inl) {
+            for (uint64_t i = 0; i < n_strips; ++i) put<uint64_t>(b, im + i * full);
+            for (uint64_t i = 0; i < n_strips; ++i) put<uint64_t>(b, i == n_strips - 1 ? last : full);
+        }
+    }
+    if (geo) {
+        put<double>(b, gt[1]);
+        put<double>(b, -gt[5]);
+        put<double>(b, 0.0);
+        put<double>(b, 0.0);
+        put<double>(b, 0.0);
+        put<double>(b, 0.0);
+        put<double>(b, gt[0]);
+        put<double>(b, gt[3]);
+        put<double>(b, 0.0);
+        for (uint16_t k : keys) put<uint16_t>(b, k);
+    }
+}
+
+// gt == null: write_tiff_no_crs; else write_geotiff(_u8)_with_crs.  force:
+// 0 = the reference's choice, 1 = classic, 2 = BigTIFF (the inner writers,
+// as the reference's tests call them).
+int striped_plan(int64_t cols, int64_t rows, int32_t bpp, const double* gt, int crs, int zone, int force,
+                 std::vector<uint8_t>* head) {
+    if (gt) {
+        const double pw = gt[1], ph = -gt[5];
+        if (!std::isfinite(pw) || pw <= 0.0 || !std::isfinite(ph) || ph <= 0.0)
+            return SARDINE_ERR_EXPORT_INVALID_GEOTRANSFORM;
+    }
+    if (rows <= 0 || cols < 0 || (bpp != 1 && bpp != 4)) return SARDINE_ERR_INVALID_ARGUMENT;
+    const std::vector<uint16_t> keys = gt ? geo_key_directory(crs, zone) : std::vector<uint16_t>();
+    const bool big = force == 2 || (force == 0 && needs_bigtiff((uint64_t)rows, (uint64_t)cols, (uint64_t)bpp, keys.size()));
+    striped_header((uint64_t)cols, (uint64_t)rows, (uint32_t)bpp, gt, keys, big, *head);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_needs_bigtiff(uint64_t rows, uint64_t cols, uint64_t bpp, uint64_t geo_shorts) {
+    return needs_bigtiff(rows, cols, bpp, geo_shorts) ? 1 : 0;
+}
+
+// The bytes before the raster; *size = their count (copied when cap allows).
+int oracle_geotiff_header(int64_t cols, int64_t rows, int32_t bpp, const double* gt, int crs, int zone, int force,
+                          uint8_t* out, uint64_t cap, uint64_t* size) {
+    std::vector<uint8_t> h;
+    const int st = striped_plan(cols, rows, bpp, gt, crs, zone, force, &h);
+    if (st) return st;
+    *size = h.size();
+    if (out && cap >= h.size()) std::memcpy(out, h.data(), h.size());
+    return 0;
+}
+
+// write_geotiff_with_crs / write_geotiff_u8_with_crs / write_tiff_no_crs:
+// the header, then the raster bytes as they lie in memory.
+int oracle_write_geotiff(const char* path, const void* data, int64_t cols, int64_t rows, int32_t bpp, const double* gt,
+                         int crs, int zone, int force) {
+    std::vector<uint8_t> h;
+    const int st = striped_plan(cols, rows, bpp, gt, crs, zone, force, &h);
+    if (st) return st;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return SARDINE_ERR_EXPORT_IO;
+    const size_t n = (size_t)cols * (size_t)rows * (size_t)bpp;
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    if (ok && n) ok = std::fwrite(data, 1, n, f) == n;
     ok = (std::fclose(f) == 0) && ok;
     return ok ? 0 : SARDINE_ERR_EXPORT_IO;
 }

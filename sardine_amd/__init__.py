@@ -8,8 +8,9 @@ from . import abi
 from .api import (DemRaster, GeocodedImage, MergedSigma0, ResamplingKernel, SceneInputs, Sigma0Array,
                   SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_speckle_filter,
                   apply_speckle_step, cog_header, cog_layout, cog_pack, deburst_layout, merged_layout,
-                  output_grid, prepare_merged_scene, synth_cint16, terrain_correction, to_db_inplace,
-                  write_cog_with_crs)
+                  geotiff_header, output_grid, prepare_merged_scene, synth_cint16, terrain_correction,
+                  to_db_inplace, write_cog_with_crs, write_geotiff_u8_with_crs, write_geotiff_with_crs,
+                  write_tiff_no_crs)
 from .errors import (CalibrationError, DeburstError, DeviceError, ExportError, MergeError, SardineError,
                      SpeckleError, TerrainCorrectionError)
 
@@ -18,6 +19,7 @@ __all__ = [
     "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_speckle_filter", "apply_speckle_step",
     "cog_header", "cog_layout", "cog_pack", "deburst_layout", "merged_layout", "output_grid",
     "prepare_merged_scene", "synth_cint16", "terrain_correction", "to_db_inplace", "write_cog_with_crs",
+    "geotiff_header", "write_geotiff_with_crs", "write_geotiff_u8_with_crs", "write_tiff_no_crs",
     "CalibrationError", "DeburstError", "DeviceError", "ExportError", "MergeError", "SardineError",
     "SpeckleError", "TerrainCorrectionError",
 ]

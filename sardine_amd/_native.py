@@ -57,6 +57,10 @@ def lib():
     L.sardine_cog_header.argtypes = [C.POINTER(abi.CogLayout), P, i32, i32, P, u64]
     L.sardine_cog_pack.argtypes = [P, C.POINTER(abi.CogLayout), P, P]
     L.sardine_write_cog_with_crs.argtypes = [C.c_char_p, P, i64, i64, P, i32, i32, C.c_uint32, P]
+    L.sardine_geotiff_header.argtypes = [i64, i64, i32, P, i32, i32, P, u64, C.POINTER(u64)]
+    L.sardine_write_geotiff_with_crs.argtypes = [C.c_char_p, P, i64, i64, P, i32, i32, P]
+    L.sardine_write_geotiff_u8_with_crs.argtypes = [C.c_char_p, P, i64, i64, P, i32, i32, P]
+    L.sardine_write_tiff_no_crs.argtypes = [C.c_char_p, P, i64, i64, P]
     L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]
     L.sardine_set_profiling.argtypes = [C.c_int]
     L.sardine_set_profiling.restype = None

@@ -237,6 +237,10 @@ EXPORTED_SYMBOLS = (
     "sardine_cog_header",
     "sardine_cog_pack",
     "sardine_write_cog_with_crs",
+    "sardine_geotiff_header",
+    "sardine_write_geotiff_with_crs",
+    "sardine_write_geotiff_u8_with_crs",
+    "sardine_write_tiff_no_crs",
     "sardine_synth_cint16",
     "sardine_set_profiling",
     "sardine_last_kernel_ms",

@@ -14,6 +14,8 @@ Names and argument meaning follow the Rust crate:
   to_db_inplace(data, noise_floor_linear)    export.rs:59-94
   write_cog_with_crs(path, data, cols, rows, geotransform, crs, tile_size)
         export.rs:329-354 (cog_layout / cog_header / cog_pack expose its parts)
+  write_geotiff_with_crs / write_geotiff_u8_with_crs / write_tiff_no_crs
+        export.rs:139-284 (geotiff_header exposes the bytes before the raster)
 Rasters are torch tensors on the GPU (torch is used only to own device
 memory and streams); every computation runs in libsardine_hip.so.
 """
@@ -411,6 +413,47 @@ def write_cog_with_crs(path, data, cols: int, rows: int, geotransform, crs: int 
     gt = (C.c_double * 6)(*geotransform)
     _check(lib().sardine_write_cog_with_crs(str(path).encode(), data.data_ptr(), cols, rows, gt, crs, utm_zone,
                                             tile_size, _stream_ptr(stream)))
+
+
+def geotiff_header(cols: int, rows: int, bytes_per_pixel: int = 4, geotransform=None, crs: int = abi.CRS_WGS84,
+                   utm_zone: int = 0) -> bytes:
+    """The bytes before the raster of the striped writers (export.rs:788-1219);
+    geotransform None = write_tiff_no_crs's plain TIFF."""
+    gt = None if geotransform is None else (C.c_double * 6)(*geotransform)
+    n = C.c_uint64()
+    _check(lib().sardine_geotiff_header(cols, rows, bytes_per_pixel, gt, crs, utm_zone, None, 0, C.byref(n)))
+    buf = (C.c_uint8 * n.value)()
+    _check(lib().sardine_geotiff_header(cols, rows, bytes_per_pixel, gt, crs, utm_zone, buf, n.value, C.byref(n)))
+    return bytes(buf)
+
+
+def _raster_arg(data, cols, rows, dtype):
+    if data.dtype != dtype or data.numel() != cols * rows or not data.is_contiguous():
+        raise ValueError(f"data must be a contiguous {dtype} raster of rows x cols")
+
+
+def write_geotiff_with_crs(path, data, cols: int, rows: int, geotransform, crs: int = abi.CRS_WGS84,
+                           utm_zone: int = 0, stream=None):
+    """write_geotiff_with_crs (export.rs:139-187) of a float32 device raster."""
+    _raster_arg(data, cols, rows, _torch().float32)
+    _check(lib().sardine_write_geotiff_with_crs(str(path).encode(), data.data_ptr(), cols, rows,
+                                                (C.c_double * 6)(*geotransform), crs, utm_zone,
+                                                _stream_ptr(stream)))
+
+
+def write_geotiff_u8_with_crs(path, data, cols: int, rows: int, geotransform, crs: int = abi.CRS_WGS84,
+                              utm_zone: int = 0, stream=None):
+    """write_geotiff_u8_with_crs (export.rs:204-245) of a uint8 device raster."""
+    _raster_arg(data, cols, rows, _torch().uint8)
+    _check(lib().sardine_write_geotiff_u8_with_crs(str(path).encode(), data.data_ptr(), cols, rows,
+                                                   (C.c_double * 6)(*geotransform), crs, utm_zone,
+                                                   _stream_ptr(stream)))
+
+
+def write_tiff_no_crs(path, data, cols: int, rows: int, stream=None):
+    """write_tiff_no_crs (export.rs:252-284) of a float32 device raster."""
+    _raster_arg(data, cols, rows, _torch().float32)
+    _check(lib().sardine_write_tiff_no_crs(str(path).encode(), data.data_ptr(), cols, rows, _stream_ptr(stream)))
 
 
 def synth_cint16(out, seed: int, index_base: int = 0, sigma: float = 100.0, stream=None):

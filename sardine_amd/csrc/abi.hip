@@ -1562,6 +1562,192 @@ struct FileCloser {
     }
 };
 
+// Device bytes → file: two pinned chunks in turn, each written while the
+// next one is copied.
+sardine_status stream_to_file(std::FILE* f, const void* dev, uint64_t bytes, hipStream_t s, const char* path) {
+    const uint64_t chunk = std::min<uint64_t>(bytes, 32ull << 20);
+    if (chunk == 0) return SARDINE_OK;
+    PinnedPair pp;
+    for (int i = 0; i < 2; ++i) {
+        SD_HIP(hipHostMalloc(&pp.p[i], chunk, hipHostMallocDefault));
+        SD_HIP(hipEventCreateWithFlags(&pp.ev[i], hipEventDisableTiming));
+    }
+    const uint8_t* src = static_cast<const uint8_t*>(dev);
+    const uint64_t n_chunks = (bytes + chunk - 1) / chunk;
+    auto issue = [&](uint64_t i) -> hipError_t {
+        const uint64_t off = i * chunk, n = std::min(chunk, bytes - off);
+        hipError_t e = hipMemcpyAsync(pp.p[i % 2], src + off, n, hipMemcpyDeviceToHost, s);
+        return e != hipSuccess ? e : hipEventRecord(pp.ev[i % 2], s);
+    };
+    SD_HIP(issue(0));
+    for (uint64_t i = 0; i < n_chunks; ++i) {
+        SD_HIP(hipEventSynchronize(pp.ev[i % 2]));
+        if (i + 1 < n_chunks) SD_HIP(issue(i + 1));
+        const uint64_t n = std::min(chunk, bytes - i * chunk);
+        if (std::fwrite(pp.p[i % 2], 1, n, f) != n)
+            return fail(SARDINE_ERR_EXPORT_IO, "writing %s: %s", path, std::strerror(errno));
+    }
+    return SARDINE_OK;
+}
+
+// needs_bigtiff (export.rs:762-786): the classic layout's size estimate
+// (header, 16- or 13-entry IFD, resolution rationals, GeoTIFF blocks, an
+// 8-byte strip table entry per 64 rows, the raster) above 2^32 - 1, with the
+// reference's saturating arithmetic.
+bool striped_needs_bigtiff(uint64_t rows, uint64_t cols, uint64_t bpp, uint64_t geo_shorts) {
+    auto mul = [](uint64_t a, uint64_t b) { return (b != 0 && a > UINT64_MAX / b) ? UINT64_MAX : a * b; };
+    auto add = [](uint64_t a, uint64_t b) { return a > UINT64_MAX - b ? UINT64_MAX : a + b; };
+    const uint64_t fixed = geo_shorts > 0 ? 8 + 198 + 16 + 24 + 48 + geo_shorts * 2 : 8 + 162 + 16;
+    const uint64_t total = add(add(fixed, mul((rows + 63) / 64, 8)), mul(mul(rows, cols), bpp));
+    return total > 0xffffffffull;
+}
+
+// The bytes before the raster of a single-IFD striped TIFF, 64 rows per
+// strip: write_geotiff_raw_inner (classic, export.rs:788-1010) or
+// write_bigtiff_raw_inner (export.rs:1023-1219).  keys empty: no GeoTIFF
+// tags (write_tiff_no_crs).  rows >= 1.  Classic offsets are 32-bit
+// arithmetic, as the reference computes them.
+void striped_header_bytes(uint64_t cols, uint64_t rows, uint32_t bpp, const double* gt,
+                          const std::vector<uint16_t>& keys, bool big, std::vector<uint8_t>& b) {
+    const bool geo = !keys.empty();
+    const uint64_t n_entries = geo ? 16 : 13;
+    const uint64_t bits = bpp * 8ull, fmt = bpp == 1 ? 1 : 3;  // SampleFormat: unsigned int / IEEE float
+    b.clear();
+    b.push_back('I');
+    b.push_back('I');
+    uint64_t strips = 0, full = 0, last = 0, arr_off = 0, cnt_off = 0, data_off = 0;
+    uint64_t res_off = 0, scale_off = 0, tie_off = 0, key_off = 0;
+    if (big) {
+        strips = (rows + 63) / 64;
+        full = 64 * cols * bpp;
+        last = (rows - (strips - 1) * 64) * cols * bpp;
+        const uint64_t ifd_end = 16 + 8 + n_entries * 20 + 8;
+        uint64_t after = ifd_end;  // one strip: its offset and count sit inline in the IFD
+        if (strips > 1) {
+            arr_off = ifd_end;
+            cnt_off = ifd_end + strips * 8;
+            after = ifd_end + 2 * strips * 8;
+        }
+        if (geo) {
+            scale_off = after;
+            tie_off = after + 24;
+            key_off = after + 72;
+            data_off = key_off + keys.size() * 2;
+        } else {
+            data_off = after;
+        }
+        put_le<uint16_t>(b, 43);
+        put_le<uint16_t>(b, 8);
+        put_le<uint16_t>(b, 0);
+        put_le<uint64_t>(b, 16);
+        put_le<uint64_t>(b, n_entries);
+    } else {
+        const uint32_t r32 = (uint32_t)rows, c32 = (uint32_t)cols;
+        const uint32_t st32 = (r32 + 63u) / 64u, row32 = c32 * bpp;
+        strips = st32;
+        full = 64u * row32;
+        last = (uint32_t)((r32 - (st32 - 1u) * 64u) * row32);
+        arr_off = 8u + (2u + (uint32_t)n_entries * 12u + 4u);
+        cnt_off = (uint32_t)arr_off + st32 * 4u;
+        res_off = (uint32_t)cnt_off + st32 * 4u;
+        scale_off = (uint32_t)res_off + 16u;
+        tie_off = scale_off + 24;
+        key_off = tie_off + 48;
+        data_off = geo ? (uint32_t)(key_off + keys.size() * 2) : (uint32_t)res_off + 16u;
+        put_le<uint16_t>(b, 42);
+        put_le<uint32_t>(b, 8);
+        put_le<uint16_t>(b, (uint16_t)n_entries);
+    }
+    auto entry = [&](uint16_t tag, uint16_t type, uint64_t count, uint64_t value) {
+        put_le<uint16_t>(b, tag);
+        put_le<uint16_t>(b, type);
+        if (big) {
+            put_le<uint64_t>(b, count);
+            put_le<uint64_t>(b, value);
+        } else {
+            put_le<uint32_t>(b, (uint32_t)count);
+            put_le<uint32_t>(b, (uint32_t)value);
+        }
+    };
+    const uint64_t rat = (1ull << 32) | 72ull;  // RATIONAL 72/1 inline (BigTIFF)
+    entry(256, 4, 1, cols);
+    entry(257, 4, 1, rows);
+    entry(258, 3, 1, bits);
+    entry(259, 3, 1, 1);
+    entry(262, 3, 1, 1);
+    entry(273, big ? 16 : 4, strips, big ? (strips > 1 ? arr_off : data_off) : arr_off);  // StripOffsets
+    entry(277, 3, 1, 1);
+    entry(278, 4, 1, 64);                                                            // RowsPerStrip
+    entry(279, big ? 16 : 4, strips, big ? (strips > 1 ? cnt_off : last) : cnt_off);  // StripByteCounts
+    entry(282, 5, 1, big ? rat : res_off);
+    entry(283, 5, 1, big ? rat : res_off + 8);
+    entry(296, 3, 1, 1);
+    entry(339, 3, 1, fmt);
+    if (geo) {
+        entry(33550, 12, 3, scale_off);
+        entry(33922, 12, 6, tie_off);
+        entry(34735, 3, keys.size(), key_off);
+    }
+    if (big) put_le<uint64_t>(b, 0);
+    else put_le<uint32_t>(b, 0);
+    if (big) {
+        if (strips > 1) {
+            for (uint64_t i = 0; i < strips; ++i) put_le<uint64_t>(b, data_off + i * full);
+            for (uint64_t i = 0; i < strips; ++i) put_le<uint64_t>(b, i + 1 == strips ? last : full);
+        }
+    } else {
+        for (uint64_t i = 0; i < strips; ++i) put_le<uint32_t>(b, (uint32_t)(data_off + i * (uint64_t)(uint32_t)full));
+        for (uint64_t i = 0; i < strips; ++i) put_le<uint32_t>(b, (uint32_t)(i + 1 == strips ? last : full));
+        for (int i = 0; i < 2; ++i) {
+            put_le<uint32_t>(b, 72);
+            put_le<uint32_t>(b, 1);
+        }
+    }
+    if (geo) {
+        const double blocks[9] = {gt[1], -gt[5], 0.0, 0.0, 0.0, 0.0, gt[0], gt[3], 0.0};
+        for (double v : blocks) put_le<double>(b, v);
+        for (uint16_t k : keys) put_le<uint16_t>(b, k);
+    }
+}
+
+// Validation and header of the striped writers: gt null = no CRS.
+sardine_status striped_plan(int64_t cols, int64_t rows, int32_t bpp, const double* gt, int32_t crs, int32_t zone,
+                            std::vector<uint8_t>* head) {
+    sardine_status st;
+    std::vector<uint16_t> keys;
+    if (gt) {
+        if ((st = cog_check_geotransform(gt))) return st;
+        if ((st = cog_geo_keys(crs, zone, &keys))) return st;
+    }
+    if (bpp != 1 && bpp != 4) return fail(SARDINE_ERR_INVALID_ARGUMENT, "bytes per pixel must be 1 or 4");
+    // rows == 0 underflows the reference's strip arithmetic (export.rs:861, 1061)
+    if (rows <= 0 || cols < 0) return fail(SARDINE_ERR_INVALID_ARGUMENT, "raster needs at least one row");
+    const bool big = striped_needs_bigtiff((uint64_t)rows, (uint64_t)cols, (uint64_t)bpp, keys.size());
+    striped_header_bytes((uint64_t)cols, (uint64_t)rows, (uint32_t)bpp, gt, keys, big, *head);
+    return SARDINE_OK;
+}
+
+sardine_status striped_write(const char* path, const void* data, int64_t cols, int64_t rows, int32_t bpp,
+                             const double* gt, int32_t crs, int32_t zone, void* stream_v) {
+    sardine_status st;
+    std::vector<uint8_t> head;
+    if ((st = striped_plan(cols, rows, bpp, gt, crs, zone, &head))) return st;
+    if (!path) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path is null");
+    const uint64_t n = (uint64_t)cols * (uint64_t)rows * (uint64_t)bpp;
+    if (n > 0 && !data) return fail(SARDINE_ERR_INVALID_ARGUMENT, "data is null");
+    if ((st = require_device())) return st;
+    FileCloser fc;
+    fc.f = std::fopen(path, "wb");
+    if (!fc.f) return fail(SARDINE_ERR_EXPORT_IO, "cannot create %s: %s", path, std::strerror(errno));
+    if (std::fwrite(head.data(), 1, head.size(), fc.f) != head.size())
+        return fail(SARDINE_ERR_EXPORT_IO, "writing %s: %s", path, std::strerror(errno));
+    if ((st = stream_to_file(fc.f, data, n, (hipStream_t)stream_v, path))) return st;
+    const int rc = std::fclose(fc.f);
+    fc.f = nullptr;
+    if (rc != 0) return fail(SARDINE_ERR_EXPORT_IO, "closing %s: %s", path, std::strerror(errno));
+    return SARDINE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1604,6 +1790,39 @@ sardine_status sardine_cog_pack(const float* data, const sardine_cog_layout* lay
     return cog_pack(data, L, tiles, (hipStream_t)stream);
 }
 
+sardine_status sardine_geotiff_header(int64_t cols, int64_t rows, int32_t bytes_per_pixel, const double geotransform[6],
+                                     int32_t crs, int32_t utm_zone, uint8_t* out, uint64_t capacity, uint64_t* size) {
+    sardine_status st;
+    std::vector<uint8_t> head;
+    if ((st = striped_plan(cols, rows, bytes_per_pixel, geotransform, crs, utm_zone, &head))) return st;
+    if (size) *size = head.size();
+    if (out) {
+        if (capacity < head.size())
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "header needs %zu bytes", head.size());
+        std::memcpy(out, head.data(), head.size());
+    }
+    return SARDINE_OK;
+}
+
+sardine_status sardine_write_geotiff_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+                                              const double geotransform[6], int32_t crs, int32_t utm_zone,
+                                              void* stream) {
+    if (!geotransform) return fail(SARDINE_ERR_INVALID_ARGUMENT, "geotransform is null");
+    return striped_write(path, data, cols, rows, 4, geotransform, crs, utm_zone, stream);
+}
+
+sardine_status sardine_write_geotiff_u8_with_crs(const char* path, const uint8_t* data, int64_t cols, int64_t rows,
+                                                 const double geotransform[6], int32_t crs, int32_t utm_zone,
+                                                 void* stream) {
+    if (!geotransform) return fail(SARDINE_ERR_INVALID_ARGUMENT, "geotransform is null");
+    return striped_write(path, data, cols, rows, 1, geotransform, crs, utm_zone, stream);
+}
+
+sardine_status sardine_write_tiff_no_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+                                         void* stream) {
+    return striped_write(path, data, cols, rows, 4, nullptr, 0, 0, stream);
+}
+
 sardine_status sardine_write_cog_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
                                           const double geotransform[6], int32_t crs, int32_t utm_zone,
                                           uint32_t tile_size, void* stream_v) {
@@ -1631,31 +1850,7 @@ sardine_status sardine_write_cog_with_crs(const char* path, const float* data, i
     if (!fc.f) return fail(SARDINE_ERR_EXPORT_IO, "cannot create %s: %s", path, std::strerror(errno));
     if (std::fwrite(head.data(), 1, head.size(), fc.f) != head.size())
         return fail(SARDINE_ERR_EXPORT_IO, "writing %s: %s", path, std::strerror(errno));
-    // Tiles: device → two pinned chunks in turn, each written while the
-    // next is copied.
-    const uint64_t chunk = std::min<uint64_t>(L.tile_bytes, 32ull << 20);
-    if (chunk > 0) {
-        PinnedPair pp;
-        for (int i = 0; i < 2; ++i) {
-            SD_HIP(hipHostMalloc(&pp.p[i], chunk, hipHostMallocDefault));
-            SD_HIP(hipEventCreateWithFlags(&pp.ev[i], hipEventDisableTiming));
-        }
-        const uint8_t* src = tiles.as<uint8_t>();
-        const uint64_t n_chunks = (L.tile_bytes + chunk - 1) / chunk;
-        auto issue = [&](uint64_t i) -> hipError_t {
-            const uint64_t off = i * chunk, n = std::min(chunk, L.tile_bytes - off);
-            hipError_t e = hipMemcpyAsync(pp.p[i % 2], src + off, n, hipMemcpyDeviceToHost, s);
-            return e != hipSuccess ? e : hipEventRecord(pp.ev[i % 2], s);
-        };
-        SD_HIP(issue(0));
-        for (uint64_t i = 0; i < n_chunks; ++i) {
-            SD_HIP(hipEventSynchronize(pp.ev[i % 2]));
-            if (i + 1 < n_chunks) SD_HIP(issue(i + 1));
-            const uint64_t n = std::min(chunk, L.tile_bytes - i * chunk);
-            if (std::fwrite(pp.p[i % 2], 1, n, fc.f) != n)
-                return fail(SARDINE_ERR_EXPORT_IO, "writing %s: %s", path, std::strerror(errno));
-        }
-    }
+    if ((st = stream_to_file(fc.f, tiles.p, L.tile_bytes, s, path))) return st;
     const int rc = std::fclose(fc.f);
     fc.f = nullptr;
     if (rc != 0) return fail(SARDINE_ERR_EXPORT_IO, "closing %s: %s", path, std::strerror(errno));

@@ -286,3 +286,48 @@ def test_cog_argument_errors_precede_device_check(tmp_path):
         assert L.sardine_write_cog_with_crs(p, d.ctypes.data, 2, 2, gt, 0, 0, 64, None) == abi.ERR_NO_DEVICE
         assert L.sardine_cog_pack(d.ctypes.data, C.byref(lay), d.ctypes.data, None) == abi.ERR_NO_DEVICE
         assert not (tmp_path / "x.tiff").exists()
+
+
+@pytest.mark.parametrize("cols,rows,bpp,crs,zone", [
+    (2, 2, 4, abi.CRS_WGS84, 0), (5, 7, 4, abi.CRS_UTM_NORTH, 32), (7, 65, 1, abi.CRS_UTM_SOUTH, 1),
+    (33, 200, 4, None, 0), (0, 3, 4, abi.CRS_WGS84, 0), (1, 1, 1, None, 0),
+    (33_000, 33_000, 4, abi.CRS_WGS84, 0), (66_000, 66_000, 1, abi.CRS_UTM_NORTH, 60), (33_000, 33_000, 1, None, 0),
+    (100_000, 20_000, 4, None, 0), (1_000_000, 64, 4, abi.CRS_WGS84, 0), (16_777_216, 64, 4, abi.CRS_WGS84, 0),
+    (16_777_000, 1, 4, abi.CRS_UTM_SOUTH, 7)])
+def test_geotiff_header_matches_oracle(cols, rows, bpp, crs, zone):
+    """The striped writers' header bytes (classic and BigTIFF, float32 and
+    uint8, with and without GeoTIFF tags) are host code, compared with the
+    oracle without a device."""
+    gt = None if crs is None else ([500000.0, 10.0, 0.0, 5.2e6, 0.0, -10.0] if crs else COG_GT)
+    head = api.geotiff_header(cols, rows, bpp, gt, crs or 0, zone)
+    st, ref = O.geotiff_header(cols, rows, bpp, gt, crs or 0, zone)
+    assert st == 0 and head == ref
+    big = O.needs_bigtiff(rows, cols, bpp, 0 if crs is None else (16 if crs == 0 else 20))
+    assert head[2] == (43 if big else 42)
+
+
+def test_geotiff_argument_errors_precede_device_check(tmp_path):
+    L = _native.lib()
+    d = np.ones(4, np.float32)
+    gt = (C.c_double * 6)(*COG_GT)
+    p = str(tmp_path / "g.tif").encode()
+    for bad in ((8.0, 0.0, 0.0, 51.0, 0.0, -1e-4), (8.0, 1e-4, 0.0, 51.0, 0.0, 1e-4)):
+        bgt = (C.c_double * 6)(*bad)
+        assert L.sardine_write_geotiff_with_crs(p, d.ctypes.data, 2, 2, bgt, 0, 0, None) == \
+            abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+        assert L.sardine_write_geotiff_u8_with_crs(p, d.ctypes.data, 2, 2, bgt, 0, 0, None) == \
+            abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+    assert L.sardine_write_geotiff_with_crs(p, d.ctypes.data, 2, 0, gt, 0, 0, None) == abi.ERR_INVALID_ARGUMENT
+    assert L.sardine_write_geotiff_with_crs(p, d.ctypes.data, 2, 2, gt, abi.CRS_UTM_SOUTH, 0, None) == \
+        abi.ERR_INVALID_ARGUMENT
+    n = C.c_uint64()
+    assert L.sardine_geotiff_header(2, 2, 2, gt, 0, 0, None, 0, C.byref(n)) == abi.ERR_INVALID_ARGUMENT
+    assert L.sardine_geotiff_header(2, 2, 4, gt, 0, 0, None, 0, C.byref(n)) == 0 and n.value == 334
+    buf = (C.c_uint8 * 16)()
+    assert L.sardine_geotiff_header(2, 2, 4, gt, 0, 0, buf, 16, C.byref(n)) == abi.ERR_INVALID_ARGUMENT
+    import torch
+
+    if not torch.cuda.is_available():
+        assert L.sardine_write_geotiff_with_crs(p, d.ctypes.data, 2, 2, gt, 0, 0, None) == abi.ERR_NO_DEVICE
+        assert L.sardine_write_tiff_no_crs(p, d.ctypes.data, 2, 2, None) == abi.ERR_NO_DEVICE
+        assert not (tmp_path / "g.tif").exists()

@@ -142,3 +142,96 @@ def test_cog_write_errors(cuda, tmp_path):
     with pytest.raises(Exception) as e:
         api.write_cog_with_crs(tmp_path / "no_such_dir" / "a.tif", d, 2, 2, GT_WGS, tile_size=64)
     assert e.value.status == abi.ERR_EXPORT_IO
+
+
+# ── striped GeoTIFF / BigTIFF writers (export.rs:139-284, 762-1219) ──────────
+@pytest.mark.parametrize("rows,cols,kind,crs,zone", [
+    (2, 2, "f32", abi.CRS_WGS84, 0), (64, 5, "f32", abi.CRS_UTM_NORTH, 32), (65, 7, "u8", abi.CRS_UTM_SOUTH, 1),
+    (200, 333, "none", 0, 0), (1, 1, "u8", abi.CRS_WGS84, 0), (3, 0, "f32", abi.CRS_WGS84, 0),
+    (1000, 1537, "f32", abi.CRS_UTM_NORTH, 60)])
+def test_geotiff_file_bytes_match_oracle(cuda, tmp_path, rows, cols, kind, crs, zone):
+    import torch
+
+    print(f"geotiff parity rows={rows} cols={cols} kind={kind} crs={crs}", flush=True)
+    rng = np.random.default_rng(rows * 3 + cols)
+    data = rng.integers(0, 256, (rows, cols)).astype(np.uint8) if kind == "u8" else _raster(rows, cols, rows + cols)
+    gt = GT_UTM if crs else GT_WGS
+    ref_path, out_path = tmp_path / "ref.tif", tmp_path / "gpu.tif"
+    dev = torch.from_numpy(data).to(cuda)
+    if kind == "none":
+        assert O.write_geotiff(ref_path, data, None) == 0
+        api.write_tiff_no_crs(out_path, dev, cols, rows)
+    elif kind == "u8":
+        assert O.write_geotiff(ref_path, data, gt, crs=crs, utm_zone=zone) == 0
+        api.write_geotiff_u8_with_crs(out_path, dev, cols, rows, gt, crs=crs, utm_zone=zone)
+    else:
+        assert O.write_geotiff(ref_path, data, gt, crs=crs, utm_zone=zone) == 0
+        api.write_geotiff_with_crs(out_path, dev, cols, rows, gt, crs=crs, utm_zone=zone)
+    a, b = out_path.read_bytes(), ref_path.read_bytes()
+    assert a == b, _first_diff(a, b)
+
+
+def test_geotiff_of_geocoded_product_and_mask(cuda, tmp_path):
+    """The pipeline's default export (run.rs:443-451): the geocoded dB raster
+    and its uint8 quality mask as striped GeoTIFFs."""
+    sc = synthetic.scene_small_plain(6, lines=512, samples=1024, out_rows=260, out_cols=310, dem_side=512)
+    ds = DeviceScene.build(sc, cuda, host_raw=True)
+    img = ds.terrain_correction(ds.prepare_merged(), ds.tc_config(compute_lia=True))
+    api.to_db_inplace(img.data, 0.0)
+    gt = list(img.geotransform)
+    for name, dev in (("db", img.data.reshape(img.rows, img.cols).contiguous()),
+                      ("mask", img.mask.reshape(img.rows, img.cols).contiguous())):
+        host = dev.cpu().numpy()
+        ref_path, out_path = tmp_path / f"{name}_ref.tif", tmp_path / f"{name}.tif"
+        assert O.write_geotiff(ref_path, host, gt) == 0
+        if name == "db":
+            api.write_geotiff_with_crs(out_path, dev, img.cols, img.rows, gt)
+        else:
+            api.write_geotiff_u8_with_crs(out_path, dev, img.cols, img.rows, gt)
+        a, b = out_path.read_bytes(), ref_path.read_bytes()
+        assert a == b, (name, _first_diff(a, b))
+
+
+def test_bigtiff_file_matches_oracle(cuda):
+    """A raster above the 4 GiB classic-TIFF limit (32768 x 32768 float32):
+    the device path's BigTIFF equals the oracle's (compared by digest; the
+    files live in /dev/shm and are removed)."""
+    import hashlib
+    import os
+    import pathlib
+
+    import torch
+
+    rows = cols = 32768
+    shm = pathlib.Path("/dev/shm")
+    if not shm.is_dir():
+        pytest.skip("no /dev/shm")
+    print("bigtiff parity 32768x32768 f32", flush=True)
+    dev = torch.arange(rows * cols, dtype=torch.int32, device=cuda).mul_(2654435761 % (1 << 31)).view(
+        torch.float32).reshape(rows, cols)
+    ref_path, out_path = shm / f"sd_big_ref_{os.getpid()}.tif", shm / f"sd_big_gpu_{os.getpid()}.tif"
+
+    def digest(p):
+        h = hashlib.sha256()
+        with open(p, "rb") as f:
+            while True:
+                b = f.read(64 << 20)
+                if not b:
+                    return h.hexdigest()
+                h.update(b)
+
+    try:
+        api.write_geotiff_with_crs(out_path, dev, cols, rows, GT_WGS)
+        host = dev.cpu().numpy()
+        del dev
+        assert O.write_geotiff(ref_path, host, GT_WGS) == 0
+        del host
+        with open(out_path, "rb") as f:
+            head = f.read(16)
+        assert head[:4] == b"II+\x00"
+        assert out_path.stat().st_size == ref_path.stat().st_size
+        assert digest(out_path) == digest(ref_path)
+    finally:
+        for p in (ref_path, out_path):
+            if p.exists():
+                p.unlink()

@@ -794,6 +794,111 @@ def test_cog_header_bigtiff_choice():
     assert len(g[324][1]) == 64 * 64 and g[325][1] == [512 * 512 * 4] * 4096
 
 
+# ── striped GeoTIFF writers (export.rs:1276-1580) ─────────────────────────
+def test_write_geotiff_rejects_invalid_geotransform(tmp_path):
+    d = np.ones((2, 2), np.float32)  # export.rs:1278-1291
+    for gt in ([8.0, 0.0, 0.0, 51.0, 0.0, -0.0001], [8.0, 0.0001, 0.0, 51.0, 0.0, 0.0001]):
+        assert O.write_geotiff(tmp_path / "g.tif", d, gt) == abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+        assert O.write_geotiff(tmp_path / "g.tif", d.astype(np.uint8), gt) == abi.ERR_EXPORT_INVALID_GEOTRANSFORM
+
+
+def test_write_geotiff_reference_file(tmp_path):
+    p = tmp_path / "g.tif"  # export.rs:1293-1321
+    assert O.write_geotiff(p, np.array([[1, 2], [3, 4]], np.float32), GT_COG) == 0
+    buf = p.read_bytes()
+    assert buf[:4] == b"II*\x00"
+    gk = 294 + 8
+    assert int.from_bytes(buf[gk:gk + 2], "little") == 1
+    assert int.from_bytes(buf[gk + 30:gk + 32], "little") == 4326
+
+
+def test_needs_bigtiff_reference_cases():
+    assert O.needs_bigtiff(33_000, 33_000, 4, 16) and not O.needs_bigtiff(1, 1, 4, 16)  # export.rs:1323-1334
+    assert O.needs_bigtiff(66_000, 66_000, 1, 16) and not O.needs_bigtiff(33_000, 33_000, 1, 16)  # 1425-1436
+    # the boundary: prelude + strip table + image crosses 2^32 - 1
+    pre = 8 + 198 + 16 + 24 + 48 + 32
+    rows = 64
+    cols = (0xFFFFFFFF - pre - 8) // (4 * rows)
+    assert not O.needs_bigtiff(rows, cols, 4, 16) and O.needs_bigtiff(rows, cols + 1, 4, 16)
+    st, h = O.geotiff_header(33_000, 33_000, 4, GT_COG)
+    assert st == 0 and h[:4] == b"II+\x00"
+
+
+def test_write_bigtiff_inner_reference_file(tmp_path):
+    p = tmp_path / "b.tif"  # export.rs:1336-1385, the inner BigTIFF writer on 2x2
+    data = np.array([[1, 2], [3, 4]], np.float32)
+    assert O.write_geotiff(p, data, GT_COG, force=2) == 0
+    buf = p.read_bytes()
+    assert buf[:2] == b"II" and int.from_bytes(buf[2:4], "little") == 43
+    assert int.from_bytes(buf[4:6], "little") == 8 and int.from_bytes(buf[8:16], "little") == 16
+    assert np.array_equal(np.frombuffer(buf, "<f4", 4, 456), data.ravel())
+
+
+def test_write_geotiff_u8_reference_file(tmp_path):
+    p = tmp_path / "u.tif"  # export.rs:1389-1422
+    assert O.write_geotiff(p, np.array([[0, 1], [5, 8]], np.uint8), GT_COG) == 0
+    buf = p.read_bytes()
+    assert buf[:4] == b"II*\x00"
+    assert int.from_bytes(buf[42:46], "little") == 8 and int.from_bytes(buf[162:166], "little") == 1
+    assert buf[334:338] == bytes([0, 1, 5, 8])
+
+
+def test_write_tiff_no_crs_reference_file(tmp_path):
+    p = tmp_path / "n.tif"  # export.rs:1440-1485
+    data = np.arange(1, 7, dtype=np.float32).reshape(2, 3)
+    assert O.write_geotiff(p, data, None) == 0
+    buf = p.read_bytes()
+    assert buf[:4] == b"II*\x00" and int.from_bytes(buf[8:10], "little") == 13
+    tags = [int.from_bytes(buf[10 + 12 * i:12 + 12 * i], "little") for i in range(13)]
+    assert not {33550, 33922, 34735} & set(tags)
+    assert np.array_equal(np.frombuffer(buf, "<f4", 6, 194), data.ravel())
+
+
+def test_write_geotiff_utm_keys(tmp_path):
+    p = tmp_path / "utm.tif"  # export.rs:1509-1578
+    gt = [691_600.0, 10.0, 0.0, 5_334_780.0, 0.0, -10.0]
+    assert O.write_geotiff(p, np.zeros((4, 4), np.float32), gt, crs=abi.CRS_UTM_NORTH, utm_zone=32) == 0
+    import tiffread
+    t = tiffread.parse(p.read_bytes())
+    assert t["ifds"][0]["tags"][34735][1] == [1, 1, 0, 4, 1024, 0, 1, 1, 1025, 0, 1, 1, 3072, 0, 1, 32632, 3076, 0,
+                                              1, 9001]
+
+
+@pytest.mark.parametrize("rows,cols,dtype,crs,force", [(1, 1, np.float32, 0, 0), (64, 5, np.float32, 1, 0),
+                                                       (65, 7, np.uint8, 2, 0), (200, 33, np.float32, -1, 0),
+                                                       (130, 9, np.float32, 0, 2), (1, 40, np.uint8, 1, 2),
+                                                       (3, 0, np.float32, 0, 0)])
+def test_write_geotiff_parsed_and_read_by_pillow(tmp_path, rows, cols, dtype, crs, force):
+    """Both TIFF flavours, strip counts 1 and many, both sample types and the
+    no-CRS form, decoded by the specification reader and by Pillow."""
+    import tiffread
+    rng = np.random.default_rng(rows + cols)
+    data = (rng.normal(0, 5, (rows, cols)).astype(np.float32) if dtype == np.float32 else
+            rng.integers(0, 256, (rows, cols)).astype(np.uint8))
+    gt = None if crs < 0 else ([500000.0, 10.0, 0.0, 5.2e6, 0.0, -10.0] if crs else GT_COG)
+    p = tmp_path / "s.tif"
+    assert O.write_geotiff(p, data, gt, crs=max(crs, 0), utm_zone=33, force=force) == 0
+    buf = p.read_bytes()
+    # single-strip BigTIFF stores its one offset and count inline (export.rs:1215-1218)
+    t = tiffread.parse(buf, pointer_tags=(273, 279) if force != 2 else ())
+    assert t["bigtiff"] == (force == 2) and len(t["ifds"]) == 1
+    g = t["ifds"][0]["tags"]
+    assert (33550 in g) == (gt is not None)
+    if gt is not None:
+        assert g[33550][1] == [gt[1], -gt[5], 0.0] and g[33922][1] == [0.0, 0.0, 0.0, gt[0], gt[3], 0.0]
+    assert np.array_equal(tiffread.strip_raster(buf, t["ifds"][0]), data)
+    assert len(buf) == g[273][1][0] + data.nbytes
+    if cols > 0 and force != 2 and rows > 64:
+        from PIL import Image
+        with Image.open(p) as im:
+            assert np.array_equal(np.asarray(im), data)
+
+
+def test_write_geotiff_rejects_empty_rows(tmp_path):
+    # rows == 0 underflows the reference's strip arithmetic (export.rs:861, 1061)
+    assert O.write_geotiff(tmp_path / "e.tif", np.zeros((0, 4), np.float32), GT_COG) == abi.ERR_INVALID_ARGUMENT
+
+
 # ── full chain invariants on a small synthetic scene ──────────────────────
 def test_small_scene_chain_invariants():
     sc = synthetic.scene_small_plain(2, lines=256, samples=512, out_rows=160, out_cols=160, dem_side=256)

@@ -86,3 +86,20 @@ def level_raster(buf: bytes, ifd: dict, padded: bool = False) -> np.ndarray:
         ty, tx = divmod(i, nx)
         out[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw] = np.frombuffer(buf, "<f4", tw * th, o).reshape(th, tw)
     return out if padded else out[:rows, :cols]
+
+
+def strip_raster(buf: bytes, ifd: dict) -> np.ndarray:
+    """The raster of a single-band striped IFD (8-bit unsigned or 32-bit
+    float samples)."""
+    t = ifd["tags"]
+    cols, rows = t[256][1][0], t[257][1][0]
+    bits, fmt = t[258][1][0], t[339][1][0]
+    dt = {(8, 1): "u1", (32, 3): "<f4"}[(bits, fmt)]
+    per = t[278][1][0]
+    offs, counts = t[273][1], t[279][1]
+    out = np.empty((rows, cols), dt)
+    for i, (o, n) in enumerate(zip(offs, counts)):
+        r0, r1 = i * per, min((i + 1) * per, rows)
+        assert n == (r1 - r0) * cols * (bits // 8)
+        out[r0:r1] = np.frombuffer(buf, dt, (r1 - r0) * cols, o).reshape(r1 - r0, cols)
+    return out
