@@ -50,13 +50,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32", "cog"],
+    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32", "cog", "multilook"],
                     default="iw_full",
                     help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
                          "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request; "
                          "fp32: the FP32 scene-centred variant's errors and speed against FP64; "
                          "cog: the COG export (pyramid + tile pack, and the whole file write) of the "
-                         "full-size dB product")
+                         "full-size dB product; multilook: the pre-TC multilook of the full merged image")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
@@ -133,6 +133,8 @@ def main():
         return bench_fp32(args, dev)
     if args.workload == "cog":
         return bench_cog(args, dev)
+    if args.workload == "multilook":
+        return bench_multilook(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -844,5 +846,94 @@ def bench_cog(args, dev):
     print(json.dumps(line), flush=True)
 
 
+
+def bench_multilook(args, dev):
+    """Pre-TC multilook (apply_multilook, scene_prep.rs:1060-1161) of the
+    benchmark scene's merged σ⁰ + NESZ (13100 x 60814), inputs in HBM.  value:
+    input Mpx/s at 4 x 1 (range x azimuth) looks; other shapes alongside."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from sardine_amd import _native, abi, api, synthetic
+    from sardine_amd.workload import DeviceScene
+
+    L = _native.lib()
+    stream = torch.cuda.current_stream(dev)
+    sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+    ds = DeviceScene.build(sc, dev)
+    m = ds.prepare_merged(stream=stream)
+    ati = sc.ati
+    torch.cuda.synchronize()
+    lines, samples = m.lines, m.samples
+    log(f"merged {lines}x{samples}")
+    shapes = [(4, 1), (2, 2), (5, 1), (10, 2), (3, 3)]
+    res = {}
+    L.sardine_set_profiling(1)
+    for r, a in shapes:
+        ol, os_ = lines // a, samples // r
+        od = torch.empty((ol, os_), dtype=torch.float32, device=dev)
+        on = torch.empty_like(od)
+        for _ in range(max(1, args.warmup)):
+            api.apply_multilook(m, r, a, ati, stream=stream, out=(od, on))
+        settle_gc()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            api.apply_multilook(m, r, a, ati, stream=stream, out=(od, on))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        k = recent_kernel_ms(L, abi.KERNEL_MULTILOOK, args.steps)
+        kms = float(np.mean(k))
+        alg = ol * a * os_ * r * 8 + ol * os_ * 8
+        res[f"{r}x{a}"] = {"kernel_ms": round(kms, 4), "step_ms": round(e0.elapsed_time(e1) / args.steps, 4),
+                           "algorithmic_bytes": alg, "GBps": round(alg / (kms / 1e3) / 1e9, 1),
+                           "hbm_frac": round(alg / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"multilook {r}x{a}: {json.dumps(res[f'{r}x{a}'])}")
+        if (r, a) == shapes[0]:
+            keep = (od, on)
+        else:
+            del od, on
+    L.sardine_set_profiling(0)
+    od, on = keep
+    cpu = None
+    same = None
+    if not args.no_cpu_baseline:
+        # the oracle (one thread) on the whole image (about a second of CPU
+        # work), which also checks the device result bit for bit at full size
+        band = lines
+        hd = m.data[:band].cpu().numpy()
+        hn = m.nesz[:band].cpu().numpy()
+        t = time.perf_counter()
+        st, rd, rn, _ = O.multilook(hd, hn, m.info, 4, 1, ati)
+        cs = time.perf_counter() - t
+        assert st == 0
+        gd = od[:band].cpu().numpy()
+        gn = on[:band].cpu().numpy()
+        same = bool(np.array_equal(gd.view(np.uint32), rd[:band].view(np.uint32)) and
+                    np.array_equal(gn.view(np.uint32), rn[:band].view(np.uint32)))
+        cpu = {"value": round(band * samples / cs / 1e6, 1), "unit": "input Mpx/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's apply_multilook (4 x 1) on the whole merged image ({band} x {samples})",
+               "seconds": round(cs, 3), "device_result_bit_exact": same}
+        log(f"cpu: {json.dumps(cpu)}")
+    main = res["4x1"]
+    px = lines * samples
+    line = {
+        "metric": "pre-TC multilook input Mpx/s (sigma0 + NESZ, 4x1 looks, inputs in HBM)",
+        "value": round(px / (main["kernel_ms"] / 1e3) / 1e6, 1), "unit": "Mpx/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": main["step_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "multilook_iw_full_13100x60814", "merged": [lines, samples], "looks": "4x1"},
+        "roofline": {"kernel": "multilook_kernel<4>", "bound": "hbm", "achieved": main["GBps"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": main["hbm_frac"], "traffic": None,
+                     "algorithmic_bytes": main["algorithmic_bytes"], "launch_ms": main["kernel_ms"]},
+        "shapes": res,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
 if __name__ == "__main__":
     main()
+

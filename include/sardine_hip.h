@@ -428,6 +428,24 @@ sardine_status sardine_apply_calibration(const int16_t* deburst /*[device] lines
                                          uint32_t* cal_gap_px, uint32_t* noise_gap_px,
                                          void* stream);
 
+/* ── Pre-TC multilook ─────────────────────────────────────────────────────── */
+
+/* apply_multilook(merged, range_looks, azimuth_looks, ati) (scene_prep.rs:
+ * 1060-1161): NaN-aware boxcar decimation of σ⁰ and NESZ (NESZ averaged over
+ * the non-NaN σ⁰ samples) to floor(lines / azimuth_looks) x floor(samples /
+ * range_looks), with the centre-aligned geometry update written to `out`
+ * (the per-swath fields are copied unchanged; they describe the input
+ * columns).  1 x 1 returns the input unchanged (copied when the output
+ * buffers differ).  With out_data and out_nesz both NULL only `out` is
+ * computed (host only: the caller's shape query).  `out` may alias `in`.
+ * Errors (anyhow bails in the reference): SARDINE_ERR_INVALID_ARGUMENT for a
+ * zero factor, factors larger than the image, and a non-finite or
+ * non-positive ATI with azimuth_looks > 1. */
+sardine_status sardine_apply_multilook(const float* data /*[device]*/, const float* nesz /*[device]*/,
+                                       const sardine_merged_info* in, int32_t range_looks, int32_t azimuth_looks,
+                                       double azimuth_time_interval_s, float* out_data /*[device]*/,
+                                       float* out_nesz /*[device]*/, sardine_merged_info* out, void* stream);
+
 /* ── dB epilogue ────────────────────────────────────────────────────────── */
 
 /* to_db_inplace(data, noise_floor_linear) (export.rs:59-94).  counts[0] =
@@ -539,7 +557,8 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_SPECKLE = 2,
     SARDINE_KERNEL_TO_DB = 3,
     SARDINE_KERNEL_TC_SEGMENT = 4, /* the per-pixel chain kernel of terrain correction alone */
-    SARDINE_KERNEL_COG_PACK = 5    /* the COG export's pyramid + tile pack (all its launches) */
+    SARDINE_KERNEL_COG_PACK = 5,   /* the COG export's pyramid + tile pack (all its launches) */
+    SARDINE_KERNEL_MULTILOOK = 6
 } sardine_kernel_family;
 /* Switches the kernel timers of the calling thread on (creating their
  * events) or off. */

@@ -78,6 +78,7 @@ def lib():
         L.oracle_needs_bigtiff.argtypes = [C.c_uint64] * 4
         L.oracle_geotiff_header.argtypes = [i64, i64, i32, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         L.oracle_write_geotiff.argtypes = [C.c_char_p, P, i64, i64, i32, P, i32, i32, i32]
+        L.oracle_multilook.argtypes = [P, P, C.POINTER(abi.MergedInfo), i32, i32, d, P, P, C.POINTER(abi.MergedInfo)]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
@@ -368,6 +369,26 @@ def write_geotiff(path, data: np.ndarray, geotransform=None, crs: int = 0, utm_z
     gt = None if geotransform is None else np.ascontiguousarray(geotransform, np.float64)
     return lib().oracle_write_geotiff(str(path).encode(), _ptr(data) if data.size else None, cols, rows,
                                       data.dtype.itemsize, None if gt is None else _ptr(gt), crs, utm_zone, force)
+
+
+def multilook(data: np.ndarray, nesz: np.ndarray, info, range_looks: int, azimuth_looks: int, ati_s: float = 0.0):
+    """apply_multilook (scene_prep.rs:1060-1161) → (status, data, nesz, info)."""
+    data = np.ascontiguousarray(data, np.float32)
+    nesz = np.ascontiguousarray(nesz, np.float32)
+    assert data.shape == nesz.shape
+    lines, samples = data.shape
+    info = abi.MergedInfo.from_buffer_copy(info)  # the raster's own size, whatever the info says
+    info.lines, info.samples = lines, samples
+    ol, os_ = (lines // max(azimuth_looks, 1), samples // max(range_looks, 1)) if (range_looks, azimuth_looks) != (1, 1) \
+        else (lines, samples)
+    od = np.empty(max(ol * os_, 1), np.float32)
+    on = np.empty(max(ol * os_, 1), np.float32)
+    out = abi.MergedInfo()
+    st = lib().oracle_multilook(_ptr(data), _ptr(nesz), C.byref(info), range_looks, azimuth_looks, ati_s, _ptr(od),
+                                _ptr(on), C.byref(out))
+    if st:
+        return st, None, None, None
+    return st, od[:ol * os_].reshape(ol, os_), on[:ol * os_].reshape(ol, os_), out
 
 
 # ── the full chain on a synthetic scene (host) ──────────────────────────

@@ -27,6 +27,9 @@
 //   downsample_2x / COG writer export.rs:1583-1733 (tile-size and geotransform
 //                              errors, header and 17-entry IFDs, tile data
 //                              round trip, 2x2 means and NaN rule)
+//   apply_multilook            run.rs:1449-1588 (pass-through, uniform and
+//                              distinct averages, NaN exclusion, all-NaN
+//                              block, geometry, azimuth shift, errors)
 //   striped GeoTIFF writers    export.rs:1276-1580 (geotransform errors, key
 //                              offsets, BigTIFF threshold and layout, u8 tags,
 //                              the no-CRS IFD, UTM keys)
@@ -1738,6 +1741,64 @@ int oracle_write_geotiff(const char* path, const void* data, int64_t cols, int64
     if (ok && n) ok = std::fwrite(data, 1, n, f) == n;
     ok = (std::fclose(f) == 0) && ok;
     return ok ? 0 : SARDINE_ERR_EXPORT_IO;
+}
+
+}  // extern "C"
+
+// ── pre-TC multilook (scene_prep.rs:1060-1161) ─────────────────────────────
+extern "C" {
+
+// apply_multilook: NaN-aware boxcar decimation of σ⁰ (and of NESZ over the
+// same non-NaN σ⁰ samples), output lines/samples = floor(in / looks), with
+// the centre-aligned geometry update.  range_looks = azimuth_looks = 1
+// returns the input unchanged.  Errors (anyhow bails in the reference):
+// zero factors, factors larger than the image, a non-positive ATI with
+// azimuth_looks > 1 — all SARDINE_ERR_INVALID_ARGUMENT.
+int oracle_multilook(const float* data, const float* nesz, const sardine_merged_info* in, int32_t range_looks,
+                     int32_t azimuth_looks, double ati_s, float* out_data, float* out_nesz,
+                     sardine_merged_info* out) {
+    const int64_t R = range_looks, A = azimuth_looks;
+    if (R == 1 && A == 1) {
+        *out = *in;
+        const size_t n = (size_t)(in->lines * in->samples);
+        if (n) {
+            std::memcpy(out_data, data, n * sizeof(float));
+            std::memcpy(out_nesz, nesz, n * sizeof(float));
+        }
+        return 0;
+    }
+    if (R <= 0 || A <= 0) return SARDINE_ERR_INVALID_ARGUMENT;
+    const int64_t il = in->lines, is = in->samples, ol = il / A, os = is / R;
+    if (ol == 0 || os == 0) return SARDINE_ERR_INVALID_ARGUMENT;
+    *out = *in;
+    out->lines = ol;
+    out->samples = os;
+    out->range_pixel_spacing_s = in->range_pixel_spacing_s * (double)R;
+    out->range_pixel_spacing_m = in->range_pixel_spacing_m * (double)R;
+    out->near_slant_range_time_s = in->near_slant_range_time_s + (double)(R - 1) / 2.0 * in->range_pixel_spacing_s;
+    if (A > 1) {
+        if (!std::isfinite(ati_s) || ati_s <= 0.0) return SARDINE_ERR_INVALID_ARGUMENT;
+        out->azimuth_start_time_us = in->azimuth_start_time_us + (int64_t)((double)(A - 1) / 2.0 * ati_s * 1e6);
+    }
+    for (int64_t l = 0; l < ol; ++l)
+        for (int64_t c = 0; c < os; ++c) {
+            double sum = 0.0, sum_n = 0.0;
+            uint32_t count = 0;
+            for (int64_t da = 0; da < A; ++da)
+                for (int64_t dr = 0; dr < R; ++dr) {
+                    const int64_t k = (l * A + da) * is + c * R + dr;
+                    const float v = data[k];
+                    if (!std::isnan(v)) {
+                        sum += (double)v;
+                        sum_n += (double)nesz[k];
+                        ++count;
+                    }
+                }
+            const float qnan = std::numeric_limits<float>::quiet_NaN();
+            out_data[l * os + c] = count ? (float)(sum / (double)count) : qnan;
+            out_nesz[l * os + c] = count ? (float)(sum_n / (double)count) : qnan;
+        }
+    return 0;
 }
 
 }  // extern "C"

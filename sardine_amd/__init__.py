@@ -6,7 +6,7 @@ package mirrors the reference's Rust interface for the path on top of it.
 """
 from . import abi
 from .api import (DemRaster, GeocodedImage, MergedSigma0, ResamplingKernel, SceneInputs, Sigma0Array,
-                  SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_speckle_filter,
+                  SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_multilook, apply_speckle_filter,
                   apply_speckle_step, cog_header, cog_layout, cog_pack, deburst_layout, merged_layout,
                   geotiff_header, output_grid, prepare_merged_scene, synth_cint16, terrain_correction,
                   to_db_inplace, write_cog_with_crs, write_geotiff_u8_with_crs, write_geotiff_with_crs,
@@ -16,7 +16,8 @@ from .errors import (CalibrationError, DeburstError, DeviceError, ExportError, M
 
 __all__ = [
     "abi", "DemRaster", "GeocodedImage", "MergedSigma0", "ResamplingKernel", "SceneInputs", "Sigma0Array",
-    "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_speckle_filter", "apply_speckle_step",
+    "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_multilook", "apply_speckle_filter",
+    "apply_speckle_step",
     "cog_header", "cog_layout", "cog_pack", "deburst_layout", "merged_layout", "output_grid",
     "prepare_merged_scene", "synth_cint16", "terrain_correction", "to_db_inplace", "write_cog_with_crs",
     "geotiff_header", "write_geotiff_with_crs", "write_geotiff_u8_with_crs", "write_tiff_no_crs",

@@ -233,6 +233,7 @@ EXPORTED_SYMBOLS = (
     "sardine_prepare_merged",
     "sardine_apply_calibration",
     "sardine_to_db_inplace",
+    "sardine_apply_multilook",
     "sardine_cog_layout_compute",
     "sardine_cog_header",
     "sardine_cog_pack",
@@ -262,3 +263,4 @@ EXPORTED_SYMBOLS = (
     "sardine_server_destroy",
 )
 KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT, KERNEL_COG_PACK = 0, 1, 2, 3, 4, 5
+KERNEL_MULTILOOK = 6

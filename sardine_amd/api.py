@@ -11,6 +11,8 @@ Names and argument meaning follow the Rust crate:
         deburst → apply_calibration → merge_subswaths, scene_prep.rs:598-699
   apply_calibration(...) -> Sigma0Array      apply_calibration.rs:374
   deburst_layout(...)                        deburst.rs:172-305
+  apply_multilook(merged, range_looks, azimuth_looks, ati) -> MergedSigma0
+        scene_prep.rs:1060-1161
   to_db_inplace(data, noise_floor_linear)    export.rs:59-94
   write_cog_with_crs(path, data, cols, rows, geotransform, crs, tile_size)
         export.rs:329-354 (cog_layout / cog_header / cog_pack expose its parts)
@@ -363,6 +365,44 @@ def apply_calibration(deburst, valid_sample_offset: int, cal: abi.Lut, noise_ran
                                            out.data_ptr(), nz.data_ptr(), C.byref(g1), C.byref(g2),
                                            _stream_ptr(stream)))
     return Sigma0Array(out, nz, lines, samples, valid_sample_offset, g1.value, g2.value)
+
+
+def apply_multilook(merged: MergedSigma0, range_looks: int, azimuth_looks: int, azimuth_time_interval_s: float,
+                    stream=None, out=None) -> MergedSigma0:
+    """apply_multilook (scene_prep.rs:1060-1161): NaN-aware boxcar decimation
+    of σ⁰ and NESZ with the centre-aligned geometry update.  1 x 1 returns
+    `merged` itself, as the reference does."""
+    if range_looks == 1 and azimuth_looks == 1:
+        return merged
+    torch = _torch()
+    if merged.nesz is None:
+        raise ValueError("multilook needs the NESZ raster")
+    if merged.info is not None:
+        info = abi.MergedInfo.from_buffer_copy(merged.info)
+    else:
+        info = abi.MergedInfo()
+    info.lines, info.samples = merged.lines, merged.samples
+    info.near_slant_range_time_s = merged.near_slant_range_time_s
+    info.range_pixel_spacing_s = merged.range_pixel_spacing_s
+    info.range_pixel_spacing_m = merged.range_pixel_spacing_m
+    info.azimuth_start_time_us = int(merged.azimuth_start_time_us)
+    info.cal_lut_extrapolation_gap_px = merged.cal_lut_extrapolation_gap_px
+    info.noise_lut_extrapolation_gap_px = merged.noise_lut_extrapolation_gap_px
+    shape = abi.MergedInfo()
+    _check(lib().sardine_apply_multilook(None, None, C.byref(info), range_looks, azimuth_looks,
+                                         azimuth_time_interval_s, None, None, C.byref(shape), None))
+    if out is None:
+        od = torch.empty((shape.lines, shape.samples), dtype=torch.float32, device=merged.data.device)
+        on = torch.empty_like(od)
+    else:
+        od, on = out
+    res = abi.MergedInfo()
+    _check(lib().sardine_apply_multilook(merged.data.data_ptr(), merged.nesz.data_ptr(), C.byref(info), range_looks,
+                                         azimuth_looks, azimuth_time_interval_s, od.data_ptr(), on.data_ptr(),
+                                         C.byref(res), _stream_ptr(stream)))
+    return MergedSigma0(od, on, res.near_slant_range_time_s, res.range_pixel_spacing_m, res.azimuth_start_time_us,
+                        res.range_pixel_spacing_s, res.cal_lut_extrapolation_gap_px,
+                        res.noise_lut_extrapolation_gap_px, res)
 
 
 def to_db_inplace(data, noise_floor_linear: float, stream=None):

@@ -18,7 +18,7 @@ import tempfile
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libsardine_hip.so"
-SOURCES = ["abi.hip", "tc.hip", "tc32.hip", "tc_batch.hip", "cal.hip", "speckle.hip", "misc.hip", "export.hip"]
+SOURCES = ["abi.hip", "tc.hip", "tc32.hip", "tc_batch.hip", "cal.hip", "speckle.hip", "misc.hip", "export.hip", "multilook.hip"]
 
 
 def hipcc() -> str:

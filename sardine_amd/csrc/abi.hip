@@ -55,6 +55,9 @@ struct CogLevelDev {  // layout of the struct in export.hip
 extern "C" hipError_t sd_launch_cog_level(const float* src, int src_tiled, const sd::CogLevelDev* S,
                                           const sd::CogLevelDev* D, const sd::CogLevelDev* copy, int lg,
                                           hipStream_t s);
+extern "C" hipError_t sd_launch_multilook(const float* data, const float* nesz, int64_t in_samples,
+                                          int64_t out_lines, int64_t out_samples, int r, int a, float* od,
+                                          float* on, hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -64,7 +67,7 @@ thread_local std::string g_last_error;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
-constexpr int kFamilies = 6;
+constexpr int kFamilies = 7;
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
 // waits here: sardine_last_kernel_ms waits for the end event when read.
@@ -1320,6 +1323,70 @@ sardine_status sardine_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint
     sardine_status st;
     if ((st = require_device())) return st;
     SD_HIP(sd_launch_synth_cint16(out, n, seed, base, sigma, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── pre-TC multilook (scene_prep.rs:1060-1161) ──────────────────────────
+extern "C" {
+
+sardine_status sardine_apply_multilook(const float* data, const float* nesz, const sardine_merged_info* in,
+                                       int32_t range_looks, int32_t azimuth_looks, double azimuth_time_interval_s,
+                                       float* out_data, float* out_nesz, sardine_merged_info* out, void* stream) {
+    if (!in || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "info is null");
+    if (in->lines < 0 || in->samples < 0) return fail(SARDINE_ERR_INVALID_ARGUMENT, "negative image size");
+    const sardine_merged_info src = *in;  // `out` may alias `in`
+    sardine_merged_info o = src;
+    const bool pass = range_looks == 1 && azimuth_looks == 1;
+    if (!pass) {
+        if (range_looks <= 0 || azimuth_looks <= 0)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "multilook factors must be >= 1 (range=%d, azimuth=%d)",
+                        range_looks, azimuth_looks);
+        o.lines = src.lines / azimuth_looks;
+        o.samples = src.samples / range_looks;
+        if (o.lines == 0 || o.samples == 0)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT,
+                        "multilook (%dx%d) is larger than image dimensions (%lld samples x %lld lines)", range_looks,
+                        azimuth_looks, (long long)src.samples, (long long)src.lines);
+        // centre-aligned geometry (scene_prep.rs:1094-1117)
+        o.range_pixel_spacing_s = src.range_pixel_spacing_s * (double)range_looks;
+        o.range_pixel_spacing_m = src.range_pixel_spacing_m * (double)range_looks;
+        o.near_slant_range_time_s =
+            src.near_slant_range_time_s + (double)(range_looks - 1) / 2.0 * src.range_pixel_spacing_s;
+        if (azimuth_looks > 1) {
+            if (!std::isfinite(azimuth_time_interval_s) || azimuth_time_interval_s <= 0.0)
+                return fail(SARDINE_ERR_INVALID_ARGUMENT,
+                            "azimuth_time_interval_s must be > 0 when azimuth_looks > 1 (got %g)",
+                            azimuth_time_interval_s);
+            o.azimuth_start_time_us =
+                src.azimuth_start_time_us +
+                (int64_t)((double)(azimuth_looks - 1) / 2.0 * azimuth_time_interval_s * 1e6);
+        }
+    }
+    if (!out_data && !out_nesz) {  // shape query: no device work
+        *out = o;
+        return SARDINE_OK;
+    }
+    const int64_t n_in = src.lines * src.samples;
+    if (n_in > 0 && (!data || !nesz || !out_data || !out_nesz))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "null raster");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream;
+    KernelTimer kt(SARDINE_KERNEL_MULTILOOK, s);
+    if (pass) {
+        // range_looks = azimuth_looks = 1 returns the input unchanged
+        if (n_in > 0 && out_data != data)
+            SD_HIP(hipMemcpyAsync(out_data, data, (size_t)n_in * sizeof(float), hipMemcpyDeviceToDevice, s));
+        if (n_in > 0 && out_nesz != nesz)
+            SD_HIP(hipMemcpyAsync(out_nesz, nesz, (size_t)n_in * sizeof(float), hipMemcpyDeviceToDevice, s));
+    } else {
+        SD_HIP(sd_launch_multilook(data, nesz, src.samples, o.lines, o.samples, range_looks, azimuth_looks, out_data,
+                                   out_nesz, s));
+    }
+    kt.stop();
+    *out = o;
     return SARDINE_OK;
 }
 

@@ -1,0 +1,132 @@
+// multilook.hip — pre-TC multilook of the merged σ⁰ and NESZ
+// (apply_multilook, scene_prep.rs:1060-1161): NaN-aware boxcar decimation by
+// range_looks x azimuth_looks, output pixel (l, c) = mean of the non-NaN σ⁰
+// samples of input block [l*A, l*A+A) x [c*R, c*R+R), NESZ averaged over the
+// same samples.  Sums run in f64 in the reference's order (azimuth outer,
+// range inner) and are divided by the count in f64, so results are the
+// reference's bit for bit.
+//
+// A thread owns one output pixel of one output line and walks its block; a
+// wave's loads of one block row cover 64*R consecutive input samples, so
+// every input byte is fetched once from HBM (the R loads of one lane hit the
+// lines its neighbours bring in).  HBM-bound: 8 B read per input pixel and
+// 8 B written per output pixel.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sd {
+
+#ifndef SD_ML_NT
+#define SD_ML_NT 0
+#endif
+
+template <typename T>
+__device__ __forceinline__ T ld(const T* p) {
+    if (SD_ML_NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// RT > 0: range_looks known at compile time (the block row is unrolled).
+// V2: rows and blocks start at even samples and the rasters are 8-byte
+// aligned, so a block row is read as RT/2 8-byte pairs (RT even).
+template <int RT, bool V2>
+__global__ __launch_bounds__(256) void multilook_kernel(const float* __restrict__ data,
+                                                        const float* __restrict__ nesz, int64_t in_samples,
+                                                        int64_t out_lines, int64_t out_samples, int r_looks,
+                                                        int a_looks, float* __restrict__ out_data,
+                                                        float* __restrict__ out_nesz) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= out_samples) return;
+    const int R = RT > 0 ? RT : r_looks;
+    const float qnan = __builtin_nanf("");
+    for (int64_t l = blockIdx.y; l < out_lines; l += gridDim.y) {
+        double sum = 0.0, sum_n = 0.0;
+        unsigned count = 0;
+        const int64_t base = l * a_looks * in_samples + c * R;
+        for (int da = 0; da < a_looks; ++da) {
+            const float* pd = data + base + da * in_samples;
+            const float* pn = nesz + base + da * in_samples;
+            if (RT > 0) {
+                float v[RT > 0 ? RT : 1], n[RT > 0 ? RT : 1];
+                if (V2) {
+#pragma unroll
+                    for (int j = 0; j < RT / 2; ++j) {
+                        const f2v a = ld(reinterpret_cast<const f2v*>(pd) + j);
+                        const f2v b = ld(reinterpret_cast<const f2v*>(pn) + j);
+                        v[2 * j] = a.x;
+                        v[2 * j + 1] = a.y;
+                        n[2 * j] = b.x;
+                        n[2 * j + 1] = b.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int dr = 0; dr < RT; ++dr) {
+                        v[dr] = ld(pd + dr);
+                        n[dr] = ld(pn + dr);
+                    }
+                }
+#pragma unroll
+                for (int dr = 0; dr < RT; ++dr) {
+                    if (!__builtin_isnan(v[dr])) {
+                        sum += (double)v[dr];
+                        sum_n += (double)n[dr];
+                        ++count;
+                    }
+                }
+            } else {
+                for (int dr = 0; dr < R; ++dr) {
+                    const float v = ld(pd + dr), n = ld(pn + dr);
+                    if (!__builtin_isnan(v)) {
+                        sum += (double)v;
+                        sum_n += (double)n;
+                        ++count;
+                    }
+                }
+            }
+        }
+        out_data[l * out_samples + c] = count ? (float)(sum / (double)count) : qnan;
+        out_nesz[l * out_samples + c] = count ? (float)(sum_n / (double)count) : qnan;
+    }
+}
+
+#ifndef SD_ML_VEC
+#define SD_ML_VEC 1
+#endif
+
+template <int RT>
+hipError_t launch_multilook(const float* data, const float* nesz, int64_t in_samples, int64_t out_lines,
+                            int64_t out_samples, int r, int a, float* od, float* on, hipStream_t s) {
+    const unsigned gx = (unsigned)((out_samples + 255) / 256);
+    const unsigned gy = (unsigned)(out_lines < 65535 ? out_lines : 65535);
+    const bool v2 = SD_ML_VEC && RT > 0 && RT % 2 == 0 && in_samples % 2 == 0 &&
+                    reinterpret_cast<uintptr_t>(data) % 8 == 0 && reinterpret_cast<uintptr_t>(nesz) % 8 == 0;
+    if (v2)
+        hipLaunchKernelGGL((multilook_kernel<RT, true>), dim3(gx, gy), dim3(256), 0, s, data, nesz, in_samples,
+                           out_lines, out_samples, r, a, od, on);
+    else
+        hipLaunchKernelGGL((multilook_kernel<RT, false>), dim3(gx, gy), dim3(256), 0, s, data, nesz, in_samples,
+                           out_lines, out_samples, r, a, od, on);
+    return hipGetLastError();
+}
+
+}  // namespace sd
+
+// Sizes as the host validated them: out_lines = lines / a, out_samples =
+// samples / r, both >= 1.
+extern "C" hipError_t sd_launch_multilook(const float* data, const float* nesz, int64_t in_samples,
+                                          int64_t out_lines, int64_t out_samples, int r, int a, float* od,
+                                          float* on, hipStream_t s) {
+    if (out_lines <= 0 || out_samples <= 0) return hipSuccess;
+    switch (r) {
+        case 1: return sd::launch_multilook<1>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        case 2: return sd::launch_multilook<2>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        case 3: return sd::launch_multilook<3>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        case 4: return sd::launch_multilook<4>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        case 5: return sd::launch_multilook<5>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        case 8: return sd::launch_multilook<8>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        case 10: return sd::launch_multilook<10>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+        default: return sd::launch_multilook<0>(data, nesz, in_samples, out_lines, out_samples, r, a, od, on, s);
+    }
+}

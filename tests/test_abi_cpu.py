@@ -331,3 +331,47 @@ def test_geotiff_argument_errors_precede_device_check(tmp_path):
         assert L.sardine_write_geotiff_with_crs(p, d.ctypes.data, 2, 2, gt, 0, 0, None) == abi.ERR_NO_DEVICE
         assert L.sardine_write_tiff_no_crs(p, d.ctypes.data, 2, 2, None) == abi.ERR_NO_DEVICE
         assert not (tmp_path / "g.tif").exists()
+
+
+def _ml_info(lines, samples):
+    i = abi.MergedInfo()
+    i.lines, i.samples = lines, samples
+    i.near_slant_range_time_s = 0.0053
+    i.range_pixel_spacing_m = 2.329562
+    i.range_pixel_spacing_s = 2.329562 / 299_792_458.0 * 2.0
+    i.azimuth_start_time_us = 1_548_221_628_000_000
+    i.n_swaths = 3
+    i.seam[0] = 17
+    return i
+
+
+@pytest.mark.parametrize("r,a,ati", [(1, 1, 0.0), (4, 1, 0.0), (3, 2, 2.055e-3), (1, 5, 1e-3), (7, 7, 2.2e-3)])
+def test_multilook_geometry_matches_oracle(r, a, ati):
+    """The shape query (null output rasters) is host code: its geometry equals
+    the oracle's apply_multilook."""
+    L = _native.lib()
+    src = _ml_info(50, 90)
+    out = abi.MergedInfo()
+    assert L.sardine_apply_multilook(None, None, C.byref(src), r, a, ati, None, None, C.byref(out), None) == 0
+    z = np.zeros((50, 90), np.float32)
+    st, _, _, ref = O.multilook(z, z, src, r, a, ati)
+    assert st == 0
+    assert bytes(out) == bytes(ref)
+    assert out.seam[0] == 17 and out.n_swaths == 3
+
+
+def test_multilook_argument_errors_precede_device_check():
+    L = _native.lib()
+    src = _ml_info(2, 4)
+    out = abi.MergedInfo()
+    q = lambda r, a, ati: L.sardine_apply_multilook(None, None, C.byref(src), r, a, ati, None, None, C.byref(out), None)
+    assert q(0, 1, 0.0) == abi.ERR_INVALID_ARGUMENT and "factors must be" in _native.last_error()
+    assert q(8, 1, 0.0) == abi.ERR_INVALID_ARGUMENT and "larger than" in _native.last_error()
+    assert q(1, 2, 0.0) == abi.ERR_INVALID_ARGUMENT and q(1, 2, float("nan")) == abi.ERR_INVALID_ARGUMENT
+    assert q(-1, 2, 1e-3) == abi.ERR_INVALID_ARGUMENT
+    import torch
+
+    if not torch.cuda.is_available():
+        d = np.zeros(8, np.float32)
+        assert L.sardine_apply_multilook(d.ctypes.data, d.ctypes.data, C.byref(src), 2, 1, 0.0, d.ctypes.data,
+                                         d.ctypes.data, C.byref(out), None) == abi.ERR_NO_DEVICE

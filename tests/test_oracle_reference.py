@@ -899,6 +899,74 @@ def test_write_geotiff_rejects_empty_rows(tmp_path):
     assert O.write_geotiff(tmp_path / "e.tif", np.zeros((0, 4), np.float32), GT_COG) == abi.ERR_INVALID_ARGUMENT
 
 
+# ── pre-TC multilook (run.rs:1449-1588) ───────────────────────────────────
+def _ml_info(lines, samples):
+    i = abi.MergedInfo()
+    i.lines, i.samples = lines, samples
+    i.near_slant_range_time_s = 0.005_300
+    i.range_pixel_spacing_m = 2.329562
+    i.range_pixel_spacing_s = 2.329562 / 299_792_458.0 * 2.0
+    i.azimuth_start_time_us = 1_548_221_628_000_000
+    return i
+
+
+def _ml(data, r, a, ati=0.0, nesz=None):
+    data = np.asarray(data, np.float32)
+    nesz = np.full(data.shape, 0.01, np.float32) if nesz is None else np.asarray(nesz, np.float32)
+    return O.multilook(data, nesz, _ml_info(*data.shape), r, a, ati)
+
+
+def test_multilook_reference_cases():
+    st, d, n, info = _ml(np.full((4, 8), 2.0), 1, 1)  # pass-through (run.rs:1473-1482)
+    assert st == 0 and d.shape == (4, 8) and d[0, 0] == 2.0 and info.near_slant_range_time_s == 0.0053
+    st, d, n, info = _ml([[1.0, 1.0, 5.0, 5.0]], 2, 1)  # :1484-1494
+    assert d.shape == (1, 2) and list(d[0]) == [1.0, 5.0]
+    st, d, n, info = _ml([[1, 3, 5, 7], [2, 4, 6, 8]], 2, 1)  # :1496-1510
+    assert d.tolist() == [[2.0, 6.0], [3.0, 7.0]]
+    st, d, n, info = _ml([[np.nan, 4.0]], 2, 1, nesz=[[np.nan, 0.1]])  # :1512-1522
+    assert d[0, 0] == 4.0 and abs(n[0, 0] - 0.1) < 1e-6
+    st, d, n, info = _ml([[np.nan, np.nan]], 2, 1)  # :1524-1531
+    assert np.isnan(d[0, 0]) and np.isnan(n[0, 0])
+    i0 = _ml_info(2, 8)
+    st, d, n, info = _ml(np.ones((2, 8)), 4, 1)  # :1533-1553
+    assert (info.samples, info.lines) == (2, 2)
+    assert abs(info.range_pixel_spacing_s - 4 * i0.range_pixel_spacing_s) < 1e-25
+    assert abs(info.range_pixel_spacing_m - 4 * i0.range_pixel_spacing_m) < 1e-12
+    assert abs(info.near_slant_range_time_s - (i0.near_slant_range_time_s + 1.5 * i0.range_pixel_spacing_s)) < 1e-25
+    st, d, n, info = _ml(np.ones((4, 2)), 1, 2, ati=2.055e-3)  # :1555-1568
+    assert (info.lines, info.samples) == (2, 2)
+    assert info.azimuth_start_time_us == i0.azimuth_start_time_us + int(0.5 * 2.055e-3 * 1e6) == \
+        i0.azimuth_start_time_us + 1027
+    assert _ml(np.ones((2, 4)), 0, 1)[0] == abi.ERR_INVALID_ARGUMENT  # :1570-1576
+    assert _ml(np.ones((2, 4)), 8, 1)[0] == abi.ERR_INVALID_ARGUMENT  # :1578-1587
+    assert _ml(np.ones((4, 4)), 1, 2, ati=0.0)[0] == abi.ERR_INVALID_ARGUMENT  # ATI must be > 0 when A > 1
+    assert _ml(np.ones((4, 4)), 1, 2, ati=np.nan)[0] == abi.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("r,a", [(2, 1), (4, 1), (3, 2), (1, 3), (5, 4), (7, 7)])
+def test_multilook_against_nanmean(r, a):
+    """Block means by numpy's nanmean over reshaped blocks (a different
+    summation route): NaN pattern exact, values within 1 ulp; NESZ is the mean
+    over the same non-NaN σ⁰ samples; the truncated edge is dropped."""
+    rng = np.random.default_rng(r * 10 + a)
+    lines, samples = 9 * a + a - 1, 11 * r + r - 1
+    d = rng.exponential(0.2, (lines, samples)).astype(np.float32)
+    nz = rng.exponential(0.01, (lines, samples)).astype(np.float32)
+    d[rng.random(d.shape) < 0.2] = np.nan
+    d[:a, :r] = np.nan  # an all-NaN block
+    st, od, on, info = _ml(d, r, a, ati=2e-3, nesz=nz)
+    assert st == 0 and od.shape == (lines // a, samples // r)
+    blk = d[: (lines // a) * a, : (samples // r) * r].astype(np.float64).reshape(lines // a, a, samples // r, r)
+    bn = nz[: (lines // a) * a, : (samples // r) * r].astype(np.float64).reshape(lines // a, a, samples // r, r)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        ed = np.nanmean(blk, axis=(1, 3)).astype(np.float32)
+        en = np.nanmean(np.where(np.isnan(blk), np.nan, bn), axis=(1, 3)).astype(np.float32)
+    assert np.array_equal(np.isnan(od), np.isnan(ed)) and np.array_equal(np.isnan(on), np.isnan(en))
+    assert _ulp_diff(od, ed) <= 1 and _ulp_diff(on, en) <= 1
+
+
 # ── full chain invariants on a small synthetic scene ──────────────────────
 def test_small_scene_chain_invariants():
     sc = synthetic.scene_small_plain(2, lines=256, samples=512, out_rows=160, out_cols=160, dem_side=256)
