@@ -50,13 +50,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32", "cog", "multilook"],
+    ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32", "cog", "multilook",
+                                                "grd"],
                     default="iw_full",
                     help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
                          "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request; "
                          "fp32: the FP32 scene-centred variant's errors and speed against FP64; "
                          "cog: the COG export (pyramid + tile pack, and the whole file write) of the "
-                         "full-size dB product; multilook: the pre-TC multilook of the full merged image")
+                         "full-size dB product; multilook: the pre-TC multilook of the full merged image; "
+                         "grd: the ground-range projection of the full merged image")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
@@ -135,6 +137,8 @@ def main():
         return bench_cog(args, dev)
     if args.workload == "multilook":
         return bench_multilook(args, dev)
+    if args.workload == "grd":
+        return bench_grd(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -934,6 +938,92 @@ def bench_multilook(args, dev):
     }
     print(json.dumps(line), flush=True)
 
+
+def bench_grd(args, dev):
+    """Ground-range projection (to_ground_range, ground_range.rs:303-442) of
+    the benchmark scene's merged σ⁰ (13100 x 60814) to 10 m, with a synthetic
+    3-swath geolocation grid over its range (incidence 30-46°), inputs in
+    HBM.  Azimuth spacing 14.08 m (1 look, the IW case) and 2.0 m (5 looks)."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from sardine_amd import _native, abi, api, synthetic
+    from sardine_amd.workload import DeviceScene
+
+    L = _native.lib()
+    stream = torch.cuda.current_stream(dev)
+    sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+    ds = DeviceScene.build(sc, dev)
+    m = ds.prepare_merged(stream=stream)
+    torch.cuda.synchronize()
+    lines, samples = m.lines, m.samples
+    per = samples // 3
+    grids = []
+    for k in range(3):
+        c0, c1 = max(k * per - 40, 0), ((k + 1) * per - 1 if k < 2 else samples - 1)
+        grids.append([(ln, m.near_slant_range_time_s + c * m.range_pixel_spacing_s, 30.0 + 16.0 * c / samples)
+                      for ln in (0, lines // 2, lines - 1) for c in np.linspace(c0, c1, 21)])
+    res = {}
+    L.sardine_set_profiling(1)
+    keep = None
+    for az in (14.08, 2.0):
+        g = api.to_ground_range(m, grids, 10.0, az, stream=stream)
+        for _ in range(max(1, args.warmup)):
+            api.to_ground_range(m, grids, 10.0, az, stream=stream, out=g.data)
+        settle_gc()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            api.to_ground_range(m, grids, 10.0, az, stream=stream, out=g.data)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        k = float(np.mean(recent_kernel_ms(L, abi.KERNEL_GRD, args.steps)))
+        # two taps per output column and input line are read (the map makes
+        # them adjacent columns), one value per output pixel written
+        taps = g.lines * g.azimuth_looks * g.samples * 8 + g.lines * g.samples * 4
+        # output columns sit ~2.6 input columns (~10 B) apart, so every 64 B
+        # DRAM segment of every input line read holds taps: the least HBM
+        # traffic is those lines whole plus the output
+        alg = g.lines * g.azimuth_looks * samples * 4 + g.lines * g.samples * 4
+        res[f"az{az}"] = {"out": [g.lines, g.samples], "azimuth_looks": g.azimuth_looks, "kernel_ms": round(k, 4),
+                          "call_ms": round(e0.elapsed_time(e1) / args.steps, 4), "algorithmic_bytes": alg,
+                          "tap_bytes": taps, "GBps": round(alg / (k / 1e3) / 1e9, 1),
+                          "hbm_frac": round(alg / (k / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"grd az {az}: {json.dumps(res[f'az{az}'])}")
+        if keep is None:
+            keep = g
+    L.sardine_set_profiling(0)
+    cpu = None
+    if not args.no_cpu_baseline:
+        hd = m.data.cpu().numpy()
+        t = time.perf_counter()
+        st, ref, img = O.to_ground_range(hd, m.info, grids, 10.0, 14.08)
+        cs = time.perf_counter() - t
+        assert st == 0
+        same = bool(np.array_equal(keep.data.cpu().numpy().view(np.uint32), ref.view(np.uint32)))
+        cpu = {"value": round(lines * samples / cs / 1e6, 1), "unit": "input Mpx/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's to_ground_range on the whole merged image ({lines} x {samples}, 14.08 m)",
+               "seconds": round(cs, 3), "device_result_bit_exact": same}
+        log(f"cpu: {json.dumps(cpu)}")
+    main = res["az14.08"]
+    line = {
+        "metric": "ground-range projection input Mpx/s (10 m, 1 azimuth look, inputs in HBM)",
+        "value": round(lines * samples / (main["kernel_ms"] / 1e3) / 1e6, 1), "unit": "Mpx/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": main["call_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "grd_iw_full_13100x60814_10m", "merged": [lines, samples], "out": main["out"]},
+        "roofline": {"kernel": "grd_kernel", "bound": "hbm", "achieved": main["GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": main["hbm_frac"], "traffic": None,
+                     "algorithmic_bytes": main["algorithmic_bytes"], "launch_ms": main["kernel_ms"]},
+        "cases": res,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
 if __name__ == "__main__":
     main()
+
 

@@ -79,6 +79,12 @@ typedef enum sardine_status {
     SARDINE_ERR_EXPORT_INVALID_GEOTRANSFORM = 60,
     SARDINE_ERR_EXPORT_INVALID_TILE_SIZE = 61,
     SARDINE_ERR_EXPORT_IO = 62,
+    /* GroundRangeError (ground_range.rs:31-47) */
+    SARDINE_ERR_GRD_NO_GRID_POINTS = 70,
+    SARDINE_ERR_GRD_EMPTY_IMAGE = 71,
+    SARDINE_ERR_GRD_INVALID_SPACING = 72,
+    SARDINE_ERR_GRD_INVALID_AZIMUTH_SPACING = 73,
+    SARDINE_ERR_GRD_INTERPOLATION_FAILED = 74,
     /* library / device */
     SARDINE_ERR_INVALID_ARGUMENT = 90,
     SARDINE_ERR_NO_DEVICE = 91,
@@ -446,6 +452,43 @@ sardine_status sardine_apply_multilook(const float* data /*[device]*/, const flo
                                        double azimuth_time_interval_s, float* out_data /*[device]*/,
                                        float* out_nesz /*[device]*/, sardine_merged_info* out, void* stream);
 
+/* ── Ground-range (GRD) projection ──────────────────────────────────────── */
+
+/* One geolocation grid point as the ground-range projection reads it
+ * (GeolocationGridPoint, types.rs: line, slant_range_time_s,
+ * incidence_angle_deg). */
+typedef struct sardine_grd_grid_point {
+    double slant_range_time_s;
+    double incidence_angle_deg;
+    uint32_t line;
+    uint32_t pad;
+} sardine_grd_grid_point;
+
+/* GroundRangeImage (ground_range.rs:54-76) without its raster. */
+typedef struct sardine_grd_image {
+    int64_t lines, samples;
+    double range_pixel_spacing_m;
+    double azimuth_pixel_spacing_m;
+    int32_t range_looks;   /* always 1 */
+    int32_t azimuth_looks;
+} sardine_grd_image;
+
+/* to_ground_range(merged, grids, target_spacing_m, azimuth_pixel_spacing_m)
+ * (ground_range.rs:303-442).  The merged image is `data` [device] with the
+ * size and range timing of `merged` (lines, samples, near_slant_range_time_s,
+ * range_pixel_spacing_s, range_pixel_spacing_m).  The grids are n_swaths
+ * consecutive runs of `points`, counts[i] points each.  The incidence profile
+ * (mid-azimuth grid row per swath, ground_range.rs:103-259), the cumulative
+ * ground range and the column map (ground_range.rs:446-490) are host work;
+ * the device resamples every line onto the ground-range columns and averages
+ * azimuth_looks lines in one pass.  `out` [device] holds image->lines x
+ * image->samples floats; with out NULL only `image` is computed (host only).
+ * Errors: SARDINE_ERR_GRD_* as the reference raises them, in its order. */
+sardine_status sardine_to_ground_range(const float* data /*[device]*/, const sardine_merged_info* merged,
+                                       const sardine_grd_grid_point* points, const int32_t* counts,
+                                       int32_t n_swaths, double target_spacing_m, double azimuth_pixel_spacing_m,
+                                       float* out /*[device]*/, sardine_grd_image* image, void* stream);
+
 /* ── dB epilogue ────────────────────────────────────────────────────────── */
 
 /* to_db_inplace(data, noise_floor_linear) (export.rs:59-94).  counts[0] =
@@ -558,7 +601,8 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_TO_DB = 3,
     SARDINE_KERNEL_TC_SEGMENT = 4, /* the per-pixel chain kernel of terrain correction alone */
     SARDINE_KERNEL_COG_PACK = 5,   /* the COG export's pyramid + tile pack (all its launches) */
-    SARDINE_KERNEL_MULTILOOK = 6
+    SARDINE_KERNEL_MULTILOOK = 6,
+    SARDINE_KERNEL_GRD = 7
 } sardine_kernel_family;
 /* Switches the kernel timers of the calling thread on (creating their
  * events) or off. */

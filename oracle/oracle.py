@@ -79,6 +79,11 @@ def lib():
         L.oracle_geotiff_header.argtypes = [i64, i64, i32, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         L.oracle_write_geotiff.argtypes = [C.c_char_p, P, i64, i64, i32, P, i32, i32, i32]
         L.oracle_multilook.argtypes = [P, P, C.POINTER(abi.MergedInfo), i32, i32, d, P, P, C.POINTER(abi.MergedInfo)]
+        L.oracle_grd_incidence_profile.argtypes = [C.POINTER(abi.GrdGridPoint), P, i32, d, d, i64, P]
+        L.oracle_grd_column_map.argtypes = [P, i64, d, i64, P, P, P]
+        L.oracle_grd_column_map.restype = None
+        L.oracle_to_ground_range.argtypes = [P, C.POINTER(abi.MergedInfo), C.POINTER(abi.GrdGridPoint), P, i32, d,
+                                             d, P, C.POINTER(abi.GrdImage)]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
@@ -389,6 +394,53 @@ def multilook(data: np.ndarray, nesz: np.ndarray, info, range_looks: int, azimut
     if st:
         return st, None, None, None
     return st, od[:ol * os_].reshape(ol, os_), on[:ol * os_].reshape(ol, os_), out
+
+
+def grd_points(grids):
+    """[[(line, slant_time_s, incidence_deg), ...] per swath] → (points, counts)."""
+    flat = [p for g in grids for p in g]
+    pts = (abi.GrdGridPoint * max(len(flat), 1))()
+    for i, (line, t, a) in enumerate(flat):
+        pts[i] = abi.GrdGridPoint(t, a, line, 0)
+    counts = np.array([len(g) for g in grids] or [0], np.int32)
+    return pts, counts
+
+
+def grd_incidence_profile(grids, near_t: float, rps_s: float, samples: int):
+    """build_incidence_profile (ground_range.rs:103-155) → (status, radians)."""
+    pts, counts = grd_points(grids)
+    out = np.empty(samples, np.float64)
+    st = lib().oracle_grd_incidence_profile(pts, _ptr(counts), len(grids), near_t, rps_s, samples, _ptr(out))
+    return st, out
+
+
+def grd_column_map(cum: np.ndarray, target: float, out_samples: int):
+    """build_ground_to_slant_mapping (ground_range.rs:446-490) → (kind, left, frac)."""
+    cum = np.ascontiguousarray(cum, np.float64)
+    kind = np.empty(out_samples, np.int32)
+    left = np.empty(out_samples, np.int64)
+    frac = np.empty(out_samples, np.float64)
+    lib().oracle_grd_column_map(_ptr(cum), cum.size, target, out_samples, _ptr(kind), _ptr(left), _ptr(frac))
+    return kind, left, frac
+
+
+def to_ground_range(data: np.ndarray, info, grids, target_spacing_m: float, azimuth_pixel_spacing_m: float,
+                    shape_only: bool = False):
+    """to_ground_range (ground_range.rs:303-442) → (status, raster, GrdImage)."""
+    data = np.ascontiguousarray(data, np.float32)
+    info = abi.MergedInfo.from_buffer_copy(info)
+    if data.ndim == 2:
+        info.lines, info.samples = data.shape
+    pts, counts = grd_points(grids)
+    img = abi.GrdImage()
+    st = lib().oracle_to_ground_range(None, C.byref(info), pts, _ptr(counts), len(grids), target_spacing_m,
+                                      azimuth_pixel_spacing_m, None, C.byref(img))
+    if st or shape_only:
+        return st, None, img
+    out = np.empty(max(img.lines * img.samples, 1), np.float32)
+    st = lib().oracle_to_ground_range(_ptr(data), C.byref(info), pts, _ptr(counts), len(grids), target_spacing_m,
+                                      azimuth_pixel_spacing_m, _ptr(out), C.byref(img))
+    return st, out[: img.lines * img.samples].reshape(img.lines, img.samples), img
 
 
 # ── the full chain on a synthetic scene (host) ──────────────────────────

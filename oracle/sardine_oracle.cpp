@@ -30,6 +30,9 @@
 //   apply_multilook            run.rs:1449-1588 (pass-through, uniform and
 //                              distinct averages, NaN exclusion, all-NaN
 //                              block, geometry, azimuth shift, errors)
+//   to_ground_range            ground_range.rs:506-724 (ground spacing, column
+//                              map monotonic / covering, range interpolator,
+//                              uniform synthetic scene, the three rejections)
 //   striped GeoTIFF writers    export.rs:1276-1580 (geotransform errors, key
 //                              offsets, BigTIFF threshold and layout, u8 tags,
 //                              the no-CRS IFD, UTM keys)
@@ -1797,6 +1800,216 @@ int oracle_multilook(const float* data, const float* nesz, const sardine_merged_
             const float qnan = std::numeric_limits<float>::quiet_NaN();
             out_data[l * os + c] = count ? (float)(sum / (double)count) : qnan;
             out_nesz[l * os + c] = count ? (float)(sum_n / (double)count) : qnan;
+        }
+    return 0;
+}
+
+}  // extern "C"
+
+// ── ground-range projection (ground_range.rs:82-490) ───────────────────────
+namespace {
+
+// RangeInterpolator (ground_range.rs:157-236): the mid-azimuth grid row of
+// one swath as (slant time, incidence °), sorted by slant time.
+struct GrdInterp {
+    std::vector<double> t, a;
+    double tmin = 0, tmax = 0;
+    // interpolate (ground_range.rs:203-235)
+    bool at(double x, double* out) const {
+        if (x < tmin || x > tmax) return false;
+        const size_t i = (size_t)(std::lower_bound(t.begin(), t.end(), x) - t.begin());
+        if (i < t.size() && t[i] == x) {
+            *out = a[i];
+            return true;
+        }
+        if (i == 0) {
+            *out = a[0];
+            return true;
+        }
+        if (i >= t.size()) {
+            *out = a.back();
+            return true;
+        }
+        const double f = (x - t[i - 1]) / (t[i] - t[i - 1]);
+        *out = a[i - 1] + f * (a[i] - a[i - 1]);
+        return true;
+    }
+};
+
+GrdInterp grd_interp(const sardine_grd_grid_point* p, int32_t n) {
+    std::vector<uint32_t> lines;
+    for (int32_t i = 0; i < n; ++i) lines.push_back(p[i].line);
+    std::sort(lines.begin(), lines.end());
+    lines.erase(std::unique(lines.begin(), lines.end()), lines.end());
+    const uint32_t mid = lines[lines.size() / 2];
+    std::vector<std::pair<double, double>> pts;
+    for (int32_t i = 0; i < n; ++i)
+        if (p[i].line == mid) pts.push_back({p[i].slant_range_time_s, p[i].incidence_angle_deg});
+    std::stable_sort(pts.begin(), pts.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    GrdInterp g;
+    for (auto& q : pts) {
+        g.t.push_back(q.first);
+        g.a.push_back(q.second);
+    }
+    g.tmin = g.t.front();
+    g.tmax = g.t.back();
+    return g;
+}
+
+// interpolate_at_slant_time (ground_range.rs:238-290)
+bool grd_angle(const std::vector<GrdInterp>& gs, double x, double* out) {
+    bool have = false;
+    double best_d = 0, best_a = 0;
+    for (const GrdInterp& g : gs) {
+        double a;
+        if (g.at(x, &a)) {
+            const double d = std::fabs(x - (g.tmin + g.tmax) / 2.0);
+            if (!have || d < best_d) {
+                have = true;
+                best_d = d;
+                best_a = a;
+            }
+        }
+    }
+    if (have) {
+        *out = best_a;
+        return true;
+    }
+    const GrdInterp* near = nullptr;
+    double near_d = 0;
+    for (const GrdInterp& g : gs) {
+        const double d = x < g.tmin ? g.tmin - x : x - g.tmax;
+        if (!near || d < near_d) {
+            near = &g;
+            near_d = d;
+        }
+    }
+    if (near && near_d < (near->tmax - near->tmin) * 0.01)
+        return near->at(std::min(std::max(x, near->tmin), near->tmax), out);
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+// build_incidence_profile (ground_range.rs:103-155): radians per merged column;
+// returns the first failing column + 1 as a negative number, or a status.
+int oracle_grd_incidence_profile(const sardine_grd_grid_point* points, const int32_t* counts, int32_t n_swaths,
+                                 double near_t, double rps_s, int64_t samples, double* out_rad) {
+    std::vector<GrdInterp> gs;
+    int64_t off = 0;
+    for (int32_t k = 0; k < n_swaths; ++k) {
+        if (counts[k] <= 0) return SARDINE_ERR_GRD_NO_GRID_POINTS;
+        gs.push_back(grd_interp(points + off, counts[k]));
+        off += counts[k];
+    }
+    std::stable_sort(gs.begin(), gs.end(), [](const GrdInterp& x, const GrdInterp& y) { return x.tmin < y.tmin; });
+    for (int64_t c = 0; c < samples; ++c) {
+        double deg;
+        if (!grd_angle(gs, near_t + (double)c * rps_s, &deg)) return SARDINE_ERR_GRD_INTERPOLATION_FAILED;
+        out_rad[c] = deg * (3.14159265358979323846 / 180.0);
+    }
+    return 0;
+}
+
+// build_ground_to_slant_mapping (ground_range.rs:446-490): kind 0 = exact
+// (left = the column), 1 = interpolate between left and left + 1.
+void oracle_grd_column_map(const double* cum, int64_t n, double target, int64_t out_samples, int32_t* kind,
+                           int64_t* left, double* frac) {
+    int64_t start = 0;
+    for (int64_t j = 0; j < out_samples; ++j) {
+        const double g = (double)j * target;
+        const double* b = cum + start;
+        const double* e = cum + n;
+        const double* lb = std::lower_bound(b, e, g);
+        if (lb != e && *lb == g) {
+            kind[j] = 0;
+            left[j] = start + (lb - b);
+            frac[j] = 0.0;
+            start = left[j];
+            continue;
+        }
+        const int64_t idx = start + (lb - b);
+        if (idx == 0) {
+            kind[j] = 0;
+            left[j] = 0;
+            frac[j] = 0.0;
+        } else if (idx >= n) {
+            kind[j] = 0;
+            left[j] = n - 1;
+            frac[j] = 0.0;
+        } else {
+            kind[j] = 1;
+            left[j] = idx - 1;
+            frac[j] = (g - cum[idx - 1]) / (cum[idx] - cum[idx - 1]);
+            start = idx - 1;
+        }
+    }
+}
+
+// to_ground_range (ground_range.rs:303-442).  out == null: only *img.
+int oracle_to_ground_range(const float* data, const sardine_merged_info* m, const sardine_grd_grid_point* points,
+                           const int32_t* counts, int32_t n_swaths, double target, double az_spacing, float* out,
+                           sardine_grd_image* img) {
+    if (m->lines == 0 || m->samples == 0) return SARDINE_ERR_GRD_EMPTY_IMAGE;
+    if (target <= 0.0) return SARDINE_ERR_GRD_INVALID_SPACING;
+    if (!std::isfinite(az_spacing) || az_spacing <= 0.0) return SARDINE_ERR_GRD_INVALID_AZIMUTH_SPACING;
+    const int64_t S = m->samples, Lines = m->lines;
+    std::vector<double> rad((size_t)S);
+    const int st = oracle_grd_incidence_profile(points, counts, n_swaths, m->near_slant_range_time_s,
+                                                m->range_pixel_spacing_s, S, rad.data());
+    if (st) return st;
+    std::vector<double> gsp((size_t)S), cum((size_t)S);
+    for (int64_t c = 0; c < S; ++c) gsp[c] = m->range_pixel_spacing_m / std::sin(rad[c]);
+    double acc = 0.0;
+    cum[0] = 0.0;
+    for (int64_t c = 1; c < S; ++c) {
+        acc += (gsp[c - 1] + gsp[c]) / 2.0;
+        cum[c] = acc;
+    }
+    const double q = std::floor(cum[S - 1] / target);
+    const int64_t os = (q >= 1.0 && q < 9.2e18) ? (int64_t)q : 0;  // `as usize`: NaN and negatives → 0
+    if (os == 0) return SARDINE_ERR_GRD_INVALID_SPACING;
+    const double aq = std::max(std::round(target / az_spacing), 1.0);
+    const int64_t A = (int64_t)aq;
+    img->lines = Lines / A;
+    img->samples = os;
+    img->range_pixel_spacing_m = target;
+    img->azimuth_pixel_spacing_m = az_spacing * (double)A;
+    img->range_looks = 1;
+    img->azimuth_looks = (int32_t)A;
+    if (!out) return 0;
+    std::vector<int32_t> kind((size_t)os);
+    std::vector<int64_t> left((size_t)os);
+    std::vector<double> frac((size_t)os);
+    oracle_grd_column_map(cum.data(), S, target, os, kind.data(), left.data(), frac.data());
+    std::vector<float> proj((size_t)(Lines * os));
+    for (int64_t l = 0; l < Lines; ++l)
+        for (int64_t j = 0; j < os; ++j) {
+            const float* row = data + l * S;
+            float v;
+            if (kind[j] == 0) {
+                v = row[left[j]];
+            } else {
+                const float v0 = row[left[j]], v1 = row[left[j] + 1];
+                v = (std::isnan(v0) || std::isnan(v1)) ? std::numeric_limits<float>::quiet_NaN()
+                                                       : v0 + (v1 - v0) * (float)frac[j];
+            }
+            proj[(size_t)(l * os + j)] = v;
+        }
+    for (int64_t ol = 0; ol < img->lines; ++ol)
+        for (int64_t j = 0; j < os; ++j) {
+            double sum = 0.0;
+            uint32_t count = 0;
+            for (int64_t l = ol * A; l < ol * A + A; ++l) {
+                const float v = proj[(size_t)(l * os + j)];
+                if (!std::isnan(v)) {
+                    sum += (double)v;
+                    ++count;
+                }
+            }
+            out[ol * os + j] = count ? (float)(sum / (double)count) : std::numeric_limits<float>::quiet_NaN();
         }
     return 0;
 }

@@ -5,22 +5,21 @@ The compute lives in libsardine_hip.so (hand-written HIP kernels); this
 package mirrors the reference's Rust interface for the path on top of it.
 """
 from . import abi
-from .api import (DemRaster, GeocodedImage, MergedSigma0, ResamplingKernel, SceneInputs, Sigma0Array,
-                  SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_multilook, apply_speckle_filter,
-                  apply_speckle_step, cog_header, cog_layout, cog_pack, deburst_layout, merged_layout,
-                  geotiff_header, output_grid, prepare_merged_scene, synth_cint16, terrain_correction,
-                  to_db_inplace, write_cog_with_crs, write_geotiff_u8_with_crs, write_geotiff_with_crs,
-                  write_tiff_no_crs)
-from .errors import (CalibrationError, DeburstError, DeviceError, ExportError, MergeError, SardineError,
-                     SpeckleError, TerrainCorrectionError)
+from .api import (DemRaster, GeocodedImage, GroundRangeImage, MergedSigma0, ResamplingKernel, SceneInputs,
+                  Sigma0Array, SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_multilook,
+                  apply_speckle_filter, apply_speckle_step, cog_header, cog_layout, cog_pack, deburst_layout,
+                  geotiff_header, merged_layout, output_grid, prepare_merged_scene, synth_cint16,
+                  terrain_correction, to_db_inplace, to_ground_range, write_cog_with_crs, write_geotiff_u8_with_crs,
+                  write_geotiff_with_crs, write_tiff_no_crs)
+from .errors import (CalibrationError, DeburstError, DeviceError, ExportError, GroundRangeError, MergeError,
+                     SardineError, SpeckleError, TerrainCorrectionError)
 
 __all__ = [
-    "abi", "DemRaster", "GeocodedImage", "MergedSigma0", "ResamplingKernel", "SceneInputs", "Sigma0Array",
-    "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_multilook", "apply_speckle_filter",
-    "apply_speckle_step",
-    "cog_header", "cog_layout", "cog_pack", "deburst_layout", "merged_layout", "output_grid",
-    "prepare_merged_scene", "synth_cint16", "terrain_correction", "to_db_inplace", "write_cog_with_crs",
-    "geotiff_header", "write_geotiff_with_crs", "write_geotiff_u8_with_crs", "write_tiff_no_crs",
-    "CalibrationError", "DeburstError", "DeviceError", "ExportError", "MergeError", "SardineError",
-    "SpeckleError", "TerrainCorrectionError",
+    "abi", "DemRaster", "GeocodedImage", "GroundRangeImage", "MergedSigma0", "ResamplingKernel", "SceneInputs",
+    "Sigma0Array", "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_multilook",
+    "apply_speckle_filter", "apply_speckle_step", "cog_header", "cog_layout", "cog_pack", "deburst_layout",
+    "geotiff_header", "merged_layout", "output_grid", "prepare_merged_scene", "synth_cint16", "terrain_correction",
+    "to_db_inplace", "to_ground_range", "write_cog_with_crs", "write_geotiff_u8_with_crs", "write_geotiff_with_crs",
+    "write_tiff_no_crs", "CalibrationError", "DeburstError", "DeviceError", "ExportError", "GroundRangeError",
+    "MergeError", "SardineError", "SpeckleError", "TerrainCorrectionError",
 ]

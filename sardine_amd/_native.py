@@ -63,6 +63,8 @@ def lib():
     L.sardine_write_tiff_no_crs.argtypes = [C.c_char_p, P, i64, i64, P]
     L.sardine_apply_multilook.argtypes = [P, P, C.POINTER(abi.MergedInfo), i32, i32, d, P, P,
                                           C.POINTER(abi.MergedInfo), P]
+    L.sardine_to_ground_range.argtypes = [P, C.POINTER(abi.MergedInfo), C.POINTER(abi.GrdGridPoint), P, i32, d, d,
+                                          P, C.POINTER(abi.GrdImage), P]
     L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]
     L.sardine_set_profiling.argtypes = [C.c_int]
     L.sardine_set_profiling.restype = None

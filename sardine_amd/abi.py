@@ -42,6 +42,11 @@ ERR_MERGE_EMPTY_SWATH = 52
 ERR_EXPORT_INVALID_GEOTRANSFORM = 60
 ERR_EXPORT_INVALID_TILE_SIZE = 61
 ERR_EXPORT_IO = 62
+ERR_GRD_NO_GRID_POINTS = 70
+ERR_GRD_EMPTY_IMAGE = 71
+ERR_GRD_INVALID_SPACING = 72
+ERR_GRD_INVALID_AZIMUTH_SPACING = 73
+ERR_GRD_INTERPOLATION_FAILED = 74
 ERR_INVALID_ARGUMENT = 90
 ERR_NO_DEVICE = 91
 ERR_HIP = 92
@@ -205,6 +210,16 @@ class MergedInfo(C.Structure):
                 ("swath_valid_offset", C.c_int64 * MAX_SWATHS)]
 
 
+class GrdGridPoint(C.Structure):
+    _fields_ = [("slant_range_time_s", C.c_double), ("incidence_angle_deg", C.c_double), ("line", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class GrdImage(C.Structure):
+    _fields_ = [("lines", C.c_int64), ("samples", C.c_int64), ("range_pixel_spacing_m", C.c_double),
+                ("azimuth_pixel_spacing_m", C.c_double), ("range_looks", C.c_int32), ("azimuth_looks", C.c_int32)]
+
+
 class CogLevel(C.Structure):
     _fields_ = [("cols", C.c_int64), ("rows", C.c_int64), ("tiles_x", C.c_int64), ("tiles_y", C.c_int64),
                 ("ifd_offset", C.c_uint64), ("tile_offsets_offset", C.c_uint64),
@@ -234,6 +249,7 @@ EXPORTED_SYMBOLS = (
     "sardine_apply_calibration",
     "sardine_to_db_inplace",
     "sardine_apply_multilook",
+    "sardine_to_ground_range",
     "sardine_cog_layout_compute",
     "sardine_cog_header",
     "sardine_cog_pack",
@@ -263,4 +279,4 @@ EXPORTED_SYMBOLS = (
     "sardine_server_destroy",
 )
 KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT, KERNEL_COG_PACK = 0, 1, 2, 3, 4, 5
-KERNEL_MULTILOOK = 6
+KERNEL_MULTILOOK, KERNEL_GRD = 6, 7

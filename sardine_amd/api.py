@@ -13,6 +13,8 @@ Names and argument meaning follow the Rust crate:
   deburst_layout(...)                        deburst.rs:172-305
   apply_multilook(merged, range_looks, azimuth_looks, ati) -> MergedSigma0
         scene_prep.rs:1060-1161
+  to_ground_range(merged, grids, target_spacing_m, azimuth_pixel_spacing_m) -> GroundRangeImage
+        ground_range.rs:303-442
   to_db_inplace(data, noise_floor_linear)    export.rs:59-94
   write_cog_with_crs(path, data, cols, rows, geotransform, crs, tile_size)
         export.rs:329-354 (cog_layout / cog_header / cog_pack expose its parts)
@@ -403,6 +405,60 @@ def apply_multilook(merged: MergedSigma0, range_looks: int, azimuth_looks: int, 
     return MergedSigma0(od, on, res.near_slant_range_time_s, res.range_pixel_spacing_m, res.azimuth_start_time_us,
                         res.range_pixel_spacing_s, res.cal_lut_extrapolation_gap_px,
                         res.noise_lut_extrapolation_gap_px, res)
+
+
+@dataclass
+class GroundRangeImage:
+    """GroundRangeImage (ground_range.rs:54-76) with a device raster."""
+    data: object
+    lines: int
+    samples: int
+    range_pixel_spacing_m: float
+    azimuth_pixel_spacing_m: float
+    range_looks: int
+    azimuth_looks: int
+
+
+def _grd_grids(grids):
+    flat = []
+    counts = []
+    for g in grids:
+        pts = g[1] if isinstance(g, tuple) and len(g) == 2 and not isinstance(g[0], (int, float)) else g
+        n = 0
+        for p in pts:
+            if isinstance(p, tuple):
+                line, t, a = p
+            else:
+                line, t, a = p.line, p.slant_range_time_s, p.incidence_angle_deg
+            flat.append(abi.GrdGridPoint(float(t), float(a), int(line), 0))
+            n += 1
+        counts.append(n)
+    arr = (abi.GrdGridPoint * max(len(flat), 1))(*flat)
+    cnt = (C.c_int32 * max(len(counts), 1))(*counts)
+    return arr, cnt, len(counts)
+
+
+def to_ground_range(merged: MergedSigma0, grids, target_spacing_m: float, azimuth_pixel_spacing_m: float,
+                    stream=None, out=None) -> GroundRangeImage:
+    """to_ground_range (ground_range.rs:303-442).  `grids`: one entry per
+    subswath, either (id, points) or points, each point a (line, slant range
+    time s, incidence °) tuple or an object with those attributes."""
+    torch = _torch()
+    info = abi.MergedInfo()
+    info.lines, info.samples = merged.lines, merged.samples
+    info.near_slant_range_time_s = merged.near_slant_range_time_s
+    info.range_pixel_spacing_s = merged.range_pixel_spacing_s
+    info.range_pixel_spacing_m = merged.range_pixel_spacing_m
+    pts, cnt, n = _grd_grids(grids)
+    img = abi.GrdImage()
+    _check(lib().sardine_to_ground_range(None, C.byref(info), pts, cnt, n, target_spacing_m, azimuth_pixel_spacing_m,
+                                         None, C.byref(img), None))
+    o = out if out is not None else torch.empty((img.lines, img.samples), dtype=torch.float32,
+                                                device=merged.data.device)
+    _check(lib().sardine_to_ground_range(merged.data.data_ptr(), C.byref(info), pts, cnt, n, target_spacing_m,
+                                         azimuth_pixel_spacing_m, o.data_ptr(), C.byref(img), _stream_ptr(stream)))
+    return GroundRangeImage(o, img.lines, img.samples, img.range_pixel_spacing_m, img.azimuth_pixel_spacing_m,
+                            img.range_looks, img.azimuth_looks)
 
 
 def to_db_inplace(data, noise_floor_linear: float, stream=None):

@@ -58,6 +58,8 @@ extern "C" hipError_t sd_launch_cog_level(const float* src, int src_tiled, const
 extern "C" hipError_t sd_launch_multilook(const float* data, const float* nesz, int64_t in_samples,
                                           int64_t out_lines, int64_t out_samples, int r, int a, float* od,
                                           float* on, hipStream_t s);
+extern "C" hipError_t sd_launch_grd(const float* data, int64_t in_samples, const void* map, int64_t out_lines,
+                                    int64_t out_samples, int a, float* out, hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -67,7 +69,7 @@ thread_local std::string g_last_error;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
-constexpr int kFamilies = 7;
+constexpr int kFamilies = 8;
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
 // waits here: sardine_last_kernel_ms waits for the end event when read.
@@ -1387,6 +1389,275 @@ sardine_status sardine_apply_multilook(const float* data, const float* nesz, con
     }
     kt.stop();
     *out = o;
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── ground-range (GRD) projection (ground_range.rs:82-490) ──────────────
+namespace {
+
+// One swath's incidence profile along slant range: the grid's mid-azimuth
+// row, sorted by slant time (RangeInterpolator, ground_range.rs:157-236).
+struct IncidenceRow {
+    std::vector<double> t, deg;
+    double lo = 0.0, hi = 0.0;
+    double centre() const { return (lo + hi) / 2.0; }
+    // Linear in slant time inside [lo, hi]; an exact node returns its value.
+    bool eval(double x, double* out) const {
+        if (x < lo || x > hi) return false;
+        const size_t k = (size_t)(std::lower_bound(t.begin(), t.end(), x) - t.begin());
+        if (k < t.size() && t[k] == x) *out = deg[k];
+        else if (k == 0) *out = deg[0];
+        else if (k >= t.size()) *out = deg.back();
+        else *out = deg[k - 1] + (x - t[k - 1]) / (t[k] - t[k - 1]) * (deg[k] - deg[k - 1]);
+        return true;
+    }
+};
+
+IncidenceRow incidence_row(const sardine_grd_grid_point* p, int32_t n) {
+    std::vector<uint32_t> lines(n);
+    for (int32_t i = 0; i < n; ++i) lines[i] = p[i].line;
+    std::sort(lines.begin(), lines.end());
+    lines.erase(std::unique(lines.begin(), lines.end()), lines.end());
+    const uint32_t mid = lines[lines.size() / 2];
+    std::vector<std::pair<double, double>> row;
+    for (int32_t i = 0; i < n; ++i)
+        if (p[i].line == mid) row.emplace_back(p[i].slant_range_time_s, p[i].incidence_angle_deg);
+    std::stable_sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    IncidenceRow r;
+    for (const auto& q : row) {
+        r.t.push_back(q.first);
+        r.deg.push_back(q.second);
+    }
+    r.lo = r.t.front();
+    r.hi = r.t.back();
+    return r;
+}
+
+// interpolate_at_slant_time (ground_range.rs:238-290): the covering swath
+// whose centre is nearest (the first on a tie), else the nearest swath's
+// edge value when within 1 % of its width.
+bool incidence_at(const std::vector<IncidenceRow>& rows, double x, double* deg) {
+    const IncidenceRow* best = nullptr;
+    double best_d = 0.0, best_v = 0.0;
+    for (const IncidenceRow& r : rows) {
+        double v;
+        if (!r.eval(x, &v)) continue;
+        const double d = std::fabs(x - r.centre());
+        if (!best || d < best_d) {
+            best = &r;
+            best_d = d;
+            best_v = v;
+        }
+    }
+    if (best) {
+        *deg = best_v;
+        return true;
+    }
+    const IncidenceRow* near = nullptr;
+    double near_d = 0.0;
+    for (const IncidenceRow& r : rows) {
+        const double d = x < r.lo ? r.lo - x : x - r.hi;
+        if (!near || d < near_d) {
+            near = &r;
+            near_d = d;
+        }
+    }
+    if (!near || !(near_d < (near->hi - near->lo) * 0.01)) return false;
+    return near->eval(std::min(std::max(x, near->lo), near->hi), deg);
+}
+
+struct GrdColumnHost {  // layout of sd::GrdColumn (multilook.hip)
+    int32_t left;
+    float frac;
+};
+
+// Everything of to_ground_range except the per-line work: validation, the
+// per-column incidence, the cumulative ground range, the output shape and
+// (when map != null) the column map.
+sardine_status grd_plan(const sardine_merged_info* m, const sardine_grd_grid_point* points, const int32_t* counts,
+                        int32_t n_swaths, double target, double az_spacing, sardine_grd_image* img,
+                        std::vector<GrdColumnHost>* map) {
+    if (m->lines == 0 || m->samples == 0)
+        return fail(SARDINE_ERR_GRD_EMPTY_IMAGE, "merged image is empty (%lld lines x %lld samples)",
+                    (long long)m->lines, (long long)m->samples);
+    if (m->lines < 0 || m->samples < 0) return fail(SARDINE_ERR_INVALID_ARGUMENT, "negative image size");
+    if (target <= 0.0) return fail(SARDINE_ERR_GRD_INVALID_SPACING, "target pixel spacing must be positive, got %g", target);
+    if (!std::isfinite(az_spacing) || az_spacing <= 0.0)
+        return fail(SARDINE_ERR_GRD_INVALID_AZIMUTH_SPACING,
+                    "input azimuth pixel spacing must be positive and finite, got %g", az_spacing);
+    if (n_swaths < 0 || (n_swaths > 0 && (!points || !counts))) return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad grids");
+    std::vector<IncidenceRow> rows;
+    int64_t off = 0;
+    for (int32_t k = 0; k < n_swaths; ++k) {
+        if (counts[k] <= 0) return fail(SARDINE_ERR_GRD_NO_GRID_POINTS, "no geolocation grid points for swath %d", k);
+        rows.push_back(incidence_row(points + off, counts[k]));
+        off += counts[k];
+    }
+    std::stable_sort(rows.begin(), rows.end(), [](const IncidenceRow& a, const IncidenceRow& b) { return a.lo < b.lo; });
+    const int64_t S = m->samples;
+    if (S > INT32_MAX) return fail(SARDINE_ERR_INVALID_ARGUMENT, "merged image wider than 2^31-1 samples");
+    // ground spacing per column: slant spacing / sin(incidence), incidence
+    // in radians as deg * (π / 180) (f64::to_radians)
+    std::vector<double> cum((size_t)S);
+    double prev_gs = 0.0, acc = 0.0;
+    for (int64_t c = 0; c < S; ++c) {
+        double deg;
+        const double t = m->near_slant_range_time_s + (double)c * m->range_pixel_spacing_s;
+        if (!incidence_at(rows, t, &deg))
+            return fail(SARDINE_ERR_GRD_INTERPOLATION_FAILED,
+                        "incidence angle interpolation failed at column %lld: slant time %.6e s outside all grid "
+                        "ranges", (long long)c, t);
+        const double gs = m->range_pixel_spacing_m / std::sin(deg * (3.14159265358979323846 / 180.0));
+        if (c > 0) acc += (prev_gs + gs) / 2.0;
+        cum[(size_t)c] = acc;
+        prev_gs = gs;
+    }
+    const double q = std::floor(cum[(size_t)S - 1] / target);
+    if (!(q >= 1.0)) return fail(SARDINE_ERR_GRD_INVALID_SPACING, "target pixel spacing must be positive, got %g", target);
+    if (q > (double)INT32_MAX)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "%g ground-range columns exceed 2^31-1", q);
+    const int64_t os = (int64_t)q;
+    const int64_t A = (int64_t)std::max(std::round(target / az_spacing), 1.0);
+    if (A > INT32_MAX) return fail(SARDINE_ERR_INVALID_ARGUMENT, "azimuth looks exceed 2^31-1");
+    img->lines = m->lines / A;
+    img->samples = os;
+    img->range_pixel_spacing_m = target;
+    img->azimuth_pixel_spacing_m = az_spacing * (double)A;
+    img->range_looks = 1;
+    img->azimuth_looks = (int32_t)A;
+    if (!map) return SARDINE_OK;
+    // build_ground_to_slant_mapping (ground_range.rs:446-490): a forward
+    // search from the last bracket (the table is increasing)
+    map->resize((size_t)os);
+    int64_t from = 0;
+    for (int64_t j = 0; j < os; ++j) {
+        const double g = (double)j * target;
+        const int64_t k = (int64_t)(std::lower_bound(cum.begin() + from, cum.end(), g) - cum.begin());
+        GrdColumnHost& e = (*map)[(size_t)j];
+        if (k < S && cum[(size_t)k] == g) {
+            e = {(int32_t)k, -1.0f};
+            from = k;
+        } else if (k == 0) {
+            e = {0, -1.0f};
+        } else if (k >= S) {
+            e = {(int32_t)(S - 1), -1.0f};
+        } else {
+            const double f = (g - cum[(size_t)k - 1]) / (cum[(size_t)k] - cum[(size_t)k - 1]);
+            e = {(int32_t)(k - 1), (float)f};  // the kernel weighs with frac as f32
+            from = k - 1;
+        }
+    }
+    return SARDINE_OK;
+}
+
+struct GrdPlanned {
+    sardine_grd_image img;
+    std::vector<GrdColumnHost> map;
+};
+
+// The plan depends only on the image geometry, the grids and the two
+// spacings, and a scene is usually projected more than once (bands,
+// re-runs, services), so plans are kept per input bytes (8, least recently
+// used first out).  Failing inputs are not kept.
+class GrdPlanCache {
+   public:
+    sardine_status get(const sardine_merged_info* m, const sardine_grd_grid_point* points, const int32_t* counts,
+                       int32_t n_swaths, double target, double az, std::shared_ptr<const GrdPlanned>* out) {
+        std::vector<char> key;
+        auto add = [&](const void* p, size_t n) {
+            key.insert(key.end(), static_cast<const char*>(p), static_cast<const char*>(p) + n);
+        };
+        add(&m->lines, sizeof m->lines);
+        add(&m->samples, sizeof m->samples);
+        add(&m->near_slant_range_time_s, sizeof(double));
+        add(&m->range_pixel_spacing_s, sizeof(double));
+        add(&m->range_pixel_spacing_m, sizeof(double));
+        add(&target, sizeof target);
+        add(&az, sizeof az);
+        add(&n_swaths, sizeof n_swaths);
+        int64_t n_pts = 0;
+        if (n_swaths > 0 && counts) {
+            add(counts, sizeof(int32_t) * (size_t)n_swaths);
+            for (int32_t k = 0; k < n_swaths; ++k) n_pts += std::max(counts[k], 0);
+        }
+        if (n_pts > 0 && points) add(points, sizeof(sardine_grd_grid_point) * (size_t)n_pts);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            ++tick_;
+            for (Entry& e : entries_)
+                if (e.key == key) {
+                    e.used = tick_;
+                    *out = e.plan;
+                    return SARDINE_OK;
+                }
+        }
+        auto plan = std::make_shared<GrdPlanned>();
+        sardine_status st = grd_plan(m, points, counts, n_swaths, target, az, &plan->img, &plan->map);
+        if (st) return st;
+        std::lock_guard<std::mutex> g(mu_);
+        Entry ne{std::move(key), plan, ++tick_};
+        if (entries_.size() < kEntries) {
+            entries_.push_back(std::move(ne));
+        } else {
+            auto lru = std::min_element(entries_.begin(), entries_.end(),
+                                        [](const Entry& a, const Entry& b) { return a.used < b.used; });
+            *lru = std::move(ne);
+        }
+        *out = plan;
+        return SARDINE_OK;
+    }
+
+   private:
+    struct Entry {
+        std::vector<char> key;
+        std::shared_ptr<const GrdPlanned> plan;
+        uint64_t used = 0;
+    };
+    static constexpr size_t kEntries = 8;
+    std::mutex mu_;
+    std::vector<Entry> entries_;
+    uint64_t tick_ = 0;
+};
+
+GrdPlanCache& grd_plan_cache() {
+    static GrdPlanCache* c = new GrdPlanCache();
+    return *c;
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_to_ground_range(const float* data, const sardine_merged_info* merged,
+                                       const sardine_grd_grid_point* points, const int32_t* counts,
+                                       int32_t n_swaths, double target_spacing_m, double azimuth_pixel_spacing_m,
+                                       float* out, sardine_grd_image* image, void* stream) {
+    if (!merged || !image) return fail(SARDINE_ERR_INVALID_ARGUMENT, "merged info or image is null");
+    sardine_status st;
+    if (!out) {
+        sardine_grd_image img;
+        if ((st = grd_plan(merged, points, counts, n_swaths, target_spacing_m, azimuth_pixel_spacing_m, &img,
+                           nullptr)))
+            return st;
+        *image = img;
+        return SARDINE_OK;
+    }
+    std::shared_ptr<const GrdPlanned> plan;
+    if ((st = grd_plan_cache().get(merged, points, counts, n_swaths, target_spacing_m, azimuth_pixel_spacing_m,
+                                   &plan)))
+        return st;
+    if (!data) return fail(SARDINE_ERR_INVALID_ARGUMENT, "data is null");
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const sardine_grd_image& img = plan->img;
+    DevBuf dm;
+    SD_HIP(upload(dm, plan->map.data(), plan->map.size(), s));
+    KernelTimer kt(SARDINE_KERNEL_GRD, s);
+    SD_HIP(sd_launch_grd(data, merged->samples, dm.p, img.lines, img.samples, img.azimuth_looks, out, s));
+    kt.stop();
+    *image = img;
     return SARDINE_OK;
 }
 

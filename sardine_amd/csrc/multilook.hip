@@ -1,4 +1,8 @@
-// multilook.hip — pre-TC multilook of the merged σ⁰ and NESZ
+// multilook.hip — the decimating radar-geometry stages: the pre-TC
+// multilook of the merged σ⁰ and NESZ, and the ground-range (GRD)
+// projection's resampling + azimuth multilook.
+//
+// Multilook
 // (apply_multilook, scene_prep.rs:1060-1161): NaN-aware boxcar decimation by
 // range_looks x azimuth_looks, output pixel (l, c) = mean of the non-NaN σ⁰
 // samples of input block [l*A, l*A+A) x [c*R, c*R+R), NESZ averaged over the
@@ -111,7 +115,89 @@ hipError_t launch_multilook(const float* data, const float* nesz, int64_t in_sam
     return hipGetLastError();
 }
 
+// Ground-range projection (to_ground_range, ground_range.rs:364-428): output
+// column j reads merged column `left` (exact) or interpolates between `left`
+// and `left + 1` with the f32 weight `frac` (NaN if either tap is NaN); the
+// azimuth multilook then averages the non-NaN projected values of A
+// consecutive lines in f64.  Both steps run per thread, so the projected
+// raster is never stored.  The column map is the host's (exact: frac < 0).
+struct GrdColumn {
+    int32_t left;
+    float frac;  // < 0: exact column
+};
+
+// One output column per thread.  With one azimuth look (the IW case) a
+// thread takes kGrdRows output lines, gridDim.y apart, and issues all their
+// tap loads before it uses any.
+constexpr int kGrdRows = 4;
+
+__device__ __forceinline__ float grd_tap(const float* row, const GrdColumn& m, bool exact) {
+    if (exact) return row[0];
+    const float v0 = row[0], v1 = row[1];
+    return (__builtin_isnan(v0) || __builtin_isnan(v1)) ? __builtin_nanf("") : v0 + (v1 - v0) * m.frac;
+}
+
+__global__ __launch_bounds__(256) void grd_kernel(const float* __restrict__ data, int64_t in_samples,
+                                                  const GrdColumn* __restrict__ map, int64_t out_lines,
+                                                  int64_t out_samples, int a_looks, float* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= out_samples) return;
+    const GrdColumn m = map[j];
+    const bool exact = m.frac < 0.0f;
+    const float qnan = __builtin_nanf("");
+    const int64_t step = gridDim.y;
+    if (a_looks == 1) {
+        for (int64_t l0 = blockIdx.y; l0 < out_lines; l0 += kGrdRows * step) {
+            float v0[kGrdRows], v1[kGrdRows];
+#pragma unroll
+            for (int u = 0; u < kGrdRows; ++u) {
+                const int64_t l = l0 + u * step;
+                const float* row = data + (l < out_lines ? l : l0) * in_samples + m.left;
+                v0[u] = row[0];
+                v1[u] = exact ? 0.0f : row[1];
+            }
+#pragma unroll
+            for (int u = 0; u < kGrdRows; ++u) {
+                const int64_t l = l0 + u * step;
+                if (l >= out_lines) break;
+                float v = v0[u];
+                if (!exact) v = (__builtin_isnan(v0[u]) || __builtin_isnan(v1[u])) ? qnan : v0[u] + (v1[u] - v0[u]) * m.frac;
+                // one look: the reference's f64 accumulation of a single value
+                // from +0.0 (so −0 becomes +0), divided by a count of 1
+                double sum = 0.0;
+                sum += (double)v;
+                out[l * out_samples + j] = __builtin_isnan(v) ? qnan : (float)(sum / 1.0);
+            }
+        }
+        return;
+    }
+    for (int64_t l = blockIdx.y; l < out_lines; l += step) {
+        double sum = 0.0;
+        unsigned count = 0;
+        const float* row = data + l * a_looks * in_samples + m.left;
+        for (int da = 0; da < a_looks; ++da, row += in_samples) {
+            const float v = grd_tap(row, m, exact);
+            if (!__builtin_isnan(v)) {
+                sum += (double)v;
+                ++count;
+            }
+        }
+        out[l * out_samples + j] = count ? (float)(sum / (double)count) : qnan;
+    }
+}
+
 }  // namespace sd
+
+extern "C" hipError_t sd_launch_grd(const float* data, int64_t in_samples, const void* map, int64_t out_lines,
+                                    int64_t out_samples, int a, float* out, hipStream_t s) {
+    if (out_lines <= 0 || out_samples <= 0) return hipSuccess;
+    const unsigned gx = (unsigned)((out_samples + 255) / 256);
+    const int64_t rows = a == 1 ? (out_lines + sd::kGrdRows - 1) / sd::kGrdRows : out_lines;
+    const unsigned gy = (unsigned)(rows < 65535 ? rows : 65535);
+    hipLaunchKernelGGL(sd::grd_kernel, dim3(gx, gy), dim3(256), 0, s, data, in_samples,
+                       static_cast<const sd::GrdColumn*>(map), out_lines, out_samples, a, out);
+    return hipGetLastError();
+}
 
 // Sizes as the host validated them: out_lines = lines / a, out_samples =
 // samples / r, both >= 1.

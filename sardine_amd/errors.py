@@ -6,6 +6,7 @@
   DeburstError            sardine/src/deburst.rs:56-121
   MergeError              sardine/src/merge_subswaths.rs:62-81
   ExportError             sardine/src/export.rs:26-39
+  GroundRangeError        sardine/src/ground_range.rs:31-47
 Each carries the C-ABI status code (`.status`) and its name (`.variant`).
 """
 from __future__ import annotations
@@ -44,6 +45,10 @@ class ExportError(SardineError):
     pass
 
 
+class GroundRangeError(SardineError):
+    pass
+
+
 class DeviceError(SardineError):
     pass
 
@@ -63,4 +68,6 @@ def raise_for(status: int, message: str):
         raise MergeError(status, message)
     if 60 <= status < 70:
         raise ExportError(status, message)
+    if 70 <= status < 80:
+        raise GroundRangeError(status, message)
     raise DeviceError(status, message)

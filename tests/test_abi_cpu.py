@@ -375,3 +375,71 @@ def test_multilook_argument_errors_precede_device_check():
         d = np.zeros(8, np.float32)
         assert L.sardine_apply_multilook(d.ctypes.data, d.ctypes.data, C.byref(src), 2, 1, 0.0, d.ctypes.data,
                                          d.ctypes.data, C.byref(out), None) == abi.ERR_NO_DEVICE
+
+
+def _grd_info(lines, samples):
+    i = abi.MergedInfo()
+    i.lines, i.samples = lines, samples
+    i.near_slant_range_time_s = 5.0e-3
+    i.range_pixel_spacing_s = 2.33 * 2.0 / 299_792_458.0
+    i.range_pixel_spacing_m = 2.33
+    return i
+
+
+def _grd_grids(samples, n_sw=3, lines=(0, 50, 99)):
+    rps = 2.33 * 2.0 / 299_792_458.0
+    per = samples // n_sw
+    grids = []
+    for k in range(n_sw):
+        c0, c1 = k * per - (8 if k else 0), (k + 1) * per - 1 if k < n_sw - 1 else samples - 1
+        grids.append([(ln, 5.0e-3 + c * rps, 30.0 + 16.0 * c / samples + 0.01 * k)
+                      for ln in lines for c in np.linspace(c0, c1, 6)])
+    return grids
+
+
+@pytest.mark.parametrize("samples,target,az", [(200, 10.0, 14.08), (3001, 10.0, 3.4), (999, 25.0, 2.0),
+                                               (5000, 12.5, 14.0)])
+def test_ground_range_shape_matches_oracle(samples, target, az):
+    """The shape query (out = NULL) is host code: output size, looks and
+    spacings equal the oracle's."""
+    L = _native.lib()
+    info = _grd_info(100, samples)
+    grids = _grd_grids(samples)
+    pts, counts = O.grd_points(grids)
+    img = abi.GrdImage()
+    assert L.sardine_to_ground_range(None, C.byref(info), pts, counts.ctypes.data, len(grids), target, az, None,
+                                     C.byref(img), None) == 0
+    st, _, ref = O.to_ground_range(np.zeros((100, samples), np.float32), info, grids, target, az, shape_only=True)
+    assert st == 0 and bytes(img) == bytes(ref)
+
+
+def test_ground_range_errors_match_oracle():
+    L = _native.lib()
+    grids = _grd_grids(100)
+    pts, counts = O.grd_points(grids)
+    img = abi.GrdImage()
+
+    def both(info, g, target, az):
+        p, c = O.grd_points(g)
+        st = L.sardine_to_ground_range(None, C.byref(info), p, c.ctypes.data, len(g), target, az, None,
+                                       C.byref(img), None)
+        ref = O.to_ground_range(np.zeros((max(info.lines, 0), max(info.samples, 0)), np.float32), info, g, target,
+                                az, shape_only=True)[0]
+        assert st == ref, (st, ref)
+        return st
+
+    assert both(_grd_info(0, 0), grids, 10.0, 14.08) == abi.ERR_GRD_EMPTY_IMAGE
+    assert both(_grd_info(10, 100), grids, -10.0, 14.08) == abi.ERR_GRD_INVALID_SPACING
+    assert both(_grd_info(10, 100), grids, float("nan"), 14.08) == abi.ERR_GRD_INVALID_SPACING
+    for az in (0.0, -1.0, float("nan"), float("inf")):
+        assert both(_grd_info(10, 100), grids, 10.0, az) == abi.ERR_GRD_INVALID_AZIMUTH_SPACING
+    assert both(_grd_info(10, 100), [], 10.0, 14.08) == abi.ERR_GRD_INTERPOLATION_FAILED
+    assert both(_grd_info(10, 100), [grids[0], []], 10.0, 14.08) == abi.ERR_GRD_NO_GRID_POINTS
+    assert both(_grd_info(10, 100), [grids[0]], 10.0, 14.08) == abi.ERR_GRD_INTERPOLATION_FAILED
+    assert both(_grd_info(10, 100), grids, 1e6, 14.08) == abi.ERR_GRD_INVALID_SPACING
+    import torch
+
+    if not torch.cuda.is_available():
+        d = np.zeros(1000, np.float32)
+        assert L.sardine_to_ground_range(d.ctypes.data, C.byref(_grd_info(10, 100)), pts, counts.ctypes.data, 3,
+                                         10.0, 14.08, d.ctypes.data, C.byref(img), None) == abi.ERR_NO_DEVICE

@@ -967,6 +967,123 @@ def test_multilook_against_nanmean(r, a):
     assert _ulp_diff(od, ed) <= 1 and _ulp_diff(on, en) <= 1
 
 
+# ── ground-range projection (ground_range.rs:492-724) ────────────────────
+C_LIGHT = 299_792_458.0
+
+
+def _grd_scene(lines=100, samples=200, fill=0.1):
+    rps_s = 2.33 * 2.0 / C_LIGHT
+    info = abi.MergedInfo()
+    info.lines, info.samples = lines, samples
+    info.near_slant_range_time_s = 5.0e-3
+    info.range_pixel_spacing_s = rps_s
+    info.range_pixel_spacing_m = 2.33
+    grid = [(line, 5.0e-3 + (i * 50) * rps_s, 30.0 + (i * 50) / 199.0 * 6.0) for line in (0, 50, 99)
+            for i in range(5)]
+    return np.full((lines, samples), fill, np.float32), info, [grid]
+
+
+def test_grd_ground_spacing_reference_case():
+    # ground_range.rs:507-518 through the profile: 30° and 45° columns
+    st, rad = O.grd_incidence_profile([[(0, 0.0, 30.0), (0, 1.0, 45.0)]], 0.0, 1.0, 2)
+    assert st == 0
+    assert abs(2.33 / math.sin(rad[0]) - 4.66) < 0.01 and abs(2.33 / math.sin(rad[1]) - 3.295) < 0.01
+
+
+def test_grd_column_map_reference_cases():
+    kind, left, frac = O.grd_column_map(np.arange(1000) * 3.5, 10.0, 349)  # ground_range.rs:520-541
+    assert len(kind) == 349 and np.all(np.diff(left) >= 0)
+    kind, left, frac = O.grd_column_map(np.arange(500) * 4.0, 10.0, int(np.floor(499 * 4.0 / 10.0)))  # :543-557
+    assert left[0] == 0
+    # against a brute-force search: the bracketing interval and the fraction of every column
+    cum = np.cumsum(np.random.default_rng(2).uniform(3.0, 4.7, 3000))
+    cum -= cum[0]
+    n_out = int(cum[-1] // 10.0)
+    kind, left, frac = O.grd_column_map(cum, 10.0, n_out)
+    g = np.arange(n_out) * 10.0
+    i = np.searchsorted(cum, g, side="left")
+    exact = (i < cum.size) & (cum[np.minimum(i, cum.size - 1)] == g)
+    assert np.array_equal(kind == 0, exact | (i == 0))
+    m = kind == 1
+    assert np.array_equal(left[m], i[m] - 1)
+    assert np.array_equal(frac[m], (g[m] - cum[i[m] - 1]) / (cum[i[m]] - cum[i[m] - 1]))
+
+
+def test_grd_range_interpolator_reference_case():
+    grid = [[(100, 5.0e-3, 30.0), (100, 6.0e-3, 36.0)]]  # ground_range.rs:559-603
+    for t, want in ((5.5e-3, 33.0), (5.0e-3, 30.0), (6.0e-3, 36.0)):
+        st, rad = O.grd_incidence_profile(grid, t, 1.0, 1)
+        assert st == 0 and abs(math.degrees(rad[0]) - want) < 0.01
+    for t in (4.9e-3, 6.1e-3):  # outside by more than 1 % of the swath: no angle
+        assert O.grd_incidence_profile(grid, t, 1.0, 1)[0] == abi.ERR_GRD_INTERPOLATION_FAILED
+    # within 1 % of the swath outside it: clamped to the edge (ground_range.rs:278-287)
+    st, rad = O.grd_incidence_profile(grid, 6.0e-3 + 0.5e-5, 1.0, 1)
+    assert st == 0 and abs(math.degrees(rad[0]) - 36.0) < 1e-12
+    # two swaths overlapping: the one whose centre is nearer wins, the earlier
+    # one on a tie (ground_range.rs:240-252, strict <)
+    two = [[(0, 1.0, 10.0), (0, 3.0, 20.0)], [(0, 2.5, 40.0), (0, 4.5, 60.0)]]
+    st, rad = O.grd_incidence_profile(two, 2.9, 1.0, 1)
+    assert st == 0 and abs(math.degrees(rad[0]) - 44.0) < 1e-9  # |2.9-3.5| < |2.9-2|
+    st, rad = O.grd_incidence_profile(two, 2.75, 1.0, 1)
+    assert st == 0 and abs(math.degrees(rad[0]) - 18.75) < 1e-9  # a tie: the first swath by slant time
+    st, rad = O.grd_incidence_profile(list(reversed(two)), 2.75, 1.0, 1)
+    assert st == 0 and abs(math.degrees(rad[0]) - 18.75) < 1e-9  # order of the grids does not matter
+
+
+def test_to_ground_range_reference_cases():
+    d, info, grids = _grd_scene()  # ground_range.rs:605-669
+    st, out, img = O.to_ground_range(d, info, grids, 10.0, 14.08)
+    assert st == 0 and 0 < img.samples < 200 and img.lines > 0 and img.range_pixel_spacing_m == 10.0
+    assert img.azimuth_looks == 1 and img.range_looks == 1 and img.azimuth_pixel_spacing_m == 14.08
+    v = out[~np.isnan(out)]
+    assert v.size and abs(v.astype(np.float64).mean() - 0.1) < 0.001
+    e = abi.MergedInfo()
+    e.lines = e.samples = 0
+    assert O.to_ground_range(np.zeros((0, 0), np.float32), e, [], 10.0, 14.08)[0] == abi.ERR_GRD_EMPTY_IMAGE
+    i10 = abi.MergedInfo.from_buffer_copy(info)
+    d10 = np.ones((10, 10), np.float32)
+    assert O.to_ground_range(d10, i10, [], -10.0, 14.08)[0] == abi.ERR_GRD_INVALID_SPACING  # :689-705
+    for az in (0.0, -1.0, float("nan")):  # :707-724
+        assert O.to_ground_range(d10, i10, [], 10.0, az)[0] == abi.ERR_GRD_INVALID_AZIMUTH_SPACING
+    assert O.to_ground_range(d10, i10, [], 10.0, 14.08)[0] == abi.ERR_GRD_INTERPOLATION_FAILED  # no grids
+    assert O.to_ground_range(d10, i10, [[]], 10.0, 14.08)[0] == abi.ERR_GRD_NO_GRID_POINTS
+    assert O.to_ground_range(d10, i10, grids, float("nan"), 14.08)[0] == abi.ERR_GRD_INVALID_SPACING
+
+
+@pytest.mark.parametrize("az_spacing,lines", [(14.08, 100), (3.4, 101), (2.0, 7)])
+def test_to_ground_range_against_numpy(az_spacing, lines):
+    """The projection and the azimuth multilook recomputed with numpy from the
+    (pinned) incidence profile: searchsorted brackets, f32 interpolation,
+    nanmean over azimuth_looks lines."""
+    rng = np.random.default_rng(lines)
+    d, info, grids = _grd_scene(lines=lines, samples=200)
+    d = rng.exponential(0.1, d.shape).astype(np.float32)
+    d[rng.random(d.shape) < 0.1] = np.nan
+    st, out, img = O.to_ground_range(d, info, grids, 10.0, az_spacing)
+    assert st == 0
+    a = max(int(np.floor(10.0 / az_spacing + 0.5)), 1)
+    assert img.azimuth_looks == a and img.lines == lines // a
+    st, rad = O.grd_incidence_profile(grids, info.near_slant_range_time_s, info.range_pixel_spacing_s, 200)
+    gs = info.range_pixel_spacing_m / np.sin(rad)
+    cum = np.concatenate([[0.0], np.cumsum((gs[:-1] + gs[1:]) / 2.0)])
+    n_out = int(np.floor(cum[-1] / 10.0))
+    assert img.samples == n_out
+    g = np.arange(n_out) * 10.0
+    i = np.searchsorted(cum, g)
+    ex = (i == 0) | (cum[np.minimum(i, 199)] == g)
+    il = np.where(ex, np.minimum(i, 199), i - 1)
+    fr = np.where(ex, 0.0, (g - cum[il]) / (cum[np.minimum(il + 1, 199)] - cum[il])).astype(np.float32)
+    v0, v1 = d[:, il], d[:, np.minimum(il + 1, 199)]
+    proj = np.where(ex, v0, v0 + (v1 - v0) * fr)
+    proj[:, ~ex] = np.where(np.isnan(v0) | np.isnan(v1), np.nan, proj)[:, ~ex]
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        want = np.nanmean(proj[: img.lines * a].astype(np.float64).reshape(img.lines, a, n_out), axis=1)
+    want = want.astype(np.float32)
+    assert np.array_equal(np.isnan(out), np.isnan(want)) and _ulp_diff(out, want) <= 1
+
+
 # ── full chain invariants on a small synthetic scene ──────────────────────
 def test_small_scene_chain_invariants():
     sc = synthetic.scene_small_plain(2, lines=256, samples=512, out_rows=160, out_cols=160, dem_side=256)
