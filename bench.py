@@ -51,14 +51,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["iw_full", "small_plain", "batch64", "serve", "fp32", "cog", "multilook",
-                                                "grd"],
+                                                "grd", "slc"],
                     default="iw_full",
                     help="iw_full: the headline step; batch64: 64 AOI tiles batched vs one call per tile; "
                          "serve: Poisson AOI requests on 4 resident scenes, served vs one call per request; "
                          "fp32: the FP32 scene-centred variant's errors and speed against FP64; "
                          "cog: the COG export (pyramid + tile pack, and the whole file write) of the "
                          "full-size dB product; multilook: the pre-TC multilook of the full merged image; "
-                         "grd: the ground-range projection of the full merged image")
+                         "grd: the ground-range projection of the full merged image; "
+                         "slc: a full-size swath measurement TIFF read into HBM")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
@@ -139,6 +140,8 @@ def main():
         return bench_multilook(args, dev)
     if args.workload == "grd":
         return bench_grd(args, dev)
+    if args.workload == "slc":
+        return bench_slc(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -1023,7 +1026,104 @@ def bench_grd(args, dev):
     }
     print(json.dumps(line), flush=True)
 
+
+def _write_slc_tiff(path, raw) -> None:
+    """A Sentinel-1-style measurement TIFF (little-endian classic TIFF,
+    uncompressed CInt16, one row per strip, contiguous rows) of raw
+    (height, width, 2) int16."""
+    import struct
+
+    h, w = raw.shape[:2]
+    tags = [(256, 4, 1, w), (257, 4, 1, h), (258, 3, 1, 32), (259, 3, 1, 1), (273, 4, h, None), (277, 3, 1, 1),
+            (278, 4, 1, 1), (279, 4, h, None), (339, 3, 1, 5)]
+    table = 8 + 2 + 12 * len(tags) + 4
+    counts = table + 4 * h
+    data = counts + 4 * h
+    with open(path, "wb") as f:
+        f.write(b"II" + struct.pack("<HI", 42, 8) + struct.pack("<H", len(tags)))
+        for tag, typ, cnt, val in tags:
+            v = table if tag == 273 else counts if tag == 279 else val
+            f.write(struct.pack("<HHI", tag, typ, cnt) + (struct.pack("<HH", v, 0) if typ == 3 else struct.pack("<I", v)))
+        f.write(struct.pack("<I", 0))
+        f.write(struct.pack(f"<{h}I", *[data + k * w * 4 for k in range(h)]))
+        f.write(struct.pack(f"<{h}I", *([w * 4] * h)))
+        f.write(raw.tobytes())
+
+
+def bench_slc(args, dev):
+    """A full-size IW swath measurement TIFF (9 bursts x 1500 lines x 21000
+    samples CInt16, 1.13 GB, in the page cache) read into HBM: the whole
+    swath in one call and burst by burst as deburst reads it
+    (deburst.rs:315-334), against the oracle's read_burst_raw into host
+    memory (one thread, as the reference reads)."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from sardine_amd import api
+
+    h, w, lpb = 13500, 21000, 1500
+    t = torch.empty((h, w, 2), dtype=torch.int16, device=dev)
+    api.synth_cint16(t, 7)
+    host = t.cpu().numpy()
+    tmp = pathlib.Path("/dev/shm") if pathlib.Path("/dev/shm").is_dir() else pathlib.Path("/tmp")
+    path = tmp / f"sardine_bench_slc_{os.getpid()}.tiff"
+    try:
+        _write_slc_tiff(path, host)
+        r = api.SlcReader.open(path)
+        out = torch.empty_like(t)
+        r.read_burst_raw(0, h, out=out)
+        torch.cuda.synchronize()
+        steps = max(args.steps, 1)
+        settle_gc()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.read_burst_raw(0, h, out=out)
+        torch.cuda.synchronize()
+        whole_s = (time.perf_counter() - t0) / steps
+        same = bool(torch.equal(out, t))
+        out.zero_()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for b in range(h // lpb):
+                r.read_burst_raw(b * lpb, lpb, out=out[b * lpb:(b + 1) * lpb])
+        torch.cuda.synchronize()
+        burst_s = (time.perf_counter() - t0) / steps
+        same = same and bool(torch.equal(out, t))
+        cpu = None
+        if not args.no_cpu_baseline:
+            st, tt, _ = O.slc_open(path)
+            t0 = time.perf_counter()
+            st2, ref = O.slc_read(path, tt, 0, h)
+            cs = time.perf_counter() - t0
+            assert st == 0 and st2 == 0 and np.array_equal(ref, host)
+            cpu = {"value": round(host.nbytes / cs / 1e9, 2), "unit": "GB/s (file → host memory)", "cores": 1,
+                   "kind": "port", "sample": "the oracle's read_burst_raw of the whole swath (1.13 GB, page cache)",
+                   "seconds": round(cs, 3)}
+    finally:
+        if path.exists():
+            path.unlink()
+    gb = host.nbytes / 1e9
+    log(f"slc: whole {whole_s * 1e3:.1f} ms ({gb / whole_s:.1f} GB/s), per burst {burst_s * 1e3:.1f} ms, "
+        f"identical={same}, cpu {cpu}")
+    line = {
+        "metric": "SLC measurement TIFF → HBM GB/s (whole swath, page cache)",
+        "value": round(gb / whole_s, 2), "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": round(whole_s * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i16", "data": "synthetic",
+        "config": {"workload": "slc_iw_swath_13500x21000", "bytes": host.nbytes},
+        "roofline": None,
+        "note": "I/O path with no kernel: parallel preads into pinned chunks, host→device copies overlapped; "
+                "bounded by host memory copy and PCIe (Gen5 x16 ≈ 64 GB/s per direction)",
+        "per_burst": {"ms": round(burst_s * 1e3, 3), "GBps": round(gb / burst_s, 2)},
+        "device_result_bit_exact": same,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
 if __name__ == "__main__":
     main()
+
 
 

@@ -85,6 +85,15 @@ typedef enum sardine_status {
     SARDINE_ERR_GRD_INVALID_SPACING = 72,
     SARDINE_ERR_GRD_INVALID_AZIMUTH_SPACING = 73,
     SARDINE_ERR_GRD_INTERPOLATION_FAILED = 74,
+    /* SlcReadError (slc_reader.rs:66-131; the multi-slice variants have no
+     * counterpart here) */
+    SARDINE_ERR_SLC_IO = 80,
+    SARDINE_ERR_SLC_NOT_TIFF = 81,
+    SARDINE_ERR_SLC_BIGTIFF = 82,
+    SARDINE_ERR_SLC_MISSING_TAG = 83,
+    SARDINE_ERR_SLC_UNSUPPORTED_FORMAT = 84,
+    SARDINE_ERR_SLC_NON_CONTIGUOUS_STRIPS = 85,
+    SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS = 86,
     /* library / device */
     SARDINE_ERR_INVALID_ARGUMENT = 90,
     SARDINE_ERR_NO_DEVICE = 91,
@@ -433,6 +442,33 @@ sardine_status sardine_apply_calibration(const int16_t* deburst /*[device] lines
                                          float* out, float* nesz,
                                          uint32_t* cal_gap_px, uint32_t* noise_gap_px,
                                          void* stream);
+
+/* ── SLC measurement TIFF → HBM ──────────────────────────────────────────── */
+
+/* The layout SlcReader::open validates (slc_reader.rs:268-461): a classic
+ * TIFF, uncompressed CInt16 (SampleFormat 5, 32 bits, 1 sample per pixel),
+ * one row per strip, contiguous strips (first and last offsets checked). */
+typedef struct sardine_slc_tiff {
+    uint32_t width, height;
+    int32_t little_endian;
+    int32_t pad;
+    uint64_t base_offset;  /* file offset of row 0 */
+    uint64_t row_bytes;    /* width * 4 */
+} sardine_slc_tiff;
+
+/* SlcReader::open (slc_reader.rs:504-515): parses and validates the IFD.
+ * Host only.  Errors: SARDINE_ERR_SLC_* as parse_ifd raises them. */
+sardine_status sardine_slc_open(const char* path, sardine_slc_tiff* tiff);
+
+/* SlcReader::read_burst_raw (slc_reader.rs:532-575) into device memory:
+ * line_count rows from first_line, as CInt16 [i, q] pairs in host order, to
+ * dst [device] (line_count * width * 2 int16).  The rows are read with
+ * parallel preads into pinned chunks, each set copied to HBM while the next
+ * is read; big-endian files are byte-swapped on the device.  Returns after
+ * the file reads; the copies are ordered on `stream`.  The line range is
+ * checked first (SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS). */
+sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* tiff, int64_t first_line,
+                                      int64_t line_count, int16_t* dst /*[device]*/, void* stream);
 
 /* ── Pre-TC multilook ─────────────────────────────────────────────────────── */
 

@@ -84,6 +84,8 @@ def lib():
         L.oracle_grd_column_map.restype = None
         L.oracle_to_ground_range.argtypes = [P, C.POINTER(abi.MergedInfo), C.POINTER(abi.GrdGridPoint), P, i32, d,
                                              d, P, C.POINTER(abi.GrdImage)]
+        L.oracle_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), C.POINTER(i32)]
+        L.oracle_slc_read.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), i64, i64, P]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
@@ -441,6 +443,22 @@ def to_ground_range(data: np.ndarray, info, grids, target_spacing_m: float, azim
     st = lib().oracle_to_ground_range(_ptr(data), C.byref(info), pts, _ptr(counts), len(grids), target_spacing_m,
                                       azimuth_pixel_spacing_m, _ptr(out), C.byref(img))
     return st, out[: img.lines * img.samples].reshape(img.lines, img.samples), img
+
+
+def slc_open(path):
+    """SlcReader::open / parse_ifd (slc_reader.rs:268-515) → (status, SlcTiff, missing tag)."""
+    t = abi.SlcTiff()
+    tag = C.c_int32(0)
+    st = lib().oracle_slc_open(str(path).encode(), C.byref(t), C.byref(tag))
+    return st, t, tag.value
+
+
+def slc_read(path, tiff, first_line: int, line_count: int):
+    """read_burst_raw (slc_reader.rs:532-575) → (status, (line_count, width, 2) int16)."""
+    out = np.empty((max(line_count, 0), tiff.width, 2), np.int16)
+    st = lib().oracle_slc_read(str(path).encode(), C.byref(tiff), first_line, line_count,
+                               _ptr(out) if out.size else None)
+    return st, out if st == 0 else None
 
 
 # ── the full chain on a synthetic scene (host) ──────────────────────────

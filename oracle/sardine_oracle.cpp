@@ -33,6 +33,8 @@
 //   to_ground_range            ground_range.rs:506-724 (ground spacing, column
 //                              map monotonic / covering, range interpolator,
 //                              uniform synthetic scene, the three rejections)
+//   SLC TIFF reader            slc_reader.rs:618-820 (non-TIFF, empty, missing
+//                              file, SamplesPerPixel 2, line bounds, row count)
 //   striped GeoTIFF writers    export.rs:1276-1580 (geotransform errors, key
 //                              offsets, BigTIFF threshold and layout, u8 tags,
 //                              the no-CRS IFD, UTM keys)
@@ -2011,6 +2013,128 @@ int oracle_to_ground_range(const float* data, const sardine_merged_info* m, cons
             }
             out[ol * os + j] = count ? (float)(sum / (double)count) : std::numeric_limits<float>::quiet_NaN();
         }
+    return 0;
+}
+
+}  // extern "C"
+
+// ── SLC measurement TIFF reader (slc_reader.rs:200-590) ────────────────────
+namespace {
+
+struct SlcFile {
+    std::FILE* f = nullptr;
+    ~SlcFile() {
+        if (f) std::fclose(f);
+    }
+};
+
+bool read_at(std::FILE* f, uint64_t off, void* dst, size_t n) {
+    return std::fseek(f, (long)off, SEEK_SET) == 0 && std::fread(dst, 1, n, f) == n;
+}
+
+uint16_t get16(const uint8_t* p, bool le) { return le ? (uint16_t)(p[0] | p[1] << 8) : (uint16_t)(p[0] << 8 | p[1]); }
+uint32_t get32(const uint8_t* p, bool le) {
+    return le ? (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24
+              : (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | (uint32_t)p[3];
+}
+
+}  // namespace
+
+extern "C" {
+
+// parse_ifd (slc_reader.rs:268-461).  *missing_tag receives the tag of a
+// MissingTag error.
+int oracle_slc_open(const char* path, sardine_slc_tiff* out, int32_t* missing_tag) {
+    SlcFile sf;
+    sf.f = std::fopen(path, "rb");
+    if (!sf.f) return SARDINE_ERR_SLC_IO;
+    uint8_t h[8];
+    if (!read_at(sf.f, 0, h, 8)) return SARDINE_ERR_SLC_IO;
+    bool le;
+    if (h[0] == 'I' && h[1] == 'I') le = true;
+    else if (h[0] == 'M' && h[1] == 'M') le = false;
+    else return SARDINE_ERR_SLC_NOT_TIFF;
+    const uint16_t magic = get16(h + 2, le);
+    if (magic == 43) return SARDINE_ERR_SLC_BIGTIFF;
+    if (magic != 42) return SARDINE_ERR_SLC_NOT_TIFF;
+    const uint64_t ifd = get32(h + 4, le);
+    uint8_t nb[2];
+    if (!read_at(sf.f, ifd, nb, 2)) return SARDINE_ERR_SLC_IO;
+    const size_t n = get16(nb, le);
+    std::vector<uint8_t> e(n * 12);
+    if (n && std::fread(e.data(), 1, e.size(), sf.f) != e.size()) return SARDINE_ERR_SLC_IO;
+    // tag → (have, value); StripOffsets keeps (type, count, raw field)
+    bool hw = false, hh = false, hb = false, hc = false, hr = false, hs = false, hf = false, ho = false;
+    uint32_t w = 0, ht = 0, bps = 0, comp = 0, rps = 0, spp = 1, fmt = 0;
+    uint16_t so_type = 0;
+    uint32_t so_count = 0, so_value = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t* p = &e[i * 12];
+        const uint16_t tag = get16(p, le), type = get16(p + 2, le);
+        const uint32_t count = get32(p + 4, le);
+        // entry_scalar: SHORT → first 2 bytes, LONG → 4 bytes, else none
+        const bool ok = type == 3 || type == 4;
+        const uint32_t v = type == 3 ? get16(p + 8, le) : get32(p + 8, le);
+        switch (tag) {
+            case 256: hw = ok; w = v; break;
+            case 257: hh = ok; ht = v; break;
+            case 258: hb = ok; bps = (uint16_t)v; break;
+            case 259: hc = ok; comp = (uint16_t)v; break;
+            case 278: hr = ok; rps = v; break;
+            case 277: hs = ok; spp = (uint16_t)v; break;
+            case 339: hf = ok; fmt = (uint16_t)v; break;
+            case 273: ho = true; so_type = type; so_count = count; so_value = get32(p + 8, le); break;
+            default: break;
+        }
+    }
+    auto missing = [&](int32_t tag) {
+        if (missing_tag) *missing_tag = tag;
+        return SARDINE_ERR_SLC_MISSING_TAG;
+    };
+    if (!hw) return missing(256);
+    if (!hh) return missing(257);
+    if (!hb) return missing(258);
+    if (!hc) return missing(259);
+    if (!hr) return missing(278);
+    if (!hs) spp = 1;  // SamplesPerPixel defaults to 1
+    if (!hf) return missing(339);
+    if (!ho) return missing(273);
+    if (comp != 1 || fmt != 5 || bps != 32 || rps != 1 || spp != 1 || so_count != ht)
+        return SARDINE_ERR_SLC_UNSUPPORTED_FORMAT;
+    const uint64_t row = (uint64_t)w * 4;
+    uint64_t base, last;
+    if (so_count == 1) {
+        base = last = so_value;
+    } else {
+        if (so_type != 3 && so_type != 4) return SARDINE_ERR_SLC_UNSUPPORTED_FORMAT;
+        const uint64_t each = so_type == 3 ? 2 : 4;
+        uint8_t b[4];
+        if (!read_at(sf.f, so_value, b, each)) return SARDINE_ERR_SLC_IO;
+        base = each == 2 ? get16(b, le) : get32(b, le);
+        if (!read_at(sf.f, so_value + ((uint64_t)so_count - 1) * each, b, each)) return SARDINE_ERR_SLC_IO;
+        last = each == 2 ? get16(b, le) : get32(b, le);
+    }
+    if (last != base + ((uint64_t)ht - 1) * row) return SARDINE_ERR_SLC_NON_CONTIGUOUS_STRIPS;
+    out->width = w;
+    out->height = ht;
+    out->little_endian = le ? 1 : 0;
+    out->pad = 0;
+    out->base_offset = base;
+    out->row_bytes = row;
+    return 0;
+}
+
+// read_burst_raw (slc_reader.rs:532-575) into host memory.
+int oracle_slc_read(const char* path, const sardine_slc_tiff* t, int64_t first, int64_t count, int16_t* dst) {
+    if (first < 0 || count < 0 || first > (int64_t)t->height || count > (int64_t)t->height - first)
+        return SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS;
+    SlcFile sf;
+    sf.f = std::fopen(path, "rb");
+    if (!sf.f) return SARDINE_ERR_SLC_IO;
+    const size_t n = (size_t)count * t->width * 4;
+    if (n && !read_at(sf.f, t->base_offset + (uint64_t)first * t->row_bytes, dst, n)) return SARDINE_ERR_SLC_IO;
+    if (!t->little_endian)
+        for (size_t i = 0; i < n / 2; ++i) dst[i] = (int16_t)(((uint16_t)dst[i] >> 8) | ((uint16_t)dst[i] << 8));
     return 0;
 }
 

@@ -47,6 +47,13 @@ ERR_GRD_EMPTY_IMAGE = 71
 ERR_GRD_INVALID_SPACING = 72
 ERR_GRD_INVALID_AZIMUTH_SPACING = 73
 ERR_GRD_INTERPOLATION_FAILED = 74
+ERR_SLC_IO = 80
+ERR_SLC_NOT_TIFF = 81
+ERR_SLC_BIGTIFF = 82
+ERR_SLC_MISSING_TAG = 83
+ERR_SLC_UNSUPPORTED_FORMAT = 84
+ERR_SLC_NON_CONTIGUOUS_STRIPS = 85
+ERR_SLC_LINE_OUT_OF_BOUNDS = 86
 ERR_INVALID_ARGUMENT = 90
 ERR_NO_DEVICE = 91
 ERR_HIP = 92
@@ -210,6 +217,11 @@ class MergedInfo(C.Structure):
                 ("swath_valid_offset", C.c_int64 * MAX_SWATHS)]
 
 
+class SlcTiff(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("little_endian", C.c_int32), ("pad", C.c_int32),
+                ("base_offset", C.c_uint64), ("row_bytes", C.c_uint64)]
+
+
 class GrdGridPoint(C.Structure):
     _fields_ = [("slant_range_time_s", C.c_double), ("incidence_angle_deg", C.c_double), ("line", C.c_uint32),
                 ("pad", C.c_uint32)]
@@ -249,6 +261,8 @@ EXPORTED_SYMBOLS = (
     "sardine_apply_calibration",
     "sardine_to_db_inplace",
     "sardine_apply_multilook",
+    "sardine_slc_open",
+    "sardine_slc_read_lines",
     "sardine_to_ground_range",
     "sardine_cog_layout_compute",
     "sardine_cog_header",

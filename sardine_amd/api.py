@@ -15,6 +15,8 @@ Names and argument meaning follow the Rust crate:
         scene_prep.rs:1060-1161
   to_ground_range(merged, grids, target_spacing_m, azimuth_pixel_spacing_m) -> GroundRangeImage
         ground_range.rs:303-442
+  SlcReader.open(path).read_burst_raw(first_line, line_count) -> device CInt16
+        slc_reader.rs:504-575
   to_db_inplace(data, noise_floor_linear)    export.rs:59-94
   write_cog_with_crs(path, data, cols, rows, geotransform, crs, tile_size)
         export.rs:329-354 (cog_layout / cog_header / cog_pack expose its parts)
@@ -405,6 +407,42 @@ def apply_multilook(merged: MergedSigma0, range_looks: int, azimuth_looks: int, 
     return MergedSigma0(od, on, res.near_slant_range_time_s, res.range_pixel_spacing_m, res.azimuth_start_time_us,
                         res.range_pixel_spacing_s, res.cal_lut_extrapolation_gap_px,
                         res.noise_lut_extrapolation_gap_px, res)
+
+
+class SlcReader:
+    """SlcReader (slc_reader.rs:466-590) reading rows straight into HBM."""
+
+    def __init__(self, path, tiff: abi.SlcTiff):
+        self.path = str(path)
+        self.tiff = tiff
+
+    @classmethod
+    def open(cls, path) -> "SlcReader":
+        t = abi.SlcTiff()
+        _check(lib().sardine_slc_open(str(path).encode(), C.byref(t)))
+        return cls(path, t)
+
+    @property
+    def width(self) -> int:
+        return self.tiff.width
+
+    @property
+    def height(self) -> int:
+        return self.tiff.height
+
+    @property
+    def little_endian(self) -> bool:
+        return bool(self.tiff.little_endian)
+
+    def read_burst_raw(self, first_line: int, line_count: int, out=None, device="cuda", stream=None):
+        """line_count rows from first_line as an int16 device tensor
+        (line_count, width, 2) of [i, q] pairs."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty((line_count, self.width, 2), dtype=torch.int16, device=device)
+        _check(lib().sardine_slc_read_lines(self.path.encode(), C.byref(self.tiff), first_line, line_count,
+                                            out.data_ptr() if out.numel() else None, _stream_ptr(stream)))
+        return out
 
 
 @dataclass

@@ -7,7 +7,9 @@
 // outcomes back onto the reference's error taxonomy.  There is no CPU
 // compute path: without a device every compute entry point fails with
 // SARDINE_ERR_NO_DEVICE.
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -60,6 +62,7 @@ extern "C" hipError_t sd_launch_multilook(const float* data, const float* nesz, 
                                           float* on, hipStream_t s);
 extern "C" hipError_t sd_launch_grd(const float* data, int64_t in_samples, const void* map, int64_t out_lines,
                                     int64_t out_samples, int a, float* out, hipStream_t s);
+extern "C" hipError_t sd_launch_swap16(uint16_t* d, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -1325,6 +1328,233 @@ sardine_status sardine_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint
     sardine_status st;
     if ((st = require_device())) return st;
     SD_HIP(sd_launch_synth_cint16(out, n, seed, base, sigma, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── SLC measurement TIFF → HBM (slc_reader.rs:200-590) ───────────────────
+namespace {
+
+struct Fd {
+    int fd = -1;
+    ~Fd() {
+        if (fd >= 0) ::close(fd);
+    }
+};
+
+// pread of exactly n bytes (retrying short reads); false on error or EOF.
+bool pread_all(int fd, void* dst, size_t n, uint64_t off) {
+    uint8_t* p = static_cast<uint8_t*>(dst);
+    while (n > 0) {
+        const ssize_t r = ::pread(fd, p, n, (off_t)off);
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) return false;
+        p += r;
+        n -= (size_t)r;
+        off += (uint64_t)r;
+    }
+    return true;
+}
+
+uint16_t rd16(const uint8_t* p, bool le) { return le ? (uint16_t)(p[0] | (p[1] << 8)) : (uint16_t)((p[0] << 8) | p[1]); }
+uint32_t rd32(const uint8_t* p, bool le) {
+    return le ? ((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24))
+              : (((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | (uint32_t)p[3]);
+}
+
+// parse_ifd (slc_reader.rs:268-461).
+sardine_status slc_parse(int fd, const char* path, sardine_slc_tiff* out) {
+    uint8_t h[8];
+    if (!pread_all(fd, h, 8, 0)) return fail(SARDINE_ERR_SLC_IO, "I/O error reading the TIFF header of %s", path);
+    bool le;
+    if (h[0] == 'I' && h[1] == 'I') le = true;
+    else if (h[0] == 'M' && h[1] == 'M') le = false;
+    else return fail(SARDINE_ERR_SLC_NOT_TIFF, "not a classic TIFF (bad byte-order marker or magic=0)");
+    const uint16_t magic = rd16(h + 2, le);
+    if (magic == 43)
+        return fail(SARDINE_ERR_SLC_BIGTIFF, "BigTIFF (magic=43) is not supported; only classic TIFF (magic=42) is accepted");
+    if (magic != 42) return fail(SARDINE_ERR_SLC_NOT_TIFF, "not a classic TIFF (bad byte-order marker or magic=%u)", magic);
+    const uint64_t ifd = rd32(h + 4, le);
+    uint8_t nb[2];
+    if (!pread_all(fd, nb, 2, ifd)) return fail(SARDINE_ERR_SLC_IO, "I/O error reading the IFD of %s", path);
+    const size_t n = rd16(nb, le);
+    std::vector<uint8_t> ent(n * 12);
+    if (n && !pread_all(fd, ent.data(), ent.size(), ifd + 2))
+        return fail(SARDINE_ERR_SLC_IO, "I/O error reading the IFD entries of %s", path);
+    // the tags the reader needs: an entry of another type than SHORT/LONG
+    // counts as absent (entry_scalar), a later entry replaces an earlier one
+    struct Tag {
+        bool have = false;
+        uint32_t v = 0;
+    } width, height, bps, comp, rps, spp, fmt;
+    bool have_so = false;
+    uint16_t so_type = 0;
+    uint32_t so_count = 0, so_field = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t* e = &ent[i * 12];
+        const uint16_t tag = rd16(e, le), type = rd16(e + 2, le);
+        Tag t;
+        t.have = type == 3 || type == 4;
+        t.v = type == 3 ? rd16(e + 8, le) : rd32(e + 8, le);
+        switch (tag) {
+            case 256: width = t; break;
+            case 257: height = t; break;
+            case 258: bps = {t.have, t.v & 0xffffu}; break;
+            case 259: comp = {t.have, t.v & 0xffffu}; break;
+            case 277: spp = {t.have, t.v & 0xffffu}; break;
+            case 278: rps = t; break;
+            case 339: fmt = {t.have, t.v & 0xffffu}; break;
+            case 273:
+                have_so = true;
+                so_type = type;
+                so_count = rd32(e + 4, le);
+                so_field = rd32(e + 8, le);
+                break;
+            default: break;
+        }
+    }
+    const std::pair<const Tag*, int> required[] = {{&width, 256}, {&height, 257}, {&bps, 258}, {&comp, 259},
+                                                   {&rps, 278}};
+    for (const auto& r : required)
+        if (!r.first->have) return fail(SARDINE_ERR_SLC_MISSING_TAG, "required TIFF tag %d is absent from the IFD", r.second);
+    const uint32_t samples_pp = spp.have ? spp.v : 1;  // TIFF default
+    if (!fmt.have) return fail(SARDINE_ERR_SLC_MISSING_TAG, "required TIFF tag 339 is absent from the IFD");
+    if (!have_so) return fail(SARDINE_ERR_SLC_MISSING_TAG, "required TIFF tag 273 is absent from the IFD");
+    if (comp.v != 1) return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: Compression=%u (expected 1=None)", comp.v);
+    if (fmt.v != 5) return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: SampleFormat=%u (expected 5=CInt)", fmt.v);
+    if (bps.v != 32) return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: BitsPerSample=%u (expected 32 for CInt16)", bps.v);
+    if (rps.v != 1) return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: RowsPerStrip=%u (expected 1)", rps.v);
+    if (samples_pp != 1)
+        return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: SamplesPerPixel=%u (expected 1 for CInt16)", samples_pp);
+    if (so_count != height.v)
+        return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: strip count (%u) != image height (%u)", so_count, height.v);
+    const uint64_t row = (uint64_t)width.v * 4;
+    uint64_t first, last;
+    if (so_count == 1) {  // the single offset sits in the entry's value field
+        first = last = so_field;
+    } else {
+        if (so_type != 3 && so_type != 4)
+            return fail(SARDINE_ERR_SLC_UNSUPPORTED_FORMAT, "unsupported TIFF format: StripOffsets field type %u is not SHORT(3) or LONG(4)", so_type);
+        const uint64_t each = so_type == 3 ? 2 : 4;
+        uint8_t b[4];
+        if (!pread_all(fd, b, each, so_field)) return fail(SARDINE_ERR_SLC_IO, "I/O error reading the strip offsets of %s", path);
+        first = each == 2 ? rd16(b, le) : rd32(b, le);
+        if (!pread_all(fd, b, each, so_field + ((uint64_t)so_count - 1) * each))
+            return fail(SARDINE_ERR_SLC_IO, "I/O error reading the strip offsets of %s", path);
+        last = each == 2 ? rd16(b, le) : rd32(b, le);
+    }
+    const uint64_t expected = first + ((uint64_t)height.v - 1) * row;
+    if (last != expected)
+        return fail(SARDINE_ERR_SLC_NON_CONTIGUOUS_STRIPS,
+                    "strip offsets not contiguous (strip[0]=%llu, strip[%u-1]=%llu, expected=%llu)",
+                    (unsigned long long)first, so_count, (unsigned long long)last, (unsigned long long)expected);
+    out->width = width.v;
+    out->height = height.v;
+    out->little_endian = le ? 1 : 0;
+    out->pad = 0;
+    out->base_offset = first;
+    out->row_bytes = row;
+    return SARDINE_OK;
+}
+
+// Pinned chunks for file → HBM reads, kept for the process: two sets of
+// kSlcSet chunks; a set is filled by parallel preads while the other one's
+// copies run.  One reader at a time.
+constexpr int kSlcSet = 4;
+constexpr size_t kSlcChunk = 16u << 20;
+struct SlcPinned {
+    std::mutex mu;
+    int dev = -1;
+    void* buf[2][kSlcSet] = {};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+};
+SlcPinned& slc_pinned() {
+    static SlcPinned* p = new SlcPinned();  // never destroyed: pinned memory outlives static teardown
+    return *p;
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_slc_open(const char* path, sardine_slc_tiff* tiff) {
+    if (!path || !tiff) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or tiff is null");
+    Fd f;
+    f.fd = ::open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return fail(SARDINE_ERR_SLC_IO, "I/O error: %s: %s", path, std::strerror(errno));
+    return slc_parse(f.fd, path, tiff);
+}
+
+sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* t, int64_t first_line,
+                                      int64_t line_count, int16_t* dst, void* stream_v) {
+    if (!path || !t) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or tiff is null");
+    if (first_line < 0 || line_count < 0 || first_line > (int64_t)t->height ||
+        line_count > (int64_t)t->height - first_line)
+        return fail(SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS, "line range [%lld, %lld) is out of image bounds [0, %u)",
+                    (long long)first_line, (long long)(first_line + line_count), t->height);
+    if (t->row_bytes != (uint64_t)t->width * 4) return fail(SARDINE_ERR_INVALID_ARGUMENT, "inconsistent TIFF layout");
+    const uint64_t bytes = (uint64_t)line_count * t->row_bytes;
+    if (bytes == 0) return SARDINE_OK;
+    if (!dst) return fail(SARDINE_ERR_INVALID_ARGUMENT, "dst is null");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    Fd f;
+    f.fd = ::open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return fail(SARDINE_ERR_SLC_IO, "I/O error: %s: %s", path, std::strerror(errno));
+    hipStream_t s = (hipStream_t)stream_v;
+    SlcPinned& P = slc_pinned();
+    std::lock_guard<std::mutex> g(P.mu);
+    int dev = 0;
+    SD_HIP(hipGetDevice(&dev));
+    if (P.dev != dev) {  // (re)create the chunks on this device's context
+        for (int k = 0; k < 2; ++k) {
+            if (P.done[k]) {
+                SD_HIP(hipEventSynchronize(P.done[k]));
+                (void)hipEventDestroy(P.done[k]);
+                P.done[k] = nullptr;
+            }
+            for (int i = 0; i < kSlcSet; ++i)
+                if (P.buf[k][i]) {
+                    (void)hipHostFree(P.buf[k][i]);
+                    P.buf[k][i] = nullptr;
+                }
+            P.used[k] = false;
+        }
+        for (int k = 0; k < 2; ++k) {
+            for (int i = 0; i < kSlcSet; ++i) SD_HIP(hipHostMalloc(&P.buf[k][i], kSlcChunk, hipHostMallocDefault));
+            SD_HIP(hipEventCreateWithFlags(&P.done[k], hipEventDisableTiming));
+        }
+        P.dev = dev;
+    }
+    const uint64_t base = t->base_offset + (uint64_t)first_line * t->row_bytes;
+    uint8_t* out = reinterpret_cast<uint8_t*>(dst);
+    const uint64_t per_set = (uint64_t)kSlcSet * kSlcChunk;
+    int set = 0;
+    for (uint64_t done = 0; done < bytes; done += per_set, set ^= 1) {
+        if (P.used[set]) SD_HIP(hipEventSynchronize(P.done[set]));  // its previous copies have finished
+        const uint64_t n_set = std::min(per_set, bytes - done);
+        const int n_chunks = (int)((n_set + kSlcChunk - 1) / kSlcChunk);
+        bool ok[kSlcSet] = {};
+        std::vector<std::thread> workers;
+        for (int i = 0; i < n_chunks; ++i) {
+            const uint64_t off = (uint64_t)i * kSlcChunk, n = std::min<uint64_t>(kSlcChunk, n_set - off);
+            workers.emplace_back([&, i, off, n] { ok[i] = pread_all(f.fd, P.buf[set][i], (size_t)n, base + done + off); });
+        }
+        for (auto& w : workers) w.join();
+        for (int i = 0; i < n_chunks; ++i)
+            if (!ok[i]) return fail(SARDINE_ERR_SLC_IO, "I/O error reading %s (file shorter than its IFD says?)", path);
+        for (int i = 0; i < n_chunks; ++i) {
+            const uint64_t off = (uint64_t)i * kSlcChunk, n = std::min<uint64_t>(kSlcChunk, n_set - off);
+            SD_HIP(hipMemcpyAsync(out + done + off, P.buf[set][i], (size_t)n, hipMemcpyHostToDevice, s));
+        }
+        SD_HIP(hipEventRecord(P.done[set], s));
+        P.used[set] = true;
+    }
+    if (!t->little_endian) SD_HIP(sd_launch_swap16(reinterpret_cast<uint16_t*>(dst), (int64_t)(bytes / 2), s));
+    // the chunks are reused by the next call only after these copies: the
+    // next call waits on the events before refilling a set
     return SARDINE_OK;
 }
 

@@ -1,4 +1,5 @@
-// misc.hip — small streaming kernels: the dB epilogue (export.rs:59-94),
+// misc.hip — small streaming kernels: the dB epilogue (export.rs:59-94), the
+// byte swap of big-endian CInt16 rows (slc_reader.rs:565-571),
 // the synthetic CInt16 generator used for benchmark inputs, and a zero fill
 // (used instead of hipMemsetAsync so the work can be captured in graphs).
 #include <hip/hip_runtime.h>
@@ -38,6 +39,30 @@ __global__ __launch_bounds__(256) void to_db_kernel(float* d, int64_t n, float f
     }
 }
 
+// Swaps the two bytes of every 16-bit value (big-endian TIFF rows → host
+// order), four values per 8-byte word; `n` counts 16-bit values.
+__global__ __launch_bounds__(256) void swap16_kernel(uint16_t* d, int64_t n) {
+    const int64_t words = n / 4;
+    uint64_t* w = reinterpret_cast<uint64_t*>(d);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = w[i];
+        w[i] = ((x & 0x00ff00ff00ff00ffull) << 8) | ((x >> 8) & 0x00ff00ff00ff00ffull);
+    }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n - words * 4) {
+        const uint16_t x = d[words * 4 + t];
+        d[words * 4 + t] = (uint16_t)((x >> 8) | (x << 8));
+    }
+}
+
+// The same, one 16-bit value per thread, for buffers not 8-byte aligned.
+__global__ __launch_bounds__(256) void swap16_scalar_kernel(uint16_t* d, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint16_t x = d[i];
+        d[i] = (uint16_t)((x >> 8) | (x << 8));
+    }
+}
+
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     uint64_t z = x;
@@ -68,6 +93,17 @@ __global__ __launch_bounds__(256) void synth_cint16_kernel(int16_t* out, int64_t
 extern "C" hipError_t sd_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(sd::zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sd_launch_swap16(uint16_t* d, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const bool aligned = reinterpret_cast<uintptr_t>(d) % 8 == 0;
+    int64_t blocks = ((aligned ? n / 4 : n) + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 65536) blocks = 65536;
+    if (aligned) hipLaunchKernelGGL(sd::swap16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n);
+    else hipLaunchKernelGGL(sd::swap16_scalar_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
   MergeError              sardine/src/merge_subswaths.rs:62-81
   ExportError             sardine/src/export.rs:26-39
   GroundRangeError        sardine/src/ground_range.rs:31-47
+  SlcReadError            sardine/src/slc_reader.rs:66-131
 Each carries the C-ABI status code (`.status`) and its name (`.variant`).
 """
 from __future__ import annotations
@@ -49,6 +50,10 @@ class GroundRangeError(SardineError):
     pass
 
 
+class SlcReadError(SardineError):
+    pass
+
+
 class DeviceError(SardineError):
     pass
 
@@ -70,4 +75,6 @@ def raise_for(status: int, message: str):
         raise ExportError(status, message)
     if 70 <= status < 80:
         raise GroundRangeError(status, message)
+    if 80 <= status < 90:
+        raise SlcReadError(status, message)
     raise DeviceError(status, message)

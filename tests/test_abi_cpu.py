@@ -443,3 +443,52 @@ def test_ground_range_errors_match_oracle():
         d = np.zeros(1000, np.float32)
         assert L.sardine_to_ground_range(d.ctypes.data, C.byref(_grd_info(10, 100)), pts, counts.ctypes.data, 3,
                                          10.0, 14.08, d.ctypes.data, C.byref(img), None) == abi.ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("case", ["ok", "be", "short", "single", "gap", "bigtiff", "magic", "comp", "fmt", "bps",
+                                  "rps", "count", "noncontig", "type_rational", "drop256", "drop257", "drop258",
+                                  "drop259", "drop278", "drop339", "drop273", "drop277", "text", "empty", "missing",
+                                  "be_short_single"])
+def test_slc_open_matches_oracle(tmp_path, case):
+    """SlcReader::open is host code: status and layout equal the oracle's."""
+    import slc_fixtures as F
+    d = np.random.default_rng(5).integers(-32768, 32768, (1 if "single" in case else 12, 9, 2)).astype(np.int16)
+    kw = {"be": dict(le=False), "short": dict(offsets_type=3), "gap": dict(data_gap=40), "bigtiff": dict(magic=43),
+          "magic": dict(magic=41), "comp": dict(overrides={259: (3, 5)}), "fmt": dict(overrides={339: (3, 1)}),
+          "bps": dict(overrides={258: (3, 16)}), "rps": dict(overrides={278: (4, 2)}), "count": dict(strip_count=11),
+          "noncontig": dict(gap_last=4), "type_rational": dict(overrides={256: (5, 9)}),
+          "be_short_single": dict(le=False, offsets_type=3)}.get(case, {})
+    if case.startswith("drop"):
+        kw = dict(drop=(int(case[4:]),))
+    p = tmp_path / "c.tiff"
+    if case == "text":
+        p.write_bytes(b"This is not a TIFF file at all!!!")
+    elif case == "empty":
+        p.write_bytes(b"")
+    elif case != "missing":
+        p.write_bytes(F.slc_tiff_bytes(d, **kw))
+    t = abi.SlcTiff()
+    st = _native.lib().sardine_slc_open(str(p).encode(), C.byref(t))
+    ref_st, ref, _ = O.slc_open(p)
+    assert st == ref_st, (st, ref_st, _native.last_error())
+    if st == 0:
+        assert bytes(t) == bytes(ref)
+
+
+def test_slc_read_errors_precede_device_check(tmp_path):
+    import slc_fixtures as F
+    d = np.zeros((12, 9, 2), np.int16)
+    F.write_slc_tiff(tmp_path / "s.tiff", d)
+    L = _native.lib()
+    t = abi.SlcTiff()
+    assert L.sardine_slc_open(str(tmp_path / "s.tiff").encode(), C.byref(t)) == 0
+    p = str(tmp_path / "s.tiff").encode()
+    buf = np.zeros(12 * 9 * 2, np.int16)
+    assert L.sardine_slc_read_lines(p, C.byref(t), 12, 1, buf.ctypes.data, None) == abi.ERR_SLC_LINE_OUT_OF_BOUNDS
+    assert L.sardine_slc_read_lines(p, C.byref(t), 10, 10, buf.ctypes.data, None) == abi.ERR_SLC_LINE_OUT_OF_BOUNDS
+    assert L.sardine_slc_read_lines(p, C.byref(t), -1, 1, buf.ctypes.data, None) == abi.ERR_SLC_LINE_OUT_OF_BOUNDS
+    assert L.sardine_slc_read_lines(p, C.byref(t), 3, 0, None, None) == 0  # nothing to read
+    import torch
+
+    if not torch.cuda.is_available():
+        assert L.sardine_slc_read_lines(p, C.byref(t), 0, 12, buf.ctypes.data, None) == abi.ERR_NO_DEVICE

@@ -1084,6 +1084,74 @@ def test_to_ground_range_against_numpy(az_spacing, lines):
     assert np.array_equal(np.isnan(out), np.isnan(want)) and _ulp_diff(out, want) <= 1
 
 
+# ── SLC measurement TIFF reader (slc_reader.rs:600-828) ───────────────────
+def _slc_data(h, w, seed=0):
+    return np.random.default_rng(seed).integers(-32768, 32768, (h, w, 2)).astype(np.int16)
+
+
+def test_slc_reader_reference_cases(tmp_path):
+    import slc_fixtures as F
+    assert O.slc_open(tmp_path / "missing.tiff")[0] == abi.ERR_SLC_IO  # :628-632
+    (tmp_path / "empty.tiff").write_bytes(b"")  # :634-644
+    assert O.slc_open(tmp_path / "empty.tiff")[0] in (abi.ERR_SLC_IO, abi.ERR_SLC_NOT_TIFF)
+    (tmp_path / "text.tiff").write_bytes(b"This is not a TIFF file at all!!!")  # :646-657
+    assert O.slc_open(tmp_path / "text.tiff")[0] == abi.ERR_SLC_NOT_TIFF
+    (tmp_path / "spp2.tiff").write_bytes(F.minimal_tiff([(256, 4, 1, 10), (257, 4, 1, 1), (258, 3, 1, 32),
+                                                         (259, 3, 1, 1), (273, 4, 1, 9999), (277, 3, 1, 2),
+                                                         (278, 4, 1, 1), (339, 3, 1, 5)]))  # :701-730
+    assert O.slc_open(tmp_path / "spp2.tiff")[0] == abi.ERR_SLC_UNSUPPORTED_FORMAT
+    d = _slc_data(40, 33)
+    F.write_slc_tiff(tmp_path / "s.tiff", d)
+    st, t, _ = O.slc_open(tmp_path / "s.tiff")
+    assert st == 0 and (t.width, t.height, t.little_endian) == (33, 40, 1)
+    st, rows = O.slc_read(tmp_path / "s.tiff", t, 0, 1)  # one row (:792-801)
+    assert st == 0 and rows.shape == (1, 33, 2) and np.array_equal(rows[0], d[0])
+    st, rows = O.slc_read(tmp_path / "s.tiff", t, 30, 10)  # exactly to the end (:755-766)
+    assert st == 0 and np.array_equal(rows, d[30:])
+    assert O.slc_read(tmp_path / "s.tiff", t, 40, 1)[0] == abi.ERR_SLC_LINE_OUT_OF_BOUNDS  # :768-778
+    assert O.slc_read(tmp_path / "s.tiff", t, 38, 10)[0] == abi.ERR_SLC_LINE_OUT_OF_BOUNDS  # :780-790
+    assert np.array_equal(rows.astype(np.float32)[..., 0], d[30:, :, 0].astype(np.float32))  # sign kept (:815-827)
+
+
+@pytest.mark.parametrize("case", ["be", "short", "single", "gap", "bigtiff", "magic", "comp", "fmt", "bps", "rps",
+                                  "count", "noncontig", "type_rational", "drop256", "drop257", "drop258", "drop259",
+                                  "drop278", "drop339", "drop273", "drop277", "truncated"])
+def test_slc_reader_format_cases(tmp_path, case):
+    """Each format rule of parse_ifd (slc_reader.rs:268-461), against the bytes
+    the fixture writer laid out."""
+    import slc_fixtures as F
+    d = _slc_data(1 if case == "single" else 12, 9, seed=3)
+    kw = {"be": dict(le=False), "short": dict(offsets_type=3), "single": {}, "gap": dict(data_gap=40),
+          "bigtiff": dict(magic=43), "magic": dict(magic=41), "comp": dict(overrides={259: (3, 5)}),
+          "fmt": dict(overrides={339: (3, 1)}), "bps": dict(overrides={258: (3, 16)}),
+          "rps": dict(overrides={278: (4, 2)}), "count": dict(strip_count=11), "noncontig": dict(gap_last=4),
+          "type_rational": dict(overrides={256: (5, 9)}), "truncated": {}}.get(case, {})
+    if case.startswith("drop"):
+        kw = dict(drop=(int(case[4:]),))
+    b = F.slc_tiff_bytes(d, **kw)
+    if case == "truncated":
+        b = b[:-10]
+    (tmp_path / "c.tiff").write_bytes(b)
+    st, t, tag = O.slc_open(tmp_path / "c.tiff")
+    want = {"bigtiff": abi.ERR_SLC_BIGTIFF, "magic": abi.ERR_SLC_NOT_TIFF, "comp": abi.ERR_SLC_UNSUPPORTED_FORMAT,
+            "fmt": abi.ERR_SLC_UNSUPPORTED_FORMAT, "bps": abi.ERR_SLC_UNSUPPORTED_FORMAT,
+            "rps": abi.ERR_SLC_UNSUPPORTED_FORMAT, "count": abi.ERR_SLC_UNSUPPORTED_FORMAT,
+            "noncontig": abi.ERR_SLC_NON_CONTIGUOUS_STRIPS, "type_rational": abi.ERR_SLC_MISSING_TAG}.get(case, 0)
+    if case.startswith("drop"):
+        want = 0 if case == "drop277" else abi.ERR_SLC_MISSING_TAG  # SamplesPerPixel defaults to 1
+        if case == "drop279":
+            want = 0
+    assert st == want, (case, st)
+    if want == abi.ERR_SLC_MISSING_TAG:
+        assert tag == (256 if case == "type_rational" else int(case[4:]))
+    if st == 0:
+        st, rows = O.slc_read(tmp_path / "c.tiff", t, 0, d.shape[0])
+        if case == "truncated":
+            assert st == abi.ERR_SLC_IO
+        else:
+            assert st == 0 and np.array_equal(rows, d)
+
+
 # ── full chain invariants on a small synthetic scene ──────────────────────
 def test_small_scene_chain_invariants():
     sc = synthetic.scene_small_plain(2, lines=256, samples=512, out_rows=160, out_cols=160, dem_side=256)
