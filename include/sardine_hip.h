@@ -94,6 +94,11 @@ typedef enum sardine_status {
     SARDINE_ERR_SLC_UNSUPPORTED_FORMAT = 84,
     SARDINE_ERR_SLC_NON_CONTIGUOUS_STRIPS = 85,
     SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS = 86,
+    /* GeoidError (geoid.rs:87-104) */
+    SARDINE_ERR_GEOID_IO = 100,
+    SARDINE_ERR_GEOID_WRONG_SIZE = 101,
+    SARDINE_ERR_GEOID_WRONG_SIZE_EGM2008 = 102,
+    SARDINE_ERR_GEOID_MALFORMED_ASCII = 103,
     /* library / device */
     SARDINE_ERR_INVALID_ARGUMENT = 90,
     SARDINE_ERR_NO_DEVICE = 91,
@@ -442,6 +447,24 @@ sardine_status sardine_apply_calibration(const int16_t* deburst /*[device] lines
                                          float* out, float* nesz,
                                          uint32_t* cal_gap_px, uint32_t* noise_gap_px,
                                          void* stream);
+
+/* ── Geoid grid files ───────────────────────────────────────────────────── */
+/* The 73 x 144 2.5-degree undulation grid the terrain correction samples
+ * (sardine_geoid kind SARDINE_GEOID_GRID_2P5): row 0 = 90N, col 0 = 0E,
+ * metres.  EGM96 and EGM2008 share layout and interpolation; the model only
+ * selects the size error's variant.  Host only. */
+typedef enum sardine_geoid_model {
+    SARDINE_GEOID_MODEL_EGM96 = 0,
+    SARDINE_GEOID_MODEL_EGM2008 = 1
+} sardine_geoid_model;
+
+/* Egm96Grid::load_binary / Egm2008Grid::load_binary (geoid.rs:132-160,
+ * 276-299): exactly 10512 little-endian float32 values. */
+sardine_status sardine_geoid_load_binary(const char* path, int32_t model, float grid[10512]);
+
+/* Egm96Grid::from_ww15mgh (geoid.rs:164-217): the NGA ASCII grid (up to two
+ * header lines, then 10512 values). */
+sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]);
 
 /* ── SLC measurement TIFF → HBM ──────────────────────────────────────────── */
 

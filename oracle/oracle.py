@@ -84,6 +84,8 @@ def lib():
         L.oracle_grd_column_map.restype = None
         L.oracle_to_ground_range.argtypes = [P, C.POINTER(abi.MergedInfo), C.POINTER(abi.GrdGridPoint), P, i32, d,
                                              d, P, C.POINTER(abi.GrdImage)]
+        L.oracle_geoid_load_binary.argtypes = [C.c_char_p, i32, P]
+        L.oracle_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]
         L.oracle_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), C.POINTER(i32)]
         L.oracle_slc_read.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), i64, i64, P]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
@@ -443,6 +445,14 @@ def to_ground_range(data: np.ndarray, info, grids, target_spacing_m: float, azim
     st = lib().oracle_to_ground_range(_ptr(data), C.byref(info), pts, _ptr(counts), len(grids), target_spacing_m,
                                       azimuth_pixel_spacing_m, _ptr(out), C.byref(img))
     return st, out[: img.lines * img.samples].reshape(img.lines, img.samples), img
+
+
+def geoid_load(path, ascii: bool = False, model: int = 0):
+    """Egm96Grid / Egm2008Grid loaders (geoid.rs:132-299) → (status, (73, 144) grid)."""
+    g = np.zeros(73 * 144, np.float32)
+    st = (lib().oracle_geoid_load_ww15mgh(str(path).encode(), _ptr(g)) if ascii else
+          lib().oracle_geoid_load_binary(str(path).encode(), model, _ptr(g)))
+    return st, g.reshape(73, 144)
 
 
 def slc_open(path):

@@ -33,6 +33,8 @@
 //   to_ground_range            ground_range.rs:506-724 (ground spacing, column
 //                              map monotonic / covering, range interpolator,
 //                              uniform synthetic scene, the three rejections)
+//   geoid grid files           geoid.rs:596-604 (wrong size) + round trips,
+//                              the WW15MGH header rules, malformed tokens
 //   SLC TIFF reader            slc_reader.rs:618-820 (non-TIFF, empty, missing
 //                              file, SamplesPerPixel 2, line bounds, row count)
 //   striped GeoTIFF writers    export.rs:1276-1580 (geotransform errors, key
@@ -47,8 +49,10 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cctype>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <limits>
 #include <thread>
 #include <vector>
@@ -2135,6 +2139,105 @@ int oracle_slc_read(const char* path, const sardine_slc_tiff* t, int64_t first, 
     if (n && !read_at(sf.f, t->base_offset + (uint64_t)first * t->row_bytes, dst, n)) return SARDINE_ERR_SLC_IO;
     if (!t->little_endian)
         for (size_t i = 0; i < n / 2; ++i) dst[i] = (int16_t)(((uint16_t)dst[i] >> 8) | ((uint16_t)dst[i] << 8));
+    return 0;
+}
+
+}  // extern "C"
+
+// ── geoid grid files (geoid.rs:132-299) ────────────────────────────────────
+namespace {
+
+// Rust's float grammar for `str::parse` (sign; inf / infinity / nan in any
+// case; or digits[.digits] / .digits with an optional exponent).
+bool rust_float_token(const char* s) {
+    if (*s == '+' || *s == '-') ++s;
+    std::string low;
+    for (const char* p = s; *p; ++p) low += (char)std::tolower((unsigned char)*p);
+    if (low == "inf" || low == "infinity" || low == "nan") return true;
+    int digits = 0;
+    while (std::isdigit((unsigned char)*s)) ++s, ++digits;
+    if (*s == '.') {
+        ++s;
+        while (std::isdigit((unsigned char)*s)) ++s, ++digits;
+    }
+    if (!digits) return false;
+    if (*s == 'e' || *s == 'E') {
+        ++s;
+        if (*s == '+' || *s == '-') ++s;
+        int e = 0;
+        while (std::isdigit((unsigned char)*s)) ++s, ++e;
+        if (!e) return false;
+    }
+    return *s == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_geoid_load_binary(const char* path, int model, float* grid) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return SARDINE_ERR_GEOID_IO;
+    std::vector<unsigned char> raw;
+    int c;
+    while ((c = std::fgetc(f)) != EOF) raw.push_back((unsigned char)c);
+    std::fclose(f);
+    if (raw.size() != 73 * 144 * 4) return model == 0 ? SARDINE_ERR_GEOID_WRONG_SIZE : SARDINE_ERR_GEOID_WRONG_SIZE_EGM2008;
+    for (size_t i = 0; i < 73 * 144; ++i) {
+        uint32_t u = (uint32_t)raw[4 * i] | (uint32_t)raw[4 * i + 1] << 8 | (uint32_t)raw[4 * i + 2] << 16 |
+                     (uint32_t)raw[4 * i + 3] << 24;
+        std::memcpy(&grid[i], &u, 4);
+    }
+    return 0;
+}
+
+int oracle_geoid_load_ww15mgh(const char* path, float* grid) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return SARDINE_ERR_GEOID_IO;
+    std::vector<float> vals;
+    int headers = 0;
+    std::string line;
+    auto flush_line = [&](const std::string& ln) -> int {
+        std::vector<std::string> tok;
+        std::string cur;
+        for (char ch : ln) {
+            if (std::isspace((unsigned char)ch)) {
+                if (!cur.empty()) tok.push_back(cur);
+                cur.clear();
+            } else {
+                cur += ch;
+            }
+        }
+        if (!cur.empty()) tok.push_back(cur);
+        if (tok.empty()) return 0;
+        if (headers < 2) {
+            bool all = true;
+            for (auto& t : tok) all = all && rust_float_token(t.c_str());
+            if (!all || tok.size() <= 6) {
+                ++headers;
+                return 0;
+            }
+        }
+        for (auto& t : tok) {
+            if (!rust_float_token(t.c_str())) return SARDINE_ERR_GEOID_MALFORMED_ASCII;
+            vals.push_back(std::strtof(t.c_str(), nullptr));
+        }
+        return 0;
+    };
+    int c, st = 0;
+    while ((c = std::fgetc(f)) != EOF && !st) {
+        if (c == '\n') {
+            st = flush_line(line);
+            line.clear();
+        } else {
+            line += (char)c;
+        }
+    }
+    if (!st) st = flush_line(line);
+    std::fclose(f);
+    if (st) return st;
+    if (vals.size() != 73 * 144) return SARDINE_ERR_GEOID_MALFORMED_ASCII;
+    std::memcpy(grid, vals.data(), vals.size() * sizeof(float));
     return 0;
 }
 

@@ -65,6 +65,8 @@ def lib():
                                           C.POINTER(abi.MergedInfo), P]
     L.sardine_to_ground_range.argtypes = [P, C.POINTER(abi.MergedInfo), C.POINTER(abi.GrdGridPoint), P, i32, d, d,
                                           P, C.POINTER(abi.GrdImage), P]
+    L.sardine_geoid_load_binary.argtypes = [C.c_char_p, i32, P]
+    L.sardine_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]
     L.sardine_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff)]
     L.sardine_slc_read_lines.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), i64, i64, P, P]
     L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]

@@ -54,6 +54,11 @@ ERR_SLC_MISSING_TAG = 83
 ERR_SLC_UNSUPPORTED_FORMAT = 84
 ERR_SLC_NON_CONTIGUOUS_STRIPS = 85
 ERR_SLC_LINE_OUT_OF_BOUNDS = 86
+ERR_GEOID_IO = 100
+ERR_GEOID_WRONG_SIZE = 101
+ERR_GEOID_WRONG_SIZE_EGM2008 = 102
+ERR_GEOID_MALFORMED_ASCII = 103
+GEOID_MODEL_EGM96, GEOID_MODEL_EGM2008 = 0, 1
 ERR_INVALID_ARGUMENT = 90
 ERR_NO_DEVICE = 91
 ERR_HIP = 92
@@ -261,6 +266,8 @@ EXPORTED_SYMBOLS = (
     "sardine_apply_calibration",
     "sardine_to_db_inplace",
     "sardine_apply_multilook",
+    "sardine_geoid_load_binary",
+    "sardine_geoid_load_ww15mgh",
     "sardine_slc_open",
     "sardine_slc_read_lines",
     "sardine_to_ground_range",

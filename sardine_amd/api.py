@@ -409,6 +409,17 @@ def apply_multilook(merged: MergedSigma0, range_looks: int, azimuth_looks: int, 
                         res.noise_lut_extrapolation_gap_px, res)
 
 
+def load_geoid_grid(path, model: int = abi.GEOID_MODEL_EGM96, ascii: bool = False) -> np.ndarray:
+    """Egm96Grid::load_binary / Egm2008Grid::load_binary / Egm96Grid::from_ww15mgh
+    (geoid.rs:132-299) → the (73, 144) float32 grid TerrainCorrectionConfig
+    takes as `geoid_grid`."""
+    g = np.zeros(73 * 144, np.float32)
+    st = (lib().sardine_geoid_load_ww15mgh(str(path).encode(), g.ctypes.data) if ascii else
+          lib().sardine_geoid_load_binary(str(path).encode(), model, g.ctypes.data))
+    _check(st)
+    return g.reshape(73, 144)
+
+
 class SlcReader:
     """SlcReader (slc_reader.rs:466-590) reading rows straight into HBM."""
 

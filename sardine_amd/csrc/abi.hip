@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <condition_variable>
@@ -1328,6 +1329,120 @@ sardine_status sardine_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint
     sardine_status st;
     if ((st = require_device())) return st;
     SD_HIP(sd_launch_synth_cint16(out, n, seed, base, sigma, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── geoid grid files (geoid.rs:132-299) ──────────────────────────────────
+namespace {
+
+constexpr size_t kGeoidValues = 73 * 144;
+
+bool read_file(const char* path, std::string* out) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    char buf[1 << 16];
+    size_t n;
+    out->clear();
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) out->append(buf, n);
+    const bool ok = !std::ferror(f);
+    std::fclose(f);
+    return ok;
+}
+
+// A token Rust's `str::parse::<f64>()` / `::<f32>()` accepts: an optional
+// sign, then "inf", "infinity" or "nan" (any case), or digits with an
+// optional fraction (at least one digit in all) and an optional exponent
+// with at least one digit.  No hex, no partial parses.
+bool is_rust_float(const std::string& t) {
+    size_t i = 0;
+    if (i < t.size() && (t[i] == '+' || t[i] == '-')) ++i;
+    std::string rest = t.substr(i);
+    for (char& c : rest) c = (char)std::tolower((unsigned char)c);
+    if (rest == "inf" || rest == "infinity" || rest == "nan") return true;
+    size_t digits = 0;
+    while (i < t.size() && std::isdigit((unsigned char)t[i])) ++i, ++digits;
+    if (i < t.size() && t[i] == '.') {
+        ++i;
+        while (i < t.size() && std::isdigit((unsigned char)t[i])) ++i, ++digits;
+    }
+    if (digits == 0) return false;
+    if (i < t.size() && (t[i] == 'e' || t[i] == 'E')) {
+        ++i;
+        if (i < t.size() && (t[i] == '+' || t[i] == '-')) ++i;
+        size_t ed = 0;
+        while (i < t.size() && std::isdigit((unsigned char)t[i])) ++i, ++ed;
+        if (ed == 0) return false;
+    }
+    return i == t.size();
+}
+
+std::vector<std::string> split_ws(const std::string& line) {
+    std::vector<std::string> out;
+    size_t i = 0;
+    while (i < line.size()) {
+        while (i < line.size() && std::isspace((unsigned char)line[i])) ++i;
+        const size_t a = i;
+        while (i < line.size() && !std::isspace((unsigned char)line[i])) ++i;
+        if (i > a) out.push_back(line.substr(a, i - a));
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_geoid_load_binary(const char* path, int32_t model, float grid[10512]) {
+    if (!path || !grid) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or grid is null");
+    if (model != SARDINE_GEOID_MODEL_EGM96 && model != SARDINE_GEOID_MODEL_EGM2008)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown geoid model %d", model);
+    std::string raw;
+    if (!read_file(path, &raw)) return fail(SARDINE_ERR_GEOID_IO, "IO error reading %s: %s", path, std::strerror(errno));
+    if (raw.size() != kGeoidValues * 4)
+        return fail(model == SARDINE_GEOID_MODEL_EGM96 ? SARDINE_ERR_GEOID_WRONG_SIZE : SARDINE_ERR_GEOID_WRONG_SIZE_EGM2008,
+                    "%s binary grid file %s has wrong size: expected %zu bytes, got %zu",
+                    model == SARDINE_GEOID_MODEL_EGM96 ? "EGM96" : "EGM2008", path, kGeoidValues * 4, raw.size());
+    std::memcpy(grid, raw.data(), raw.size());  // little-endian host, as the file
+    return SARDINE_OK;
+}
+
+sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]) {
+    if (!path || !grid) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or grid is null");
+    std::string text;
+    if (!read_file(path, &text)) return fail(SARDINE_ERR_GEOID_IO, "IO error reading %s: %s", path, std::strerror(errno));
+    std::vector<float> values;
+    values.reserve(kGeoidValues);
+    int headers = 0;
+    size_t pos = 0;
+    while (pos <= text.size()) {
+        size_t nl = text.find('\n', pos);
+        if (nl == std::string::npos) nl = text.size();
+        const std::vector<std::string> tok = split_ws(text.substr(pos, nl - pos));
+        pos = nl + 1;
+        if (tok.empty()) continue;
+        // up to two header lines: text, or the numeric bounds line (<= 6 numbers)
+        if (headers < 2) {
+            bool numeric = true;
+            for (const std::string& t : tok) numeric = numeric && is_rust_float(t);
+            if (!numeric || tok.size() <= 6) {
+                ++headers;
+                continue;
+            }
+        }
+        for (const std::string& t : tok) {
+            if (!is_rust_float(t))
+                return fail(SARDINE_ERR_GEOID_MALFORMED_ASCII,
+                            "EGM96 ASCII grid file %s is malformed: cannot parse token '%s' as f32", path, t.c_str());
+            values.push_back(std::strtof(t.c_str(), nullptr));
+        }
+    }
+    if (values.size() != kGeoidValues)
+        return fail(SARDINE_ERR_GEOID_MALFORMED_ASCII,
+                    "EGM96 ASCII grid file %s is malformed: expected %zu undulation values, found %zu", path,
+                    kGeoidValues, values.size());
+    std::memcpy(grid, values.data(), kGeoidValues * sizeof(float));
     return SARDINE_OK;
 }
 

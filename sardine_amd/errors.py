@@ -8,6 +8,7 @@
   ExportError             sardine/src/export.rs:26-39
   GroundRangeError        sardine/src/ground_range.rs:31-47
   SlcReadError            sardine/src/slc_reader.rs:66-131
+  GeoidError              sardine/src/geoid.rs:87-104
 Each carries the C-ABI status code (`.status`) and its name (`.variant`).
 """
 from __future__ import annotations
@@ -54,6 +55,10 @@ class SlcReadError(SardineError):
     pass
 
 
+class GeoidError(SardineError):
+    pass
+
+
 class DeviceError(SardineError):
     pass
 
@@ -77,4 +82,6 @@ def raise_for(status: int, message: str):
         raise GroundRangeError(status, message)
     if 80 <= status < 90:
         raise SlcReadError(status, message)
+    if 100 <= status < 110:
+        raise GeoidError(status, message)
     raise DeviceError(status, message)

@@ -492,3 +492,46 @@ def test_slc_read_errors_precede_device_check(tmp_path):
 
     if not torch.cuda.is_available():
         assert L.sardine_slc_read_lines(p, C.byref(t), 0, 12, buf.ctypes.data, None) == abi.ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("case", ["bin96", "bin2008", "short96", "short2008", "missing", "ascii", "ascii_crlf",
+                                  "ascii_one_header", "ascii_short_lines", "ascii_bad_token", "ascii_hex",
+                                  "ascii_specials", "ascii_no_header", "ascii_extra_value"])
+def test_geoid_loaders_match_oracle(tmp_path, case):
+    """The grid loaders are host code: status and values equal the oracle's."""
+    import test_oracle_reference as T
+    g = (np.random.default_rng(4).normal(0, 30, (73, 144)) * 1e5).round() / 1e5
+    p = tmp_path / "g"
+    model = 1 if "2008" in case else 0
+    if case.startswith("bin"):
+        p.write_bytes(g.astype("<f4").tobytes())
+    elif case.startswith("short"):
+        p.write_bytes(b"short")
+    elif case == "ascii":
+        p.write_text(T._ww15mgh_text(g))
+    elif case == "ascii_crlf":
+        p.write_text(T._ww15mgh_text(g, eol="\r\n"))
+    elif case == "ascii_one_header":
+        p.write_text(T._ww15mgh_text(g, header=("-90 90 0 360 2.5 2.5",)))
+    elif case == "ascii_short_lines":
+        p.write_text(T._ww15mgh_text(g, per_line=6))
+    elif case == "ascii_no_header":
+        p.write_text(T._ww15mgh_text(g, header=()))
+    elif case == "ascii_extra_value":
+        p.write_text(T._ww15mgh_text(g) + "1.0\n")
+    elif case in ("ascii_bad_token", "ascii_hex", "ascii_specials"):
+        lines = T._ww15mgh_text(g).split("\n")
+        first = lines[7].split()[0]
+        lines[7] = lines[7].replace(first, {"ascii_bad_token": "1.2.3", "ascii_hex": "0x1p3",
+                                            "ascii_specials": "-inf"}[case], 1)
+        lines[8] = lines[8].replace(lines[8].split()[0], "NaN" if case == "ascii_specials" else "1e", 1)
+        p.write_text("\n".join(lines))
+    ascii_ = case.startswith("ascii")
+    out = np.zeros(73 * 144, np.float32)
+    L = _native.lib()
+    st = (L.sardine_geoid_load_ww15mgh(str(p).encode(), out.ctypes.data) if ascii_ else
+          L.sardine_geoid_load_binary(str(p).encode(), model, out.ctypes.data))
+    ref_st, ref = O.geoid_load(p, ascii=ascii_, model=model)
+    assert st == ref_st, (st, ref_st, _native.last_error())
+    if st == 0:
+        assert np.array_equal(out.view(np.uint32), ref.ravel().view(np.uint32))

@@ -1174,3 +1174,45 @@ def test_small_scene_chain_invariants():
     r2 = O.terrain_correction(s0, nz, meta, sc.scene_c(keep), sc.dem(), sc.dem_spec, sc.tc_config(compute_lia=1),
                               threads=1)
     assert np.array_equal(r.data, r2.data, equal_nan=True) and np.array_equal(r.mask, r2.mask)
+
+
+# ── geoid grid files (geoid.rs:132-299, 596-604) ──────────────────────────
+def _ww15mgh_text(vals, header=("WW15MGH.GRD synthetic", "-90.000000 90.000000 .000000 360.000000 2.500000 2.500000"),
+                  per_line=8, eol="\n"):
+    lines = list(header)
+    flat = [f"{v:.6f}" for v in vals.ravel()]
+    lines += [" ".join(flat[i:i + per_line]) for i in range(0, len(flat), per_line)]
+    return eol.join(lines) + eol
+
+
+def test_geoid_binary_loader_reference_cases(tmp_path):
+    g = np.random.default_rng(1).normal(0, 30, (73, 144)).astype(np.float32)
+    (tmp_path / "egm.bin").write_bytes(g.astype("<f4").tobytes())
+    for model in (0, 1):
+        st, out = O.geoid_load(tmp_path / "egm.bin", model=model)
+        assert st == 0 and np.array_equal(out, g)
+    (tmp_path / "short.bin").write_bytes(b"short")  # geoid.rs:596-604
+    assert O.geoid_load(tmp_path / "short.bin", model=1)[0] == abi.ERR_GEOID_WRONG_SIZE_EGM2008
+    assert O.geoid_load(tmp_path / "short.bin", model=0)[0] == abi.ERR_GEOID_WRONG_SIZE
+    assert O.geoid_load(tmp_path / "missing.bin")[0] == abi.ERR_GEOID_IO
+
+
+def test_geoid_ww15mgh_loader(tmp_path):
+    g = (np.random.default_rng(2).normal(0, 30, (73, 144)) * 1e6).round() / 1e6
+    p = tmp_path / "ww.grd"
+    p.write_text(_ww15mgh_text(g))
+    st, out = O.geoid_load(p, ascii=True)
+    assert st == 0 and np.array_equal(out, g.astype(np.float32))
+    p.write_text(_ww15mgh_text(g, header=("-90 90 0 360 2.5 2.5",), eol="\r\n"))  # one header line, CRLF
+    st, out = O.geoid_load(p, ascii=True)
+    assert st == 0 and np.array_equal(out, g.astype(np.float32))
+    p.write_text(_ww15mgh_text(g, header=(), per_line=5))  # short data lines are read as headers: 10 values lost
+    assert O.geoid_load(p, ascii=True)[0] == abi.ERR_GEOID_MALFORMED_ASCII
+    bad = _ww15mgh_text(g).split("\n")
+    bad[5] = bad[5] + " 12.3x"
+    p.write_text("\n".join(bad))
+    assert O.geoid_load(p, ascii=True)[0] == abi.ERR_GEOID_MALFORMED_ASCII
+    bad = _ww15mgh_text(g).split("\n")
+    bad[5] = bad[5].replace(bad[5].split()[0], "0x10", 1)  # hex is not a Rust float
+    p.write_text("\n".join(bad))
+    assert O.geoid_load(p, ascii=True)[0] == abi.ERR_GEOID_MALFORMED_ASCII
