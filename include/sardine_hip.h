@@ -99,6 +99,11 @@ typedef enum sardine_status {
     SARDINE_ERR_GEOID_WRONG_SIZE = 101,
     SARDINE_ERR_GEOID_WRONG_SIZE_EGM2008 = 102,
     SARDINE_ERR_GEOID_MALFORMED_ASCII = 103,
+    /* DemError (dem.rs:54-109), the variants the SRTM loader raises */
+    SARDINE_ERR_DEM_IO = 110,
+    SARDINE_ERR_DEM_BAD_SIZE = 111,
+    SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME = 114,
+    SARDINE_ERR_DEM_NO_TILES = 115,
     /* library / device */
     SARDINE_ERR_INVALID_ARGUMENT = 90,
     SARDINE_ERR_NO_DEVICE = 91,
@@ -465,6 +470,21 @@ sardine_status sardine_geoid_load_binary(const char* path, int32_t model, float 
 /* Egm96Grid::from_ww15mgh (geoid.rs:164-217): the NGA ASCII grid (up to two
  * header lines, then 10512 values). */
 sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]);
+
+/* ── SRTM 1" tiles → one device DEM raster ─────────────────────────────── */
+
+/* DemMosaic::load_srtm_files (dem.rs:628-640; SrtmTile::load dem.rs:129-178)
+ * as one raster for the terrain correction: each `.hgt` file (named by its
+ * SW corner, N50E010.hgt; 3601 x 3601 big-endian int16, -32768 = void) lands
+ * at its place in the bounding rectangle of all tiles, neighbours sharing
+ * their edge row/column; voids and cells no tile covers are NaN (the
+ * reference counts both as dem_missing).  The raster: origin = the NW
+ * corner (the first sample), pixel_deg = 1/3600.  With dst NULL only `out`
+ * is filled (host only); else dst [device] holds out->width * out->height
+ * floats and out->data = dst.  Errors: SARDINE_ERR_DEM_NO_TILES,
+ * _UNRECOGNISED_TILE_NAME, _IO, _BAD_SIZE in the reference's order. */
+sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths, float* dst /*[device]*/,
+                                       sardine_dem_raster* out, void* stream);
 
 /* ── SLC measurement TIFF → HBM ──────────────────────────────────────────── */
 

@@ -86,6 +86,7 @@ def lib():
                                              d, P, C.POINTER(abi.GrdImage)]
         L.oracle_geoid_load_binary.argtypes = [C.c_char_p, i32, P]
         L.oracle_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]
+        L.oracle_srtm_sample.argtypes = [P, i32, P, P, i64, P, P]
         L.oracle_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), C.POINTER(i32)]
         L.oracle_slc_read.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), i64, i64, P]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
@@ -453,6 +454,18 @@ def geoid_load(path, ascii: bool = False, model: int = 0):
     st = (lib().oracle_geoid_load_ww15mgh(str(path).encode(), _ptr(g)) if ascii else
           lib().oracle_geoid_load_binary(str(path).encode(), model, _ptr(g)))
     return st, g.reshape(73, 144)
+
+
+def srtm_sample(paths, lat, lon):
+    """SrtmTile::load + DemMosaic::elevation_at_linear (dem.rs:129-178, 730-739)
+    → (status, heights, ok) per point (ok 0 = OutOfBounds, incl. void taps)."""
+    arr = (C.c_char_p * max(len(paths), 1))(*[str(p).encode() for p in paths])
+    lat = np.ascontiguousarray(lat, np.float64)
+    lon = np.ascontiguousarray(lon, np.float64)
+    out = np.empty(max(lat.size, 1), np.float32)
+    ok = np.zeros(max(lat.size, 1), np.int32)
+    st = lib().oracle_srtm_sample(arr, len(paths), _ptr(lat), _ptr(lon), lat.size, _ptr(out), _ptr(ok))
+    return st, out[: lat.size], ok[: lat.size]
 
 
 def slc_open(path):

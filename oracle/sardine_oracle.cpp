@@ -35,6 +35,8 @@
 //                              uniform synthetic scene, the three rejections)
 //   geoid grid files           geoid.rs:596-604 (wrong size) + round trips,
 //                              the WW15MGH header rules, malformed tokens
+//   SRTM tiles + mosaic        dem.rs:905-1026 style known values on synthetic
+//                              tiles, the name rules, void → missing, edges
 //   SLC TIFF reader            slc_reader.rs:618-820 (non-TIFF, empty, missing
 //                              file, SamplesPerPixel 2, line bounds, row count)
 //   striped GeoTIFF writers    export.rs:1276-1580 (geotransform errors, key
@@ -2238,6 +2240,97 @@ int oracle_geoid_load_ww15mgh(const char* path, float* grid) {
     if (st) return st;
     if (vals.size() != 73 * 144) return SARDINE_ERR_GEOID_MALFORMED_ASCII;
     std::memcpy(grid, vals.data(), vals.size() * sizeof(float));
+    return 0;
+}
+
+}  // extern "C"
+
+// ── SRTM 1" tiles and the mosaic lookup (dem.rs:129-178, 385-437, 694-739) ──
+namespace {
+
+struct OracleSrtm {
+    int lat_sw, lon_sw;
+    std::vector<float> data;  // 3601 x 3601, north row first, NaN voids
+};
+
+// bilinear_on_grid (dem.rs:385-409); false = a NaN tap (None)
+bool grid_bilinear(const std::vector<float>& d, size_t dim, double row_f, double col_f, size_t max_rc, float* out) {
+    const size_t r0 = std::min((size_t)row_f, max_rc >= 1 ? max_rc - 1 : 0);
+    const size_t c0 = std::min((size_t)col_f, dim >= 2 ? dim - 2 : 0);
+    const size_t r1 = r0 + 1, c1 = c0 + 1;
+    const double dr = row_f - (double)r0, dc = col_f - (double)c0;
+    const float v00 = d[r0 * dim + c0], v01 = d[r0 * dim + c1], v10 = d[r1 * dim + c0], v11 = d[r1 * dim + c1];
+    if (std::isnan(v00) || std::isnan(v01) || std::isnan(v10) || std::isnan(v11)) return false;
+    const double v = (double)v00 * (1.0 - dr) * (1.0 - dc) + (double)v01 * (1.0 - dr) * dc +
+                     (double)v10 * dr * (1.0 - dc) + (double)v11 * dr * dc;
+    *out = (float)v;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// SrtmTile::load for each path then DemMosaic::elevation_at_linear per
+// point: ok[i] = 1 with out[i] the height (NaN for a void), 0 = OutOfBounds.
+int oracle_srtm_sample(const char* const* paths, int32_t n, const double* lat, const double* lon, int64_t m,
+                       float* out, int32_t* ok) {
+    std::vector<OracleSrtm> tiles;
+    for (int32_t i = 0; i < n; ++i) {
+        std::string name = paths[i];
+        const size_t sl = name.find_last_of('/');
+        if (sl != std::string::npos) name = name.substr(sl + 1);
+        const size_t dot = name.find_last_of('.');
+        if (dot != std::string::npos && dot > 0) name = name.substr(0, dot);
+        if (name.size() < 7) return SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME;
+        auto field = [](const std::string& f, int* v) {
+            char* end = nullptr;
+            if (f.empty() || std::isspace((unsigned char)f[0])) return false;
+            const long x = std::strtol(f.c_str(), &end, 10);
+            if (*end || (f.size() == 1 && (f[0] == '+' || f[0] == '-'))) return false;
+            *v = (int)x;
+            return true;
+        };
+        int la, lo, ls, ms;
+        if (name[0] == 'N' || name[0] == 'n') ls = 1;
+        else if (name[0] == 'S' || name[0] == 's') ls = -1;
+        else return SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME;
+        if (!field(name.substr(1, 2), &la)) return SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME;
+        if (name[3] == 'E' || name[3] == 'e') ms = 1;
+        else if (name[3] == 'W' || name[3] == 'w') ms = -1;
+        else return SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME;
+        if (!field(name.substr(4, 3), &lo)) return SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME;
+        std::FILE* f = std::fopen(paths[i], "rb");
+        if (!f) return SARDINE_ERR_DEM_IO;
+        std::vector<unsigned char> raw;
+        int c;
+        while ((c = std::fgetc(f)) != EOF) raw.push_back((unsigned char)c);
+        std::fclose(f);
+        if (raw.size() != 3601u * 3601u * 2u) return SARDINE_ERR_DEM_BAD_SIZE;
+        OracleSrtm t{la * ls, lo * ms, std::vector<float>(3601u * 3601u)};
+        for (size_t k = 0; k < t.data.size(); ++k) {
+            const int16_t v = (int16_t)(uint16_t)(raw[2 * k] << 8 | raw[2 * k + 1]);
+            t.data[k] = v == -32768 ? std::numeric_limits<float>::quiet_NaN() : (float)v;
+        }
+        tiles.push_back(std::move(t));
+    }
+    if (tiles.empty()) return SARDINE_ERR_DEM_NO_TILES;
+    for (int64_t i = 0; i < m; ++i) {
+        const double lo = lon[i] >= 180.0 ? lon[i] - 360.0 : lon[i];
+        ok[i] = 0;
+        for (const OracleSrtm& t : tiles) {
+            const double fl = lat[i] - (double)t.lat_sw, fo = lo - (double)t.lon_sw;
+            if (fl < 0.0 || fl > 1.0 || fo < 0.0 || fo > 1.0) continue;
+            float v;
+            // a NaN tap makes the tile answer None and the scan go on
+            if (grid_bilinear(t.data, 3601, (1.0 - fl) * 3600.0, fo * 3600.0, 3600, &v)) {
+                out[i] = v;
+                ok[i] = 1;
+                break;
+            }
+        }
+        if (!ok[i]) out[i] = std::numeric_limits<float>::quiet_NaN();
+    }
     return 0;
 }
 

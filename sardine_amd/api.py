@@ -420,6 +420,18 @@ def load_geoid_grid(path, model: int = abi.GEOID_MODEL_EGM96, ascii: bool = Fals
     return g.reshape(73, 144)
 
 
+def load_srtm_mosaic(paths, device="cuda", stream=None) -> "DemRaster":
+    """DemMosaic::load_srtm_files (dem.rs:628-640) as one device DEM raster
+    for terrain_correction (NaN for voids and for cells no tile covers)."""
+    torch = _torch()
+    arr = (C.c_char_p * max(len(paths), 1))(*[str(p).encode() for p in paths])
+    r = abi.DemRaster()
+    _check(lib().sardine_dem_srtm_mosaic(arr, len(paths), None, C.byref(r), None))
+    data = torch.empty((r.height, r.width), dtype=torch.float32, device=device)
+    _check(lib().sardine_dem_srtm_mosaic(arr, len(paths), data.data_ptr(), C.byref(r), _stream_ptr(stream)))
+    return DemRaster(data, r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg)
+
+
 class SlcReader:
     """SlcReader (slc_reader.rs:466-590) reading rows straight into HBM."""
 

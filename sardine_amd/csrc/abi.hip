@@ -9,6 +9,7 @@
 // SARDINE_ERR_NO_DEVICE.
 #include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
@@ -64,6 +65,9 @@ extern "C" hipError_t sd_launch_multilook(const float* data, const float* nesz, 
 extern "C" hipError_t sd_launch_grd(const float* data, int64_t in_samples, const void* map, int64_t out_lines,
                                     int64_t out_samples, int a, float* out, hipStream_t s);
 extern "C" hipError_t sd_launch_swap16(uint16_t* d, int64_t n, hipStream_t s);
+extern "C" hipError_t sd_launch_srtm_tile(const uint16_t* be, int dim, float* dst, int64_t dst_w, int64_t row0,
+                                          int64_t col0, hipStream_t s);
+extern "C" hipError_t sd_launch_fill_nan(float* d, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -1670,6 +1674,128 @@ sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* 
     if (!t->little_endian) SD_HIP(sd_launch_swap16(reinterpret_cast<uint16_t*>(dst), (int64_t)(bytes / 2), s));
     // the chunks are reused by the next call only after these copies: the
     // next call waits on the events before refilling a set
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── SRTM 1" tiles → one device DEM raster (dem.rs:129-178, 414-437) ──────
+namespace {
+
+constexpr int kSrtmDim = 3601;
+
+// Path::file_stem: the file name without its last extension (a leading dot
+// is part of the name).
+std::string file_stem(const std::string& path) {
+    const size_t slash = path.find_last_of('/');
+    std::string name = slash == std::string::npos ? path : path.substr(slash + 1);
+    const size_t dot = name.find_last_of('.');
+    if (dot != std::string::npos && dot > 0) name = name.substr(0, dot);
+    return name;
+}
+
+// Rust's `i32::from_str` on a short field: an optional sign then digits.
+bool parse_i32_field(const std::string& s, int* out) {
+    size_t i = 0;
+    bool neg = false;
+    if (i < s.size() && (s[i] == '+' || s[i] == '-')) neg = s[i++] == '-';
+    if (i == s.size()) return false;
+    long v = 0;
+    for (; i < s.size(); ++i) {
+        if (!std::isdigit((unsigned char)s[i])) return false;
+        v = v * 10 + (s[i] - '0');
+    }
+    *out = (int)(neg ? -v : v);
+    return true;
+}
+
+// parse_srtm_tile_name (dem.rs:414-437): "N50E010" → (50, 10); characters
+// after the seventh are ignored.
+bool srtm_tile_name(const std::string& stem, int* lat, int* lon) {
+    if (stem.size() < 7) return false;
+    const char a = stem[0], b = stem[3];
+    int ls, ms, la, lo;
+    if (a == 'N' || a == 'n') ls = 1;
+    else if (a == 'S' || a == 's') ls = -1;
+    else return false;
+    if (!parse_i32_field(stem.substr(1, 2), &la)) return false;
+    if (b == 'E' || b == 'e') ms = 1;
+    else if (b == 'W' || b == 'w') ms = -1;
+    else return false;
+    if (!parse_i32_field(stem.substr(4, 3), &lo)) return false;
+    *lat = la * ls;
+    *lon = lo * ms;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths, float* dst,
+                                       sardine_dem_raster* out, void* stream) {
+    if (!out || (n_paths > 0 && !paths)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_paths <= 0) return fail(SARDINE_ERR_DEM_NO_TILES, "no DEM tiles");
+    // tile by tile in the given order, as load_srtm_files loads them: the
+    // name, then (device path) the file and its size
+    const size_t bytes = (size_t)kSrtmDim * kSrtmDim * 2;
+    std::vector<std::pair<int, int>> sw((size_t)n_paths);
+    for (int32_t i = 0; i < n_paths; ++i) {
+        const std::string stem = file_stem(paths[i] ? paths[i] : "");
+        if (!srtm_tile_name(stem, &sw[i].first, &sw[i].second))
+            return fail(SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME,
+                        "cannot determine tile origin for '%s': filename does not match SRTM (N50E010) naming",
+                        stem.c_str());
+        if (!dst) continue;
+        Fd f;
+        f.fd = ::open(paths[i], O_RDONLY | O_CLOEXEC);
+        struct stat stt;
+        if (f.fd < 0 || ::fstat(f.fd, &stt) != 0)
+            return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", paths[i], std::strerror(errno));
+        if ((size_t)stt.st_size != bytes)
+            return fail(SARDINE_ERR_DEM_BAD_SIZE, "tile %s has wrong size: expected %zu bytes, got %lld", paths[i],
+                        bytes, (long long)stt.st_size);
+    }
+    int lat_lo = INT32_MAX, lat_hi = INT32_MIN, lon_lo = INT32_MAX, lon_hi = INT32_MIN;
+    for (const auto& t : sw) {
+        lat_lo = std::min(lat_lo, t.first);
+        lat_hi = std::max(lat_hi, t.first + 1);
+        lon_lo = std::min(lon_lo, t.second);
+        lon_hi = std::max(lon_hi, t.second + 1);
+    }
+    const int64_t w = (int64_t)(lon_hi - lon_lo) * (kSrtmDim - 1) + 1;
+    const int64_t h = (int64_t)(lat_hi - lat_lo) * (kSrtmDim - 1) + 1;
+    if (w > INT32_MAX || h > INT32_MAX || w * h > (int64_t)1 << 40)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "tiles span %lld x %lld samples", (long long)w, (long long)h);
+    sardine_dem_raster r;
+    r.data = dst;
+    r.width = w;
+    r.height = h;
+    r.origin_lat_deg = (double)lat_hi;
+    r.origin_lon_deg = (double)lon_lo;
+    r.pixel_deg = 1.0 / 3600.0;
+    if (!dst) {  // layout query: names only (the files are read on the device path)
+        *out = r;
+        return SARDINE_OK;
+    }
+    sardine_status st;
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream;
+    SD_HIP(sd_launch_fill_nan(dst, w * h, s));
+    DevBuf tile;
+    SD_HIP(tile.alloc(bytes, s));
+    std::vector<uint8_t> host(bytes);
+    for (int32_t i = 0; i < n_paths; ++i) {
+        Fd f;
+        f.fd = ::open(paths[i], O_RDONLY | O_CLOEXEC);
+        if (f.fd < 0 || !pread_all(f.fd, host.data(), bytes, 0))
+            return fail(SARDINE_ERR_DEM_IO, "IO error reading %s", paths[i]);
+        SD_HIP(staging().copy(tile.p, host.data(), bytes, s));
+        const int64_t row0 = (int64_t)(lat_hi - (sw[i].first + 1)) * (kSrtmDim - 1);
+        const int64_t col0 = (int64_t)(sw[i].second - lon_lo) * (kSrtmDim - 1);
+        SD_HIP(sd_launch_srtm_tile(tile.as<uint16_t>(), kSrtmDim, dst, w, row0, col0, s));
+    }
+    *out = r;
     return SARDINE_OK;
 }
 

@@ -63,6 +63,23 @@ __global__ __launch_bounds__(256) void swap16_scalar_kernel(uint16_t* d, int64_t
     }
 }
 
+// One SRTM 1" tile (big-endian int16, -32768 = void) into its place in a
+// float DEM raster: row r, column c of the tile → (row0 + r, col0 + c).
+__global__ __launch_bounds__(256) void srtm_tile_kernel(const uint16_t* __restrict__ be, int dim, float* dst,
+                                                        int64_t dst_w, int64_t row0, int64_t col0) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= dim) return;
+    const uint16_t u = be[(size_t)r * dim + c];
+    const int16_t v = (int16_t)(uint16_t)((u >> 8) | (u << 8));
+    dst[(row0 + r) * dst_w + col0 + c] = v == -32768 ? __builtin_nanf("") : (float)v;
+}
+
+__global__ __launch_bounds__(256) void fill_nan_kernel(float* d, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = __builtin_nanf("");
+}
+
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     uint64_t z = x;
@@ -104,6 +121,21 @@ extern "C" hipError_t sd_launch_swap16(uint16_t* d, int64_t n, hipStream_t s) {
     if (blocks > 65536) blocks = 65536;
     if (aligned) hipLaunchKernelGGL(sd::swap16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n);
     else hipLaunchKernelGGL(sd::swap16_scalar_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sd_launch_srtm_tile(const uint16_t* be, int dim, float* dst, int64_t dst_w, int64_t row0,
+                                          int64_t col0, hipStream_t s) {
+    hipLaunchKernelGGL(sd::srtm_tile_kernel, dim3((unsigned)((dim + 255) / 256), (unsigned)dim), dim3(256), 0, s, be,
+                       dim, dst, dst_w, row0, col0);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sd_launch_fill_nan(float* d, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(sd::fill_nan_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n);
     return hipGetLastError();
 }
 

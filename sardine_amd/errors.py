@@ -9,6 +9,7 @@
   GroundRangeError        sardine/src/ground_range.rs:31-47
   SlcReadError            sardine/src/slc_reader.rs:66-131
   GeoidError              sardine/src/geoid.rs:87-104
+  DemError                sardine/src/dem.rs:54-109
 Each carries the C-ABI status code (`.status`) and its name (`.variant`).
 """
 from __future__ import annotations
@@ -59,6 +60,10 @@ class GeoidError(SardineError):
     pass
 
 
+class DemError(SardineError):
+    pass
+
+
 class DeviceError(SardineError):
     pass
 
@@ -84,4 +89,6 @@ def raise_for(status: int, message: str):
         raise SlcReadError(status, message)
     if 100 <= status < 110:
         raise GeoidError(status, message)
+    if 110 <= status < 120:
+        raise DemError(status, message)
     raise DeviceError(status, message)

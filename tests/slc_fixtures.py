@@ -74,3 +74,36 @@ def minimal_tiff(entries) -> bytes:
         out += struct.pack("<HHI", tag, typ, count) + field
     out += struct.pack("<I", 0) + b"\0"
     return bytes(out)
+
+
+def srtm_field(lat_sw: int, lon_sw: int, voids=()) -> np.ndarray:
+    """A 3601 x 3601 SRTM 1" tile of a smooth synthetic height field sampled on
+    its grid (row 0 = the north edge), so neighbouring tiles agree on their
+    shared edges; `voids` = (row, col) cells set to -32768."""
+    r = np.arange(3601)
+    lat = (lat_sw + 1) - r / 3600.0
+    lon = lon_sw + r / 3600.0
+    h = 800.0 + 600.0 * np.sin(np.radians(lat[:, None] * 37.0)) * np.cos(np.radians(lon[None, :] * 53.0))
+    t = np.round(h).astype(np.int16)
+    for rc in voids:
+        t[rc] = -32768
+    return t
+
+
+def write_hgt(path, tile: np.ndarray) -> None:
+    with open(path, "wb") as f:
+        f.write(tile.astype(">i2").tobytes())
+
+
+def mosaic(tiles: dict) -> tuple:
+    """{(lat_sw, lon_sw): tile} → (raster float32 with NaN voids/holes, north, west)."""
+    lats = [k[0] for k in tiles]
+    lons = [k[1] for k in tiles]
+    north, south, west, east = max(lats) + 1, min(lats), min(lons), max(lons) + 1
+    out = np.full(((north - south) * 3600 + 1, (east - west) * 3600 + 1), np.nan, np.float32)
+    for (la, lo), t in tiles.items():
+        f = t.astype(np.float32)
+        f[t == -32768] = np.nan
+        r0, c0 = (north - (la + 1)) * 3600, (lo - west) * 3600
+        out[r0:r0 + 3601, c0:c0 + 3601] = f
+    return out, north, west

@@ -535,3 +535,31 @@ def test_geoid_loaders_match_oracle(tmp_path, case):
     assert st == ref_st, (st, ref_st, _native.last_error())
     if st == 0:
         assert np.array_equal(out.view(np.uint32), ref.ravel().view(np.uint32))
+
+
+def test_srtm_mosaic_layout_and_errors(tmp_path):
+    L = _native.lib()
+
+    def q(names, dst=None):
+        arr = (C.c_char_p * max(len(names), 1))(*[str(tmp_path / n).encode() for n in names])
+        r = abi.DemRaster()
+        return L.sardine_dem_srtm_mosaic(arr, len(names), dst, C.byref(r), None), r
+
+    st, r = q(["N50E010.hgt", "N51E011.hgt"])
+    assert st == 0 and (r.width, r.height) == (7201, 7201)
+    assert (r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg) == (52.0, 10.0, 1.0 / 3600.0)
+    st, r = q(["s05w120.hgt", "S06W120.hgt", "s05W119_extra.hgt"])
+    assert st == 0 and (r.width, r.height, r.origin_lat_deg, r.origin_lon_deg) == (7201, 7201, -4.0, -120.0)
+    assert q([])[0] == abi.ERR_DEM_NO_TILES
+    for bad in ("X50E010.hgt", "N50E01.hgt", "N5xE010.hgt", "N50Q010.hgt"):
+        assert q([bad])[0] == abi.ERR_DEM_UNRECOGNISED_TILE_NAME, bad
+    # device path: files are checked in order (name, then the file) before any device work
+    import torch
+
+    buf = np.zeros(4, np.float32)
+    (tmp_path / "N50E010.hgt").write_bytes(b"\0" * 10)
+    assert q(["N50E010.hgt", "X.hgt"], buf.ctypes.data)[0] == abi.ERR_DEM_BAD_SIZE
+    assert q(["N51E010.hgt"], buf.ctypes.data)[0] == abi.ERR_DEM_IO
+    if not torch.cuda.is_available():
+        (tmp_path / "N52E010.hgt").write_bytes(b"\0" * (3601 * 3601 * 2))
+        assert q(["N52E010.hgt"], buf.ctypes.data)[0] == abi.ERR_NO_DEVICE

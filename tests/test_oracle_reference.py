@@ -1216,3 +1216,71 @@ def test_geoid_ww15mgh_loader(tmp_path):
     bad[5] = bad[5].replace(bad[5].split()[0], "0x10", 1)  # hex is not a Rust float
     p.write_text("\n".join(bad))
     assert O.geoid_load(p, ascii=True)[0] == abi.ERR_GEOID_MALFORMED_ASCII
+
+
+# ── SRTM tiles and the mosaic lookup (dem.rs:129-178, 385-437, 694-739) ─────
+@pytest.fixture(scope="module")
+def srtm_tiles(tmp_path_factory):
+    import slc_fixtures as F
+    d = tmp_path_factory.mktemp("srtm")
+    tiles = {}
+    for la, lo in ((50, 10), (50, 11), (51, 10), (51, 11)):
+        voids = [(1800, 1800), (1800, 1801), (100, 3000)] if (la, lo) == (50, 11) else []
+        tiles[(la, lo)] = F.srtm_field(la, lo, voids)
+        F.write_hgt(d / f"N{la:02d}E{lo:03d}.hgt", tiles[(la, lo)])
+    paths = sorted(str(p) for p in d.glob("*.hgt"))
+    return paths, tiles
+
+
+def test_srtm_tile_known_values(srtm_tiles):
+    paths, tiles = srtm_tiles
+    t = tiles[(51, 10)]
+    # node points return the node value; the tile's south-west corner is row 3600, col 0
+    lat = np.array([52.0, 51.0, 51.0 + 1800 / 3600.0, 51.5 - 0.5 / 3600.0])
+    lon = np.array([10.0, 10.0, 10.0 + 900 / 3600.0, 10.25])
+    st, h, ok = O.srtm_sample(paths, lat, lon)
+    assert st == 0 and ok.all()
+    assert h[0] == t[0, 0] and h[1] == t[3600, 0] and h[2] == t[1800, 900]
+    assert abs(h[3] - (t[1800, 900] + t[1801, 900]) / 2.0) <= 0.5 + 1e-3
+    # outside every tile (and past the half-pixel the single raster allows): out of bounds
+    st, h, ok = O.srtm_sample(paths, [49.9, 52.1, 50.5, 50.5], [10.5, 10.5, 9.99, 12.01])
+    assert st == 0 and not ok.any()
+    # a void tap makes the point missing (dem.rs:400-403, 167-178)
+    st, h, ok = O.srtm_sample(paths, [51.0 - 1800.5 / 3600.0], [11.0 + 1800.5 / 3600.0])
+    assert st == 0 and not ok[0]
+    # longitudes >= 180 are shifted by -360 once (dem.rs:697): 370.5 → 10.5
+    # (covered, the same height), 190.5 → -169.5 (not covered)
+    st, h2, ok2 = O.srtm_sample(paths, [50.5, 50.5, 50.5], [370.5, 10.5, 190.5])
+    assert st == 0 and list(ok2) == [1, 1, 0] and h2[0] == h2[1]
+
+
+def test_srtm_mosaic_matches_single_raster_rule(srtm_tiles):
+    """The joined raster sampled by the terrain correction's DEM rule gives the
+    per-tile lookup's heights away from the mosaic's outer half-pixel."""
+    import slc_fixtures as F
+    paths, tiles = srtm_tiles
+    ras, north, west = F.mosaic(tiles)
+    spec = abi.DemRaster(0, ras.shape[1], ras.shape[0], float(north), float(west), 1.0 / 3600.0)
+    rng = np.random.default_rng(9)
+    lat = rng.uniform(50.001, 51.999, 4000)
+    lon = rng.uniform(10.001, 11.999, 4000)
+    lat[:50] = 51.0 + rng.integers(-5, 5, 50) / 3600.0  # near the tile seams
+    lon[50:100] = 11.0 + rng.integers(-5, 5, 50) / 3600.0
+    lat[100], lon[100] = 51.0 - 1800.2 / 3600.0, 11.0 + 1800.7 / 3600.0  # on a void
+    st, ref, ok = O.srtm_sample(paths, lat, lon)
+    assert st == 0
+    single = np.array([np.nan if (v := O.dem_elevation(ras, spec, a, b)) is None else v for a, b in zip(lat, lon)],
+                      np.float32)
+    assert np.array_equal(np.isfinite(single), ok.astype(bool))
+    both = ok.astype(bool)
+    assert np.abs(single[both] - ref[both]).max() <= np.spacing(np.abs(ref[both])).max()
+
+
+def test_srtm_load_errors(srtm_tiles, tmp_path):
+    paths, tiles = srtm_tiles
+    assert O.srtm_sample([], [50.5], [10.5])[0] == abi.ERR_DEM_NO_TILES
+    assert O.srtm_sample([str(tmp_path / "X50E010.hgt")], [50.5], [10.5])[0] == abi.ERR_DEM_UNRECOGNISED_TILE_NAME
+    assert O.srtm_sample([str(tmp_path / "N50E01.hgt")], [50.5], [10.5])[0] == abi.ERR_DEM_UNRECOGNISED_TILE_NAME
+    assert O.srtm_sample([str(tmp_path / "N50E010.hgt")], [50.5], [10.5])[0] == abi.ERR_DEM_IO
+    (tmp_path / "s05w120.hgt").write_bytes(b"\0" * 10)
+    assert O.srtm_sample([str(tmp_path / "s05w120.hgt")], [50.5], [10.5])[0] == abi.ERR_DEM_BAD_SIZE
