@@ -94,6 +94,10 @@ typedef enum sardine_status {
     SARDINE_ERR_SLC_UNSUPPORTED_FORMAT = 84,
     SARDINE_ERR_SLC_NON_CONTIGUOUS_STRIPS = 85,
     SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS = 86,
+    SARDINE_ERR_SLC_CROSS_SLICE_BOUNDARY = 87,
+    /* MultiSlcOpenError (multi_slc_reader.rs:54-71) */
+    SARDINE_ERR_SLC_MULTI_EMPTY_INPUT = 88,
+    SARDINE_ERR_SLC_MULTI_WIDTH_MISMATCH = 89,
     /* GeoidError (geoid.rs:87-104) */
     SARDINE_ERR_GEOID_IO = 100,
     SARDINE_ERR_GEOID_WRONG_SIZE = 101,
@@ -512,6 +516,16 @@ sardine_status sardine_slc_open(const char* path, sardine_slc_tiff* tiff);
  * checked first (SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS). */
 sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* tiff, int64_t first_line,
                                       int64_t line_count, int16_t* dst /*[device]*/, void* stream);
+
+/* MultiSlcReader (multi_slc_reader.rs:75-193): one swath stored as
+ * n_slices consecutive measurement TIFFs (slice order), read in its logical
+ * line space (the slices' heights summed).  The read must lie inside one
+ * slice.  Errors: SARDINE_ERR_SLC_MULTI_EMPTY_INPUT, _MULTI_WIDTH_MISMATCH
+ * (construction), then _LINE_OUT_OF_BOUNDS, _CROSS_SLICE_BOUNDARY and the
+ * single-slice read's errors. */
+sardine_status sardine_slc_read_lines_multi(const char* const* paths, const sardine_slc_tiff* tiffs,
+                                            int32_t n_slices, int64_t first_line, int64_t line_count,
+                                            int16_t* dst /*[device]*/, void* stream);
 
 /* ── Pre-TC multilook ─────────────────────────────────────────────────────── */
 

@@ -88,6 +88,7 @@ def lib():
         L.oracle_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]
         L.oracle_srtm_sample.argtypes = [P, i32, P, P, i64, P, P]
         L.oracle_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), C.POINTER(i32)]
+        L.oracle_slc_multi_read.argtypes = [P, C.POINTER(abi.SlcTiff), i32, i64, i64, P]
         L.oracle_slc_read.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), i64, i64, P]
         L.oracle_cog_header.argtypes = [i64, i64, P, i32, i32, i32, P, C.c_uint64, C.POINTER(C.c_uint64)]
         _lib = L
@@ -481,6 +482,16 @@ def slc_read(path, tiff, first_line: int, line_count: int):
     out = np.empty((max(line_count, 0), tiff.width, 2), np.int16)
     st = lib().oracle_slc_read(str(path).encode(), C.byref(tiff), first_line, line_count,
                                _ptr(out) if out.size else None)
+    return st, out if st == 0 else None
+
+
+def slc_multi_read(paths, tiffs, first_line: int, line_count: int):
+    """MultiSlcReader (multi_slc_reader.rs:88-193) → (status, rows)."""
+    arr = (C.c_char_p * max(len(paths), 1))(*[str(p).encode() for p in paths])
+    ts = (abi.SlcTiff * max(len(tiffs), 1))(*tiffs)
+    w = tiffs[0].width if tiffs else 0
+    out = np.empty((max(line_count, 0), w, 2), np.int16)
+    st = lib().oracle_slc_multi_read(arr, ts, len(paths), first_line, line_count, _ptr(out) if out.size else None)
     return st, out if st == 0 else None
 
 

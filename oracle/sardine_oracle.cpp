@@ -2335,3 +2335,25 @@ int oracle_srtm_sample(const char* const* paths, int32_t n, const double* lat, c
 }
 
 }  // extern "C"
+
+// ── MultiSlcReader (multi_slc_reader.rs:88-193) ─────────────────────────────
+extern "C" int oracle_slc_multi_read(const char* const* paths, const sardine_slc_tiff* t, int32_t n, int64_t first,
+                                     int64_t count, int16_t* dst) {
+    if (n <= 0) return SARDINE_ERR_SLC_MULTI_EMPTY_INPUT;
+    for (int32_t i = 1; i < n; ++i)
+        if (t[i].width != t[0].width) return SARDINE_ERR_SLC_MULTI_WIDTH_MISMATCH;
+    std::vector<int64_t> offsets;
+    int64_t cum = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        offsets.push_back(cum);
+        cum += t[i].height;
+    }
+    if (first < 0 || count < 0 || first + count > cum) return SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS;
+    // partition_point(off <= first) - 1, saturating
+    int64_t k = 0;
+    while (k < n && offsets[(size_t)k] <= first) ++k;
+    k = k > 0 ? k - 1 : 0;
+    const int64_t end = k + 1 < n ? offsets[(size_t)k + 1] : cum;
+    if (first + count > end) return SARDINE_ERR_SLC_CROSS_SLICE_BOUNDARY;
+    return oracle_slc_read(paths[k], &t[k], first - offsets[(size_t)k], count, dst);
+}

@@ -5,7 +5,7 @@ The compute lives in libsardine_hip.so (hand-written HIP kernels); this
 package mirrors the reference's Rust interface for the path on top of it.
 """
 from . import abi
-from .api import (DemRaster, GeocodedImage, GroundRangeImage, MergedSigma0, ResamplingKernel, SceneInputs,
+from .api import (DemRaster, GeocodedImage, GroundRangeImage, MergedSigma0, MultiSlcReader, ResamplingKernel, SceneInputs,
                   Sigma0Array, SlcReader, SpeckleFilter, TerrainCorrectionConfig, apply_calibration, apply_multilook,
                   apply_speckle_filter, apply_speckle_step, cog_header, cog_layout, cog_pack, deburst_layout,
                   geotiff_header, load_geoid_grid, load_srtm_mosaic, merged_layout, output_grid, prepare_merged_scene, synth_cint16,
@@ -15,7 +15,7 @@ from .errors import (CalibrationError, DeburstError, DemError, DeviceError, Expo
                      SardineError, SlcReadError, SpeckleError, TerrainCorrectionError)
 
 __all__ = [
-    "abi", "DemRaster", "GeocodedImage", "GroundRangeImage", "MergedSigma0", "ResamplingKernel", "SceneInputs",
+    "abi", "DemRaster", "GeocodedImage", "GroundRangeImage", "MergedSigma0", "MultiSlcReader", "ResamplingKernel", "SceneInputs",
     "Sigma0Array", "SlcReader", "SpeckleFilter", "TerrainCorrectionConfig", "apply_calibration", "apply_multilook",
     "apply_speckle_filter", "apply_speckle_step", "cog_header", "cog_layout", "cog_pack", "deburst_layout",
     "geotiff_header", "load_geoid_grid", "load_srtm_mosaic", "merged_layout", "output_grid", "prepare_merged_scene", "synth_cint16", "terrain_correction",

@@ -69,6 +69,7 @@ def lib():
     L.sardine_geoid_load_binary.argtypes = [C.c_char_p, i32, P]
     L.sardine_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]
     L.sardine_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff)]
+    L.sardine_slc_read_lines_multi.argtypes = [P, C.POINTER(abi.SlcTiff), i32, i64, i64, P, P]
     L.sardine_slc_read_lines.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff), i64, i64, P, P]
     L.sardine_synth_cint16.argtypes = [P, i64, u64, u64, d, P]
     L.sardine_set_profiling.argtypes = [C.c_int]

@@ -54,6 +54,9 @@ ERR_SLC_MISSING_TAG = 83
 ERR_SLC_UNSUPPORTED_FORMAT = 84
 ERR_SLC_NON_CONTIGUOUS_STRIPS = 85
 ERR_SLC_LINE_OUT_OF_BOUNDS = 86
+ERR_SLC_CROSS_SLICE_BOUNDARY = 87
+ERR_SLC_MULTI_EMPTY_INPUT = 88
+ERR_SLC_MULTI_WIDTH_MISMATCH = 89
 ERR_GEOID_IO = 100
 ERR_GEOID_WRONG_SIZE = 101
 ERR_GEOID_WRONG_SIZE_EGM2008 = 102
@@ -275,6 +278,7 @@ EXPORTED_SYMBOLS = (
     "sardine_geoid_load_ww15mgh",
     "sardine_slc_open",
     "sardine_slc_read_lines",
+    "sardine_slc_read_lines_multi",
     "sardine_to_ground_range",
     "sardine_cog_layout_compute",
     "sardine_cog_header",

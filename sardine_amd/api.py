@@ -468,6 +468,40 @@ class SlcReader:
         return out
 
 
+class MultiSlcReader:
+    """MultiSlcReader (multi_slc_reader.rs:40-193): the slices of one swath
+    read in their logical line space, straight into HBM."""
+
+    def __init__(self, readers):
+        self.readers = list(readers)
+        paths = [r.path.encode() for r in self.readers]
+        self._paths = (C.c_char_p * max(len(paths), 1))(*paths)
+        self._tiffs = (abi.SlcTiff * max(len(self.readers), 1))(*[r.tiff for r in self.readers])
+
+    @classmethod
+    def from_readers(cls, readers) -> "MultiSlcReader":
+        m = cls(readers)
+        # construction errors are raised here, as the reference raises them
+        _check(lib().sardine_slc_read_lines_multi(m._paths, m._tiffs, len(m.readers), 0, 0, None, None))
+        return m
+
+    @property
+    def width(self) -> int:
+        return self.readers[0].width
+
+    @property
+    def height(self) -> int:
+        return sum(r.height for r in self.readers)
+
+    def read_burst_raw(self, first_line: int, line_count: int, out=None, device="cuda", stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty((line_count, self.width, 2), dtype=torch.int16, device=device)
+        _check(lib().sardine_slc_read_lines_multi(self._paths, self._tiffs, len(self.readers), first_line, line_count,
+                                                  out.data_ptr() if out.numel() else None, _stream_ptr(stream)))
+        return out
+
+
 @dataclass
 class GroundRangeImage:
     """GroundRangeImage (ground_range.rs:54-76) with a device raster."""

@@ -1677,6 +1677,35 @@ sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* 
     return SARDINE_OK;
 }
 
+sardine_status sardine_slc_read_lines_multi(const char* const* paths, const sardine_slc_tiff* tiffs,
+                                            int32_t n_slices, int64_t first_line, int64_t line_count, int16_t* dst,
+                                            void* stream) {
+    if (n_slices <= 0) return fail(SARDINE_ERR_SLC_MULTI_EMPTY_INPUT, "MultiSlcReader requires at least one slice reader");
+    if (!paths || !tiffs) return fail(SARDINE_ERR_INVALID_ARGUMENT, "paths or tiffs is null");
+    for (int32_t k = 1; k < n_slices; ++k)
+        if (tiffs[k].width != tiffs[0].width)
+            return fail(SARDINE_ERR_SLC_MULTI_WIDTH_MISMATCH, "slice %d width %u != slice 0 width %u", k, tiffs[k].width,
+                        tiffs[0].width);
+    // offset[k] = the heights of the slices before k
+    std::vector<int64_t> off((size_t)n_slices);
+    int64_t total = 0;
+    for (int32_t k = 0; k < n_slices; ++k) {
+        off[(size_t)k] = total;
+        total += tiffs[k].height;
+    }
+    if (first_line < 0 || line_count < 0 || first_line > total || line_count > total - first_line)
+        return fail(SARDINE_ERR_SLC_LINE_OUT_OF_BOUNDS, "line range [%lld, %lld) is out of image bounds [0, %lld)",
+                    (long long)first_line, (long long)(first_line + line_count), (long long)total);
+    // the owning slice: the last k with offset[k] <= first_line
+    const int32_t k = (int32_t)(std::upper_bound(off.begin(), off.end(), first_line) - off.begin()) - 1;
+    const int64_t slice_end = k + 1 < n_slices ? off[(size_t)k + 1] : total;
+    if (first_line + line_count > slice_end)
+        return fail(SARDINE_ERR_SLC_CROSS_SLICE_BOUNDARY,
+                    "burst read [%lld, %lld) crosses a slice boundary at logical line %lld",
+                    (long long)first_line, (long long)(first_line + line_count), (long long)slice_end);
+    return sardine_slc_read_lines(paths[k], &tiffs[k], first_line - off[(size_t)k], line_count, dst, stream);
+}
+
 }  // extern "C"
 
 // ── SRTM 1" tiles → one device DEM raster (dem.rs:129-178, 414-437) ──────

@@ -563,3 +563,30 @@ def test_srtm_mosaic_layout_and_errors(tmp_path):
     if not torch.cuda.is_available():
         (tmp_path / "N52E010.hgt").write_bytes(b"\0" * (3601 * 3601 * 2))
         assert q(["N52E010.hgt"], buf.ctypes.data)[0] == abi.ERR_NO_DEVICE
+
+
+def test_multi_slc_errors_precede_device_check(tmp_path):
+    import slc_fixtures as F
+    d = np.zeros((30, 11, 2), np.int16)
+    F.write_slc_tiff(tmp_path / "a.tiff", d[:18])
+    F.write_slc_tiff(tmp_path / "b.tiff", d[18:])
+    F.write_slc_tiff(tmp_path / "w.tiff", np.zeros((5, 12, 2), np.int16))
+    ra, rb, rw = (api.SlcReader.open(tmp_path / n) for n in ("a.tiff", "b.tiff", "w.tiff"))
+    m = api.MultiSlcReader.from_readers([ra, rb])
+    assert (m.width, m.height) == (11, 30)
+    from sardine_amd import SlcReadError
+    with pytest.raises(SlcReadError) as e:
+        api.MultiSlcReader.from_readers([ra, rw])
+    assert e.value.status == abi.ERR_SLC_MULTI_WIDTH_MISMATCH
+    L = _native.lib()
+    buf = np.zeros(30 * 11 * 2, np.int16)
+    ts = (abi.SlcTiff * 2)(ra.tiff, rb.tiff)
+    ps = (C.c_char_p * 2)(ra.path.encode(), rb.path.encode())
+    assert L.sardine_slc_read_lines_multi(ps, ts, 0, 0, 1, buf.ctypes.data, None) == abi.ERR_SLC_MULTI_EMPTY_INPUT
+    assert L.sardine_slc_read_lines_multi(ps, ts, 2, 15, 5, buf.ctypes.data, None) == \
+        abi.ERR_SLC_CROSS_SLICE_BOUNDARY
+    assert L.sardine_slc_read_lines_multi(ps, ts, 2, 25, 6, buf.ctypes.data, None) == abi.ERR_SLC_LINE_OUT_OF_BOUNDS
+    import torch
+
+    if not torch.cuda.is_available():
+        assert L.sardine_slc_read_lines_multi(ps, ts, 2, 20, 10, buf.ctypes.data, None) == abi.ERR_NO_DEVICE

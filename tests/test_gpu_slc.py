@@ -74,3 +74,15 @@ def test_slc_read_feeds_the_scene(cuda, tmp_path):
         parts = [r.read_burst_raw(b * sw.lines_per_burst, sw.lines_per_burst) for b in range(sw.n_bursts)]
         import torch
         assert np.array_equal(torch.cat(parts).cpu().numpy(), raw)
+
+
+def test_multi_slice_reads(cuda, tmp_path):
+    """MultiSlcReader (multi_slc_reader.rs:151-193): a swath split in three
+    slice files read per burst in its logical line space."""
+    d = _data(90, 77, 13)
+    for k, (a, b) in enumerate(((0, 30), (30, 60), (60, 90))):
+        F.write_slc_tiff(tmp_path / f"s{k}.tiff", d[a:b], le=(k != 1))
+    m = api.MultiSlcReader.from_readers([api.SlcReader.open(tmp_path / f"s{k}.tiff") for k in range(3)])
+    assert (m.width, m.height) == (77, 90)
+    for first, count in ((0, 30), (30, 30), (65, 25), (31, 7)):
+        assert np.array_equal(m.read_burst_raw(first, count).cpu().numpy(), d[first:first + count])

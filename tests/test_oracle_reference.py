@@ -1284,3 +1284,25 @@ def test_srtm_load_errors(srtm_tiles, tmp_path):
     assert O.srtm_sample([str(tmp_path / "N50E010.hgt")], [50.5], [10.5])[0] == abi.ERR_DEM_IO
     (tmp_path / "s05w120.hgt").write_bytes(b"\0" * 10)
     assert O.srtm_sample([str(tmp_path / "s05w120.hgt")], [50.5], [10.5])[0] == abi.ERR_DEM_BAD_SIZE
+
+
+def test_multi_slc_reader_reference_cases(tmp_path):
+    # multi_slc_reader.rs:217-315: one swath split in two slices
+    import slc_fixtures as F
+    d = _slc_data(30, 11, seed=8)
+    F.write_slc_tiff(tmp_path / "a.tiff", d[:18])
+    F.write_slc_tiff(tmp_path / "b.tiff", d[18:])
+    F.write_slc_tiff(tmp_path / "w.tiff", _slc_data(5, 12))
+    paths = [tmp_path / "a.tiff", tmp_path / "b.tiff"]
+    ts = [O.slc_open(p)[1] for p in paths]
+    assert O.slc_multi_read([], [], 0, 0)[0] == abi.ERR_SLC_MULTI_EMPTY_INPUT
+    tw = O.slc_open(tmp_path / "w.tiff")[1]
+    assert O.slc_multi_read([paths[0], tmp_path / "w.tiff"], [ts[0], tw], 0, 1)[0] == abi.ERR_SLC_MULTI_WIDTH_MISMATCH
+    st, rows = O.slc_multi_read(paths[:1], ts[:1], 0, 18)  # one slice: a pass-through
+    assert st == 0 and np.array_equal(rows, d[:18])
+    st, rows = O.slc_multi_read(paths, ts, 20, 10)  # second slice
+    assert st == 0 and np.array_equal(rows, d[20:30])
+    st, rows = O.slc_multi_read(paths, ts, 0, 18)
+    assert st == 0 and np.array_equal(rows, d[:18])
+    assert O.slc_multi_read(paths, ts, 15, 5)[0] == abi.ERR_SLC_CROSS_SLICE_BOUNDARY
+    assert O.slc_multi_read(paths, ts, 25, 6)[0] == abi.ERR_SLC_LINE_OUT_OF_BOUNDS
