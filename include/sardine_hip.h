@@ -106,6 +106,8 @@ typedef enum sardine_status {
     /* DemError (dem.rs:54-109), the variants the SRTM loader raises */
     SARDINE_ERR_DEM_IO = 110,
     SARDINE_ERR_DEM_BAD_SIZE = 111,
+    SARDINE_ERR_DEM_TIFF_DECODE = 112,
+    SARDINE_ERR_DEM_UNSUPPORTED_PIXEL_TYPE = 113,
     SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME = 114,
     SARDINE_ERR_DEM_NO_TILES = 115,
     /* library / device */
@@ -489,6 +491,19 @@ sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]);
  * _UNRECOGNISED_TILE_NAME, _IO, _BAD_SIZE in the reference's order. */
 sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths, float* dst /*[device]*/,
                                        sardine_dem_raster* out, void* stream);
+
+/* Glo30Tile::load (dem.rs:248-352): one Copernicus GLO-30 GeoTIFF tile
+ * (float32, int16 or uint16; any of the usual compressions and predictors)
+ * as a device DEM raster.  Georeferencing from ModelPixelScale +
+ * ModelTiepoint (the NW pixel centre; pixel size the mean of the two
+ * scales), else from the file name (Copernicus_DSM_COG_10_N50_00_E010_00_DEM
+ * or N50E010); GDAL_NODATA and non-finite floats become NaN.  The tile's
+ * sampling rule is the terrain correction's DEM rule, so the raster is the
+ * reference's tile exactly.  With dst NULL only `out` is filled from the
+ * header (host only).  Errors: SARDINE_ERR_DEM_IO, _TIFF_DECODE,
+ * _UNRECOGNISED_TILE_NAME, _UNSUPPORTED_PIXEL_TYPE in the reference's order. */
+sardine_status sardine_dem_glo30_tile(const char* path, float* dst /*[device]*/, sardine_dem_raster* out,
+                                      void* stream);
 
 /* ── SLC measurement TIFF → HBM ──────────────────────────────────────────── */
 

@@ -65,6 +65,7 @@ def lib():
                                           C.POINTER(abi.MergedInfo), P]
     L.sardine_to_ground_range.argtypes = [P, C.POINTER(abi.MergedInfo), C.POINTER(abi.GrdGridPoint), P, i32, d, d,
                                           P, C.POINTER(abi.GrdImage), P]
+    L.sardine_dem_glo30_tile.argtypes = [C.c_char_p, P, C.POINTER(abi.DemRaster), P]
     L.sardine_dem_srtm_mosaic.argtypes = [P, i32, P, C.POINTER(abi.DemRaster), P]
     L.sardine_geoid_load_binary.argtypes = [C.c_char_p, i32, P]
     L.sardine_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]

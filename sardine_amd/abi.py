@@ -63,6 +63,8 @@ ERR_GEOID_WRONG_SIZE_EGM2008 = 102
 ERR_GEOID_MALFORMED_ASCII = 103
 ERR_DEM_IO = 110
 ERR_DEM_BAD_SIZE = 111
+ERR_DEM_TIFF_DECODE = 112
+ERR_DEM_UNSUPPORTED_PIXEL_TYPE = 113
 ERR_DEM_UNRECOGNISED_TILE_NAME = 114
 ERR_DEM_NO_TILES = 115
 GEOID_MODEL_EGM96, GEOID_MODEL_EGM2008 = 0, 1
@@ -275,6 +277,7 @@ EXPORTED_SYMBOLS = (
     "sardine_apply_multilook",
     "sardine_geoid_load_binary",
     "sardine_dem_srtm_mosaic",
+    "sardine_dem_glo30_tile",
     "sardine_geoid_load_ww15mgh",
     "sardine_slc_open",
     "sardine_slc_read_lines",

@@ -432,6 +432,17 @@ def load_srtm_mosaic(paths, device="cuda", stream=None) -> "DemRaster":
     return DemRaster(data, r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg)
 
 
+def load_glo30_tile(path, device="cuda", stream=None) -> "DemRaster":
+    """Glo30Tile::load (dem.rs:248-352): one Copernicus GLO-30 GeoTIFF tile as
+    a device DEM raster (NaN for nodata and non-finite samples)."""
+    torch = _torch()
+    r = abi.DemRaster()
+    _check(lib().sardine_dem_glo30_tile(str(path).encode(), None, C.byref(r), None))
+    data = torch.empty((r.height, r.width), dtype=torch.float32, device=device)
+    _check(lib().sardine_dem_glo30_tile(str(path).encode(), data.data_ptr(), C.byref(r), _stream_ptr(stream)))
+    return DemRaster(data, r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg)
+
+
 class SlcReader:
     """SlcReader (slc_reader.rs:466-590) reading rows straight into HBM."""
 

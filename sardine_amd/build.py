@@ -18,7 +18,7 @@ import tempfile
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libsardine_hip.so"
-SOURCES = ["abi.hip", "tc.hip", "tc32.hip", "tc_batch.hip", "cal.hip", "speckle.hip", "misc.hip", "export.hip", "multilook.hip"]
+SOURCES = ["abi.hip", "tc.hip", "tc32.hip", "tc_batch.hip", "cal.hip", "speckle.hip", "misc.hip", "export.hip", "multilook.hip", "dem_io.hip"]
 
 
 def hipcc() -> str:
@@ -57,7 +57,7 @@ def build(verbose: bool = False) -> pathlib.Path:
         with ThreadPoolExecutor(workers) as ex:
             list(ex.map(lambda c: _run(c, verbose), cmds))
         tmp = LIB.with_suffix(".so.tmp")
-        _run([hipcc(), "--offload-arch=gfx950", "-shared", "-o", str(tmp), *map(str, objs)], verbose)
+        _run([hipcc(), "--offload-arch=gfx950", "-shared", "-o", str(tmp), *map(str, objs), "-lz"], verbose)
         os.replace(tmp, LIB)
     return LIB
 

@@ -32,6 +32,7 @@
 
 #include "../../include/sardine_hip.h"
 #include "cal_params.hpp"
+#include "dem_io.hpp"
 #include "geom.hpp"
 #include "tc_params.hpp"
 
@@ -1824,6 +1825,142 @@ sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths
         const int64_t col0 = (int64_t)(sw[i].second - lon_lo) * (kSrtmDim - 1);
         SD_HIP(sd_launch_srtm_tile(tile.as<uint16_t>(), kSrtmDim, dst, w, row0, col0, s));
     }
+    *out = r;
+    return SARDINE_OK;
+}
+
+}  // extern "C"
+
+// ── Copernicus GLO-30 GeoTIFF tile → device DEM raster (dem.rs:248-352) ──
+namespace {
+
+// parse_glo30_tile_name (dem.rs:445-500): an SRTM-style 7-character stem, or
+// the Copernicus name's N../S.. and E../W.. tokens each followed by "00".
+bool glo30_tile_name(const std::string& stem, int* lat, int* lon) {
+    if (stem.size() == 7 && srtm_tile_name(stem, lat, lon)) return true;
+    std::vector<std::string> parts;
+    size_t a = 0;
+    while (true) {
+        const size_t u = stem.find('_', a);
+        parts.push_back(stem.substr(a, u == std::string::npos ? std::string::npos : u - a));
+        if (u == std::string::npos) break;
+        a = u + 1;
+    }
+    bool have_lat = false, have_lon = false;
+    for (size_t i = 0; i < parts.size(); ++i) {
+        const std::string& p = parts[i];
+        const bool next00 = i + 1 < parts.size() && parts[i + 1] == "00";
+        int v;
+        if (!have_lat) {
+            if (p.size() >= 3 && (p[0] == 'N' || p[0] == 'S') && parse_i32_field(p.substr(1), &v) && next00) {
+                *lat = p[0] == 'N' ? v : -v;
+                have_lat = true;
+            }
+        } else if (!have_lon) {
+            if (p.size() >= 3 && (p[0] == 'E' || p[0] == 'W') && parse_i32_field(p.substr(1), &v) && next00) {
+                *lon = p[0] == 'E' ? v : -v;
+                have_lon = true;
+            }
+        }
+        if (have_lat && have_lon) break;
+    }
+    return have_lat && have_lon;
+}
+
+// Rust's saturating float → integer cast (NaN → 0).
+template <typename T>
+T sat_cast(float x) {
+    if (std::isnan(x)) return 0;
+    const double lo = (double)std::numeric_limits<T>::min(), hi = (double)std::numeric_limits<T>::max();
+    const double t = std::trunc((double)x);
+    return (T)(t < lo ? lo : t > hi ? hi : t);
+}
+
+}  // namespace
+
+extern "C" {
+
+sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_raster* out, void* stream) {
+    if (!path || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or out is null");
+    sd::GeoTiffRaster g;
+    std::string why;
+    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/false);
+    if (rc == 1) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", path, why.c_str());
+    if (rc == 2) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
+    double lat0, lon0, px;
+    if (g.pixel_scale.size() >= 2 && g.tiepoint.size() >= 6) {
+        const double sx = g.pixel_scale[0], sy = g.pixel_scale[1];
+        lon0 = g.tiepoint[3] + (0.5 - g.tiepoint[0]) * sx;
+        lat0 = g.tiepoint[4] - (0.5 - g.tiepoint[1]) * sy;
+        px = (sx + sy) / 2.0;
+    } else {
+        int la, lo;
+        const std::string stem = file_stem(path);
+        if (!glo30_tile_name(stem, &la, &lo))
+            return fail(SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME,
+                        "cannot determine tile origin for '%s': filename does not match SRTM (N50E010) or GLO-30 "
+                        "(Copernicus_DSM_COG_10_N50_00_E010_00_DEM) naming conventions", stem.c_str());
+        px = 1.0 / (double)g.width;
+        lat0 = (double)(la + 1) - 0.5 * px;
+        lon0 = (double)lo + 0.5 * px;
+    }
+    sardine_dem_raster r;
+    r.data = dst;
+    r.width = g.width;
+    r.height = g.height;
+    r.origin_lat_deg = lat0;
+    r.origin_lon_deg = lon0;
+    r.pixel_deg = px;
+    if (!dst) {
+        *out = r;
+        return SARDINE_OK;
+    }
+    const int rc2 = sd::read_geotiff(path, &g, &why, /*decode=*/true);
+    if (rc2 == 1) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", path, why.c_str());
+    if (rc2 == 2) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
+    if (g.sample == sd::TiffSample::Other)
+        return fail(SARDINE_ERR_DEM_UNSUPPORTED_PIXEL_TYPE,
+                    "GLO-30 tile %s has unsupported pixel type (expected f32 or i16)", path);
+    // GDAL_NODATA parsed as Rust parses an f32 (after trimming)
+    bool have_nd = false;
+    float nd = 0.0f;
+    if (g.has_nodata) {
+        size_t a = 0, b = g.nodata.size();
+        while (a < b && std::isspace((unsigned char)g.nodata[a])) ++a;
+        while (b > a && std::isspace((unsigned char)g.nodata[b - 1])) --b;
+        const std::string t = g.nodata.substr(a, b - a);
+        if (is_rust_float(t)) {
+            have_nd = true;
+            nd = std::strtof(t.c_str(), nullptr);
+        }
+    }
+    const size_t n = (size_t)(g.width * g.height);
+    std::vector<float> h(n);
+    const float qnan = std::numeric_limits<float>::quiet_NaN();
+    if (g.sample == sd::TiffSample::F32) {
+        for (size_t i = 0; i < n; ++i) {
+            float x;
+            std::memcpy(&x, &g.pixels[4 * i], 4);
+            h[i] = (!std::isfinite(x) || (have_nd && x == nd)) ? qnan : x;
+        }
+    } else if (g.sample == sd::TiffSample::I16) {
+        const int16_t ndv = sat_cast<int16_t>(nd);
+        for (size_t i = 0; i < n; ++i) {
+            int16_t x;
+            std::memcpy(&x, &g.pixels[2 * i], 2);
+            h[i] = (have_nd && x == ndv) ? qnan : (float)x;
+        }
+    } else {
+        const uint16_t ndv = sat_cast<uint16_t>(nd);
+        for (size_t i = 0; i < n; ++i) {
+            uint16_t x;
+            std::memcpy(&x, &g.pixels[2 * i], 2);
+            h[i] = (have_nd && x == ndv) ? qnan : (float)x;
+        }
+    }
+    sardine_status st;
+    if ((st = require_device())) return st;
+    SD_HIP(staging().copy(dst, h.data(), n * sizeof(float), (hipStream_t)stream));
     *out = r;
     return SARDINE_OK;
 }

@@ -1,0 +1,29 @@
+// dem_io.hpp — host-side reader of single-band GeoTIFF DEM tiles (the
+// Copernicus GLO-30 case of Glo30Tile::load, dem.rs:248-352).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace sd {
+
+enum class TiffSample { F32, I16, U16, Other };
+
+struct GeoTiffRaster {
+    int64_t width = 0, height = 0;
+    TiffSample sample = TiffSample::Other;
+    std::vector<uint8_t> pixels;  // width * height samples, host byte order, row-major
+    std::vector<double> pixel_scale, tiepoint;  // ModelPixelScale (33550), ModelTiepoint (33922); empty if absent
+    bool has_nodata = false;
+    std::string nodata;  // GDAL_NODATA (42113), as stored
+};
+
+// Decodes the first image of a TIFF file (with decode false: its size,
+// sample type and GeoTIFF tags only).  Returns 0, or 1 with *err set when
+// the file cannot be read, 2 when it is not a TIFF this reader decodes (the
+// message says why).
+int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool decode = true);
+
+}  // namespace sd

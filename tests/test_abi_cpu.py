@@ -590,3 +590,52 @@ def test_multi_slc_errors_precede_device_check(tmp_path):
 
     if not torch.cuda.is_available():
         assert L.sardine_slc_read_lines_multi(ps, ts, 2, 20, 10, buf.ctypes.data, None) == abi.ERR_NO_DEVICE
+
+
+def test_glo30_tile_georeferencing_and_errors(tmp_path):
+    """Glo30Tile::load's georeferencing (dem.rs:262-303) and error order, host
+    side (decode errors surface before the device check)."""
+    import slc_fixtures as F
+    L = _native.lib()
+    a = np.full((36, 36), 100.0, np.float32)
+
+    def head(name, **kw):
+        p = tmp_path / name
+        p.write_bytes(F.geotiff_dem_bytes(a, **kw))
+        r = abi.DemRaster()
+        return L.sardine_dem_glo30_tile(str(p).encode(), None, C.byref(r), None), r
+
+    # tiepoint at the NW corner, pixel-is-area: the first pixel centre is half a pixel inside
+    st, r = head("t.tif", scale=(1 / 3600, 1 / 3600, 0), tiepoint=(0, 0, 0, 10.0, 51.0, 0))
+    assert st == 0 and (r.width, r.height) == (36, 36)
+    assert r.origin_lon_deg == 10.0 + 0.5 * (1 / 3600) and r.origin_lat_deg == 51.0 - 0.5 * (1 / 3600)
+    assert r.pixel_deg == 1 / 3600
+    # a tiepoint at a pixel centre (I = J = 0.5) and unequal scales: their mean
+    st, r = head("t2.tif", scale=(2e-4, 4e-4, 0), tiepoint=(0.5, 0.5, 0, 10.0, 51.0, 0))
+    assert st == 0 and (r.origin_lon_deg, r.origin_lat_deg) == (10.0, 51.0) and r.pixel_deg == (2e-4 + 4e-4) / 2
+    # without GeoTIFF tags: the origin from the name, the pixel from the width
+    for name in ("Copernicus_DSM_COG_10_N50_00_E010_00_DEM.tif", "N50E010.tif", "s05_00_w120_00.tif"):
+        st, r = head(name)
+        la, lo = (50, 10) if "N50" in name else (-5, -120)
+        if name.startswith("s05"):
+            assert st == abi.ERR_DEM_UNRECOGNISED_TILE_NAME  # lower-case n/s are not Copernicus tokens
+            continue
+        assert st == 0 and r.pixel_deg == 1 / 36
+        assert r.origin_lat_deg == (la + 1) - 0.5 / 36 and r.origin_lon_deg == lo + 0.5 / 36
+    assert head("Copernicus_DSM_COG_10_N50_01_E010_00_DEM.tif")[0] == abi.ERR_DEM_UNRECOGNISED_TILE_NAME
+    r = abi.DemRaster()
+    assert L.sardine_dem_glo30_tile(str(tmp_path / "missing.tif").encode(), None, C.byref(r), None) == abi.ERR_DEM_IO
+    (tmp_path / "junk.tif").write_bytes(b"not a tiff at all")
+    assert L.sardine_dem_glo30_tile(str(tmp_path / "junk.tif").encode(), None, C.byref(r), None) == \
+        abi.ERR_DEM_TIFF_DECODE
+    buf = np.zeros(4, np.float32)
+    # decode errors and pixel types come before the device check
+    p = tmp_path / "u8.tif"
+    p.write_bytes(F.geotiff_dem_bytes(np.zeros((8, 8), np.uint8), scale=(1, 1, 0), tiepoint=(0, 0, 0, 0, 0, 0)))
+    assert L.sardine_dem_glo30_tile(str(p).encode(), buf.ctypes.data, C.byref(r), None) == \
+        abi.ERR_DEM_UNSUPPORTED_PIXEL_TYPE
+    b = bytearray(F.geotiff_dem_bytes(a, compression="deflate", scale=(1, 1, 0), tiepoint=(0, 0, 0, 0, 0, 0)))
+    b[-40:] = b"\xff" * 40  # a corrupt deflate stream
+    (tmp_path / "bad.tif").write_bytes(bytes(b))
+    assert L.sardine_dem_glo30_tile(str(tmp_path / "bad.tif").encode(), buf.ctypes.data, C.byref(r), None) == \
+        abi.ERR_DEM_TIFF_DECODE

@@ -36,3 +36,53 @@ def test_srtm_mosaic_raster(cuda, tmp_path, layout):
     single = np.array([np.nan if (v := O.dem_elevation(got, spec, a, b)) is None else v for a, b in zip(lat, lon)],
                       np.float32)
     assert st == 0 and np.array_equal(np.isfinite(single), ok.astype(bool))
+
+
+def _expect(a, nodata):
+    """Glo30Tile::load's sample rules (dem.rs:317-342): non-finite floats and
+    the nodata value (cast to the sample type, saturating) become NaN."""
+    f = a.astype(np.float32)
+    if a.dtype == np.float32:
+        bad = ~np.isfinite(a)
+        if nodata is not None:
+            bad |= a == np.float32(nodata)
+    else:
+        bad = np.zeros(a.shape, bool)
+        if nodata is not None:
+            nd = float(nodata)
+            info = np.iinfo(a.dtype)
+            cast = 0 if np.isnan(nd) else int(np.clip(np.trunc(nd), info.min, info.max))
+            bad = a == cast
+    f[bad] = np.nan
+    return f
+
+
+CASES = [(dt, comp, pred, tile, le, big)
+         for dt in ("f4", "i2", "u2")
+         for comp, pred in (("none", 1), ("deflate", 1), ("deflate", 2), ("lzw", 2), ("packbits", 1), ("deflate", 3),
+                            ("lzw", 3))
+         if not (pred == 3 and dt != "f4")
+         for tile, le, big in ((None, True, False), ((64, 32), True, True), ((32, 32), False, False))]
+
+
+@pytest.mark.parametrize("dt,comp,pred,tile,le,big", CASES)
+def test_glo30_tile_decode(cuda, tmp_path, dt, comp, pred, tile, le, big):
+    rng = np.random.default_rng(hash((dt, comp, pred)) % 1000)
+    shape = (131, 97)
+    if dt == "f4":
+        a = rng.normal(800, 300, shape).astype(np.float32)
+        a[3, 4], a[5, 6], a[7, 8] = np.nan, np.inf, -32767.0
+        nodata = "-32767"
+    else:
+        a = rng.integers(-400 if dt == "i2" else 0, 4000, shape).astype(dt)
+        a[10, 10] = -32768 if dt == "i2" else 65535
+        nodata = " -32768.7 " if dt == "i2" else "1e9"  # trimmed; saturating casts: -32768 and 65535
+    p = tmp_path / "N46E007.tif"
+    p.write_bytes(F.geotiff_dem_bytes(a, tile=tile, compression=comp, predictor=pred, le=le, big=big,
+                                      scale=(1 / 3600, 1 / 3600, 0), tiepoint=(0, 0, 0, 7.0, 47.0, 0), nodata=nodata,
+                                      rows_per_strip=9))
+    dem = api.load_glo30_tile(p, device=cuda)
+    got = dem.data.cpu().numpy()
+    want = _expect(a, nodata)
+    assert np.array_equal(got, want, equal_nan=True)
+    assert (dem.origin_lat_deg, dem.origin_lon_deg) == (47.0 - 0.5 / 3600, 7.0 + 0.5 / 3600)
