@@ -144,14 +144,7 @@ def main():
         return bench_slc(args, dev)
     L.sardine_set_tc_segment(args.tc_segment)
     if args.copy_calibration:
-        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
-        a.fill_(1.0)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 1 GiB read + 1 GiB written by one copy kernel")
-        del a, b
+        copy_calibration(dev)
     stream = torch.cuda.current_stream(dev)
 
     t0 = time.time()
@@ -346,6 +339,22 @@ def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
     log("cpu baseline:", sample)
     return {"value": round(value, 4), "unit": "Mpx/s", "cores": cores, "kind": "port", "sample": sample,
             "seconds_measured": round(dt + dt_tc + dt_cal + (dt_sp if filt is not None else 0.0), 2)}
+
+
+def copy_calibration(dev) -> None:
+    """One plain copy kernel of known size (1 GiB read, 1 GiB written), whose
+    FETCH_SIZE / WRITE_SIZE counter readings scale the counters of the
+    measured kernels to bytes."""
+    import torch
+
+    a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 1 GiB read + 1 GiB written by one copy kernel")
+    del a, b
 
 
 def bench_batch64(args, dev):
@@ -726,14 +735,7 @@ def bench_cog(args, dev):
     L = _native.lib()
     stream = torch.cuda.current_stream(dev)
     if args.copy_calibration:
-        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
-        a.fill_(1.0)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 1 GiB read + 1 GiB written by one copy kernel")
-        del a, b
+        copy_calibration(dev)
     t0 = time.time()
     sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
     ds = DeviceScene.build(sc, dev)
@@ -867,6 +869,8 @@ def bench_multilook(args, dev):
     from sardine_amd.workload import DeviceScene
 
     L = _native.lib()
+    if args.copy_calibration:
+        copy_calibration(dev)
     stream = torch.cuda.current_stream(dev)
     sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
     ds = DeviceScene.build(sc, dev)
@@ -927,6 +931,7 @@ def bench_multilook(args, dev):
         log(f"cpu: {json.dumps(cpu)}")
     main = res["4x1"]
     px = lines * samples
+    traffic, traffic_src = measured_traffic("multilook_kernel", "multilook_iw_full_13100x60814")
     line = {
         "metric": "pre-TC multilook input Mpx/s (sigma0 + NESZ, 4x1 looks, inputs in HBM)",
         "value": round(px / (main["kernel_ms"] / 1e3) / 1e6, 1), "unit": "Mpx/s",
@@ -934,7 +939,8 @@ def bench_multilook(args, dev):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "multilook_iw_full_13100x60814", "merged": [lines, samples], "looks": "4x1"},
         "roofline": {"kernel": "multilook_kernel<4>", "bound": "hbm", "achieved": main["GBps"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": main["hbm_frac"], "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": main["hbm_frac"], "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes": main["algorithmic_bytes"], "launch_ms": main["kernel_ms"]},
         "shapes": res,
         "cpu_baseline": cpu,
@@ -956,6 +962,8 @@ def bench_grd(args, dev):
     from sardine_amd.workload import DeviceScene
 
     L = _native.lib()
+    if args.copy_calibration:
+        copy_calibration(dev)
     stream = torch.cuda.current_stream(dev)
     sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
     ds = DeviceScene.build(sc, dev)
@@ -1012,6 +1020,7 @@ def bench_grd(args, dev):
                "seconds": round(cs, 3), "device_result_bit_exact": same}
         log(f"cpu: {json.dumps(cpu)}")
     main = res["az14.08"]
+    traffic, traffic_src = measured_traffic("grd_kernel", "grd_iw_full_13100x60814_10m")
     line = {
         "metric": "ground-range projection input Mpx/s (10 m, 1 azimuth look, inputs in HBM)",
         "value": round(lines * samples / (main["kernel_ms"] / 1e3) / 1e6, 1), "unit": "Mpx/s",
@@ -1019,7 +1028,7 @@ def bench_grd(args, dev):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "grd_iw_full_13100x60814_10m", "merged": [lines, samples], "out": main["out"]},
         "roofline": {"kernel": "grd_kernel", "bound": "hbm", "achieved": main["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": main["hbm_frac"], "traffic": None,
+                     "unit": "GB/s", "frac": main["hbm_frac"], "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes": main["algorithmic_bytes"], "launch_ms": main["kernel_ms"]},
         "cases": res,
         "cpu_baseline": cpu,
