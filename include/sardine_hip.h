@@ -710,7 +710,8 @@ typedef enum sardine_kernel_family {
     SARDINE_KERNEL_TC_SEGMENT = 4, /* the per-pixel chain kernel of terrain correction alone */
     SARDINE_KERNEL_COG_PACK = 5,   /* the COG export's pyramid + tile pack (all its launches) */
     SARDINE_KERNEL_MULTILOOK = 6,
-    SARDINE_KERNEL_GRD = 7
+    SARDINE_KERNEL_GRD = 7,
+    SARDINE_KERNEL_DEM = 8 /* a DEM loader's device decode (one entry per SRTM tile; the GLO-30 sample rule) */
 } sardine_kernel_family;
 /* Switches the kernel timers of the calling thread on (creating their
  * events) or off. */

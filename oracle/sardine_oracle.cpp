@@ -2302,9 +2302,9 @@ int oracle_srtm_sample(const char* const* paths, int32_t n, const double* lat, c
         if (!field(name.substr(4, 3), &lo)) return SARDINE_ERR_DEM_UNRECOGNISED_TILE_NAME;
         std::FILE* f = std::fopen(paths[i], "rb");
         if (!f) return SARDINE_ERR_DEM_IO;
-        std::vector<unsigned char> raw;
-        int c;
-        while ((c = std::fgetc(f)) != EOF) raw.push_back((unsigned char)c);
+        // the whole file (std::fs::read), at most one byte past the expected size
+        std::vector<unsigned char> raw(3601u * 3601u * 2u + 1u);
+        raw.resize(std::fread(raw.data(), 1, raw.size(), f));
         std::fclose(f);
         if (raw.size() != 3601u * 3601u * 2u) return SARDINE_ERR_DEM_BAD_SIZE;
         OracleSrtm t{la * ls, lo * ms, std::vector<float>(3601u * 3601u)};

@@ -312,4 +312,4 @@ EXPORTED_SYMBOLS = (
     "sardine_server_destroy",
 )
 KERNEL_TC, KERNEL_CALIBRATE, KERNEL_SPECKLE, KERNEL_TO_DB, KERNEL_TC_SEGMENT, KERNEL_COG_PACK = 0, 1, 2, 3, 4, 5
-KERNEL_MULTILOOK, KERNEL_GRD = 6, 7
+KERNEL_MULTILOOK, KERNEL_GRD, KERNEL_DEM = 6, 7, 8

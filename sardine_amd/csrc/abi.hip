@@ -69,6 +69,8 @@ extern "C" hipError_t sd_launch_swap16(uint16_t* d, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_srtm_tile(const uint16_t* be, int dim, float* dst, int64_t dst_w, int64_t row0,
                                           int64_t col0, hipStream_t s);
 extern "C" hipError_t sd_launch_fill_nan(float* d, int64_t n, hipStream_t s);
+extern "C" hipError_t sd_launch_dem_values(const void* raw, int kind, int have_nd, float nd_f, int32_t nd_i,
+                                           float* dst, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -78,7 +80,7 @@ thread_local std::string g_last_error;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
-constexpr int kFamilies = 8;
+constexpr int kFamilies = 9;
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
 // waits here: sardine_last_kernel_ms waits for the end event when read.
@@ -1581,18 +1583,83 @@ sardine_status slc_parse(int fd, const char* path, sardine_slc_tiff* out) {
 // Pinned chunks for file → HBM reads, kept for the process: two sets of
 // kSlcSet chunks; a set is filled by parallel preads while the other one's
 // copies run.  One reader at a time.
-constexpr int kSlcSet = 4;
-constexpr size_t kSlcChunk = 16u << 20;
+constexpr int kSlcSet = 8;
+constexpr size_t kSlcChunk = 8u << 20;
 struct SlcPinned {
     std::mutex mu;
     int dev = -1;
     void* buf[2][kSlcSet] = {};
     hipEvent_t done[2] = {nullptr, nullptr};
     bool used[2] = {false, false};
+    int next = 0;  // the set the next read fills first (alternates across calls)
 };
 SlcPinned& slc_pinned() {
     static SlcPinned* p = new SlcPinned();  // never destroyed: pinned memory outlives static teardown
     return *p;
+}
+
+// Bytes [base, base + bytes) of an open file into device memory at `out`,
+// in stream order on s.  A short read fails with io_err and io_msg.  The
+// chunks are reused by the next call only after these copies: a call waits
+// on a set's event before refilling it.
+sardine_status file_to_device(int fd, uint64_t base, uint64_t bytes, uint8_t* out, hipStream_t s,
+                              sardine_status io_err, const std::string& io_msg) {
+    if (bytes == 0) return SARDINE_OK;
+    SlcPinned& P = slc_pinned();
+    std::lock_guard<std::mutex> g(P.mu);
+    int dev = 0;
+    SD_HIP(hipGetDevice(&dev));
+    if (P.dev != dev) {  // (re)create the chunks on this device's context
+        for (int k = 0; k < 2; ++k) {
+            if (P.done[k]) {
+                SD_HIP(hipEventSynchronize(P.done[k]));
+                (void)hipEventDestroy(P.done[k]);
+                P.done[k] = nullptr;
+            }
+            for (int i = 0; i < kSlcSet; ++i)
+                if (P.buf[k][i]) {
+                    (void)hipHostFree(P.buf[k][i]);
+                    P.buf[k][i] = nullptr;
+                }
+            P.used[k] = false;
+        }
+        P.next = 0;
+        for (int k = 0; k < 2; ++k) {
+            for (int i = 0; i < kSlcSet; ++i) SD_HIP(hipHostMalloc(&P.buf[k][i], kSlcChunk, hipHostMallocDefault));
+            SD_HIP(hipEventCreateWithFlags(&P.done[k], hipEventDisableTiming));
+        }
+        P.dev = dev;
+    }
+    const uint64_t per_set = (uint64_t)kSlcSet * kSlcChunk;
+    // start on the set the previous read did not end on, so back-to-back
+    // reads (tiles, bursts) overlap one read's copies with the next's preads
+    int set = P.next;
+    for (uint64_t done = 0; done < bytes; done += per_set, set ^= 1) {
+        if (P.used[set]) SD_HIP(hipEventSynchronize(P.done[set]));  // its previous copies have finished
+        const uint64_t n_set = std::min(per_set, bytes - done);
+        // the set's bytes spread over all its chunks (4 KiB multiples), so a
+        // short read still runs kSlcSet preads in parallel
+        const uint64_t chunk = std::min<uint64_t>(kSlcChunk, ((n_set + kSlcSet - 1) / kSlcSet + 4095) & ~(uint64_t)4095);
+        const int n_chunks = (int)((n_set + chunk - 1) / chunk);
+        bool ok[kSlcSet] = {};
+        std::vector<std::thread> workers;
+        for (int i = 1; i < n_chunks; ++i) {
+            const uint64_t off = (uint64_t)i * chunk, n = std::min<uint64_t>(chunk, n_set - off);
+            workers.emplace_back([&, i, off, n] { ok[i] = pread_all(fd, P.buf[set][i], (size_t)n, base + done + off); });
+        }
+        ok[0] = pread_all(fd, P.buf[set][0], (size_t)std::min<uint64_t>(chunk, n_set), base + done);
+        for (auto& w : workers) w.join();
+        for (int i = 0; i < n_chunks; ++i)
+            if (!ok[i]) return fail(io_err, "%s", io_msg.c_str());
+        for (int i = 0; i < n_chunks; ++i) {
+            const uint64_t off = (uint64_t)i * chunk, n = std::min<uint64_t>(chunk, n_set - off);
+            SD_HIP(hipMemcpyAsync(out + done + off, P.buf[set][i], (size_t)n, hipMemcpyHostToDevice, s));
+        }
+        SD_HIP(hipEventRecord(P.done[set], s));
+        P.used[set] = true;
+        P.next = set ^ 1;
+    }
+    return SARDINE_OK;
 }
 
 }  // namespace
@@ -1624,57 +1691,11 @@ sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* 
     f.fd = ::open(path, O_RDONLY | O_CLOEXEC);
     if (f.fd < 0) return fail(SARDINE_ERR_SLC_IO, "I/O error: %s: %s", path, std::strerror(errno));
     hipStream_t s = (hipStream_t)stream_v;
-    SlcPinned& P = slc_pinned();
-    std::lock_guard<std::mutex> g(P.mu);
-    int dev = 0;
-    SD_HIP(hipGetDevice(&dev));
-    if (P.dev != dev) {  // (re)create the chunks on this device's context
-        for (int k = 0; k < 2; ++k) {
-            if (P.done[k]) {
-                SD_HIP(hipEventSynchronize(P.done[k]));
-                (void)hipEventDestroy(P.done[k]);
-                P.done[k] = nullptr;
-            }
-            for (int i = 0; i < kSlcSet; ++i)
-                if (P.buf[k][i]) {
-                    (void)hipHostFree(P.buf[k][i]);
-                    P.buf[k][i] = nullptr;
-                }
-            P.used[k] = false;
-        }
-        for (int k = 0; k < 2; ++k) {
-            for (int i = 0; i < kSlcSet; ++i) SD_HIP(hipHostMalloc(&P.buf[k][i], kSlcChunk, hipHostMallocDefault));
-            SD_HIP(hipEventCreateWithFlags(&P.done[k], hipEventDisableTiming));
-        }
-        P.dev = dev;
-    }
     const uint64_t base = t->base_offset + (uint64_t)first_line * t->row_bytes;
-    uint8_t* out = reinterpret_cast<uint8_t*>(dst);
-    const uint64_t per_set = (uint64_t)kSlcSet * kSlcChunk;
-    int set = 0;
-    for (uint64_t done = 0; done < bytes; done += per_set, set ^= 1) {
-        if (P.used[set]) SD_HIP(hipEventSynchronize(P.done[set]));  // its previous copies have finished
-        const uint64_t n_set = std::min(per_set, bytes - done);
-        const int n_chunks = (int)((n_set + kSlcChunk - 1) / kSlcChunk);
-        bool ok[kSlcSet] = {};
-        std::vector<std::thread> workers;
-        for (int i = 0; i < n_chunks; ++i) {
-            const uint64_t off = (uint64_t)i * kSlcChunk, n = std::min<uint64_t>(kSlcChunk, n_set - off);
-            workers.emplace_back([&, i, off, n] { ok[i] = pread_all(f.fd, P.buf[set][i], (size_t)n, base + done + off); });
-        }
-        for (auto& w : workers) w.join();
-        for (int i = 0; i < n_chunks; ++i)
-            if (!ok[i]) return fail(SARDINE_ERR_SLC_IO, "I/O error reading %s (file shorter than its IFD says?)", path);
-        for (int i = 0; i < n_chunks; ++i) {
-            const uint64_t off = (uint64_t)i * kSlcChunk, n = std::min<uint64_t>(kSlcChunk, n_set - off);
-            SD_HIP(hipMemcpyAsync(out + done + off, P.buf[set][i], (size_t)n, hipMemcpyHostToDevice, s));
-        }
-        SD_HIP(hipEventRecord(P.done[set], s));
-        P.used[set] = true;
-    }
+    if ((st = file_to_device(f.fd, base, bytes, reinterpret_cast<uint8_t*>(dst), s, SARDINE_ERR_SLC_IO,
+                             std::string("I/O error reading ") + path + " (file shorter than its IFD says?)")))
+        return st;
     if (!t->little_endian) SD_HIP(sd_launch_swap16(reinterpret_cast<uint16_t*>(dst), (int64_t)(bytes / 2), s));
-    // the chunks are reused by the next call only after these copies: the
-    // next call waits on the events before refilling a set
     return SARDINE_OK;
 }
 
@@ -1812,18 +1833,23 @@ sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths
     if ((st = require_device())) return st;
     hipStream_t s = (hipStream_t)stream;
     SD_HIP(sd_launch_fill_nan(dst, w * h, s));
+    // each tile's bytes go file → pinned chunks → HBM (the next tile's reads
+    // overlap this tile's copies and decode), then the big-endian decode
+    // places it in the mosaic; the tile buffer is reused in stream order
     DevBuf tile;
     SD_HIP(tile.alloc(bytes, s));
-    std::vector<uint8_t> host(bytes);
     for (int32_t i = 0; i < n_paths; ++i) {
         Fd f;
         f.fd = ::open(paths[i], O_RDONLY | O_CLOEXEC);
-        if (f.fd < 0 || !pread_all(f.fd, host.data(), bytes, 0))
-            return fail(SARDINE_ERR_DEM_IO, "IO error reading %s", paths[i]);
-        SD_HIP(staging().copy(tile.p, host.data(), bytes, s));
+        if (f.fd < 0) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s", paths[i]);
+        if ((st = file_to_device(f.fd, 0, bytes, tile.as<uint8_t>(), s, SARDINE_ERR_DEM_IO,
+                                 std::string("IO error reading ") + paths[i])))
+            return st;
         const int64_t row0 = (int64_t)(lat_hi - (sw[i].first + 1)) * (kSrtmDim - 1);
         const int64_t col0 = (int64_t)(sw[i].second - lon_lo) * (kSrtmDim - 1);
+        KernelTimer kt(SARDINE_KERNEL_DEM, s);
         SD_HIP(sd_launch_srtm_tile(tile.as<uint16_t>(), kSrtmDim, dst, w, row0, col0, s));
+        kt.stop();
     }
     *out = r;
     return SARDINE_OK;
@@ -1884,7 +1910,9 @@ sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_
     if (!path || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or out is null");
     sd::GeoTiffRaster g;
     std::string why;
-    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/false);
+    // one read of the file; decode errors (rc 3) are reported after the
+    // georeferencing, as the reference decodes the image after it
+    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/dst != nullptr);
     if (rc == 1) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", path, why.c_str());
     if (rc == 2) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
     double lat0, lon0, px;
@@ -1915,9 +1943,7 @@ sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_
         *out = r;
         return SARDINE_OK;
     }
-    const int rc2 = sd::read_geotiff(path, &g, &why, /*decode=*/true);
-    if (rc2 == 1) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", path, why.c_str());
-    if (rc2 == 2) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
+    if (rc == 3) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
     if (g.sample == sd::TiffSample::Other)
         return fail(SARDINE_ERR_DEM_UNSUPPORTED_PIXEL_TYPE,
                     "GLO-30 tile %s has unsupported pixel type (expected f32 or i16)", path);
@@ -1934,33 +1960,20 @@ sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_
             nd = std::strtof(t.c_str(), nullptr);
         }
     }
-    const size_t n = (size_t)(g.width * g.height);
-    std::vector<float> h(n);
-    const float qnan = std::numeric_limits<float>::quiet_NaN();
-    if (g.sample == sd::TiffSample::F32) {
-        for (size_t i = 0; i < n; ++i) {
-            float x;
-            std::memcpy(&x, &g.pixels[4 * i], 4);
-            h[i] = (!std::isfinite(x) || (have_nd && x == nd)) ? qnan : x;
-        }
-    } else if (g.sample == sd::TiffSample::I16) {
-        const int16_t ndv = sat_cast<int16_t>(nd);
-        for (size_t i = 0; i < n; ++i) {
-            int16_t x;
-            std::memcpy(&x, &g.pixels[2 * i], 2);
-            h[i] = (have_nd && x == ndv) ? qnan : (float)x;
-        }
-    } else {
-        const uint16_t ndv = sat_cast<uint16_t>(nd);
-        for (size_t i = 0; i < n; ++i) {
-            uint16_t x;
-            std::memcpy(&x, &g.pixels[2 * i], 2);
-            h[i] = (have_nd && x == ndv) ? qnan : (float)x;
-        }
-    }
+    const int64_t n = g.width * g.height;
+    const int kind = g.sample == sd::TiffSample::F32 ? 0 : g.sample == sd::TiffSample::I16 ? 1 : 2;
+    // the nodata value cast to the sample type as Rust's `as` casts (saturating, NaN → 0)
+    const int32_t nd_i = kind == 1 ? (int32_t)sat_cast<int16_t>(nd) : (int32_t)sat_cast<uint16_t>(nd);
     sardine_status st;
     if ((st = require_device())) return st;
-    SD_HIP(staging().copy(dst, h.data(), n * sizeof(float), (hipStream_t)stream));
+    hipStream_t s = (hipStream_t)stream;
+    // the decoded samples go to HBM as they are; the sample rule runs there
+    DevBuf raw;
+    SD_HIP(raw.alloc(g.pixels.size(), s));
+    SD_HIP(staging().copy(raw.p, g.pixels.data(), g.pixels.size(), s));
+    KernelTimer kt(SARDINE_KERNEL_DEM, s);
+    SD_HIP(sd_launch_dem_values(raw.p, kind, have_nd ? 1 : 0, nd, nd_i, dst, n, s));
+    kt.stop();
     *out = r;
     return SARDINE_OK;
 }

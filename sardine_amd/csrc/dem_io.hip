@@ -3,16 +3,22 @@
 // the `tiff` crate (0.11).  Classic TIFF and BigTIFF, either byte order,
 // strips or tiles, no compression / LZW / Deflate / PackBits, horizontal
 // (2) and floating-point (3) predictors; the first image only.
+#include <sys/stat.h>
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 
 #include "dem_io.hpp"
 
 namespace sd {
 namespace {
+
+// worker threads for block decoding (a share of a host's cores)
+constexpr uint64_t kDecodeThreads = 16;
 
 struct File {
     std::vector<uint8_t> b;
@@ -191,10 +197,14 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
             *err = std::string("cannot open ") + path;
             return 1;
         }
-        uint8_t buf[1 << 16];
-        size_t k;
-        while ((k = std::fread(buf, 1, sizeof buf, fp)) > 0) f.b.insert(f.b.end(), buf, buf + k);
-        const bool bad = std::ferror(fp) != 0;
+        // a regular file, read whole in one call
+        struct stat st;
+        bool bad = fstat(fileno(fp), &st) != 0 || !S_ISREG(st.st_mode);
+        const long long size = bad ? 0 : (long long)st.st_size;
+        if (!bad) {
+            f.b.resize((size_t)size);
+            bad = std::fread(f.b.data(), 1, f.b.size(), fp) != f.b.size();
+        }
         std::fclose(fp);
         if (bad) {
             *err = std::string("cannot read ") + path;
@@ -277,6 +287,11 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
         }
     }
     if (!decode) return 0;
+    // from here on the header fields above are set: errors are decode errors
+    auto bad_block = [&](const std::string& why) {
+        *err = why;
+        return 3;
+    };
     // blocks: tiles or strips
     const bool tiled = find(322) != nullptr;
     uint64_t bw, bh;
@@ -284,76 +299,94 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
     if (tiled) {
         if (!scalar(322, 0, &bw) || !scalar(323, 0, &bh) || !find(324) || !find(325) ||
             !ints(f, *find(324), &offs) || !ints(f, *find(325), &counts))
-            return bad("unreadable tile tags");
+            return bad_block("unreadable tile tags");
     } else {
         uint64_t rps;
         if (!scalar(278, h, &rps) || !find(273) || !find(279) || !ints(f, *find(273), &offs) ||
             !ints(f, *find(279), &counts))
-            return bad("unreadable strip tags");
+            return bad_block("unreadable strip tags");
         bw = w;
         bh = std::min<uint64_t>(std::max<uint64_t>(rps, 1), h);
     }
-    if (bw == 0 || bh == 0) return bad("zero block size");
+    if (bw == 0 || bh == 0) return bad_block("zero block size");
     const uint64_t nx = (w + bw - 1) / bw, ny = (h + bh - 1) / bh;
-    if (offs.size() != nx * ny || counts.size() != nx * ny) return bad("block count does not match the image");
+    if (offs.size() != nx * ny || counts.size() != nx * ny) return bad_block("block count does not match the image");
     if (comp != 1 && comp != 5 && comp != 8 && comp != 32946 && comp != 32773)
-        return bad("unsupported compression " + std::to_string(comp));
-    if (pred != 1 && pred != 2 && pred != 3) return bad("unsupported predictor " + std::to_string(pred));
+        return bad_block("unsupported compression " + std::to_string(comp));
+    if (pred != 1 && pred != 2 && pred != 3) return bad_block("unsupported predictor " + std::to_string(pred));
     out->pixels.assign((size_t)(w * h * bps), 0);
-    std::vector<uint8_t> blk, tmp;
-    for (uint64_t by = 0; by < ny; ++by)
-        for (uint64_t bx = 0; bx < nx; ++bx) {
-            const uint64_t k = by * nx + bx;
-            // a strip holds its own rows only; a tile is always full size
-            const uint64_t rows = tiled ? bh : std::min(bh, h - by * bh);
-            const size_t want = (size_t)(rows * bw * bps);
-            if (!f.has(offs[k], counts[k])) return bad("block outside the file");
-            const uint8_t* src = &f.b[offs[k]];
-            const size_t sn = (size_t)counts[k];
-            bool ok;
-            if (comp == 1) {
-                ok = sn >= want;
-                if (ok) blk.assign(src, src + want);
-            } else if (comp == 5) {
-                ok = lzw(src, sn, blk, want);
-            } else if (comp == 32773) {
-                ok = packbits(src, sn, blk, want);
+    // Blocks decode independently into disjoint parts of the image, so they
+    // are spread over worker threads; the error reported is the first block's
+    // in file order, as a sequential decode would report it.
+    const uint64_t n_blocks = nx * ny;
+    std::vector<uint8_t> fault((size_t)n_blocks, 0);  // 1: outside the file, 2: corrupt
+    auto decode_block = [&](uint64_t k, std::vector<uint8_t>& blk, std::vector<uint8_t>& tmp) -> uint8_t {
+        const uint64_t by = k / nx, bx = k % nx;
+        // a strip holds its own rows only; a tile is always full size
+        const uint64_t rows = tiled ? bh : std::min(bh, h - by * bh);
+        const size_t want = (size_t)(rows * bw * bps);
+        if (!f.has(offs[k], counts[k])) return 1;
+        const uint8_t* src = &f.b[offs[k]];
+        const size_t sn = (size_t)counts[k];
+        bool ok;
+        if (comp == 1) {
+            ok = sn >= want;
+            if (ok) blk.assign(src, src + want);
+        } else if (comp == 5) {
+            ok = lzw(src, sn, blk, want);
+        } else if (comp == 32773) {
+            ok = packbits(src, sn, blk, want);
+        } else {
+            ok = inflate_all(src, sn, blk, want);
+        }
+        if (!ok) return 2;
+        blk.resize(want);
+        const size_t row_bytes = (size_t)(bw * bps);
+        for (uint64_t r = 0; r < rows; ++r) {
+            uint8_t* p = &blk[(size_t)r * row_bytes];
+            if (pred == 3) {
+                // floating-point predictor: byte differences along the row,
+                // then byte planes (most significant first) to samples
+                for (size_t i = 1; i < row_bytes; ++i) p[i] = (uint8_t)(p[i] + p[i - 1]);
+                tmp.assign(p, p + row_bytes);
+                for (uint64_t s = 0; s < bw; ++s)
+                    for (int byte = 0; byte < bps; ++byte) p[s * bps + byte] = tmp[(size_t)(bps - byte - 1) * bw + s];
             } else {
-                ok = inflate_all(src, sn, blk, want);
-            }
-            if (!ok) return bad("corrupt compressed block");
-            blk.resize(want);
-            const size_t row_bytes = (size_t)(bw * bps);
-            for (uint64_t r = 0; r < rows; ++r) {
-                uint8_t* p = &blk[(size_t)r * row_bytes];
-                if (pred == 3) {
-                    // floating-point predictor: byte differences along the row,
-                    // then byte planes (most significant first) to samples
-                    for (size_t i = 1; i < row_bytes; ++i) p[i] = (uint8_t)(p[i] + p[i - 1]);
-                    tmp.assign(p, p + row_bytes);
-                    for (uint64_t s = 0; s < bw; ++s)
-                        for (int byte = 0; byte < bps; ++byte)
-                            p[s * bps + byte] = tmp[(size_t)(bps - byte - 1) * bw + s];
-                } else {
-                    if (!f.le && bps > 1)
-                        for (uint64_t s = 0; s < bw; ++s) std::reverse(p + s * bps, p + (s + 1) * bps);
-                    if (pred == 2) {
-                        for (uint64_t s = 1; s < bw; ++s) {
-                            uint64_t a = 0, b = 0;
-                            std::memcpy(&a, p + (s - 1) * bps, (size_t)bps);
-                            std::memcpy(&b, p + s * bps, (size_t)bps);
-                            const uint64_t sum = a + b;
-                            std::memcpy(p + s * bps, &sum, (size_t)bps);
-                        }
+                if (!f.le && bps > 1)
+                    for (uint64_t s = 0; s < bw; ++s) std::reverse(p + s * bps, p + (s + 1) * bps);
+                if (pred == 2) {
+                    for (uint64_t s = 1; s < bw; ++s) {
+                        uint64_t a = 0, b = 0;
+                        std::memcpy(&a, p + (s - 1) * bps, (size_t)bps);
+                        std::memcpy(&b, p + s * bps, (size_t)bps);
+                        const uint64_t sum = a + b;
+                        std::memcpy(p + s * bps, &sum, (size_t)bps);
                     }
                 }
-                // the block's row into the image (tiles are cropped at the edges)
-                const uint64_t y = by * bh + r;
-                if (y >= h) break;
-                const uint64_t x0 = bx * bw, cols = std::min(bw, w - x0);
-                std::memcpy(&out->pixels[(size_t)((y * w + x0) * bps)], p, (size_t)(cols * bps));
             }
+            // the block's row into the image (tiles are cropped at the edges)
+            const uint64_t y = by * bh + r;
+            if (y >= h) break;
+            const uint64_t x0 = bx * bw, cols = std::min(bw, w - x0);
+            std::memcpy(&out->pixels[(size_t)((y * w + x0) * bps)], p, (size_t)(cols * bps));
         }
+        return 0;
+    };
+    std::atomic<uint64_t> next{0};
+    auto worker = [&]() {
+        std::vector<uint8_t> blk, tmp;
+        for (uint64_t k; (k = next.fetch_add(1)) < n_blocks;) fault[(size_t)k] = decode_block(k, blk, tmp);
+    };
+    const uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint64_t n_threads = std::min<uint64_t>(std::min<uint64_t>(hw, kDecodeThreads), n_blocks);
+    std::vector<std::thread> pool;
+    for (uint64_t t = 1; t < n_threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    for (uint64_t k = 0; k < n_blocks; ++k) {
+        if (fault[(size_t)k] == 1) return bad_block("block outside the file");
+        if (fault[(size_t)k] == 2) return bad_block("corrupt compressed block");
+    }
     return 0;
 }
 

@@ -21,9 +21,10 @@ struct GeoTiffRaster {
 };
 
 // Decodes the first image of a TIFF file (with decode false: its size,
-// sample type and GeoTIFF tags only).  Returns 0, or 1 with *err set when
-// the file cannot be read, 2 when it is not a TIFF this reader decodes (the
-// message says why).
+// sample type and GeoTIFF tags only).  Returns 0, or with *err set: 1 when
+// the file cannot be read, 2 when its header is not one this reader takes,
+// 3 when the header was read (the size, sample type and tags are set) but
+// the image data cannot be decoded (the message says why).
 int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool decode = true);
 
 }  // namespace sd

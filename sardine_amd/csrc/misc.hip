@@ -75,6 +75,26 @@ __global__ __launch_bounds__(256) void srtm_tile_kernel(const uint16_t* __restri
     dst[(row0 + r) * dst_w + col0 + c] = v == -32768 ? __builtin_nanf("") : (float)v;
 }
 
+// Glo30Tile::load's sample rule (dem.rs:317-342) on decoded samples in host
+// byte order: kind 0 = f32 (non-finite or equal to the nodata value → NaN),
+// 1 = i16 and 2 = u16 (equal to the nodata value, already cast to the sample
+// type, → NaN; otherwise widened to f32).
+__global__ __launch_bounds__(256) void dem_values_kernel(const void* __restrict__ raw, int kind, int have_nd,
+                                                         float nd_f, int32_t nd_i, float* __restrict__ dst,
+                                                         int64_t n) {
+    const float qnan = __builtin_nanf("");
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (kind == 0) {
+            const float x = static_cast<const float*>(raw)[i];
+            dst[i] = (!__builtin_isfinite(x) || (have_nd && x == nd_f)) ? qnan : x;
+        } else {
+            const int32_t x = kind == 1 ? (int32_t) static_cast<const int16_t*>(raw)[i]
+                                        : (int32_t) static_cast<const uint16_t*>(raw)[i];
+            dst[i] = (have_nd && x == nd_i) ? qnan : (float)x;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void fill_nan_kernel(float* d, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         d[i] = __builtin_nanf("");
@@ -128,6 +148,16 @@ extern "C" hipError_t sd_launch_srtm_tile(const uint16_t* be, int dim, float* ds
                                           int64_t col0, hipStream_t s) {
     hipLaunchKernelGGL(sd::srtm_tile_kernel, dim3((unsigned)((dim + 255) / 256), (unsigned)dim), dim3(256), 0, s, be,
                        dim, dst, dst_w, row0, col0);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sd_launch_dem_values(const void* raw, int kind, int have_nd, float nd_f, int32_t nd_i,
+                                           float* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(sd::dem_values_kernel, dim3((unsigned)blocks), dim3(256), 0, s, raw, kind, have_nd, nd_f, nd_i,
+                       dst, n);
     return hipGetLastError();
 }
 
