@@ -46,6 +46,24 @@ __device__ __forceinline__ void stage_tile(const SpeckleParams& p, float* tile, 
     }
 }
 
+// Refined Lee's tile: each staged sample as {x, x²} in f64 (NaN outside the
+// raster), so the 48 taps of a pixel are one 16-byte LDS read each and no
+// pixel repeats a neighbour's conversion or square.
+constexpr int kRlPitch = kTileW + 6;
+constexpr size_t kRlTileBytes = (size_t)kRlPitch * (kTileH + 6) * sizeof(double2);
+
+__device__ __forceinline__ void stage_tile_rl(const SpeckleParams& p, double2* tile, int64_t r0, int64_t c0) {
+    const int th = kTileH + 6;
+    const double qnan = __builtin_nan("");
+    for (int i = threadIdx.x; i < kRlPitch * th; i += 256) {
+        int ty = i / kRlPitch, tx = i - ty * kRlPitch;
+        int64_t r = r0 - 3 + ty, c = c0 - 3 + tx;
+        double x = qnan;
+        if (r >= 0 && r < p.rows && c >= 0 && c < p.cols) x = (double)p.in[r * p.cols + c];
+        tile[i] = double2{x, x * x};
+    }
+}
+
 template <int D>
 __device__ __forceinline__ double div_const(double x);
 
@@ -225,7 +243,7 @@ __device__ __forceinline__ float rl_update(float best_cv, float best_mean, float
 // 9 (diagonal bands), and the sums are the reference's.  A non-finite
 // neighbour makes some direction's sum non-finite (every neighbour belongs
 // to at least one direction) and the pixel is redone with the skipping rule.
-__device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, float centre, float cv_n_sq) {
+__device__ __forceinline__ float refined_lee_pixel(const double2* t, int pitch, float centre, float cv_n_sq) {
     double sum[8], sq[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) { sum[d] = 0.0; sq[d] = 0.0; }
@@ -236,7 +254,8 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
             const unsigned m = rl_dirs(dy, dx);
             if (m == 0u) continue;
             const unsigned first = kRlFirst.m[dy + 3][dx + 3];
-            const double x = (double)t[dy * pitch + dx], xx = x * x;
+            const double2 e = t[dy * pitch + dx];
+            const double x = e.x, xx = e.y;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
                 if (m & (1u << d)) {
@@ -252,9 +271,12 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
             // all 48 LDS reads (and their registers) to the top
             if (dx == 3) __builtin_amdgcn_sched_barrier(0);
         }
+    // finite exactly when every neighbour is (f64 sums of f32 values and
+    // their squares cannot overflow); the squares' sums are part of the test
+    // so that they are formed in this pass, not kept as 48 live squares
     double all = 0.0;
 #pragma unroll
-    for (int d = 0; d < 8; ++d) all += sum[d];
+    for (int d = 0; d < 8; ++d) all += sum[d] + sq[d];
     float best_cv = __builtin_inff(), best_mean = centre, best_var = 0.0f;
     if (isfinite(all)) {
         // Every direction's mean and variance, and its CV² = var / mean² from
@@ -323,9 +345,9 @@ __device__ __forceinline__ float refined_lee_pixel(const float* t, int pitch, fl
         for (int dx = -3; dx <= 3; ++dx) {
             const unsigned m = rl_dirs(dy, dx);
             if (m == 0u) continue;
-            float v = t[dy * pitch + dx];
-            if (!isfinite(v)) continue;
-            double x = (double)v, xx = x * x;
+            const double2 e = t[dy * pitch + dx];
+            if (!isfinite(e.x)) continue;
+            const double x = e.x, xx = e.y;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
                 if (m & (1u << d)) {
@@ -347,7 +369,9 @@ template <int KIND, int HALF>
 __device__ __forceinline__ void speckle_tile(const SpeckleParams& p, float* tile, int64_t r0, int64_t c0) {
     const int half = KIND == SK_RLEE ? 3 : (HALF > 0 ? HALF : p.half);
     const int pitch = kTileW + 2 * half;
-    stage_tile(p, tile, half, r0, c0);
+    double2* rl_tile = reinterpret_cast<double2*>(tile);
+    if constexpr (KIND == SK_RLEE) stage_tile_rl(p, rl_tile, r0, c0);
+    else stage_tile(p, tile, half, r0, c0);
     __syncthreads();
     const int tx = threadIdx.x & (kTileW - 1);
     const int ty0 = threadIdx.x >> 6;  // 0..3
@@ -357,12 +381,15 @@ __device__ __forceinline__ void speckle_tile(const SpeckleParams& p, float* tile
         const int ty = ty0 + 4 * k;
         const int64_t r = r0 + ty, c = c0 + tx;
         if (r >= p.rows || c >= p.cols) continue;
-        const float* t = tile + (ty + half) * pitch + (tx + half);
-        const float centre = t[0];
         float o = __builtin_nanf("");
-        if (isfinite(centre)) {
-            if (KIND == SK_RLEE) o = refined_lee_pixel(t, pitch, centre, cv_n_sq);
-            else o = filter_pixel<KIND, HALF>(p, t, pitch, centre);
+        if constexpr (KIND == SK_RLEE) {
+            const double2* t = rl_tile + (ty + 3) * kRlPitch + (tx + 3);
+            const float centre = (float)t[0].x;  // exact: the sample was an f32
+            if (isfinite(centre)) o = refined_lee_pixel(t, kRlPitch, centre, cv_n_sq);
+        } else {
+            const float* t = tile + (ty + half) * pitch + (tx + half);
+            const float centre = t[0];
+            if (isfinite(centre)) o = filter_pixel<KIND, HALF>(p, t, pitch, centre);
         }
         p.out[r * p.cols + c] = o;
     }
@@ -370,14 +397,14 @@ __device__ __forceinline__ void speckle_tile(const SpeckleParams& p, float* tile
 
 template <int KIND, int HALF>
 __global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
-    extern __shared__ float tile[];
+    extern __shared__ __attribute__((aligned(16))) float tile[];
     speckle_tile<KIND, HALF>(p, tile, (int64_t)blockIdx.y * kTileH, (int64_t)blockIdx.x * kTileW);
 }
 
 // Refined Lee keeps 16 f64 accumulators per pixel; 4 waves per SIMD fit
 // without spilling.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void refined_lee_kernel(SpeckleParams p) {
-    extern __shared__ float tile[];
+    extern __shared__ __attribute__((aligned(16))) float tile[];
     speckle_tile<SK_RLEE, 3>(p, tile, (int64_t)blockIdx.y * kTileH, (int64_t)blockIdx.x * kTileW);
 }
 
@@ -412,7 +439,7 @@ __device__ __forceinline__ void batch_tile_kind(const SpeckleParams& p, float* t
 // the LDS tile is sized for the largest window of the batch.
 __global__ __launch_bounds__(256) void batch_speckle_kernel(const SpeckleBatchHeader* __restrict__ H,
                                                             const SpeckleJob* __restrict__ J, unsigned int* queue) {
-    extern __shared__ float tile[];
+    extern __shared__ __attribute__((aligned(16))) float tile[];
     __shared__ unsigned int s_item;
     const int n_jobs = H->n_jobs;
     const int64_t n = H->n_tiles;
@@ -445,7 +472,7 @@ __global__ __launch_bounds__(256) void batch_speckle_kernel(const SpeckleBatchHe
 template <int KIND, int HALF>
 static hipError_t launch_one(const SpeckleParams& p, hipStream_t s) {
     const int half = KIND == SK_RLEE ? 3 : p.half;
-    size_t lds = (size_t)(kTileW + 2 * half) * (kTileH + 2 * half) * sizeof(float);
+    size_t lds = KIND == SK_RLEE ? kRlTileBytes : (size_t)(kTileW + 2 * half) * (kTileH + 2 * half) * sizeof(float);
     dim3 grid((unsigned)((p.cols + kTileW - 1) / kTileW), (unsigned)((p.rows + kTileH - 1) / kTileH));
     if constexpr (KIND == SK_RLEE) hipLaunchKernelGGL(refined_lee_kernel, grid, dim3(256), lds, s, p);
     else hipLaunchKernelGGL((speckle_kernel<KIND, HALF>), grid, dim3(256), lds, s, p);
@@ -499,7 +526,9 @@ extern "C" void sd_speckle_job_fill(void* entry, int kind, const float* in, floa
 
 extern "C" hipError_t sd_launch_batch_speckle(const void* H, const void* J, unsigned int* queue, int n_groups,
                                               int max_half, hipStream_t stream) {
-    const size_t lds = (size_t)(sd::kTileW + 2 * max_half) * (sd::kTileH + 2 * max_half) * sizeof(float);
+    // Refined Lee jobs (half 3) stage {x, x²} pairs: the larger of the tiles
+    size_t lds = (size_t)(sd::kTileW + 2 * max_half) * (sd::kTileH + 2 * max_half) * sizeof(float);
+    if (max_half >= 3 && lds < sd::kRlTileBytes) lds = sd::kRlTileBytes;
     hipLaunchKernelGGL(sd::batch_speckle_kernel, dim3((unsigned)n_groups), dim3(256), lds, stream,
                        static_cast<const sd::SpeckleBatchHeader*>(H), static_cast<const sd::SpeckleJob*>(J), queue);
     return hipGetLastError();
