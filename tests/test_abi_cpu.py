@@ -639,3 +639,53 @@ def test_glo30_tile_georeferencing_and_errors(tmp_path):
     (tmp_path / "bad.tif").write_bytes(bytes(b))
     assert L.sardine_dem_glo30_tile(str(tmp_path / "bad.tif").encode(), buf.ctypes.data, C.byref(r), None) == \
         abi.ERR_DEM_TIFF_DECODE
+
+
+def _patch_tile(buf: bytearray, k: int, outside: bool = False, corrupt: bool = False) -> None:
+    """Block k of a little-endian classic tiled TIFF: its offset moved past
+    the end of the file, or its Deflate stream overwritten with garbage."""
+    import struct
+
+    import tiffread as T
+    ifd = T.parse(bytes(buf))["ifds"][0]
+    offs, counts = ifd["tags"][324][1], ifd["tags"][325][1]
+    if corrupt:
+        buf[offs[k]:offs[k] + counts[k]] = b"\xff" * counts[k]
+    if outside:
+        n = struct.unpack_from("<H", buf, ifd["offset"])[0]
+        for i in range(n):
+            e = ifd["offset"] + 2 + 12 * i
+            if struct.unpack_from("<H", buf, e)[0] == 324:
+                table = struct.unpack_from("<I", buf, e + 8)[0]
+                struct.pack_into("<I", buf, table + 4 * k, len(buf) + 4096)
+
+
+def test_glo30_decode_errors_in_file_order(tmp_path):
+    """Blocks decode in parallel, but the error reported is the first failing
+    block's in file order, as a sequential decode (the `tiff` crate's)
+    reports it; and decode errors come after the georeferencing checks
+    (Glo30Tile::load reads the georeferencing before the image)."""
+    import slc_fixtures as F
+    L = _native.lib()
+    a = np.arange(64 * 64, dtype=np.float32).reshape(64, 64)
+    good = bytearray(F.geotiff_dem_bytes(a, tile=(16, 16), compression="deflate"))
+    buf = np.zeros(4, np.float32)
+    r = abi.DemRaster()
+    for first, second, want in (("corrupt", "outside", "corrupt compressed block"),
+                                ("outside", "corrupt", "block outside the file")):
+        b = bytearray(good)
+        _patch_tile(b, 2, **{first: True})
+        _patch_tile(b, 11, **{second: True})
+        p = tmp_path / "N50E010.tif"
+        p.write_bytes(bytes(b))
+        for _ in range(5):  # the same answer whatever the threads' timing
+            assert L.sardine_dem_glo30_tile(str(p).encode(), buf.ctypes.data, C.byref(r), None) == \
+                abi.ERR_DEM_TIFF_DECODE
+            assert want in _native.last_error()
+        # the layout query does not decode: the header is good
+        assert L.sardine_dem_glo30_tile(str(p).encode(), None, C.byref(r), None) == 0
+        # a bad name without GeoTIFF tags is reported before the bad block
+        q = tmp_path / "not_a_tile.tif"
+        q.write_bytes(bytes(b))
+        assert L.sardine_dem_glo30_tile(str(q).encode(), buf.ctypes.data, C.byref(r), None) == \
+            abi.ERR_DEM_UNRECOGNISED_TILE_NAME
