@@ -71,6 +71,9 @@ extern "C" hipError_t sd_launch_srtm_tile(const uint16_t* be, int dim, float* ds
 extern "C" hipError_t sd_launch_fill_nan(float* d, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_dem_values(const void* raw, int kind, int have_nd, float nd_f, int32_t nd_i,
                                            float* dst, int64_t n, hipStream_t s);
+extern "C" hipError_t sd_launch_fp_predictor(const uint8_t* raw, int64_t bw, int64_t bh, int64_t blocks_x,
+                                             int64_t blocks_y, int64_t w, int64_t h, int have_nd, float nd,
+                                             float* dst, hipStream_t s);
 extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t seed, uint64_t base, double sigma,
                                              hipStream_t s);
 
@@ -1912,7 +1915,7 @@ sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_
     std::string why;
     // one read of the file; decode errors (rc 3) are reported after the
     // georeferencing, as the reference decodes the image after it
-    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/dst != nullptr);
+    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/dst != nullptr, /*defer_fp_predictor=*/true);
     if (rc == 1) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", path, why.c_str());
     if (rc == 2) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
     double lat0, lon0, px;
@@ -1972,7 +1975,11 @@ sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_
     SD_HIP(raw.alloc(g.pixels.size(), s));
     SD_HIP(staging().copy(raw.p, g.pixels.data(), g.pixels.size(), s));
     KernelTimer kt(SARDINE_KERNEL_DEM, s);
-    SD_HIP(sd_launch_dem_values(raw.p, kind, have_nd ? 1 : 0, nd, nd_i, dst, n, s));
+    if (g.fp_deferred)  // the floating-point predictor and the sample rule in one pass
+        SD_HIP(sd_launch_fp_predictor(raw.as<uint8_t>(), g.block_w, g.block_h, g.blocks_x, g.blocks_y, g.width,
+                                      g.height, have_nd ? 1 : 0, nd, dst, s));
+    else
+        SD_HIP(sd_launch_dem_values(raw.p, kind, have_nd ? 1 : 0, nd, nd_i, dst, n, s));
     kt.stop();
     *out = r;
     return SARDINE_OK;

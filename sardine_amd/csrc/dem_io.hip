@@ -189,7 +189,7 @@ bool inflate_all(const uint8_t* in, size_t n, std::vector<uint8_t>& out, size_t 
 
 }  // namespace
 
-int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool decode) {
+int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool decode, bool defer_fp_predictor) {
     File f;
     {
         std::FILE* fp = std::fopen(path, "rb");
@@ -314,7 +314,18 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
     if (comp != 1 && comp != 5 && comp != 8 && comp != 32946 && comp != 32773)
         return bad_block("unsupported compression " + std::to_string(comp));
     if (pred != 1 && pred != 2 && pred != 3) return bad_block("unsupported predictor " + std::to_string(pred));
-    out->pixels.assign((size_t)(w * h * bps), 0);
+    const bool defer = defer_fp_predictor && pred == 3 && bps == 4 && out->sample == TiffSample::F32 &&
+                       (int64_t)(bw * bps) <= kMaxDeferredRowBytes;
+    if (defer) {
+        out->fp_deferred = true;
+        out->block_w = (int64_t)bw;
+        out->block_h = (int64_t)bh;
+        out->blocks_x = (int64_t)nx;
+        out->blocks_y = (int64_t)ny;
+        out->pixels.assign((size_t)(nx * ny * bh * bw * bps), 0);
+    } else {
+        out->pixels.assign((size_t)(w * h * bps), 0);
+    }
     // Blocks decode independently into disjoint parts of the image, so they
     // are spread over worker threads; the error reported is the first block's
     // in file order, as a sequential decode would report it.
@@ -340,6 +351,10 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
             ok = inflate_all(src, sn, blk, want);
         }
         if (!ok) return 2;
+        if (defer) {  // the block as inflated, into its slot
+            std::memcpy(&out->pixels[(size_t)(k * bh * bw * bps)], blk.data(), want);
+            return 0;
+        }
         blk.resize(want);
         const size_t row_bytes = (size_t)(bw * bps);
         for (uint64_t r = 0; r < rows; ++r) {

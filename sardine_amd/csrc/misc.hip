@@ -95,6 +95,52 @@ __global__ __launch_bounds__(256) void dem_values_kernel(const void* __restrict_
     }
 }
 
+// TIFF's floating-point predictor (3) undone on the device, fused with
+// Glo30Tile::load's f32 sample rule.  One workgroup per stored block row
+// (blocks in file order, block_h rows of bw * 4 bytes each): the row's bytes
+// are an inclusive prefix sum mod 256 of the stored ones; the result holds 4
+// byte planes, most significant first, from which sample s is rebuilt and
+// written to the image (cropped at the right and bottom edges).
+__global__ __launch_bounds__(256) void fp_predictor_kernel(const uint8_t* __restrict__ raw, int64_t bw, int64_t bh,
+                                                           int64_t nx, int64_t w, int64_t h, int have_nd, float nd,
+                                                           float* __restrict__ dst) {
+    extern __shared__ uint8_t row[];  // bw * 4 bytes
+    __shared__ uint32_t part[256];
+    const int64_t k = blockIdx.x / bh, r = blockIdx.x % bh;
+    const int64_t by = k / nx, bx = k % nx;
+    const int64_t y = by * bh + r;
+    if (y >= h) return;  // the same for the whole workgroup
+    const int64_t rb = bw * 4;
+    const uint8_t* src = raw + (size_t)blockIdx.x * rb;
+    // each thread a run of consecutive bytes; the runs' sums scanned in LDS
+    const int64_t chunk = (rb + 255) / 256;
+    const int64_t b0 = min((int64_t)threadIdx.x * chunk, rb), b1 = min(b0 + chunk, rb);
+    uint32_t acc = 0;
+    for (int64_t i = b0; i < b1; ++i) acc += src[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - acc;  // exclusive: the bytes before this run
+    for (int64_t i = b0; i < b1; ++i) {
+        run += src[i];
+        row[i] = (uint8_t)run;
+    }
+    __syncthreads();
+    const int64_t x0 = bx * bw, cols = min(bw, w - x0);
+    const float qnan = __builtin_nanf("");
+    for (int64_t s = threadIdx.x; s < cols; s += 256) {
+        const uint32_t u = (uint32_t)row[s] << 24 | (uint32_t)row[bw + s] << 16 | (uint32_t)row[2 * bw + s] << 8 |
+                           (uint32_t)row[3 * bw + s];
+        const float x = __uint_as_float(u);
+        dst[y * w + x0 + s] = (!__builtin_isfinite(x) || (have_nd && x == nd)) ? qnan : x;
+    }
+}
+
 __global__ __launch_bounds__(256) void fill_nan_kernel(float* d, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         d[i] = __builtin_nanf("");
@@ -158,6 +204,18 @@ extern "C" hipError_t sd_launch_dem_values(const void* raw, int kind, int have_n
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(sd::dem_values_kernel, dim3((unsigned)blocks), dim3(256), 0, s, raw, kind, have_nd, nd_f, nd_i,
                        dst, n);
+    return hipGetLastError();
+}
+
+// raw: blocks_x * blocks_y blocks of bh rows of bw * 4 bytes (bw * 4 <=
+// 32768, the dynamic LDS row); blocks_x * blocks_y * bh < 2^31 rows.
+extern "C" hipError_t sd_launch_fp_predictor(const uint8_t* raw, int64_t bw, int64_t bh, int64_t blocks_x,
+                                             int64_t blocks_y, int64_t w, int64_t h, int have_nd, float nd,
+                                             float* dst, hipStream_t s) {
+    const int64_t rows = blocks_x * blocks_y * bh;
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sd::fp_predictor_kernel, dim3((unsigned)rows), dim3(256), (size_t)(bw * 4), s, raw, bw, bh,
+                       blocks_x, w, h, have_nd, nd, dst);
     return hipGetLastError();
 }
 
