@@ -129,7 +129,10 @@ struct GrdColumn {
 // One output column per thread.  With one azimuth look (the IW case) a
 // thread takes kGrdRows output lines, gridDim.y apart, and issues all their
 // tap loads before it uses any.
-constexpr int kGrdRows = 4;
+#ifndef SD_GRD_ROWS
+#define SD_GRD_ROWS 4
+#endif
+constexpr int kGrdRows = SD_GRD_ROWS;
 
 __device__ __forceinline__ float grd_tap(const float* row, const GrdColumn& m, bool exact) {
     if (exact) return row[0];
