@@ -448,7 +448,11 @@ __global__ __launch_bounds__(256) void speckle_kernel(SpeckleParams p) {
 
 // Refined Lee keeps 16 f64 accumulators per pixel; 4 waves per SIMD fit
 // without spilling.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void refined_lee_kernel(SpeckleParams p) {
+#ifndef SD_RL_WAVES
+#define SD_RL_WAVES 4
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SD_RL_WAVES))) void refined_lee_kernel(
+    SpeckleParams p) {
     extern __shared__ __attribute__((aligned(16))) float tile[];
     speckle_tile<SK_RLEE, 3>(p, tile, (int64_t)blockIdx.y * kTileH, (int64_t)blockIdx.x * kTileW);
 }
