@@ -86,3 +86,31 @@ def test_glo30_tile_decode(cuda, tmp_path, dt, comp, pred, tile, le, big):
     want = _expect(a, nodata)
     assert np.array_equal(got, want, equal_nan=True)
     assert (dem.origin_lat_deg, dem.origin_lon_deg) == (47.0 - 0.5 / 3600, 7.0 + 0.5 / 3600)
+
+
+def test_srtm_back_to_back_loads(cuda, tmp_path):
+    """Mosaic loads issued back to back on one stream, with no
+    synchronisation between them, each equal their own tiles: the pinned
+    chunks a load reads through are refilled by the next load only after the
+    copies out of them have run."""
+    import torch
+
+    layouts = [((10, 20), (10, 21)), ((-3, 100),), ((40, -5), (41, -5))]
+    paths, wants = [], []
+    for i, layout in enumerate(layouts):
+        d = tmp_path / f"set{i}"
+        d.mkdir()
+        tiles = {}
+        for la, lo in layout:
+            tiles[(la, lo)] = F.srtm_field(la, lo, [(11 * i + 3, 5)])
+            ns, ew = ("N" if la >= 0 else "S"), ("E" if lo >= 0 else "W")
+            F.write_hgt(d / f"{ns}{abs(la):02d}{ew}{abs(lo):03d}.hgt", tiles[(la, lo)])
+        paths.append(sorted(str(p) for p in d.glob("*.hgt")))
+        wants.append(F.mosaic(tiles)[0])
+    print("srtm back-to-back loads", flush=True)
+    s = torch.cuda.current_stream()
+    order = [0, 1, 2, 0, 2, 1, 1, 0]
+    got = [api.load_srtm_mosaic(paths[k], device=cuda, stream=s) for k in order]
+    torch.cuda.synchronize()
+    for k, dem in zip(order, got):
+        assert np.array_equal(dem.data.cpu().numpy(), wants[k], equal_nan=True)
