@@ -7,6 +7,9 @@ values that pin the oracle's closed forms without trusting its arithmetic.
                       rule, evaluated in exact rational-like 40-digit arithmetic
                       (accel = exact derivative of the velocity polynomial)
   utm               — Krüger n^6 forward TM at 6 points (Karney 2011 series)
+  meridian_arc      — northing on a central meridian, k0·M(φ), with the
+                      meridian arc M(φ) = a(1-e²)∫_0^φ (1-e² sin²t)^(-3/2) dt
+                      integrated by mpmath quadrature (no series involved)
 
 Run: python tests/golden/make_vectors.py   (needs mpmath; numpy for the orbit)
 """
@@ -89,6 +92,13 @@ def utm_fwd(zone, lon, lat):
     return 500000 + k0 * A_ * eta, k0 * A_ * xi
 
 
+def meridian_northing(lat):
+    phi = mp.mpf(lat) * mp.pi / 180
+    m = A * (1 - E2) * mp.quad(lambda t: (1 - E2 * mp.sin(t) ** 2) ** mp.mpf(-1.5), [0, phi])
+    k0 = mp.mpf("0.9996")
+    return k0 * m + (10_000_000 if lat < 0 else 0)
+
+
 def main():
     from sardine_amd import synthetic
 
@@ -111,6 +121,8 @@ def main():
     ut = [(32, 11.5, 48.1), (32, 9.0, 45.0), (33, 15.0, 0.5), (31, 0.5, 60.0), (32, 12.9, 47.9), (1, -177.0, 10.0)]
     out["utm_north"] = [{"zone": z, "lon": lo, "lat": la, "xy": [str(v) for v in utm_fwd(z, lo, la)]}
                         for z, lo, la in ut]
+    lats = [-80.0, -45.5, -10.0, -0.25, 0.5, 30.0, 48.1374, 60.0, 72.5, 84.0]
+    out["meridian_arc"] = [{"lat": la, "northing": str(meridian_northing(la))} for la in lats]
     (HERE / "hp_vectors.json").write_text(json.dumps(out, indent=1))
     print("wrote", HERE / "hp_vectors.json")
 

@@ -356,16 +356,13 @@ def test_output_grid_utm_contains_corners():
     assert 50_000 < c * 50 < 200_000 and 40_000 < r * 50 < 100_000
 
 
-# ── UTM (proj4rs behind output_crs.rs; parity unpinned below ±1 m) ────────
+# ── UTM (proj4rs behind output_crs.rs) ──────────────────────────────────
 def test_utm_high_precision_and_round_trip():
     for c in GOLD["utm_north"]:
         x, y = O.utm_forward(c["zone"], False, c["lon"], c["lat"])
         assert abs(x - float(c["xy"][0])) < 1e-6 and abs(y - float(c["xy"][1])) < 1e-6
         lon, lat = O.utm_inverse(c["zone"], False, x, y)
         assert abs(lon - c["lon"]) < 1e-10 and abs(lat - c["lat"]) < 1e-10
-    # Munich in zone 32N: the reference pins proj4rs to ±1 m here (output_crs.rs:597-616)
-    x, y = O.utm_forward(32, False, 11.5755, 48.1374)
-    assert abs(x - 691600) < 1000 and abs(y - 5334700) < 1000
     # central meridian, equator: x = 500 km, y = 0
     x, y = O.utm_forward(33, False, 15.0, 0.0)
     assert abs(x - 500000.0) < 1e-6 and abs(y) < 1e-6
@@ -374,11 +371,106 @@ def test_utm_high_precision_and_round_trip():
     assert abs(ys - (yn + 10_000_000.0)) < 1e-6
 
 
-def test_project_row_anchor_interpolation():
+def test_utm32n_munich_reference_case():
+    # output_crs.rs:596-616: the reference's proj4rs result at Marienplatz
+    # within ±1 m of 691 603.03 / 5 334 780.03 (its comment gives PROJ 8.2.1's
+    # 691 603.032 / 5 334 780.031), and the round trip within 1e-7°.
+    x, y = O.utm_forward(32, False, 11.5755, 48.1374)
+    assert abs(x - 691_603.03) < 1.0 and abs(y - 5_334_780.03) < 1.0
+    # tighter than the reference asks: the documented PROJ value to 5 mm
+    # (a wrong k0 moves this point by ~76 m, a wrong A by metres)
+    assert abs(x - 691_603.032) < 5e-3 and abs(y - 5_334_780.031) < 5e-3
+    lon, lat = O.utm_inverse(32, False, x, y)
+    assert abs(lon - 11.5755) < 1e-7 and abs(lat - 48.1374) < 1e-7
+
+
+def test_utm56s_sydney_reference_case():
+    # output_crs.rs:618-635: round trip within 1e-7°, easting inside the UTM
+    # band, southern northing offset by 10 000 km
+    x, y = O.utm_forward(56, True, 151.2153, -33.8568)
+    lon, lat = O.utm_inverse(56, True, x, y)
+    assert abs(lon - 151.2153) < 1e-7 and abs(lat + 33.8568) < 1e-7
+    assert 100_000.0 < x < 900_000.0
+    assert y > 5_000_000.0
+
+
+def test_utm_central_meridian_is_scaled_meridian_arc():
+    # On the central meridian the transverse Mercator northing is k0 times
+    # the meridian arc length; the golden values integrate the arc by
+    # quadrature (no series), so this pins k0, the rectifying radius and
+    # every forward/inverse series coefficient through their sin(2jξ) terms.
+    for c in GOLD["meridian_arc"]:
+        south = c["lat"] < 0
+        for zone in (1, 32, 56):
+            cm = zone * 6 - 183
+            x, y = O.utm_forward(zone, south, float(cm), c["lat"])
+            assert abs(x - 500_000.0) < 1e-6
+            assert abs(y - float(c["northing"])) < 1e-6, (zone, c["lat"], y, c["northing"])
+            lon, lat = O.utm_inverse(zone, south, 500_000.0, float(c["northing"]))
+            assert abs(lat - c["lat"]) < 1e-11 and abs(lon - cm) < 1e-11
+
+
+def _isometric_latitude(lat_deg):
+    e = math.sqrt(2 * F - F * F)
+    s = math.sin(math.radians(lat_deg))
+    return math.atanh(s) - e * math.atanh(e * s)
+
+
+def test_utm_forward_is_conformal():
+    # y + i x is a holomorphic function of the isometric latitude ψ plus i λ
+    # (that is what makes the projection transverse Mercator), so the
+    # Cauchy–Riemann equations ∂y/∂ψ = ∂x/∂λ and ∂y/∂λ = -∂x/∂ψ hold.  A sign
+    # or a cosh/sinh swap in the η terms of the series breaks them at the
+    # 1e-3 level; the central differences below resolve ~1e-8.
+    e = math.sqrt(2 * F - F * F)
+    h = 1e-4  # radians
+
+    def xy(psi, lam):
+        # invert ψ → φ by fixed point on φ = gd(ψ + e atanh(e sin φ))
+        phi = math.atan(math.sinh(psi))
+        for _ in range(60):
+            phi = math.atan(math.sinh(psi + e * math.atanh(e * math.sin(phi))))
+        return O.utm_forward(32, False, 9.0 + math.degrees(lam), math.degrees(phi))
+
+    for lat, dlon in [(10.0, 2.5), (48.1374, 2.5755), (60.0, -2.9), (-33.0, 1.7), (75.0, 3.0)]:
+        psi, lam = _isometric_latitude(lat), math.radians(dlon)
+        (xa, ya), (xb, yb) = xy(psi + h, lam), xy(psi - h, lam)
+        (xc, yc), (xd, yd) = xy(psi, lam + h), xy(psi, lam - h)
+        dy_dpsi, dx_dpsi = (ya - yb) / (2 * h), (xa - xb) / (2 * h)
+        dy_dlam, dx_dlam = (yc - yd) / (2 * h), (xc - xd) / (2 * h)
+        scale = math.hypot(dy_dpsi, dx_dpsi)
+        assert abs(dy_dpsi - dx_dlam) < 1e-6 * scale, (lat, dy_dpsi, dx_dlam)
+        assert abs(dy_dlam + dx_dpsi) < 1e-6 * scale, (lat, dy_dlam, dx_dpsi)
+
+
+def test_project_row_matches_per_pixel_munich_and_zone_edge():
+    # output_crs.rs:688-770: 25 000 columns at 10 m with 256 anchors; the
+    # blended row within 1e-6° of the per-pixel inverse at 64 (Munich) and
+    # 128 (western zone edge) evenly spaced columns.  Every column is
+    # checked here.
+    n_cols, step = 25_000, 10.0
+    for x_start, y in ((600_000.0, 5_334_780.0), (285_000.0, 5_540_000.0)):
+        lon, lat = O.project_row(32, False, y, x_start, step, n_cols, 256)
+        assert lon.shape == (n_cols,)
+        xs = x_start + np.arange(n_cols) * step
+        ref = np.array([O.utm_inverse(32, False, float(x), y) for x in xs])
+        assert np.max(np.abs(lon - ref[:, 0])) < 1e-6
+        assert np.max(np.abs(lat - ref[:, 1])) < 1e-6
+    # terrain_correction.rs:1334-1339 quotes the blend at ≤ 2e-7° on S-1 rows
     lon, lat = O.project_row(32, False, 5_300_000.0, 400_005.0, 10.0, 25000, 256)
     for col in (0, 777, 12_345, 24_999):
         elon, elat = O.utm_inverse(32, False, 400_005.0 + col * 10.0, 5_300_000.0)
-        assert abs(lon[col] - elon) < 2e-7 and abs(lat[col] - elat) < 2e-7  # terrain_correction.rs:1334-1339
+        assert abs(lon[col] - elon) < 2e-7 and abs(lat[col] - elat) < 2e-7
+
+
+def test_project_row_single_pixel_and_empty():
+    # output_crs.rs:772-799: one column equals the per-pixel result (1e-12);
+    # zero columns return nothing
+    lon, lat = O.project_row(32, False, 5_334_780.0, 691_603.0, 10.0, 1, 16)
+    elon, elat = O.utm_inverse(32, False, 691_603.0, 5_334_780.0)
+    assert lon.shape == (1,) and abs(lon[0] - elon) < 1e-12 and abs(lat[0] - elat) < 1e-12
+    lon, lat = O.project_row(32, False, 5_000_000.0, 600_000.0, 10.0, 0, 16)
+    assert lon.shape == (0,)
 
 
 # ── calibration (apply_calibration.rs:583-747) ───────────────────────────
