@@ -63,7 +63,7 @@ def parse():
                          "dem: SRTM tile mosaics and a GLO-30 GeoTIFF tile loaded into HBM")
     ap.add_argument("--requests", type=int, default=600, help="serve: requests per run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline work (all runs)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--tc-segment", type=int, default=0, help="columns per lane segment (0 = automatic)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -226,7 +226,7 @@ def main():
     traffic, traffic_src = measured_traffic("tc_segment_kernel", workload)
     roofline = {
         "kernel": "tc_segment_kernel",
-        "bound": "mfma",
+        "bound": "fp64_valu",
         "achieved": round(tflops, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -282,9 +282,37 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
-    """Oracle (FP64 C++ restatement of the reference, 64-row chunks on all
-    host threads) on a bounded sample of the same workload."""
+def host_cpu_share():
+    """Threads the CPU baseline runs on, and a description of the host.
+
+    All CPUs this process may run on (its affinity set), lowered by the job's
+    thread budget when one is set (OMP_NUM_THREADS: a one-GPU job's share of
+    a shared host) and by SARDINE_CPU_THREADS; either variable can only lower
+    the count."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    cores = affinity
+    for var in ("OMP_NUM_THREADS", "SARDINE_CPU_THREADS"):
+        v = os.environ.get(var, "")
+        if v.strip().isdigit() and int(v) > 0:
+            cores = min(cores, int(v))
+    model = "unknown"
+    try:
+        for ln in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, {"cpu_model": model, "host_logical_cpus": os.cpu_count(), "affinity_cpus": affinity}
+
+
+def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s, runs=3):
+    """Oracle (FP64 C++ restatement of the reference, 64-row chunks on every
+    host thread of the job) on a bounded sample of the same workload, the
+    median of `runs` timings of each stage."""
     import numpy as np
     import torch
 
@@ -293,10 +321,10 @@ def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
 
     from sardine_amd import abi
 
-    cores = min(os.cpu_count() or 1, int(os.environ.get("SARDINE_CPU_THREADS", "16")))
+    cores, host = host_cpu_share()
     t = time.time()
     s0h = s0.cpu().numpy()
-    log(f"cpu baseline: merged copied to host in {time.time() - t:.1f}s, {cores} threads")
+    log(f"cpu baseline: merged copied to host in {time.time() - t:.1f}s, {cores} threads on {host}")
     keep = []
     scene_c = sc.scene_c(keep)
     meta = dict(lines=info.lines, samples=info.samples, azimuth_start_time_us=info.azimuth_start_time_us,
@@ -313,9 +341,12 @@ def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
     probe_rows = 64 * cores  # one 64-row chunk per thread, as the full run uses them
     dt, _ = tc_sample(r_mid - probe_rows // 2, probe_rows)
     per_row = dt / probe_rows
-    n_tc = int(max(64 * cores, min(rows // 2, 0.6 * target_s / max(per_row, 1e-6))))
-    dt_tc, res = tc_sample(max(0, r_mid - n_tc // 2), n_tc)
-    tc_per_px = dt_tc / (n_tc * cols)
+    n_tc = int(max(64 * cores, min(rows // 2, 0.6 * target_s / runs / max(per_row, 1e-6))))
+    tc_runs = []
+    for _ in range(runs):
+        dt_tc, res = tc_sample(max(0, r_mid - n_tc // 2), n_tc)
+        tc_runs.append(dt_tc)
+    tc_per_px = float(np.median(tc_runs)) / (n_tc * cols)
     # calibration on the first merged lines of swath 0 (raw DN copied back)
     sw = sc.swaths[0]
     n_cal = 4096
@@ -323,25 +354,32 @@ def cpu_baseline(sc, ds, info, cfg, rows, cols, s0, nz, filt, target_s):
     fv = int(max(sw.first_valid))
     lv = int(min(sw.last_valid))
     deb = np.ascontiguousarray(raw[:, fv:lv])
-    t1 = time.perf_counter()
-    st, *_ = O.apply_calibration(deb, fv, sw.cal.as_c(), sw.noise.as_c(), sw.noise_az.as_c(), 0, threads=cores)
-    dt_cal = time.perf_counter() - t1
-    cal_per_px = dt_cal / deb.shape[0] / deb.shape[1]
-    sp_per_px = 0.0
-    if filt is not None:
+    cal_runs = []
+    for _ in range(runs):
         t1 = time.perf_counter()
-        O.speckle(res.data, abi.SPECKLE_REFINED_LEE, 7, filt.enl, 1.0, threads=cores)
-        dt_sp = time.perf_counter() - t1
-        sp_per_px = dt_sp / res.data.size
+        st, *_ = O.apply_calibration(deb, fv, sw.cal.as_c(), sw.noise.as_c(), sw.noise_az.as_c(), 0, threads=cores)
+        cal_runs.append(time.perf_counter() - t1)
+    cal_per_px = float(np.median(cal_runs)) / deb.shape[0] / deb.shape[1]
+    sp_per_px = 0.0
+    sp_runs = []
+    if filt is not None:
+        for _ in range(runs):
+            t1 = time.perf_counter()
+            O.speckle(res.data, abi.SPECKLE_REFINED_LEE, 7, filt.enl, 1.0, threads=cores)
+            sp_runs.append(time.perf_counter() - t1)
+        sp_per_px = float(np.median(sp_runs)) / res.data.size
     total_s = cal_per_px * info.lines * info.samples + (tc_per_px + sp_per_px) * rows * cols
     value = rows * cols / total_s / 1e6
-    sample = (f"TC on {n_tc} full-width grid rows ({n_tc * cols} px), calibration on {deb.shape[0]}x{deb.shape[1]} "
-              f"DN, Refined Lee on the TC sample; whole job extrapolated per pixel: cal "
-              f"{cal_per_px * 1e9:.1f} ns/merged px, TC {tc_per_px * 1e9:.1f} ns/px, speckle "
-              f"{sp_per_px * 1e9:.1f} ns/px -> {total_s:.1f} s per scene")
+    sample = (f"median of {runs} runs each: TC on {n_tc} full-width grid rows ({n_tc * cols} px), calibration on "
+              f"{deb.shape[0]}x{deb.shape[1]} DN, Refined Lee on the TC sample; whole job extrapolated per pixel: "
+              f"cal {cal_per_px * 1e9:.1f} ns/merged px, TC {tc_per_px * 1e9:.1f} ns/px, speckle "
+              f"{sp_per_px * 1e9:.1f} ns/px -> {total_s:.1f} s per scene; {cores} threads on {host['cpu_model']}")
     log("cpu baseline:", sample)
     return {"value": round(value, 4), "unit": "Mpx/s", "cores": cores, "kind": "port", "sample": sample,
-            "seconds_measured": round(dt + dt_tc + dt_cal + (dt_sp if filt is not None else 0.0), 2)}
+            "runs": runs, **host,
+            "stage_seconds_runs": {"tc": [round(x, 3) for x in tc_runs], "cal": [round(x, 3) for x in cal_runs],
+                                   "speckle": [round(x, 3) for x in sp_runs]},
+            "seconds_measured": round(dt + sum(tc_runs) + sum(cal_runs) + sum(sp_runs), 2)}
 
 
 def copy_calibration(dev) -> None:
