@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SARDINE_ABI_VERSION 1
+#define SARDINE_ABI_VERSION 2
 
 /* ── Status codes ─────────────────────────────────────────────────────────
  * One code per reference error variant on the path:
@@ -120,6 +120,28 @@ typedef enum sardine_status {
 
 /* Thread-local message for the last failing call on this thread. */
 const char* sardine_last_error(void);
+
+/* The fields of the reference's error variants that a status alone does not
+ * carry, for the last failing call on this thread (all zero except `status`
+ * where the variant has no such field):
+ *   SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW  OrbitError::OutsideOrbitWindow
+ *       { requested, first, last } (orbit.rs:58-66, raised at orbit.rs:513-517)
+ *       as µs on the state vectors' clock; for terrain correction `requested`
+ *       is the Newton iterate that fell furthest outside the window (the
+ *       reference reports whichever pixel its threads reach first);
+ *   SARDINE_ERR_TC_ORBIT_INSUFFICIENT    OrbitError::InsufficientCoverage
+ *       (orbit.rs:54-55, raised at orbit.rs:503-511): vectors present/needed;
+ *   SARDINE_ERR_TC_DEM                   TerrainCorrectionError::Dem(DemError)
+ *       (terrain_correction.rs:51, dem.rs:54-109): dem_status names the
+ *       DemError variant (SARDINE_ERR_DEM_*), with BadSize's byte counts. */
+typedef struct sardine_error_detail {
+    int32_t status;
+    int32_t dem_status;
+    int64_t requested_us, first_us, last_us;
+    int32_t have_vectors, need_vectors;
+    uint64_t dem_expected_bytes, dem_actual_bytes;
+} sardine_error_detail;
+sardine_status sardine_last_error_detail(sardine_error_detail* out);
 int sardine_abi_version(void);
 /* Number of visible gfx9xx devices (0 when none); never initialises kernels. */
 int sardine_device_count(void);
@@ -832,9 +854,15 @@ sardine_status sardine_server_create(const sardine_resident_scene* scenes, int32
 sardine_status sardine_server_submit(sardine_server server, const sardine_request* request, uint64_t* ticket);
 /* Copies the request's response; with block != 0 waits until it is done. */
 sardine_status sardine_server_wait(sardine_server server, uint64_t ticket, int32_t block, sardine_response* out);
-/* Frees a finished request's outputs and forgets the ticket. */
-sardine_status sardine_server_release(sardine_server server, uint64_t ticket);
-/* Stops the loop after the requests already admitted have finished. */
+/* Frees a finished request's outputs and forgets the ticket.  The memory is
+ * returned only after all work enqueued on `stream` before this call has
+ * run (the caller's reads of image.data / speckled belong there; NULL is the
+ * null stream); work on other streams that reads the outputs must be
+ * ordered before that point by the caller.  The pointers are invalid once
+ * this returns: later requests reuse the memory. */
+sardine_status sardine_server_release(sardine_server server, uint64_t ticket, void* stream);
+/* Stops the loop after the requests already admitted have finished; waits
+ * for all work on the device before it frees outputs still unreleased. */
 void sardine_server_destroy(sardine_server server);
 
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */

@@ -34,6 +34,7 @@ def lib():
     P, i32, i64, d, f, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_float, C.c_uint64
     L.sardine_last_error.restype = C.c_char_p
     L.sardine_last_error.argtypes = []
+    L.sardine_last_error_detail.argtypes = [C.POINTER(abi.ErrorDetail)]
     L.sardine_abi_version.restype = C.c_int
     L.sardine_device_count.restype = C.c_int
     L.sardine_tc_config_default.argtypes = [C.POINTER(abi.TcConfig)]
@@ -97,7 +98,7 @@ def lib():
     L.sardine_server_create.argtypes = [PP, i32, C.POINTER(abi.ServerOptions), PP]
     L.sardine_server_submit.argtypes = [P, C.POINTER(abi.Request), C.POINTER(u64)]
     L.sardine_server_wait.argtypes = [P, u64, i32, C.POINTER(abi.Response)]
-    L.sardine_server_release.argtypes = [P, u64]
+    L.sardine_server_release.argtypes = [P, u64, P]
     L.sardine_server_destroy.argtypes = [P]
     L.sardine_server_destroy.restype = None
     for name in abi.EXPORTED_SYMBOLS:
@@ -115,3 +116,9 @@ def lib():
 
 def last_error() -> str:
     return lib().sardine_last_error().decode(errors="replace")
+
+
+def last_error_detail() -> "abi.ErrorDetail":
+    d = abi.ErrorDetail()
+    lib().sardine_last_error_detail(C.byref(d))
+    return d

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 NEWTON_HIST_SIZE = 16
 MAX_BURSTS = 64
 MAX_SWATHS = 8
@@ -259,9 +259,16 @@ class CogLayout(C.Structure):
                 ("geo_keys_offset", C.c_uint64), ("level", CogLevel * COG_MAX_LEVELS)]
 
 
+class ErrorDetail(C.Structure):
+    _fields_ = [("status", C.c_int32), ("dem_status", C.c_int32), ("requested_us", C.c_int64),
+                ("first_us", C.c_int64), ("last_us", C.c_int64), ("have_vectors", C.c_int32),
+                ("need_vectors", C.c_int32), ("dem_expected_bytes", C.c_uint64), ("dem_actual_bytes", C.c_uint64)]
+
+
 # Exported symbols of libsardine_hip.so (every function include/sardine_hip.h declares).
 EXPORTED_SYMBOLS = (
     "sardine_last_error",
+    "sardine_last_error_detail",
     "sardine_abi_version",
     "sardine_device_count",
     "sardine_tc_config_default",

@@ -35,7 +35,7 @@ from typing import Optional
 import numpy as np
 
 from . import abi
-from ._native import last_error, lib
+from ._native import last_error, last_error_detail, lib
 from .errors import SpeckleError, raise_for
 
 
@@ -53,7 +53,7 @@ def _stream_ptr(stream) -> Optional[int]:
 
 def _check(status: int):
     if status != abi.OK:
-        raise_for(status, last_error())
+        raise_for(status, last_error(), last_error_detail())
 
 
 # ─── inputs ───────────────────────────────────────────────────────────────
@@ -876,8 +876,13 @@ class Server:
         return ServedResult(r.status, img, sp, r.t_submit_us, r.t_admit_us, r.t_launch_us, r.t_done_us,
                             r.batch_size)
 
-    def release(self, ticket: int):
-        _check(lib().sardine_server_release(self.handle, ticket))
+    def release(self, ticket: int, stream=None):
+        """Returns the request's output memory once the work enqueued so far
+        on `stream` (default: torch's current stream) has run; its tensors
+        must not be used afterwards."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(lib().sardine_server_release(self.handle, ticket, C.c_void_p(s.cuda_stream)))
 
     def close(self):
         if self.handle:

@@ -80,6 +80,7 @@ extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t s
 namespace {
 
 thread_local std::string g_last_error;
+thread_local sardine_error_detail g_last_detail;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
@@ -146,6 +147,8 @@ sardine_status fail(sardine_status st, const char* fmt, ...) {
     std::vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     g_last_error = buf;
+    std::memset(&g_last_detail, 0, sizeof g_last_detail);
+    g_last_detail.status = st;
     return st;
 }
 
@@ -156,6 +159,21 @@ sardine_status fail(sardine_status st, const char* fmt, ...) {
             return fail(_e == hipErrorOutOfMemory ? SARDINE_ERR_OUT_OF_MEMORY : SARDINE_ERR_HIP,      \
                         "%s failed: %s", #call, hipGetErrorString(_e));                               \
     } while (0)
+
+// Host exceptions never cross the C ABI: an allocation failure or any other
+// std::exception raised by host code (file parsing, planning, staging)
+// becomes a status and a message.
+template <typename F>
+sardine_status guarded(sardine_status on_error, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(on_error == SARDINE_ERR_INVALID_ARGUMENT ? SARDINE_ERR_OUT_OF_MEMORY : on_error,
+                    "host memory allocation failed");
+    } catch (const std::exception& e) {
+        return fail(on_error, "%s", e.what());
+    }
+}
 
 // Device presence + gfx950 check, once per process.
 sardine_status require_device() {
@@ -556,8 +574,24 @@ sardine_status validate_tc(const sardine_radar_image* merged, const sardine_scen
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "radar image has no raster");
     if (merged->lines >= INT32_MAX || merged->samples >= INT32_MAX)
         return fail(SARDINE_ERR_CAPACITY, "radar image dimensions must be below 2^31");
-    if (!dem->data || dem->width < 2 || dem->height < 2 || !(dem->pixel_deg > 0.0))
-        return fail(SARDINE_ERR_TC_DEM, "DEM raster must be at least 2x2 with positive spacing");
+    if (!dem->data || dem->width < 2 || dem->height < 2 || !(dem->pixel_deg > 0.0)) {
+        // TerrainCorrectionError::Dem(DemError::…): no raster is NoTiles, a
+        // raster below 2 x 2 samples BadSize, a bad spacing TiffDecode
+        const sardine_status st = fail(SARDINE_ERR_TC_DEM, "DEM error: %s",
+                                       !dem->data ? "no tiles loaded"
+                                       : (dem->width < 2 || dem->height < 2)
+                                           ? "DEM raster must be at least 2x2 samples"
+                                           : "DEM raster spacing must be positive");
+        g_last_detail.dem_status = !dem->data ? SARDINE_ERR_DEM_NO_TILES
+                                   : (dem->width < 2 || dem->height < 2) ? SARDINE_ERR_DEM_BAD_SIZE
+                                                                         : SARDINE_ERR_DEM_TIFF_DECODE;
+        if (g_last_detail.dem_status == SARDINE_ERR_DEM_BAD_SIZE) {
+            g_last_detail.dem_expected_bytes = 2 * 2 * sizeof(float);
+            g_last_detail.dem_actual_bytes =
+                (uint64_t)std::max<int64_t>(dem->width, 0) * (uint64_t)std::max<int64_t>(dem->height, 0) * sizeof(float);
+        }
+        return st;
+    }
     if (dem->width >= INT32_MAX || dem->height >= INT32_MAX)
         return fail(SARDINE_ERR_CAPACITY, "DEM dimensions must be below 2^31");
     if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 && !cfg->geoid.grid_73x144)
@@ -772,14 +806,34 @@ void assign_work(TcPlan& pl, const TcWork& w, char* base) {
 }
 
 // Host result of one job from its counters (GeocodedImage fields).
-sardine_status finish_tc(const TcPlan& pl, const unsigned long long* cnt, int n_sv, sardine_geocoded_image* out) {
+sardine_status finish_tc(const TcPlan& pl, const unsigned long long* cnt, const sardine_orbit& orbit,
+                         sardine_geocoded_image* out) {
     const int err = (int)(cnt[sd::kNumCounters] & 0xffffffffull);
-    if (err & 1)
-        return fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW,
-                    "orbit interpolation failed: interpolation time outside orbit window; refusing to extrapolate");
-    if (err & 2)
-        return fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT,
-                    "orbit interpolation failed: need at least 8 vectors for interpolation, have %d", n_sv);
+    const int n_sv = orbit.count;
+    if (err & 1) {
+        // OutsideOrbitWindow { requested, first, last } (orbit.rs:513-517);
+        // requested is the iterate furthest outside the window
+        const int64_t first = orbit.state_vectors[0].time_us, last = orbit.state_vectors[n_sv - 1].time_us;
+        const int64_t req = cnt[sd::kCntOrbitBelow] ? first - (int64_t)cnt[sd::kCntOrbitBelow]
+                                                    : last + (int64_t)cnt[sd::kCntOrbitAbove];
+        const sardine_status st =
+            fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW,
+                 "orbit interpolation failed: interpolation time %lld outside orbit window [%lld, %lld]; refusing "
+                 "to extrapolate", (long long)req, (long long)first, (long long)last);
+        g_last_detail.requested_us = req;
+        g_last_detail.first_us = first;
+        g_last_detail.last_us = last;
+        return st;
+    }
+    if (err & 2) {
+        const sardine_status st =
+            fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT,
+                 "orbit interpolation failed: EOF orbit does not cover scene: need at least 8 vectors for "
+                 "interpolation, have %d", n_sv);
+        g_last_detail.have_vectors = n_sv;
+        g_last_detail.need_vectors = sd::kLagrange;
+        return st;
+    }
     const Grid& g = pl.g;
     const double sp = pl.p.spacing;
     out->rows = pl.nr;
@@ -811,6 +865,11 @@ sardine_status finish_tc(const TcPlan& pl, const unsigned long long* cnt, int n_
 extern "C" {
 
 const char* sardine_last_error(void) { return g_last_error.c_str(); }
+sardine_status sardine_last_error_detail(sardine_error_detail* out) {
+    if (!out) return SARDINE_ERR_INVALID_ARGUMENT;
+    *out = g_last_detail;
+    return SARDINE_OK;
+}
 void sardine_set_profiling(int on) {
     g_profile = on != 0;
     if (!g_profile) return;
@@ -886,16 +945,32 @@ sardine_status sardine_orbit_interpolate(const sardine_orbit* orbit, int64_t tim
     const int64_t epoch = orbit->count > 0 ? orbit->state_vectors[0].time_us : 0;
     sd::d3 P, V, A;
     const int e = sd::orbit_interp(*o, time_us - epoch, &P, &V, &A);
-    if (e == sd::ORB_INSUFFICIENT)
-        return fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT, "need at least 8 state vectors, have %d", orbit->count);
-    if (e == sd::ORB_OUTSIDE) return fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW, "time outside the orbit window");
+    if (e == sd::ORB_INSUFFICIENT) {
+        const sardine_status st = fail(SARDINE_ERR_TC_ORBIT_INSUFFICIENT,
+                                       "EOF orbit does not cover scene: need at least 8 vectors for interpolation, "
+                                       "have %d", orbit->count);
+        g_last_detail.have_vectors = orbit->count;
+        g_last_detail.need_vectors = sd::kLagrange;
+        return st;
+    }
+    if (e == sd::ORB_OUTSIDE) {
+        const int64_t first = orbit->state_vectors[0].time_us, last = orbit->state_vectors[orbit->count - 1].time_us;
+        const sardine_status st =
+            fail(SARDINE_ERR_TC_ORBIT_OUTSIDE_WINDOW,
+                 "interpolation time %lld outside orbit window [%lld, %lld]; refusing to extrapolate",
+                 (long long)time_us, (long long)first, (long long)last);
+        g_last_detail.requested_us = time_us;
+        g_last_detail.first_us = first;
+        g_last_detail.last_us = last;
+        return st;
+    }
     pos[0] = P.x; pos[1] = P.y; pos[2] = P.z;
     vel[0] = V.x; vel[1] = V.y; vel[2] = V.z;
     acc[0] = A.x; acc[1] = A.y; acc[2] = A.z;
     return SARDINE_OK;
 }
 
-sardine_status sardine_terrain_correction(const sardine_radar_image* merged, const sardine_scene* scene,
+static sardine_status sardine_terrain_correction_impl(const sardine_radar_image* merged, const sardine_scene* scene,
                                           const sardine_dem_raster* dem, const sardine_tc_config* cfg,
                                           sardine_geocoded_image* out, void* stream_v) {
     if (!merged || !scene || !dem || !cfg || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
@@ -926,8 +1001,8 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
     SD_HIP(d_work.alloc(w.total(), stream));
     assign_work(pl, w, d_work.as<char>());
     if (w.geoid) SD_HIP(staging().copy(const_cast<float*>(pl.p.geoid), cfg->geoid.grid_73x144, 73 * 144 * sizeof(float), stream));
-    SD_HIP(d_cnt.alloc((sd::kNumCounters + 1) * sizeof(unsigned long long), stream));
-    SD_HIP(sd_launch_zero_u64(d_cnt.as<unsigned long long>(), sd::kNumCounters + 1, stream));
+    SD_HIP(d_cnt.alloc(sd::kCntSlots * sizeof(unsigned long long), stream));
+    SD_HIP(sd_launch_zero_u64(d_cnt.as<unsigned long long>(), sd::kCntSlots, stream));
     pl.p.counters = d_cnt.as<unsigned long long>();
     pl.p.error_flag = reinterpret_cast<int*>(d_cnt.as<unsigned long long>() + sd::kNumCounters);
     SD_HIP(sd_launch_tc_rows(&pl.q, stream));
@@ -962,10 +1037,17 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged, con
         if (seg) seg->pending = true;
         kt.stop();
     }
-    unsigned long long cnt[sd::kNumCounters + 1];
+    unsigned long long cnt[sd::kCntSlots];
     SD_HIP(hipMemcpyAsync(cnt, d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost, stream));
     SD_HIP(hipStreamSynchronize(stream));
-    return finish_tc(pl, cnt, orb.n, out);
+    return finish_tc(pl, cnt, scene->orbit, out);
+}
+sardine_status sardine_terrain_correction(const sardine_radar_image* merged, const sardine_scene* scene,
+                                          const sardine_dem_raster* dem, const sardine_tc_config* cfg,
+                                          sardine_geocoded_image* out, void* stream_v) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_terrain_correction_impl(merged, scene, dem, cfg, out, stream_v);
+    });
 }
 
 // ── speckle (speckle.rs:131-180) ─────────────────────────────────────────
@@ -992,7 +1074,7 @@ sardine_status sardine_speckle_apply(const float* data, int64_t data_len, int64_
 }
 
 // ── deburst layout (deburst.rs:172-305) ──────────────────────────────────
-sardine_status sardine_deburst_layout_compute(const sardine_subswath* sw, const sardine_burst* b, int32_t n,
+static sardine_status sardine_deburst_layout_compute_impl(const sardine_subswath* sw, const sardine_burst* b, int32_t n,
                                               int64_t tiff_height, sardine_deburst_layout* out) {
     if (!sw || !out || (n > 0 && !b)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     std::memset(out, 0, sizeof *out);
@@ -1041,6 +1123,12 @@ sardine_status sardine_deburst_layout_compute(const sardine_subswath* sw, const 
     out->samples = lv - fv;
     out->valid_sample_offset = fv;
     return SARDINE_OK;
+}
+sardine_status sardine_deburst_layout_compute(const sardine_subswath* sw, const sardine_burst* b, int32_t n,
+                                              int64_t tiff_height, sardine_deburst_layout* out) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_deburst_layout_compute_impl(sw, b, n, tiff_height, out);
+    });
 }
 
 }  // extern "C"
@@ -1156,7 +1244,7 @@ sardine_status stage_swath(const sardine_lut* cal, const sardine_lut* nr, int64_
 
 extern "C" {
 
-sardine_status sardine_prepare_merged_layout(const sardine_swath_input* sw, int32_t n, sardine_merged_info* info) {
+static sardine_status sardine_prepare_merged_layout_impl(const sardine_swath_input* sw, int32_t n, sardine_merged_info* info) {
     if (!info || (n > 0 && !sw)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     std::memset(info, 0, sizeof *info);
     if (n <= 0) return fail(SARDINE_ERR_MERGE_EMPTY_INPUTS, "no subswaths to merge");
@@ -1210,8 +1298,11 @@ sardine_status sardine_prepare_merged_layout(const sardine_swath_input* sw, int3
     info->n_swaths = n;
     return SARDINE_OK;
 }
+sardine_status sardine_prepare_merged_layout(const sardine_swath_input* sw, int32_t n, sardine_merged_info* info) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] { return sardine_prepare_merged_layout_impl(sw, n, info); });
+}
 
-sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, float* sigma0, float* nesz,
+static sardine_status sardine_prepare_merged_impl(const sardine_swath_input* sw, int32_t n, float* sigma0, float* nesz,
                                       sardine_merged_info* info, void* stream_v) {
     sardine_status st = sardine_prepare_merged_layout(sw, n, info);
     if (st) return st;
@@ -1264,8 +1355,14 @@ sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, 
     }
     return SARDINE_OK;  // stream-ordered: the work buffers are released after the kernel
 }
+sardine_status sardine_prepare_merged(const sardine_swath_input* sw, int32_t n, float* sigma0, float* nesz,
+                                      sardine_merged_info* info, void* stream_v) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_prepare_merged_impl(sw, n, sigma0, nesz, info, stream_v);
+    });
+}
 
-sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, int64_t samples, int64_t col_off,
+static sardine_status sardine_apply_calibration_impl(const int16_t* deburst, int64_t lines, int64_t samples, int64_t col_off,
                                          const sardine_lut* cal, const sardine_lut* nr, const sardine_az_lut* az,
                                          int64_t origin, float* out, float* nesz, uint32_t* cal_gap, uint32_t* nr_gap,
                                          void* stream_v) {
@@ -1308,6 +1405,14 @@ sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, 
     D.valid_end = samples;
     SD_HIP(sd_launch_cal_merge(&P, s));
     return SARDINE_OK;
+}
+sardine_status sardine_apply_calibration(const int16_t* deburst, int64_t lines, int64_t samples, int64_t col_off,
+                                         const sardine_lut* cal, const sardine_lut* nr, const sardine_az_lut* az,
+                                         int64_t origin, float* out, float* nesz, uint32_t* cal_gap, uint32_t* nr_gap,
+                                         void* stream_v) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_apply_calibration_impl(deburst, lines, samples, col_off, cal, nr, az, origin, out, nesz, cal_gap, nr_gap, stream_v);
+    });
 }
 
 sardine_status sardine_to_db_inplace(float* data, int64_t n, float floor_lin, uint64_t counts[2], void* stream_v) {
@@ -1404,7 +1509,7 @@ std::vector<std::string> split_ws(const std::string& line) {
 
 extern "C" {
 
-sardine_status sardine_geoid_load_binary(const char* path, int32_t model, float grid[10512]) {
+static sardine_status sardine_geoid_load_binary_impl(const char* path, int32_t model, float grid[10512]) {
     if (!path || !grid) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or grid is null");
     if (model != SARDINE_GEOID_MODEL_EGM96 && model != SARDINE_GEOID_MODEL_EGM2008)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "unknown geoid model %d", model);
@@ -1417,8 +1522,11 @@ sardine_status sardine_geoid_load_binary(const char* path, int32_t model, float 
     std::memcpy(grid, raw.data(), raw.size());  // little-endian host, as the file
     return SARDINE_OK;
 }
+sardine_status sardine_geoid_load_binary(const char* path, int32_t model, float grid[10512]) {
+    return guarded(SARDINE_ERR_GEOID_IO, [&] { return sardine_geoid_load_binary_impl(path, model, grid); });
+}
 
-sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]) {
+static sardine_status sardine_geoid_load_ww15mgh_impl(const char* path, float grid[10512]) {
     if (!path || !grid) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or grid is null");
     std::string text;
     if (!read_file(path, &text)) return fail(SARDINE_ERR_GEOID_IO, "IO error reading %s: %s", path, std::strerror(errno));
@@ -1454,6 +1562,9 @@ sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]) {
                     kGeoidValues, values.size());
     std::memcpy(grid, values.data(), kGeoidValues * sizeof(float));
     return SARDINE_OK;
+}
+sardine_status sardine_geoid_load_ww15mgh(const char* path, float grid[10512]) {
+    return guarded(SARDINE_ERR_GEOID_IO, [&] { return sardine_geoid_load_ww15mgh_impl(path, grid); });
 }
 
 }  // extern "C"
@@ -1669,15 +1780,18 @@ sardine_status file_to_device(int fd, uint64_t base, uint64_t bytes, uint8_t* ou
 
 extern "C" {
 
-sardine_status sardine_slc_open(const char* path, sardine_slc_tiff* tiff) {
+static sardine_status sardine_slc_open_impl(const char* path, sardine_slc_tiff* tiff) {
     if (!path || !tiff) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or tiff is null");
     Fd f;
     f.fd = ::open(path, O_RDONLY | O_CLOEXEC);
     if (f.fd < 0) return fail(SARDINE_ERR_SLC_IO, "I/O error: %s: %s", path, std::strerror(errno));
     return slc_parse(f.fd, path, tiff);
 }
+sardine_status sardine_slc_open(const char* path, sardine_slc_tiff* tiff) {
+    return guarded(SARDINE_ERR_SLC_IO, [&] { return sardine_slc_open_impl(path, tiff); });
+}
 
-sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* t, int64_t first_line,
+static sardine_status sardine_slc_read_lines_impl(const char* path, const sardine_slc_tiff* t, int64_t first_line,
                                       int64_t line_count, int16_t* dst, void* stream_v) {
     if (!path || !t) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or tiff is null");
     if (first_line < 0 || line_count < 0 || first_line > (int64_t)t->height ||
@@ -1701,8 +1815,14 @@ sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* 
     if (!t->little_endian) SD_HIP(sd_launch_swap16(reinterpret_cast<uint16_t*>(dst), (int64_t)(bytes / 2), s));
     return SARDINE_OK;
 }
+sardine_status sardine_slc_read_lines(const char* path, const sardine_slc_tiff* t, int64_t first_line,
+                                      int64_t line_count, int16_t* dst, void* stream_v) {
+    return guarded(SARDINE_ERR_SLC_IO, [&] {
+        return sardine_slc_read_lines_impl(path, t, first_line, line_count, dst, stream_v);
+    });
+}
 
-sardine_status sardine_slc_read_lines_multi(const char* const* paths, const sardine_slc_tiff* tiffs,
+static sardine_status sardine_slc_read_lines_multi_impl(const char* const* paths, const sardine_slc_tiff* tiffs,
                                             int32_t n_slices, int64_t first_line, int64_t line_count, int16_t* dst,
                                             void* stream) {
     if (n_slices <= 0) return fail(SARDINE_ERR_SLC_MULTI_EMPTY_INPUT, "MultiSlcReader requires at least one slice reader");
@@ -1729,6 +1849,13 @@ sardine_status sardine_slc_read_lines_multi(const char* const* paths, const sard
                     "burst read [%lld, %lld) crosses a slice boundary at logical line %lld",
                     (long long)first_line, (long long)(first_line + line_count), (long long)slice_end);
     return sardine_slc_read_lines(paths[k], &tiffs[k], first_line - off[(size_t)k], line_count, dst, stream);
+}
+sardine_status sardine_slc_read_lines_multi(const char* const* paths, const sardine_slc_tiff* tiffs,
+                                            int32_t n_slices, int64_t first_line, int64_t line_count, int16_t* dst,
+                                            void* stream) {
+    return guarded(SARDINE_ERR_SLC_IO, [&] {
+        return sardine_slc_read_lines_multi_impl(paths, tiffs, n_slices, first_line, line_count, dst, stream);
+    });
 }
 
 }  // extern "C"
@@ -1786,7 +1913,7 @@ bool srtm_tile_name(const std::string& stem, int* lat, int* lon) {
 
 extern "C" {
 
-sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths, float* dst,
+static sardine_status sardine_dem_srtm_mosaic_impl(const char* const* paths, int32_t n_paths, float* dst,
                                        sardine_dem_raster* out, void* stream) {
     if (!out || (n_paths > 0 && !paths)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     if (n_paths <= 0) return fail(SARDINE_ERR_DEM_NO_TILES, "no DEM tiles");
@@ -1839,9 +1966,13 @@ sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths
     // each tile's bytes go file → pinned chunks → HBM (the next tile's reads
     // overlap this tile's copies and decode), then the big-endian decode
     // places it in the mosaic; the tile buffer is reused in stream order
+    // Tiles are placed last to first, so where tiles overlap (the shared
+    // edge rows and columns of adjacent SRTM tiles) the first tile in the
+    // list supplies the value, as DemMosaic's lookup answers from the first
+    // covering tile.
     DevBuf tile;
     SD_HIP(tile.alloc(bytes, s));
-    for (int32_t i = 0; i < n_paths; ++i) {
+    for (int32_t i = n_paths - 1; i >= 0; --i) {
         Fd f;
         f.fd = ::open(paths[i], O_RDONLY | O_CLOEXEC);
         if (f.fd < 0) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s", paths[i]);
@@ -1856,6 +1987,12 @@ sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths
     }
     *out = r;
     return SARDINE_OK;
+}
+sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths, float* dst,
+                                       sardine_dem_raster* out, void* stream) {
+    return guarded(SARDINE_ERR_DEM_IO, [&] {
+        return sardine_dem_srtm_mosaic_impl(paths, n_paths, dst, out, stream);
+    });
 }
 
 }  // extern "C"
@@ -1909,7 +2046,7 @@ T sat_cast(float x) {
 
 extern "C" {
 
-sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_raster* out, void* stream) {
+static sardine_status sardine_dem_glo30_tile_impl(const char* path, float* dst, sardine_dem_raster* out, void* stream) {
     if (!path || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or out is null");
     sd::GeoTiffRaster g;
     std::string why;
@@ -1983,6 +2120,9 @@ sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_
     kt.stop();
     *out = r;
     return SARDINE_OK;
+}
+sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_raster* out, void* stream) {
+    return guarded(SARDINE_ERR_DEM_TIFF_DECODE, [&] { return sardine_dem_glo30_tile_impl(path, dst, out, stream); });
 }
 
 }  // extern "C"
@@ -2287,7 +2427,7 @@ GrdPlanCache& grd_plan_cache() {
 
 extern "C" {
 
-sardine_status sardine_to_ground_range(const float* data, const sardine_merged_info* merged,
+static sardine_status sardine_to_ground_range_impl(const float* data, const sardine_merged_info* merged,
                                        const sardine_grd_grid_point* points, const int32_t* counts,
                                        int32_t n_swaths, double target_spacing_m, double azimuth_pixel_spacing_m,
                                        float* out, sardine_grd_image* image, void* stream) {
@@ -2316,6 +2456,14 @@ sardine_status sardine_to_ground_range(const float* data, const sardine_merged_i
     kt.stop();
     *image = img;
     return SARDINE_OK;
+}
+sardine_status sardine_to_ground_range(const float* data, const sardine_merged_info* merged,
+                                       const sardine_grd_grid_point* points, const int32_t* counts,
+                                       int32_t n_swaths, double target_spacing_m, double azimuth_pixel_spacing_m,
+                                       float* out, sardine_grd_image* image, void* stream) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_to_ground_range_impl(data, merged, points, counts, n_swaths, target_spacing_m, azimuth_pixel_spacing_m, out, image, stream);
+    });
 }
 
 }  // extern "C"
@@ -2799,26 +2947,46 @@ sardine_status sardine_geotiff_header(int64_t cols, int64_t rows, int32_t bytes_
     return SARDINE_OK;
 }
 
-sardine_status sardine_write_geotiff_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+static sardine_status sardine_write_geotiff_with_crs_impl(const char* path, const float* data, int64_t cols, int64_t rows,
                                               const double geotransform[6], int32_t crs, int32_t utm_zone,
                                               void* stream) {
     if (!geotransform) return fail(SARDINE_ERR_INVALID_ARGUMENT, "geotransform is null");
     return striped_write(path, data, cols, rows, 4, geotransform, crs, utm_zone, stream);
 }
+sardine_status sardine_write_geotiff_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+                                              const double geotransform[6], int32_t crs, int32_t utm_zone,
+                                              void* stream) {
+    return guarded(SARDINE_ERR_EXPORT_IO, [&] {
+        return sardine_write_geotiff_with_crs_impl(path, data, cols, rows, geotransform, crs, utm_zone, stream);
+    });
+}
 
-sardine_status sardine_write_geotiff_u8_with_crs(const char* path, const uint8_t* data, int64_t cols, int64_t rows,
+static sardine_status sardine_write_geotiff_u8_with_crs_impl(const char* path, const uint8_t* data, int64_t cols, int64_t rows,
                                                  const double geotransform[6], int32_t crs, int32_t utm_zone,
                                                  void* stream) {
     if (!geotransform) return fail(SARDINE_ERR_INVALID_ARGUMENT, "geotransform is null");
     return striped_write(path, data, cols, rows, 1, geotransform, crs, utm_zone, stream);
 }
+sardine_status sardine_write_geotiff_u8_with_crs(const char* path, const uint8_t* data, int64_t cols, int64_t rows,
+                                                 const double geotransform[6], int32_t crs, int32_t utm_zone,
+                                                 void* stream) {
+    return guarded(SARDINE_ERR_EXPORT_IO, [&] {
+        return sardine_write_geotiff_u8_with_crs_impl(path, data, cols, rows, geotransform, crs, utm_zone, stream);
+    });
+}
 
-sardine_status sardine_write_tiff_no_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+static sardine_status sardine_write_tiff_no_crs_impl(const char* path, const float* data, int64_t cols, int64_t rows,
                                          void* stream) {
     return striped_write(path, data, cols, rows, 4, nullptr, 0, 0, stream);
 }
+sardine_status sardine_write_tiff_no_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+                                         void* stream) {
+    return guarded(SARDINE_ERR_EXPORT_IO, [&] {
+        return sardine_write_tiff_no_crs_impl(path, data, cols, rows, stream);
+    });
+}
 
-sardine_status sardine_write_cog_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+static sardine_status sardine_write_cog_with_crs_impl(const char* path, const float* data, int64_t cols, int64_t rows,
                                           const double geotransform[6], int32_t crs, int32_t utm_zone,
                                           uint32_t tile_size, void* stream_v) {
     sardine_status st;
@@ -2850,6 +3018,13 @@ sardine_status sardine_write_cog_with_crs(const char* path, const float* data, i
     fc.f = nullptr;
     if (rc != 0) return fail(SARDINE_ERR_EXPORT_IO, "closing %s: %s", path, std::strerror(errno));
     return SARDINE_OK;
+}
+sardine_status sardine_write_cog_with_crs(const char* path, const float* data, int64_t cols, int64_t rows,
+                                          const double geotransform[6], int32_t crs, int32_t utm_zone,
+                                          uint32_t tile_size, void* stream_v) {
+    return guarded(SARDINE_ERR_EXPORT_IO, [&] {
+        return sardine_write_cog_with_crs_impl(path, data, cols, rows, geotransform, crs, utm_zone, tile_size, stream_v);
+    });
 }
 
 }  // extern "C"
@@ -2898,7 +3073,7 @@ struct sardine_batch_s {
     sd::BatchJob* d_J = nullptr;
     void* d_SH = nullptr;
     void* d_SJ = nullptr;
-    unsigned long long* d_cnt = nullptr;  // max_jobs x (kNumCounters + 1)
+    unsigned long long* d_cnt = nullptr;  // max_jobs x kCntSlots
     unsigned long long* d_queue = nullptr;  // [0] chain items, [1] speckle tiles
     void* h_mem = nullptr;
     sd::BatchHeader* h_H = nullptr;
@@ -2915,7 +3090,7 @@ struct sardine_batch_s {
 
 namespace {
 
-size_t cnt_stride() { return sd::kNumCounters + 1; }
+size_t cnt_stride() { return sd::kCntSlots; }
 
 // The launch sequence of one run (eager or under capture).
 hipError_t batch_launches(sardine_batch_s* b, int max_half, hipStream_t s) {
@@ -3216,7 +3391,7 @@ sardine_status batch_end(sardine_batch_s* b, const sardine_resident_scene* scene
         int k = 0;
         for (int i : run.run_idx) {
             const sardine_status st = finish_tc(run.plans[(size_t)i], b->h_cnt + (size_t)k * cnt_stride(),
-                                                scenes[jobs[i].scene]->scene.orbit.count, jobs[i].out);
+                                                scenes[jobs[i].scene]->scene.orbit, jobs[i].out);
             jobs[i].status = st;
             note_failure(run, st);
             ++k;
@@ -3231,7 +3406,7 @@ sardine_status batch_end(sardine_batch_s* b, const sardine_resident_scene* scene
 
 extern "C" {
 
-sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+static sardine_status sardine_batch_run_impl(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
                                  sardine_batch_job* jobs, int32_t n_jobs, void* stream_v, double* device_ms) {
     BatchRun run;
     sardine_status st = batch_begin(b, scenes, n_scenes, jobs, n_jobs, (hipStream_t)stream_v, run);
@@ -3241,8 +3416,14 @@ sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* 
     }
     return batch_end(b, scenes, jobs, run, device_ms);
 }
+sardine_status sardine_batch_run(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                 sardine_batch_job* jobs, int32_t n_jobs, void* stream_v, double* device_ms) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_batch_run_impl(b, scenes, n_scenes, jobs, n_jobs, stream_v, device_ms);
+    });
+}
 
-sardine_status sardine_batch_autotune(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+static sardine_status sardine_batch_autotune_impl(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
                                       sardine_batch_job* jobs, int32_t n_jobs, void* stream,
                                       sardine_batch_tuning* best, double* best_ms) {
     if (!b) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
@@ -3295,6 +3476,13 @@ sardine_status sardine_batch_autotune(sardine_batch b, const sardine_resident_sc
     if (best) *best = chosen;
     if (best_ms) *best_ms = chosen_ms;
     return SARDINE_OK;
+}
+sardine_status sardine_batch_autotune(sardine_batch b, const sardine_resident_scene* scenes, int32_t n_scenes,
+                                      sardine_batch_job* jobs, int32_t n_jobs, void* stream,
+                                      sardine_batch_tuning* best, double* best_ms) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
+        return sardine_batch_autotune_impl(b, scenes, n_scenes, jobs, n_jobs, stream, best, best_ms);
+    });
 }
 
 }  // extern "C"
@@ -3503,7 +3691,7 @@ sardine_status sardine_server_create(const sardine_resident_scene* scenes, int32
     return SARDINE_OK;
 }
 
-sardine_status sardine_server_submit(sardine_server sv, const sardine_request* r, uint64_t* ticket) {
+static sardine_status sardine_server_submit_impl(sardine_server sv, const sardine_request* r, uint64_t* ticket) {
     if (!sv || !r || !ticket) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     std::unique_ptr<ServerReq> q(new ServerReq());
     q->req = *r;
@@ -3521,6 +3709,9 @@ sardine_status sardine_server_submit(sardine_server sv, const sardine_request* r
     sv->cv_work.notify_one();
     return SARDINE_OK;
 }
+sardine_status sardine_server_submit(sardine_server sv, const sardine_request* r, uint64_t* ticket) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] { return sardine_server_submit_impl(sv, r, ticket); });
+}
 
 sardine_status sardine_server_wait(sardine_server sv, uint64_t ticket, int32_t block, sardine_response* out) {
     if (!sv || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
@@ -3533,7 +3724,7 @@ sardine_status sardine_server_wait(sardine_server sv, uint64_t ticket, int32_t b
     return SARDINE_OK;
 }
 
-sardine_status sardine_server_release(sardine_server sv, uint64_t ticket) {
+sardine_status sardine_server_release(sardine_server sv, uint64_t ticket, void* stream_v) {
     if (!sv) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> g(sv->mu);
     auto it = sv->reqs.find(ticket);
@@ -3541,6 +3732,14 @@ sardine_status sardine_server_release(sardine_server sv, uint64_t ticket) {
     ServerReq* q = it->second.get();
     if (!q->resp.done) return fail(SARDINE_ERR_INVALID_ARGUMENT, "request %llu has not finished", (unsigned long long)ticket);
     if (q->d_out) {
+        // the free waits for the caller's work on `stream` enqueued so far
+        // (its reads of the outputs), then the memory returns to the pool
+        hipEvent_t ev = nullptr;
+        SD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        hipError_t e = hipEventRecord(ev, (hipStream_t)stream_v);
+        if (e == hipSuccess) e = hipStreamWaitEvent(sv->release_stream, ev, 0);
+        (void)hipEventDestroy(ev);
+        SD_HIP(e);
         (void)hipFreeAsync(q->d_out, sv->release_stream);
         sv->used -= q->out_bytes;
     }
@@ -3564,6 +3763,9 @@ void sardine_server_destroy(sardine_server sv) {
         sv->cv_done.notify_all();
     }
     if (sv->loop.joinable()) sv->loop.join();
+    // outputs never released may still be read by the caller's queued work
+    (void)hipSetDevice(sv->device);
+    (void)hipDeviceSynchronize();
     for (auto& kv : sv->reqs)
         if (kv.second->d_out) (void)hipFreeAsync(kv.second->d_out, sv->release_stream);
     (void)hipStreamSynchronize(sv->release_stream);

@@ -19,6 +19,14 @@ namespace {
 
 // worker threads for block decoding (a share of a host's cores)
 constexpr uint64_t kDecodeThreads = 16;
+// Bounds on what a header may ask the decoder to allocate: the decoded image
+// (a GLO-30 tile is 3600 x 3600 x 4 B = 52 MB), one block, and a block side.
+constexpr uint64_t kMaxImageBytes = 1ull << 32;
+constexpr uint64_t kMaxBlockBytes = 1ull << 28;
+constexpr uint64_t kMaxBlockSide = 1ull << 16;
+
+// a * b, false when it does not fit in 64 bits
+bool mul_ok(uint64_t a, uint64_t b, uint64_t* r) { return !__builtin_mul_overflow(a, b, r); }
 
 struct File {
     std::vector<uint8_t> b;
@@ -53,8 +61,8 @@ int type_size(uint16_t t) {
 // Where an entry's values start: inline in the value field when they fit.
 bool values_at(const File& f, const Entry& e, uint64_t* off) {
     const int sz = type_size(e.type);
-    if (!sz) return false;
-    const uint64_t n = e.count * (uint64_t)sz;
+    uint64_t n;
+    if (!sz || !mul_ok(e.count, (uint64_t)sz, &n)) return false;
     const int field = f.big ? 8 : 4;
     *off = n <= (uint64_t)field ? e.field : f.get(e.field, field);
     return f.has(*off, n);
@@ -70,9 +78,12 @@ bool ints(const File& f, const Entry& e, std::vector<uint64_t>* out) {
     return true;
 }
 
+// The entry's values as doubles; on failure `out` is left empty (an entry of
+// another type counts as absent, never as zeros).
 bool doubles(const File& f, const Entry& e, std::vector<double>* out) {
+    out->clear();
     uint64_t off;
-    if (!values_at(f, e, &off)) return false;
+    if (!values_at(f, e, &off) || (e.type != 11 && e.type != 12)) return false;
     out->resize(e.count);
     for (uint64_t i = 0; i < e.count; ++i) {
         if (e.type == 12) {
@@ -83,8 +94,6 @@ bool doubles(const File& f, const Entry& e, std::vector<double>* out) {
             float x;
             std::memcpy(&x, &u, 4);
             (*out)[i] = x;
-        } else {
-            return false;
         }
     }
     return true;
@@ -189,7 +198,9 @@ bool inflate_all(const uint8_t* in, size_t n, std::vector<uint8_t>& out, size_t 
 
 }  // namespace
 
-int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool decode, bool defer_fp_predictor) {
+namespace {
+
+int read_geotiff_impl(const char* path, GeoTiffRaster* out, std::string* err, bool decode, bool defer_fp_predictor) {
     File f;
     {
         std::FILE* fp = std::fopen(path, "rb");
@@ -233,7 +244,7 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
     const int cnt_size = f.big ? 8 : 2, ent_size = f.big ? 20 : 12;
     if (!f.has(ifd, cnt_size)) return bad("IFD outside the file");
     const uint64_t n = f.get(ifd, cnt_size);
-    if (!f.has(ifd + cnt_size, n * ent_size)) return bad("IFD entries outside the file");
+    if (n > f.b.size() / ent_size || !f.has(ifd + cnt_size, n * ent_size)) return bad("IFD entries outside the file");
     std::vector<Entry> ents;
     for (uint64_t i = 0; i < n; ++i) {
         const uint64_t o = ifd + cnt_size + i * ent_size;
@@ -309,6 +320,15 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
         bh = std::min<uint64_t>(std::max<uint64_t>(rps, 1), h);
     }
     if (bw == 0 || bh == 0) return bad_block("zero block size");
+    // every size below is checked before anything is allocated from it
+    uint64_t img_bytes, block_bytes;
+    if (!mul_ok(w, h, &img_bytes) || !mul_ok(img_bytes, (uint64_t)bps, &img_bytes) || img_bytes > kMaxImageBytes)
+        return bad_block("image of " + std::to_string(w) + " x " + std::to_string(h) + " samples is too large");
+    // (a strip is never larger than the image; a tile may be, so its sides
+    // and bytes have their own bounds)
+    if ((tiled && (bw > kMaxBlockSide || bh > kMaxBlockSide)) || !mul_ok(bw, bh, &block_bytes) ||
+        !mul_ok(block_bytes, (uint64_t)bps, &block_bytes) || (tiled && block_bytes > kMaxBlockBytes))
+        return bad_block("block of " + std::to_string(bw) + " x " + std::to_string(bh) + " samples is too large");
     const uint64_t nx = (w + bw - 1) / bw, ny = (h + bh - 1) / bh;
     if (offs.size() != nx * ny || counts.size() != nx * ny) return bad_block("block count does not match the image");
     if (comp != 1 && comp != 5 && comp != 8 && comp != 32946 && comp != 32773)
@@ -316,26 +336,29 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
     if (pred != 1 && pred != 2 && pred != 3) return bad_block("unsupported predictor " + std::to_string(pred));
     const bool defer = defer_fp_predictor && pred == 3 && bps == 4 && out->sample == TiffSample::F32 &&
                        (int64_t)(bw * bps) <= kMaxDeferredRowBytes;
+    uint64_t deferred_bytes = 0;
+    if (defer && (!mul_ok(nx * ny, block_bytes, &deferred_bytes) || deferred_bytes > kMaxImageBytes))
+        return bad_block("tiled image is too large");
     if (defer) {
         out->fp_deferred = true;
         out->block_w = (int64_t)bw;
         out->block_h = (int64_t)bh;
         out->blocks_x = (int64_t)nx;
         out->blocks_y = (int64_t)ny;
-        out->pixels.assign((size_t)(nx * ny * bh * bw * bps), 0);
+        out->pixels.assign((size_t)deferred_bytes, 0);
     } else {
-        out->pixels.assign((size_t)(w * h * bps), 0);
+        out->pixels.assign((size_t)img_bytes, 0);
     }
     // Blocks decode independently into disjoint parts of the image, so they
     // are spread over worker threads; the error reported is the first block's
     // in file order, as a sequential decode would report it.
     const uint64_t n_blocks = nx * ny;
-    std::vector<uint8_t> fault((size_t)n_blocks, 0);  // 1: outside the file, 2: corrupt
+    std::vector<uint8_t> fault((size_t)n_blocks, 0);  // 1: outside the file, 2: corrupt, 3: out of memory
     auto decode_block = [&](uint64_t k, std::vector<uint8_t>& blk, std::vector<uint8_t>& tmp) -> uint8_t {
         const uint64_t by = k / nx, bx = k % nx;
         // a strip holds its own rows only; a tile is always full size
         const uint64_t rows = tiled ? bh : std::min(bh, h - by * bh);
-        const size_t want = (size_t)(rows * bw * bps);
+        const size_t want = (size_t)(rows * bw * bps);  // <= block_bytes, checked above
         if (!f.has(offs[k], counts[k])) return 1;
         const uint8_t* src = &f.b[offs[k]];
         const size_t sn = (size_t)counts[k];
@@ -390,7 +413,13 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
     std::atomic<uint64_t> next{0};
     auto worker = [&]() {
         std::vector<uint8_t> blk, tmp;
-        for (uint64_t k; (k = next.fetch_add(1)) < n_blocks;) fault[(size_t)k] = decode_block(k, blk, tmp);
+        for (uint64_t k; (k = next.fetch_add(1)) < n_blocks;) {
+            try {  // nothing may escape a worker thread (std::terminate)
+                fault[(size_t)k] = decode_block(k, blk, tmp);
+            } catch (const std::exception&) {
+                fault[(size_t)k] = 3;
+            }
+        }
     };
     const uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
     const uint64_t n_threads = std::min<uint64_t>(std::min<uint64_t>(hw, kDecodeThreads), n_blocks);
@@ -401,8 +430,23 @@ int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool de
     for (uint64_t k = 0; k < n_blocks; ++k) {
         if (fault[(size_t)k] == 1) return bad_block("block outside the file");
         if (fault[(size_t)k] == 2) return bad_block("corrupt compressed block");
+        if (fault[(size_t)k] == 3) return bad_block("out of memory decoding a block");
     }
     return 0;
+}
+
+}  // namespace
+
+int read_geotiff(const char* path, GeoTiffRaster* out, std::string* err, bool decode, bool defer_fp_predictor) {
+    try {
+        return read_geotiff_impl(path, out, err, decode, defer_fp_predictor);
+    } catch (const std::bad_alloc&) {
+        *err = "out of memory reading the file";
+    } catch (const std::exception& e) {
+        *err = e.what();
+    }
+    out->pixels.clear();
+    return 3;
 }
 
 }  // namespace sd

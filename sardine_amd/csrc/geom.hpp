@@ -223,7 +223,10 @@ SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter,
             A = cache->A;
         } else {
             int e = orbit_interp(o, t, &P, &V, &A);
-            if (e == ORB_OUTSIDE) return ZD_ORBIT_OUTSIDE;
+            if (e == ORB_OUTSIDE) {
+                *t_out = t;  // the requested time, for the error report
+                return ZD_ORBIT_OUTSIDE;
+            }
             if (e == ORB_INSUFFICIENT) return ZD_ORBIT_INSUFFICIENT;
             cache->t = t;
             cache->P = P;

@@ -93,7 +93,10 @@ __device__ __forceinline__ int zero_doppler32(const OrbitF32& o, f3 T, float t0,
     for (int i = 0; i < max_iter; ++i) {
         f3 P, V, A;
         const int e = orbit_interp32(o, t, &P, &V, &A);
-        if (e == ORB_OUTSIDE) return ZD_ORBIT_OUTSIDE;
+        if (e == ORB_OUTSIDE) {
+            *t_out = t;  // the requested time, for the error report
+            return ZD_ORBIT_OUTSIDE;
+        }
         if (e == ORB_INSUFFICIENT) return ZD_ORBIT_INSUFFICIENT;
         const f3 r = T - P;
         const float f = dot3(r, V);
@@ -412,8 +415,12 @@ __global__ __launch_bounds__(256) void tc32_kernel(TcParams p, const float* ends
                 const double flat_area = p.need_geom ? p.row_flat_area[rr] : 0.0;
                 v = finish32<RS>(p, q, T, oc, t, sat, flat_area);
                 if (v.cat == CAT_ORBIT) {
-                    if (oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
-                    else C.err_short += 1;
+                    if (oc == ZD_ORBIT_OUTSIDE) {
+                        C.err_outside += 1;
+                        note_orbit_outside(p, *p.orbit, p.t_ref_us + (int64_t)llrint((double)t * 1e6));
+                    } else {
+                        C.err_short += 1;
+                    }
                 } else {
                     count(C, v.cat, it, 1, p.counters);
                 }

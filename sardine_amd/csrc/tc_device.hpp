@@ -240,6 +240,15 @@ __device__ __forceinline__ void zero_counts(LaneCounts& C) {
     C.err_outside = C.err_short = 0;
 }
 
+// Records how far an iterate outside the orbit window reached beyond it
+// (OutsideOrbitWindow's `requested`, orbit.rs:513-517).
+__device__ __forceinline__ void note_orbit_outside(const TcParams& p, const OrbitDev& o, int64_t t) {
+    if (t < o.t_us[0])
+        atomicMax(&p.counters[kCntOrbitBelow], (unsigned long long)(o.t_us[0] - t));
+    else
+        atomicMax(&p.counters[kCntOrbitAbove], (unsigned long long)(t - o.t_us[o.n - 1]));
+}
+
 // Wave-reduce each lane's counts and add them to the global counters.
 __device__ __forceinline__ void flush_counts(const TcParams& p, const LaneCounts& C, bool set_errors) {
     unsigned long long* G = p.counters;
@@ -505,8 +514,12 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
             PixelOut v = finish_pixel<RS>(p, q, s.oc, s.t, s.sat, inv_flat_area);
             if (row_ok) {
                 if (v.cat == CAT_ORBIT) {
-                    if (s.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
-                    else C.err_short += 1;
+                    if (s.oc == ZD_ORBIT_OUTSIDE) {
+                        C.err_outside += 1;
+                        note_orbit_outside(p, orb, s.t);
+                    } else {
+                        C.err_short += 1;
+                    }
                 } else {
                     count(C, v.cat, s.it, 1, p.counters);
                 }
@@ -644,8 +657,12 @@ __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb
                         count(C, vo.cat, o.it, -1, p.counters);
                     }
                     if (vn.cat == CAT_ORBIT) {
-                        if (n.oc == ZD_ORBIT_OUTSIDE) C.err_outside += 1;
-                        else C.err_short += 1;
+                        if (n.oc == ZD_ORBIT_OUTSIDE) {
+                            C.err_outside += 1;
+                            note_orbit_outside(p, orb, n.t);
+                        } else {
+                            C.err_short += 1;
+                        }
                     } else {
                         count(C, vn.cat, n.it, 1, p.counters);
                     }

@@ -10,6 +10,12 @@ namespace sd {
 constexpr int kHistSize = 16;                // NEWTON_ITER_HIST_SIZE (terrain_correction.rs:95)
 constexpr int kCntRedo = 7 + kHistSize;      // counters[kCntRedo] = chain redo rounds
 constexpr int kNumCounters = kCntRedo + 1;
+// After the counters: the orbit-error flags word, then how far the Newton
+// iterates that fell outside the orbit window reached below its first and
+// above its last state vector (µs, 0 = none).
+constexpr int kCntOrbitBelow = kNumCounters + 1;
+constexpr int kCntOrbitAbove = kNumCounters + 2;
+constexpr int kCntSlots = kNumCounters + 3;
 constexpr int kMaxAnchorPts = 1025;          // SARDINE_TC_PROJECTOR_ANCHORS ≤ 1024
 constexpr size_t kMaxTcLds = 64 * 1024;      // dynamic LDS budget of the UTM anchors
 

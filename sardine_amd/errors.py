@@ -18,14 +18,31 @@ from . import abi
 
 
 class SardineError(RuntimeError):
-    def __init__(self, status: int, message: str):
+    def __init__(self, status: int, message: str, detail=None):
         self.status = status
         self.variant = abi.STATUS_NAMES.get(status, str(status))
         super().__init__(f"{self.variant}: {message}")
 
 
 class TerrainCorrectionError(SardineError):
-    pass
+    """With the fields of the reference's nested variants:
+    Orbit(OutsideOrbitWindow { requested, first, last }) → `requested_us`,
+    `first_us`, `last_us` (µs on the state vectors' clock);
+    Orbit(InsufficientCoverage) → `have_vectors`, `need_vectors`;
+    Dem(DemError::…) → `dem_variant` (a DemError status name)."""
+
+    def __init__(self, status: int, message: str, detail=None):
+        super().__init__(status, message)
+        self.requested_us = self.first_us = self.last_us = None
+        self.have_vectors = self.need_vectors = None
+        self.dem_variant = None
+        if detail is not None and detail.status == status:
+            if status == abi.ERR_TC_ORBIT_OUTSIDE_WINDOW:
+                self.requested_us, self.first_us, self.last_us = detail.requested_us, detail.first_us, detail.last_us
+            elif status == abi.ERR_TC_ORBIT_INSUFFICIENT:
+                self.have_vectors, self.need_vectors = detail.have_vectors, detail.need_vectors
+            elif status == abi.ERR_TC_DEM:
+                self.dem_variant = abi.STATUS_NAMES.get(detail.dem_status, str(detail.dem_status))
 
 
 class SpeckleError(SardineError):
@@ -68,11 +85,11 @@ class DeviceError(SardineError):
     pass
 
 
-def raise_for(status: int, message: str):
+def raise_for(status: int, message: str, detail=None):
     if status == abi.OK:
         return
     if 10 <= status < 20:
-        raise TerrainCorrectionError(status, message)
+        raise TerrainCorrectionError(status, message, detail)
     if 20 <= status < 30:
         raise SpeckleError(status, message)
     if 30 <= status < 40:

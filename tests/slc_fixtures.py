@@ -97,12 +97,13 @@ def write_hgt(path, tile: np.ndarray) -> None:
 
 
 def mosaic(tiles: dict) -> tuple:
-    """{(lat_sw, lon_sw): tile} → (raster float32 with NaN voids/holes, north, west)."""
+    """{(lat_sw, lon_sw): tile} → (raster float32 with NaN voids/holes, north, west).
+    Where tiles overlap (shared edges) the first tile in dict order wins."""
     lats = [k[0] for k in tiles]
     lons = [k[1] for k in tiles]
     north, south, west, east = max(lats) + 1, min(lats), min(lons), max(lons) + 1
     out = np.full(((north - south) * 3600 + 1, (east - west) * 3600 + 1), np.nan, np.float32)
-    for (la, lo), t in tiles.items():
+    for (la, lo), t in reversed(list(tiles.items())):
         f = t.astype(np.float32)
         f[t == -32768] = np.nan
         r0, c0 = (north - (la + 1)) * 3600, (lo - west) * 3600

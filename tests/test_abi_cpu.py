@@ -127,9 +127,47 @@ def test_orbit_model_matches_oracle():
         assert max(abs(float(ma[d] - exact_a[d])) for d in range(3)) <= 1e-12, t
     assert L.sardine_orbit_interpolate(C.byref(orb), t1 + 1, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == \
         abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
+    # OutsideOrbitWindow { requested, first, last } (orbit.rs:58-66, 513-517)
+    d = _native.last_error_detail()
+    assert d.status == abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
+    assert (d.requested_us, d.first_us, d.last_us) == (t1 + 1, int(sc.sv_t_us[0]), int(sc.sv_t_us[-1]))
+    assert L.sardine_orbit_interpolate(C.byref(orb), t0 - 7, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == \
+        abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
+    assert _native.last_error_detail().requested_us == t0 - 7
     short = O.orbit_c(sc.sv_t_us[:7], sc.sv_pos[:7], sc.sv_vel[:7], keep)
     assert L.sardine_orbit_interpolate(C.byref(short), t0 + 5, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == \
         abi.ERR_TC_ORBIT_INSUFFICIENT
+    d = _native.last_error_detail()
+    assert (d.status, d.have_vectors, d.need_vectors, d.requested_us) == (abi.ERR_TC_ORBIT_INSUFFICIENT, 7, 8, 0)
+    # a later failure of another kind resets the detail
+    assert L.sardine_orbit_interpolate(None, 0, _np_ptr(p), _np_ptr(v), _np_ptr(a)) == abi.ERR_INVALID_ARGUMENT
+    d = _native.last_error_detail()
+    assert (d.status, d.have_vectors) == (abi.ERR_INVALID_ARGUMENT, 0)
+
+
+def test_tc_dem_error_carries_the_dem_variant():
+    """TerrainCorrectionError::Dem(DemError::…) (terrain_correction.rs:51):
+    an unusable DEM raster is reported with the DemError variant (host
+    validation, before the device check)."""
+    from sardine_amd import TerrainCorrectionError
+    sc = synthetic.scene_small_plain(0, lines=64, samples=64, out_rows=8, out_cols=8, dem_side=8)
+    keep = []
+    scene = sc.scene_c(keep)
+    img = abi.RadarImage(1, 0, 64, 64, int(sc.sv_t_us[0]) + 10_000_000, 0.0053, float(sc.rps_m))
+    cfg = sc.tc_config()
+    out = abi.GeocodedImage()
+    L = _native.lib()
+    for dem, want in ((abi.DemRaster(0, 8, 8, 50.0, 10.0, 1e-3), abi.ERR_DEM_NO_TILES),
+                      (abi.DemRaster(1, 1, 8, 50.0, 10.0, 1e-3), abi.ERR_DEM_BAD_SIZE),
+                      (abi.DemRaster(1, 8, 8, 50.0, 10.0, 0.0), abi.ERR_DEM_TIFF_DECODE)):
+        st = L.sardine_terrain_correction(C.byref(img), C.byref(scene), C.byref(dem), C.byref(cfg), C.byref(out), None)
+        assert st == abi.ERR_TC_DEM
+        d = _native.last_error_detail()
+        assert (d.status, d.dem_status) == (abi.ERR_TC_DEM, want)
+        if want == abi.ERR_DEM_BAD_SIZE:
+            assert (d.dem_expected_bytes, d.dem_actual_bytes) == (16, 32)
+        e = TerrainCorrectionError(st, _native.last_error(), d)
+        assert e.dem_variant == abi.STATUS_NAMES[want]
 
 
 def _np_ptr(x):
@@ -689,3 +727,100 @@ def test_glo30_decode_errors_in_file_order(tmp_path):
         q.write_bytes(bytes(b))
         assert L.sardine_dem_glo30_tile(str(q).encode(), buf.ctypes.data, C.byref(r), None) == \
             abi.ERR_DEM_UNRECOGNISED_TILE_NAME
+
+
+def _crafted_tiff(entries, big=False, tail=b""):
+    """A little-endian TIFF (classic or BigTIFF) with one IFD of raw
+    (tag, type, count, field-bytes) entries, sorted by tag, then `tail`."""
+    import struct
+
+    if big:
+        out = bytearray(b"II") + struct.pack("<HHHQ", 43, 8, 0, 16) + struct.pack("<Q", len(entries))
+        for tag, typ, count, field in sorted(entries):
+            out += struct.pack("<HHQ", tag, typ, count) + field.ljust(8, b"\0")
+        out += struct.pack("<Q", 0)
+    else:
+        out = bytearray(b"II") + struct.pack("<HI", 42, 8) + struct.pack("<H", len(entries))
+        for tag, typ, count, field in sorted(entries):
+            out += struct.pack("<HHI", tag, typ, count) + field.ljust(4, b"\0")
+        out += struct.pack("<I", 0)
+    return bytes(out + tail)
+
+
+def _u32(*v):
+    import struct
+    return struct.pack("<" + "I" * len(v), *v)
+
+
+def _u16(v):
+    import struct
+    return struct.pack("<H", v)
+
+
+@pytest.mark.parametrize("case", ["huge_tiles", "huge_image", "bigtiff_count_wrap", "bigtiff_ifd_wrap",
+                                  "huge_strip_count"])
+def test_glo30_crafted_headers_are_decode_errors(tmp_path, case):
+    """Headers whose sizes overflow 64-bit products or ask for absurd
+    allocations are TIFF decode errors (the reference's `tiff` crate refuses
+    them with an error), never a crash: the reader checks every count x size,
+    block and image product before allocating, and host exceptions do not
+    cross the C ABI.  Decode errors come before the device check, so no GPU
+    is needed."""
+    L = _native.lib()
+    base = [(258, 3, 1, _u16(32)), (259, 3, 1, _u16(1)), (277, 3, 1, _u16(1)), (339, 3, 1, _u16(3))]
+    if case == "huge_tiles":  # 2^31 x 2^31 tiles of a 16 x 16 image: 2^64 bytes per tile wraps to 0
+        ents = base + [(256, 4, 1, _u32(16)), (257, 4, 1, _u32(16)), (322, 4, 1, _u32(1 << 31)),
+                       (323, 4, 1, _u32(1 << 31)), (324, 4, 1, _u32(200)), (325, 4, 1, _u32(16))]
+        data = _crafted_tiff(ents, tail=b"\0" * 256)
+    elif case == "huge_image":  # 2^30 x 2^30 samples of 8 bytes: 2^63 bytes
+        ents = [(256, 4, 1, _u32(1 << 30)), (257, 4, 1, _u32(1 << 30)), (258, 3, 1, _u16(64)),
+                (259, 3, 1, _u16(1)), (273, 4, 1, _u32(0)), (277, 3, 1, _u16(1)), (278, 4, 1, _u32(1 << 30)),
+                (279, 4, 1, _u32(8)), (339, 3, 1, _u16(3))]
+        data = _crafted_tiff(ents)
+    elif case == "bigtiff_count_wrap":  # ModelPixelScale with 2^61 doubles: count x 8 wraps to 0
+        import struct
+        ents = base + [(256, 4, 1, _u32(4)), (257, 4, 1, _u32(4)), (273, 4, 1, _u32(0)),
+                       (279, 4, 1, _u32(64)), (33550, 12, 1 << 61, struct.pack("<Q", 0))]
+        data = _crafted_tiff(ents, big=True)
+    elif case == "bigtiff_ifd_wrap":  # an IFD entry count whose x 20 wraps
+        import struct
+        data = bytes(bytearray(b"II") + struct.pack("<HHHQ", 43, 8, 0, 16) + struct.pack("<Q", (1 << 64) // 20 + 1)
+                     + b"\0" * 64)
+    else:  # one strip per row of a 2^30-row image, but a 4-entry strip table
+        ents = base + [(256, 4, 1, _u32(1 << 30)), (257, 4, 1, _u32(1 << 30)), (273, 4, 4, _u32(200)),
+                       (278, 4, 1, _u32(1)), (279, 4, 4, _u32(216))]
+        data = _crafted_tiff(ents, tail=b"\0" * 64)
+    p = tmp_path / "N50E010.tif"
+    p.write_bytes(data)
+    r = abi.DemRaster()
+    if case == "bigtiff_count_wrap":
+        # the unreadable scale counts as absent (no read past the buffer):
+        # the georeferencing comes from the name
+        assert L.sardine_dem_glo30_tile(str(p).encode(), None, C.byref(r), None) == 0
+        assert r.pixel_deg == 1 / 4 and (r.width, r.height) == (4, 4)
+        return
+    buf = np.zeros(4, np.float32)
+    st = L.sardine_dem_glo30_tile(str(p).encode(), buf.ctypes.data, C.byref(r), None)
+    assert st == abi.ERR_DEM_TIFF_DECODE, (case, st, _native.last_error())
+
+
+def test_glo30_mistyped_georeferencing_counts_as_absent(tmp_path):
+    """A ModelPixelScale / ModelTiepoint of a non-floating type is not read
+    as zeros: the tile falls back to the name's georeferencing."""
+    import slc_fixtures as F
+    L = _native.lib()
+    a = np.full((36, 36), 5.0, np.float32)
+    raw = bytearray(F.geotiff_dem_bytes(a, scale=(1.0, 1.0, 0.0), tiepoint=(0, 0, 0, 3.0, 4.0, 0)))
+    import tiffread as T
+    import struct
+    ifd = T.parse(bytes(raw))["ifds"][0]
+    n = struct.unpack_from("<H", raw, ifd["offset"])[0]
+    for i in range(n):  # retype both tags as SHORT (3)
+        e = ifd["offset"] + 2 + 12 * i
+        if struct.unpack_from("<H", raw, e)[0] in (33550, 33922):
+            struct.pack_into("<H", raw, e + 2, 3)
+    p = tmp_path / "N50E010.tif"
+    p.write_bytes(bytes(raw))
+    r = abi.DemRaster()
+    assert L.sardine_dem_glo30_tile(str(p).encode(), None, C.byref(r), None) == 0
+    assert r.pixel_deg == 1 / 36 and r.origin_lat_deg == 51 - 0.5 / 36 and r.origin_lon_deg == 10 + 0.5 / 36

@@ -2,6 +2,9 @@
 dem.rs:628-640, as one device raster): the assembled raster equals the
 tiles' values at their places (big-endian decode, voids and holes NaN), and
 the terrain correction's DEM lookup on it gives the per-tile heights."""
+import pathlib
+import zlib
+
 import numpy as np
 import pytest
 
@@ -16,11 +19,17 @@ pytestmark = pytest.mark.gpu
 def test_srtm_mosaic_raster(cuda, tmp_path, layout):
     print(f"srtm mosaic {layout}", flush=True)
     tiles = {}
-    for la, lo in layout:
-        tiles[(la, lo)] = F.srtm_field(la, lo, [(7, 9), (3600, 3600)] if (la, lo) == layout[0] else [])
+    # voids in the first tile: one inside, one a sample away from the corner
+    # it shares with its neighbours (on a shared edge the per-tile lookup
+    # moves on to the next tile when a tap is void, which one joined raster
+    # cannot do; real SRTM tiles agree on their shared edges)
+    for la, lo in sorted(layout):
+        tiles[(la, lo)] = F.srtm_field(la, lo, [(7, 9), (3599, 3599)] if (la, lo) == min(layout) else [])
         ns, ew = ("N" if la >= 0 else "S"), ("E" if lo >= 0 else "W")
         F.write_hgt(tmp_path / f"{ns}{abs(la):02d}{ew}{abs(lo):03d}.hgt", tiles[(la, lo)])
     paths = sorted(str(p) for p in tmp_path.glob("*.hgt"))
+    assert [pathlib.Path(p).stem for p in paths] == [
+        f"{'N' if la >= 0 else 'S'}{abs(la):02d}{'E' if lo >= 0 else 'W'}{abs(lo):03d}" for la, lo in sorted(layout)]
     dem = api.load_srtm_mosaic(paths, device=cuda)
     want, north, west = F.mosaic(tiles)
     got = dem.data.cpu().numpy()
@@ -28,14 +37,24 @@ def test_srtm_mosaic_raster(cuda, tmp_path, layout):
     assert got.shape == want.shape and np.array_equal(got, want, equal_nan=True)
     # the device raster under the TC's DEM rule vs the per-tile lookup
     rng = np.random.default_rng(len(layout))
-    la0, lo0 = layout[0]
+    la0, lo0 = min(layout)
     lat = rng.uniform(la0 + 0.001, la0 + 0.999, 300)
     lon = rng.uniform(lo0 + 0.001, lo0 + 0.999, 300)
+    # points around the voids too
+    lat = np.concatenate([lat, la0 + 1 - (7 + rng.uniform(-1, 1, 20)) / 3600, la0 + 1 - (3599 + rng.uniform(-1, 1, 20)) / 3600])
+    lon = np.concatenate([lon, lo0 + (9 + rng.uniform(-1, 1, 20)) / 3600, lo0 + (3599 + rng.uniform(-1, 1, 20)) / 3600])
     st, ref, ok = O.srtm_sample(paths, lat, lon)
     spec = abi.DemRaster(0, got.shape[1], got.shape[0], north, west, 1.0 / 3600.0)
     single = np.array([np.nan if (v := O.dem_elevation(got, spec, a, b)) is None else v for a, b in zip(lat, lon)],
                       np.float32)
     assert st == 0 and np.array_equal(np.isfinite(single), ok.astype(bool))
+    assert (~ok.astype(bool)).sum() >= 2  # the voids were hit
+    # heights: the joined raster's row coordinate (north - lat) / pix and the
+    # tile's (1 - frac) * 3600 differ in the last bit of the f64 weight, so
+    # the f32 heights agree to 1 ulp
+    both = np.isfinite(single) & ok.astype(bool)
+    ulps = np.abs(single[both].view(np.int32).astype(np.int64) - ref[both].astype(np.float32).view(np.int32))
+    assert ulps.max(initial=0) <= 1
 
 
 def _expect(a, nodata):
@@ -67,7 +86,7 @@ CASES = [(dt, comp, pred, tile, le, big)
 
 @pytest.mark.parametrize("dt,comp,pred,tile,le,big", CASES)
 def test_glo30_tile_decode(cuda, tmp_path, dt, comp, pred, tile, le, big):
-    rng = np.random.default_rng(hash((dt, comp, pred)) % 1000)
+    rng = np.random.default_rng(zlib.crc32(f"{dt}/{comp}/{pred}".encode()))
     shape = (131, 97)
     if dt == "f4":
         a = rng.normal(800, 300, shape).astype(np.float32)

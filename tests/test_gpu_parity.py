@@ -286,6 +286,10 @@ def test_orbit_window_error_aborts(small):
     with pytest.raises(TerrainCorrectionError) as e:
         api.terrain_correction(m, scene, ds.dem, ds.tc_config())
     assert getattr(e.value, "status", None) == abi.ERR_TC_ORBIT_OUTSIDE_WINDOW
+    # Orbit(OutsideOrbitWindow { requested, first, last }): the window of the
+    # 9 vectors, and an iterate past its end
+    assert (e.value.first_us, e.value.last_us) == (int(scene.sv_time_us[0]), int(scene.sv_time_us[-1]))
+    assert e.value.requested_us > e.value.last_us
 
 
 @pytest.mark.parametrize("min_seg,lead", [(1, 2), (5, 2), (64, 2), (100000, 2), (5, 0), (16, 0), (32, 1)])

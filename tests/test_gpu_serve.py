@@ -103,3 +103,31 @@ def test_server_rejects_oversized_and_bad_scene(scenes):
     assert srv.wait(t).status == 0
     srv.release(t)
     srv.close()
+
+
+def test_release_waits_for_queued_reads(scenes):
+    """A result released right after the client enqueued a read of it (behind
+    long-running work on its stream) is read intact: the server returns the
+    memory only after that stream's queued work, even though the next
+    requests reuse it at once."""
+    import torch
+
+    res_scenes = [s[3] for s in scenes]
+    reqs = _requests(scenes, 12, seed=11)
+    srv = api.Server(res_scenes, hbm_budget_bytes=1 << 30, max_batch=4, slots=2)
+    ref = _alone(scenes, *reqs[0])[0].data.cpu().numpy()
+    t = srv.submit(*reqs[0])
+    r = srv.wait(t)
+    assert r.status == 0
+    a = torch.randn(4096, 4096, device="cuda")
+    for _ in range(40):  # tens of ms of queued work ahead of the read
+        a = (a @ a) * (1.0 / 64.0)
+    copy = r.image.data.clone()
+    srv.release(t)
+    others = [srv.submit(*q) for q in reqs[1:]]  # these outputs reuse the freed memory
+    for o in others:
+        assert srv.wait(o).status == 0
+        srv.release(o)
+    torch.cuda.synchronize()
+    assert np.array_equal(copy.cpu().numpy(), ref, equal_nan=True)
+    srv.close()
