@@ -338,6 +338,24 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged,
                                           sardine_geocoded_image* out,
                                           void* stream);
 
+/* sardine_terrain_correction without device memory allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and results as sardine_terrain_correction;
+ * every device buffer of the call is a slice of a fixed 64 MiB scratch array
+ * that each GPU holds until the process ends (the caller still passes the
+ * inputs and outputs).  A call whose work needs more scratch (records, row
+ * seeds and anchors grow with the AOI: about 0.5-2 bytes per pixel plus 4 KiB
+ * per UTM row) returns SARDINE_ERR_CAPACITY.  All calls on one GPU share its
+ * array: calls that overlap on one GPU (from several threads or streams)
+ * race; calls in sequence do not, since each returns after its work. */
+sardine_status sardine_terrain_correction_noalloc(const sardine_radar_image* merged,
+                                                  const sardine_scene* scene,
+                                                  const sardine_dem_raster* dem,
+                                                  const sardine_tc_config* cfg,
+                                                  sardine_geocoded_image* out,
+                                                  void* stream);
+
 /* ── Speckle ────────────────────────────────────────────────────────────── */
 
 /* SpeckleFilter, speckle.rs:61-91 */

@@ -45,6 +45,7 @@ def lib():
     L.sardine_terrain_correction.argtypes = [C.POINTER(abi.RadarImage), C.POINTER(abi.Scene),
                                              C.POINTER(abi.DemRaster), C.POINTER(abi.TcConfig),
                                              C.POINTER(abi.GeocodedImage), P]
+    L.sardine_terrain_correction_noalloc.argtypes = L.sardine_terrain_correction.argtypes
     L.sardine_speckle_apply.argtypes = [P, i64, i64, i64, C.POINTER(abi.SpeckleFilterC), P, P]
     L.sardine_deburst_layout_compute.argtypes = [C.POINTER(abi.SubSwath), C.POINTER(abi.Burst), i32, i64,
                                                  C.POINTER(abi.DeburstLayout)]

@@ -275,6 +275,7 @@ EXPORTED_SYMBOLS = (
     "sardine_tc_output_grid",
     "sardine_orbit_interpolate",
     "sardine_terrain_correction",
+    "sardine_terrain_correction_noalloc",
     "sardine_speckle_apply",
     "sardine_deburst_layout_compute",
     "sardine_prepare_merged_layout",

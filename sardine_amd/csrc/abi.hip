@@ -175,6 +175,41 @@ sardine_status guarded(sardine_status on_error, F&& body) {
     }
 }
 
+// Scratch of the no-allocation terrain-correction entry point: every device
+// buffer of such a call (counters, orbit, seed table, chain records, row
+// seeds, anchors, flat areas, geoid) is a slice of this array.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code must
+// not use the array's name as a device address: its address on a device
+// comes from hipGetSymbolAddress (scratch_on_current_device).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+sardine_status scratch_on_current_device(char** out) {
+    int dev = 0;
+    const hipError_t ge = hipGetDevice(&dev);
+    if (ge != hipSuccess) return fail(SARDINE_ERR_HIP, "hipGetDevice failed: %s", hipGetErrorString(ge));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(SARDINE_ERR_CAPACITY, "device %d is beyond the %d devices the scratch table holds", dev,
+                    kMaxScratchDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SARDINE_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<char*>(g_scratch_ptr[dev]);
+    return SARDINE_OK;
+}
+
 // Device presence + gfx950 check, once per process.
 sardine_status require_device() {
     static int state = -1;  // -1 unknown, 0 ok, 1 none
@@ -970,11 +1005,53 @@ sardine_status sardine_orbit_interpolate(const sardine_orbit* orbit, int64_t tim
     return SARDINE_OK;
 }
 
-static sardine_status sardine_terrain_correction_impl(const sardine_radar_image* merged, const sardine_scene* scene,
-                                          const sardine_dem_raster* dem, const sardine_tc_config* cfg,
-                                          sardine_geocoded_image* out, void* stream_v) {
+// Device memory of one terrain-correction call: stream-ordered allocations,
+// or consecutive 256-byte aligned slices of a fixed region (the
+// no-allocation entry point's scratch array).
+class TcMemory {
+   public:
+    TcMemory(hipStream_t s, char* fixed, size_t cap) : s_(s), fixed_(fixed), cap_(cap) {}
+    bool fixed() const { return fixed_ != nullptr; }
+    size_t used() const { return used_; }
+    // n bytes, or nullptr with *e set (hipErrorOutOfMemory when the fixed
+    // region is full)
+    void* get(size_t n, hipError_t* e) {
+        *e = hipSuccess;
+        if (fixed_) {
+            const size_t at = round256(used_);
+            if (at + n > cap_) {
+                *e = hipErrorOutOfMemory;
+                used_ = at + n;  // reported in the capacity error
+                return nullptr;
+            }
+            used_ = at + n;
+            return fixed_ + at;
+        }
+        bufs_.emplace_back(new DevBuf());
+        if ((*e = bufs_.back()->alloc(n, s_)) != hipSuccess) return nullptr;
+        return bufs_.back()->p;
+    }
+    // host → device: pinned staging for allocated buffers; for the fixed
+    // region a direct copy, which needs no pinned block (the call waits for
+    // its stream before returning, so the host source outlives the copy)
+    hipError_t copy_in(void* dst, const void* src, size_t n) {
+        if (!n) return hipSuccess;
+        if (fixed_) return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s_);
+        return staging().copy(dst, src, n, s_);
+    }
+
+   private:
+    hipStream_t s_;
+    char* fixed_;
+    size_t cap_;
+    size_t used_ = 0;
+    std::vector<std::unique_ptr<DevBuf>> bufs_;
+};
+
+// terrain_correction over `mem`: planning, uploads, the kernels, counters.
+sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* scene, const sardine_dem_raster* dem,
+                       const sardine_tc_config* cfg, sardine_geocoded_image* out, hipStream_t stream, TcMemory& mem) {
     if (!merged || !scene || !dem || !cfg || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
-    hipStream_t stream = (hipStream_t)stream_v;
     const SeedLut lut = scene->grid_points && scene->n_grid_points > 0
                             ? build_seed_lut(scene->grid_points, scene->n_grid_points)
                             : SeedLut();
@@ -989,25 +1066,40 @@ static sardine_status sardine_terrain_correction_impl(const sardine_radar_image*
     // scene context of this call: orbit and seed table on the device
     const std::shared_ptr<const sd::OrbitDev> orb_p = orbit_cache().get(scene->orbit);
     const sd::OrbitDev& orb = *orb_p;
-    DevBuf d_orb, d_lut_lat, d_lut_t, d_work, d_cnt;
-    SD_HIP(upload_orbit(d_orb, orb, stream));
-    SD_HIP(upload(d_lut_lat, lut.lat.data(), lut.lat.size(), stream));
-    SD_HIP(upload(d_lut_t, lut.t.data(), lut.t.size(), stream));
-    pl.p.orbit = d_orb.as<sd::OrbitDev>();
-    pl.q.lut_lat = d_lut_lat.as<double>();
-    pl.q.lut_t = d_lut_t.as<int64_t>();
     set_decomposition(pl.p, g_tc_seg.load());
     const TcWork w = tc_work_bytes(pl, cfg);
-    SD_HIP(d_work.alloc(w.total(), stream));
-    assign_work(pl, w, d_work.as<char>());
-    if (w.geoid) SD_HIP(staging().copy(const_cast<float*>(pl.p.geoid), cfg->geoid.grid_73x144, 73 * 144 * sizeof(float), stream));
-    SD_HIP(d_cnt.alloc(sd::kCntSlots * sizeof(unsigned long long), stream));
-    SD_HIP(sd_launch_zero_u64(d_cnt.as<unsigned long long>(), sd::kCntSlots, stream));
-    pl.p.counters = d_cnt.as<unsigned long long>();
-    pl.p.error_flag = reinterpret_cast<int*>(d_cnt.as<unsigned long long>() + sd::kNumCounters);
+    const bool fp32 = cfg->precision == SARDINE_PRECISION_FP32_CENTRED;
+    hipError_t e = hipSuccess;
+    auto get = [&](size_t n) { return e == hipSuccess ? static_cast<char*>(mem.get(n, &e)) : nullptr; };
+    char* d_cnt = get(sd::kCntSlots * sizeof(unsigned long long));
+    char* d_orb = get(sizeof(sd::OrbitDev));
+    char* d_lut_lat = get(lut.lat.size() * sizeof(double) + 1);
+    char* d_lut_t = get(lut.t.size() * sizeof(int64_t) + 1);
+    char* d_work = get(w.total());
+    char* d_orb32 = fp32 ? get(sizeof(sd::OrbitF32)) : nullptr;
+    char* d_ends = fp32 ? get((size_t)pl.nr * 2 * sizeof(float)) : nullptr;
+    if (e != hipSuccess) {
+        if (mem.fixed() && e == hipErrorOutOfMemory)
+            return fail(SARDINE_ERR_CAPACITY,
+                        "terrain correction of %lld x %lld pixels needs %zu bytes of device scratch; the "
+                        "no-allocation entry point has %zu",
+                        (long long)pl.nr, (long long)pl.nc, round256(mem.used()), kNoAllocScratchBytes);
+        SD_HIP(e);
+    }
+    SD_HIP(mem.copy_in(d_orb, &orb, sd::orbit_lds_bytes(orb.n)));
+    SD_HIP(mem.copy_in(d_lut_lat, lut.lat.data(), lut.lat.size() * sizeof(double)));
+    SD_HIP(mem.copy_in(d_lut_t, lut.t.data(), lut.t.size() * sizeof(int64_t)));
+    pl.p.orbit = reinterpret_cast<const sd::OrbitDev*>(d_orb);
+    pl.q.lut_lat = reinterpret_cast<const double*>(d_lut_lat);
+    pl.q.lut_t = reinterpret_cast<const int64_t*>(d_lut_t);
+    assign_work(pl, w, d_work);
+    if (w.geoid) SD_HIP(mem.copy_in(const_cast<float*>(pl.p.geoid), cfg->geoid.grid_73x144, 73 * 144 * sizeof(float)));
+    // every counter starts at zero (a kernel, not a memset: capturable)
+    SD_HIP(sd_launch_zero_u64(reinterpret_cast<unsigned long long*>(d_cnt), sd::kCntSlots, stream));
+    pl.p.counters = reinterpret_cast<unsigned long long*>(d_cnt);
+    pl.p.error_flag = reinterpret_cast<int*>(pl.p.counters + sd::kNumCounters);
     SD_HIP(sd_launch_tc_rows(&pl.q, stream));
-    DevBuf d_orb32;
-    if (cfg->precision == SARDINE_PRECISION_FP32_CENTRED) {
+    if (fp32) {
         // scene centre: the bounding box centre on the ellipsoid; reference
         // time: the middle of the orbit
         const sardine_bbox& bb = scene->bounding_box;
@@ -1019,14 +1111,12 @@ static sardine_status sardine_terrain_correction_impl(const sardine_radar_image*
         pl.p.t_ref_us = orb.n > 0 ? (orb.t_us[0] + orb.t_us[orb.n - 1]) / 2 : 0;
         pl.p.tol32 = 3e-5f;  // above the ~1e-5 s rounding noise of the FP32 Doppler residual
         const sd::OrbitF32 o32 = make_orbit_f32(orb, pl.p.centre, pl.p.t_ref_us);
-        SD_HIP(upload(d_orb32, &o32, 1, stream));
-        pl.p.orbit32 = d_orb32.as<sd::OrbitF32>();
+        SD_HIP(mem.copy_in(d_orb32, &o32, sizeof o32));
+        pl.p.orbit32 = reinterpret_cast<const sd::OrbitF32*>(d_orb32);
         SD_HIP(sd_launch_tc_anchors(&pl.p, stream));
         KernelTimer kt(SARDINE_KERNEL_TC, stream);
         KernelTimer kt2(SARDINE_KERNEL_TC_SEGMENT, stream);
-        DevBuf d_ends;
-        SD_HIP(d_ends.alloc((size_t)pl.nr * 2 * sizeof(float), stream));
-        SD_HIP(sd_launch_tc_fp32(&pl.p, d_ends.as<float>(), stream));
+        SD_HIP(sd_launch_tc_fp32(&pl.p, reinterpret_cast<float*>(d_ends), stream));
         kt2.stop();
         kt.stop();
     } else {
@@ -1038,15 +1128,35 @@ static sardine_status sardine_terrain_correction_impl(const sardine_radar_image*
         kt.stop();
     }
     unsigned long long cnt[sd::kCntSlots];
-    SD_HIP(hipMemcpyAsync(cnt, d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost, stream));
+    SD_HIP(hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, stream));
     SD_HIP(hipStreamSynchronize(stream));
     return finish_tc(pl, cnt, scene->orbit, out);
+}
+
+static sardine_status sardine_terrain_correction_impl(const sardine_radar_image* merged, const sardine_scene* scene,
+                                                      const sardine_dem_raster* dem, const sardine_tc_config* cfg,
+                                                      sardine_geocoded_image* out, void* stream_v) {
+    TcMemory mem((hipStream_t)stream_v, nullptr, 0);
+    return tc_call(merged, scene, dem, cfg, out, (hipStream_t)stream_v, mem);
 }
 sardine_status sardine_terrain_correction(const sardine_radar_image* merged, const sardine_scene* scene,
                                           const sardine_dem_raster* dem, const sardine_tc_config* cfg,
                                           sardine_geocoded_image* out, void* stream_v) {
     return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&] {
         return sardine_terrain_correction_impl(merged, scene, dem, cfg, out, stream_v);
+    });
+}
+
+sardine_status sardine_terrain_correction_noalloc(const sardine_radar_image* merged, const sardine_scene* scene,
+                                                  const sardine_dem_raster* dem, const sardine_tc_config* cfg,
+                                                  sardine_geocoded_image* out, void* stream_v) {
+    return guarded(SARDINE_ERR_INVALID_ARGUMENT, [&]() -> sardine_status {
+        sardine_status st;
+        if ((st = require_device())) return st;
+        char* base = nullptr;
+        if ((st = scratch_on_current_device(&base))) return st;
+        TcMemory mem((hipStream_t)stream_v, base, kNoAllocScratchBytes);
+        return tc_call(merged, scene, dem, cfg, out, (hipStream_t)stream_v, mem);
     });
 }
 
