@@ -309,6 +309,15 @@ typedef struct sardine_geocoded_image {
     uint64_t newton_iter_histogram[SARDINE_NEWTON_HIST_SIZE];
     /* diagnostics of the GPU seed-chain emulation (no reference counterpart) */
     uint64_t chain_redo_rounds;
+    /* diagnostics of the Newton steps whose µs truncation or convergence test
+     * lay within the band where the kernels' polynomial orbit cannot decide
+     * it and were re-evaluated with the reference's own arithmetic (counted
+     * over every chain the kernels run, speculative and replayed ones
+     * included): how many, how many decisions that changed, and the largest
+     * difference in δ·1e6 (µs) between the two evaluations */
+    uint64_t tie_steps;
+    uint64_t tie_decisions_changed;
+    double tie_max_dev_us;
 } sardine_geocoded_image;
 
 /* Output grid of terrain_correction: compute_output_grid_extent
@@ -765,6 +774,13 @@ void sardine_set_tc_segment(int64_t columns);
  * every segment at the row seed).  A tuning knob with no effect on results:
  * a wrong prediction only costs the fix-up pass a replay. */
 void sardine_set_tc_lead(int32_t columns);
+/* Half-width (µs) of the band around a Newton step's decision points (the
+ * µs truncation, the convergence test) inside which the step is decided by
+ * the reference's own arithmetic instead of the kernels' (default 2^-18;
+ * values outside (0, 0.5] restore it).  A tuning knob with no effect on
+ * results as long as the band holds the kernels' deviation (about 3e-7 µs);
+ * a wide band sends many more pixels through the exact replays. */
+void sardine_set_tc_tie_band(double us);
 /* Milliseconds of the last profiled kernel of `family` on this thread, -1 if
  * none; waits for that kernel to finish. */
 double sardine_last_kernel_ms(int family);
@@ -882,6 +898,15 @@ sardine_status sardine_server_release(sardine_server server, uint64_t ticket, vo
 /* Stops the loop after the requests already admitted have finished; waits
  * for all work on the device before it frees outputs still unreleased. */
 void sardine_server_destroy(sardine_server server);
+
+/* ── Diagnostics (no reference counterpart) ──────────────────────────────
+ * Evaluates, on the device, element by element over n doubles (device
+ * buffers): fn 0-8 the portable libm of the geodesy and projection (sin,
+ * cos, exp, log, sinh, cosh, atanh, atan, atan2(y, x)), 9 x / y, 10 sqrt x.
+ * For comparing the device's bits with a host evaluation of the same
+ * functions.  Returns once the work is enqueued. */
+sardine_status sardine_diag_math(int32_t fn, const double* x /*[device]*/, const double* y /*[device]*/,
+                                 double* out /*[device]*/, int64_t n, void* stream);
 
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */
 

@@ -36,6 +36,8 @@ def lib():
         d, i64, i32, f = C.c_double, C.c_int64, C.c_int, C.c_float
         P = C.c_void_p
         L.oracle_geodetic_to_ecef.argtypes = [d, d, d, P]
+        L.oracle_pmath.argtypes = [i32, P, P, P, i64]
+        L.oracle_pmath.restype = None
         L.oracle_ecef_to_geodetic.argtypes = [P, P]
         L.oracle_interpolate_orbit_and_accel.argtypes = [C.POINTER(abi.Orbit), i64, P, P, P]
         L.oracle_solve_zero_doppler.argtypes = [P, C.POINTER(abi.Orbit), i64, i32, d, C.POINTER(i64),
@@ -455,6 +457,18 @@ def geoid_load(path, ascii: bool = False, model: int = 0):
     st = (lib().oracle_geoid_load_ww15mgh(str(path).encode(), _ptr(g)) if ascii else
           lib().oracle_geoid_load_binary(str(path).encode(), model, _ptr(g)))
     return st, g.reshape(73, 144)
+
+
+PMATH_FUNCS = ("sin", "cos", "exp", "log", "sinh", "cosh", "atanh", "atan", "atan2", "div", "sqrt")
+
+
+def pmath(fn: str, x, y=None):
+    """The portable libm (and IEEE division / sqrt) evaluated on the host."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(x if y is None else y, np.float64)
+    out = np.empty_like(x)
+    lib().oracle_pmath(PMATH_FUNCS.index(fn), _ptr(x), _ptr(y), _ptr(out), x.size)
+    return out
 
 
 def srtm_sample(paths, lat, lon):

@@ -46,6 +46,13 @@
 //                              forward/inverse round trip (proj4rs is not
 //                              vendored: parity unpinned below ±1 m)
 #include "../include/sardine_hip.h"
+// The libm of this restatement: the reference's results depend on the
+// platform's sin/cos/atan/... (Rust's f64 methods call the system libm), so
+// the oracle fixes one, a portable libm built from correctly rounded IEEE
+// operations (pinned to ~1 ulp of 50-digit values by the tests), which the
+// GPU kernels evaluate with the same bits.  The geodesy and the projection
+// use it; everything else keeps <cmath>.
+#include "../sardine_amd/csrc/pmath.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -89,7 +96,9 @@ inline V3 normalize3(V3 v) {  // geodesy.rs:125-132
 V3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
     double lat = lat_deg * kPi / 180.0;
     double lon = lon_deg * kPi / 180.0;
-    double sl = std::sin(lat), cl = std::cos(lat), so = std::sin(lon), co = std::cos(lon);
+    double sl, cl, so, co;
+    pm::sincos(lat, &sl, &cl);
+    pm::sincos(lon, &so, &co);
     double n = WGS84_A / std::sqrt(1.0 - WGS84_E2 * sl * sl);
     return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - WGS84_E2) + h) * sl};
 }
@@ -402,50 +411,69 @@ const TmCoef& TM() {
 }
 constexpr double UTM_K0 = 0.9996, UTM_E0 = 500000.0, UTM_N0_SOUTH = 10000000.0;
 
+// Σ_j c_j sin(2jξ) cosh(2jη) and Σ_j c_j cos(2jξ) sinh(2jη) (Karney 2011,
+// eqs. 11 and 35) summed by Clenshaw's recurrence on z = ξ + iη:
+// sum = Im/Re parts of sin(2z)·y0 with y_k = 2cos(2z)·y_{k+1} - y_{k+2} + c_k.
+void tm_clenshaw(const double* c /* c[1..6] */, double xi, double eta, double* sx, double* sy) {
+    double s2, c2;
+    pm::sincos(2 * xi, &s2, &c2);
+    const double sh2 = pm::sinh(2 * eta), ch2 = pm::cosh(2 * eta);
+    double ar = 2 * c2 * ch2, ai = -2 * s2 * sh2;
+    double y0r = 0, y0i = 0, y1r = 0, y1i = 0;
+    for (int k = 6; k >= 1; --k) {
+        double tr = ar * y0r - ai * y0i - y1r + c[k];
+        double ti = ar * y0i + ai * y0r - y1i;
+        y1r = y0r; y1i = y0i;
+        y0r = tr; y0i = ti;
+    }
+    double zr = s2 * ch2, zi = c2 * sh2;
+    *sx = zr * y0r - zi * y0i;
+    *sy = zr * y0i + zi * y0r;
+}
+
 void utm_forward(int zone, bool south, double lon_deg, double lat_deg, double* x, double* y) {
     const TmCoef& c = TM();
     double lon0 = (double)(zone * 6 - 183);
     double phi = lat_deg * kPi / 180.0, lam = (lon_deg - lon0) * kPi / 180.0;
-    double sphi = std::sin(phi);
-    double t = std::sinh(std::atanh(sphi) - c.e * std::atanh(c.e * sphi));
-    double xip = std::atan2(t, std::cos(lam));
-    double etap = std::atanh(std::sin(lam) / std::sqrt(1.0 + t * t));
-    double xi = xip, eta = etap;
-    for (int j = 1; j <= 6; ++j) {
-        xi += c.alpha[j] * std::sin(2 * j * xip) * std::cosh(2 * j * etap);
-        eta += c.alpha[j] * std::cos(2 * j * xip) * std::sinh(2 * j * etap);
-    }
-    *x = UTM_E0 + UTM_K0 * c.A * eta;
-    *y = (south ? UTM_N0_SOUTH : 0.0) + UTM_K0 * c.A * xi;
+    double sphi = pm::sin(phi);
+    double sl, cl;
+    pm::sincos(lam, &sl, &cl);
+    // conformal latitude: τ' = sinh(atanh(sin φ) - e atanh(e sin φ)) (eq. 7-9)
+    double t = pm::sinh(pm::atanh(sphi) - c.e * pm::atanh(c.e * sphi));
+    double xip = pm::atan2(t, cl);
+    double etap = pm::atanh(sl / std::sqrt(1.0 + t * t));
+    double sx, sy;
+    tm_clenshaw(c.alpha, xip, etap, &sx, &sy);
+    *x = UTM_E0 + UTM_K0 * c.A * (etap + sy);
+    *y = (south ? UTM_N0_SOUTH : 0.0) + UTM_K0 * c.A * (xip + sx);
 }
 
 void utm_inverse(int zone, bool south, double x, double y, double* lon_deg, double* lat_deg) {
     const TmCoef& c = TM();
-    double lon0 = (double)(zone * 6 - 183);
     double xi = (y - (south ? UTM_N0_SOUTH : 0.0)) / (UTM_K0 * c.A);
     double eta = (x - UTM_E0) / (UTM_K0 * c.A);
-    double xip = xi, etap = eta;
-    for (int j = 1; j <= 6; ++j) {
-        xip -= c.beta[j] * std::sin(2 * j * xi) * std::cosh(2 * j * eta);
-        etap -= c.beta[j] * std::cos(2 * j * xi) * std::sinh(2 * j * eta);
-    }
-    double sh = std::sinh(etap), cx = std::cos(xip);
-    double taup = std::sin(xip) / std::sqrt(sh * sh + cx * cx);
-    double lam = std::atan2(sh, cx);
+    double sx, sy;
+    tm_clenshaw(c.beta, xi, eta, &sx, &sy);
+    double xip = xi - sx, etap = eta - sy;
+    double sxp, cx;
+    pm::sincos(xip, &sxp, &cx);
+    double sh = pm::sinh(etap);
+    double taup = sxp / std::sqrt(sh * sh + cx * cx);
+    double lam = pm::atan2(sh, cx);
     // Solve tau from tau' (Karney 2011 eqs. 19-21), Newton iteration.
     double e2 = WGS84_E2, e = c.e;
     double tau = taup;
     for (int i = 0; i < 8; ++i) {
         double s1 = std::sqrt(1.0 + tau * tau);
-        double sig = std::sinh(e * std::atanh(e * tau / s1));
+        double sig = pm::sinh(e * pm::atanh(e * tau / s1));
         double taupi = tau * std::sqrt(1.0 + sig * sig) - sig * s1;
         double dtau = (taup - taupi) / std::sqrt(1.0 + taupi * taupi) * (1.0 + (1.0 - e2) * tau * tau) /
                       ((1.0 - e2) * s1);
         tau += dtau;
         if (std::fabs(dtau) < 1e-14) break;
     }
-    *lat_deg = std::atan(tau) * 180.0 / kPi;
-    *lon_deg = lon0 + lam * 180.0 / kPi;
+    *lat_deg = pm::atan(tau) * 180.0 / kPi;
+    *lon_deg = (double)(zone * 6 - 183) + lam * 180.0 / kPi;
 }
 
 // Projector::project_row, output_crs.rs:364-417
@@ -2356,4 +2384,27 @@ extern "C" int oracle_slc_multi_read(const char* const* paths, const sardine_slc
     const int64_t end = k + 1 < n ? offsets[(size_t)k + 1] : cum;
     if (first + count > end) return SARDINE_ERR_SLC_CROSS_SLICE_BOUNDARY;
     return oracle_slc_read(paths[k], &t[k], first - offsets[(size_t)k], count, dst);
+}
+
+// The portable libm and the IEEE division / square root on the host, element
+// by element (fn as sardine_diag_math): the host side of the bit comparison.
+extern "C" void oracle_pmath(int fn, const double* x, const double* y, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double a = x[i], b = y[i];
+        double r;
+        switch (fn) {
+            case 0: r = pm::sin(a); break;
+            case 1: r = pm::cos(a); break;
+            case 2: r = pm::exp(a); break;
+            case 3: r = pm::log(a); break;
+            case 4: r = pm::sinh(a); break;
+            case 5: r = pm::cosh(a); break;
+            case 6: r = pm::atanh(a); break;
+            case 7: r = pm::atan(a); break;
+            case 8: r = pm::atan2(b, a); break;
+            case 9: r = a / b; break;
+            default: r = std::sqrt(a); break;
+        }
+        out[i] = r;
+    }
 }

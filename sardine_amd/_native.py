@@ -84,6 +84,8 @@ def lib():
     L.sardine_set_tc_segment.restype = None
     L.sardine_set_tc_lead.argtypes = [C.c_int32]
     L.sardine_set_tc_lead.restype = None
+    L.sardine_set_tc_tie_band.argtypes = [C.c_double]
+    L.sardine_set_tc_tie_band.restype = None
     PP = C.POINTER(C.c_void_p)
     L.sardine_scene_create.argtypes = [C.POINTER(abi.RadarImage), C.POINTER(abi.Scene), C.POINTER(abi.DemRaster), PP]
     L.sardine_scene_destroy.argtypes = [P]
@@ -101,11 +103,13 @@ def lib():
     L.sardine_server_wait.argtypes = [P, u64, i32, C.POINTER(abi.Response)]
     L.sardine_server_release.argtypes = [P, u64, P]
     L.sardine_server_destroy.argtypes = [P]
+    L.sardine_diag_math.argtypes = [i32, P, P, P, i64, P]
     L.sardine_server_destroy.restype = None
     for name in abi.EXPORTED_SYMBOLS:
         if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",
                         "sardine_tc_config_default", "sardine_set_profiling", "sardine_last_kernel_ms",
-                        "sardine_set_tc_segment", "sardine_set_tc_lead", "sardine_scene_destroy",
+                        "sardine_set_tc_segment", "sardine_set_tc_lead", "sardine_set_tc_tie_band",
+                        "sardine_scene_destroy",
                         "sardine_batch_destroy", "sardine_server_destroy"):
             getattr(L, name).restype = C.c_int
     if L.sardine_abi_version() != abi.ABI_VERSION:

@@ -154,7 +154,8 @@ class GeocodedImage(C.Structure):
                 ("non_converged_count", C.c_uint64), ("degenerate_geometry_count", C.c_uint64),
                 ("flat_masked_count", C.c_uint64), ("noise_masked_count", C.c_uint64),
                 ("newton_iter_histogram", C.c_uint64 * NEWTON_HIST_SIZE),
-                ("chain_redo_rounds", C.c_uint64)]
+                ("chain_redo_rounds", C.c_uint64), ("tie_steps", C.c_uint64),
+                ("tie_decisions_changed", C.c_uint64), ("tie_max_dev_us", C.c_double)]
 
 
 class SpeckleFilterC(C.Structure):
@@ -300,11 +301,13 @@ EXPORTED_SYMBOLS = (
     "sardine_write_geotiff_u8_with_crs",
     "sardine_write_tiff_no_crs",
     "sardine_synth_cint16",
+    "sardine_diag_math",
     "sardine_set_profiling",
     "sardine_last_kernel_ms",
     "sardine_recent_kernel_ms",
     "sardine_set_tc_segment",
     "sardine_set_tc_lead",
+    "sardine_set_tc_tie_band",
     "sardine_scene_create",
     "sardine_scene_destroy",
     "sardine_batch_create",

@@ -204,6 +204,9 @@ class GeocodedImage:
     full_cols: int = 0
     chain_redo_rounds: int = 0
     radar_coords: object = None
+    tie_steps: int = 0
+    tie_decisions_changed: int = 0
+    tie_max_dev_us: float = 0.0
 
     @property
     def origin_lat_deg(self):
@@ -257,7 +260,8 @@ def terrain_correction(merged: MergedSigma0, scene: SceneInputs, dem: DemRaster,
                          img.valid_pixel_count, img.dem_missing_count, img.outside_footprint_count,
                          img.non_converged_count, img.degenerate_geometry_count, img.flat_masked_count,
                          img.noise_masked_count, list(img.newton_iter_histogram), cl, mk, img.full_rows,
-                         img.full_cols, img.chain_redo_rounds, rc)
+                         img.full_cols, img.chain_redo_rounds, rc,
+                         img.tie_steps, img.tie_decisions_changed, img.tie_max_dev_us)
 
 
 # ─── speckle ──────────────────────────────────────────────────────────────
@@ -793,7 +797,8 @@ class Batch:
                                img.valid_pixel_count, img.dem_missing_count, img.outside_footprint_count,
                                img.non_converged_count, img.degenerate_geometry_count, img.flat_masked_count,
                                img.noise_masked_count, list(img.newton_iter_histogram), cl, mk, img.full_rows,
-                               img.full_cols, img.chain_redo_rounds, None)
+                               img.full_cols, img.chain_redo_rounds, None,
+                         img.tie_steps, img.tie_decisions_changed, img.tie_max_dev_us)
             out.append(BatchResult(gi, j.speckle_out if j.speckle is not None else None, cj[i].status))
         return out
 
@@ -870,7 +875,8 @@ class Server:
                                 g.valid_pixel_count, g.dem_missing_count, g.outside_footprint_count,
                                 g.non_converged_count, g.degenerate_geometry_count, g.flat_masked_count,
                                 g.noise_masked_count, list(g.newton_iter_histogram), None, None, g.full_rows,
-                                g.full_cols, g.chain_redo_rounds, None)
+                                g.full_cols, g.chain_redo_rounds, None,
+                                g.tie_steps, g.tie_decisions_changed, g.tie_max_dev_us)
             if r.speckled:
                 sp = torch.as_tensor(_DeviceView(r.speckled, (g.rows, g.cols), "<f4"), device="cuda")
         return ServedResult(r.status, img, sp, r.t_submit_us, r.t_admit_us, r.t_launch_us, r.t_done_us,

@@ -69,6 +69,8 @@ extern "C" hipError_t sd_launch_swap16(uint16_t* d, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_srtm_tile(const uint16_t* be, int dim, float* dst, int64_t dst_w, int64_t row0,
                                           int64_t col0, hipStream_t s);
 extern "C" hipError_t sd_launch_fill_nan(float* d, int64_t n, hipStream_t s);
+extern "C" hipError_t sd_launch_pmath(int fn, const double* x, const double* y, double* out, int64_t n,
+                                      hipStream_t s);
 extern "C" hipError_t sd_launch_dem_values(const void* raw, int kind, int have_nd, float nd_f, int32_t nd_i,
                                            float* dst, int64_t n, hipStream_t s);
 extern "C" hipError_t sd_launch_fp_predictor(const uint8_t* raw, int64_t bw, int64_t bh, int64_t blocks_x,
@@ -84,6 +86,7 @@ thread_local sardine_error_detail g_last_detail;
 thread_local bool g_profile = false;
 std::atomic<int64_t> g_tc_seg{0};  // columns per lane segment, 0 = automatic (sardine_set_tc_segment)
 std::atomic<int32_t> g_tc_lead{2};  // phase-A lead columns (sardine_set_tc_lead)
+std::atomic<double> g_tc_tie_band{sd::kTieBandUs};  // µs (sardine_set_tc_tie_band)
 constexpr int kFamilies = 9;
 
 // Brackets one kernel launch with events when profiling is on.  Nothing
@@ -743,6 +746,7 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     p.cols = nc;
     p.max_iter = cfg->max_newton_iterations;
     p.tol = cfg->newton_tolerance_s;
+    p.tie_band = g_tc_tie_band.load();
     p.flatten = cfg->flatten ? 1 : 0;
     p.compute_lia = compute_lia ? 1 : 0;
     p.need_geom = need_geom ? 1 : 0;
@@ -806,8 +810,9 @@ void set_decomposition(sd::TcParams& p, int64_t requested) {
 
 // Bytes of the per-job work buffers, each rounded to 256.
 struct TcWork {
-    size_t records = 0, flags = 0, anchors = 0, seed = 0, flat = 0, geoid = 0;
-    size_t total() const { return records + flags + anchors + seed + flat + geoid; }
+    size_t records = 0, flags = 0, ties = 0, anchors = 0, seed = 0, flat = 0, geoid = 0;
+    int64_t tie_cap = 0;
+    size_t total() const { return records + flags + ties + anchors + seed + flat + geoid; }
 };
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
@@ -817,6 +822,10 @@ TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
                     ? 0
                     : round256((size_t)pl.nr * pl.p.nseg * sizeof(sd::SegRecord));
     w.flags = w.records ? round256((size_t)pl.nr * sizeof(int32_t)) : 0;
+    // tie-only segments replayed in parallel: room for 8 per row (more send
+    // their rows to the sequential walk)
+    w.tie_cap = w.records ? std::min<int64_t>(pl.nr * (int64_t)pl.p.nseg, std::max<int64_t>(64, pl.nr * 8)) : 0;
+    w.ties = round256((size_t)w.tie_cap * sizeof(int2));
     w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * pl.p.anchor_stride * sizeof(double)) : 0;
     w.seed = round256((size_t)pl.nr * sizeof(int64_t));
     w.flat = pl.need_geom ? round256((size_t)pl.nr * sizeof(double)) : 0;
@@ -829,6 +838,9 @@ void assign_work(TcPlan& pl, const TcWork& w, char* base) {
     base += w.records;
     pl.p.row_flags = w.flags ? reinterpret_cast<int32_t*>(base) : nullptr;
     base += w.flags;
+    pl.p.tie_list = w.tie_cap ? reinterpret_cast<int2*>(base) : nullptr;
+    pl.p.tie_cap = (int)w.tie_cap;
+    base += w.ties;
     pl.p.anchors = w.anchors ? reinterpret_cast<double*>(base) : nullptr;
     base += w.anchors;
     pl.p.row_seed = reinterpret_cast<const int64_t*>(base);
@@ -891,6 +903,9 @@ sardine_status finish_tc(const TcPlan& pl, const unsigned long long* cnt, const 
     out->noise_masked_count = cnt[6];
     for (int i = 0; i < SARDINE_NEWTON_HIST_SIZE; ++i) out->newton_iter_histogram[i] = cnt[7 + i];
     out->chain_redo_rounds = cnt[sd::kCntRedo];
+    out->tie_steps = cnt[sd::kCntTie + sd::TIE_STEPS];
+    out->tie_decisions_changed = cnt[sd::kCntTie + sd::TIE_FLIPS];
+    std::memcpy(&out->tie_max_dev_us, &cnt[sd::kCntTie + sd::TIE_MAX_DEV], sizeof(double));
     return SARDINE_OK;
 }
 
@@ -917,6 +932,7 @@ void sardine_set_profiling(int on) {
 }
 void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : columns; }
 void sardine_set_tc_lead(int32_t columns) { g_tc_lead = columns < 0 ? 0 : (columns > 64 ? 64 : columns); }
+void sardine_set_tc_tie_band(double us) { g_tc_tie_band = us > 0.0 && us <= 0.5 ? us : sd::kTieBandUs; }
 double sardine_last_kernel_ms(int family) {
     if (family < 0 || family >= kFamilies) return -1.0;
     TimerSlot& t = g_slots[family];
@@ -1546,6 +1562,16 @@ sardine_status sardine_to_db_inplace(float* data, int64_t n, float floor_lin, ui
         counts[0] = h[0];
         counts[1] = h[1];
     }
+    return SARDINE_OK;
+}
+
+sardine_status sardine_diag_math(int32_t fn, const double* x, const double* y, double* out, int64_t n,
+                                 void* stream) {
+    if (fn < 0 || fn > 10 || n < 0 || (n > 0 && (!x || !y || !out)))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad diagnostic math request");
+    sardine_status st;
+    if ((st = require_device())) return st;
+    SD_HIP(sd_launch_pmath(fn, x, y, out, n, (hipStream_t)stream));
     return SARDINE_OK;
 }
 
@@ -3411,6 +3437,9 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
     for (int i : run.run_idx) {
         TcPlan& pl = run.plans[(size_t)i];
         assign_work(pl, works[(size_t)i], base);
+        // the batch's fix-up pass walks tie rows itself (no parallel tie pass)
+        pl.p.tie_list = nullptr;
+        pl.p.tie_cap = 0;
         if (works[(size_t)i].geoid)
             SD_HIP(staging().copy(const_cast<float*>(pl.p.geoid), jobs[i].config.geoid.grid_73x144,
                                   73 * 144 * sizeof(float), s));

@@ -14,6 +14,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "pmath.hpp"
+
 #define SD_HD __host__ __device__ __forceinline__
 
 namespace sd {
@@ -47,20 +49,15 @@ SD_HD double div_rn(double x, double d, double y) {
     return fma(r, y, q0);
 }
 
-// geodesy.rs:38-55 (same operation order: lat_deg*PI/180, n = a/sqrt(1-e2 s^2))
+// geodesy.rs:38-55 (same operation order: lat_deg*PI/180, n = a/sqrt(1-e2 s^2)).
+// The sines and cosines are the portable ones (pmath.hpp), the same bits
+// on the host and on the device.
 SD_HD d3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
     double lat = lat_deg * kPi / 180.0;
     double lon = lon_deg * kPi / 180.0;
     double sl, cl, so, co;
-#ifdef __HIP_DEVICE_COMPILE__
-    sincos(lat, &sl, &cl);
-    sincos(lon, &so, &co);
-#else
-    sl = sin(lat);
-    cl = cos(lat);
-    so = sin(lon);
-    co = cos(lon);
-#endif
+    pm::sincos(lat, &sl, &cl);
+    pm::sincos(lon, &so, &co);
     double n = kA / sqrt(1.0 - kE2 * sl * sl);
     return {(n + h) * cl * co, (n + h) * cl * so, (n * (1.0 - kE2) + h) * sl};
 }
@@ -69,12 +66,7 @@ SD_HD d3 geodetic_to_ecef(double lat_deg, double lon_deg, double h) {
 // ((deg·π) / 180, the division exact by div_rn).
 SD_HD void sincos_deg(double deg, double* s, double* c) {
     const double r = div_rn(deg * kPi, 180.0, 1.0 / 180.0);
-#ifdef __HIP_DEVICE_COMPILE__
-    sincos(r, s, c);
-#else
-    *s = sin(r);
-    *c = cos(r);
-#endif
+    pm::sincos(r, s, c);
 }
 
 // geodetic_to_ecef from the sines/cosines of its (degree) arguments: the
@@ -188,6 +180,80 @@ SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
     return ORB_OK;
 }
 
+// interpolate_orbit_and_accel (orbit.rs:536-596) evaluated as the reference
+// evaluates it, for a time t inside the window of an orbit of at least 8
+// vectors: the product-form Lagrange basis with its 56 divisions, the 1 ns
+// reciprocal guard, and every sum over the window in the reference's order,
+// each operation separately rounded.  Given the same inputs it returns the
+// reference's bits; the Newton solve calls it only for the steps the
+// interval polynomials above cannot decide (zero_doppler).
+SD_HD void orbit_interp_reference(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
+    const int n = o.n;
+    int c = 0;
+    while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
+    const int start = orbit_window_start(c, n);
+    const double ts = (double)(t - o.t_us[0]) / 1e6;
+    double rd[kLagrange];
+    double tot = 0.0;
+    for (int j = 0; j < kLagrange; ++j) {
+        const double diff = ts - o.tsec[start + j];
+        rd[j] = 0.0;
+        if (fabs(diff) > 1e-9) {
+            rd[j] = 1.0 / diff;
+            tot += rd[j];
+        }
+    }
+    d3 p = {0.0, 0.0, 0.0}, v = {0.0, 0.0, 0.0}, a = {0.0, 0.0, 0.0};
+    for (int k = 0; k < kLagrange; ++k) {
+        double basis = 1.0;
+        for (int j = 0; j < kLagrange; ++j)
+            if (j != k) basis *= (ts - o.tsec[start + j]) / (o.tsec[start + k] - o.tsec[start + j]);
+        const double db = basis * (tot - rd[k]);
+        const double* pp = o.pos[start + k];
+        const double* vv = o.vel[start + k];
+        p.x += basis * pp[0];
+        p.y += basis * pp[1];
+        p.z += basis * pp[2];
+        v.x += basis * vv[0];
+        v.y += basis * vv[1];
+        v.z += basis * vv[2];
+        a.x += db * vv[0];
+        a.y += db * vv[1];
+        a.z += db * vv[2];
+    }
+    *P = p;
+    *V = v;
+    *A = a;
+}
+
+// One Newton step's δ and f' at time t exactly as solve_zero_doppler
+// computes them (terrain_correction.rs:1035-1044): the reference's orbit
+// evaluation, then f = (T - P)·V, f' = -|V|² + (T - P)·A, δ = -f / f'.
+SD_HD double zero_doppler_delta_reference(const OrbitDev& o, d3 target, int64_t t, double* fp_out) {
+    d3 P, V, A;
+    orbit_interp_reference(o, t, &P, &V, &A);
+    const double rx = target.x - P.x, ry = target.y - P.y, rz = target.z - P.z;
+    const double f = rx * V.x + ry * V.y + rz * V.z;
+    const double vsq = V.x * V.x + V.y * V.y + V.z * V.z;
+    const double fp = -vsq + (rx * A.x + ry * A.y + rz * A.z);
+    *fp_out = fp;
+    return -f / fp;
+}
+
+// Default half-width (µs) of the band around a decision point inside which
+// a Newton step is a tie: the kernels' arithmetic cannot decide it, and it
+// takes the reference's.  The polynomial orbit (within 1e-9 m of the exact
+// interpolant, as the reference's product form is within 3e-9 m of it) and
+// the contracted f, f' move δ·1e6 by at most ~6e-7 µs from the reference's
+// value (measured: 3.1e-7 over thousands of such steps); the band is 2^-18 µs
+// = 3.8e-6 µs.
+constexpr double kTieBandUs = 0x1p-18;
+
+// Counters of the re-evaluated steps: how many, how many changed a decision
+// (the µs truncation or the convergence test), and the largest |Δ(δ·1e6)|
+// between the two evaluations (the bits of a non-negative double).
+enum : int { TIE_STEPS = 0, TIE_FLIPS = 1, TIE_MAX_DEV = 2 };
+
 enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT_OUTSIDE = 3, ZD_ORBIT_INSUFFICIENT = 4 };
 
 // The last orbit evaluation of a chain.  A pixel's Newton solve starts at
@@ -210,9 +276,17 @@ SD_HD OrbitCache orbit_cache_empty() {
 }
 
 // solve_zero_doppler (terrain_correction.rs:1010-1073): convergence is tested
-// before the step, steps are truncated to whole µs.
-SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter, double tol, int64_t* t_out,
-                       int* iters, d3* sat, OrbitCache* cache) {
+// before the step, steps are truncated to whole µs.  A tie step (δ·1e6
+// within `band` µs of an integer, where the truncation changes, or |δ| that
+// close to the tolerance) sets *tie; with EXACT it takes the reference's own
+// δ (zero_doppler_delta_reference), so every decision is the reference's
+// given the same target, and tie_stats (device, optional) counts such steps.
+// The chain kernels solve without EXACT and record where ties fell; the
+// fix-up pass replays those pixels with EXACT.
+template <bool EXACT>
+SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter, double tol, double band,
+                       int64_t* t_out, int* iters, d3* sat, OrbitCache* cache, bool* tie,
+                       unsigned long long* tie_stats) {
 #pragma clang fp contract(fast)
     int64_t t = seed;
     for (int i = 0; i < max_iter; ++i) {
@@ -240,6 +314,27 @@ SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter,
         double fp = -vsq + dot(r, A);
         if (fabs(fp) < 1.0) return ZD_DEGENERATE;
         double delta = -f / fp;
+        const double x = delta * 1e6;
+        if (fabs(x - rint(x)) < band || fabs(fabs(delta) - tol) < band * 1e-6) {
+            *tie = true;
+            if constexpr (EXACT) {
+                double fp_ref;
+                const double d_ref = zero_doppler_delta_reference(o, target, t, &fp_ref);
+#ifdef __HIP_DEVICE_COMPILE__
+                if (tie_stats) {
+                    const double x_ref = d_ref * 1e6;
+                    const bool flip =
+                        (int64_t)x != (int64_t)x_ref || (fabs(delta) < tol) != (fabs(d_ref) < tol);
+                    const double dev = fabs(x - x_ref);
+                    atomicAdd(&tie_stats[TIE_STEPS], 1ull);
+                    if (flip) atomicAdd(&tie_stats[TIE_FLIPS], 1ull);
+                    atomicMax(&tie_stats[TIE_MAX_DEV], (unsigned long long)__double_as_longlong(dev));
+                }
+#endif
+                if (fabs(fp_ref) < 1.0) return ZD_DEGENERATE;
+                delta = d_ref;
+            }
+        }
         if (fabs(delta) < tol) {
             *t_out = t;
             *iters = i + 1 < 255 ? i + 1 : 255;
@@ -361,7 +456,9 @@ constexpr double kUtmN0South = 10000000.0;
 // Σ_j c_j sin(2j ξ) cosh(2j η) and Σ_j c_j cos(2j ξ) sinh(2j η) by the complex
 // Clenshaw recurrence on z = ξ + iη.
 SD_HD void clenshaw_series(const double* c, double xi, double eta, double* sx, double* sy) {
-    double s2 = sin(2 * xi), c2 = cos(2 * xi), sh2 = sinh(2 * eta), ch2 = cosh(2 * eta);
+    double s2, c2;
+    pm::sincos(2 * xi, &s2, &c2);
+    const double sh2 = pm::sinh(2 * eta), ch2 = pm::cosh(2 * eta);
     // a = 2 cos(2z) = 2(c2 ch2 - i s2 sh2)
     double ar = 2 * c2 * ch2, ai = -2 * s2 * sh2;
     double y0r = 0, y0i = 0, y1r = 0, y1i = 0;
@@ -384,30 +481,34 @@ SD_HD void utm_inverse(const TmSeries& s, int zone, bool south, double x, double
     double sx, sy;
     clenshaw_series(s.beta, xi, eta, &sx, &sy);
     double xip = xi - sx, etap = eta - sy;
-    double sh = sinh(etap), cx = cos(xip);
-    double taup = sin(xip) / sqrt(sh * sh + cx * cx);
-    double lam = atan2(sh, cx);
+    double sxp, cx;
+    pm::sincos(xip, &sxp, &cx);
+    double sh = pm::sinh(etap);
+    double taup = sxp / sqrt(sh * sh + cx * cx);
+    double lam = pm::atan2(sh, cx);
     const double e = s.e, e2 = kE2;
     double tau = taup;
     for (int i = 0; i < 8; ++i) {
         double s1 = sqrt(1.0 + tau * tau);
-        double sig = sinh(e * atanh(e * tau / s1));
+        double sig = pm::sinh(e * pm::atanh(e * tau / s1));
         double taui = tau * sqrt(1.0 + sig * sig) - sig * s1;
         double dt = (taup - taui) / sqrt(1.0 + taui * taui) * (1.0 + (1.0 - e2) * tau * tau) / ((1.0 - e2) * s1);
         tau += dt;
         if (fabs(dt) < 1e-14) break;
     }
-    *lat_deg = atan(tau) * 180.0 / kPi;
+    *lat_deg = pm::atan(tau) * 180.0 / kPi;
     *lon_deg = (double)(zone * 6 - 183) + lam * 180.0 / kPi;
 }
 
 SD_HD void utm_forward(const TmSeries& s, int zone, bool south, double lon_deg, double lat_deg, double* x,
                        double* y) {
     double phi = lat_deg * kPi / 180.0, lam = (lon_deg - (double)(zone * 6 - 183)) * kPi / 180.0;
-    double sp = sin(phi);
-    double t = sinh(atanh(sp) - s.e * atanh(s.e * sp));
-    double xip = atan2(t, cos(lam));
-    double etap = atanh(sin(lam) / sqrt(1.0 + t * t));
+    double sp = pm::sin(phi);
+    double sl, cl;
+    pm::sincos(lam, &sl, &cl);
+    double t = pm::sinh(pm::atanh(sp) - s.e * pm::atanh(s.e * sp));
+    double xip = pm::atan2(t, cl);
+    double etap = pm::atanh(sl / sqrt(1.0 + t * t));
     double sx, sy;
     clenshaw_series(s.alpha, xip, etap, &sx, &sy);
     *x = kUtmE0 + kUtmK0 * s.A * (etap + sy);

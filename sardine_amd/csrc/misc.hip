@@ -6,6 +6,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "pmath.hpp"
+
 namespace sd {
 
 __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
@@ -242,5 +244,39 @@ extern "C" hipError_t sd_launch_synth_cint16(int16_t* out, int64_t n, uint64_t s
     int64_t blocks = (n + 255) / 256;
     if (blocks > 262144) blocks = 262144;
     hipLaunchKernelGGL(sd::synth_cint16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, seed, base, sigma);
+    return hipGetLastError();
+}
+
+// The portable libm (pmath.hpp) and the IEEE division and square root on the
+// device, element by element: a diagnostic that lets the host compare the
+// device's bits with its own evaluation of the same functions.
+namespace sd {
+__global__ __launch_bounds__(256) void pmath_kernel(int fn, const double* x, const double* y, double* out,
+                                                    int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i], b = y[i];
+    double r;
+    switch (fn) {
+        case 0: r = pm::sin(a); break;
+        case 1: r = pm::cos(a); break;
+        case 2: r = pm::exp(a); break;
+        case 3: r = pm::log(a); break;
+        case 4: r = pm::sinh(a); break;
+        case 5: r = pm::cosh(a); break;
+        case 6: r = pm::atanh(a); break;
+        case 7: r = pm::atan(a); break;
+        case 8: r = pm::atan2(b, a); break;
+        case 9: r = a / b; break;
+        default: r = sqrt(a); break;
+    }
+    out[i] = r;
+}
+}  // namespace sd
+
+extern "C" hipError_t sd_launch_pmath(int fn, const double* x, const double* y, double* out, int64_t n,
+                                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sd::pmath_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, x, y, out, n);
     return hipGetLastError();
 }

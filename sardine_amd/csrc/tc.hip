@@ -59,6 +59,19 @@ __global__ __launch_bounds__(256) void tc_check_kernel(TcParams p) {
     if (rr < p.rows) check_row(p, rr);
 }
 
+// The tie list's replays, one lane per entry (the list's length is on the
+// device; the grid covers its capacity).  The counter keeps counting past
+// the capacity (those entries were not stored, and their rows went to the
+// sequential walk), so the entries read are the first min(count, capacity).
+template <int RS, bool UTM, bool WRITE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tc_tie_kernel(TcParams p) {
+    __shared__ OrbitDev orb;
+    const int64_t n = min((int64_t)p.counters[kCntTieList], (int64_t)p.tie_cap);
+    if ((int64_t)blockIdx.x * 256 >= n) return;  // whole workgroup
+    load_orbit(p, orb);
+    tie_item<RS, UTM, WRITE>(p, orb, (int64_t)blockIdx.x * 256 + threadIdx.x, n);
+}
+
 template <int RS, bool UTM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tc_fixup_kernel(TcParams p) {
     __shared__ OrbitDev orb;
@@ -77,6 +90,11 @@ void launch_chain(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_star
     hipLaunchKernelGGL((sd::tc_segment_kernel<RS, UTM>), grid, dim3(256), 0, stream, *p);
     if (seg_end) (void)hipEventRecord(seg_end, stream);
     hipLaunchKernelGGL(sd::tc_check_kernel, dim3((unsigned)((p->rows + 3) / 4)), dim3(256), 0, stream, *p);
+    if (p->tie_list && p->tie_cap > 0) {
+        const dim3 g((unsigned)((p->tie_cap + 255) / 256));
+        hipLaunchKernelGGL((sd::tc_tie_kernel<RS, UTM, false>), g, dim3(256), 0, stream, *p);
+        hipLaunchKernelGGL((sd::tc_tie_kernel<RS, UTM, true>), g, dim3(256), 0, stream, *p);
+    }
     hipLaunchKernelGGL((sd::tc_fixup_kernel<RS, UTM>), dim3((unsigned)((p->rows + 255) / 256)), dim3(256), 0, stream,
                        *p);
 }

@@ -351,7 +351,9 @@ __global__ __launch_bounds__(256) void tc32_seed_kernel(TcParams p, float* ends)
         int64_t t = 0;
         int it = 0;
         d3 sat;
-        if (zero_doppler(*p.orbit, q.target, p.row_seed[rr], 40, p.tol, &t, &it, &sat, &cache) == ZD_CONVERGED)
+        bool tie = false;  // a seed only: the kernel's own decision is good enough
+        if (zero_doppler<false>(*p.orbit, q.target, p.row_seed[rr], 40, p.tol, kTieBandUs, &t, &it, &sat, &cache,
+                                &tie, nullptr) == ZD_CONVERGED)
             e[k] = (double)(t - p.t_ref_us) * 1e-6;
     }
     ends[2 * rr] = (float)e[0];

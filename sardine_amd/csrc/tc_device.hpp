@@ -216,20 +216,26 @@ __device__ __forceinline__ void write_dem_missing(const TcParams& p, int64_t o) 
     }
 }
 
-// Newton from `seed` for one pixel (dem_ok pixels only).
+// Newton from `seed` for one pixel (dem_ok pixels only); `tie` is set when
+// a step was a tie (decided by the reference's arithmetic with EXACT, by the
+// kernel's own otherwise).
 struct Solve {
     int oc, it;
     int64_t t;
     d3 sat;
+    bool tie;
 };
 
+template <bool EXACT = false>
 __device__ __forceinline__ Solve solve(const TcParams& p, const OrbitDev& orb, const PixelPos& q, int64_t seed,
                                        int max_iter, OrbitCache& cache) {
     Solve s;
     s.t = 0;
     s.it = 0;
     s.sat = {0.0, 0.0, 0.0};
-    s.oc = zero_doppler(orb, q.target, seed, max_iter, p.tol, &s.t, &s.it, &s.sat, &cache);
+    s.tie = false;
+    s.oc = zero_doppler<EXACT>(orb, q.target, seed, max_iter, p.tol, p.tie_band, &s.t, &s.it, &s.sat, &cache, &s.tie,
+                               EXACT ? p.counters + kCntTie : nullptr);
     return s;
 }
 
@@ -353,6 +359,11 @@ struct WaveStage {
     float d[64][kStageCols + 1];  // +1 word: row-to-row bank skew
     float c[64][kStageCols + 1];
     unsigned char m[64][kMaskCols + 4];
+    // each lane's tie bookkeeping (rarely written, so kept out of registers):
+    // the seed entering its first tie pixel, the first and last tie columns,
+    // and flags (bit 0: a tie was met; bit 1: a pixel converged before it)
+    long long tie_seed[64];
+    int tie_first[64], tie_last[64], tie_flags[64];
 };
 
 // Orders the LDS accesses of one wave's lanes (lanes of a wave execute in
@@ -488,6 +499,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
     zero_counts(C);
     long long seed = S, last_t = 0;
     bool last_has = false, any_newton = false;
+    st.tie_flags[lane] = 0;
     for (int64_t cc = c_begin; cc < c_end; ++cc) {
         const int j = (int)(cc - c_begin);
         PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
@@ -506,6 +518,14 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
         } else {
             any_newton = true;
             Solve s = solve(p, orb, q, seed, p.max_iter, cache);
+            if (s.tie) {  // the fix-up passes replay this pixel with the reference's arithmetic
+                if (!st.tie_flags[lane]) {
+                    st.tie_seed[lane] = seed;
+                    st.tie_first[lane] = (int)cc;
+                    st.tie_flags[lane] = 1 | (last_has ? 2 : 0);
+                }
+                st.tie_last[lane] = (int)cc;
+            }
             if (s.oc == ZD_CONVERGED) {
                 seed = s.t;
                 last_t = s.t;
@@ -546,12 +566,16 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
         }
     }
     if (row_ok) {
+        const int tf = st.tie_flags[lane];
         SegRecord rec;
         rec.seed = S;
         rec.last = last_t;
-        rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0);
+        rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0) | ((tf & 1) ? 4 : 0) | ((tf & 2) ? 8 : 0);
         rec.err_outside = C.err_outside;
         rec.err_short = C.err_short;
+        rec.tie_first = tf ? st.tie_first[lane] : 0;
+        rec.tie_seed = tf ? st.tie_seed[lane] : 0;
+        rec.tie_last = tf ? st.tie_last[lane] : 0;
         rec.pad = 0;
         p.records[rr * p.nseg + seg] = rec;
     }
@@ -561,22 +585,28 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
 // One wave per AOI row, before the fix-up pass: a segment that ran a Newton
 // solve must have started from the chained value to its left (the last
 // converged time of the nearest segment to its left that has one, or the
-// row seed).  The row's records are read 64 at a time (coalesced); the
-// chained value entering each lane is the `last` of the nearest lane below
-// it whose segment has one (ballot + shuffle), else the value carried in
-// from the previous 64.  A row with any mismatch is flagged for the fix-up
-// walk, which also adds its orbit-error counts; the others add theirs here.
+// row seed), and must have met no tie step.  The row's records are read 64
+// at a time (coalesced); the chained value entering each lane is the `last`
+// of the nearest lane below it whose segment has one (ballot + shuffle),
+// else the value carried in from the previous 64.  row_flags:
+//   0  nothing to correct: the row's orbit-error counts are added here;
+//   2  every seed matches and only tie segments need a replay: they are
+//      appended to the tie list (p.tie_list), whose replays run in parallel
+//      (tie_item) unless one of them finds the chains do not meet;
+//   1  the sequential walk (fixup_row) corrects the row, and adds its
+//      orbit-error counts (also for tie rows with orbit errors, tie rows
+//      without a list, and list overflow).
 __device__ __forceinline__ void check_row(const TcParams& p, int64_t rr) {
     const int lane = threadIdx.x & 63;
     const SegRecord* recs = p.records + rr * p.nseg;
     long long carry = p.row_seed[rr];
-    bool bad = false;
+    bool bad = false, tie = false;
     int eo = 0, es = 0;
     for (int64_t k0 = 0; k0 < p.nseg; k0 += 64) {
         const int64_t k = k0 + lane;
         SegRecord r;
-        r.seed = r.last = 0;
-        r.flags = r.err_outside = r.err_short = r.pad = 0;
+        r.seed = r.last = r.tie_seed = 0;
+        r.flags = r.err_outside = r.err_short = r.tie_first = r.tie_last = r.pad = 0;
         if (k < p.nseg) r = recs[k];
         const unsigned long long has = __ballot((r.flags & 1) != 0);
         const unsigned long long below = has & ((1ull << lane) - 1ull);
@@ -584,40 +614,148 @@ __device__ __forceinline__ void check_row(const TcParams& p, int64_t rr) {
         const long long left = __shfl(r.last, src);
         const long long E = below ? left : carry;
         if ((r.flags & 2) && r.seed != E) bad = true;
+        if ((r.flags & 6) == 6) tie = true;
         eo += r.err_outside;
         es += r.err_short;
         if (has) carry = __shfl(r.last, 63 - __clzll((long long)has));
     }
-    const bool row_bad = __any(bad);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         eo += __shfl_xor(eo, o);
         es += __shfl_xor(es, o);
     }
+    const bool row_tie = __any(tie);
+    bool row_bad = __any(bad) || (row_tie && (eo > 0 || es > 0 || !p.tie_list));
+    if (row_tie && !row_bad) {  // append the tie segments to the list
+        bool overflow = false;
+        for (int64_t k0 = 0; k0 < p.nseg; k0 += 64) {
+            const int64_t k = k0 + lane;
+            const bool mine = k < p.nseg && (recs[k].flags & 6) == 6;
+            const unsigned long long m = __ballot(mine);
+            if (!m) continue;
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&p.counters[kCntTieList], (unsigned long long)__popcll(m));
+            base = __shfl(base, 0);
+            const unsigned long long idx = base + __popcll(m & ((1ull << lane) - 1ull));
+            if (mine) {
+                if (idx < (unsigned long long)p.tie_cap) p.tie_list[idx] = make_int2((int)rr, (int)k);
+                else overflow = true;
+            }
+        }
+        row_bad = __any(overflow);
+    }
     if (lane == 0) {
-        p.row_flags[rr] = row_bad ? 1 : 0;
+        p.row_flags[rr] = row_bad ? 1 : (row_tie ? 2 : 0);
         if (!row_bad && eo > 0) atomicOr(p.error_flag, 1);
         if (!row_bad && es > 0) atomicOr(p.error_flag, 2);
     }
 }
 
+// Replays one segment's old chain (the chain pass's arithmetic, from `so`)
+// and its corrected chain (ties decided by the reference's arithmetic, from
+// `sn`) in lockstep over AOI columns [c_begin, c_end), until both converge
+// to the same µs at a column >= sync_from; from there on the two are the
+// same chain.  With WRITE every pixel before that point is rewritten with
+// the corrected chain's outcome and C receives the counter corrections;
+// without, nothing is written (a dry run that finds out whether the chains
+// meet).  Returns whether they met; *new_last / *new_has are the corrected
+// chain's last converged time over the replayed columns.
+template <int RS, bool UTM, bool WRITE>
+__device__ __forceinline__ bool replay_segment(const TcParams& p, const OrbitDev& orb, int64_t rr, int64_t c_begin,
+                                               int64_t c_end, int64_t sync_from, long long so, long long sn,
+                                               LaneCounts& C, long long* new_last, bool* new_has) {
+    const int64_t row = p.row0 + rr;
+    const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
+    const int npts = p.anchor_stride;
+    const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
+    const double* alat = UTM ? alon + npts : nullptr;
+    const double inv_flat_area = p.need_geom ? 1.0 / p.row_flat_area[rr] : 0.0;
+    const int64_t obase = rr * p.cols;
+    OrbitCache co = orbit_cache_empty(), cn = orbit_cache_empty();
+    *new_has = false;
+    *new_last = 0;
+    for (int64_t cc = c_begin; cc < c_end; ++cc) {
+        PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
+        if (!q.dem_ok) continue;  // outcome independent of the seed
+        Solve o = solve(p, orb, q, so, p.max_iter, co);
+        Solve n = solve<true>(p, orb, q, sn, p.max_iter, cn);
+        if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t && cc >= sync_from) {
+            if (WRITE && o.it != n.it) {  // same result, different iteration count
+                PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
+                count(C, v.cat, o.it, -1, p.counters);
+                count(C, v.cat, n.it, 1, p.counters);
+            }
+            return true;
+        }
+        if (WRITE) {
+            PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
+            PixelOut vn = finish_pixel<RS>(p, q, n.oc, n.t, n.sat, inv_flat_area);
+            if (vo.cat == CAT_ORBIT) {
+                if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
+                else C.err_short -= 1;
+            } else {
+                count(C, vo.cat, o.it, -1, p.counters);
+            }
+            if (vn.cat == CAT_ORBIT) {
+                if (n.oc == ZD_ORBIT_OUTSIDE) {
+                    C.err_outside += 1;
+                    note_orbit_outside(p, orb, n.t);
+                } else {
+                    C.err_short += 1;
+                }
+            } else {
+                count(C, vn.cat, n.it, 1, p.counters);
+            }
+            write_pixel(p, obase + cc, vn);
+        }
+        if (o.oc == ZD_CONVERGED) so = o.t;
+        if (n.oc == ZD_CONVERGED) {
+            sn = n.t;
+            *new_last = n.t;
+            *new_has = true;
+        }
+    }
+    return false;
+}
+
+// One entry of the tie list: the replay of one tie-only segment, from its
+// first tie pixel.  The dry pass (WRITE false) sends the row to the
+// sequential walk when the chains do not meet inside the segment (the
+// segment's last converged time then changes, and the segments after it
+// must be checked in order); the write pass replays the others.
+template <int RS, bool UTM, bool WRITE>
+__device__ __forceinline__ void tie_item(const TcParams& p, const OrbitDev& orb, int64_t i, int64_t n) {
+    LaneCounts C;
+    zero_counts(C);
+    if (i < n) {
+        const int2 e = p.tie_list[i];
+        const int64_t rr = e.x, sg = e.y;
+        if (!(p.row_flags[rr] & 1)) {
+            const SegRecord rec = p.records[rr * p.nseg + sg];
+            const int64_t c_end = min((int64_t)(sg + 1) * p.seg_len, p.cols);
+            long long nl;
+            bool nh;
+            const bool met = replay_segment<RS, UTM, WRITE>(p, orb, rr, rec.tie_first, c_end, rec.tie_last,
+                                                            rec.tie_seed, rec.tie_seed, C, &nl, &nh);
+            if (!WRITE && !met) atomicOr(&p.row_flags[rr], 1);
+        }
+    }
+    if (WRITE) flush_counts(p, C, true);
+}
+
 // One lane per row: walk the row's segments in order and correct every
-// segment whose starting seed differs from the chained value to its left.
-// Only rows check_row flagged are walked; the other lanes (and lanes whose
+// segment whose starting seed differs from the chained value to its left or
+// whose chain met tie steps (rows check_row or the tie pass sent here,
+// row_flags bit 0).  The old and corrected chains are replayed from the
+// segment's start, or from its first tie pixel when only ties are at issue,
+// until they meet past the last tie pixel.  The other lanes (and lanes whose
 // row is past the end) only join the wave-wide counter reduction.
 template <int RS, bool UTM>
 __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb, int64_t rr, bool valid) {
     LaneCounts C;
     zero_counts(C);
     unsigned redone = 0;
-    if (valid && p.row_flags[rr]) {
-        const int64_t row = p.row0 + rr;
-        const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
-        const int npts = p.anchor_stride;
-        const double* alon = UTM ? p.anchors + rr * 2 * npts : nullptr;
-        const double* alat = UTM ? alon + npts : nullptr;
-        const double inv_flat_area = p.need_geom ? 1.0 / p.row_flat_area[rr] : 0.0;
-        const int64_t obase = rr * p.cols;
+    if (valid && (p.row_flags[rr] & 1)) {
         SegRecord* recs = p.records + rr * p.nseg;
         long long E = p.row_seed[rr];
         SegRecord r1 = recs[0], r2 = p.nseg > 1 ? recs[1] : r1;  // two records in flight ahead
@@ -627,56 +765,24 @@ __device__ __forceinline__ void fixup_row(const TcParams& p, const OrbitDev& orb
             if (sg + 2 < p.nseg) r2 = recs[sg + 2];
             C.err_outside += rec.err_outside;
             C.err_short += rec.err_short;
-            if ((rec.flags & 2) && rec.seed != E) {
+            const bool ties = (rec.flags & 4) != 0;
+            if ((rec.flags & 2) && (rec.seed != E || ties)) {
                 ++redone;
-                // replay the old (rec.seed) and corrected (E) chains in lockstep
-                const int64_t c_begin = sg * p.seg_len, c_end = min(c_begin + p.seg_len, p.cols);
-                long long so = rec.seed, sn = E, new_last = 0;
-                OrbitCache co = orbit_cache_empty(), cn = orbit_cache_empty();
-                bool new_has = false, synced = false;
-                for (int64_t cc = c_begin; cc < c_end; ++cc) {
-                    PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
-                    if (!q.dem_ok) continue;  // outcome independent of the seed
-                    Solve o = solve(p, orb, q, so, p.max_iter, co);
-                    Solve n = solve(p, orb, q, sn, p.max_iter, cn);
-                    if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t) {
-                        if (o.it != n.it) {  // same result, different iteration count
-                            PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
-                            count(C, v.cat, o.it, -1, p.counters);
-                            count(C, v.cat, n.it, 1, p.counters);
-                        }
-                        synced = true;
-                        break;
-                    }
-                    PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
-                    PixelOut vn = finish_pixel<RS>(p, q, n.oc, n.t, n.sat, inv_flat_area);
-                    if (vo.cat == CAT_ORBIT) {
-                        if (o.oc == ZD_ORBIT_OUTSIDE) C.err_outside -= 1;
-                        else C.err_short -= 1;
-                    } else {
-                        count(C, vo.cat, o.it, -1, p.counters);
-                    }
-                    if (vn.cat == CAT_ORBIT) {
-                        if (n.oc == ZD_ORBIT_OUTSIDE) {
-                            C.err_outside += 1;
-                            note_orbit_outside(p, orb, n.t);
-                        } else {
-                            C.err_short += 1;
-                        }
-                    } else {
-                        count(C, vn.cat, n.it, 1, p.counters);
-                    }
-                    write_pixel(p, obase + cc, vn);
-                    if (o.oc == ZD_CONVERGED) so = o.t;
-                    if (n.oc == ZD_CONVERGED) {
-                        sn = n.t;
-                        new_last = n.t;
-                        new_has = true;
-                    }
-                }
-                if (!synced) {
-                    rec.last = new_last;
-                    rec.flags = (rec.flags & ~1) | (new_has ? 1 : 0);
+                const int64_t c_seg = sg * p.seg_len, c_end = min(c_seg + p.seg_len, p.cols);
+                const bool from_tie = rec.seed == E;  // the chain is right up to its first tie pixel
+                long long new_last;
+                bool new_has;
+                const bool met = replay_segment<RS, UTM, true>(
+                    p, orb, rr, from_tie ? (int64_t)rec.tie_first : c_seg, c_end,
+                    ties ? (int64_t)rec.tie_last : c_seg, from_tie ? rec.tie_seed : rec.seed,
+                    from_tie ? rec.tie_seed : E, C, &new_last, &new_has);
+                if (!met) {
+                    // the corrected chain's last converged time; when the
+                    // replay began at the first tie pixel and nothing
+                    // converged from there on, the one before that pixel
+                    const bool has = new_has || (from_tie && (rec.flags & 8));
+                    rec.last = new_has ? new_last : (from_tie ? rec.tie_seed : 0);
+                    rec.flags = (rec.flags & ~1) | (has ? 1 : 0);
                 }
             }
             if (rec.flags & 1) E = rec.last;

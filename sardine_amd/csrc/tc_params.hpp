@@ -15,18 +15,29 @@ constexpr int kNumCounters = kCntRedo + 1;
 // above its last state vector (µs, 0 = none).
 constexpr int kCntOrbitBelow = kNumCounters + 1;
 constexpr int kCntOrbitAbove = kNumCounters + 2;
-constexpr int kCntSlots = kNumCounters + 3;
+// Then the Newton steps re-evaluated with the reference's arithmetic
+// (zero_doppler's tie_stats: steps, decisions changed, largest deviation).
+constexpr int kCntTie = kNumCounters + 3;
+// Then the length of the tie list (check_row).
+constexpr int kCntTieList = kNumCounters + 6;
+constexpr int kCntSlots = kNumCounters + 7;
 constexpr int kMaxAnchorPts = 1025;          // SARDINE_TC_PROJECTOR_ANCHORS ≤ 1024
 constexpr size_t kMaxTcLds = 64 * 1024;      // dynamic LDS budget of the UTM anchors
 
 // Chain bookkeeping of one (row, segment): the seed the segment started
-// from, its last converged time, and its orbit-error counts.
+// from, its last converged time, its orbit-error counts, and where its chain
+// met tie steps (Newton steps its arithmetic could not decide): the seed
+// entering the first such pixel and the first and last such columns.
 struct SegRecord {
     int64_t seed;
     int64_t last;
-    int32_t flags;      // bit 0: last is valid; bit 1: the segment ran a Newton solve
+    int32_t flags;      // bit 0: last is valid; bit 1: the segment ran a Newton solve; bit 2: ties;
+                        // bit 3: a pixel converged before the first tie pixel
     int32_t err_outside;
     int32_t err_short;
+    int32_t tie_first;  // AOI column of the first tie pixel (flags bit 2)
+    int64_t tie_seed;   // the chain's seed entering that pixel
+    int32_t tie_last;   // AOI column of the last tie pixel
     int32_t pad;
 };
 
@@ -76,12 +87,15 @@ struct TcParams {
     double* anchors;                  // UTM: per AOI row lon[stride], lat[stride] of anchors anchor0..
     int anchor0, anchor_stride;       // first stored anchor index, stored anchors per row
     SegRecord* records;               // per (AOI row, segment)
-    int32_t* row_flags;               // per AOI row: 1 = the fix-up pass walks the row
+    int32_t* row_flags;               // per AOI row: 1 = the fix-up pass walks the row, 2 = tie list only
+    int2* tie_list;                   // (AOI row, segment) of tie-only segments, or NULL
+    int tie_cap;                      // its capacity
     const int64_t* row_seed;      // per AOI row, µs relative to the first state vector
     const double* row_flat_area;  // per AOI row (need_geom)
     // config
     int max_iter;
     double tol;
+    double tie_band;  // µs (kTieBandUs unless set with sardine_set_tc_tie_band)
     int flatten, compute_lia, need_geom, resampling;
     double step_deg;
     double step_sin, step_cos;  // sin/cos of step_deg in radians

@@ -313,3 +313,33 @@ def test_tc_chain_segmentation_invariant(small, min_seg, lead):
     assert img.valid_pixel_count == ref.counters()["valid"]
     if lead == 0 and min_seg < 64:
         assert img.chain_redo_rounds > 0  # the replay path ran
+
+
+@pytest.mark.parametrize("band,lead", [(0.4, 2), (0.4, 0), (0.05, 2)])
+def test_tc_tie_replays_change_no_bit(small, band, lead):
+    """Steps inside the tie band are replayed with the reference's own
+    arithmetic (the parallel tie-list pass for rows whose seeds all match,
+    the sequential walk for the others and for list overflow).  A wide band
+    sends most pixels through those replays: the output must be the same
+    bits as with the default band, and match the oracle."""
+    from sardine_amd import _native
+
+    sc, ds, info, s0, nz = small
+    m = ds.prepare_merged()
+    L = _native.lib()
+    cfg = ds.tc_config(compute_lia=True, resampling=abi.RESAMPLE_BICUBIC)
+    base = ds.terrain_correction(m, cfg, radar_coords=True)
+    try:
+        L.sardine_set_tc_tie_band(band)
+        L.sardine_set_tc_lead(lead)
+        img, ref = _tc_compare(sc, ds, info, s0, nz, m, resampling=abi.RESAMPLE_BICUBIC)
+    finally:
+        L.sardine_set_tc_tie_band(0.0)
+        L.sardine_set_tc_lead(2)
+    assert img.tie_steps > 0.2 * band * img.valid_pixel_count  # ~2 band of every step is a tie
+    assert img.tie_decisions_changed <= img.tie_steps
+    assert img.tie_max_dev_us < 2.0 ** -18
+    for a, b in ((img.data, base.data), (img.mask, base.mask), (img.cos_lia, base.cos_lia),
+                 (img.radar_coords, base.radar_coords)):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy(), equal_nan=True)
+    assert img.newton_iter_histogram == base.newton_iter_histogram
