@@ -461,7 +461,7 @@ def _grid_points(orbit, t_start_s, ati, lines, near_range, rps, samples, n_az=10
 
 
 def scene_small_plain(seed: int = 0, lines: int = 2048, samples: int = 4096, out_rows: int = 2048,
-                      out_cols: int = 2048, dem_side: int = 1024) -> SyntheticScene:
+                      out_cols: int = 2048, dem_side: int = 1024, sat_lon_deg: float = None) -> SyntheticScene:
     """Plain method, small: merged 2048 x 4096 intensity from CInt16 DN, LUT
     calibration, 1-arcsec 1024 x 1024 DEM of hills, 2048 x 2048 EPSG:4326
     output at 10 m (10/111320 deg).  Smaller arguments give scaled copies of
@@ -471,7 +471,10 @@ def scene_small_plain(seed: int = 0, lines: int = 2048, samples: int = 4096, out
     rps = 2.329562
     near_range = 800000.0
     t_mid = 80.0
-    orbit = make_orbit(45.3 + 0.2 * rng.uniform(-1, 1), 4.4 + 0.3 * rng.uniform(-1, 1), t_mid)
+    sat_lat = 45.3 + 0.2 * rng.uniform(-1, 1)
+    sat_lon = 4.4 + 0.3 * rng.uniform(-1, 1)  # drawn in any case: the seed's stream stays the same
+    # (the scene centre lies ~4.7° east of the sub-satellite longitude)
+    orbit = make_orbit(sat_lat, sat_lon if sat_lon_deg is None else sat_lon_deg, t_mid)
     t_us, pos, vel = state_vectors(orbit)
     t_start_s = t_mid - 0.5 * lines * ati
     t_start_us = int(round(t_start_s * 1e6))

@@ -174,9 +174,17 @@ def _np_ptr(x):
     return x.ctypes.data_as(C.c_void_p)
 
 
-def test_deburst_and_merge_layout_match_oracle():
-    sc = synthetic.scene_iw_full(4, n_swaths=3, n_bursts=4, lines_per_burst=500, raw_cols=1500, overlap_lines=77,
-                                 swath_overlap=100)
+@pytest.mark.parametrize("full", [False, True])
+def test_deburst_and_merge_layout_match_oracle(full):
+    """The integer deburst/merge layout (deburst.rs:172-305,
+    merge_subswaths.rs:207-311), host only: a small scene and the full
+    3 x 9 x 1500 x 21000 IW scene of the benchmark (50-line burst overlaps)."""
+    if full:
+        sc = synthetic.scene_iw_full(1, out_rows=18000, out_cols=25000)
+    else:
+        sc = synthetic.scene_iw_full(4, n_swaths=3, n_bursts=4, lines_per_burst=500, raw_cols=1500,
+                                     overlap_lines=77, swath_overlap=100)
+    nb = sc.swaths[0].n_bursts
     keep = []
     swc = sc.swath_inputs_c([0, 0, 0], keep)
     info = api.merged_layout(swc, 3)
@@ -188,9 +196,9 @@ def test_deburst_and_merge_layout_match_oracle():
         assert st == 0
         for f in ("lines", "samples", "valid_sample_offset", "n_bursts"):
             assert getattr(Lo, f) == getattr(Ln, f)
-        assert list(Lo.overlap)[:3] == list(Ln.overlap)[:3]
-        assert list(Lo.local_first)[:4] == list(Ln.local_first)[:4]
-        assert list(Lo.local_end)[:4] == list(Ln.local_end)[:4]
+        assert list(Lo.overlap)[:nb - 1] == list(Ln.overlap)[:nb - 1]
+        assert list(Lo.local_first)[:nb] == list(Ln.local_first)[:nb]
+        assert list(Lo.local_end)[:nb] == list(Ln.local_end)[:nb]
         assert info.swath_lines[i] == Lo.lines
     # merge offsets/seams from the oracle's merge on dummy arrays of the same shapes
     sig = [np.zeros((int(info.swath_lines[i]), int(info.swath_samples[i])), np.float32) for i in range(3)]
@@ -203,6 +211,9 @@ def test_deburst_and_merge_layout_match_oracle():
     assert list(mi.seam)[:2] == list(info.seam)[:2]
     assert mi.near_slant_range_time_s == info.near_slant_range_time_s
     assert mi.azimuth_start_time_us == info.azimuth_start_time_us
+    if full:
+        assert (info.lines, info.samples) == (13100, 60814)
+        assert all(int(x) == 50 for x in list(Lo.overlap)[:nb - 1])
 
 
 def test_deburst_errors_through_abi():

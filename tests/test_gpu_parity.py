@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 from sardine_amd import TerrainCorrectionError, abi, api, synthetic
+from test_oracle_reference import _ww15mgh_text
 from sardine_amd.workload import DeviceScene
 
 pytestmark = pytest.mark.gpu
@@ -65,11 +66,17 @@ def test_deburst_merge_calibration_iw_bit_exact(iw):
     assert np.array_equal(b, nz, equal_nan=True)
 
 
-def _tc_compare(sc, ds, info, s0, nz, m, **kw):
+def _tc_compare(sc, ds, info, s0, nz, m, geoid=None, **kw):
     cfg = ds.tc_config(compute_lia=True, **kw)
+    if geoid is not None:
+        cfg.geoid_grid = geoid
     img = ds.terrain_correction(m, cfg, radar_coords=True)
     keep = []
     ccfg = sc.tc_config(compute_lia=1, **{k: (int(v) if isinstance(v, bool) else v) for k, v in kw.items()})
+    if geoid is not None:
+        g = np.ascontiguousarray(geoid, np.float32)
+        keep.append(g)
+        ccfg.geoid = abi.Geoid(abi.GEOID_GRID_2P5, g.ctypes.data)
     ref = O.terrain_correction(s0, nz, _meta(info), sc.scene_c(keep), sc.dem(), sc.dem_spec, ccfg, coords=True)
     assert ref.status == 0, ref.error
     mk = img.mask.cpu().numpy()
@@ -343,3 +350,34 @@ def test_tc_tie_replays_change_no_bit(small, band, lead):
                  (img.radar_coords, base.radar_coords)):
         assert np.array_equal(a.cpu().numpy(), b.cpu().numpy(), equal_nan=True)
     assert img.newton_iter_histogram == base.newton_iter_histogram
+
+
+@pytest.mark.parametrize("sat_lon", [None, -5.8])
+def test_tc_geoid_grid_ww15mgh(cuda, tmp_path, sat_lon):
+    """GeoidModel::Egm96 from a grid file (geoid.rs:220-251; added per pixel
+    at terrain_correction.rs:1545): a smooth synthetic 73 x 144 grid written
+    as an NGA WW15MGH file, read by sardine_geoid_load_ww15mgh and used by
+    the kernels, against the oracle.  sat_lon -5.8 puts the scene at about
+    -1.1° longitude, in the cell that wraps from 357.5° to 360° = 0°."""
+    print(f"geoid scene sat_lon {sat_lon}", flush=True)
+    sc = synthetic.scene_small_plain(9, lines=512, samples=1024, out_rows=300, out_cols=300, dem_side=512,
+                                     sat_lon_deg=sat_lon)
+    if sat_lon is not None:
+        assert -2.5 < sc.bbox[2] and sc.bbox[3] < 0.0
+    ds = DeviceScene.build(sc, cuda, host_raw=True)
+    info, s0, nz, _ = O.prepare_merged(sc)
+    lat = 90.0 - 2.5 * np.arange(73)[:, None]
+    lon = 2.5 * np.arange(144)[None, :]
+    g = (40 * np.sin(np.radians(2 * lat)) * np.cos(np.radians(lon)) + 12 * np.cos(np.radians(3 * lon))
+         + 5 * np.sin(np.radians(5 * lat + lon))).round(6)
+    p = tmp_path / "WW15MGH.GRD"
+    p.write_text(_ww15mgh_text(g))
+    grid = api.load_geoid_grid(p, ascii=True)
+    assert grid.shape == (73, 144) and np.array_equal(grid, g.astype(np.float32))
+    m = ds.prepare_merged()
+    img, ref = _tc_compare(sc, ds, info, s0, nz, m, geoid=grid, flatten=True)
+    assert img.valid_pixel_count > 10000
+    # the undulation (tens of metres here) moves the geolocation
+    flat = ds.terrain_correction(m, ds.tc_config(compute_lia=True, flatten=True), radar_coords=True)
+    moved = np.nanmax(np.abs(flat.radar_coords.cpu().numpy()[..., 1] - img.radar_coords.cpu().numpy()[..., 1]))
+    assert moved > 1.0
