@@ -255,6 +255,31 @@ def test_refined_lee_pick_edge_cases(cuda):
         assert np.array_equal(out, ref, equal_nan=True), np.argwhere(~((out == ref) | (np.isnan(out) & np.isnan(ref))))[:5]
 
 
+@pytest.mark.parametrize("rows,cols,nan_frac,neg", [
+    (300, 200, 0.4, False),   # most windows hold a no-data sample: nearly every pixel takes the exact path
+    (250, 131, 0.0, True),    # signed values (means near zero, skipped directions), partial last strip
+    (1000, 5, 0.02, False),   # one strip narrower than the halo, eight row blocks
+    (3, 70, 0.0, False),      # fewer rows than the window
+    (1, 1, 0.0, False),
+])
+def test_refined_lee_walk_shapes(cuda, rows, cols, nan_frac, neg):
+    """The column walk at raster edges: partial strips and row blocks, rasters
+    smaller than the window, dense no-data (the exact path's pixel list fills
+    up) and signed samples; bit-exact against the oracle."""
+    import torch
+
+    rng = np.random.default_rng(rows * 7 + cols)
+    x = rng.exponential(1.0, (rows, cols)).astype(np.float32)
+    if neg:
+        x = (x - 1.0).astype(np.float32)
+    x[rng.random((rows, cols)) < nan_frac] = np.nan
+    st, ref = O.speckle(x, abi.SPECKLE_REFINED_LEE, 7, 2.0, 1.0)
+    assert st == 0
+    out = api.SpeckleFilter(abi.SPECKLE_REFINED_LEE, 7, 2.0, 1.0).apply(
+        torch.from_numpy(x).to(cuda), cols, rows).cpu().numpy()
+    assert np.array_equal(out, ref, equal_nan=True)
+
+
 def test_to_db(cuda):
     import torch
 
