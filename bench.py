@@ -103,6 +103,29 @@ def tc_flops(img, need_geom: bool) -> float:
     return float(f)
 
 
+def measured_valu(kernel: str, workload: str, launch_ms: float):
+    """Hardware FP64 VALU use of `kernel` from the committed rocprofv3 --pmc
+    summary of this bench command (SQ_INSTS_VALU_*_F64, SQ_INSTS_VALU_FLOPS_FP64,
+    SQ_ACTIVE_INST_VALU, ...): the FP64 lane flops one launch issues, divided
+    by this run's launch time, against the FP64 peak; plus the counter run's
+    VALU-busy share and FP64 share of VALU instructions.  None without a
+    summary for this workload."""
+    f = ROOT / "profiles" / f"valu_counters_{workload}.json"
+    if not f.exists():
+        return None
+    k = json.loads(f.read_text())["kernels"].get(kernel)
+    if k is None:
+        return None
+    flops = k["fp64_lane_flops"]
+    return {"fp64_lane_flops_per_launch": flops,
+            "tflops": round(flops / (launch_ms / 1e3) / 1e12, 4),
+            "frac": round(flops / (launch_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 5),
+            "valu_busy": round(k["valu_busy"], 4),
+            "f64_share_of_valu_instructions": round(k["f64_share_of_valu_instructions"], 4),
+            "vgprs": k["vgprs"], "waves_per_simd": round(k["waves_resident_per_simd"], 2),
+            "source": str(f.relative_to(ROOT))}
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE passes of this bench command, each
@@ -236,7 +259,10 @@ def main():
         "flops_per_launch": flops,
         "launch_ms": round(seg_ms, 4),
         "tc_total_ms": round(tc_ms, 4),
-        "note": "FP64 VALU-bound (no dense contraction on the path); peak is MI355X FP64 78.6 TFLOP/s",
+        "note": "FP64 VALU-bound (no dense contraction on the path); peak is MI355X FP64 78.6 TFLOP/s; "
+                "achieved counts the reference algorithm's flops (product-form Lagrange), hw_fp64 the FP64 "
+                "lane flops the kernel issues (hardware counters)",
+        "hw_fp64": measured_valu("tc_segment_kernel", workload, seg_ms),
     }
     other = {
         "calibrate_merge": {"ms": round(cal_ms, 4), "algorithmic_bytes": cal_bytes,
@@ -249,7 +275,8 @@ def main():
         sb = px * 8
         other["refined_lee7"] = {"ms": round(sp_ms, 4), "algorithmic_bytes": sb,
                                  "GBps": round(sb / (sp_ms / 1e3) / 1e9, 1),
-                                 "hbm_frac": round(sb / (sp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+                                 "hbm_frac": round(sb / (sp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "hw_fp64": measured_valu("refined_lee_walk_kernel", workload, sp_ms)}
 
     cpu = None
     if not args.no_cpu_baseline:
