@@ -7,11 +7,14 @@ call raises NativeLibraryError.  Build it with `python -m sardine_amd.build`
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 from . import abi
 
-LIB_PATH = pathlib.Path(__file__).resolve().parent / "libsardine_hip.so"
+# SARDINE_LIB selects another build of the same library (the host-sanitizer
+# build, libsardine_hip_san.so); the default is the in-tree release build.
+LIB_PATH = pathlib.Path(os.environ.get("SARDINE_LIB") or pathlib.Path(__file__).resolve().parent / "libsardine_hip.so")
 
 
 class NativeLibraryError(RuntimeError):
