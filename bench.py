@@ -711,11 +711,12 @@ def bench_serve(args, dev):
 
 
 def bench_fp32(args, dev):
-    """The full IW scene geocoded by the FP32 scene-centred variant and by the
-    FP64 path (which the parity tests hold to the oracle), per resampler and
-    with flattening: max/RMS azimuth-time error (s), range-pixel error and
-    σ0 dB error of the variant over the pixels valid in both, mask
-    agreement, and the TC kernel time of each."""
+    """The full IW scene geocoded by the FP32 scene-centred variant, the mixed
+    form (FP64 Newton, FP32 resampling and terrain geometry) and the FP64
+    path (which the parity tests hold to the oracle), per resampler and with
+    flattening: max/RMS azimuth-time error (s), range-pixel error and σ0 dB
+    error of each variant over the pixels valid in both, mask agreement, and
+    the chain kernel time of each."""
     import numpy as np
     import torch
 
@@ -745,34 +746,47 @@ def bench_fp32(args, dev):
         L.sardine_set_profiling(0)
         return img, ms
 
-    for name, kw in variants:
-        ref, _ = run(abi.PRECISION_FP64, kw, True)
-        v32, _ = run(abi.PRECISION_FP32_CENTRED, kw, True)
-        _, ms64 = run(abi.PRECISION_FP64, kw, False)
-        _, ms32 = run(abi.PRECISION_FP32_CENTRED, kw, False)
-        a, b = ref.radar_coords, v32.radar_coords
+    def errors(ref, v):
+        a, b = ref.radar_coords, v.radar_coords
         both = torch.isfinite(a[..., 0]) & torch.isfinite(b[..., 0])
         dl = (b[..., 0] - a[..., 0])[both].double()
         ds_ = (b[..., 1] - a[..., 1])[both].double()
-        s64, s32 = ref.data, v32.data
+        s64, s32 = ref.data, v.data
         ok = torch.isfinite(s64) & torch.isfinite(s32) & (s64 > 0) & (s32 > 0)
         ddb = (10.0 * torch.log10(s32[ok].double() / s64[ok].double())).abs()
-        mask_eq = float((ref.mask == v32.mask).double().mean())
-        out[name] = {
+        return {
             "azimuth_time_err_s": {"max": float(dl.abs().max() * ati), "rms": float(dl.pow(2).mean().sqrt() * ati)},
             "range_px_err": {"max": float(ds_.abs().max()), "rms": float(ds_.pow(2).mean().sqrt())},
             "sigma0_db_err": {"max": float(ddb.max()), "rms": float(ddb.pow(2).mean().sqrt()),
-                              "p99": float(torch.quantile(ddb[:: max(1, ddb.numel() // 2000000)].float(), 0.99))},
-            "mask_agreement": mask_eq,
-            "valid_fp64": ref.valid_pixel_count, "valid_fp32": v32.valid_pixel_count,
-            "tc_kernel_ms": {"fp64": round(ms64, 3), "fp32": round(ms32, 3)},
-            "mpx_s": {"fp64": round(rows * cols / ms64 / 1e3, 1), "fp32": round(rows * cols / ms32 / 1e3, 1)},
+                              "p99": float(torch.quantile(ddb[:: max(1, ddb.numel() // 2000000)].float(), 0.99)),
+                              "above_1e-3_db": int((ddb > 1e-3).sum())},
+            "mask_agreement": float((ref.mask == v.mask).double().mean()),
+            "valid": v.valid_pixel_count,
+        }
+
+    for name, kw in variants:
+        ref, _ = run(abi.PRECISION_FP64, kw, True)
+        v32, _ = run(abi.PRECISION_FP32_CENTRED, kw, True)
+        e32 = errors(ref, v32)
+        del v32
+        vmx, _ = run(abi.PRECISION_MIXED, kw, True)
+        emx = errors(ref, vmx)
+        valid64 = ref.valid_pixel_count
+        del ref, vmx
+        torch.cuda.empty_cache()
+        _, ms64 = run(abi.PRECISION_FP64, kw, False)
+        _, ms32 = run(abi.PRECISION_FP32_CENTRED, kw, False)
+        _, msmx = run(abi.PRECISION_MIXED, kw, False)
+        out[name] = {
+            "fp32_centred": e32, "mixed": emx, "valid_fp64": valid64,
+            "tc_kernel_ms": {"fp64": round(ms64, 3), "fp32": round(ms32, 3), "mixed": round(msmx, 3)},
+            "mpx_s": {"fp64": round(rows * cols / ms64 / 1e3, 1), "fp32": round(rows * cols / ms32 / 1e3, 1),
+                      "mixed": round(rows * cols / msmx / 1e3, 1)},
         }
         log(f"{name}: {json.dumps(out[name])}")
-        del ref, v32
         torch.cuda.empty_cache()
     line = {
-        "metric": "FP32 scene-centred variant vs FP64: errors and TC Mpx/s",
+        "metric": "FP32 scene-centred and mixed-precision variants vs FP64: errors and TC Mpx/s",
         "value": out["bilinear"]["mpx_s"]["fp32"],
         "unit": "Mpx/s",
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": out["bilinear"]["tc_kernel_ms"]["fp32"],

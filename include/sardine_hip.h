@@ -239,8 +239,13 @@ typedef enum sardine_resampling {
 
 /* Arithmetic of the zero-Doppler / orbit / geometry part. */
 typedef enum sardine_precision {
-    SARDINE_PRECISION_FP64 = 0,       /* the reference's arithmetic */
-    SARDINE_PRECISION_FP32_CENTRED = 1 /* scene-centred FP32 variant */
+    SARDINE_PRECISION_FP64 = 0,        /* the reference's arithmetic */
+    SARDINE_PRECISION_FP32_CENTRED = 1, /* scene-centred FP32 variant */
+    /* FP64 orbit, Newton, slant range and image coordinates; FP32 resampling
+     * weights and sums, NESZ lookup, and terrain geometry (normal, look
+     * vector, cos LIA, flattening weight from FP64 ECEF differences).
+     * sardine_terrain_correction only (not batched or served jobs). */
+    SARDINE_PRECISION_MIXED = 2
 } sardine_precision;
 
 /* TerrainCorrectionConfig, terrain_correction.rs:100-235, plus the two env

@@ -80,7 +80,7 @@ STATUS_NAMES = {v: k for k, v in globals().items() if k.startswith("ERR_") or k 
 GEOID_ZERO, GEOID_GRID_2P5 = 0, 1
 CRS_WGS84, CRS_UTM_NORTH, CRS_UTM_SOUTH = 0, 1, 2
 RESAMPLE_BILINEAR, RESAMPLE_BICUBIC, RESAMPLE_LANCZOS3 = 0, 1, 2
-PRECISION_FP64, PRECISION_FP32_CENTRED = 0, 1
+PRECISION_FP64, PRECISION_FP32_CENTRED, PRECISION_MIXED = 0, 1, 2
 SPECKLE_BOXCAR, SPECKLE_LEE, SPECKLE_FROST, SPECKLE_GAMMA_MAP, SPECKLE_REFINED_LEE = 0, 1, 2, 3, 4
 
 # mask_code (terrain_correction.rs:346-376)

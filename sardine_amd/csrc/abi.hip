@@ -601,7 +601,8 @@ sardine_status validate_tc(const sardine_radar_image* merged, const sardine_scen
     if (cfg->max_newton_iterations < 0 || cfg->max_newton_iterations > 255)
         return fail(SARDINE_ERR_TC_CONFIG, "max_newton_iterations %d outside 0..255", cfg->max_newton_iterations);
     if (cfg->resampling < 0 || cfg->resampling > 2) return fail(SARDINE_ERR_TC_CONFIG, "bad resampling kernel");
-    if (cfg->precision != SARDINE_PRECISION_FP64 && cfg->precision != SARDINE_PRECISION_FP32_CENTRED)
+    if (cfg->precision != SARDINE_PRECISION_FP64 && cfg->precision != SARDINE_PRECISION_FP32_CENTRED &&
+        cfg->precision != SARDINE_PRECISION_MIXED)
         return fail(SARDINE_ERR_TC_CONFIG, "bad precision selector");
     if (scene->orbit.count > sd::kMaxSv || scene->orbit.count < 0)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "orbit has %d state vectors (max %d)", scene->orbit.count, sd::kMaxSv);
@@ -751,6 +752,7 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     p.compute_lia = compute_lia ? 1 : 0;
     p.need_geom = need_geom ? 1 : 0;
     p.resampling = cfg->resampling;
+    p.mixed = cfg->precision == SARDINE_PRECISION_MIXED ? 1 : 0;
     p.step_deg = step_deg;
     p.step_sin = std::sin(step_deg * sd::kPi / 180.0);
     p.step_cos = std::cos(step_deg * sd::kPi / 180.0);
@@ -3399,8 +3401,9 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
         const sardine_resident_scene_s* r = st ? nullptr : scenes[jb.scene];
         if (!st && r->device != b->device) st = fail(SARDINE_ERR_INVALID_ARGUMENT, "job %d: scene on another device", i);
         if (!st) st = plan_tc(&r->merged, &r->scene, &r->dem, &jb.config, jb.out, r->dev, &run.plans[(size_t)i]);
-        if (!st && jb.config.precision == SARDINE_PRECISION_FP32_CENTRED)
-            st = fail(SARDINE_ERR_TC_CONFIG, "job %d: the FP32 variant runs through sardine_terrain_correction", i);
+        if (!st && jb.config.precision != SARDINE_PRECISION_FP64)
+            st = fail(SARDINE_ERR_TC_CONFIG, "job %d: the FP32 and mixed variants run through sardine_terrain_correction",
+                      i);
         if (!st && jb.speckle.kind >= 0) {
             int win = 0;
             st = validate_filter(&jb.speckle, &win);

@@ -417,6 +417,41 @@ SD_HD int terrain_geometry_sc(double sl, double cl, double so, double co, double
     return TG_VALID;
 }
 
+// terrain_geometry_sc in the mixed form: the neighbour points and their
+// differences in FP64 (the differences of ECEF coordinates ~6.4e6 m need
+// it), the normal, the look vector's products, the two inverse square roots
+// and the thresholds in FP32.
+SD_HD int terrain_geometry_mixed(double sl, double cl, double so, double co, double n_c, double ss, double cs,
+                                 double he, double hw, double hn, double hs, d3 sat, d3 target, double inv_flat_area,
+                                 float* w_out, float* cos_lia_out) {
+    const double soe = so * cs + co * ss, coe = co * cs - so * ss;  // lon + s
+    const double sow = so * cs - co * ss, cow = co * cs + so * ss;  // lon - s
+    const double sln = sl * cs + cl * ss, cln = cl * cs - sl * ss;  // lat + s
+    const double sls = sl * cs - cl * ss, cls = cl * cs + sl * ss;  // lat - s
+    const d3 pe = ecef_from_n(n_c, sl, cl, soe, coe, he), pw = ecef_from_n(n_c, sl, cl, sow, cow, hw);
+    const d3 pn = ecef_from_n(prime_vertical_near(n_c, sl, sln), sln, cln, so, co, hn);
+    const d3 ps = ecef_from_n(prime_vertical_near(n_c, sl, sls), sls, cls, so, co, hs);
+    const d3 dew = pe - pw, dns = pn - ps, lk64 = sat - target;
+    const float ax = (float)dew.x, ay = (float)dew.y, az = (float)dew.z;
+    const float bx = (float)dns.x, by = (float)dns.y, bz = (float)dns.z;
+    const float lx = (float)lk64.x, ly = (float)lk64.y, lz = (float)lk64.z;
+    const float tx = ay * bz - az * by, ty = az * bx - ax * bz, tz = ax * by - ay * bx;
+    const float l2 = lx * lx + ly * ly + lz * lz;
+    const float lr = tx * lx + ty * ly + tz * lz;
+    if (!(l2 > 0.0f) || lr <= 0.0f) return TG_SHADOW;
+    const float lp = lr * (1.0f / sqrtf(l2));
+    const float t2 = tx * tx + ty * ty + tz * tz;
+    if (t2 == 0.0f) return TG_SHADOW;
+    float cosl = lp * (1.0f / sqrtf(t2));
+    cosl = cosl < 0.0f ? 0.0f : (cosl > 1.0f ? 1.0f : cosl);
+    const float w = lp * (float)inv_flat_area;
+    *w_out = w;
+    *cos_lia_out = cosl;
+    if (w < 0.05f) return TG_FORESHORTENING;
+    if (w > 2.0f) return TG_LAYOVER;
+    return TG_VALID;
+}
+
 // ── inverse transverse Mercator (UTM) ────────────────────────────────────
 // Krüger series of order n^6 in the third flattening n (Karney 2011,
 // "Transverse Mercator with an accuracy of a few nanometers", eqs. 7, 11, 35,

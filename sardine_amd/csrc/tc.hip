@@ -120,13 +120,20 @@ extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t strea
     const bool utm = p->crs != 0;
     hipError_t e = sd_launch_tc_anchors(p, stream);
     if (e != hipSuccess) return e;
-    switch (p->resampling * 2 + (utm ? 1 : 0)) {
+    // RS 0..2: the resampler in FP64; 3..5: the same in the mixed form
+    switch ((p->resampling + (p->mixed ? 3 : 0)) * 2 + (utm ? 1 : 0)) {
         case 0: launch_chain<0, false>(p, stream, seg_start, seg_end); break;
         case 1: launch_chain<0, true>(p, stream, seg_start, seg_end); break;
         case 2: launch_chain<1, false>(p, stream, seg_start, seg_end); break;
         case 3: launch_chain<1, true>(p, stream, seg_start, seg_end); break;
         case 4: launch_chain<2, false>(p, stream, seg_start, seg_end); break;
-        default: launch_chain<2, true>(p, stream, seg_start, seg_end); break;
+        case 5: launch_chain<2, true>(p, stream, seg_start, seg_end); break;
+        case 6: launch_chain<3, false>(p, stream, seg_start, seg_end); break;
+        case 7: launch_chain<3, true>(p, stream, seg_start, seg_end); break;
+        case 8: launch_chain<4, false>(p, stream, seg_start, seg_end); break;
+        case 9: launch_chain<4, true>(p, stream, seg_start, seg_end); break;
+        case 10: launch_chain<5, false>(p, stream, seg_start, seg_end); break;
+        default: launch_chain<5, true>(p, stream, seg_start, seg_end); break;
     }
     return hipGetLastError();
 }

@@ -132,14 +132,16 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
         o.cat = CAT_OUT;
         return o;
     }
-    // the NESZ lookup's taps (same position) are loaded with the image's
+    // the NESZ lookup's taps (same position) are loaded with the image's;
+    // RS >= 3 is the mixed form (FP32 resampling and terrain geometry)
+    constexpr bool MIX = RS >= 3;
     const Bilin nb_idx = bilin_index(p.lines, p.samples, line, sample);
     float sigma0;
     DemTaps nzt = {0.0f, 0.0f, 0.0f, 0.0f};
-    if constexpr (RS == 0) {
+    if constexpr (RS == 0 || RS == 3) {
         const DemTaps it = bilin_load(p.img, p.samples, nb_idx);
         if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
-        sigma0 = bilin_interp(nb_idx, it);
+        sigma0 = MIX ? bilin_interp_mx(nb_idx, it) : bilin_interp(nb_idx, it);
     } else {
         if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
         sigma0 = resample<RS>(p, line, sample);
@@ -157,8 +159,17 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
             }
         } else {
             double w = 0.0, cl = 0.0;
-            int g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w, q.nb.n,
+            int g;
+            if constexpr (MIX) {
+                float wf = 0.0f, clf = 0.0f;
+                g = terrain_geometry_mixed(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w,
+                                           q.nb.n, q.nb.s, sat, q.target, inv_flat_area, &wf, &clf);
+                w = wf;
+                cl = clf;
+            } else {
+                g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w, q.nb.n,
                                         q.nb.s, sat, q.target, inv_flat_area, &w, &cl);
+            }
             if (g == TG_VALID) {
                 o.cos_lia = (float)cl;
                 if (p.flatten) sigma0 /= (float)w;
@@ -179,7 +190,7 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
         }
     }
     if (p.nesz) {
-        float nz = bilin_interp(nb_idx, nzt);
+        float nz = MIX ? bilin_interp_mx(nb_idx, nzt) : bilin_interp(nb_idx, nzt);
         if (!isnan(nz) && sigma0 <= nz * p.nesz_factor) {
             o.cat = CAT_NOISE;
             o.mask = 8;
@@ -455,7 +466,7 @@ __device__ __forceinline__ TcParams vgpr_params(const TcParams& p) {
 // PIN: use vgpr_params (pays where the kernel has spare VGPRs: the UTM
 // bilinear and bicubic chains; the Lanczos, EPSG:4326 and batched kernels
 // are at the register limit already and would spill to scratch instead).
-template <int RS, bool UTM, bool PIN = (UTM && RS <= 1)>
+template <int RS, bool UTM, bool PIN = (UTM && RS % 3 <= 1)>
 __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                              int64_t seg) {
     const TcParams p = PIN ? vgpr_params(p_arg) : p_arg;

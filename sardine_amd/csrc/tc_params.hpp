@@ -97,6 +97,7 @@ struct TcParams {
     double tol;
     double tie_band;  // µs (kTieBandUs unless set with sardine_set_tc_tie_band)
     int flatten, compute_lia, need_geom, resampling;
+    int mixed;  // FP32 resampling and terrain geometry (the chain kernels' RS + 3)
     double step_deg;
     double step_sin, step_cos;  // sin/cos of step_deg in radians
     float nesz_factor;

@@ -184,14 +184,21 @@ __device__ __forceinline__ float bicubic(const float* d, int64_t nl, int64_t ns,
         ws[si] = keys_w((double)(si - 1) - ds);
         cs[si] = clampi(sf + si - 1, 0, (int)ns - 1);
     }
+    float v[16];  // all gathers in flight before any is used
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+        const float* row = image_row(d, ns, clampi(lf + li - 1, 0, (int)nl - 1));
+#pragma unroll
+        for (int si = 0; si < 4; ++si) v[li * 4 + si] = row[cs[si]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
     double acc = 0.0;  // row by row, as lanczos3
 #pragma unroll
     for (int li = -1; li <= 2; ++li) {
-        const float* row = image_row(d, ns, clampi(lf + li, 0, (int)nl - 1));
         const double wl = keys_w((double)li - dl);
         double rs = 0.0;
 #pragma unroll
-        for (int si = 0; si < 4; ++si) rs += (double)row[cs[si]] * ws[si];
+        for (int si = 0; si < 4; ++si) rs += (double)v[(li + 1) * 4 + si] * ws[si];
         acc += rs * wl;
     }
     return (float)acc;
@@ -289,22 +296,164 @@ __device__ __forceinline__ float lanczos3(const float* d, int64_t nl, int64_t ns
     int cs[6];
 #pragma unroll
     for (int si = 0; si < 6; ++si) cs[si] = clampi(sf + si - 2, 0, (int)ns - 1);
-    double acc = 0.0;
+    float v[36];  // all gathers in flight before any is used
 #pragma unroll
     for (int li = 0; li < 6; ++li) {
         const float* row = image_row(d, ns, clampi(lf + li - 2, 0, (int)nl - 1));
+#pragma unroll
+        for (int si = 0; si < 6; ++si) v[li * 6 + si] = row[cs[si]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double acc = 0.0;
+#pragma unroll
+    for (int li = 0; li < 6; ++li) {
         double rs = 0.0;
 #pragma unroll
-        for (int si = 0; si < 6; ++si) rs += (double)row[cs[si]] * ws[si];
+        for (int si = 0; si < 6; ++si) rs += (double)v[li * 6 + si] * ws[si];
         acc += rs * wl[li];
     }
     return (float)acc;
 }
 
+// ── the mixed form's FP32 resampling ────────────────────────────────────
+// The same taps and rules as above; the weights (from the fractional
+// coordinates rounded to f32) and the weighted sums are f32.
+
+__device__ __forceinline__ float bilin_interp_mx(const Bilin& b, const DemTaps& t) {
+    const float dl = (float)b.dl, ds = (float)b.ds;
+    const float w00 = (1.0f - dl) * (1.0f - ds), w01 = (1.0f - dl) * ds, w10 = dl * (1.0f - ds), w11 = dl * ds;
+    return t.v00 * w00 + t.v01 * w01 + t.v10 * w10 + t.v11 * w11;
+}
+
+__device__ __forceinline__ float keys_w_mx(float t) {
+    t = fabsf(t);
+    if (t < 1.0f) return 1.5f * t * t * t - 2.5f * t * t + 1.0f;
+    if (t < 2.0f) return -0.5f * t * t * t + 2.5f * t * t - 4.0f * t + 2.0f;
+    return 0.0f;
+}
+
+__device__ __forceinline__ float bicubic_mx(const float* d, int64_t nl, int64_t ns, double line, double sample) {
+    const int lf = (int)floor(line), sf = (int)floor(sample);
+    const float dl = (float)(line - (double)lf), ds = (float)(sample - (double)sf);
+    float ws[4];
+    int cs[4];
+#pragma unroll
+    for (int si = 0; si < 4; ++si) {
+        ws[si] = keys_w_mx((float)(si - 1) - ds);
+        cs[si] = clampi(sf + si - 1, 0, (int)ns - 1);
+    }
+    float v[16];  // all gathers in flight before any is used
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+        const float* row = image_row(d, ns, clampi(lf + li - 1, 0, (int)nl - 1));
+#pragma unroll
+        for (int si = 0; si < 4; ++si) v[li * 4 + si] = row[cs[si]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float acc = 0.0f;
+#pragma unroll
+    for (int li = -1; li <= 2; ++li) {
+        const float wl = keys_w_mx((float)li - dl);
+        float rs = 0.0f;
+#pragma unroll
+        for (int si = 0; si < 4; ++si) rs += v[(li + 1) * 4 + si] * ws[si];
+        acc += rs * wl;
+    }
+    return acc;
+}
+
+// sin and cos of |φ| <= π/6 by Taylor series to φ^9 / φ^10 (first omitted
+// terms below 3e-11, beneath f32 resolution).
+__device__ __forceinline__ void sincos_small_mx(float f, float* s, float* c) {
+    const float z = f * f;
+    float ps = 2.7557319e-06f;  // 1/9!
+    ps = ps * z - 1.9841270e-04f;
+    ps = ps * z + 8.3333333e-03f;
+    ps = ps * z - 1.6666667e-01f;
+    float pc = 2.4801587e-05f;  // 1/8! (the 1/10! term is below 3e-11 here)
+    pc = pc * z - 1.3888889e-03f;
+    pc = pc * z + 4.1666668e-02f;
+    pc = pc * z - 0.5f;
+    *s = f + (f * z) * ps;
+    *c = 1.0f + z * pc;
+}
+
+// lanczos_axis in f32: the same expansion about the nearest offset r.
+__device__ __forceinline__ void lanczos_axis_mx(double u64, float w[6]) {
+    const bool up = u64 >= 0.5;
+    const float r = (float)(up ? u64 - 1.0 : u64);
+    float s3, c3;
+    sincos_small_mx(3.14159265f * r * (1.0f / 3.0f), &s3, &c3);
+    const float s1 = s3 * (3.0f - 4.0f * (s3 * s3));
+    constexpr float h = 0.866025404f;  // sin(π/3)
+    constexpr float K = 0.303963551f;  // 3/π²
+    const float CM[7] = {-1.0f, -0.5f, 0.5f, 1.0f, 0.5f, -0.5f, -1.0f};
+    const float SM[7] = {0.0f, -h, -h, 0.0f, h, h, 0.0f};
+    float a[7], d[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const int m = i - 3;
+        const float x = r + (float)m;
+        const float ax = fabsf(x);
+        const float sp = (m & 1) ? -s1 : s1;
+        // selects, not branches: the chain kernel keeps one basic block here
+        const bool tiny = ax < 1e-10f, in = ax < 3.0f;
+        const float av = sp * (s3 * CM[i] + c3 * SM[i]) * K;
+        a[i] = tiny ? 1.0f : (in ? av : 0.0f);
+        d[i] = (tiny || !in) ? 1.0f : x * x;
+    }
+    float pre[8], suf[8];
+    pre[0] = 1.0f;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) pre[i + 1] = pre[i] * d[i];
+    suf[7] = 1.0f;
+#pragma unroll
+    for (int i = 6; i >= 0; --i) suf[i] = suf[i + 1] * d[i];
+    const float inv = __builtin_amdgcn_rcpf(pre[7]);  // within 1 ulp: the weights are f32 anyway
+    float W[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) W[i] = a[i] * (inv * (pre[i] * suf[i + 1]));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = up ? W[6 - k] : W[5 - k];
+}
+
+__device__ __forceinline__ float lanczos3_mx(const float* d, int64_t nl, int64_t ns, double line, double sample) {
+    const int lf = (int)floor(line), sf = (int)floor(sample);
+    int cs[6];
+#pragma unroll
+    for (int si = 0; si < 6; ++si) cs[si] = clampi(sf + si - 2, 0, (int)ns - 1);
+    // all 36 gathers in flight before any is used (left to itself the
+    // scheduler issues them one at a time in this kernel)
+    float v[36];
+#pragma unroll
+    for (int li = 0; li < 6; ++li) {
+        const float* row = image_row(d, ns, clampi(lf + li - 2, 0, (int)nl - 1));
+#pragma unroll
+        for (int si = 0; si < 6; ++si) v[li * 6 + si] = row[cs[si]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float ws[6], wl[6];
+    lanczos_axis_mx(sample - (double)sf, ws);
+    lanczos_axis_mx(line - (double)lf, wl);
+    float acc = 0.0f;
+#pragma unroll
+    for (int li = 0; li < 6; ++li) {
+        float rs = 0.0f;
+#pragma unroll
+        for (int si = 0; si < 6; ++si) rs += v[li * 6 + si] * ws[si];
+        acc += rs * wl[li];
+    }
+    return acc;
+}
+
+// RS 0..2: bilinear, bicubic, Lanczos-3 in FP64 (the reference's
+// arithmetic); RS 3..5: the same in the mixed form's FP32.
 template <int RS>
 __device__ __forceinline__ float resample(const TcParams& p, double line, double sample) {
     if constexpr (RS == 1) return bicubic(p.img, p.lines, p.samples, line, sample);
     else if constexpr (RS == 2) return lanczos3(p.img, p.lines, p.samples, line, sample);
+    else if constexpr (RS == 4) return bicubic_mx(p.img, p.lines, p.samples, line, sample);
+    else if constexpr (RS == 5) return lanczos3_mx(p.img, p.lines, p.samples, line, sample);
     else return bilinear(p.img, p.lines, p.samples, line, sample);
 }
 

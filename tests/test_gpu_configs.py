@@ -221,6 +221,56 @@ def test_c2_fp32_variant_errors_against_oracle(c1):
             assert mask_eq > 0.999
 
 
+def test_mixed_precision_against_oracle(c1):
+    """The mixed form (FP64 orbit, Newton, slant range and image coordinates;
+    FP32 resampling weights and sums, NESZ lookup and terrain geometry) on
+    full-width bands of the full IW scene against the FP64 oracle, for every
+    resampler and with flattening.  Newton is the FP64 path's, so the
+    iteration histogram and the geolocation hold the FP64 bars (bit-exact,
+    1e-3 px).  σ0 holds 1e-3 dB except where the signed bicubic/Lanczos
+    weights nearly cancel, where an f32 sum is within 2e-6 of Σ|taps| (36
+    taps, 2^-24 each); the FP32 terrain geometry may move a pixel across a
+    foreshortening/layover threshold (w = 0.05, 2.0), counted and bounded."""
+    sc, ds, m, s0 = c1
+    rows, cols, _ = api.output_grid(ds.scene, ds.tc_config())
+    keep = []
+    scene_c = sc.scene_c(keep)
+    n = 96
+    for band, r0 in (("middle", rows // 2), ("bottom", rows - n)):
+        for kind, flatten in ((abi.RESAMPLE_BILINEAR, False), (abi.RESAMPLE_BICUBIC, False),
+                              (abi.RESAMPLE_LANCZOS3, False), (abi.RESAMPLE_BILINEAR, True)):
+            cfg = ds.tc_config(compute_lia=True, flatten=flatten, resampling=kind, aoi=(r0, 0, n, cols),
+                               precision=abi.PRECISION_MIXED)
+            img = ds.terrain_correction(m, cfg, radar_coords=True)
+            c64 = ds.tc_config(compute_lia=True, flatten=flatten, resampling=kind, aoi=(r0, 0, n, cols))
+            ref = O.terrain_correction(s0, None, _meta(m), scene_c, sc.dem(), sc.dem_spec, c64.as_c(), coords=True)
+            assert ref.status == 0
+            assert list(img.newton_iter_histogram) == ref.counters()["hist"]
+            rc = img.radar_coords.cpu().numpy()
+            both = np.isfinite(ref.coords[..., 0])
+            assert np.array_equal(np.isfinite(rc[..., 0]), both)
+            assert np.abs(rc[..., 0][both] - ref.coords[..., 0][both]).max(initial=0) <= 1e-3
+            assert np.abs(rc[..., 1][both] - ref.coords[..., 1][both]).max(initial=0) <= 1e-3
+            mk = img.mask.cpu().numpy()
+            flip = mk != ref.mask
+            geo = {0, 6, 7}  # valid, layover, foreshortening: the classes a threshold separates
+            assert all(int(a) in geo and int(b) in geo for a, b in zip(mk[flip], ref.mask[flip]))
+            d = img.data.cpu().numpy()
+            v = np.isfinite(ref.data) & ~flip
+            assert np.array_equal(np.isfinite(d)[~flip], np.isfinite(ref.data)[~flip])
+            rel = np.abs(d[v] - ref.data[v]) / np.abs(ref.data[v]).clip(1e-30)
+            off = rel > DB_REL
+            n_off = int(off.sum())
+            if n_off:
+                mag = _taps_magnitude(s0, ref.coords[..., 0][v][off], ref.coords[..., 1][v][off], kind)
+                assert (np.abs(d[v][off] - ref.data[v][off]) <= 2e-6 * mag).all()
+            ddb = np.abs(10 * np.log10(d[v][d[v] > 0] / ref.data[v][d[v] > 0]))
+            print(f"mixed {band} rs{kind} flatten={flatten}: σ0 max {ddb.max():.2e} dB, rms {np.sqrt((ddb**2).mean()):.2e} dB, "
+                  f"{n_off} cancelling-sum pixel(s) beyond 1e-3 dB, {int(flip.sum())} mask flip(s) of {mk.size}",
+                  flush=True)
+            assert flip.sum() <= 1e-4 * mk.size
+
+
 @pytest.fixture(scope="module")
 def c4(cuda):
     import torch
