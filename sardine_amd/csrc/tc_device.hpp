@@ -49,13 +49,24 @@ struct PixelPos {
     NbHeights nb;  // the terrain geometry's neighbour heights (when p.need_geom)
 };
 
+// The project_row anchors of one anchor interval.  The interval index
+// depends on the column only: in the chain pass, where the lanes of a wave
+// walk the same columns of 64 rows, a walk reloads the anchors once per
+// interval (anchor_dx / spacing columns) instead of per pixel.  (Passes whose
+// lanes sit at different columns do not use the cache.)
+struct AnchorCache {
+    int a;
+    double lon0, lon1, lat0, lat1;
+};
+__device__ __forceinline__ AnchorCache anchor_cache_empty() { return AnchorCache{-1, 0.0, 0.0, 0.0, 0.0}; }
+
 // Output-grid position of a column and its DEM/ECEF target.  The centre's
 // and (for the terrain geometry) the neighbours' DEM taps are issued
 // together: one memory round trip instead of two; only the four neighbour
 // heights stay live across the Newton solve.
 template <bool UTM, bool NB = true>
 __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, double row_y, const double* alon,
-                                               const double* alat) {
+                                               const double* alat, AnchorCache* ac = nullptr) {
     PixelPos q;
     if constexpr (!UTM) {
         q.lon = p.lon_west + ((double)col + 0.5) * p.spacing;
@@ -68,10 +79,23 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
         while (seg > 0 && !(x > p.x_start + (double)seg * p.anchor_dx)) --seg;
         while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
         const double x0 = p.x_start + (double)seg * p.anchor_dx, x1 = p.x_start + (double)(seg + 1) * p.anchor_dx;
-        const double t = (x1 == x0) ? 0.0 : (x - x0) / (x1 - x0);
         const int a = seg - p.anchor0;  // the row stores anchors anchor0.. only
-        q.lon = alon[a] + t * (alon[a + 1] - alon[a]);
-        q.lat = alat[a] + t * (alat[a + 1] - alat[a]);
+        double lon0, lon1, lat0, lat1;
+        if (ac && ac->a == a) {
+            lon0 = ac->lon0;
+            lon1 = ac->lon1;
+            lat0 = ac->lat0;
+            lat1 = ac->lat1;
+        } else {
+            lon0 = alon[a];  // the four loads in flight together
+            lon1 = alon[a + 1];
+            lat0 = alat[a];
+            lat1 = alat[a + 1];
+            if (ac) *ac = AnchorCache{a, lon0, lon1, lat0, lat1};
+        }
+        const double t = (x1 == x0) ? 0.0 : (x - x0) / (x1 - x0);
+        q.lon = lon0 + t * (lon1 - lon0);
+        q.lat = lat0 + t * (lat1 - lat0);
     }
     const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
     const DemTaps ct = dem_load(p, rc, cc);
@@ -511,9 +535,10 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
     long long seed = S, last_t = 0;
     bool last_has = false, any_newton = false;
     st.tie_flags[lane] = 0;
+    AnchorCache ac = anchor_cache_empty();
     for (int64_t cc = c_begin; cc < c_end; ++cc) {
         const int j = (int)(cc - c_begin);
-        PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
+        PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat, &ac);
         float v_s0, v_cos;
         unsigned char v_mask;
         if (!q.dem_ok) {  // DEM_MISSING (terrain_correction.rs:1529-1536)
