@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SARDINE_ABI_VERSION 2
+#define SARDINE_ABI_VERSION 3
 
 /* ── Status codes ─────────────────────────────────────────────────────────
  * One code per reference error variant on the path:
@@ -203,6 +203,29 @@ typedef struct sardine_radar_image {
 /* A geographic DEM raster with the GLO-30 tile sampling rule
  * (dem.rs:351-372 + bilinear_on_grid dem.rs:385-409): origin is the centre of
  * pixel (0,0), rows run south.  NaN cells are voids. */
+/* A DemMosaic of GLO-30 tiles (dem.rs:559-747) held as one raster: the
+ * tiles, all of one size and pixel size, sit in a grid of `rows` x `cols`
+ * blocks of the raster (north row first, west column first), each block at
+ * its tile's own origin.  The lookup is DemMosaic::elevation_at_linear's:
+ * the first tile in load order (`rank`) whose half-pixel-margined extent
+ * holds the point and whose four taps are all finite answers, with its own
+ * edge clamping (Glo30Tile::elevation_at, dem.rs:351-372); no tile → no
+ * height.  Adjacent tile rows (columns) sit one tile apart, north to south
+ * (west to east), to a quarter pixel; the call rejects other tilings.
+ * (elevation_at, the indexed form, tries the thread's last-hit tile first
+ * and so may answer from the other tile on the line where two tiles'
+ * extents meet, both bounds being inclusive; the linear form is
+ * deterministic.) */
+#define SARDINE_DEM_MAX_TILES_AXIS 8
+typedef struct sardine_dem_tiling {
+    int32_t rows, cols;               /* tile grid, each 1..SARDINE_DEM_MAX_TILES_AXIS */
+    int32_t tile_height, tile_width;  /* pixels per tile (>= 2) */
+    double origin_lat_deg[SARDINE_DEM_MAX_TILES_AXIS]; /* NW pixel centre latitude of each tile row */
+    double origin_lon_deg[SARDINE_DEM_MAX_TILES_AXIS]; /* NW pixel centre longitude of each tile column */
+    int32_t rank[SARDINE_DEM_MAX_TILES_AXIS * SARDINE_DEM_MAX_TILES_AXIS]; /* load order of tile
+                                      [row * 8 + col]; -1: no tile there (NaN block) */
+} sardine_dem_tiling;
+
 typedef struct sardine_dem_raster {
     const float* data; /* [device] height*width */
     int64_t width;
@@ -210,6 +233,10 @@ typedef struct sardine_dem_raster {
     double origin_lat_deg;
     double origin_lon_deg;
     double pixel_deg;
+    /* NULL: one raster sampled by the tile rule (dem.rs:351-372) over its
+     * whole extent; else the mosaic described above (height = rows x
+     * tile_height, width = cols x tile_width). */
+    const sardine_dem_tiling* tiling;
 } sardine_dem_raster;
 
 /* GeoidModel, geoid.rs:342-380 */
@@ -559,6 +586,22 @@ sardine_status sardine_dem_srtm_mosaic(const char* const* paths, int32_t n_paths
  * _UNRECOGNISED_TILE_NAME, _UNSUPPORTED_PIXEL_TYPE in the reference's order. */
 sardine_status sardine_dem_glo30_tile(const char* path, float* dst /*[device]*/, sardine_dem_raster* out,
                                       void* stream);
+
+/* DemMosaic::load_glo30_files (dem.rs:642-653) + elevation_at_linear: the
+ * tiles of `paths`, in that order (the load order decides which tile answers
+ * where two meet; load_directory sorts the paths), decoded as
+ * sardine_dem_glo30_tile does and placed into one raster dst of
+ * out->height x out->width with the grid in *tiling (out->tiling = tiling).
+ * The tiles must share size and pixel size, tiles of one grid row one origin
+ * latitude and of one grid column one origin longitude, adjacent rows and
+ * columns must abut (origins one tile apart), at most one tile per cell,
+ * at most 8 x 8 cells; cells without a tile are NaN.  With dst NULL only
+ * `out` and `tiling` are filled from the headers (host only).  Errors: those
+ * of sardine_dem_glo30_tile per file in order, SARDINE_ERR_DEM_NO_TILES for
+ * an empty list, SARDINE_ERR_INVALID_ARGUMENT for tiles that do not form
+ * such a grid. */
+sardine_status sardine_dem_glo30_mosaic(const char* const* paths, int32_t n_paths, float* dst /*[device]*/,
+                                        sardine_dem_raster* out, sardine_dem_tiling* tiling, void* stream);
 
 /* ── SLC measurement TIFF → HBM ──────────────────────────────────────────── */
 
@@ -913,6 +956,13 @@ void sardine_server_destroy(sardine_server server);
  * functions.  Returns once the work is enqueued. */
 sardine_status sardine_diag_math(int32_t fn, const double* x /*[device]*/, const double* y /*[device]*/,
                                  double* out /*[device]*/, int64_t n, void* stream);
+
+/* Diagnostics, no reference counterpart: the terrain correction's DEM
+ * lookup (the tile rule on one raster, dem.rs:351-409, or a GLO-30 mosaic,
+ * DemMosaic::elevation_at_linear) at n points; NaN where the DEM gives no
+ * height.  For comparing the device lookup with a host one. */
+sardine_status sardine_diag_dem(const sardine_dem_raster* dem, const double* lat /*[device]*/,
+                                const double* lon /*[device]*/, float* out /*[device]*/, int64_t n, void* stream);
 
 /* ── Synthetic inputs (test / benchmark data, no reference counterpart) ── */
 

@@ -53,6 +53,7 @@ def lib():
         L.oracle_flat_cell_area.restype = d
         L.oracle_terrain_geometry.argtypes = [d, d, d, d, d, d, P, P, d, C.POINTER(d), C.POINTER(d)]
         L.oracle_dem_elevation.argtypes = [P, i64, i64, d, d, d, d, d, C.POINTER(f)]
+        L.oracle_dem_mosaic_elevation.argtypes = [P, i64, i64, d, C.POINTER(abi.DemTiling), d, d, C.POINTER(f)]
         L.oracle_geoid_undulation.argtypes = [P, d, d]
         L.oracle_geoid_undulation.restype = d
         L.oracle_utm_forward.argtypes = [i32, i32, d, d, C.POINTER(d), C.POINTER(d)]
@@ -175,6 +176,15 @@ def dem_elevation(dem: np.ndarray, spec, lat, lon):
     return (h.value if ok else None)
 
 
+def dem_mosaic_elevation(dem: np.ndarray, pixel_deg, tiling, lat, lon):
+    """DemMosaic::elevation_at_linear over GLO-30 tiles held as one raster."""
+    dem = np.ascontiguousarray(dem, dtype=np.float32)
+    h = C.c_float()
+    ok = lib().oracle_dem_mosaic_elevation(_ptr(dem), dem.shape[1], dem.shape[0], pixel_deg, C.byref(tiling), lat,
+                                           lon, C.byref(h))
+    return (h.value if ok else None)
+
+
 def geoid_undulation(grid, lat, lon):
     return lib().oracle_geoid_undulation(None if grid is None else _ptr(grid), lat, lon)
 
@@ -231,8 +241,9 @@ def terrain_correction(sigma0: np.ndarray, nesz, merged_meta: dict, scene: abi.S
                          merged_meta["near_slant_range_time_s"], merged_meta["range_pixel_spacing_m"])
     nesz_keep = nesz
     dem = np.ascontiguousarray(dem, dtype=np.float32)
+    tiling = getattr(dem_spec, "tiling", None)  # a GLO-30 mosaic held as one raster
     dr = abi.DemRaster(_ptr(dem), dem.shape[1], dem.shape[0], dem_spec.origin_lat_deg, dem_spec.origin_lon_deg,
-                       dem_spec.pixel_deg)
+                       dem_spec.pixel_deg, C.pointer(tiling) if tiling is not None else None)
     st, rows, cols, gt = tc_output_grid(scene, cfg)
     if st:
         return TcResult(st, abi.GeocodedImage(), None, None, None, "grid")

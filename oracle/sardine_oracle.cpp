@@ -301,11 +301,38 @@ struct HostDem {
     const float* data;
     int64_t width, height;
     double olat, olon, pix;
+    const sardine_dem_tiling* tiling = nullptr;  // a GLO-30 mosaic held as one raster
+    int64_t stride = 0;                          // row pitch of data in floats (0: width)
 };
+
+bool dem_elevation_one(const HostDem& d, double lat, double lon, float* h);
 
 // Returns false for "no value" (outside coverage or a NaN tap): both become
 // DEM_MISSING / a missing neighbour in terrain_correction.
 bool dem_elevation(const HostDem& d, double lat, double lon, float* h) {
+    if (!d.tiling) return dem_elevation_one(d, lat, lon, h);
+    // DemMosaic::elevation_at_linear (dem.rs:730-747): the tiles in load
+    // order, each a Glo30Tile (its block of the raster, its own origin);
+    // the first that answers Some wins.  The mosaic normalises the
+    // longitude before the tile sees it (dem.rs:735), as the tile would.
+    const sardine_dem_tiling& t = *d.tiling;
+    const int M = SARDINE_DEM_MAX_TILES_AXIS;
+    std::vector<std::pair<int, int>> order;  // (rank, cell)
+    for (int i = 0; i < M * M; ++i)
+        if (t.rank[i] >= 0) order.push_back({t.rank[i], i});
+    std::sort(order.begin(), order.end());
+    for (const auto& rc : order) {
+        const int r = rc.second / M, c = rc.second % M;
+        // the tile's block: rows r*th.., cols c*tw.., row pitch the raster width
+        HostDem tile{d.data + (int64_t)r * t.tile_height * d.width + (int64_t)c * t.tile_width, t.tile_width,
+                     t.tile_height, t.origin_lat_deg[r], t.origin_lon_deg[c], d.pix, nullptr, d.width};
+        if (dem_elevation_one(tile, lat, lon, h)) return true;
+    }
+    return false;
+}
+
+// Glo30Tile::elevation_at (dem.rs:351-372) on one raster.
+bool dem_elevation_one(const HostDem& d, double lat, double lon, float* h) {
     double lon_n = lon >= 180.0 ? lon - 360.0 : lon;  // dem.rs:697
     double row_f = (d.olat - lat) / d.pix;
     double col_f = (lon_n - d.olon) / d.pix;
@@ -316,13 +343,13 @@ bool dem_elevation(const HostDem& d, double lat, double lon, float* h) {
     row_f = std::clamp(row_f, 0.0, max_row);
     col_f = std::clamp(col_f, 0.0, max_col);
     // bilinear_on_grid(data, width, row_f, col_f, height-1)
-    int64_t dim = d.width, max_rc = d.height - 1;
+    int64_t dim = d.width, max_rc = d.height - 1, pitch = d.stride ? d.stride : d.width;
     int64_t r0 = std::min<int64_t>((int64_t)row_f, max_rc > 0 ? max_rc - 1 : 0);
     int64_t c0 = std::min<int64_t>((int64_t)col_f, dim >= 2 ? dim - 2 : 0);
     int64_t r1 = r0 + 1, c1 = c0 + 1;
     double dr = row_f - (double)r0, dc = col_f - (double)c0;
-    float v00 = d.data[r0 * dim + c0], v01 = d.data[r0 * dim + c1];
-    float v10 = d.data[r1 * dim + c0], v11 = d.data[r1 * dim + c1];
+    float v00 = d.data[r0 * pitch + c0], v01 = d.data[r0 * pitch + c1];
+    float v10 = d.data[r1 * pitch + c0], v11 = d.data[r1 * pitch + c1];
     if (std::isnan(v00) || std::isnan(v01) || std::isnan(v10) || std::isnan(v11)) return false;
     double v = (double)v00 * (1.0 - dr) * (1.0 - dc) + (double)v01 * (1.0 - dr) * dc +
                (double)v10 * dr * (1.0 - dc) + (double)v11 * dr * dc;
@@ -625,6 +652,13 @@ int oracle_dem_elevation(const float* data, int64_t width, int64_t height, doubl
     return dem_elevation(d, lat, lon, h) ? 1 : 0;
 }
 
+// The same over a GLO-30 mosaic held as one raster (sardine_dem_tiling).
+int oracle_dem_mosaic_elevation(const float* data, int64_t width, int64_t height, double pix,
+                                const sardine_dem_tiling* tiling, double lat, double lon, float* h) {
+    HostDem d{data, width, height, tiling->origin_lat_deg[0], tiling->origin_lon_deg[0], pix, tiling};
+    return dem_elevation(d, lat, lon, h) ? 1 : 0;
+}
+
 double oracle_geoid_undulation(const float* grid, double lat, double lon) {
     sardine_geoid g{grid ? SARDINE_GEOID_GRID_2P5 : SARDINE_GEOID_ZERO, grid};
     return geoid_undulation(g, lat, lon);
@@ -727,7 +761,7 @@ int oracle_terrain_correction(const sardine_radar_image* merged, const sardine_s
     const bool nesz_on = std::isfinite(cfg->noise_floor_margin_db) && merged->nesz != nullptr;
     const float nesz_factor = std::pow(10.0f, cfg->noise_floor_margin_db / 10.0f);
     HostDem dem{dem_r->data, dem_r->width, dem_r->height, dem_r->origin_lat_deg, dem_r->origin_lon_deg,
-                dem_r->pixel_deg};
+                dem_r->pixel_deg, dem_r->tiling};
 
     const float NaN = std::numeric_limits<float>::quiet_NaN();
     struct RowAcc {

@@ -72,6 +72,7 @@ def lib():
                                           P, C.POINTER(abi.GrdImage), P]
     L.sardine_dem_glo30_tile.argtypes = [C.c_char_p, P, C.POINTER(abi.DemRaster), P]
     L.sardine_dem_srtm_mosaic.argtypes = [P, i32, P, C.POINTER(abi.DemRaster), P]
+    L.sardine_dem_glo30_mosaic.argtypes = [P, i32, P, C.POINTER(abi.DemRaster), C.POINTER(abi.DemTiling), P]
     L.sardine_geoid_load_binary.argtypes = [C.c_char_p, i32, P]
     L.sardine_geoid_load_ww15mgh.argtypes = [C.c_char_p, P]
     L.sardine_slc_open.argtypes = [C.c_char_p, C.POINTER(abi.SlcTiff)]
@@ -107,6 +108,7 @@ def lib():
     L.sardine_server_release.argtypes = [P, u64, P]
     L.sardine_server_destroy.argtypes = [P]
     L.sardine_diag_math.argtypes = [i32, P, P, P, i64, P]
+    L.sardine_diag_dem.argtypes = [C.POINTER(abi.DemRaster), P, P, P, i64, P]
     L.sardine_server_destroy.restype = None
     for name in abi.EXPORTED_SYMBOLS:
         if name not in ("sardine_last_error", "sardine_abi_version", "sardine_device_count",

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 NEWTON_HIST_SIZE = 16
 MAX_BURSTS = 64
 MAX_SWATHS = 8
@@ -125,9 +125,19 @@ class RadarImage(C.Structure):
                 ("range_pixel_spacing_m", C.c_double)]
 
 
+DEM_MAX_TILES_AXIS = 8
+
+
+class DemTiling(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("tile_height", C.c_int32), ("tile_width", C.c_int32),
+                ("origin_lat_deg", C.c_double * DEM_MAX_TILES_AXIS), ("origin_lon_deg", C.c_double * DEM_MAX_TILES_AXIS),
+                ("rank", C.c_int32 * (DEM_MAX_TILES_AXIS * DEM_MAX_TILES_AXIS))]
+
+
 class DemRaster(C.Structure):
     _fields_ = [("data", C.c_void_p), ("width", C.c_int64), ("height", C.c_int64),
-                ("origin_lat_deg", C.c_double), ("origin_lon_deg", C.c_double), ("pixel_deg", C.c_double)]
+                ("origin_lat_deg", C.c_double), ("origin_lon_deg", C.c_double), ("pixel_deg", C.c_double),
+                ("tiling", C.POINTER(DemTiling))]
 
 
 class Geoid(C.Structure):
@@ -287,6 +297,7 @@ EXPORTED_SYMBOLS = (
     "sardine_geoid_load_binary",
     "sardine_dem_srtm_mosaic",
     "sardine_dem_glo30_tile",
+    "sardine_dem_glo30_mosaic",
     "sardine_geoid_load_ww15mgh",
     "sardine_slc_open",
     "sardine_slc_read_lines",
@@ -302,6 +313,7 @@ EXPORTED_SYMBOLS = (
     "sardine_write_tiff_no_crs",
     "sardine_synth_cint16",
     "sardine_diag_math",
+    "sardine_diag_dem",
     "sardine_set_profiling",
     "sardine_last_kernel_ms",
     "sardine_recent_kernel_ms",

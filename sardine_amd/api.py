@@ -120,15 +120,18 @@ class MergedSigma0:
 
 @dataclass
 class DemRaster:
-    """Geographic DEM raster with the GLO-30 sampling rule (dem.rs:351-409)."""
+    """Geographic DEM raster with the GLO-30 sampling rule (dem.rs:351-409);
+    with `tiling`, a GLO-30 mosaic held as one raster (DemMosaic, dem.rs:559-747)."""
     data: object                # torch.float32 (height, width) on cuda
     origin_lat_deg: float       # centre of pixel (0, 0)
     origin_lon_deg: float
     pixel_deg: float
+    tiling: object = None       # abi.DemTiling or None
 
     def as_c(self) -> abi.DemRaster:
         return abi.DemRaster(self.data.data_ptr(), int(self.data.shape[1]), int(self.data.shape[0]),
-                             float(self.origin_lat_deg), float(self.origin_lon_deg), float(self.pixel_deg))
+                             float(self.origin_lat_deg), float(self.origin_lon_deg), float(self.pixel_deg),
+                             C.pointer(self.tiling) if self.tiling is not None else None)
 
 
 # ─── terrain correction ───────────────────────────────────────────────────
@@ -445,6 +448,21 @@ def load_glo30_tile(path, device="cuda", stream=None) -> "DemRaster":
     data = torch.empty((r.height, r.width), dtype=torch.float32, device=device)
     _check(lib().sardine_dem_glo30_tile(str(path).encode(), data.data_ptr(), C.byref(r), _stream_ptr(stream)))
     return DemRaster(data, r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg)
+
+
+def load_glo30_mosaic(paths, device="cuda", stream=None) -> "DemRaster":
+    """DemMosaic::load_glo30_files (dem.rs:642-653): GLO-30 tiles, in load
+    order, as one device raster with its tile grid; the terrain correction
+    samples it as DemMosaic::elevation_at_linear (first tile in load order
+    that answers, each with its own origin and edge rule)."""
+    torch = _torch()
+    arr = (C.c_char_p * max(len(paths), 1))(*[str(p).encode() for p in paths])
+    r, t = abi.DemRaster(), abi.DemTiling()
+    _check(lib().sardine_dem_glo30_mosaic(arr, len(paths), None, C.byref(r), C.byref(t), None))
+    data = torch.empty((r.height, r.width), dtype=torch.float32, device=device)
+    _check(lib().sardine_dem_glo30_mosaic(arr, len(paths), data.data_ptr(), C.byref(r), C.byref(t),
+                                          _stream_ptr(stream)))
+    return DemRaster(data, r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg, t)
 
 
 class SlcReader:

@@ -44,6 +44,8 @@ extern "C" hipError_t sd_launch_tc_fp32(const sd::TcParams* p, float* ends, hipS
 extern "C" hipError_t sd_launch_lut_dense(const uint32_t* pixels, const float* values, const int64_t* offsets,
                                           int n_vec, int64_t dense_cols, float* dense, hipStream_t stream);
 extern "C" hipError_t sd_launch_cal_merge(const sd::CalParams* p, hipStream_t stream);
+extern "C" hipError_t sd_launch_dem_probe(const sd::TcParams* p, const double* lat, const double* lon, float* out,
+                                          int64_t n, hipStream_t stream);
 extern "C" hipError_t sd_launch_speckle(int kind, const float* in, float* out, int64_t cols, int64_t rows, int half,
                                         float enl, float damping, hipStream_t stream);
 extern "C" hipError_t sd_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
@@ -562,8 +564,88 @@ hipError_t upload_orbit(DevBuf& b, const sd::OrbitDev& o, hipStream_t s) {
 }
 
 // Device copies of a scene's orbit and seed table.
+// The tiling of a mosaic raster: grid within bounds, covering the raster,
+// ranks in range and only in cells of the grid.
+sardine_status validate_dem_tiling(const sardine_dem_raster* dem) {
+    const sardine_dem_tiling* tl = dem->tiling;
+    if (!tl) return SARDINE_OK;
+    const int M = SARDINE_DEM_MAX_TILES_AXIS;
+    if (tl->rows < 1 || tl->rows > M || tl->cols < 1 || tl->cols > M || tl->tile_height < 2 || tl->tile_width < 2 ||
+        (int64_t)tl->rows * tl->tile_height != dem->height || (int64_t)tl->cols * tl->tile_width != dem->width)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT,
+                    "DEM tiling: %d x %d tiles of %d x %d pixels (at most %d x %d) must cover the %lld x %lld raster",
+                    tl->rows, tl->cols, tl->tile_height, tl->tile_width, M, M, (long long)dem->height,
+                    (long long)dem->width);
+    for (int i = 0; i < M * M; ++i)
+        if (tl->rank[i] >= M * M || (tl->rank[i] >= 0 && (i / M >= tl->rows || i % M >= tl->cols)))
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "DEM tiling: bad rank at cell %d", i);
+    // The lookup finds a point's tile from tile 0's origin, one step either
+    // way, and looks for a second holding tile only within half a pixel of
+    // an edge: adjacent tile rows (columns) must sit one tile apart, north
+    // to south (west to east), to a quarter pixel.
+    const double px = dem->pixel_deg;
+    for (int i = 0; i < tl->rows; ++i) {
+        const double d = i ? (tl->origin_lat_deg[i - 1] - tl->origin_lat_deg[i]) / px - tl->tile_height : 0.0;
+        if (!std::isfinite(tl->origin_lat_deg[i]) || !(std::fabs(d) <= 0.25))
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "DEM tiling: tile row %d is not one tile south of row %d", i,
+                        i - 1);
+    }
+    for (int i = 0; i < tl->cols; ++i) {
+        const double d = i ? (tl->origin_lon_deg[i] - tl->origin_lon_deg[i - 1]) / px - tl->tile_width : 0.0;
+        if (!std::isfinite(tl->origin_lon_deg[i]) || !(std::fabs(d) <= 0.25))
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "DEM tiling: tile column %d is not one tile east of column %d",
+                        i, i - 1);
+    }
+    return SARDINE_OK;
+}
+
+// The DEM fields of the kernel arguments (raster validated: 2 <= width,
+// height < 2^31, tiling checked); tab: the device copy of the tiling table.
+void fill_dem_params(sd::TcParams& p, const sardine_dem_raster* dem, const double* tab) {
+    p.dem = dem->data;
+    p.dem_w = dem->width;
+    p.dem_h = dem->height;
+    p.dem_olat = dem->origin_lat_deg;
+    p.dem_olon = dem->origin_lon_deg;
+    p.dem_pix = dem->pixel_deg;
+    p.dem_inv_pix = 1.0 / dem->pixel_deg;
+    p.dem_max_row = (double)(dem->height - 1);
+    p.dem_max_col = (double)(dem->width - 1);
+    p.dem_hi_row = p.dem_max_row + 0.5;
+    p.dem_hi_col = p.dem_max_col + 0.5;
+    p.dem_lim_row = (int)(dem->height - 2);
+    p.dem_lim_col = (int)(dem->width - 2);
+    if (const sardine_dem_tiling* tl = dem->tiling) {
+        p.dem_tiled = 1;
+        p.dem_trows = tl->rows;
+        p.dem_tcols = tl->cols;
+        p.dem_th = tl->tile_height;
+        p.dem_tw = tl->tile_width;
+        p.dem_tab = tab;
+    }
+}
+
+// The device table of a DEM tiling (TcParams::dem_tab's layout).
+void dem_tile_table(const sardine_dem_tiling* tl, double tab[sd::kDemTabWords]) {
+    const int M = SARDINE_DEM_MAX_TILES_AXIS;
+    for (int i = 0; i < sd::kDemTabWords; ++i) tab[i] = 0.0;
+    for (int i = 0; i < M; ++i) {
+        tab[i] = tl->origin_lat_deg[i];
+        tab[M + i] = tl->origin_lon_deg[i];
+    }
+    for (int r = 0; r < M; ++r) {
+        uint64_t w = 0;
+        for (int c = 0; c < M; ++c) {
+            const int32_t k = tl->rank[r * M + c];
+            w |= (uint64_t)(k < 0 ? 0xff : k) << (8 * c);
+        }
+        std::memcpy(&tab[2 * M + r], &w, sizeof w);
+    }
+}
+
 struct SceneDev {
     const sd::OrbitDev* orb = nullptr;
+    const double* dem_tab = nullptr;  // GLO-30 mosaic table (sd::kDemTabWords), mosaics only
     const double* lut_lat = nullptr;
     const int64_t* lut_t = nullptr;
     int n_lut = 0;
@@ -633,6 +715,7 @@ sardine_status validate_tc(const sardine_radar_image* merged, const sardine_scen
     }
     if (dem->width >= INT32_MAX || dem->height >= INT32_MAX)
         return fail(SARDINE_ERR_CAPACITY, "DEM dimensions must be below 2^31");
+    if (sardine_status st = validate_dem_tiling(dem)) return st;
     if (cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 && !cfg->geoid.grid_73x144)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "geoid grid pointer is null");
     return SARDINE_OK;
@@ -717,19 +800,7 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     p.line_per_us = 1.0 / (1e6 * ati);
     p.inv_ati = 1.0 / ati;
     p.inv_rps = 1.0 / p.rps;
-    p.dem = dem->data;
-    p.dem_w = dem->width;
-    p.dem_h = dem->height;
-    p.dem_olat = dem->origin_lat_deg;
-    p.dem_olon = dem->origin_lon_deg;
-    p.dem_pix = dem->pixel_deg;
-    p.dem_inv_pix = 1.0 / dem->pixel_deg;
-    p.dem_max_row = (double)(dem->height - 1);
-    p.dem_max_col = (double)(dem->width - 1);
-    p.dem_hi_row = p.dem_max_row + 0.5;
-    p.dem_hi_col = p.dem_max_col + 0.5;
-    p.dem_lim_row = (int)(dem->height - 2);  // validated: 2 <= width, height < 2^31
-    p.dem_lim_col = (int)(dem->width - 2);
+    fill_dem_params(p, dem, sdv.dem_tab);
     p.crs = utm ? 1 : 0;
     p.utm_zone = cfg->utm_zone;
     p.utm_south = south ? 1 : 0;
@@ -1095,6 +1166,7 @@ sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* s
     char* d_lut_t = get(lut.t.size() * sizeof(int64_t) + 1);
     char* d_work = get(w.total());
     char* d_orb32 = fp32 ? get(sizeof(sd::OrbitF32)) : nullptr;
+    char* d_dem_tab = dem->tiling ? get(sd::kDemTabWords * sizeof(double)) : nullptr;
     char* d_ends = fp32 ? get((size_t)pl.nr * 2 * sizeof(float)) : nullptr;
     if (e != hipSuccess) {
         if (mem.fixed() && e == hipErrorOutOfMemory)
@@ -1108,6 +1180,12 @@ sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* s
     SD_HIP(mem.copy_in(d_lut_lat, lut.lat.data(), lut.lat.size() * sizeof(double)));
     SD_HIP(mem.copy_in(d_lut_t, lut.t.data(), lut.t.size() * sizeof(int64_t)));
     pl.p.orbit = reinterpret_cast<const sd::OrbitDev*>(d_orb);
+    if (d_dem_tab) {
+        double tab[sd::kDemTabWords];
+        dem_tile_table(dem->tiling, tab);
+        SD_HIP(mem.copy_in(d_dem_tab, tab, sizeof tab));
+        pl.p.dem_tab = reinterpret_cast<const double*>(d_dem_tab);
+    }
     pl.q.lut_lat = reinterpret_cast<const double*>(d_lut_lat);
     pl.q.lut_t = reinterpret_cast<const int64_t*>(d_lut_t);
     assign_work(pl, w, d_work);
@@ -1574,6 +1652,32 @@ sardine_status sardine_diag_math(int32_t fn, const double* x, const double* y, d
     sardine_status st;
     if ((st = require_device())) return st;
     SD_HIP(sd_launch_pmath(fn, x, y, out, n, (hipStream_t)stream));
+    return SARDINE_OK;
+}
+
+sardine_status sardine_diag_dem(const sardine_dem_raster* dem, const double* lat, const double* lon, float* out,
+                                int64_t n, void* stream) {
+    if (!dem || n < 0 || (n > 0 && (!lat || !lon || !out)))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad diagnostic DEM request");
+    if (!dem->data || dem->width < 2 || dem->height < 2 || dem->width >= INT32_MAX || dem->height >= INT32_MAX ||
+        !(dem->pixel_deg > 0.0))
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "bad DEM raster");
+    sardine_status st;
+    if ((st = validate_dem_tiling(dem))) return st;
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream;
+    sd::TcParams p;
+    std::memset(&p, 0, sizeof p);
+    DevBuf tab;
+    double host_tab[sd::kDemTabWords];
+    if (dem->tiling) {
+        dem_tile_table(dem->tiling, host_tab);
+        SD_HIP(tab.alloc(sizeof host_tab, s));
+        SD_HIP(hipMemcpyAsync(tab.p, host_tab, sizeof host_tab, hipMemcpyHostToDevice, s));
+    }
+    fill_dem_params(p, dem, tab.as<double>());
+    SD_HIP(sd_launch_dem_probe(&p, lat, lon, out, n, s));
+    if (dem->tiling) SD_HIP(hipStreamSynchronize(s));  // host_tab and the table buffer end here
     return SARDINE_OK;
 }
 
@@ -2087,6 +2191,7 @@ static sardine_status sardine_dem_srtm_mosaic_impl(const char* const* paths, int
     if (w > INT32_MAX || h > INT32_MAX || w * h > (int64_t)1 << 40)
         return fail(SARDINE_ERR_INVALID_ARGUMENT, "tiles span %lld x %lld samples", (long long)w, (long long)h);
     sardine_dem_raster r;
+    r.tiling = nullptr;
     r.data = dst;
     r.width = w;
     r.height = h;
@@ -2184,13 +2289,18 @@ T sat_cast(float x) {
 
 extern "C" {
 
-static sardine_status sardine_dem_glo30_tile_impl(const char* path, float* dst, sardine_dem_raster* out, void* stream) {
+// dst null: the header and georeferencing only, unless full_check, which
+// also decodes the blocks on the host (every error Glo30Tile::load can
+// report) and uploads nothing.
+static sardine_status sardine_dem_glo30_tile_impl(const char* path, float* dst, sardine_dem_raster* out, void* stream,
+                                                  bool full_check = false) {
     if (!path || !out) return fail(SARDINE_ERR_INVALID_ARGUMENT, "path or out is null");
     sd::GeoTiffRaster g;
     std::string why;
     // one read of the file; decode errors (rc 3) are reported after the
     // georeferencing, as the reference decodes the image after it
-    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/dst != nullptr, /*defer_fp_predictor=*/true);
+    const int rc = sd::read_geotiff(path, &g, &why, /*decode=*/dst != nullptr || full_check,
+                                    /*defer_fp_predictor=*/true);
     if (rc == 1) return fail(SARDINE_ERR_DEM_IO, "IO error reading %s: %s", path, why.c_str());
     if (rc == 2) return fail(SARDINE_ERR_DEM_TIFF_DECODE, "TIFF decode error reading %s: %s", path, why.c_str());
     double lat0, lon0, px;
@@ -2211,13 +2321,14 @@ static sardine_status sardine_dem_glo30_tile_impl(const char* path, float* dst, 
         lon0 = (double)lo + 0.5 * px;
     }
     sardine_dem_raster r;
+    r.tiling = nullptr;
     r.data = dst;
     r.width = g.width;
     r.height = g.height;
     r.origin_lat_deg = lat0;
     r.origin_lon_deg = lon0;
     r.pixel_deg = px;
-    if (!dst) {
+    if (!dst && !full_check) {
         *out = r;
         return SARDINE_OK;
     }
@@ -2225,6 +2336,10 @@ static sardine_status sardine_dem_glo30_tile_impl(const char* path, float* dst, 
     if (g.sample == sd::TiffSample::Other)
         return fail(SARDINE_ERR_DEM_UNSUPPORTED_PIXEL_TYPE,
                     "GLO-30 tile %s has unsupported pixel type (expected f32 or i16)", path);
+    if (!dst) {
+        *out = r;
+        return SARDINE_OK;
+    }
     // GDAL_NODATA parsed as Rust parses an f32 (after trimming)
     bool have_nd = false;
     float nd = 0.0f;
@@ -2261,6 +2376,120 @@ static sardine_status sardine_dem_glo30_tile_impl(const char* path, float* dst, 
 }
 sardine_status sardine_dem_glo30_tile(const char* path, float* dst, sardine_dem_raster* out, void* stream) {
     return guarded(SARDINE_ERR_DEM_TIFF_DECODE, [&] { return sardine_dem_glo30_tile_impl(path, dst, out, stream); });
+}
+
+// DemMosaic::load_glo30_files (dem.rs:642-653): every tile loaded as one
+// GLO-30 tile (errors in path order), then placed into one raster on a grid
+// of tile rows (origin latitude, north first) and columns (origin
+// longitude, west first); the lookup keeps each tile's own origin and edge
+// rule and the load order (TcParams::dem_tab, tc_pixel.hpp).
+static sardine_status sardine_dem_glo30_mosaic_impl(const char* const* paths, int32_t n, float* dst,
+                                                    sardine_dem_raster* out, sardine_dem_tiling* tiling,
+                                                    void* stream) {
+    if (!out || !tiling || (n > 0 && !paths)) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null argument");
+    if (n <= 0) return fail(SARDINE_ERR_DEM_NO_TILES, "no DEM tiles");
+    const int M = SARDINE_DEM_MAX_TILES_AXIS;
+    std::vector<sardine_dem_raster> hd((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        const sardine_status st = sardine_dem_glo30_tile_impl(paths[i], nullptr, &hd[(size_t)i], nullptr);
+        if (st) {
+            // the reference loads tiles whole, in path order: an earlier
+            // tile's decode or pixel-type error comes first
+            const std::string msg = g_last_error;
+            const sardine_error_detail det = g_last_detail;
+            for (int32_t j = 0; j < i; ++j) {
+                sardine_dem_raster hj;
+                const sardine_status sj = sardine_dem_glo30_tile_impl(paths[j], nullptr, &hj, nullptr, true);
+                if (sj) return sj;
+            }
+            g_last_error = msg;
+            g_last_detail = det;
+            return st;
+        }
+    }
+    const sardine_dem_raster& h0 = hd[0];
+    std::vector<double> lats, lons;
+    for (int32_t i = 0; i < n; ++i) {
+        const sardine_dem_raster& h = hd[(size_t)i];
+        if (h.width != h0.width || h.height != h0.height || h.pixel_deg != h0.pixel_deg)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT,
+                        "GLO-30 mosaic: tile %d (%s) is %lld x %lld at %.17g deg, tile 0 %lld x %lld at %.17g deg; "
+                        "the tiles must share size and pixel size", i, paths[i], (long long)h.height,
+                        (long long)h.width, h.pixel_deg, (long long)h0.height, (long long)h0.width, h0.pixel_deg);
+        if (std::find(lats.begin(), lats.end(), h.origin_lat_deg) == lats.end()) lats.push_back(h.origin_lat_deg);
+        if (std::find(lons.begin(), lons.end(), h.origin_lon_deg) == lons.end()) lons.push_back(h.origin_lon_deg);
+    }
+    if ((int)lats.size() > M || (int)lons.size() > M)
+        return fail(SARDINE_ERR_INVALID_ARGUMENT, "GLO-30 mosaic: %zu x %zu tile rows x columns (at most %d x %d)",
+                    lats.size(), lons.size(), M, M);
+    std::sort(lats.begin(), lats.end(), [](double a, double b) { return a > b; });
+    std::sort(lons.begin(), lons.end());
+    // adjacent rows and columns abut: origins one tile apart (to 1e-6 pixel)
+    const double px = h0.pixel_deg, tol = 1e-6 * px;
+    for (size_t i = 1; i < lats.size(); ++i)
+        if (std::fabs((lats[i - 1] - lats[i]) - (double)h0.height * px) > tol)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "GLO-30 mosaic: tile rows at %.12g and %.12g do not abut",
+                        lats[i - 1], lats[i]);
+    for (size_t i = 1; i < lons.size(); ++i)
+        if (std::fabs((lons[i] - lons[i - 1]) - (double)h0.width * px) > tol)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "GLO-30 mosaic: tile columns at %.12g and %.12g do not abut",
+                        lons[i - 1], lons[i]);
+    sardine_dem_tiling t;
+    std::memset(&t, 0, sizeof t);
+    t.rows = (int32_t)lats.size();
+    t.cols = (int32_t)lons.size();
+    t.tile_height = (int32_t)h0.height;
+    t.tile_width = (int32_t)h0.width;
+    for (int i = 0; i < M * M; ++i) t.rank[i] = -1;
+    for (size_t i = 0; i < lats.size(); ++i) t.origin_lat_deg[i] = lats[i];
+    for (size_t i = 0; i < lons.size(); ++i) t.origin_lon_deg[i] = lons[i];
+    std::vector<int> cell((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        const int r = (int)(std::find(lats.begin(), lats.end(), hd[(size_t)i].origin_lat_deg) - lats.begin());
+        const int c = (int)(std::find(lons.begin(), lons.end(), hd[(size_t)i].origin_lon_deg) - lons.begin());
+        if (t.rank[r * M + c] >= 0)
+            return fail(SARDINE_ERR_INVALID_ARGUMENT, "GLO-30 mosaic: tiles %d and %d (%s) share a grid cell",
+                        t.rank[r * M + c], i, paths[i]);
+        t.rank[r * M + c] = i;
+        cell[(size_t)i] = r * M + c;
+    }
+    const int64_t W = (int64_t)t.cols * h0.width, H = (int64_t)t.rows * h0.height;
+    if (W >= INT32_MAX || H >= INT32_MAX) return fail(SARDINE_ERR_CAPACITY, "GLO-30 mosaic raster too large");
+    sardine_dem_raster r = h0;
+    r.data = dst;
+    r.width = W;
+    r.height = H;
+    r.origin_lat_deg = lats[0];
+    r.origin_lon_deg = lons[0];
+    r.tiling = tiling;
+    if (!dst) {
+        *tiling = t;
+        *out = r;
+        return SARDINE_OK;
+    }
+    sardine_status st;
+    if ((st = require_device())) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // cells without a tile stay NaN (no tile answers there)
+    SD_HIP(hipMemsetAsync(dst, 0xff, (size_t)W * (size_t)H * sizeof(float), s));
+    DevBuf tmp;
+    SD_HIP(tmp.alloc((size_t)h0.width * (size_t)h0.height * sizeof(float), s));
+    for (int32_t i = 0; i < n; ++i) {
+        sardine_dem_raster ri;
+        if ((st = sardine_dem_glo30_tile_impl(paths[i], tmp.as<float>(), &ri, stream))) return st;
+        const int rr = cell[(size_t)i] / M, cc = cell[(size_t)i] % M;
+        SD_HIP(hipMemcpy2DAsync(dst + (size_t)rr * h0.height * W + (size_t)cc * h0.width, (size_t)W * sizeof(float),
+                                tmp.p, (size_t)h0.width * sizeof(float), (size_t)h0.width * sizeof(float),
+                                (size_t)h0.height, hipMemcpyDeviceToDevice, s));
+    }
+    *tiling = t;
+    *out = r;
+    return SARDINE_OK;
+}
+sardine_status sardine_dem_glo30_mosaic(const char* const* paths, int32_t n_paths, float* dst, sardine_dem_raster* out,
+                                        sardine_dem_tiling* tiling, void* stream) {
+    return guarded(SARDINE_ERR_DEM_TIFF_DECODE,
+                   [&] { return sardine_dem_glo30_mosaic_impl(paths, n_paths, dst, out, tiling, stream); });
 }
 
 }  // extern "C"
@@ -3190,6 +3419,7 @@ struct SpeckleBatchHeaderHost {  // layout of sd::SpeckleBatchHeader (speckle.hi
 struct sardine_resident_scene_s {
     sardine_radar_image merged;
     sardine_dem_raster dem;
+    sardine_dem_tiling tiling;  // dem.tiling points here for a mosaic
     sardine_scene scene;  // points into the vectors below
     std::vector<sardine_state_vector> svs;
     std::vector<sardine_grid_point> gps;
@@ -3257,6 +3487,10 @@ sardine_status sardine_scene_create(const sardine_radar_image* merged, const sar
     std::unique_ptr<sardine_resident_scene_s> r(new sardine_resident_scene_s());
     r->merged = *merged;
     r->dem = *dem;
+    if (dem->tiling) {
+        r->tiling = *dem->tiling;
+        r->dem.tiling = &r->tiling;
+    }
     r->scene = *scene;
     r->svs.assign(scene->orbit.state_vectors, scene->orbit.state_vectors + scene->orbit.count);
     r->gps.assign(scene->grid_points, scene->grid_points + scene->n_grid_points);
@@ -3266,7 +3500,8 @@ sardine_status sardine_scene_create(const sardine_radar_image* merged, const sar
     const sd::OrbitDev orb = make_orbit_dev(r->scene.orbit);
     const SeedLut lut = build_seed_lut(r->scene.grid_points, r->scene.n_grid_points);
     const size_t n = lut.lat.size();
-    const size_t bytes = round256(sizeof orb) + round256(n * sizeof(double)) + round256(n * sizeof(int64_t));
+    const size_t tab_off = round256(sizeof orb) + round256(n * sizeof(double)) + round256(n * sizeof(int64_t));
+    const size_t bytes = tab_off + (dem->tiling ? sd::kDemTabWords * sizeof(double) : 0);
     SD_HIP(hipMalloc(&r->d_mem, bytes));
     char* base = static_cast<char*>(r->d_mem);
     SD_HIP(hipMemcpy(base, &orb, sizeof orb, hipMemcpyHostToDevice));
@@ -3278,6 +3513,12 @@ sardine_status sardine_scene_create(const sardine_radar_image* merged, const sar
     r->dev.lut_t = reinterpret_cast<const int64_t*>(base + round256(sizeof orb) + round256(n * sizeof(double)));
     r->dev.n_lut = (int)n;
     r->dev.epoch = orb.n > 0 ? r->svs[0].time_us : 0;
+    if (dem->tiling) {
+        double tab[sd::kDemTabWords];
+        dem_tile_table(dem->tiling, tab);
+        SD_HIP(hipMemcpy(base + tab_off, tab, sizeof tab, hipMemcpyHostToDevice));
+        r->dev.dem_tab = reinterpret_cast<const double*>(base + tab_off);
+    }
     *out = r.release();
     return SARDINE_OK;
 }

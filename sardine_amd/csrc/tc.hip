@@ -80,6 +80,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     fixup_row<RS, UTM>(p, orb, rr, rr < p.rows);
 }
 
+// The terrain correction's DEM lookup at given points (diagnostics).
+__global__ __launch_bounds__(256) void dem_probe_kernel(TcParams p, const double* lat, const double* lon, float* out,
+                                                        int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[i] = p.dem_tiled ? dem_height_mosaic(mos_args(p), lat[i], lon[i]) : dem_sample(p, lat[i], lon[i]);
+}
+
 }  // namespace sd
 
 namespace {
@@ -99,6 +107,14 @@ void launch_chain(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_star
                        *p);
 }
 }  // namespace
+
+extern "C" hipError_t sd_launch_dem_probe(const sd::TcParams* p, const double* lat, const double* lon, float* out,
+                                          int64_t n, hipStream_t stream) {
+    if (n > 0)
+        hipLaunchKernelGGL(sd::dem_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p, lat, lon,
+                           out, n);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t stream) {
     if (q->rows > 0)

@@ -97,10 +97,24 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
         q.lon = lon0 + t * (lon1 - lon0);
         q.lat = lat0 + t * (lat1 - lat0);
     }
-    const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
-    const DemTaps ct = dem_load(p, rc, cc);
-    if (NB && p.need_geom) q.nb = nb_heights(p, q.lat, q.lon, rc, cc);
-    const float hf = dem_interp(rc, cc, ct);
+    float hf;
+    if (p.dem_tiled) {  // GLO-30 mosaic: one tile lookup per point (dem.rs:730-747)
+        const MosArgs m = mos_args(p);
+        hf = dem_height_mosaic(m, q.lat, q.lon);
+        if (NB && p.need_geom) {
+            const double st = p.step_deg;
+            q.nb.e = dem_height_mosaic(m, q.lat, q.lon + st);
+            q.nb.w = dem_height_mosaic(m, q.lat, q.lon - st);
+            q.nb.n = dem_height_mosaic(m, q.lat + st, q.lon);
+            q.nb.s = dem_height_mosaic(m, q.lat - st, q.lon);
+            q.nb.ok = isfinite(q.nb.e) && isfinite(q.nb.w) && isfinite(q.nb.n) && isfinite(q.nb.s);
+        }
+    } else {
+        const DemAxis rc = dem_row_axis(p, q.lat), cc = dem_col_axis(p, q.lon);
+        const DemTaps ct = dem_load(p, rc, cc);
+        if (NB && p.need_geom) q.nb = nb_heights(p, q.lat, q.lon, rc, cc);
+        hf = dem_interp(rc, cc, ct);
+    }
     q.dem_ok = isfinite(hf);
     q.target = {0.0, 0.0, 0.0};
     q.sl = q.cl = q.so = q.co = q.n = 0.0;

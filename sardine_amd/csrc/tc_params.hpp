@@ -52,6 +52,8 @@ struct alignas(16) OrbitF32 {
     float bary[kMaxSv][kLagrange];
 };
 
+constexpr int kDemTabWords = 24;
+
 struct TcParams {
     const OrbitDev* orbit;
     // FP32 scene-centred variant (precision SARDINE_PRECISION_FP32_CENTRED)
@@ -75,6 +77,13 @@ struct TcParams {
     double dem_max_row, dem_max_col;   // height-1, width-1
     double dem_hi_row, dem_hi_col;     // height-0.5, width-0.5 (the outer margin)
     int dem_lim_row, dem_lim_col;      // largest lower bilinear index
+    // GLO-30 mosaic (dem_tiled): a grid of tiles of dem_th x dem_tw pixels;
+    // dem_tab (device, kDemTabWords): [0, 8) the origin latitude of each
+    // tile row, [8, 16) the origin longitude of each tile column, [16, 24)
+    // per tile row the load order of its tiles as bytes (column c in byte c,
+    // 0xff: no tile), stored as the bits of a double
+    int dem_tiled, dem_trows, dem_tcols, dem_th, dem_tw;
+    const double* dem_tab;
     const float* geoid;     // device 73x144 or NULL
     // output grid
     int crs, utm_zone, utm_south, n_anchor_pts;

@@ -99,6 +99,103 @@ __device__ __forceinline__ NbHeights nb_heights(const TcParams& p, double lat, d
     return h;
 }
 
+// ── GLO-30 mosaic (p.dem_tiled): DemMosaic::elevation_at_linear ──────────
+// (dem.rs:730-747) over Glo30Tile::elevation_at (dem.rs:351-372): the first
+// tile in load order whose extent [-0.5, n - 0.5] (in its own pixel
+// coordinates, from its own origin) holds the point and whose four taps are
+// finite answers, clamped to its own edges.  Tiles meet on half-pixel
+// strips, so at most two tile rows and two tile columns can hold a point.
+// What the mosaic lookup reads (passed by value: the lookup is out of line).
+struct MosArgs {
+    const float* dem;
+    const double* tab;  // TcParams::dem_tab
+    int64_t dem_w;
+    double pix, ipx;
+    int trows, tcols, th, tw;
+};
+__device__ __forceinline__ MosArgs mos_args(const TcParams& p) {
+    return MosArgs{p.dem, p.dem_tab, p.dem_w, p.dem_pix, p.dem_inv_pix, p.dem_trows, p.dem_tcols, p.dem_th, p.dem_tw};
+}
+
+// The mosaic tables: origins and load orders.
+__device__ __forceinline__ double mos_olat(const MosArgs& m, int r) { return m.tab[r]; }
+__device__ __forceinline__ double mos_olon(const MosArgs& m, int c) { return m.tab[8 + c]; }
+__device__ __forceinline__ int mos_rank(const MosArgs& m, int r, int c) {
+    const unsigned long long w = (unsigned long long)__double_as_longlong(m.tab[16 + r]);
+    return (int)((w >> (8 * c)) & 0xffu);  // 0xff: no tile
+}
+
+// The tile rows (or columns) whose extent holds x: t0 (and t1 when two
+// meet there, else -1) with their pixel coordinates.  `f(t)` is the exact
+// per-tile coordinate; the first guess comes from tile 0's origin.
+template <typename F>
+__device__ __forceinline__ void mos_axis(int n, int tile, double guess, F f, int* t0, double* x0, int* t1,
+                                         double* x1) {
+    int t = (int)floor((guess + 0.5) / (double)tile);
+    t = t < 0 ? 0 : (t > n - 1 ? n - 1 : t);
+    double x = f(t);
+    if (x < -0.5 && t > 0) x = f(--t);
+    else if (x > (double)tile - 0.5 && t < n - 1) x = f(++t);
+    *t0 = t;
+    *x0 = x;
+    *t1 = -1;
+    *x1 = 0.0;
+    if (x < 0.0 && t > 0) {  // the half pixel before its first centre: the previous tile may reach here
+        const double y = f(t - 1);
+        if (y <= (double)tile - 0.5) *t1 = t - 1, *x1 = y;
+    } else if (x > (double)tile - 1.0 && t < n - 1) {
+        const double y = f(t + 1);
+        if (y >= -0.5) *t1 = t + 1, *x1 = y;
+    }
+}
+
+// Glo30Tile::elevation_at of tile (tr, tc) at its pixel coordinates.
+__device__ __forceinline__ float mos_tile_height(const MosArgs& m, int tr, double rf, int tc, double cf) {
+    const int th = m.th, tw = m.tw;
+    if (!(rf >= -0.5 && rf <= (double)th - 0.5 && cf >= -0.5 && cf <= (double)tw - 0.5)) return __builtin_nanf("");
+    rf = fmin(fmax(rf, 0.0), (double)(th - 1));
+    cf = fmin(fmax(cf, 0.0), (double)(tw - 1));
+    DemAxis r, c;
+    r.i0 = min((int)rf, th - 2);
+    c.i0 = min((int)cf, tw - 2);
+    r.f = rf - (double)r.i0;
+    c.f = cf - (double)c.i0;
+    const float* d = m.dem + (size_t)(unsigned)(r.i0 + tr * th) * (size_t)m.dem_w + (unsigned)(c.i0 + tc * tw);
+    const DemTaps t{d[0], d[1], d[m.dem_w], d[m.dem_w + 1]};
+    return dem_interp(r, c, t);  // NaN when a tap is NaN
+}
+
+// Out of line: the chain kernels run at their register limit and only
+// mosaic DEMs take this path.
+__device__ __noinline__ float dem_height_mosaic(MosArgs m, double lat, double lon) {
+    const double lon_n = lon >= 180.0 ? lon - 360.0 : lon;  // dem.rs:697
+    int r0, r1, c0, c1;
+    double y0, y1, x0, x1;
+    mos_axis(m.trows, m.th, (mos_olat(m, 0) - lat) * m.ipx,
+             [&](int t) { return div_rn(mos_olat(m, t) - lat, m.pix, m.ipx); }, &r0, &y0, &r1, &y1);
+    mos_axis(m.tcols, m.tw, (lon_n - mos_olon(m, 0)) * m.ipx,
+             [&](int t) { return div_rn(lon_n - mos_olon(m, t), m.pix, m.ipx); }, &c0, &x0, &c1, &x1);
+    // the (at most four) holding tiles in load order; the first with finite
+    // taps answers
+    int k0 = mos_rank(m, r0, c0), k1 = r1 >= 0 ? mos_rank(m, r1, c0) : 0xff;
+    int k2 = c1 >= 0 ? mos_rank(m, r0, c1) : 0xff, k3 = (r1 >= 0 && c1 >= 0) ? mos_rank(m, r1, c1) : 0xff;
+    float h = __builtin_nanf("");
+    for (int pass = 0; pass < 4; ++pass) {
+        const int m01 = k0 < k1 ? k0 : k1, m23 = k2 < k3 ? k2 : k3, mm = m01 < m23 ? m01 : m23;
+        if (mm == 0xff) break;
+        const int best = mm == k0 ? 0 : (mm == k1 ? 1 : (mm == k2 ? 2 : 3));
+        k0 = best == 0 ? 0xff : k0;
+        k1 = best == 1 ? 0xff : k1;
+        k2 = best == 2 ? 0xff : k2;
+        k3 = best == 3 ? 0xff : k3;
+        const int tr = (best & 1) ? r1 : r0, tc = (best & 2) ? c1 : c0;
+        const double rf = (best & 1) ? y1 : y0, cf = (best & 2) ? x1 : x0;
+        h = mos_tile_height(m, tr, rf, tc, cf);
+        if (!isnan(h)) break;
+    }
+    return h;
+}
+
 __device__ __forceinline__ double geoid_n(const TcParams& p, double lat, double lon) {
     // geoid.rs:220-251 (2.5° grid, lon wrap, lat clamp)
     if (!p.geoid) return 0.0;

@@ -170,6 +170,48 @@ def test_tc_dem_error_carries_the_dem_variant():
         assert e.dem_variant == abi.STATUS_NAMES[want]
 
 
+def test_tc_rejects_a_dem_tiling_the_lookup_cannot_serve():
+    """A caller-built tiling (sardine_dem_tiling) is checked on the host
+    before the device: grid shape against the raster, ranks inside the grid,
+    tile rows one tile apart north to south and columns west to east."""
+    sc = synthetic.scene_small_plain(0, lines=64, samples=64, out_rows=8, out_cols=8, dem_side=8)
+    keep = []
+    scene = sc.scene_c(keep)
+    img = abi.RadarImage(1, 0, 64, 64, int(sc.sv_t_us[0]) + 10_000_000, 0.0053, float(sc.rps_m))
+    cfg = sc.tc_config()
+    out = abi.GeocodedImage()
+    L = _native.lib()
+
+    def tiling(**kw):
+        t = abi.DemTiling()
+        t.rows, t.cols, t.tile_height, t.tile_width = 2, 2, 4, 4
+        t.origin_lat_deg[0], t.origin_lat_deg[1] = 50.0, 50.0 - 4e-3
+        t.origin_lon_deg[0], t.origin_lon_deg[1] = 10.0, 10.0 + 4e-3
+        for i in range(64):
+            t.rank[i] = -1
+        t.rank[0], t.rank[9] = 0, 1
+        for k, v in kw.items():
+            if k in ("olat1", "olon1"):
+                getattr(t, "origin_lat_deg" if k == "olat1" else "origin_lon_deg")[1] = v
+            elif k == "rank":
+                t.rank[v[0]] = v[1]
+            else:
+                setattr(t, k, v)
+        return t
+
+    for kw, ok in (({}, True), ({"olat1": 50.0 - 4.2e-3}, True), ({"olat1": 50.0 - 4.3e-3}, False),
+                   ({"olat1": 50.0 + 4e-3}, False), ({"olon1": 10.0 + 3.7e-3}, False), ({"olon1": float("nan")}, False),
+                   ({"rows": 3}, False), ({"tile_width": 5}, False), ({"rank": (2, 0)}, False),
+                   ({"rank": (0, 64)}, False)):
+        t = tiling(**kw)
+        dem = abi.DemRaster(1, 8, 8, 50.0, 10.0, 1e-3)
+        dem.tiling = C.pointer(t)
+        st = L.sardine_terrain_correction(C.byref(img), C.byref(scene), C.byref(dem), C.byref(cfg), C.byref(out), None)
+        # a good tiling passes and the call stops at a later check (no output buffers)
+        assert (st == abi.ERR_INVALID_ARGUMENT and "DEM tiling" in _native.last_error()) != ok, \
+            (kw, _native.last_error())
+
+
 def _np_ptr(x):
     return x.ctypes.data_as(C.c_void_p)
 
@@ -688,6 +730,68 @@ def test_glo30_tile_georeferencing_and_errors(tmp_path):
     (tmp_path / "bad.tif").write_bytes(bytes(b))
     assert L.sardine_dem_glo30_tile(str(tmp_path / "bad.tif").encode(), buf.ctypes.data, C.byref(r), None) == \
         abi.ERR_DEM_TIFF_DECODE
+
+
+def _glo30_tiles(tmp_path, cells, px=0.25, n=4, shape=None):
+    """GLO-30 tiles of n x n pixels of px degrees with pixel-corner tiepoints
+    at (lat_n, lon_w); cells: {name: (lat_n, lon_w)}."""
+    import slc_fixtures as F
+    paths = {}
+    for k, (name, (la, lo)) in enumerate(cells.items()):
+        a = np.full(shape or (n, n), 10.0 * (k + 1), np.float32)
+        p = tmp_path / name
+        p.write_bytes(F.geotiff_dem_bytes(a, scale=(px, px, 0), tiepoint=(0, 0, 0, lo, la, 0)))
+        paths[name] = str(p)
+    return paths
+
+
+def test_glo30_mosaic_layout_and_errors(tmp_path):
+    """DemMosaic::load_glo30_files as one raster (host side, headers only):
+    tile rows north first and columns west first at each tile's own origin,
+    ranks = load order, empty cells; and the grid errors."""
+    L = _native.lib()
+    paths = _glo30_tiles(tmp_path, {"a.tif": (47.0, 8.0), "b.tif": (47.0, 7.0), "c.tif": (46.0, 8.0)})
+
+    def load(names):
+        arr = (C.c_char_p * len(names))(*[paths[x].encode() for x in names]) if names else None
+        r, t = abi.DemRaster(), abi.DemTiling()
+        st = L.sardine_dem_glo30_mosaic(arr, len(names), None, C.byref(r), C.byref(t), None)
+        return st, r, t
+
+    st, r, t = load(["a.tif", "b.tif", "c.tif"])
+    assert st == 0
+    assert (t.rows, t.cols, t.tile_height, t.tile_width) == (2, 2, 4, 4) and (r.height, r.width) == (8, 8)
+    assert list(t.origin_lat_deg[:2]) == [47.0 - 0.125, 46.0 - 0.125]
+    assert list(t.origin_lon_deg[:2]) == [7.0 + 0.125, 8.0 + 0.125]
+    assert (r.origin_lat_deg, r.origin_lon_deg, r.pixel_deg) == (47.0 - 0.125, 7.0 + 0.125, 0.25)
+    assert [t.rank[0], t.rank[1], t.rank[8], t.rank[9]] == [1, 0, -1, 2]  # b at NW, a at NE, none at SW, c at SE
+    assert all(t.rank[i] == -1 for i in range(64) if i not in (0, 1, 8, 9))
+    assert load([])[0] == abi.ERR_DEM_NO_TILES
+    # tiles of another size, a gap between columns, two tiles in one cell, a missing file first
+    paths.update(_glo30_tiles(tmp_path, {"big.tif": (46.0, 7.0)}, shape=(5, 4)))
+    paths.update(_glo30_tiles(tmp_path, {"gap.tif": (47.0, 10.0)}))
+    paths.update(_glo30_tiles(tmp_path, {"dup.tif": (47.0, 8.0)}))
+    assert load(["a.tif", "big.tif"])[0] == abi.ERR_INVALID_ARGUMENT
+    assert load(["a.tif", "gap.tif"])[0] == abi.ERR_INVALID_ARGUMENT
+    assert load(["a.tif", "dup.tif"])[0] == abi.ERR_INVALID_ARGUMENT
+    paths["nofile.tif"] = str(tmp_path / "nofile.tif")
+    assert load(["nofile.tif", "big.tif"])[0] == abi.ERR_DEM_IO
+    # the tiles load whole in path order: a bad block in the first tile is
+    # reported before the second tile's missing file, and after it when second
+    import slc_fixtures as F
+    b = bytearray(F.geotiff_dem_bytes(np.zeros((64, 64), np.float32), scale=(1 / 64, 1 / 64, 0),
+                                      tiepoint=(0, 0, 0, 5.0, 47.0, 0), tile=(16, 16), compression="deflate"))
+    _patch_tile(b, 3, corrupt=True)
+    paths["bad.tif"] = str(tmp_path / "bad.tif")
+    (tmp_path / "bad.tif").write_bytes(bytes(b))
+    assert load(["bad.tif"])[0] == 0  # the header is good: the layout query does not decode
+    assert load(["bad.tif", "nofile.tif"])[0] == abi.ERR_DEM_TIFF_DECODE
+    assert "corrupt compressed block" in _native.last_error()
+    assert load(["nofile.tif", "bad.tif"])[0] == abi.ERR_DEM_IO
+    # nine tile columns: more than the grid holds
+    nine = _glo30_tiles(tmp_path, {f"w{i}.tif": (47.0, 20.0 + i) for i in range(9)})
+    paths.update(nine)
+    assert load(list(nine))[0] == abi.ERR_INVALID_ARGUMENT
 
 
 def _patch_tile(buf: bytearray, k: int, outside: bool = False, corrupt: bool = False) -> None:

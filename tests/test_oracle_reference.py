@@ -1440,3 +1440,78 @@ def test_portable_libm_geodesy_against_high_precision():
     for c in GOLD["geodetic_to_ecef"]:
         e = O.geodetic_to_ecef(c["lat"], c["lon"], c["h"])
         assert max(abs(e[k] - float(c["xyz"][k])) for k in range(3)) < 2e-9
+
+
+# ── GLO-30 mosaic lookup (DemMosaic::elevation_at_linear over Glo30Tile) ──
+# Four 4 x 4-pixel tiles of 1° (0.25° pixels, pixel-corner tiepoints at
+# integer degrees), each holding a different linear field h = A + B·i + C·j
+# of its own pixel row i and column j.  Bilinear interpolation of a linear
+# field is the field itself, so every answer has a closed form: the
+# answering tile's field at its clamped pixel coordinates.
+_MT_PIX = 0.25
+_MT_COEF = {(0, 0): (100.0, 3.0, 5.0), (0, 1): (200.0, -2.0, 7.0), (1, 0): (300.0, 11.0, -1.0),
+            (1, 1): (400.0, 13.0, 17.0)}
+
+
+def _mosaic(ranks):
+    t = abi.DemTiling()
+    t.rows, t.cols, t.tile_height, t.tile_width = 2, 2, 4, 4
+    t.origin_lat_deg[0], t.origin_lat_deg[1] = 47.0 - 0.5 * _MT_PIX, 46.0 - 0.5 * _MT_PIX
+    t.origin_lon_deg[0], t.origin_lon_deg[1] = 7.0 + 0.5 * _MT_PIX, 8.0 + 0.5 * _MT_PIX
+    for i in range(64):
+        t.rank[i] = -1
+    dem = np.full((8, 8), np.nan, np.float32)
+    i, j = np.mgrid[0:4, 0:4]
+    for (r, c), k in ranks.items():
+        t.rank[r * 8 + c] = k
+        a, b, cc = _MT_COEF[(r, c)]
+        dem[4 * r:4 * r + 4, 4 * c:4 * c + 4] = a + b * i + cc * j
+    return dem, t
+
+
+def _field(tile, lat, lon):
+    r, c = tile
+    rf = min(max((47.0 - 0.5 * _MT_PIX - r - lat) / _MT_PIX, 0.0), 3.0)
+    cf = min(max((lon - (7.0 + 0.5 * _MT_PIX + c)) / _MT_PIX, 0.0), 3.0)
+    a, b, cc = _MT_COEF[tile]
+    return a + b * rf + cc * cf
+
+
+def test_glo30_mosaic_lookup_known_answers():
+    dem, t = _mosaic({(0, 0): 0, (0, 1): 1, (1, 0): 2, (1, 1): 3})
+    h = lambda la, lo: O.dem_mosaic_elevation(dem, _MT_PIX, t, la, lo)
+    # inside each tile: its own field
+    for (r, c) in _MT_COEF:
+        for la, lo in ((46.6 - r, 7.3 + c), (46.9 - r, 7.9 + c), (46.2 - r, 7.05 + c)):
+            assert h(la, lo) == pytest.approx(_field((r, c), la, lo), abs=1e-4)
+    # the half pixel south of tile row 0's last row centre (46.125 .. 46.0)
+    # belongs to it and is clamped to that row: a constant in latitude
+    assert h(46.06, 7.5) == pytest.approx(_field((0, 0), 46.125, 7.5), abs=1e-4)
+    assert h(46.01, 7.5) == pytest.approx(_field((0, 0), 46.125, 7.5), abs=1e-4)
+    # the half pixel north of tile row 1's first centre (46.0 .. 45.875):
+    # tile (1, 0) clamped to its first row
+    assert h(45.99, 7.5) == pytest.approx(_field((1, 0), 45.875, 7.5), abs=1e-4)
+    # on the shared edge both hold the point: the first in load order answers
+    assert h(46.0, 7.5) == pytest.approx(_field((0, 0), 46.0, 7.5), abs=1e-4)
+    dem2, t2 = _mosaic({(0, 0): 2, (0, 1): 1, (1, 0): 0, (1, 1): 3})
+    assert O.dem_mosaic_elevation(dem2, _MT_PIX, t2, 46.0, 7.5) == pytest.approx(_field((1, 0), 46.0, 7.5), abs=1e-4)
+    # the shared corner of all four: the lowest rank of the four
+    dem3, t3 = _mosaic({(0, 0): 3, (0, 1): 2, (1, 0): 1, (1, 1): 0})
+    assert O.dem_mosaic_elevation(dem3, _MT_PIX, t3, 46.0, 8.0) == pytest.approx(_field((1, 1), 46.0, 8.0), abs=1e-4)
+    # a NaN tap in the first tile: the next tile holding the point answers
+    dem[3, 1] = np.nan
+    assert h(46.0, 7.3) == pytest.approx(_field((1, 0), 46.0, 7.3), abs=1e-4)
+    assert h(46.06, 7.3) is None  # only tile (0, 0) holds it
+    # no tile: no height, except on the edge a neighbour shares
+    dem4, t4 = _mosaic({(0, 0): 0, (0, 1): 1, (1, 0): 2})
+    assert O.dem_mosaic_elevation(dem4, _MT_PIX, t4, 45.5, 8.5) is None
+    assert O.dem_mosaic_elevation(dem4, _MT_PIX, t4, 46.0, 8.5) == pytest.approx(_field((0, 1), 46.0, 8.5), abs=1e-4)
+    assert O.dem_mosaic_elevation(dem4, _MT_PIX, t4, 45.5, 8.0) == pytest.approx(_field((1, 0), 45.5, 8.0), abs=1e-4)
+    # outside the mosaic
+    assert h(47.2, 7.5) is None and h(46.5, 9.1) is None
+    # longitude normalisation (dem.rs:697): 187.5° becomes -172.5°, outside
+    assert h(46.5, 187.5) is None
+    # ... and inside the same mosaic moved 180° west
+    for k in range(2):
+        t.origin_lon_deg[k] -= 180.0
+    assert h(46.5, 187.5) == h(46.5, -172.5) == pytest.approx(_field((0, 0), 46.5, 7.5), abs=1e-4)
