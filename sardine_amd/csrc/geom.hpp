@@ -313,7 +313,20 @@ SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter,
         double vsq = dot(V, V);
         double fp = -vsq + dot(r, A);
         if (fabs(fp) < 1.0) return ZD_DEGENERATE;
+#ifdef __HIP_DEVICE_COMPILE__
+        // -f / fp from the hardware reciprocal and two Newton-Raphson steps:
+        // within 2 ulp of the quotient (|fp| >= 1 here), far inside the tie
+        // band the decisions below are checked against
+        double delta;
+        {
+            double r = __builtin_amdgcn_rcp(fp);
+            r = fma(fma(-fp, r, 1.0), r, r);
+            r = fma(fma(-fp, r, 1.0), r, r);
+            delta = -f * r;
+        }
+#else
         double delta = -f / fp;
+#endif
         const double x = delta * 1e6;
         if (fabs(x - rint(x)) < band || fabs(fabs(delta) - tol) < band * 1e-6) {
             *tie = true;

@@ -176,12 +176,30 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
     const Bilin nb_idx = bilin_index(p.lines, p.samples, line, sample);
     float sigma0;
     DemTaps nzt = {0.0f, 0.0f, 0.0f, 0.0f};
+    // The terrain geometry does not depend on the image: it is computed
+    // while the bilinear taps are in flight.
+    DemTaps it = {0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (RS == 0 || RS == 3) {
-        const DemTaps it = bilin_load(p.img, p.samples, nb_idx);
-        if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
+        it = bilin_load(p.img, p.samples, nb_idx);
+    }
+    if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
+    double w = 0.0, cl = 0.0;
+    int g = TG_SHADOW;
+    if (p.need_geom && q.nb.ok) {
+        if constexpr (MIX) {
+            float wf = 0.0f, clf = 0.0f;
+            g = terrain_geometry_mixed(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w,
+                                       q.nb.n, q.nb.s, sat, q.target, inv_flat_area, &wf, &clf);
+            w = wf;
+            cl = clf;
+        } else {
+            g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w, q.nb.n,
+                                    q.nb.s, sat, q.target, inv_flat_area, &w, &cl);
+        }
+    }
+    if constexpr (RS == 0 || RS == 3) {
         sigma0 = MIX ? bilin_interp_mx(nb_idx, it) : bilin_interp(nb_idx, it);
     } else {
-        if (p.nesz) nzt = bilin_load(p.nesz, p.samples, nb_idx);
         sigma0 = resample<RS>(p, line, sample);
     }
     if (isnan(sigma0)) {
@@ -196,18 +214,6 @@ __device__ __forceinline__ PixelOut finish_pixel(const TcParams& p, const PixelP
                 return o;
             }
         } else {
-            double w = 0.0, cl = 0.0;
-            int g;
-            if constexpr (MIX) {
-                float wf = 0.0f, clf = 0.0f;
-                g = terrain_geometry_mixed(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w,
-                                           q.nb.n, q.nb.s, sat, q.target, inv_flat_area, &wf, &clf);
-                w = wf;
-                cl = clf;
-            } else {
-                g = terrain_geometry_sc(q.sl, q.cl, q.so, q.co, q.n, p.step_sin, p.step_cos, q.nb.e, q.nb.w, q.nb.n,
-                                        q.nb.s, sat, q.target, inv_flat_area, &w, &cl);
-            }
             if (g == TG_VALID) {
                 o.cos_lia = (float)cl;
                 if (p.flatten) sigma0 /= (float)w;
