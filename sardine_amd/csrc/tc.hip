@@ -42,6 +42,16 @@ __global__ __launch_bounds__(256) void utm_anchor_kernel(TcParams p) {
     if (i < p.rows * p.anchor_stride) anchor_point(p, i);
 }
 
+// One lane per AOI column (the column's project_row blend).
+__global__ __launch_bounds__(256) void utm_col_kernel(TcParams p) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.cols) return;
+    ColBlend b;
+    col_blend(p, p.col0 + i, &b.a, &b.t);
+    b.pad = 0;
+    p.col_blend[i] = b;
+}
+
 template <int RS, bool UTM>
 __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(TcParams p) {
     __shared__ OrbitDev orb;
@@ -128,6 +138,8 @@ extern "C" hipError_t sd_launch_tc_anchors(const sd::TcParams* p, hipStream_t st
     if (p->crs == 0) return hipSuccess;
     const int64_t n = p->rows * p->anchor_stride;
     hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
+    if (p->col_blend)
+        hipLaunchKernelGGL(sd::utm_col_kernel, dim3((unsigned)((p->cols + 255) / 256)), dim3(256), 0, stream, *p);
     return hipGetLastError();
 }
 

@@ -39,7 +39,16 @@ __global__ __launch_bounds__(256) void batch_anchor_kernel(const BatchHeader* __
     const int64_t n = H->n_anchor;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int j = find_job(n_jobs, i, [&](int k) { return J[k].anchor_begin; });
-        anchor_point(J[j].p, i - J[j].anchor_begin);
+        const TcParams& p = J[j].p;
+        const int64_t k = i - J[j].anchor_begin, na = p.rows * p.anchor_stride;
+        if (k < na) {
+            anchor_point(p, k);
+        } else {  // then the job's column blends
+            ColBlend b;
+            col_blend(p, p.col0 + (k - na), &b.a, &b.t);
+            b.pad = 0;
+            p.col_blend[k - na] = b;
+        }
     }
 }
 

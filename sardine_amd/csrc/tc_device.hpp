@@ -60,6 +60,21 @@ struct AnchorCache {
 };
 __device__ __forceinline__ AnchorCache anchor_cache_empty() { return AnchorCache{-1, 0.0, 0.0, 0.0, 0.0}; }
 
+// The project_row blend of grid column `col` (output_crs.rs:395-414): the
+// anchor interval holding x (relative to the first stored anchor) and the
+// weight of its right anchor.
+__device__ __forceinline__ void col_blend(const TcParams& p, int64_t col, int* a_out, double* t_out) {
+    const double x = p.x_start + (double)col * p.spacing;
+    const int na = p.n_anchor_pts - 1;
+    int seg = (int)floor((x - p.x_start) * p.inv_anchor_dx);  // a guess: the walk below decides
+    seg = seg < 0 ? 0 : (seg > na - 1 ? na - 1 : seg);
+    while (seg > 0 && !(x > p.x_start + (double)seg * p.anchor_dx)) --seg;
+    while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
+    const double x0 = p.x_start + (double)seg * p.anchor_dx, x1 = p.x_start + (double)(seg + 1) * p.anchor_dx;
+    *a_out = seg - p.anchor0;  // the row stores anchors anchor0.. only
+    *t_out = (x1 == x0) ? 0.0 : (x - x0) / (x1 - x0);
+}
+
 // Output-grid position of a column and its DEM/ECEF target.  The centre's
 // and (for the terrain geometry) the neighbours' DEM taps are issued
 // together: one memory round trip instead of two; only the four neighbour
@@ -72,14 +87,15 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
         q.lon = p.lon_west + ((double)col + 0.5) * p.spacing;
         q.lat = row_y;
     } else {  // Projector::project_row blend (output_crs.rs:395-414)
-        const double x = p.x_start + (double)col * p.spacing;
-        const int na = p.n_anchor_pts - 1;
-        int seg = (int)floor((x - p.x_start) * p.inv_anchor_dx);  // a guess: the walk below decides
-        seg = seg < 0 ? 0 : (seg > na - 1 ? na - 1 : seg);
-        while (seg > 0 && !(x > p.x_start + (double)seg * p.anchor_dx)) --seg;
-        while (seg + 1 < na && x > p.x_start + (double)(seg + 1) * p.anchor_dx) ++seg;
-        const double x0 = p.x_start + (double)seg * p.anchor_dx, x1 = p.x_start + (double)(seg + 1) * p.anchor_dx;
-        const int a = seg - p.anchor0;  // the row stores anchors anchor0.. only
+        int a;
+        double t;
+        if (p.col_blend) {
+            const ColBlend cb = p.col_blend[col - p.col0];
+            a = cb.a;
+            t = cb.t;
+        } else {
+            col_blend(p, col, &a, &t);
+        }
         double lon0, lon1, lat0, lat1;
         if (ac && ac->a == a) {
             lon0 = ac->lon0;
@@ -93,7 +109,6 @@ __device__ __forceinline__ PixelPos prep_pixel(const TcParams& p, int64_t col, d
             lat1 = alat[a + 1];
             if (ac) *ac = AnchorCache{a, lon0, lon1, lat0, lat1};
         }
-        const double t = (x1 == x0) ? 0.0 : (x - x0) / (x1 - x0);
         q.lon = lon0 + t * (lon1 - lon0);
         q.lat = lat0 + t * (lat1 - lat0);
     }

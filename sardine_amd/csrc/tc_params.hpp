@@ -54,6 +54,16 @@ struct alignas(16) OrbitF32 {
 
 constexpr int kDemTabWords = 24;
 
+// Projector::project_row's blend of one output column (output_crs.rs:395-414):
+// the anchor interval (relative to the first stored anchor) and the weight
+// t = (x - x0) / (x1 - x0).  Both depend on the column only, so they are
+// computed once per AOI column (utm_col_kernel) instead of per pixel.
+struct ColBlend {
+    double t;
+    int32_t a;
+    int32_t pad;
+};
+
 struct TcParams {
     const OrbitDev* orbit;
     // FP32 scene-centred variant (precision SARDINE_PRECISION_FP32_CENTRED)
@@ -94,6 +104,7 @@ struct TcParams {
     int seg_len, nseg, rows_per_wg;  // work decomposition (see tc.hip)
     int lead;                        // phase-A lead columns
     double* anchors;                  // UTM: per AOI row lon[stride], lat[stride] of anchors anchor0..
+    ColBlend* col_blend;              // UTM: per AOI column, or NULL (then computed per pixel)
     int anchor0, anchor_stride;       // first stored anchor index, stored anchors per row
     SegRecord* records;               // per (AOI row, segment)
     int32_t* row_flags;               // per AOI row: 1 = the fix-up pass walks the row, 2 = tie list only
