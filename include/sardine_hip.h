@@ -387,7 +387,8 @@ sardine_status sardine_terrain_correction(const sardine_radar_image* merged,
  * that each GPU holds until the process ends (the caller still passes the
  * inputs and outputs).  A call whose work needs more scratch (chain records,
  * row seeds, the tie list and anchors grow with the AOI: about 0.75-3 bytes
- * per pixel plus 0.1 KiB per row and 4 KiB per UTM row) returns
+ * per pixel plus 0.1 KiB per row, 4 KiB per UTM row and 16 bytes per UTM
+ * column) returns
  * SARDINE_ERR_CAPACITY.  All calls on one GPU share its
  * array: calls that overlap on one GPU (from several threads or streams)
  * race; calls in sequence do not, since each returns after its work. */
