@@ -900,7 +900,7 @@ TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
     w.tie_cap = w.records ? std::min<int64_t>(pl.nr * (int64_t)pl.p.nseg, std::max<int64_t>(64, pl.nr * 8)) : 0;
     w.ties = round256((size_t)w.tie_cap * sizeof(int2));
     w.anchors = pl.utm ? round256((size_t)pl.nr * 2 * pl.p.anchor_stride * sizeof(double)) : 0;
-    w.blend = pl.utm ? round256((size_t)pl.nc * sizeof(sd::ColBlend)) : 0;
+    w.blend = round256((size_t)pl.nc * (pl.utm ? sizeof(sd::ColBlend) : sizeof(sd::ColGeo)));
     w.seed = round256((size_t)pl.nr * sizeof(int64_t));
     w.flat = pl.need_geom ? round256((size_t)pl.nr * sizeof(double)) : 0;
     w.geoid = cfg->geoid.kind == SARDINE_GEOID_GRID_2P5 ? round256(73 * 144 * sizeof(float)) : 0;
@@ -917,7 +917,8 @@ void assign_work(TcPlan& pl, const TcWork& w, char* base) {
     base += w.ties;
     pl.p.anchors = w.anchors ? reinterpret_cast<double*>(base) : nullptr;
     base += w.anchors;
-    pl.p.col_blend = w.blend ? reinterpret_cast<sd::ColBlend*>(base) : nullptr;
+    pl.p.col_blend = pl.utm ? reinterpret_cast<sd::ColBlend*>(base) : nullptr;
+    pl.p.col_geo = pl.utm ? nullptr : reinterpret_cast<sd::ColGeo*>(base);
     base += w.blend;
     pl.p.row_seed = reinterpret_cast<const int64_t*>(base);
     pl.q.row_seed = reinterpret_cast<int64_t*>(base);
@@ -3706,7 +3707,8 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
         J.row_begin = H.n_rows;
         H.n_rows += pl.nr;
         J.anchor_begin = H.n_anchor;
-        if (pl.utm) H.n_anchor += pl.nr * pl.p.anchor_stride + pl.nc;  // anchors, then column blends
+        // UTM: anchors, then column blends; EPSG:4326: the column terms
+        H.n_anchor += pl.utm ? pl.nr * pl.p.anchor_stride + pl.nc : pl.nc;
         const sardine_batch_job& jb = jobs[i];
         int64_t tiles_x = 1, tiles = 0;
         if (jb.speckle.kind >= 0) {

@@ -52,6 +52,12 @@ __global__ __launch_bounds__(256) void utm_col_kernel(TcParams p) {
     p.col_blend[i] = b;
 }
 
+// One lane per AOI column of an EPSG:4326 grid (the column's longitude terms).
+__global__ __launch_bounds__(256) void wgs_col_kernel(TcParams p) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < p.cols) p.col_geo[i] = col_geo(p, p.col0 + i);
+}
+
 template <int RS, bool UTM>
 __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(TcParams p) {
     __shared__ OrbitDev orb;
@@ -135,7 +141,11 @@ extern "C" hipError_t sd_launch_tc_rows(const sd::RowPrepParams* q, hipStream_t 
 // seg_start / seg_end: optional events recorded around the segment kernel.
 // project_row anchors of the AOI rows (UTM grids only).
 extern "C" hipError_t sd_launch_tc_anchors(const sd::TcParams* p, hipStream_t stream) {
-    if (p->crs == 0) return hipSuccess;
+    if (p->crs == 0) {
+        if (p->col_geo)
+            hipLaunchKernelGGL(sd::wgs_col_kernel, dim3((unsigned)((p->cols + 255) / 256)), dim3(256), 0, stream, *p);
+        return hipGetLastError();
+    }
     const int64_t n = p->rows * p->anchor_stride;
     hipLaunchKernelGGL(sd::utm_anchor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p);
     if (p->col_blend)

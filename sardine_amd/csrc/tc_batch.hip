@@ -40,7 +40,12 @@ __global__ __launch_bounds__(256) void batch_anchor_kernel(const BatchHeader* __
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int j = find_job(n_jobs, i, [&](int k) { return J[k].anchor_begin; });
         const TcParams& p = J[j].p;
-        const int64_t k = i - J[j].anchor_begin, na = p.rows * p.anchor_stride;
+        const int64_t k = i - J[j].anchor_begin;
+        if (p.crs == 0) {  // EPSG:4326: the column terms only
+            p.col_geo[k] = col_geo(p, p.col0 + k);
+            continue;
+        }
+        const int64_t na = p.rows * p.anchor_stride;
         if (k < na) {
             anchor_point(p, k);
         } else {  // then the job's column blends
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(256) void batch_anchor_kernel(const BatchHeader* __
 template <int RS, bool UTM>
 __device__ __forceinline__ void batch_item(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                            int64_t seg) {
-    segment_wave<RS, UTM, false>(p, orb, st, rr0, seg);
+    segment_wave<RS, UTM, false, false>(p, orb, st, rr0, seg);
 }
 
 // Persistent chain pass: `queue` is zero when the launch starts.

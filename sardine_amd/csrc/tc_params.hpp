@@ -64,6 +64,18 @@ struct ColBlend {
     int32_t pad;
 };
 
+// The longitude-only terms of one output column of an EPSG:4326 grid (the
+// longitude is the same on every row): the sine and cosine of the target's
+// longitude and the DEM column axes of the centre and of the east / west
+// terrain-geometry neighbours (index and fraction as dem_col_axis returns
+// them).  Computed once per AOI column (wgs_col_kernel).
+struct ColGeo {
+    double so, co;
+    double fc, fe, fw;
+    int32_t ic, ie, iw;
+    int32_t pad;
+};
+
 struct TcParams {
     const OrbitDev* orbit;
     // FP32 scene-centred variant (precision SARDINE_PRECISION_FP32_CENTRED)
@@ -105,6 +117,7 @@ struct TcParams {
     int lead;                        // phase-A lead columns
     double* anchors;                  // UTM: per AOI row lon[stride], lat[stride] of anchors anchor0..
     ColBlend* col_blend;              // UTM: per AOI column, or NULL (then computed per pixel)
+    ColGeo* col_geo;                  // EPSG:4326: per AOI column, or NULL (then computed per pixel)
     int anchor0, anchor_stride;       // first stored anchor index, stored anchors per row
     SegRecord* records;               // per (AOI row, segment)
     int32_t* row_flags;               // per AOI row: 1 = the fix-up pass walks the row, 2 = tie list only
