@@ -156,6 +156,9 @@ extern "C" hipError_t sd_launch_tc_anchors(const sd::TcParams* p, hipStream_t st
 extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start,
                                         hipEvent_t seg_end) {
     const bool utm = p->crs != 0;
+    // EPSG:4326 chains read their grid's column table (UTM ones fall back to
+    // the per-pixel blend without one)
+    if (!utm && p->col_geo == nullptr) return hipErrorInvalidValue;
     hipError_t e = sd_launch_tc_anchors(p, stream);
     if (e != hipSuccess) return e;
     // RS 0..2: the resampler in FP64; 3..5: the same in the mixed form
