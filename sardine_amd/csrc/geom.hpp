@@ -290,24 +290,22 @@ SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter,
 #pragma clang fp contract(fast)
     int64_t t = seed;
     for (int i = 0; i < max_iter; ++i) {
-        d3 P, V, A;
-        if (cache->valid && cache->t == t) {
-            P = cache->P;
-            V = cache->V;
-            A = cache->A;
-        } else {
-            int e = orbit_interp(o, t, &P, &V, &A);
-            if (e == ORB_OUTSIDE) {
-                *t_out = t;  // the requested time, for the error report
-                return ZD_ORBIT_OUTSIDE;
+        // the evaluation writes the cache itself: one copy of (P, V, A)
+        // lives across the iterations, not a cache and a working set
+        if (!(cache->valid && cache->t == t)) {
+            int e = orbit_interp(o, t, &cache->P, &cache->V, &cache->A);
+            if (e != ORB_OK) {
+                cache->valid = false;
+                if (e == ORB_OUTSIDE) {
+                    *t_out = t;  // the requested time, for the error report
+                    return ZD_ORBIT_OUTSIDE;
+                }
+                return ZD_ORBIT_INSUFFICIENT;
             }
-            if (e == ORB_INSUFFICIENT) return ZD_ORBIT_INSUFFICIENT;
             cache->t = t;
-            cache->P = P;
-            cache->V = V;
-            cache->A = A;
             cache->valid = true;
         }
+        const d3 P = cache->P, V = cache->V, A = cache->A;
         d3 r = target - P;
         double f = dot(r, V);
         double vsq = dot(V, V);
