@@ -352,7 +352,17 @@ SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter,
             *sat = P;
             return ZD_CONVERGED;
         }
+#ifdef __HIP_DEVICE_COMPILE__
+        // the µs step truncated toward zero as the 64-bit conversion does;
+        // steps below 2^31 µs (all but divergent ones) by the 32-bit one
+        const double xs = delta * 1e6;
+        int64_t step;
+        if (__builtin_expect(fabs(xs) < 0x1p31, 1)) step = (int64_t)(int32_t)xs;
+        else step = (int64_t)xs;
+        t = t + step;
+#else
         t = t + (int64_t)(delta * 1e6);
+#endif
     }
     return ZD_NOT_CONVERGED;
 }

@@ -395,10 +395,14 @@ __device__ __forceinline__ Solve solve(const TcParams& p, const OrbitDev& orb, c
     Solve s;
     s.t = 0;
     s.it = 0;
-    s.sat = {0.0, 0.0, 0.0};
     s.tie = false;
-    s.oc = zero_doppler<EXACT>(orb, q.target, seed, max_iter, p.tol, p.tie_band, &s.t, &s.it, &s.sat, &cache, &s.tie,
-                               EXACT ? p.counters + kCntTie : nullptr);
+    d3 sat_unused;
+    s.oc = zero_doppler<EXACT>(orb, q.target, seed, max_iter, p.tol, p.tie_band, &s.t, &s.it, &sat_unused, &cache,
+                               &s.tie, EXACT ? p.counters + kCntTie : nullptr);
+    // a converged solve's last evaluation is at its time: the satellite
+    // position is the evaluation cache's (the registers it already holds,
+    // not a copy per exit of the Newton loop); unused otherwise
+    s.sat = cache.P;
     return s;
 }
 
