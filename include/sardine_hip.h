@@ -816,8 +816,9 @@ typedef enum sardine_kernel_family {
  * events) or off. */
 void sardine_set_profiling(int on);
 /* Number of output columns one lane chains sequentially in the
- * terrain-correction kernel (0 = automatic, the default).  A tuning knob
- * with no effect on results: the seed-chain fix-up makes every value exact. */
+ * terrain-correction kernel (0 = automatic, the default; at most 255).  A
+ * tuning knob with no effect on results: the seed-chain fix-up makes every
+ * value exact. */
 void sardine_set_tc_segment(int64_t columns);
 /* Columns before each segment over which the terrain-correction kernel runs
  * a short chain to predict the segment's starting seed (default 2; 0 starts

@@ -875,7 +875,8 @@ void set_decomposition(sd::TcParams& p, int64_t requested) {
         // L = 32 25.5 ms, 64 25.1 ms, 128 25.1 ms)
         L = std::min<int64_t>(64, std::max<int64_t>(16, (p.rows * p.cols + target_lanes - 1) / target_lanes));
     }
-    L = std::min<int64_t>(L, p.cols);
+    // at most 255: the chain pass counts a segment's pixels in bytes
+    L = std::min<int64_t>(std::min<int64_t>(L, 255), p.cols);
     p.seg_len = (int)L;
     p.nseg = (int)((p.cols + L - 1) / L);
     p.rows_per_wg = 64;
@@ -1007,7 +1008,7 @@ void sardine_set_profiling(int on) {
             if (!e.b) (void)hipEventCreate(&e.b);
         }
 }
-void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : columns; }
+void sardine_set_tc_segment(int64_t columns) { g_tc_seg = columns < 0 ? 0 : (columns > 255 ? 255 : columns); }
 void sardine_set_tc_lead(int32_t columns) { g_tc_lead = columns < 0 ? 0 : (columns > 64 ? 64 : columns); }
 void sardine_set_tc_tie_band(double us) { g_tc_tie_band = us > 0.0 && us <= 0.5 ? us : sd::kTieBandUs; }
 double sardine_last_kernel_ms(int family) {

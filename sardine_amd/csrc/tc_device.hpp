@@ -438,6 +438,49 @@ __device__ __forceinline__ void flush_counts(const TcParams& p, const LaneCounts
     if (set_errors && C.err_short > 0) atomicOr(p.error_flag, 2);
 }
 
+// The chain pass's per-lane counts, one byte each (a lane counts at most
+// the pixels of one segment, <= 64): bytes 0..6 the categories CAT_VALID ..
+// CAT_NOISE, 7..9 histogram bins 1..3, 10 and 11 the orbit errors
+// (outside the window, too few vectors).  Three registers instead of twelve
+// live across the walk.
+struct PackedCounts {
+    uint32_t w[3];
+};
+constexpr int kPcErrOutside = 10, kPcErrShort = 11;
+
+__device__ __forceinline__ void pc_add(PackedCounts& P, int byte) {
+    const uint32_t one = 1u << (8 * (byte & 3));
+    P.w[0] += byte < 4 ? one : 0u;
+    P.w[1] += (byte >> 2) == 1 ? one : 0u;
+    P.w[2] += byte >= 8 ? one : 0u;
+}
+
+// count() for a pixel of the chain pass (sign +1); histogram bins past 3 go
+// straight to the global counters as there.
+__device__ __forceinline__ void pc_count(PackedCounts& P, int cat, int it, unsigned long long* g) {
+    if (cat <= CAT_NOISE) pc_add(P, cat);
+    if (cat == CAT_VALID || cat == CAT_OUT || cat == CAT_FLAT || cat == CAT_NOISE) {
+        if (it >= 1 && it <= 3) pc_add(P, 6 + it);
+        else atomicAdd(&g[7 + (it < kHistSize - 1 ? it : kHistSize - 1)], 1ull);
+    }
+}
+
+__device__ __forceinline__ int pc_get(const PackedCounts& P, int byte) {
+    return (int)((P.w[byte >> 2] >> (8 * (byte & 3))) & 0xffu);
+}
+
+__device__ __forceinline__ LaneCounts pc_unpack(const PackedCounts& P) {
+    LaneCounts C;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) C.c[i] = pc_get(P, i);
+    C.h1 = pc_get(P, 7);
+    C.h2 = pc_get(P, 8);
+    C.h3 = pc_get(P, 9);
+    C.err_outside = pc_get(P, kPcErrOutside);
+    C.err_short = pc_get(P, kPcErrShort);
+    return C;
+}
+
 // Copies the orbit into LDS up to its last interval polynomial.
 __device__ __forceinline__ void load_orbit(const TcParams& p, OrbitDev& orb) {
     const uint4* src = reinterpret_cast<const uint4*>(p.orbit);
@@ -665,8 +708,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
     }
 
     // phase B: the reference's sequential chain over the segment
-    LaneCounts C;
-    zero_counts(C);
+    PackedCounts C = {{0u, 0u, 0u}};
     long long seed = S, last_t = 0;
     bool last_has = false, any_newton = false;
     st.tie_flags[lane] = 0;
@@ -689,7 +731,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
             v_s0 = v_cos = __builtin_nanf("");
             v_mask = 2;
             if (row_ok) {
-                C.c[1] += 1;
+                pc_add(C, CAT_DEM);
                 if (p.radar_coords) {
                     p.radar_coords[2 * (obase + cc)] = __builtin_nan("");
                     p.radar_coords[2 * (obase + cc) + 1] = __builtin_nan("");
@@ -715,13 +757,13 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
             if (row_ok) {
                 if (v.cat == CAT_ORBIT) {
                     if (s.oc == ZD_ORBIT_OUTSIDE) {
-                        C.err_outside += 1;
+                        pc_add(C, kPcErrOutside);
                         note_orbit_outside(p, orb, s.t);
                     } else {
-                        C.err_short += 1;
+                        pc_add(C, kPcErrShort);
                     }
                 } else {
-                    count(C, v.cat, s.it, 1, p.counters);
+                    pc_count(C, v.cat, s.it, p.counters);
                 }
                 if (p.radar_coords) {
                     p.radar_coords[2 * (obase + cc)] = v.line;
@@ -751,15 +793,15 @@ __device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitD
         rec.seed = S;
         rec.last = last_t;
         rec.flags = (last_has ? 1 : 0) | (any_newton ? 2 : 0) | ((tf & 1) ? 4 : 0) | ((tf & 2) ? 8 : 0);
-        rec.err_outside = C.err_outside;
-        rec.err_short = C.err_short;
+        rec.err_outside = pc_get(C, kPcErrOutside);
+        rec.err_short = pc_get(C, kPcErrShort);
         rec.tie_first = tf ? st.tie_first[lane] : 0;
         rec.tie_seed = tf ? st.tie_seed[lane] : 0;
         rec.tie_last = tf ? st.tie_last[lane] : 0;
         rec.pad = 0;
         p.records[rr * p.nseg + seg] = rec;
     }
-    flush_counts(p, C, false);  // orbit errors are decided after the fix-up
+    flush_counts(p, pc_unpack(C), false);  // orbit errors are decided after the fix-up
 }
 
 // One wave per AOI row, before the fix-up pass: a segment that ran a Newton
