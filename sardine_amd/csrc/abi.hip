@@ -789,6 +789,7 @@ sardine_status plan_tc(const sardine_radar_image* merged, const sardine_scene* s
     sd::TcParams& p = pl->p;
     std::memset(&p, 0, sizeof p);
     p.orbit = sdv.orb;
+    p.orbit_lds = (int32_t)sd::orbit_lds_bytes(scene->orbit.count);
     p.img = merged->data;
     p.nesz = (std::isfinite(cfg->noise_floor_margin_db) && merged->nesz) ? merged->nesz : nullptr;
     p.lines = merged->lines;
@@ -1185,6 +1186,7 @@ sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* s
     SD_HIP(mem.copy_in(d_lut_lat, lut.lat.data(), lut.lat.size() * sizeof(double)));
     SD_HIP(mem.copy_in(d_lut_t, lut.t.data(), lut.t.size() * sizeof(int64_t)));
     pl.p.orbit = reinterpret_cast<const sd::OrbitDev*>(d_orb);
+    pl.p.orbit_lds = (int32_t)sd::orbit_lds_bytes(orb.n);
     if (d_dem_tab) {
         double tab[sd::kDemTabWords];
         dem_tile_table(dem->tiling, tab);

@@ -58,9 +58,14 @@ __global__ __launch_bounds__(256) void wgs_col_kernel(TcParams p) {
     if (i < p.cols) p.col_geo[i] = col_geo(p, p.col0 + i);
 }
 
+// The orbit's LDS copy is dynamic, sized to the orbit (p.orbit_lds bytes:
+// 13 KiB for 17 state vectors, at most 40 KiB), so that three workgroups fit
+// a CU where the chain runs three waves per SIMD.
 template <int RS, bool UTM>
-__global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void tc_segment_kernel(TcParams p) {
-    __shared__ OrbitDev orb;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(chain_waves<RS, UTM>()))) void tc_segment_kernel(
+    TcParams p) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char tc_orbit_lds[];
+    OrbitDev& orb = *reinterpret_cast<OrbitDev*>(tc_orbit_lds);
     __shared__ WaveStage stage[4];
     load_orbit(p, orb);
     const int w = threadIdx.x >> 6;
@@ -111,7 +116,8 @@ template <int RS, bool UTM>
 void launch_chain(const sd::TcParams* p, hipStream_t stream, hipEvent_t seg_start, hipEvent_t seg_end) {
     dim3 grid((unsigned)((p->rows + 63) / 64), (unsigned)((p->nseg + 3) / 4));
     if (seg_start) (void)hipEventRecord(seg_start, stream);
-    hipLaunchKernelGGL((sd::tc_segment_kernel<RS, UTM>), grid, dim3(256), 0, stream, *p);
+    const size_t orbit_lds = ((size_t)p->orbit_lds + 15) & ~(size_t)15;
+    hipLaunchKernelGGL((sd::tc_segment_kernel<RS, UTM>), grid, dim3(256), orbit_lds, stream, *p);
     if (seg_end) (void)hipEventRecord(seg_end, stream);
     hipLaunchKernelGGL(sd::tc_check_kernel, dim3((unsigned)((p->rows + 3) / 4)), dim3(256), 0, stream, *p);
     if (p->tie_list && p->tie_cap > 0) {
@@ -157,8 +163,9 @@ extern "C" hipError_t sd_launch_tc_fp64(const sd::TcParams* p, hipStream_t strea
                                         hipEvent_t seg_end) {
     const bool utm = p->crs != 0;
     // EPSG:4326 chains read their grid's column table (UTM ones fall back to
-    // the per-pixel blend without one)
+    // the per-pixel blend without one); the orbit's LDS copy must be sized
     if (!utm && p->col_geo == nullptr) return hipErrorInvalidValue;
+    if (p->orbit_lds <= 0 || (size_t)p->orbit_lds > sizeof(sd::OrbitDev)) return hipErrorInvalidValue;
     hipError_t e = sd_launch_tc_anchors(p, stream);
     if (e != hipSuccess) return e;
     // RS 0..2: the resampler in FP64; 3..5: the same in the mixed form

@@ -612,66 +612,29 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
     wave_sync();
 }
 
-// Occupancy floor of the chain kernels (waves per SIMD the register
-// allocator must allow); a build option so that it can be tuned per compiler.
-#ifndef SD_TC_SEGMENT_WAVES_PER_EU
-#define SD_TC_SEGMENT_WAVES_PER_EU 2
-#endif
-#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SD_TC_SEGMENT_WAVES_PER_EU)))
-
-// The FP64 constants the per-pixel code reads (grid, DEM, radar geometry,
-// Newton tolerance) as a copy of the kernel arguments held in vector
-// registers: kept in scalar registers across the pixel loop they
-// exceed the 102 SGPRs of a wave and are spilled and reloaded one
-// v_readlane per dword per use; the kernel has spare VGPRs at its
-// occupancy.  The empty asm only pins each value to a VGPR.
-template <typename T>
-__device__ __forceinline__ void pin_vgpr(T& x) {
-    asm volatile("" : "+v"(x));
+// Occupancy of the chain kernels (waves per SIMD the register allocator
+// must allow).  The per-call chains of the UTM bilinear and bicubic forms
+// (FP64 and mixed) run at three: 168 VGPRs without spills, and their orbit's
+// LDS copy sized to the orbit (sd_launch_tc_fp64) leaves room for three
+// workgroups per CU; 18.7 against 20.9 ms at two on the full scene.  The
+// Lanczos-3 and EPSG:4326 chains and the batched kernel (every job class in
+// one kernel) need more registers and stay at two.
+template <int RS, bool UTM>
+constexpr int chain_waves() {
+    return UTM && RS % 3 != 2 ? 3 : 2;
 }
-
-__device__ __forceinline__ TcParams vgpr_params(const TcParams& p) {
-    TcParams q = p;
-    pin_vgpr(q.spacing);
-    pin_vgpr(q.lat_north);
-    pin_vgpr(q.lon_west);
-    pin_vgpr(q.x_start);
-    pin_vgpr(q.anchor_dx);
-    pin_vgpr(q.inv_anchor_dx);
-    pin_vgpr(q.dem_olat);
-    pin_vgpr(q.dem_olon);
-    pin_vgpr(q.dem_pix);
-    pin_vgpr(q.dem_inv_pix);
-    pin_vgpr(q.dem_max_row);
-    pin_vgpr(q.dem_max_col);
-    pin_vgpr(q.dem_hi_row);
-    pin_vgpr(q.dem_hi_col);
-    pin_vgpr(q.step_deg);
-    pin_vgpr(q.step_sin);
-    pin_vgpr(q.step_cos);
-    pin_vgpr(q.near_range_m);
-    pin_vgpr(q.rps);
-    pin_vgpr(q.inv_rps);
-    pin_vgpr(q.ati);
-    pin_vgpr(q.inv_ati);
-    pin_vgpr(q.tol);
-    return q;
-}
+#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
 
 // Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
 // a workgroup take 4 consecutive segments.  Lanes past the last row compute
 // the last row again and discard it, so every lane reaches the wave-wide
 // flushes.
-// PIN: use vgpr_params (pays where the kernel has spare VGPRs: the UTM
-// bilinear and bicubic chains; the Lanczos, EPSG:4326 and batched kernels
-// are at the register limit already and would spill to scratch instead).
 // COLS: EPSG:4326 chains take their row and column terms from RowGeo and
 // the column table (the batched kernel, which holds every job class in one
 // kernel, keeps the per-pixel form: the row terms would raise its spills).
-template <int RS, bool UTM, bool PIN = (UTM && RS % 3 <= 1), bool COLS = true>
-__device__ __forceinline__ void segment_wave(const TcParams& p_arg, const OrbitDev& orb, WaveStage& st, int64_t rr0,
+template <int RS, bool UTM, bool COLS = true>
+__device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                              int64_t seg) {
-    const TcParams p = PIN ? vgpr_params(p_arg) : p_arg;
     const int lane = threadIdx.x & 63;
     const bool row_ok = rr0 + lane < p.rows;
     const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;

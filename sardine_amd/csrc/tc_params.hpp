@@ -78,6 +78,8 @@ struct ColGeo {
 
 struct TcParams {
     const OrbitDev* orbit;
+    int32_t orbit_lds;      // bytes of its LDS copy, orbit_lds_bytes(n) (the per-call chain sizes its LDS by it)
+    int32_t orbit_lds_pad;
     // FP32 scene-centred variant (precision SARDINE_PRECISION_FP32_CENTRED)
     const OrbitF32* orbit32;
     double centre[3];       // scene centre, ECEF metres
