@@ -613,15 +613,19 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
 }
 
 // Occupancy of the chain kernels (waves per SIMD the register allocator
-// must allow).  The per-call chains of the UTM bilinear and bicubic forms
-// (FP64 and mixed) run at three: 168 VGPRs without spills, and their orbit's
-// LDS copy sized to the orbit (sd_launch_tc_fp64) leaves room for three
-// workgroups per CU; 18.7 against 20.9 ms at two on the full scene.  The
-// Lanczos-3 and EPSG:4326 chains and the batched kernel (every job class in
-// one kernel) need more registers and stay at two.
+// must allow), measured per form on the full scene.  At three: the per-call
+// UTM chains of the bilinear and bicubic forms (FP64 and mixed) and of the
+// mixed Lanczos-3 form (165-168 VGPRs; bilinear 18.1 against 20.9 ms at two,
+// mixed Lanczos-3 27.5 against 31.4) and the EPSG:4326 bicubic chains
+// (FP64 30.8 -> 29.5 ms, mixed 31.0 -> 28.0 on an 18001 x 36067 grid).  The
+// orbit's LDS copy is sized to the orbit (sd_launch_tc_fp64), which leaves
+// room for three workgroups per CU.  At two: the FP64 Lanczos-3 chains
+// (they spill at three: UTM 36.0 -> 44.1 ms), the EPSG:4326 bilinear chains
+// (even either way; at two they keep the row/column terms, see
+// segment_wave) and the batched kernel (every job class in one kernel).
 template <int RS, bool UTM>
 constexpr int chain_waves() {
-    return UTM && RS % 3 != 2 ? 3 : 2;
+    return UTM ? (RS % 3 != 2 || RS == 5 ? 3 : 2) : (RS % 3 == 1 ? 3 : 2);
 }
 #define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
 
@@ -676,11 +680,12 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     bool last_has = false, any_newton = false;
     st.tie_flags[lane] = 0;
     AnchorCache ac = anchor_cache_empty();
-    // EPSG:4326: the row's terms once, the column's from the table (every
-    // EPSG:4326 job has one).  Not for Lanczos-3, whose chain is at the
-    // register limit already: the row terms held across the walk make it
-    // spill (45.3 -> 48.5 ms on a 649 Mpx grid).
-    constexpr bool WGS = COLS && !UTM && RS % 3 != 2;
+    // EPSG:4326 bilinear: the row's terms once, the column's from the table
+    // (every EPSG:4326 job has one).  Not for Lanczos-3, whose chain is at
+    // the register limit already (the row terms held across the walk made it
+    // spill: 45.3 -> 48.5 ms on a 649 Mpx grid), nor for bicubic, which runs
+    // at three waves per SIMD instead (chain_waves).
+    constexpr bool WGS = COLS && !UTM && RS % 3 == 0;
     RowGeo rg;
     if constexpr (WGS) rg = row_geo(p, row_y);
     for (int64_t cc = c_begin; cc < c_end; ++cc) {
