@@ -137,16 +137,23 @@ SD_HD int orbit_window_start(int c, int n) {
 // node's is exactly 0 and the node's is exactly 1, so P and V are the node's
 // state vector and A = v_c · Σ_{j≠c} 1/(t - t_j) over the window, summed in
 // window order.
-SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A) {
+// `hint` (optional): the interval of the caller's previous evaluation; a
+// time strictly inside it (no node, inside the window) skips the search,
+// and the interval found is stored back.
+SD_HD int orbit_interp(const OrbitDev& o, int64_t t, d3* P, d3* V, d3* A, int* hint = nullptr) {
     const int n = o.n;
     if (n < kLagrange) return ORB_INSUFFICIENT;
-    if (t < o.t_us[0] || t > o.t_us[n - 1]) return ORB_OUTSIDE;
     const double tr = (double)(t - o.t_us[0]);
     const double ts = div_rn(tr, 1e6, 1e-6);
-    int c = (int)(tr * o.guess_scale);  // a guess: the walks below decide
-    c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
-    while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
-    while (c > 0 && o.t_us[c] > t) --c;
+    int c = hint ? *hint : -1;
+    if (!(c >= 0 && c + 1 < n && o.t_us[c] < t && t < o.t_us[c + 1])) {
+        if (t < o.t_us[0] || t > o.t_us[n - 1]) return ORB_OUTSIDE;
+        c = (int)(tr * o.guess_scale);  // a guess: the walks below decide
+        c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+        while (c + 1 < n && o.t_us[c + 1] <= t) ++c;
+        while (c > 0 && o.t_us[c] > t) --c;
+        if (hint) *hint = c;
+    }
     if (t == o.t_us[c]) {
         const int start = orbit_window_start(c, n);
         double tot = 0.0;
@@ -264,11 +271,13 @@ enum : int { ZD_CONVERGED = 0, ZD_NOT_CONVERGED = 1, ZD_DEGENERATE = 2, ZD_ORBIT
 struct OrbitCache {
     int64_t t;
     d3 P, V, A;
+    int c;  // interval of the last evaluation (orbit_interp's hint)
     bool valid;
 };
 
 SD_HD OrbitCache orbit_cache_empty() {
     OrbitCache c;
+    c.c = -1;
     c.t = 0;
     c.P = c.V = c.A = d3{0.0, 0.0, 0.0};
     c.valid = false;
@@ -293,7 +302,7 @@ SD_HD int zero_doppler(const OrbitDev& o, d3 target, int64_t seed, int max_iter,
         // the evaluation writes the cache itself: one copy of (P, V, A)
         // lives across the iterations, not a cache and a working set
         if (!(cache->valid && cache->t == t)) {
-            int e = orbit_interp(o, t, &cache->P, &cache->V, &cache->A);
+            int e = orbit_interp(o, t, &cache->P, &cache->V, &cache->A, &cache->c);
             if (e != ORB_OK) {
                 cache->valid = false;
                 if (e == ORB_OUTSIDE) {
