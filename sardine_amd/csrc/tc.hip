@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(chain_waves
     const int w = threadIdx.x >> 6;
     const int64_t seg = (int64_t)blockIdx.y * 4 + w;
     if (seg >= p.nseg) return;  // whole wave; no block barriers below
-    segment_wave<RS, UTM>(p, orb, stage[w], (int64_t)blockIdx.x * 64, seg);
+    segment_wave<RS, UTM, true, 1>(p, orb, stage[w], (int64_t)blockIdx.x * 64, seg);
 }
 
 // One wave per AOI row (check_row).

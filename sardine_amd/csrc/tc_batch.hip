@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void batch_anchor_kernel(const BatchHeader* __
 template <int RS, bool UTM>
 __device__ __forceinline__ void batch_item(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                            int64_t seg) {
-    segment_wave<RS, UTM, false>(p, orb, st, rr0, seg);
+    segment_wave<RS, UTM, false, 2>(p, orb, st, rr0, seg);   // the job table is read-only here
 }
 
 // Persistent chain pass: `queue` is zero when the launch starts.

@@ -613,19 +613,18 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
 }
 
 // Occupancy of the chain kernels (waves per SIMD the register allocator
-// must allow), measured per form on the full scene.  At three: the per-call
-// UTM chains of the bilinear and bicubic forms (FP64 and mixed) and of the
-// mixed Lanczos-3 form (165-168 VGPRs; bilinear 18.1 against 20.9 ms at two,
-// mixed Lanczos-3 27.5 against 31.4) and the EPSG:4326 bicubic chains
-// (FP64 30.8 -> 29.5 ms, mixed 31.0 -> 28.0 on an 18001 x 36067 grid).  The
-// orbit's LDS copy is sized to the orbit (sd_launch_tc_fp64), which leaves
-// room for three workgroups per CU.  At two: the FP64 Lanczos-3 chains
-// (they spill at three: UTM 36.0 -> 44.1 ms), the EPSG:4326 bilinear chains
-// (even either way; at two they keep the row/column terms, see
-// segment_wave) and the batched kernel (every job class in one kernel).
+// must allow), measured per form on the full scene.  At three: every
+// per-call chain but the FP64 Lanczos-3 ones (155-168 VGPRs with the
+// parameters reloaded in the pixel loop; UTM bilinear 18.1 against 20.9 ms
+// at two, mixed Lanczos-3 27.5 against 31.4; EPSG:4326 bilinear 24.7 ->
+// 20.0 ms, mixed 24.3 -> 21.2, mixed Lanczos-3 38.0 -> 35.7 on an
+// 18001 x 36067 grid).  The orbit's LDS copy is sized to the orbit
+// (sd_launch_tc_fp64), which leaves room for three workgroups per CU.  At
+// two: the FP64 Lanczos-3 chains (214-219 VGPRs; they spill at three: UTM
+// 36.0 -> 44.1 ms) and the batched kernel (every job class in one kernel).
 template <int RS, bool UTM>
 constexpr int chain_waves() {
-    return UTM ? (RS % 3 != 2 || RS == 5 ? 3 : 2) : (RS % 3 == 1 ? 3 : 2);
+    return RS == 2 ? 2 : 3;
 }
 #define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
 
@@ -636,9 +635,31 @@ constexpr int chain_waves() {
 // COLS: EPSG:4326 chains take their row and column terms from RowGeo and
 // the column table (the batched kernel, which holds every job class in one
 // kernel, keeps the per-pixel form: the row terms would raise its spills).
-template <int RS, bool UTM, bool COLS = true>
-__device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
+// The chain's parameters read again where the pixel loop uses them: the
+// empty asm hides their address, so the loads cannot be hoisted out of the
+// loop.  Each use becomes a scalar load (scalar memory, not VALU) instead of
+// a uniform value held across the loop, which at the register limit spills
+// to a VGPR lane and costs a VALU readlane per use.  RELOAD 0: p is used as
+// it is; 1: p is the kernel's only explicit argument, which the
+// kernel-argument segment holds at offset 0; 2: p lies in device memory
+// that no thread writes during the kernel, at a wave-uniform address (a
+// batch's job table).
+template <int RELOAD>
+__device__ __forceinline__ const TcParams& params_reload(const TcParams& p) {
+    using KPtr = const __attribute__((address_space(4))) TcParams*;
+    if constexpr (RELOAD == 0) {
+        return p;
+    } else {
+        KPtr k = RELOAD == 1 ? (KPtr)__builtin_amdgcn_kernarg_segment_ptr() : (KPtr)&p;
+        asm volatile("" : "+s"(k));
+        return *(const TcParams*)k;
+    }
+}
+
+template <int RS, bool UTM, bool COLS = true, int RELOAD = 0>
+__device__ __forceinline__ void segment_wave(const TcParams& p0, const OrbitDev& orb, WaveStage& st, int64_t rr0,
                                              int64_t seg) {
+    const TcParams& p = p0;
     const int lane = threadIdx.x & 63;
     const bool row_ok = rr0 + lane < p.rows;
     const int64_t rr = row_ok ? rr0 + lane : p.rows - 1;
@@ -664,6 +685,7 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     if (seg > 0) {
         bool have = false;
         for (int64_t cc = c_begin > p.lead ? c_begin - p.lead : 0; cc < c_begin; ++cc) {
+            const TcParams& p = params_reload<RELOAD>(p0);
             PixelPos q = prep_pixel<UTM, false>(p, p.col0 + cc, row_y, alon, alat);
             if (!q.dem_ok) continue;
             Solve s = solve(p, orb, q, S, have ? p.max_iter : 40, cache);
@@ -681,14 +703,15 @@ __device__ __forceinline__ void segment_wave(const TcParams& p, const OrbitDev& 
     st.tie_flags[lane] = 0;
     AnchorCache ac = anchor_cache_empty();
     // EPSG:4326 bilinear: the row's terms once, the column's from the table
-    // (every EPSG:4326 job has one).  Not for Lanczos-3, whose chain is at
-    // the register limit already (the row terms held across the walk made it
-    // spill: 45.3 -> 48.5 ms on a 649 Mpx grid), nor for bicubic, which runs
-    // at three waves per SIMD instead (chain_waves).
+    // (every EPSG:4326 job has one).  Not for bicubic and Lanczos-3: with the
+    // row terms held across the walk their chains spill at the occupancy
+    // they run at and gain nothing (FP64 bicubic 25.5 -> 25.3 ms, Lanczos-3
+    // 43.6 -> 44.3, mixed Lanczos-3 35.7 -> 37.1 on an 18001 x 36067 grid).
     constexpr bool WGS = COLS && !UTM && RS % 3 == 0;
     RowGeo rg;
     if constexpr (WGS) rg = row_geo(p, row_y);
     for (int64_t cc = c_begin; cc < c_end; ++cc) {
+        const TcParams& p = params_reload<RELOAD>(p0);
         const int j = (int)(cc - c_begin);
         PixelPos q;
         if constexpr (WGS) q = prep_pixel_wgs(p, rg, p.col_geo[cc], p.col0 + cc);
