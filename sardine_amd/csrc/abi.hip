@@ -269,6 +269,14 @@ struct DevBuf {
     }
 };
 
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// A piece of a gathered host→device copy: n bytes from src to offset off.
+struct UploadPart {
+    const void* src;
+    size_t off, n;
+};
+
 // Host→device copies go through pinned staging blocks: from pageable memory
 // hipMemcpyAsync copies synchronously, which would make every call wait for
 // all earlier work on its stream.  A block is reused once the event recorded
@@ -276,6 +284,12 @@ struct DevBuf {
 class StagingPool {
    public:
     hipError_t copy(void* dst, const void* src, size_t n, hipStream_t s) {
+        const UploadPart part{src, 0, n};
+        return copy_parts(dst, &part, 1, n, s);
+    }
+    // One copy of `total` bytes to dst whose pieces are gathered from
+    // several host arrays (the bytes between pieces are undefined).
+    hipError_t copy_parts(void* dst, const UploadPart* parts, size_t np, size_t total, hipStream_t s) {
         int dev = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e != hipSuccess) return e;
@@ -285,11 +299,11 @@ class StagingPool {
         for (Block& x : blocks_) {
             if (x.busy && hipEventQuery(x.ev) == hipSuccess) x.busy = false;
             busy += x.busy ? 1 : 0;
-            if (!x.busy && x.dev == dev && x.cap >= n && (!b || x.cap < b->cap)) b = &x;
+            if (!x.busy && x.dev == dev && x.cap >= total && (!b || x.cap < b->cap)) b = &x;
         }
         if (!b && busy >= kMaxBusy) {  // bound the pinned footprint: wait for the oldest copy
             for (Block& x : blocks_)
-                if (x.busy && x.dev == dev && x.cap >= n) {
+                if (x.busy && x.dev == dev && x.cap >= total) {
                     if ((e = hipEventSynchronize(x.ev)) != hipSuccess) return e;
                     x.busy = false;
                     b = &x;
@@ -300,7 +314,7 @@ class StagingPool {
             Block nb;
             nb.dev = dev;
             nb.cap = 64 * 1024;
-            while (nb.cap < n) nb.cap *= 2;
+            while (nb.cap < total) nb.cap *= 2;
             if ((e = hipHostMalloc(&nb.p, nb.cap, hipHostMallocDefault)) != hipSuccess) return e;
             if ((e = hipEventCreateWithFlags(&nb.ev, hipEventDisableTiming)) != hipSuccess) {
                 (void)hipHostFree(nb.p);
@@ -309,8 +323,9 @@ class StagingPool {
             blocks_.push_back(nb);
             b = &blocks_.back();
         }
-        std::memcpy(b->p, src, n);
-        if ((e = hipMemcpyAsync(dst, b->p, n, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+        for (size_t i = 0; i < np; ++i)
+            if (parts[i].n) std::memcpy(static_cast<char*>(b->p) + parts[i].off, parts[i].src, parts[i].n);
+        if ((e = hipMemcpyAsync(dst, b->p, total, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
         if ((e = hipEventRecord(b->ev, s)) != hipSuccess) return e;
         b->busy = true;
         return hipSuccess;
@@ -341,6 +356,39 @@ hipError_t upload(DevBuf& b, const T* host, size_t n, hipStream_t s) {
     if (n) return staging().copy(b.p, host, n * sizeof(T), s);
     return hipSuccess;
 }
+
+// A call's host inputs gathered into one host→device copy.  Every copy from
+// the host is a transfer the stream waits for, which leaves the GPU idle for
+// tens of microseconds each whatever its size, so the small arrays a call
+// uploads (LUTs, line tables, the orbit, the seed table) travel together:
+// add() places each at a 256-byte aligned offset of one device region.
+class UploadPack {
+   public:
+    // offset of `n` bytes from src; `reserve` (>= n) keeps more room
+    size_t add(const void* src, size_t n, size_t reserve = 0) {
+        const size_t off = round256(size_);
+        parts_.push_back(UploadPart{src, off, n});
+        size_ = off + std::max(n, reserve);
+        return off;
+    }
+    size_t size() const { return size_ ? size_ : 1; }
+    // through a pinned staging block: the host arrays may go once this returns
+    hipError_t copy(void* dst, hipStream_t s) const {
+        return staging().copy_parts(dst, parts_.data(), parts_.size(), size(), s);
+    }
+    // straight from host memory (the caller keeps `host` and the arrays
+    // alive until the stream has passed the copy)
+    hipError_t copy_direct(void* dst, std::vector<char>& host, hipStream_t s) const {
+        host.assign(size(), 0);
+        for (const UploadPart& q : parts_)
+            if (q.n) std::memcpy(host.data() + q.off, q.src, q.n);
+        return hipMemcpyAsync(dst, host.data(), size(), hipMemcpyHostToDevice, s);
+    }
+
+   private:
+    std::vector<UploadPart> parts_;
+    size_t size_ = 0;
+};
 
 // ── seed LUT (terrain_correction.rs:1080-1137) ──────────────────────────
 struct SeedLut {
@@ -889,7 +937,6 @@ struct TcWork {
     int64_t tie_cap = 0;
     size_t total() const { return records + flags + ties + anchors + blend + seed + flat + geoid; }
 };
-size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 TcWork tc_work_bytes(const TcPlan& pl, const sardine_tc_config* cfg) {
     TcWork w;
     // the FP32 variant solves every pixel on its own: no chain records
@@ -1134,6 +1181,11 @@ class TcMemory {
         if (fixed_) return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s_);
         return staging().copy(dst, src, n, s_);
     }
+    // a gathered upload (size() bytes at dst), the same way
+    hipError_t copy_in(void* dst, const UploadPack& up) {
+        if (fixed_) return up.copy_direct(dst, host_, s_);
+        return up.copy(dst, s_);
+    }
 
    private:
     hipStream_t s_;
@@ -1141,6 +1193,7 @@ class TcMemory {
     size_t cap_;
     size_t used_ = 0;
     std::vector<std::unique_ptr<DevBuf>> bufs_;
+    std::vector<char> host_;  // a direct gathered upload's source
 };
 
 // terrain_correction over `mem`: planning, uploads, the kernels, counters.
@@ -1166,13 +1219,19 @@ sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* s
     const bool fp32 = cfg->precision == SARDINE_PRECISION_FP32_CENTRED;
     hipError_t e = hipSuccess;
     auto get = [&](size_t n) { return e == hipSuccess ? static_cast<char*>(mem.get(n, &e)) : nullptr; };
+    // the orbit, the seed table, the DEM tile table and the geoid grid: one upload
+    double tab[sd::kDemTabWords];
+    if (dem->tiling) dem_tile_table(dem->tiling, tab);
+    UploadPack up;
+    const size_t o_orb = up.add(&orb, sd::orbit_lds_bytes(orb.n), sizeof(sd::OrbitDev));
+    const size_t o_lat = up.add(lut.lat.data(), lut.lat.size() * sizeof(double));
+    const size_t o_t = up.add(lut.t.data(), lut.t.size() * sizeof(int64_t));
+    const size_t o_tab = dem->tiling ? up.add(tab, sizeof tab) : 0;
+    const size_t o_geoid = w.geoid ? up.add(cfg->geoid.grid_73x144, 73 * 144 * sizeof(float)) : 0;
     char* d_cnt = get(sd::kCntSlots * sizeof(unsigned long long));
-    char* d_orb = get(sizeof(sd::OrbitDev));
-    char* d_lut_lat = get(lut.lat.size() * sizeof(double) + 1);
-    char* d_lut_t = get(lut.t.size() * sizeof(int64_t) + 1);
+    char* d_up = get(up.size());
     char* d_work = get(w.total());
     char* d_orb32 = fp32 ? get(sizeof(sd::OrbitF32)) : nullptr;
-    char* d_dem_tab = dem->tiling ? get(sd::kDemTabWords * sizeof(double)) : nullptr;
     char* d_ends = fp32 ? get((size_t)pl.nr * 2 * sizeof(float)) : nullptr;
     if (e != hipSuccess) {
         if (mem.fixed() && e == hipErrorOutOfMemory)
@@ -1182,21 +1241,14 @@ sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* s
                         (long long)pl.nr, (long long)pl.nc, round256(mem.used()), kNoAllocScratchBytes);
         SD_HIP(e);
     }
-    SD_HIP(mem.copy_in(d_orb, &orb, sd::orbit_lds_bytes(orb.n)));
-    SD_HIP(mem.copy_in(d_lut_lat, lut.lat.data(), lut.lat.size() * sizeof(double)));
-    SD_HIP(mem.copy_in(d_lut_t, lut.t.data(), lut.t.size() * sizeof(int64_t)));
-    pl.p.orbit = reinterpret_cast<const sd::OrbitDev*>(d_orb);
+    SD_HIP(mem.copy_in(d_up, up));
+    pl.p.orbit = reinterpret_cast<const sd::OrbitDev*>(d_up + o_orb);
     pl.p.orbit_lds = (int32_t)sd::orbit_lds_bytes(orb.n);
-    if (d_dem_tab) {
-        double tab[sd::kDemTabWords];
-        dem_tile_table(dem->tiling, tab);
-        SD_HIP(mem.copy_in(d_dem_tab, tab, sizeof tab));
-        pl.p.dem_tab = reinterpret_cast<const double*>(d_dem_tab);
-    }
-    pl.q.lut_lat = reinterpret_cast<const double*>(d_lut_lat);
-    pl.q.lut_t = reinterpret_cast<const int64_t*>(d_lut_t);
+    if (dem->tiling) pl.p.dem_tab = reinterpret_cast<const double*>(d_up + o_tab);
+    pl.q.lut_lat = reinterpret_cast<const double*>(d_up + o_lat);
+    pl.q.lut_t = reinterpret_cast<const int64_t*>(d_up + o_t);
     assign_work(pl, w, d_work);
-    if (w.geoid) SD_HIP(mem.copy_in(const_cast<float*>(pl.p.geoid), cfg->geoid.grid_73x144, 73 * 144 * sizeof(float)));
+    if (w.geoid) pl.p.geoid = reinterpret_cast<const float*>(d_up + o_geoid);
     // every counter starts at zero (a kernel, not a memset: capturable)
     SD_HIP(sd_launch_zero_u64(reinterpret_cast<unsigned long long*>(d_cnt), sd::kCntSlots, stream));
     pl.p.counters = reinterpret_cast<unsigned long long*>(d_cnt);
@@ -1430,26 +1482,59 @@ std::vector<sd::CalLine> line_table(const sardine_lut* cal, const sardine_lut* n
     return t;
 }
 
+// Device inputs of the calibration kernel for one or more swaths: every
+// swath's LUT vectors and line table in one gathered upload, the dense LUTs
+// in one allocation.
 struct SwathDev {
-    DevBuf cal_px, cal_val, cal_off, nr_px, nr_val, nr_off, cal_dense, nr_dense, lines;
+    const float* cal_dense;
+    const float* nr_dense;
+    const sd::CalLine* lines;
+};
+struct SwathsDev {
+    DevBuf packed, dense;
+    std::vector<SwathDev> sw;
 };
 
-sardine_status stage_swath(const sardine_lut* cal, const sardine_lut* nr, int64_t dense_cols,
-                           const std::vector<sd::CalLine>& lt, SwathDev& d, hipStream_t s) {
-    const int64_t nc = cal->offsets[cal->n_vectors], nn = nr->offsets[nr->n_vectors];
-    SD_HIP(upload(d.cal_px, cal->pixels, (size_t)nc, s));
-    SD_HIP(upload(d.cal_val, cal->values, (size_t)nc, s));
-    SD_HIP(upload(d.cal_off, cal->offsets, (size_t)cal->n_vectors + 1, s));
-    SD_HIP(upload(d.nr_px, nr->pixels, (size_t)nn, s));
-    SD_HIP(upload(d.nr_val, nr->values, (size_t)nn, s));
-    SD_HIP(upload(d.nr_off, nr->offsets, (size_t)nr->n_vectors + 1, s));
-    SD_HIP(d.cal_dense.alloc((size_t)cal->n_vectors * dense_cols * sizeof(float), s));
-    SD_HIP(d.nr_dense.alloc((size_t)nr->n_vectors * dense_cols * sizeof(float), s));
-    SD_HIP(sd_launch_lut_dense(d.cal_px.as<uint32_t>(), d.cal_val.as<float>(), d.cal_off.as<int64_t>(), cal->n_vectors,
-                               dense_cols, d.cal_dense.as<float>(), s));
-    SD_HIP(sd_launch_lut_dense(d.nr_px.as<uint32_t>(), d.nr_val.as<float>(), d.nr_off.as<int64_t>(), nr->n_vectors,
-                               dense_cols, d.nr_dense.as<float>(), s));
-    SD_HIP(upload(d.lines, lt.data(), lt.size(), s));
+sardine_status stage_swaths(int n, const sardine_lut* const* cal, const sardine_lut* const* nr,
+                            const int64_t* dense_cols, const std::vector<sd::CalLine>* lt, SwathsDev& d,
+                            hipStream_t s) {
+    UploadPack up;
+    std::vector<size_t> o((size_t)n * 7);
+    std::vector<size_t> od((size_t)n * 2);
+    size_t dense = 0;
+    for (int i = 0; i < n; ++i) {
+        const int64_t nc = cal[i]->offsets[cal[i]->n_vectors], nn = nr[i]->offsets[nr[i]->n_vectors];
+        size_t* q = &o[(size_t)i * 7];
+        q[0] = up.add(cal[i]->pixels, (size_t)nc * sizeof(uint32_t));
+        q[1] = up.add(cal[i]->values, (size_t)nc * sizeof(float));
+        q[2] = up.add(cal[i]->offsets, ((size_t)cal[i]->n_vectors + 1) * sizeof(int64_t));
+        q[3] = up.add(nr[i]->pixels, (size_t)nn * sizeof(uint32_t));
+        q[4] = up.add(nr[i]->values, (size_t)nn * sizeof(float));
+        q[5] = up.add(nr[i]->offsets, ((size_t)nr[i]->n_vectors + 1) * sizeof(int64_t));
+        q[6] = up.add(lt[i].data(), lt[i].size() * sizeof(sd::CalLine));
+        od[(size_t)i * 2] = dense;
+        dense = round256(dense + (size_t)cal[i]->n_vectors * dense_cols[i] * sizeof(float));
+        od[(size_t)i * 2 + 1] = dense;
+        dense = round256(dense + (size_t)nr[i]->n_vectors * dense_cols[i] * sizeof(float));
+    }
+    SD_HIP(d.packed.alloc(up.size(), s));
+    SD_HIP(up.copy(d.packed.p, s));
+    SD_HIP(d.dense.alloc(dense, s));
+    char* base = d.packed.as<char>();
+    char* dbase = d.dense.as<char>();
+    d.sw.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const size_t* q = &o[(size_t)i * 7];
+        float* cd = reinterpret_cast<float*>(dbase + od[(size_t)i * 2]);
+        float* nd = reinterpret_cast<float*>(dbase + od[(size_t)i * 2 + 1]);
+        SD_HIP(sd_launch_lut_dense(reinterpret_cast<const uint32_t*>(base + q[0]), reinterpret_cast<const float*>(base + q[1]),
+                                   reinterpret_cast<const int64_t*>(base + q[2]), cal[i]->n_vectors, dense_cols[i], cd,
+                                   s));
+        SD_HIP(sd_launch_lut_dense(reinterpret_cast<const uint32_t*>(base + q[3]), reinterpret_cast<const float*>(base + q[4]),
+                                   reinterpret_cast<const int64_t*>(base + q[5]), nr[i]->n_vectors, dense_cols[i], nd,
+                                   s));
+        d.sw[(size_t)i] = SwathDev{cd, nd, reinterpret_cast<const sd::CalLine*>(base + q[6])};
+    }
     return SARDINE_OK;
 }
 
@@ -1522,8 +1607,9 @@ static sardine_status sardine_prepare_merged_impl(const sardine_swath_input* sw,
     if (!sigma0 || !nesz) return fail(SARDINE_ERR_INVALID_ARGUMENT, "null output raster");
     if ((st = require_device())) return st;
     hipStream_t s = (hipStream_t)stream_v;
-    std::vector<SwathDev> dev((size_t)n);
     std::vector<std::vector<sd::CalLine>> tables((size_t)n);
+    std::vector<const sardine_lut*> cals((size_t)n), nrs((size_t)n);
+    std::vector<int64_t> dcols((size_t)n);
     sd::CalParams P;
     std::memset(&P, 0, sizeof P);
     if (info->samples >= INT32_MAX) return fail(SARDINE_ERR_CAPACITY, "merged raster wider than 2^31");
@@ -1545,21 +1631,27 @@ static sardine_status sardine_prepare_merged_impl(const sardine_swath_input* sw,
         const int64_t dense_cols = L.valid_sample_offset + L.samples;
         tables[(size_t)i] = line_table(&sw[i].cal, &sw[i].noise_range, &sw[i].noise_azimuth, sw[i].bursts[0].first_line,
                                        info->lines, &L, sw[i].bursts);
-        if ((st = stage_swath(&sw[i].cal, &sw[i].noise_range, dense_cols, tables[(size_t)i], dev[(size_t)i], s))) return st;
+        cals[(size_t)i] = &sw[i].cal;
+        nrs[(size_t)i] = &sw[i].noise_range;
+        dcols[(size_t)i] = dense_cols;
         sd::CalSwathDev& D = P.sw[i];
         D.raw = sw[i].raw;
         D.raw_cols = sw[i].raw_cols;
         D.first_valid = L.valid_sample_offset;
-        D.cal_dense = dev[(size_t)i].cal_dense.as<float>();
-        D.nr_dense = dev[(size_t)i].nr_dense.as<float>();
         D.dense_cols = dense_cols;
-        D.lines = dev[(size_t)i].lines.as<sd::CalLine>();
         D.out_off = info->sw_out_offset[i];
         const int64_t col_start = i == 0 ? 0 : info->seam[i - 1];
         const int64_t col_end = i == n - 1 ? info->samples : info->seam[i];
         D.own_end = col_end;
         D.valid_start = std::max(col_start, D.out_off);
         D.valid_end = std::min(col_end, D.out_off + L.samples);
+    }
+    SwathsDev dev;
+    if ((st = stage_swaths(n, cals.data(), nrs.data(), dcols.data(), tables.data(), dev, s))) return st;
+    for (int i = 0; i < n; ++i) {
+        P.sw[i].cal_dense = dev.sw[(size_t)i].cal_dense;
+        P.sw[i].nr_dense = dev.sw[(size_t)i].nr_dense;
+        P.sw[i].lines = dev.sw[(size_t)i].lines;
     }
     {
         KernelTimer kt(SARDINE_KERNEL_CALIBRATE, s);
@@ -1594,8 +1686,10 @@ static sardine_status sardine_apply_calibration_impl(const int16_t* deburst, int
     if ((st = require_device())) return st;
     hipStream_t s = (hipStream_t)stream_v;
     auto lt = line_table(cal, nr, az, origin, lines, nullptr, nullptr);
-    SwathDev d;
-    if ((st = stage_swath(cal, nr, col_off + samples, lt, d, s))) return st;
+    SwathsDev dev;
+    const int64_t dcols = col_off + samples;
+    if ((st = stage_swaths(1, &cal, &nr, &dcols, &lt, dev, s))) return st;
+    const SwathDev& d = dev.sw[0];
     sd::CalParams P;
     std::memset(&P, 0, sizeof P);
     P.n_sw = 1;
@@ -1608,10 +1702,10 @@ static sardine_status sardine_apply_calibration_impl(const int16_t* deburst, int
     D.raw = deburst - 2 * col_off;  // address arithmetic only; indices stay in [col_off, col_off+samples)
     D.raw_cols = samples;
     D.first_valid = col_off;
-    D.cal_dense = d.cal_dense.as<float>();
-    D.nr_dense = d.nr_dense.as<float>();
-    D.dense_cols = col_off + samples;
-    D.lines = d.lines.as<sd::CalLine>();
+    D.cal_dense = d.cal_dense;
+    D.nr_dense = d.nr_dense;
+    D.dense_cols = dcols;
+    D.lines = d.lines;
     D.out_off = 0;
     D.own_end = samples;
     D.valid_start = 0;
