@@ -1147,6 +1147,20 @@ sardine_status sardine_orbit_interpolate(const sardine_orbit* orbit, int64_t tim
     return SARDINE_OK;
 }
 
+// A pinned block per host thread for a call's counters (allocated on the
+// thread's first call, kept until the process ends).
+hipError_t pinned_counters(unsigned long long** out) {
+    thread_local unsigned long long* block = nullptr;
+    if (!block) {
+        void* p = nullptr;
+        const hipError_t e = hipHostMalloc(&p, sd::kCntSlots * sizeof(unsigned long long), hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        block = static_cast<unsigned long long*>(p);
+    }
+    *out = block;
+    return hipSuccess;
+}
+
 // Device memory of one terrain-correction call: stream-ordered allocations,
 // or consecutive 256-byte aligned slices of a fixed region (the
 // no-allocation entry point's scratch array).
@@ -1181,6 +1195,9 @@ class TcMemory {
         if (fixed_) return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s_);
         return staging().copy(dst, src, n, s_);
     }
+    // free the allocated buffers (stream-ordered, after the work enqueued
+    // so far); the fixed region needs nothing
+    void release() { bufs_.clear(); }
     // a gathered upload (size() bytes at dst), the same way
     hipError_t copy_in(void* dst, const UploadPack& up) {
         if (fixed_) return up.copy_direct(dst, host_, s_);
@@ -1282,9 +1299,17 @@ sardine_status tc_call(const sardine_radar_image* merged, const sardine_scene* s
         if (seg) seg->pending = true;
         kt.stop();
     }
-    unsigned long long cnt[sd::kCntSlots];
-    SD_HIP(hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, stream));
+    // The counters come back through a pinned block (a copy into pageable
+    // memory would wait for the kernels by itself), and the call's buffers
+    // are released before the wait: the frees are stream-ordered, and their
+    // host cost then overlaps the kernels instead of delaying the return.
+    unsigned long long* hc = nullptr;
+    SD_HIP(pinned_counters(&hc));
+    SD_HIP(hipMemcpyAsync(hc, d_cnt, sd::kCntSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    mem.release();
     SD_HIP(hipStreamSynchronize(stream));
+    unsigned long long cnt[sd::kCntSlots];
+    std::memcpy(cnt, hc, sizeof cnt);
     return finish_tc(pl, cnt, scene->orbit, out);
 }
 
