@@ -872,11 +872,16 @@ __device__ __forceinline__ void check_row(const TcParams& p, int64_t rr) {
 // the corrected chain's outcome and C receives the counter corrections;
 // without, nothing is written (a dry run that finds out whether the chains
 // meet).  Returns whether they met; *new_last / *new_has are the corrected
-// chain's last converged time over the replayed columns.
+// chain's last converged time over the replayed columns, and *changed
+// whether any pixel's outcome differs between the chains.  A pixel both
+// chains converge on at the same µs in the same number of iterations has
+// the same outcome bits (the satellite position is the evaluation of that
+// µs), so it is neither finished nor written again.
 template <int RS, bool UTM, bool WRITE>
 __device__ __forceinline__ bool replay_segment(const TcParams& p, const OrbitDev& orb, int64_t rr, int64_t c_begin,
                                                int64_t c_end, int64_t sync_from, long long so, long long sn,
-                                               LaneCounts& C, long long* new_last, bool* new_has) {
+                                               LaneCounts& C, long long* new_last, bool* new_has,
+                                               bool* changed = nullptr) {
     const int64_t row = p.row0 + rr;
     const double row_y = p.lat_north - ((double)row + 0.5) * p.spacing;
     const int npts = p.anchor_stride;
@@ -892,13 +897,20 @@ __device__ __forceinline__ bool replay_segment(const TcParams& p, const OrbitDev
         if (!q.dem_ok) continue;  // outcome independent of the seed
         Solve o = solve(p, orb, q, so, p.max_iter, co);
         Solve n = solve<true>(p, orb, q, sn, p.max_iter, cn);
-        if (o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t && cc >= sync_from) {
+        const bool same_t = o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t;
+        if (changed && !(same_t && o.it == n.it)) *changed = true;
+        if (same_t && (cc >= sync_from || o.it == n.it)) {
             if (WRITE && o.it != n.it) {  // same result, different iteration count
                 PixelOut v = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
                 count(C, v.cat, o.it, -1, p.counters);
                 count(C, v.cat, n.it, 1, p.counters);
             }
-            return true;
+            if (cc >= sync_from) return true;
+            so = o.t;  // the same outcome: nothing to rewrite
+            sn = n.t;
+            *new_last = n.t;
+            *new_has = true;
+            continue;
         }
         if (WRITE) {
             PixelOut vo = finish_pixel<RS>(p, q, o.oc, o.t, o.sat, inv_flat_area);
@@ -931,6 +943,10 @@ __device__ __forceinline__ bool replay_segment(const TcParams& p, const OrbitDev
     return false;
 }
 
+// Tie list entry flag (in the segment index): the dry pass found the
+// corrected chain equal to the old one on every pixel.
+constexpr int kTieUnchanged = 1 << 30;
+
 // One entry of the tie list: the replay of one tie-only segment, from its
 // first tie pixel.  The dry pass (WRITE false) sends the row to the
 // sequential walk when the chains do not meet inside the segment (the
@@ -942,15 +958,17 @@ __device__ __forceinline__ void tie_item(const TcParams& p, const OrbitDev& orb,
     zero_counts(C);
     if (i < n) {
         const int2 e = p.tie_list[i];
-        const int64_t rr = e.x, sg = e.y;
-        if (!(p.row_flags[rr] & 1)) {
+        const int64_t rr = e.x, sg = e.y & ~kTieUnchanged;
+        // the write pass skips the entries whose dry run found every outcome unchanged
+        if (!(p.row_flags[rr] & 1) && !(WRITE && (e.y & kTieUnchanged))) {
             const SegRecord rec = p.records[rr * p.nseg + sg];
             const int64_t c_end = min((int64_t)(sg + 1) * p.seg_len, p.cols);
             long long nl;
-            bool nh;
+            bool nh, changed = false;
             const bool met = replay_segment<RS, UTM, WRITE>(p, orb, rr, rec.tie_first, c_end, rec.tie_last,
-                                                            rec.tie_seed, rec.tie_seed, C, &nl, &nh);
+                                                            rec.tie_seed, rec.tie_seed, C, &nl, &nh, &changed);
             if (!WRITE && !met) atomicOr(&p.row_flags[rr], 1);
+            if (!WRITE && met && !changed) p.tie_list[i] = make_int2(e.x, e.y | kTieUnchanged);
         }
     }
     if (WRITE) flush_counts(p, C, true);
