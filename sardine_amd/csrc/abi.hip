@@ -3847,10 +3847,13 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
     SH.n_jobs = k;
     if (H.n_tc >= 0xffffffffLL || SH.n_tiles >= 0xffffffffLL)
         return fail(SARDINE_ERR_CAPACITY, "batch work above 2^32 items");
-    SD_HIP(hipMemcpyAsync(b->d_H, b->h_H, sizeof(sd::BatchHeader), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(b->d_J, b->h_J, (size_t)k * sizeof(sd::BatchJob), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(b->d_SH, b->h_SH, sizeof(SpeckleBatchHeaderHost), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(b->d_SJ, b->h_SJ, (size_t)k * b->sp_job_size, hipMemcpyHostToDevice, s));
+    // the four tables lie at the same offsets in the pinned and the device
+    // block: one transfer from the header to the last used speckle job (each
+    // transfer leaves the stream idle for tens of microseconds)
+    {
+        const size_t n = (size_t)(b->h_SJ - reinterpret_cast<char*>(b->h_H)) + (size_t)k * b->sp_job_size;
+        SD_HIP(hipMemcpyAsync(b->d_H, b->h_H, n, hipMemcpyHostToDevice, s));
+    }
     SD_HIP(hipEventRecord(b->t0, s));
     if (b->use_graph) {
         const sardine_batch_tuning& t = b->tuning;
