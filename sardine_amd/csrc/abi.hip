@@ -3526,7 +3526,7 @@ sardine_status sardine_write_cog_with_crs(const char* path, const float* data, i
 namespace {
 
 extern "C" hipError_t sd_launch_batch_tc(const sd::BatchHeader* H, const sd::BatchJob* J, unsigned int* queue,
-                                         int n_cu, int tc_groups_per_cu, hipStream_t stream);
+                                         int n_cu, int tc_groups_per_cu, int orbit_lds, hipStream_t stream);
 extern "C" int64_t sd_speckle_tiles(int64_t cols, int64_t rows, int64_t* tiles_x);
 extern "C" size_t sd_speckle_job_size(void);
 extern "C" void sd_speckle_job_fill(void* entry, int kind, const float* in, float* out, int64_t cols, int64_t rows,
@@ -3568,7 +3568,7 @@ struct sardine_batch_s {
     void* d_SH = nullptr;
     void* d_SJ = nullptr;
     unsigned long long* d_cnt = nullptr;  // max_jobs x kCntSlots
-    unsigned long long* d_queue = nullptr;  // [0] chain items, [1] speckle tiles
+    unsigned long long* d_queue = nullptr;  // as 32-bit counters: [0..5] chain items per job class, [6] speckle tiles
     void* h_mem = nullptr;
     sd::BatchHeader* h_H = nullptr;
     sd::BatchJob* h_J = nullptr;
@@ -3580,6 +3580,8 @@ struct sardine_batch_s {
     hipGraphExec_t exec = nullptr;
     sardine_batch_tuning graph_tuning{-1, -1, -1};
     int graph_half = -1;
+    int orbit_lds = 0;        // the chain kernels' orbit LDS (bytes) of the next launch
+    int graph_orbit_lds = -1; // and of the captured graph
 };
 
 namespace {
@@ -3590,10 +3592,12 @@ size_t cnt_stride() { return sd::kCntSlots; }
 hipError_t batch_launches(sardine_batch_s* b, int max_half, hipStream_t s) {
     hipError_t e;
     if ((e = sd_launch_zero_u64(b->d_cnt, (int64_t)b->max_jobs * cnt_stride(), s)) != hipSuccess) return e;
-    if ((e = sd_launch_zero_u64(b->d_queue, 2, s)) != hipSuccess) return e;
+    if ((e = sd_launch_zero_u64(b->d_queue, 4, s)) != hipSuccess) return e;
     unsigned int* q = reinterpret_cast<unsigned int*>(b->d_queue);
-    if ((e = sd_launch_batch_tc(b->d_H, b->d_J, q, b->n_cu, b->tuning.tc_groups_per_cu, s)) != hipSuccess) return e;
-    return sd_launch_batch_speckle(b->d_SH, b->d_SJ, q + 2, b->n_cu * b->tuning.speckle_groups_per_cu, max_half, s);
+    if ((e = sd_launch_batch_tc(b->d_H, b->d_J, q, b->n_cu, b->tuning.tc_groups_per_cu, b->orbit_lds, s)) !=
+        hipSuccess)
+        return e;
+    return sd_launch_batch_speckle(b->d_SH, b->d_SJ, q + 6, b->n_cu * b->tuning.speckle_groups_per_cu, max_half, s);
 }
 
 }  // namespace
@@ -3789,6 +3793,10 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
         run.run_idx.push_back(i);
     }
     if (run.run_idx.empty()) return SARDINE_OK;
+    // the job table ordered by class (resampler * 2 + UTM): each class's chain
+    // kernel takes a contiguous range of the chain items
+    auto job_class = [&](int i) { return run.plans[(size_t)i].p.resampling * 2 + (run.plans[(size_t)i].utm ? 1 : 0); };
+    std::stable_sort(run.run_idx.begin(), run.run_idx.end(), [&](int a, int c) { return job_class(a) < job_class(c); });
     if ((st = require_device())) return st;
     hipStream_t s = b->use_graph ? b->work : stream;
     run.s = s;
@@ -3802,7 +3810,7 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
     std::memset(&H, 0, sizeof H);
     SpeckleBatchHeaderHost& SH = *b->h_SH;
     std::memset(&SH, 0, sizeof SH);
-    int max_half = 1;
+    int max_half = 1, orbit_lds = 0;
     int k = 0;
     for (int i : run.run_idx) {
         TcPlan& pl = run.plans[(size_t)i];
@@ -3820,10 +3828,12 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
         std::memset(&J, 0, sizeof J);
         J.p = pl.p;
         J.q = pl.q;
-        J.cls = pl.p.resampling * 2 + (pl.utm ? 1 : 0);
+        J.cls = job_class(i);
         J.tc_row_blocks = (pl.nr + 63) / 64;
         J.tc_begin = H.n_tc;
         H.n_tc += J.tc_row_blocks * ((pl.p.nseg + 3) / 4);
+        for (int c = J.cls + 1; c < 7; ++c) H.tc_cls[c] = H.n_tc;  // classes ascend along the table
+        orbit_lds = std::max(orbit_lds, (int)pl.p.orbit_lds);
         J.fix_begin = H.n_fix;
         H.n_fix += (pl.nr + 255) / 256;
         J.row_begin = H.n_rows;
@@ -3845,6 +3855,7 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
     }
     H.n_jobs = k;
     SH.n_jobs = k;
+    b->orbit_lds = orbit_lds;
     if (H.n_tc >= 0xffffffffLL || SH.n_tiles >= 0xffffffffLL)
         return fail(SARDINE_ERR_CAPACITY, "batch work above 2^32 items");
     // the four tables lie at the same offsets in the pinned and the device
@@ -3859,7 +3870,7 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
         const sardine_batch_tuning& t = b->tuning;
         const sardine_batch_tuning& g = b->graph_tuning;
         if (!b->exec || g.tc_groups_per_cu != t.tc_groups_per_cu || g.speckle_groups_per_cu != t.speckle_groups_per_cu ||
-            b->graph_half != max_half) {
+            b->graph_half != max_half || b->graph_orbit_lds != b->orbit_lds) {
             if (b->exec) {
                 SD_HIP(hipGraphExecDestroy(b->exec));
                 b->exec = nullptr;
@@ -3877,6 +3888,7 @@ sardine_status batch_begin(sardine_batch_s* b, const sardine_resident_scene* sce
             if (ie != hipSuccess) return fail(SARDINE_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
             b->graph_tuning = t;
             b->graph_half = max_half;
+            b->graph_orbit_lds = b->orbit_lds;
         }
         SD_HIP(hipGraphLaunch(b->exec, s));
     } else {

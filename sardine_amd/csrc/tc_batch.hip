@@ -57,28 +57,32 @@ __global__ __launch_bounds__(256) void batch_anchor_kernel(const BatchHeader* __
     }
 }
 
+// Persistent chain pass of one job class (resampler RS, UTM or EPSG:4326),
+// at that class's occupancy: each class has its own kernel, so the bilinear
+// and bicubic jobs run at three waves per SIMD while the Lanczos-3 ones run
+// at two.  The job table is ordered by class, so a class's items are the
+// contiguous range [tc_cls[c], tc_cls[c + 1]) of the combined index space;
+// queue[c] counts the items taken (zero when the launch starts).  The
+// orbit's LDS copy is dynamic, sized by the host to the largest orbit of
+// the batch's scenes.
 template <int RS, bool UTM>
-__device__ __forceinline__ void batch_item(const TcParams& p, const OrbitDev& orb, WaveStage& st, int64_t rr0,
-                                           int64_t seg) {
-    segment_wave<RS, UTM, false, 2>(p, orb, st, rr0, seg);   // the job table is read-only here
-}
-
-// Persistent chain pass: `queue` is zero when the launch starts.
-__global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void batch_tc_kernel(const BatchHeader* __restrict__ H,
-                                                                            const BatchJob* __restrict__ J,
-                                                                            unsigned int* queue) {
-    __shared__ OrbitDev orb;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(chain_waves<RS, UTM>()))) void batch_tc_kernel(
+    const BatchHeader* __restrict__ H, const BatchJob* __restrict__ J, unsigned int* queue) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char batch_orbit_lds[];
+    OrbitDev& orb = *reinterpret_cast<OrbitDev*>(batch_orbit_lds);
     __shared__ WaveStage stage[4];
     __shared__ unsigned int s_item;
+    constexpr int CLS = RS * 2 + (UTM ? 1 : 0);
     const int w = threadIdx.x >> 6;
     const int n_jobs = H->n_jobs;
-    const int64_t n_items = H->n_tc;
+    const int64_t i0 = H->tc_cls[CLS], n_items = H->tc_cls[CLS + 1] - i0;
     const OrbitDev* loaded = nullptr;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        if (threadIdx.x == 0) s_item = atomicAdd(queue + CLS, 1u);
         __syncthreads();
-        const int64_t item = (int64_t)s_item;
-        if (item >= n_items) break;  // the same value for the whole workgroup
+        const int64_t taken = (int64_t)s_item;
+        if (taken >= n_items) break;  // the same value for the whole workgroup
+        const int64_t item = i0 + taken;
         const int j = __builtin_amdgcn_readfirstlane(find_job(n_jobs, item, [&](int k) { return J[k].tc_begin; }));
         const BatchJob& job = J[j];
         if (job.p.orbit != loaded) {  // workgroup-uniform
@@ -88,17 +92,7 @@ __global__ __launch_bounds__(256) SD_SEGMENT_OCCUPANCY void batch_tc_kernel(cons
         const int64_t local = item - job.tc_begin;
         const int64_t rb = local % job.tc_row_blocks, sg = local / job.tc_row_blocks;
         const int64_t seg = sg * 4 + w;
-        if (seg < job.p.nseg) {
-            const int64_t rr0 = rb * 64;
-            switch (job.cls) {
-                case 0: batch_item<0, false>(job.p, orb, stage[w], rr0, seg); break;
-                case 1: batch_item<0, true>(job.p, orb, stage[w], rr0, seg); break;
-                case 2: batch_item<1, false>(job.p, orb, stage[w], rr0, seg); break;
-                case 3: batch_item<1, true>(job.p, orb, stage[w], rr0, seg); break;
-                case 4: batch_item<2, false>(job.p, orb, stage[w], rr0, seg); break;
-                default: batch_item<2, true>(job.p, orb, stage[w], rr0, seg); break;
-            }
-        }
+        if (seg < job.p.nseg) segment_wave<RS, UTM, false, 2>(job.p, orb, stage[w], rb * 64, seg);
         __syncthreads();  // the item slot and the orbit are rewritten next
     }
 }
@@ -145,14 +139,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
 
 }  // namespace sd
 
-// The batch's TC passes: rows, anchors, persistent chain, fix-up.  `queue`
-// must be zero on entry (stream order).
+namespace {
+// One class's persistent chain kernel: tc_groups_per_cu 1 puts one
+// workgroup on each CU, more fills the class's occupancy.
+template <int RS, bool UTM>
+void launch_batch_chain(const sd::BatchHeader* H, const sd::BatchJob* J, unsigned int* queue, int n_cu,
+                        int tc_groups_per_cu, size_t orbit_lds, hipStream_t stream) {
+    const int groups = tc_groups_per_cu <= 1 ? 1 : sd::chain_waves<RS, UTM>();
+    hipLaunchKernelGGL((sd::batch_tc_kernel<RS, UTM>), dim3((unsigned)(n_cu * groups)), dim3(256), orbit_lds, stream,
+                       H, J, queue);
+}
+}  // namespace
+
+// The batch's TC passes: rows, anchors, the persistent chain kernels (one per
+// job class), fix-up.  queue[0..5] (the classes' item counters) must be zero
+// on entry (stream order); orbit_lds: bytes of the largest orbit's LDS copy.
 extern "C" hipError_t sd_launch_batch_tc(const sd::BatchHeader* H, const sd::BatchJob* J, unsigned int* queue,
-                                         int n_cu, int tc_groups_per_cu, hipStream_t stream) {
+                                         int n_cu, int tc_groups_per_cu, int orbit_lds, hipStream_t stream) {
+    if (orbit_lds <= 0 || (size_t)orbit_lds > sizeof(sd::OrbitDev)) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)orbit_lds + 15) & ~(size_t)15;
     hipLaunchKernelGGL(sd::batch_rows_kernel, dim3(256), dim3(256), 0, stream, H, J);
     hipLaunchKernelGGL(sd::batch_anchor_kernel, dim3(1024), dim3(256), 0, stream, H, J);
-    hipLaunchKernelGGL(sd::batch_tc_kernel, dim3((unsigned)(n_cu * tc_groups_per_cu)), dim3(256), 0, stream, H, J,
-                       queue);
+    launch_batch_chain<0, false>(H, J, queue, n_cu, tc_groups_per_cu, lds, stream);
+    launch_batch_chain<0, true>(H, J, queue, n_cu, tc_groups_per_cu, lds, stream);
+    launch_batch_chain<1, false>(H, J, queue, n_cu, tc_groups_per_cu, lds, stream);
+    launch_batch_chain<1, true>(H, J, queue, n_cu, tc_groups_per_cu, lds, stream);
+    launch_batch_chain<2, false>(H, J, queue, n_cu, tc_groups_per_cu, lds, stream);
+    launch_batch_chain<2, true>(H, J, queue, n_cu, tc_groups_per_cu, lds, stream);
     hipLaunchKernelGGL(sd::batch_check_kernel, dim3((unsigned)(n_cu * 8)), dim3(256), 0, stream, H, J);
     hipLaunchKernelGGL(sd::batch_fixup_kernel, dim3((unsigned)(n_cu * 2)), dim3(256), 0, stream, H, J);
     return hipGetLastError();

@@ -621,12 +621,12 @@ __device__ __forceinline__ void flush_mask(const TcParams& p, const WaveStage& s
 // 18001 x 36067 grid).  The orbit's LDS copy is sized to the orbit
 // (sd_launch_tc_fp64), which leaves room for three workgroups per CU.  At
 // two: the FP64 Lanczos-3 chains (214-219 VGPRs; they spill at three: UTM
-// 36.0 -> 44.1 ms) and the batched kernel (every job class in one kernel).
+// 36.0 -> 44.1 ms).  The batched chain kernels (one per job class) follow
+// the same table.
 template <int RS, bool UTM>
 constexpr int chain_waves() {
     return RS == 2 ? 2 : 3;
 }
-#define SD_SEGMENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
 
 // Lane (row, segment): wave = 64 adjacent rows of one segment; the 4 waves of
 // a workgroup take 4 consecutive segments.  Lanes past the last row compute

@@ -177,6 +177,9 @@ struct BatchHeader {
     int32_t n_jobs;
     int32_t pad;
     int64_t n_tc, n_fix, n_rows, n_anchor;  // totals of the passes
+    // chain items of job class c (resampler * 2 + UTM): [tc_cls[c], tc_cls[c + 1])
+    // (the job table is ordered by class)
+    int64_t tc_cls[7];
 };
 
 }  // namespace sd
