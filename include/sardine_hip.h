@@ -868,7 +868,9 @@ typedef struct sardine_batch_job {
 } sardine_batch_job;
 
 /* Launch shape of a batch: chain segment length (0 = automatic), persistent
- * workgroups per CU of the chain pass and of the speckle pass. */
+ * workgroups per CU of the chain pass (1: one per CU; 2 or more: as many as
+ * each job class's chain kernel holds, three for bilinear and bicubic and
+ * two for Lanczos-3) and of the speckle pass. */
 typedef struct sardine_batch_tuning {
     int32_t segment_columns;
     int32_t tc_groups_per_cu;
@@ -878,8 +880,9 @@ typedef struct sardine_batch_tuning {
 typedef struct sardine_batch_s* sardine_batch;
 
 /* An executor for up to max_jobs jobs per run.  use_graph != 0 captures the
- * launch sequence once as a HIP graph (recaptured when the tuning or the
- * largest speckle window changes) and replays it every run. */
+ * launch sequence once as a HIP graph (recaptured when the tuning, the
+ * largest speckle window or the largest orbit changes) and replays it every
+ * run. */
 sardine_status sardine_batch_create(int32_t max_jobs, int32_t use_graph, sardine_batch* out);
 void sardine_batch_destroy(sardine_batch batch);
 sardine_status sardine_batch_set_tuning(sardine_batch batch, const sardine_batch_tuning* tuning);
