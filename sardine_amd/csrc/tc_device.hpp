@@ -896,7 +896,11 @@ __device__ __forceinline__ bool replay_segment(const TcParams& p, const OrbitDev
         PixelPos q = prep_pixel<UTM>(p, p.col0 + cc, row_y, alon, alat);
         if (!q.dem_ok) continue;  // outcome independent of the seed
         Solve o = solve(p, orb, q, so, p.max_iter, co);
-        Solve n = solve<true>(p, orb, q, sn, p.max_iter, cn);
+        // From the same seed the corrected chain takes the old chain's steps
+        // exactly unless one of them is a tie step (the reference's
+        // arithmetic replaces only those, and the tie test is the same in
+        // both), so its own solve is needed only then.
+        const Solve n = (so == sn && !o.tie) ? o : solve<true>(p, orb, q, sn, p.max_iter, cn);
         const bool same_t = o.oc == ZD_CONVERGED && n.oc == ZD_CONVERGED && o.t == n.t;
         if (changed && !(same_t && o.it == n.it)) *changed = true;
         if (same_t && (cc >= sync_from || o.it == n.it)) {
