@@ -697,7 +697,7 @@ __device__ __forceinline__ void rl_step(RlWalk& w, RlWalkLds& L, const RlWalkArg
     // a row with no finite centre in the strip (no-data) skips the pick;
     // the branch also bounds the scheduler's region to one step
     if (__builtin_amdgcn_ballot_w64(isfinite(centre)) == 0) {
-        w.o[PH] = centre;
+        w.o[PH] = __builtin_nanf("");  // a non-finite centre gives NaN (an infinite one too)
         return;
     }
     bool undecided = false;

@@ -280,6 +280,31 @@ def test_refined_lee_walk_shapes(cuda, rows, cols, nan_frac, neg):
     assert np.array_equal(out, ref, equal_nan=True)
 
 
+def test_refined_lee_walk_nodata_bands(cuda):
+    """No-data across whole strips: full-width NaN bands of 6, 7, 8 and 20
+    rows (windows wholly inside no-data, and the exact path on the rows
+    around them), a band of +inf rows (a row of the walk with no finite
+    centre gives NaN for infinite centres too), a no-data corner like a
+    rotated footprint's and a partial band; bit-exact against the oracle."""
+    import torch
+
+    rows, cols = 420, 300
+    rng = np.random.default_rng(11)
+    x = rng.exponential(1.0, (rows, cols)).astype(np.float32)
+    for r0, h in ((20, 6), (60, 7), (100, 8), (150, 20)):
+        x[r0:r0 + h] = np.nan
+    x[200:212] = np.inf
+    yy, xx = np.mgrid[0:rows, 0:cols]
+    x[(yy > 260) & (xx + (yy - 260) * 2 < 250)] = np.nan
+    x[330:345, :140] = np.nan
+    x[400:] = np.nan  # the last rows: a run that reaches the raster's end
+    st, ref = O.speckle(x, abi.SPECKLE_REFINED_LEE, 7, 2.0, 1.0)
+    assert st == 0
+    out = api.SpeckleFilter(abi.SPECKLE_REFINED_LEE, 7, 2.0, 1.0).apply(
+        torch.from_numpy(x).to(cuda), cols, rows).cpu().numpy()
+    assert np.array_equal(out, ref, equal_nan=True), np.argwhere(~((out == ref) | (np.isnan(out) & np.isnan(ref))))[:5]
+
+
 def test_to_db(cuda):
     import torch
 
