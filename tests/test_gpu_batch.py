@@ -100,6 +100,41 @@ def test_batch_equals_per_call(scenes, use_graph):
     b.close()
 
 
+def test_batch_scenes_with_longer_orbit(cuda, scenes):
+    """A resident scene whose orbit has more state vectors (33 at 5 s instead
+    of 17 at 10 s) joins a batch: the chain kernels' LDS holds the largest
+    orbit of the batch, and the graph is captured again whenever that size
+    changes (small, large, small).  Every job equals its per-call result."""
+    import dataclasses
+
+    sc = synthetic.scene_small_plain(6, lines=512, samples=1024, out_rows=300, out_cols=300, dem_side=512)
+    t, pos, vel = synthetic.state_vectors(sc.orbit, n=33, spacing_s=5.0)
+    sc = dataclasses.replace(sc, sv_t_us=t, sv_pos=pos, sv_vel=vel)
+    ds = DeviceScene.build(sc, cuda)
+    m = ds.prepare_merged()
+    all3 = list(scenes) + [(sc, ds, m, api.ResidentScene(m, ds.scene, ds.dem))]
+    res_scenes = [s[3] for s in all3]
+    (_, da, _, _), (_, db, _, _) = scenes
+    F = api.SpeckleFilter
+
+    def small():
+        return [api.BatchJob(0, da.tc_config(aoi=(10, 10, 120, 130)), None),
+                api.BatchJob(1, db.tc_config(aoi=(5, 5, 60, 70), resampling=api.ResamplingKernel.Bicubic),
+                             F.Lee(5, 4.0))]
+
+    def large():
+        return small() + [api.BatchJob(2, ds.tc_config(compute_lia=True), F.RefinedLee(1.0)),
+                          api.BatchJob(2, ds.tc_config(aoi=(0, 0, 50, 300),
+                                                       resampling=api.ResamplingKernel.Lanczos3), None)]
+
+    b = api.Batch(max_jobs=8, use_graph=True)
+    for make in (small, large, small):
+        jobs = make()
+        print(f"batch of {len(jobs)} jobs", flush=True)
+        _check(b.run(res_scenes, jobs), [_alone(all3, j) for j in jobs])
+    b.close()
+
+
 def test_batch_invalid_job_is_skipped(scenes):
     res_scenes = [s[3] for s in scenes]
     jobs = _jobs(scenes)[:3]
