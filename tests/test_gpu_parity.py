@@ -402,6 +402,25 @@ def test_tc_tie_replays_change_no_bit(small, band, lead):
     assert img.newton_iter_histogram == base.newton_iter_histogram
 
 
+@pytest.mark.parametrize("n_sv,spacing_s", [(9, 20.0), (33, 5.0), (64, 2.5)])
+def test_tc_orbit_lengths(cuda, n_sv, spacing_s):
+    """Orbits of 9, 33 and 64 state vectors (the chain kernels' LDS copy of
+    the orbit is sized to its vectors; 64 is the most the device form
+    holds): the per-call path against the oracle, bilinear and bicubic."""
+    import dataclasses
+
+    print(f"orbit of {n_sv} vectors at {spacing_s} s", flush=True)
+    sc = synthetic.scene_small_plain(5, lines=512, samples=1024, out_rows=400, out_cols=400, dem_side=512)
+    t, pos, vel = synthetic.state_vectors(sc.orbit, n=n_sv, spacing_s=spacing_s,
+                                          t0_s=80.0 - 0.5 * spacing_s * (n_sv - 1))  # centred on the scene
+    sc = dataclasses.replace(sc, sv_t_us=t, sv_pos=pos, sv_vel=vel)
+    ds = DeviceScene.build(sc, cuda, host_raw=True)
+    info, s0, nz, _ = O.prepare_merged(sc)
+    m = ds.prepare_merged()
+    for rs in (abi.RESAMPLE_BILINEAR, abi.RESAMPLE_BICUBIC):
+        _tc_compare(sc, ds, info, s0, nz, m, resampling=rs)
+
+
 @pytest.mark.parametrize("sat_lon", [None, -5.8])
 def test_tc_geoid_grid_ww15mgh(cuda, tmp_path, sat_lon):
     """GeoidModel::Egm96 from a grid file (geoid.rs:220-251; added per pixel
